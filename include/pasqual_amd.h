@@ -1,0 +1,182 @@
+/*
+ * pasqual_amd.h -- C ABI of the MI355X (gfx950) suffix-array / FM-index
+ * overlap library.
+ *
+ * Two layers are exported by libpasqual_amd_u32.so / libpasqual_amd_u64.so:
+ *
+ *  1. The reference's own entry points, with the reference's struct layouts
+ *     for the INDEX_TYPE_T of the library variant (declared in
+ *     pasqual_refabi.h): construct_index, construct_string_graph,
+ *     destroy_graph, free_index and the global time_importreads.  These are
+ *     drop-in replacements for index.o + sufsort.o + string_graph.o of the
+ *     reference build.
+ *
+ *  2. The lower interface below: status-returning calls on an opaque context.
+ *     Every call returns PQ_OK (0) or a negative PQ_ERR_* code and never exits;
+ *     pq_last_error() describes the last failure.  Plain pointers and sizes
+ *     only.  A context owns one HIP stream and all device memory; it is not
+ *     re-entrant (one caller thread at a time).
+ *
+ * Codes: reads are arrays of nucleotide codes 1..4 (A,C,G,T), the encoding of
+ * index.c:280-321.  Read r occupies [off[r], off[r+1]) of the concatenated
+ * array; all reads may have the same length (the reference's contract) or
+ * different lengths (the generalisation used for batches of uneven reads).
+ */
+#ifndef PASQUAL_AMD_H
+#define PASQUAL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PQ_OK            0
+#define PQ_ERR_ARG      -1   /* invalid argument */
+#define PQ_ERR_IO       -2   /* file could not be opened / read */
+#define PQ_ERR_FORMAT   -3   /* malformed reads file (index.c:218-321 rules) */
+#define PQ_ERR_HIP      -4   /* HIP runtime error or no device */
+#define PQ_ERR_NOMEM    -5   /* host or device allocation failed */
+#define PQ_ERR_STATE    -6   /* call out of order (e.g. search before build) */
+#define PQ_ERR_LIMIT    -7   /* a node needs more than 65535 edges
+                                (string_graph.c:469-472), or a size limit */
+
+#define PQ_DIR_FW 0          /* index over the reads */
+#define PQ_DIR_BW 1          /* index over the reversed reads */
+
+#define PQ_SEARCH_PLAIN   0  /* bidirectional backward search from depth 1 */
+#define PQ_SEARCH_LEARNED 1  /* search seeded by the learned index */
+
+typedef struct pq_ctx pq_ctx_t;
+
+/* ---- context ---------------------------------------------------------------- */
+int pq_ctx_create(pq_ctx_t **out, int device);
+void pq_ctx_destroy(pq_ctx_t *ctx);
+const char *pq_last_error(const pq_ctx_t *ctx);
+
+/* ---- input -------------------------------------------------------------------- */
+/* Copy reads to the device.  codes: off[n] bytes of 1..4; off: n+1 offsets.
+ * min_overlap: t of the overlap search (t >= 1, t < every read length that is
+ * expected to overlap).  Replaces any previous reads and index. */
+int pq_upload_reads(pq_ctx_t *ctx, const uint8_t *codes, const uint64_t *off,
+                    uint64_t n, uint32_t min_overlap);
+
+/* ---- index build (index.c:408-706 + sufsort.c) ------------------------------------ */
+/* Builds both directions: suffix arrays, BWT occurrence lines, reads_index,
+ * C and the end-of-read counts. */
+int pq_build_index(pq_ctx_t *ctx);
+
+int pq_get_sa(pq_ctx_t *ctx, int dir, uint64_t *sa);              /* N entries */
+int pq_get_bwt(pq_ctx_t *ctx, int dir, uint8_t *bwt);             /* N codes 0..4, 0='$' */
+int pq_get_reads_index(pq_ctx_t *ctx, int dir, uint64_t *ri);     /* n entries */
+int pq_get_counts(pq_ctx_t *ctx, uint64_t C[6], uint64_t suffix_count[4],
+                  uint64_t prefix_count[4]);
+/* Occurrence table in the reference host layout (index.c:26-142, 587-615):
+ * rows = (N+64)/64; bitmap gets 5*rows words (row c at c*rows), sample gets
+ * 5*rows entries of sample_bytes (4 or 8) each. */
+int pq_get_occ_ref(pq_ctx_t *ctx, int dir, uint64_t *bitmap, void *sample,
+                   int sample_bytes);
+/* Interval shortcut of every read for the reference's search entry
+ * (index.h:31-37).  Depth-1 intervals: the SA interval of the last (fw) or
+ * first (bw) code of the read. */
+int pq_get_shortcut(pq_ctx_t *ctx, int dir, uint64_t *start, uint64_t *size,
+                    uint16_t *depth);
+uint64_t pq_num_reads(const pq_ctx_t *ctx);
+uint64_t pq_text_len(const pq_ctx_t *ctx);
+
+/* ---- overlap search (string_graph.c:162-731) ------------------------------------------- */
+int pq_find_overlaps(pq_ctx_t *ctx, int mode);
+/* Sizes of the last search result. */
+int pq_graph_sizes(pq_ctx_t *ctx, uint64_t *n_reads, uint64_t *n_edges);
+/* Copy the last search result: per read the candidate counts overlaps_B/F
+ * (string_graph.c:820), the number of irreducible B and F edges, and the edges
+ * in CSR order (read r: [edge_off[r], edge_off[r+1]), B list then F list),
+ * as (dest read, overlap length, type BF=0 FB=1 FF=2 BB=3).  Any output
+ * pointer may be NULL. */
+int pq_graph_copy(pq_ctx_t *ctx, int32_t *overlaps_B, int32_t *overlaps_F,
+                  uint32_t *num_B, uint32_t *num_F, uint64_t *edge_off,
+                  uint64_t *dest, uint16_t *len, uint8_t *type);
+
+/* ---- learned index --------------------------------------------------------------------- */
+typedef struct {
+    uint32_t k;              /* prefix length fed to the model (<= 31), 21 */
+    uint32_t hidden;         /* MLP hidden width, 64 */
+    uint32_t leaves;         /* piecewise-linear leaves, 4096 */
+    uint32_t epochs;         /* SGD passes over the sampled training set */
+    uint32_t batch;          /* minibatch size */
+    uint32_t samples;        /* training samples drawn from each SA */
+    float lr;                /* learning rate */
+    uint64_t seed;
+} pq_learned_cfg_t;
+
+typedef struct {
+    double mean_window[2];   /* mean |predicted row - true row| bound used, fw/bw */
+    uint64_t max_window[2];  /* largest correction window, fw/bw */
+    double train_loss[2];    /* final MSE of the MLP on normalised rows */
+    double train_ms;
+} pq_learned_stats_t;
+
+void pq_learned_default(pq_learned_cfg_t *cfg);
+int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg);
+int pq_learned_stats(pq_ctx_t *ctx, pq_learned_stats_t *st);
+
+/* ---- timing ---------------------------------------------------------------------------- */
+typedef struct {
+    double upload_ms;
+    double sa_ms[2];         /* suffix sort per direction */
+    double occ_ms[2];        /* BWT + occurrence lines + reads_index */
+    double find_ms;          /* candidate intervals (bidirectional search) */
+    double extract_ms;       /* irreducible-edge extraction */
+    double compact_ms;       /* edge CSR compaction */
+    double learned_ms;       /* learned seeding (learned mode only) */
+    uint64_t find_launches;
+    double find_kernel_ms;   /* summed duration of the candidate-search kernel */
+} pq_timing_t;
+int pq_get_timing(pq_ctx_t *ctx, pq_timing_t *t);
+int pq_sync(pq_ctx_t *ctx);
+
+/* ---- host utilities (no device use) --------------------------------------------------- */
+/* Seeded synthetic workloads (see pq_synth.c for the generator contract). */
+typedef struct {
+    uint64_t genome_len;
+    uint32_t n_units, unit_len, n_copies;   /* interspersed repeat units */
+    double copy_div;
+    double family_frac;                      /* repeat families */
+    uint32_t fam_len_min, fam_len_max;
+    double fam_div_min, fam_div_max;
+    uint32_t fam_copies_min, fam_copies_max;
+    uint32_t len_min, len_max;               /* read length range */
+    double coverage;
+    uint64_t n_reads;                        /* 0: ceil(coverage*G/mean length) */
+    double sub_rate;                         /* i.i.d. substitutions per base */
+    uint32_t min_overlap;                    /* t of the workload */
+    uint64_t seed;
+} pq_synth_cfg_t;
+
+int pq_synth_preset(int config, pq_synth_cfg_t *cfg);
+uint64_t pq_synth_num_reads(const pq_synth_cfg_t *cfg);
+int pq_synth_genome(const pq_synth_cfg_t *cfg, uint8_t *genome);
+int pq_synth_lengths(const pq_synth_cfg_t *cfg, uint64_t *off);
+int pq_synth_reads(const pq_synth_cfg_t *cfg, const uint8_t *genome,
+                   const uint64_t *off, uint8_t *codes);
+int pq_synth_fasta(const pq_synth_cfg_t *cfg, const char *path);
+
+/* Read import with duplicate removal, the rules of import_singlereads
+ * (index.c:145-405) and bloom_filter (bloom.c:91-135) with hashes in
+ * index_bits (32 or 64) arithmetic.  *reads is malloc'ed (n*read_len codes). */
+typedef struct {
+    uint8_t *reads;
+    uint64_t n;
+    uint64_t removed;
+    uint64_t C[6], suffix_count[4], prefix_count[4];
+    double ms;
+    long bad_line;           /* line number reported on PQ_ERR_FORMAT */
+} pq_import_t;
+int pq_import_fasta(const char *path, int read_len, int bloom_m, int n_hashfunc,
+                    int index_bits, pq_import_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

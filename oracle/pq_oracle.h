@@ -1,0 +1,111 @@
+/*
+ * pq_oracle.h -- CPU oracle for the suffix-array build and the FM-index overlap
+ * search of PASQUAL.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker: tests, the package's
+ * smoke check and the CPU-baseline leg of the benchmark may call it; the
+ * product (the HIP library under pasqual_amd/) never links, loads or calls it.
+ *
+ * Reads are passed as one concatenated array of nucleotide codes 1..4
+ * (A,C,G,T; index.c:280-321) with read offsets off[0..n] (off[0] = 0).  All
+ * index values are returned as uint64_t whatever INDEX_TYPE_T the reference was
+ * built with; unsigned wrap-around of the reference's index arithmetic is
+ * reproduced in 64 bits, which gives the same outcomes for every input whose
+ * text fits the narrower type.
+ *
+ * Pinning: every function here is checked in tests/ against golden vectors
+ * written by the reference itself (tests/golden/), against brute force on
+ * small inputs, and against the explicit-string restatement below.
+ */
+#ifndef PQ_ORACLE_H
+#define PQ_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Direction text: dir 0 = reads as given; dir 1 = every read reversed (not
+ * complemented) -- transform_to_triplet_reverse, sufsort.c:80-121. */
+void orc_dir_text(const uint8_t *text, const uint64_t *off, uint64_t n, int dir,
+                  uint8_t *out);
+
+/* Canonical suffix array of a direction text: all positions g sorted by the
+ * suffix string up to the end of its read (end of read smallest, codes 1<2<3<4),
+ * ties by g.  This is the order the reference's MSD sort produces at one
+ * thread (sufsort.c:154-237 stable insertion sort with STNI_strcmp's
+ * shorter-first rule; sufsort.c:240-383 stable counting passes; sufsort.c:
+ * 483-546 text-order scatter).  Returns 0 on success. */
+int orc_sa(const uint8_t *dtext, const uint64_t *off, uint64_t n, uint64_t *sa);
+
+/* BWT codes (0 = '$' for a suffix starting a read, else the preceding code)
+ * and reads_index (read ids of '$' rows in SA order): index.c:53-79. */
+void orc_bwt(const uint8_t *dtext, const uint64_t *off, uint64_t n,
+             const uint64_t *sa, uint8_t *bwt, uint64_t *reads_index);
+
+/* C[6], suffix_count[4] (reads ending in c), prefix_count[4] (reads starting
+ * with c): index.c:335-346, 366-372. */
+void orc_counts(const uint8_t *text, const uint64_t *off, uint64_t n,
+                uint64_t C[6], uint64_t suffix_count[4],
+                uint64_t prefix_count[4]);
+
+/* Occurrence table in the reference host layout: rows = (N+64)/64;
+ * bitmap[c*rows + w] has bit (j+1) set iff bwt[64w+j] == c for j <= 62;
+ * sample[5w + c] = fix[c] + #{i < 64w : bwt[i] == c}, fix = {0, f0..f3}
+ * (index.c:26-142, 587-615). */
+void orc_occ_ref(const uint8_t *bwt, uint64_t N, const uint64_t fix4[4],
+                 uint64_t *bitmap, uint64_t *sample);
+
+/* Interval shortcut of every read (fixed read length L): the reference's
+ * interval_start/end/depth arrays (index.c:495-502, 548-555 initialise depth 1;
+ * sufsort.c:177-209, 322-334 record deeper buckets; sufsort.c:354-376 and
+ * 594-611 decide which buckets are visited). */
+int orc_shortcut(const uint8_t *dtext, uint64_t n, uint32_t L, uint32_t t,
+                 const uint64_t *sa, const uint64_t C[6], uint64_t *istart,
+                 uint64_t *iend, uint64_t *idepth);
+
+/* ---- overlap search, restated on the reference host layout ---------------- */
+
+typedef struct {
+    uint64_t n, N;
+    uint32_t L, t;
+    const uint8_t *reads;          /* fixed-length reads, codes 1..4 */
+    uint64_t C[6];
+    const uint64_t *bitmap[2];     /* [dir] -> 5*rows words */
+    const uint64_t *sample[2];     /* [dir] -> 5*rows */
+    const uint64_t *reads_index[2];
+    const uint64_t *istart[2], *iend[2], *idepth[2];
+} orc_fmindex_t;
+
+/* Per-read search result in CSR form; edges of read r are
+ * [edge_off[r], edge_off[r+1]), the B list first (num_B of them), then F. */
+typedef struct {
+    uint64_t n;
+    int64_t *overlaps_B, *overlaps_F;
+    uint64_t *num_B, *num_F;
+    uint64_t *edge_off;
+    uint64_t *dest;
+    uint32_t *len;
+    uint8_t *type;                 /* BF=0 FB=1 FF=2 BB=3 (string_graph.h:13-16) */
+    uint64_t rank_evals;           /* rank-position evaluations (see .c) */
+    uint64_t rank_evals_find;      /* ... of which in find_intervals */
+} orc_graph_t;
+
+/* find_intervals + extract_{bw,fw}_edges for every read (string_graph.c:
+ * 162-731, driven as string_graph.c:800-830).  Returns 0, or -1 if a node
+ * exceeds 65535 edges (string_graph.c:469-472). */
+int orc_search_fm(const orc_fmindex_t *ix, orc_graph_t *out);
+
+/* Independent restatement on explicit strings (generalised to reads of
+ * different lengths): candidates (A.8) and irreducible edges (A.9) of every
+ * read by hashing read ends, no FM index involved. */
+int orc_search_explicit(const uint8_t *text, const uint64_t *off, uint64_t n,
+                        uint32_t t, orc_graph_t *out);
+
+void orc_graph_free(orc_graph_t *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,329 @@
+"""Host-side Python binding of the MI355X suffix-array / FM-index overlap library.
+
+The library itself is C ABI (include/pasqual_amd.h, include/pasqual_refabi.h);
+this module loads the in-tree build (pasqual_amd/lib/libpasqual_amd_u32.so or
+_u64.so) through ctypes and wraps the lower pq_* interface with numpy arrays.
+There is no fallback: if the library is missing the import of `lib()` raises.
+
+Mirror of the reference's operator path (string_graph.c / index.c):
+    ctx = Context()
+    ctx.upload(codes, off, min_overlap)     # reads, codes 1..4
+    ctx.build_index()                       # construct_index's device part
+    ctx.find_overlaps()                     # construct_string_graph's search
+    g = ctx.graph()                         # per-read counts + CSR edges
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+
+PQ_OK = 0
+ERRORS = {-1: "PQ_ERR_ARG", -2: "PQ_ERR_IO", -3: "PQ_ERR_FORMAT", -4: "PQ_ERR_HIP",
+          -5: "PQ_ERR_NOMEM", -6: "PQ_ERR_STATE", -7: "PQ_ERR_LIMIT"}
+SEARCH_PLAIN, SEARCH_LEARNED = 0, 1
+BF, FB, FF, BB = 0, 1, 2, 3
+
+# exported symbols per header; the ABI test checks the library against these
+PQ_SYMBOLS = [
+    "pq_ctx_create", "pq_ctx_destroy", "pq_last_error", "pq_upload_reads", "pq_build_index",
+    "pq_get_sa", "pq_get_bwt", "pq_get_reads_index", "pq_get_counts", "pq_get_occ_ref",
+    "pq_get_shortcut", "pq_num_reads", "pq_text_len", "pq_find_overlaps", "pq_graph_sizes",
+    "pq_graph_copy", "pq_learned_default", "pq_learned_train", "pq_learned_stats",
+    "pq_get_timing", "pq_sync", "pq_synth_preset", "pq_synth_num_reads", "pq_synth_genome",
+    "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
+]
+REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
+               "time_importreads"]
+
+
+class PqError(RuntimeError):
+    pass
+
+
+class SynthCfg(C.Structure):
+    _fields_ = [
+        ("genome_len", C.c_uint64),
+        ("n_units", C.c_uint32), ("unit_len", C.c_uint32), ("n_copies", C.c_uint32),
+        ("copy_div", C.c_double),
+        ("family_frac", C.c_double),
+        ("fam_len_min", C.c_uint32), ("fam_len_max", C.c_uint32),
+        ("fam_div_min", C.c_double), ("fam_div_max", C.c_double),
+        ("fam_copies_min", C.c_uint32), ("fam_copies_max", C.c_uint32),
+        ("len_min", C.c_uint32), ("len_max", C.c_uint32),
+        ("coverage", C.c_double),
+        ("n_reads", C.c_uint64),
+        ("sub_rate", C.c_double),
+        ("min_overlap", C.c_uint32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class ImportResult(C.Structure):
+    _fields_ = [
+        ("reads", C.POINTER(C.c_uint8)), ("n", C.c_uint64), ("removed", C.c_uint64),
+        ("C", C.c_uint64 * 6), ("suffix_count", C.c_uint64 * 4), ("prefix_count", C.c_uint64 * 4),
+        ("ms", C.c_double), ("bad_line", C.c_long),
+    ]
+
+
+class LearnedCfg(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("hidden", C.c_uint32), ("leaves", C.c_uint32),
+                ("epochs", C.c_uint32), ("batch", C.c_uint32), ("samples", C.c_uint32),
+                ("lr", C.c_float), ("seed", C.c_uint64)]
+
+
+class LearnedStats(C.Structure):
+    _fields_ = [("mean_window", C.c_double * 2), ("max_window", C.c_uint64 * 2),
+                ("train_loss", C.c_double * 2), ("train_ms", C.c_double)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("upload_ms", C.c_double), ("sa_ms", C.c_double * 2), ("occ_ms", C.c_double * 2),
+                ("find_ms", C.c_double), ("extract_ms", C.c_double), ("compact_ms", C.c_double),
+                ("learned_ms", C.c_double), ("find_launches", C.c_uint64),
+                ("find_kernel_ms", C.c_double)]
+
+    def as_dict(self):
+        return {"upload_ms": self.upload_ms, "sa_ms": list(self.sa_ms),
+                "occ_ms": list(self.occ_ms), "find_ms": self.find_ms,
+                "extract_ms": self.extract_ms, "compact_ms": self.compact_ms,
+                "learned_ms": self.learned_ms, "find_launches": self.find_launches,
+                "find_kernel_ms": self.find_kernel_ms}
+
+
+_LIBS = {}
+
+
+def lib_path(width="u32"):
+    return os.path.join(LIBDIR, f"libpasqual_amd_{width}.so")
+
+
+def lib(width="u32"):
+    """Load the in-tree library (raises if it was not built)."""
+    if width in _LIBS:
+        return _LIBS[width]
+    path = lib_path(width)
+    if not os.path.exists(path):
+        raise PqError(f"{path} is missing: build it with `python -m pasqual_amd.build`")
+    L = C.CDLL(path)
+    P, U64, U32, I32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
+    sig = {
+        "pq_ctx_create": (I32, [C.POINTER(P), I32]),
+        "pq_ctx_destroy": (None, [P]),
+        "pq_last_error": (C.c_char_p, [P]),
+        "pq_upload_reads": (I32, [P, P, P, U64, U32]),
+        "pq_build_index": (I32, [P]),
+        "pq_get_sa": (I32, [P, I32, P]),
+        "pq_get_bwt": (I32, [P, I32, P]),
+        "pq_get_reads_index": (I32, [P, I32, P]),
+        "pq_get_counts": (I32, [P, P, P, P]),
+        "pq_get_occ_ref": (I32, [P, I32, P, P, I32]),
+        "pq_get_shortcut": (I32, [P, I32, P, P, P]),
+        "pq_num_reads": (U64, [P]),
+        "pq_text_len": (U64, [P]),
+        "pq_find_overlaps": (I32, [P, I32]),
+        "pq_graph_sizes": (I32, [P, P, P]),
+        "pq_graph_copy": (I32, [P, P, P, P, P, P, P, P, P]),
+        "pq_learned_default": (None, [P]),
+        "pq_learned_train": (I32, [P, P]),
+        "pq_learned_stats": (I32, [P, P]),
+        "pq_get_timing": (I32, [P, P]),
+        "pq_sync": (I32, [P]),
+        "pq_synth_preset": (I32, [I32, P]),
+        "pq_synth_num_reads": (U64, [P]),
+        "pq_synth_genome": (I32, [P, P]),
+        "pq_synth_lengths": (I32, [P, P]),
+        "pq_synth_reads": (I32, [P, P, P, P]),
+        "pq_synth_fasta": (I32, [P, C.c_char_p]),
+        "pq_import_fasta": (I32, [C.c_char_p, I32, I32, I32, I32, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIBS[width] = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+# ---- host utilities ------------------------------------------------------------------
+
+def synth_preset(config):
+    cfg = SynthCfg()
+    if lib().pq_synth_preset(config, C.byref(cfg)) != 0:
+        raise PqError(f"unknown workload config {config}")
+    return cfg
+
+
+def synth_reads(cfg):
+    """(codes uint8[N] in 1..4, off uint64[n+1]) of a synthetic workload."""
+    L = lib()
+    n = L.pq_synth_num_reads(C.byref(cfg))
+    genome = np.empty(max(1, cfg.genome_len), dtype=np.uint8)
+    if L.pq_synth_genome(C.byref(cfg), _ptr(genome)) != 0:
+        raise PqError("genome synthesis failed")
+    off = np.empty(n + 1, dtype=np.uint64)
+    if L.pq_synth_lengths(C.byref(cfg), _ptr(off)) != 0:
+        raise PqError("read length synthesis failed")
+    codes = np.empty(max(1, int(off[-1])), dtype=np.uint8)
+    L.pq_synth_reads(C.byref(cfg), _ptr(genome), _ptr(off), _ptr(codes))
+    return codes[:int(off[-1])], off
+
+
+def synth_fasta(cfg, path):
+    if lib().pq_synth_fasta(C.byref(cfg), str(path).encode()) != 0:
+        raise PqError(f"writing {path} failed")
+
+
+def import_fasta(path, read_len, bloom_m=36, n_hashfunc=2, index_bits=32):
+    """Reads import with duplicate removal (index.c:145-405); returns a dict."""
+    res = ImportResult()
+    rc = lib().pq_import_fasta(str(path).encode(), read_len, bloom_m, n_hashfunc, index_bits,
+                               C.byref(res))
+    if rc != 0:
+        raise PqError(f"import of {path} failed: {ERRORS.get(rc, rc)} line {res.bad_line}")
+    n = int(res.n)
+    reads = np.ctypeslib.as_array(res.reads, shape=(n * read_len,)).copy() if n else \
+        np.zeros(0, np.uint8)
+    C.CDLL(None).free(C.cast(res.reads, C.c_void_p))
+    return {"reads": reads, "n": n, "removed": int(res.removed), "C": list(res.C),
+            "suffix_count": list(res.suffix_count), "prefix_count": list(res.prefix_count),
+            "ms": res.ms}
+
+
+# ---- device context ------------------------------------------------------------------
+
+class Context:
+    """One device, one stream, one index (not thread-safe)."""
+
+    def __init__(self, device=0, width="u32"):
+        self._L = lib(width)
+        self._h = C.c_void_p()
+        rc = self._L.pq_ctx_create(C.byref(self._h), device)
+        if rc != 0:
+            raise PqError(f"pq_ctx_create failed: {ERRORS.get(rc, rc)}")
+
+    def close(self):
+        if self._h:
+            self._L.pq_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._L.pq_last_error(self._h)
+            raise PqError(f"{what}: {ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    @property
+    def n(self):
+        return int(self._L.pq_num_reads(self._h))
+
+    @property
+    def N(self):
+        return int(self._L.pq_text_len(self._h))
+
+    def upload(self, codes, off, min_overlap):
+        codes = np.ascontiguousarray(codes, dtype=np.uint8)
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        self._check(self._L.pq_upload_reads(self._h, _ptr(codes), _ptr(off), len(off) - 1,
+                                             int(min_overlap)), "upload")
+
+    def build_index(self):
+        self._check(self._L.pq_build_index(self._h), "build_index")
+
+    def sa(self, d):
+        out = np.empty(max(1, self.N), dtype=np.uint64)
+        self._check(self._L.pq_get_sa(self._h, d, _ptr(out)), "get_sa")
+        return out[:self.N]
+
+    def bwt(self, d):
+        out = np.empty(max(1, self.N), dtype=np.uint8)
+        self._check(self._L.pq_get_bwt(self._h, d, _ptr(out)), "get_bwt")
+        return out[:self.N]
+
+    def reads_index(self, d):
+        out = np.empty(max(1, self.n), dtype=np.uint64)
+        self._check(self._L.pq_get_reads_index(self._h, d, _ptr(out)), "get_reads_index")
+        return out[:self.n]
+
+    def counts(self):
+        Cc = np.zeros(6, np.uint64)
+        s = np.zeros(4, np.uint64)
+        p = np.zeros(4, np.uint64)
+        self._check(self._L.pq_get_counts(self._h, _ptr(Cc), _ptr(s), _ptr(p)), "get_counts")
+        return Cc, s, p
+
+    def occ_ref(self, d, sample_bytes=4):
+        rows = (self.N + 64) // 64
+        bm = np.empty(5 * rows, np.uint64)
+        sm = np.empty(5 * rows, np.uint32 if sample_bytes == 4 else np.uint64)
+        self._check(self._L.pq_get_occ_ref(self._h, d, _ptr(bm), _ptr(sm), sample_bytes),
+                    "get_occ_ref")
+        return bm.reshape(5, rows), sm
+
+    def shortcut(self, d):
+        n = self.n
+        st = np.empty(max(1, n), np.uint64)
+        sz = np.empty(max(1, n), np.uint64)
+        dp = np.empty(max(1, n), np.uint16)
+        self._check(self._L.pq_get_shortcut(self._h, d, _ptr(st), _ptr(sz), _ptr(dp)),
+                    "get_shortcut")
+        return st[:n], sz[:n], dp[:n]
+
+    def find_overlaps(self, mode=SEARCH_PLAIN):
+        self._check(self._L.pq_find_overlaps(self._h, mode), "find_overlaps")
+
+    def graph(self):
+        n, E = C.c_uint64(), C.c_uint64()
+        self._check(self._L.pq_graph_sizes(self._h, C.byref(n), C.byref(E)), "graph_sizes")
+        n, E = n.value, E.value
+        ovB = np.empty(max(1, n), np.int32)
+        ovF = np.empty(max(1, n), np.int32)
+        nB = np.empty(max(1, n), np.uint32)
+        nF = np.empty(max(1, n), np.uint32)
+        eo = np.empty(n + 1, np.uint64)
+        dst = np.empty(max(1, E), np.uint64)
+        ln = np.empty(max(1, E), np.uint16)
+        ty = np.empty(max(1, E), np.uint8)
+        self._check(self._L.pq_graph_copy(self._h, _ptr(ovB), _ptr(ovF), _ptr(nB), _ptr(nF),
+                                          _ptr(eo), _ptr(dst), _ptr(ln), _ptr(ty)), "graph_copy")
+        return {"overlaps_B": ovB[:n], "overlaps_F": ovF[:n], "num_B": nB[:n], "num_F": nF[:n],
+                "edge_off": eo, "dest": dst[:E], "len": ln[:E], "type": ty[:E]}
+
+    def learned_train(self, **kw):
+        cfg = LearnedCfg()
+        self._L.pq_learned_default(C.byref(cfg))
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        self._check(self._L.pq_learned_train(self._h, C.byref(cfg)), "learned_train")
+
+    def learned_stats(self):
+        st = LearnedStats()
+        self._check(self._L.pq_learned_stats(self._h, C.byref(st)), "learned_stats")
+        return {"mean_window": list(st.mean_window), "max_window": list(st.max_window),
+                "train_loss": list(st.train_loss), "train_ms": st.train_ms}
+
+    def timing(self):
+        t = Timing()
+        self._check(self._L.pq_get_timing(self._h, C.byref(t)), "get_timing")
+        return t.as_dict()
+
+    def sync(self):
+        self._check(self._L.pq_sync(self._h), "sync")

@@ -1,0 +1,352 @@
+/*
+ * pq_dropin.c -- the reference's entry points on top of the device library.
+ *
+ * Compiled once per index width (-D__INDEX_U32__ / -D__INDEX_U64__) so that the
+ * structs of pasqual_refabi.h have the layout of the matching reference build.
+ * Linked in place of index.o + sufsort.o + string_graph.o, the reference's CLI
+ * (test.c), bloom.c, common.c and assemble.c run unchanged on top of it.
+ *
+ *   construct_index         index.c:709-745: reads import (pq_import.c), then
+ *                           both suffix arrays / BWTs / occurrence tables on the
+ *                           device; the host-layout arrays of index_t are filled
+ *                           from the device index unless PQ_HOST_INDEX=0.
+ *   construct_string_graph  string_graph.c:734-1054: overlap search on the
+ *                           device, node records built here exactly as
+ *                           string_graph.c:832-1021 builds them.
+ *
+ * Messages and exit behaviour follow the reference: its errors print
+ * "Error: ..." to stderr and exit with status 0 (common.h:87-94).  Device
+ * failures, which the reference cannot have, exit with status 1.
+ *
+ * Environment: PQ_DEVICE (HIP device, default 0), PQ_SEARCH=learned (seed the
+ * search with the learned index), PQ_HOST_INDEX=0 (leave the host occurrence
+ * arrays of index_t NULL; free(NULL) in the caller stays valid).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#include "../../../include/pasqual_amd.h"
+#include "../../../include/pasqual_refabi.h"
+
+double time_importreads = 0.0;
+
+static pq_ctx_t *g_ctx;
+static const index_t *g_index;
+
+#define REF_ERROR_EXIT(...)                                                    \
+    do {                                                                       \
+        fprintf(stderr, "Error: ");                                            \
+        fprintf(stderr, __VA_ARGS__);                                          \
+        fflush(stderr);                                                        \
+        exit(0);                                                               \
+    } while (0)
+
+static void dev_fail(const char *what, int rc)
+{
+    fprintf(stderr, "Error: %s failed (%d): %s\n", what, rc,
+            g_ctx ? pq_last_error(g_ctx) : "no device context");
+    fflush(stderr);
+    exit(1);
+}
+
+static double now_ms(void)
+{
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    return tv.tv_sec * 1000.0 + tv.tv_usec / 1000.0;
+}
+
+static int env_int(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return v && *v ? atoi(v) : dflt;
+}
+
+static void ensure_ctx(void)
+{
+    if (g_ctx)
+        return;
+    int rc = pq_ctx_create(&g_ctx, env_int("PQ_DEVICE", 0));
+    if (rc)
+        dev_fail("device context", rc);
+}
+
+static void device_build(const char *reads, uint64_t n, int L, int t)
+{
+    ensure_ctx();
+    uint64_t *off = malloc(sizeof(uint64_t) * (n + 1));
+    if (!off)
+        dev_fail("offsets", PQ_ERR_NOMEM);
+    for (uint64_t i = 0; i <= n; i++)
+        off[i] = i * (uint64_t) L;
+    int rc = pq_upload_reads(g_ctx, (const uint8_t *) reads, off, n, (uint32_t) t);
+    free(off);
+    if (rc)
+        dev_fail("read upload", rc);
+    if ((rc = pq_build_index(g_ctx)))
+        dev_fail("index build", rc);
+}
+
+static void export_host_index(index_t *ix)
+{
+    uint64_t n = ix->num_reads, N = n * (uint64_t) ix->read_len, rows = (N + 64) / 64;
+    uint64_t *bm = malloc(sizeof(uint64_t) * 5 * rows);
+    uint64_t *ri = malloc(sizeof(uint64_t) * (n + 1));
+    uint64_t *ss = malloc(sizeof(uint64_t) * (n + 1));
+    uint64_t *sz = malloc(sizeof(uint64_t) * (n + 1));
+    if (!bm || !ri || !ss || !sz)
+        dev_fail("host index buffers", PQ_ERR_NOMEM);
+    for (int d = 0; d < 2; d++) {
+        INDEX_TYPE_T *sample = malloc(sizeof(INDEX_TYPE_T) * 5 * rows);
+        if (!sample)
+            dev_fail("host index buffers", PQ_ERR_NOMEM);
+        int rc = pq_get_occ_ref(g_ctx, d, bm, sample, (int) sizeof(INDEX_TYPE_T));
+        if (rc)
+            dev_fail("occurrence export", rc);
+        unsigned long long **rowp = d ? ix->OCC_bitmap_bw : ix->OCC_bitmap;
+        for (int c = 0; c < 5; c++) {
+            rowp[c] = malloc(sizeof(unsigned long long) * rows);
+            if (!rowp[c])
+                dev_fail("host index buffers", PQ_ERR_NOMEM);
+            memcpy(rowp[c], bm + (uint64_t) c * rows, sizeof(uint64_t) * rows);
+        }
+        INDEX_TYPE_T *rix = malloc(sizeof(INDEX_TYPE_T) * (n ? n : 1));
+        INDEX_TYPE_T *st = malloc(sizeof(INDEX_TYPE_T) * (n ? n : 1));
+        INDEX_TYPE_T *en = malloc(sizeof(INDEX_TYPE_T) * (n ? n : 1));
+        unsigned short *dp = malloc(sizeof(unsigned short) * (n ? n : 1));
+        if (!rix || !st || !en || !dp)
+            dev_fail("host index buffers", PQ_ERR_NOMEM);
+        if ((rc = pq_get_reads_index(g_ctx, d, ri)))
+            dev_fail("reads_index export", rc);
+        if ((rc = pq_get_shortcut(g_ctx, d, ss, sz, dp)))
+            dev_fail("shortcut export", rc);
+        for (uint64_t i = 0; i < n; i++) {
+            rix[i] = (INDEX_TYPE_T) ri[i];
+            st[i] = (INDEX_TYPE_T) ss[i];
+            en[i] = (INDEX_TYPE_T) sz[i];
+        }
+        if (d == 0) {
+            ix->OCC_sample = sample;
+            ix->reads_index = rix;
+            ix->interval_start = st;
+            ix->interval_end = en;
+            ix->interval_depth = dp;
+        } else {
+            ix->OCC_sample_bw = sample;
+            ix->reads_index_bw = rix;
+            ix->interval_start_bw = st;
+            ix->interval_end_bw = en;
+            ix->interval_depth_bw = dp;
+        }
+    }
+    free(bm);
+    free(ri);
+    free(ss);
+    free(sz);
+}
+
+index_t *construct_index(char *single_file, char *pair_file1, char *pair_file2,
+                         char *mate_file1, char *mate_file2, int min_overlap,
+                         int read_len, int bloom_m, int n_hashfunc)
+{
+    index_t *ix = calloc(1, sizeof(index_t));
+    if (!ix)
+        dev_fail("index record", PQ_ERR_NOMEM);
+    ix->min_overlap = (unsigned short) min_overlap;
+    ix->read_len = (unsigned short) read_len;
+    ix->n_hashfunc = n_hashfunc;
+    if (single_file != NULL) {
+        pq_import_t imp;
+        printf("Importing reads from %s ... ", single_file);
+        fflush(stdout);
+        int rc = pq_import_fasta(single_file, read_len, bloom_m, n_hashfunc,
+                                 (int) (8 * sizeof(INDEX_TYPE_T)), &imp);
+        if (rc == PQ_ERR_IO)
+            REF_ERROR_EXIT("open reads file %s failed.\n", single_file);
+        if (rc == PQ_ERR_FORMAT)
+            REF_ERROR_EXIT("incorrect reads file at line %ld.\n", imp.bad_line);
+        if (rc)
+            dev_fail("reads import", rc);
+        ix->reads = (char *) imp.reads;
+        ix->num_reads = (INDEX_TYPE_T) imp.n;
+        for (int c = 0; c < 6; c++)
+            ix->C[c] = (INDEX_TYPE_T) imp.C[c];
+        for (int c = 0; c < 4; c++) {
+            ix->suffix_count[c] = (INDEX_TYPE_T) imp.suffix_count[c];
+            ix->prefix_count[c] = (INDEX_TYPE_T) imp.prefix_count[c];
+        }
+        printf("done\n");
+        printf("  has imported %llu reads (removed %llu duplicate reads)\n",
+               (unsigned long long) imp.n, (unsigned long long) imp.removed);
+        printf("  takes %.3lf ms\n", imp.ms);
+        fflush(stdout);
+        time_importreads = imp.ms;
+    } else if (pair_file1 != NULL && pair_file2 != NULL) {
+        REF_ERROR_EXIT("We do not support paired end reads.\nPlease input them as single reads.\n");
+    } else if (mate_file1 != NULL && mate_file2 != NULL) {
+        REF_ERROR_EXIT("We do not support mate pair reads.\nPlease input them as single reads.\n");
+    }
+
+    device_build(ix->reads, ix->num_reads, read_len, min_overlap);
+    g_index = ix;
+    pq_timing_t tm;
+    pq_get_timing(g_ctx, &tm);
+    printf("Sorting forward suffixes ... done\n  takes %.3lf ms\n", tm.sa_ms[0] + tm.occ_ms[0]);
+    printf("Sorting backward suffixes ... done\n  takes %.3lf ms\n", tm.sa_ms[1] + tm.occ_ms[1]);
+    fflush(stdout);
+    if (env_int("PQ_HOST_INDEX", 1))
+        export_host_index(ix);
+    return ix;
+}
+
+void free_index(index_t *ix)
+{
+    if (!ix)
+        return;
+    for (int c = 0; c < 5; c++) {
+        free(ix->OCC_bitmap[c]);
+        free(ix->OCC_bitmap_bw[c]);
+    }
+    free(ix->OCC_sample);
+    free(ix->OCC_sample_bw);
+    free(ix->reads_index);
+    free(ix->reads_index_bw);
+    free(ix->interval_start);
+    free(ix->interval_end);
+    free(ix->interval_depth);
+    free(ix->interval_start_bw);
+    free(ix->interval_end_bw);
+    free(ix->interval_depth_bw);
+    free(ix->reads);
+    if (g_index == ix)
+        g_index = NULL;
+    free(ix);
+}
+
+string_graph_t *construct_string_graph(index_t *ix)
+{
+    double t0 = now_ms();
+    printf("Building overlap graph ... ");
+    fflush(stdout);
+    if (g_index != ix) {
+        device_build(ix->reads, ix->num_reads, ix->read_len, ix->min_overlap);
+        g_index = ix;
+    }
+    int mode = PQ_SEARCH_PLAIN;
+    const char *sm = getenv("PQ_SEARCH");
+    if (sm && strcmp(sm, "learned") == 0) {
+        pq_learned_cfg_t cfg;
+        pq_learned_default(&cfg);
+        int rc = pq_learned_train(g_ctx, &cfg);
+        if (rc)
+            dev_fail("learned index training", rc);
+        mode = PQ_SEARCH_LEARNED;
+    }
+    int rc = pq_find_overlaps(g_ctx, mode);
+    if (rc == PQ_ERR_LIMIT)
+        REF_ERROR_EXIT("assembly failed. Try a larger min overlap length.\n");
+    if (rc)
+        dev_fail("overlap search", rc);
+    uint64_t n = 0, E = 0;
+    pq_graph_sizes(g_ctx, &n, &E);
+    int32_t *ovB = malloc(sizeof(int32_t) * (n + 1));
+    int32_t *ovF = malloc(sizeof(int32_t) * (n + 1));
+    uint32_t *nB = malloc(sizeof(uint32_t) * (n + 1));
+    uint32_t *nF = malloc(sizeof(uint32_t) * (n + 1));
+    uint64_t *eo = malloc(sizeof(uint64_t) * (n + 1));
+    uint64_t *dst = malloc(sizeof(uint64_t) * (E + 1));
+    uint16_t *len = malloc(sizeof(uint16_t) * (E + 1));
+    uint8_t *typ = malloc(E + 1);
+    string_graph_t *g = malloc(sizeof(string_graph_t));
+    if (!ovB || !ovF || !nB || !nF || !eo || !dst || !len || !typ || !g)
+        dev_fail("graph buffers", PQ_ERR_NOMEM);
+    if ((rc = pq_graph_copy(g_ctx, ovB, ovF, nB, nF, eo, dst, len, typ)))
+        dev_fail("graph copy", rc);
+    g->nodes = calloc(n ? n : 1, sizeof(node_t));
+    if (!g->nodes)
+        dev_fail("graph nodes", PQ_ERR_NOMEM);
+    g->num_reads = (INDEX_TYPE_T) n;
+    g->read_len = ix->read_len;
+    g->min_overlap = ix->min_overlap;
+    for (uint64_t i = 0; i < n; i++)
+        if (nB[i] > 65535 || nF[i] > 65535)
+            REF_ERROR_EXIT("assembly failed. Try a larger min overlap length.\n");
+
+    unsigned long long overlaps = 0;
+    double overlap_len = 0.0;
+    const int RL = ix->read_len;
+    /* node records, string_graph.c:832-1021 */
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : overlaps, overlap_len)
+    for (long long ii = 0; ii < (long long) n; ii++) {
+        uint64_t i = (uint64_t) ii;
+        node_t *nd = &g->nodes[i];
+        unsigned nb = nB[i], nf = nF[i];
+        nd->max_B = nd->num_B = (unsigned short) nb;
+        nd->max_F = nd->num_F = (unsigned short) nf;
+        nd->status = PQ_REF_DEFAULT;
+        nd->path_q = NULL;
+        nd->edges = malloc((nb + nf) * sizeof(edge_t));
+        int a = ovB[i], b = ovF[i], mx = a > b ? a : b;
+        double ratio = mx == 0 ? 0 : ((double) abs(a - b)) / mx;
+        nd->marked = ratio > 0.6 ? PQ_REF_SUSPECT : 0;
+        if (nb != 0 || nf != 0) {
+            nd->B_type = nb == 1 ? PQ_REF_ONE : nb > 1 ? PQ_REF_BRANCH : PQ_REF_ENDING;
+            nd->F_type = nf == 1 ? PQ_REF_ONE : nf > 1 ? PQ_REF_BRANCH : PQ_REF_ENDING;
+        } else {
+            nd->B_type = nd->F_type = PQ_REF_ENDING;
+            nd->status = PQ_REF_REMOVED;
+        }
+        int with_paths = (nd->B_type == PQ_REF_BRANCH && nd->F_type != PQ_REF_ENDING) ||
+                         (nd->F_type == PQ_REF_BRANCH && nd->B_type != PQ_REF_ENDING);
+        path_q_t *pq = NULL;
+        if (with_paths) {
+            pq = nd->path_q = malloc(sizeof(path_q_t));
+            pq->max_B_endlen = pq->max_F_endlen = 0;
+            pq->max_B_len = pq->max_F_len = 0;
+            pq->max_F_trans = pq->max_B_trans = 0;   /* never read by assemble.c */
+            pq->path = malloc(sizeof(path_t) * (nb + nf));
+        }
+        uint64_t e0 = eo[i];
+        unsigned olen = 0;
+        for (unsigned j = 0; j < nb + nf; j++) {
+            edge_t *ed = &nd->edges[j];
+            ed->dest = (INDEX_TYPE_T) dst[e0 + j];
+            ed->overlap_len = len[e0 + j];
+            ed->type = (char) typ[e0 + j];
+            olen += ed->overlap_len;
+            if (pq) {
+                path_t *p = &pq->path[j];
+                memset(p, 0, sizeof(*p));
+                p->end = p->hop = ed->dest;
+                p->len = 1;
+                p->end_len = (unsigned short) (RL - ed->overlap_len);
+                p->bubble_len = 0;
+                p->type = PQ_REF_PATH_NONE;
+                p->direction = j < nb ? (ed->type == PQ_REF_BB) : (ed->type == PQ_REF_FB);
+            }
+        }
+        overlaps += nb + nf;
+        overlap_len += (double) olen;
+    }
+    g->average_overlap = overlap_len / (double) overlaps;
+    free(ovB);
+    free(ovF);
+    free(nB);
+    free(nF);
+    free(eo);
+    free(dst);
+    free(len);
+    free(typ);
+    printf("  takes %.3lf ms\n", now_ms() - t0);   /* string_graph.c:1043 */
+    printf("  average overlap length: %lf\n", g->average_overlap);
+    fflush(stdout);
+    return g;
+}
+
+void destroy_graph(string_graph_t *graph)
+{
+    (void) graph;       /* the reference leaves node memory to process exit */
+}

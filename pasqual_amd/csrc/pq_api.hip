@@ -1,0 +1,347 @@
+// pq_api.hip -- context lifetime, device memory, index build entry point and
+// read-back of the index in the reference's host layout.
+#include "pq_ctx.h"
+
+#include <chrono>
+#include <cstring>
+
+void pq_index_release(pq_ctx_t *ctx);
+
+int pq_alloc(pq_ctx_t *ctx, void **p, size_t bytes)
+{
+    *p = nullptr;
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void) hipGetLastError();
+        return ctx->fail(PQ_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) +
+                                           " bytes: " + hipGetErrorString(e));
+    }
+    return PQ_OK;
+}
+
+void pq_free(void *p)
+{
+    if (p)
+        (void) hipFree(p);
+}
+
+void *pq_ctx::tmp(int slot, size_t bytes)
+{
+    if ((size_t) slot >= scratch.size())
+        scratch.resize(slot + 1);
+    DevBuf &b = scratch[slot];
+    if (b.bytes >= bytes)
+        return b.p;
+    if (b.p) {
+        (void) hipStreamSynchronize(stream);
+        (void) hipFree(b.p);
+    }
+    b.p = nullptr;
+    b.bytes = 0;
+    size_t want = bytes + bytes / 4 + 256;
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        (void) hipGetLastError();
+        b.p = nullptr;
+        return nullptr;
+    }
+    b.bytes = want;
+    return b.p;
+}
+
+extern "C" int pq_ctx_create(pq_ctx_t **out, int device)
+{
+    if (!out)
+        return PQ_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void) hipGetLastError();
+        return PQ_ERR_HIP;
+    }
+    if (device < 0 || device >= ndev)
+        return PQ_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess)
+        return PQ_ERR_HIP;
+    pq_ctx_t *ctx = new pq_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return PQ_ERR_HIP;
+    }
+    *out = ctx;
+    return PQ_OK;
+}
+
+extern "C" void pq_ctx_destroy(pq_ctx_t *ctx)
+{
+    if (!ctx)
+        return;
+    (void) hipSetDevice(ctx->device);
+    (void) hipStreamSynchronize(ctx->stream);
+    pq_index_release(ctx);
+    pq_free(ctx->d_ovB);
+    pq_free(ctx->d_ovF);
+    pq_free(ctx->d_numB);
+    pq_free(ctx->d_numF);
+    pq_free(ctx->d_edge_off);
+    pq_free(ctx->d_edges);
+    pq_learned_free(ctx);
+    for (DevBuf &b : ctx->scratch)
+        pq_free(b.p);
+    (void) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char *pq_last_error(const pq_ctx_t *ctx)
+{
+    return ctx ? ctx->err.c_str() : "no context";
+}
+
+extern "C" uint64_t pq_num_reads(const pq_ctx_t *ctx)
+{
+    return ctx ? ctx->n : 0;
+}
+
+extern "C" uint64_t pq_text_len(const pq_ctx_t *ctx)
+{
+    return ctx ? ctx->N : 0;
+}
+
+extern "C" int pq_sync(pq_ctx_t *ctx)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    return PQ_OK;
+}
+
+extern "C" int pq_build_index(pq_ctx_t *ctx)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (!ctx->dir[0].text2)
+        return ctx->fail(PQ_ERR_STATE, "no reads uploaded");
+    PQ_CHECK(hipSetDevice(ctx->device));
+    ctx->searched = false;
+    pq_learned_free(ctx);
+    hipEvent_t ev[5];
+    for (auto &e : ev)
+        PQ_CHECK(hipEventCreate(&e));
+    int rc = PQ_OK;
+    for (int d = 0; d < 2 && rc == PQ_OK; d++) {
+        PQ_CHECK(hipEventRecord(ev[0], ctx->stream));
+        rc = pq_build_sa_dir(ctx, d);
+        if (rc)
+            break;
+        PQ_CHECK(hipEventRecord(ev[1], ctx->stream));
+        rc = pq_build_occ_dir(ctx, d);
+        if (rc)
+            break;
+        PQ_CHECK(hipEventRecord(ev[2], ctx->stream));
+        PQ_CHECK(hipEventSynchronize(ev[2]));
+        float a = 0, b = 0;
+        (void) hipEventElapsedTime(&a, ev[0], ev[1]);
+        (void) hipEventElapsedTime(&b, ev[1], ev[2]);
+        ctx->timing.sa_ms[d] = a;
+        ctx->timing.occ_ms[d] = b;
+    }
+    for (auto &e : ev)
+        (void) hipEventDestroy(e);
+    return rc;
+}
+
+// ---- read-back ------------------------------------------------------------------------------
+
+__global__ void k_bwt_bytes(const OccLine *occ, uint64_t N, uint8_t *out)
+{
+    uint64_t row = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= N)
+        return;
+    const OccLine &ln = occ[row >> 7];
+    int h = (int) ((row >> 6) & 1);
+    unsigned j = (unsigned) (row & 63);
+    unsigned s = (unsigned) ((ln.p0[h] >> j) & 1) | (unsigned) (((ln.p1[h] >> j) & 1) << 1) |
+                 (unsigned) (((ln.p2[h] >> j) & 1) << 2);
+    out[row] = (uint8_t) s;
+}
+
+__global__ void k_occ_ref(const OccLine *occ, uint64_t rows, uint4 fix, uint64_t *bitmap,
+                          void *sample, int sbytes)
+{
+    uint64_t w = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= rows)
+        return;
+    const OccLine &ln = occ[w >> 1];
+    int h = (int) (w & 1);
+    uint64_t a = ln.p0[h], b = ln.p1[h], c = ln.p2[h];
+    uint64_t eq[5] = {~a & ~b & ~c, a & ~b & ~c, ~a & b & ~c, a & b & ~c, ~a & ~b & c};
+    uint64_t base[5];
+    uint64_t sum = (uint64_t) ln.cnt[0] + ln.cnt[1] + ln.cnt[2] + ln.cnt[3];
+    base[0] = (w >> 1) * 128 - sum;
+    for (int k = 1; k < 5; k++)
+        base[k] = ln.cnt[k - 1];
+    if (h) {
+        uint64_t a0 = ln.p0[0], b0 = ln.p1[0], c0 = ln.p2[0];
+        uint64_t eq0[5] = {~a0 & ~b0 & ~c0, a0 & ~b0 & ~c0, ~a0 & b0 & ~c0, a0 & b0 & ~c0,
+                           ~a0 & ~b0 & c0};
+        for (int k = 0; k < 5; k++)
+            base[k] += (uint64_t) __popcll(eq0[k]);
+    }
+    const uint32_t fx[5] = {0u, fix.x, fix.y, fix.z, fix.w};
+    for (int k = 0; k < 5; k++) {
+        bitmap[(uint64_t) k * rows + w] = eq[k] << 1;   // bit j+1 = row 64w+j (j <= 62)
+        uint64_t v = base[k] + fx[k];
+        if (sbytes == 4)
+            ((uint32_t *) sample)[5 * w + k] = (uint32_t) v;
+        else
+            ((uint64_t *) sample)[5 * w + k] = v;
+    }
+}
+
+__global__ void k_shortcut(Geom G, const uint64_t *fwt, uint64_t n, int dir, uint4 Clo,
+                           uint32_t C5, uint64_t *start, uint64_t *size, uint16_t *depth)
+{
+    uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    uint64_t s = pq_read_start(G, r), L = pq_read_len(G, r);
+    unsigned c = pq_code(fwt, dir == 0 ? s + L - 1 : s) + 1;
+    const uint32_t Cc[6] = {0u, 0u, Clo.y, Clo.z, Clo.w, C5};
+    start[r] = Cc[c];
+    size[r] = Cc[c + 1] - Cc[c];
+    depth[r] = 1;
+}
+
+static int dir_ok(pq_ctx_t *ctx, int dir)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (dir != 0 && dir != 1)
+        return ctx->fail(PQ_ERR_ARG, "dir must be 0 or 1");
+    if (!ctx->dir[dir].built)
+        return ctx->fail(PQ_ERR_STATE, "index not built");
+    return PQ_OK;
+}
+
+extern "C" int pq_get_sa(pq_ctx_t *ctx, int dir, uint64_t *sa)
+{
+    int rc = dir_ok(ctx, dir);
+    if (rc)
+        return rc;
+    std::vector<uint32_t> h(ctx->N);
+    if (ctx->N)
+        PQ_CHECK(hipMemcpy(h.data(), ctx->dir[dir].sa, ctx->N * 4, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < ctx->N; i++)
+        sa[i] = h[i];
+    return PQ_OK;
+}
+
+extern "C" int pq_get_bwt(pq_ctx_t *ctx, int dir, uint8_t *bwt)
+{
+    int rc = dir_ok(ctx, dir);
+    if (rc)
+        return rc;
+    if (!ctx->N)
+        return PQ_OK;
+    uint8_t *d = (uint8_t *) ctx->tmp(4, ctx->N);
+    if (!d)
+        return ctx->fail(PQ_ERR_NOMEM, "bwt scratch");
+    hipLaunchKernelGGL(k_bwt_bytes, dim3(pq_blocks(ctx->N, 256)), dim3(256), 0, ctx->stream,
+                       ctx->dir[dir].occ, ctx->N, d);
+    PQ_CHECK(hipMemcpyAsync(bwt, d, ctx->N, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    return PQ_OK;
+}
+
+extern "C" int pq_get_reads_index(pq_ctx_t *ctx, int dir, uint64_t *ri)
+{
+    int rc = dir_ok(ctx, dir);
+    if (rc)
+        return rc;
+    std::vector<uint32_t> h(ctx->n);
+    if (ctx->n)
+        PQ_CHECK(hipMemcpy(h.data(), ctx->dir[dir].reads_index, ctx->n * 4, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < ctx->n; i++)
+        ri[i] = h[i];
+    return PQ_OK;
+}
+
+extern "C" int pq_get_counts(pq_ctx_t *ctx, uint64_t C[6], uint64_t suffix_count[4],
+                             uint64_t prefix_count[4])
+{
+    int rc = dir_ok(ctx, 0);
+    if (rc)
+        return rc;
+    if (C)
+        memcpy(C, ctx->C, sizeof(ctx->C));
+    if (suffix_count)
+        memcpy(suffix_count, ctx->sfx, sizeof(ctx->sfx));
+    if (prefix_count)
+        memcpy(prefix_count, ctx->pfx, sizeof(ctx->pfx));
+    return PQ_OK;
+}
+
+extern "C" int pq_get_occ_ref(pq_ctx_t *ctx, int dir, uint64_t *bitmap, void *sample,
+                              int sample_bytes)
+{
+    int rc = dir_ok(ctx, dir);
+    if (rc)
+        return rc;
+    if (sample_bytes != 4 && sample_bytes != 8)
+        return ctx->fail(PQ_ERR_ARG, "sample_bytes must be 4 or 8");
+    uint64_t rows = (ctx->N + 64) / 64;
+    size_t bbytes = rows * 5 * 8, sbytes = rows * 5 * (size_t) sample_bytes;
+    char *d = (char *) ctx->tmp(4, bbytes + sbytes);
+    if (!d)
+        return ctx->fail(PQ_ERR_NOMEM, "occ export scratch");
+    const uint64_t *fix = dir == 0 ? ctx->sfx : ctx->pfx;
+    uint4 f = make_uint4((uint32_t) fix[0], (uint32_t) fix[1], (uint32_t) fix[2], (uint32_t) fix[3]);
+    hipLaunchKernelGGL(k_occ_ref, dim3(pq_blocks(rows, 256)), dim3(256), 0, ctx->stream,
+                       ctx->dir[dir].occ, rows, f, (uint64_t *) d, (void *) (d + bbytes),
+                       sample_bytes);
+    PQ_CHECK(hipGetLastError());
+    if (bitmap)
+        PQ_CHECK(hipMemcpyAsync(bitmap, d, bbytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (sample)
+        PQ_CHECK(hipMemcpyAsync(sample, d + bbytes, sbytes, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    return PQ_OK;
+}
+
+extern "C" int pq_get_shortcut(pq_ctx_t *ctx, int dir, uint64_t *start, uint64_t *size,
+                               uint16_t *depth)
+{
+    int rc = dir_ok(ctx, dir);
+    if (rc)
+        return rc;
+    uint64_t n = ctx->n;
+    if (!n)
+        return PQ_OK;
+    char *d = (char *) ctx->tmp(4, n * 18 + 64);
+    if (!d)
+        return ctx->fail(PQ_ERR_NOMEM, "shortcut scratch");
+    uint64_t *ds = (uint64_t *) d, *dz = ds + n;
+    uint16_t *dd = (uint16_t *) (dz + n);
+    uint4 Clo = make_uint4(0, (uint32_t) ctx->C[2], (uint32_t) ctx->C[3], (uint32_t) ctx->C[4]);
+    hipLaunchKernelGGL(k_shortcut, dim3(pq_blocks(n, 256)), dim3(256), 0, ctx->stream,
+                       ctx->geom(), ctx->dir[0].text2, n, dir, Clo, (uint32_t) ctx->C[5], ds, dz,
+                       dd);
+    PQ_CHECK(hipGetLastError());
+    if (start)
+        PQ_CHECK(hipMemcpyAsync(start, ds, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (size)
+        PQ_CHECK(hipMemcpyAsync(size, dz, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (depth)
+        PQ_CHECK(hipMemcpyAsync(depth, dd, n * 2, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    return PQ_OK;
+}
+
+extern "C" int pq_get_timing(pq_ctx_t *ctx, pq_timing_t *t)
+{
+    if (!ctx || !t)
+        return PQ_ERR_ARG;
+    *t = ctx->timing;
+    return PQ_OK;
+}

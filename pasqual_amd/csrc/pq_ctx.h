@@ -1,0 +1,115 @@
+// pq_ctx.h -- library context (internal).  One context = one device, one HIP
+// stream, and every device buffer of the index and of the last search.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/pasqual_amd.h"
+#include "pq_dev.h"
+
+#define PQ_CHECK(call)                                                         \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess)                                                  \
+            return ctx->fail(PQ_ERR_HIP, std::string(#call) + ": " +           \
+                                             hipGetErrorString(e_));           \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+// Per-direction device index.
+struct DirIndex {
+    uint64_t *text2 = nullptr;     // packed direction text
+    uint32_t *sa = nullptr;
+    OccLine *occ = nullptr;
+    uint32_t *reads_index = nullptr;
+    uint64_t nlines = 0;
+    uint32_t cf[5] = {0, 0, 0, 0, 0};
+    bool built = false;
+};
+
+struct Learned;                    // pq_learned.hip
+
+struct pq_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // reads
+    uint64_t n = 0, N = 0;
+    uint32_t L = 0, Lmax = 0, Lmin = 0, t = 0;
+    std::vector<uint64_t> h_off;   // host copy of offsets
+    uint64_t C[6] = {0}, sfx[4] = {0}, pfx[4] = {0};
+    uint64_t *d_off = nullptr;
+    uint64_t *d_ends = nullptr;    // end-of-read bitmap
+    uint32_t *d_end_rank = nullptr;
+
+    DirIndex dir[2];
+
+    // search result (device)
+    int32_t *d_ovB = nullptr, *d_ovF = nullptr;
+    uint32_t *d_numB = nullptr, *d_numF = nullptr;
+    uint64_t *d_edge_off = nullptr;
+    uint64_t *d_edges = nullptr;   // packed (dest<<32 | len<<8 | type)
+    uint64_t n_edges = 0;
+    bool searched = false;
+
+    Learned *learned = nullptr;
+    pq_timing_t timing{};
+
+    std::vector<DevBuf> scratch;   // reusable temporaries, by slot
+
+    int fail(int code, const std::string &msg)
+    {
+        err = msg;
+        return code;
+    }
+    Geom geom() const
+    {
+        Geom g;
+        g.n = n;
+        g.N = N;
+        g.L = L;
+        g.Lmax = Lmax;
+        g.off = d_off;
+        g.ends = d_ends;
+        g.end_rank = d_end_rank;
+        return g;
+    }
+    DirView view(int d) const
+    {
+        DirView v;
+        v.text2 = dir[d].text2;
+        v.sa = dir[d].sa;
+        v.occ = dir[d].occ;
+        v.reads_index = dir[d].reads_index;
+        for (int c = 0; c < 5; c++)
+            v.cf[c] = dir[d].cf[c];
+        return v;
+    }
+    // Scratch slot `slot` with at least `bytes` bytes (grown, never shrunk).
+    void *tmp(int slot, size_t bytes);
+};
+
+// helpers shared across translation units
+int pq_alloc(pq_ctx_t *ctx, void **p, size_t bytes);
+void pq_free(void *p);
+int pq_scan_u32(pq_ctx_t *ctx, const uint32_t *in, uint32_t *out, uint64_t count,
+                uint32_t *total_host);
+int pq_scan_u64(pq_ctx_t *ctx, const uint64_t *in, uint64_t *out, uint64_t count,
+                uint64_t *total_host);
+int pq_build_sa_dir(pq_ctx_t *ctx, int d);
+int pq_build_occ_dir(pq_ctx_t *ctx, int d);
+void pq_learned_free(pq_ctx_t *ctx);
+int pq_learned_seed(pq_ctx_t *ctx, int chain_dir, const uint64_t *d_kmers, uint64_t count,
+                    uint32_t *d_lo, uint32_t *d_hi);
+
+static inline unsigned pq_blocks(uint64_t work, unsigned per)
+{
+    return (unsigned) ((work + per - 1) / per);
+}

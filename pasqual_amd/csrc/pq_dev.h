@@ -1,0 +1,172 @@
+// pq_dev.h -- device-side data layout shared by the index build and the overlap
+// search kernels (gfx950, wave64).
+//
+// HBM layout of one direction of the index (forward text = the reads; backward
+// text = every read reversed, not complemented, as in sufsort.c:80-121):
+//
+//   text2  : 2-bit packed text, 32 codes per 64-bit word, code j of word w in
+//            bits [63-2j-1, 63-2j] (most significant first) so that a word
+//            compares like the string it holds.  A=0 C=1 G=2 T=3.  One spare
+//            word at the end so that any 32-code window can be read with two
+//            loads.
+//   ends   : 1 bit per text position, set on the last code of every read
+//            (shared by both directions: a reversed read occupies the same
+//            range as the read).
+//   sa     : suffix array, uint32 positions.
+//   occ    : one 64-byte OccLine per 128 BWT rows -- three bit planes of the
+//            3-bit row symbol ($=0 A=1 C=2 G=3 T=4, padding 7) plus the counts
+//            of A, C, G, T before the line.  A backward-search step touches the
+//            lines of l and u+1 only (one line when both fall in it), and a
+//            line holds every symbol, so a bidirectional step needs no other
+//            memory.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PQ_WAVE 64
+
+struct __attribute__((aligned(64))) OccLine {
+    uint64_t p0[2], p1[2], p2[2];
+    uint32_t cnt[4];
+};
+static_assert(sizeof(OccLine) == 64, "OccLine must be one 64-byte line");
+
+// Everything a search kernel needs about one direction.
+struct DirView {
+    const uint64_t *text2;
+    const uint32_t *sa;
+    const OccLine *occ;
+    const uint32_t *reads_index;
+    uint32_t cf[5];          // backward-step base: C[c] + fix[c]; cf[0] = 0
+};
+
+// Read geometry: either a fixed read length or per-read offsets + end bitmap.
+struct Geom {
+    uint64_t n;              // reads
+    uint64_t N;              // text length
+    uint32_t L;              // fixed length, or 0 when lengths vary
+    uint32_t Lmax;
+    const uint64_t *off;     // n+1 read offsets
+    const uint64_t *ends;    // end-of-read bitmap
+    const uint32_t *end_rank;// #reads ending before each 64-position word
+};
+
+__device__ __forceinline__ uint64_t pq_window(const uint64_t *t2, uint64_t g)
+{
+    // 32 codes starting at text position g, most significant first
+    uint64_t w = g >> 5;
+    unsigned s = (unsigned) (g & 31) * 2;
+    uint64_t a = t2[w];
+    if (s == 0)
+        return a;
+    return (a << s) | (t2[w + 1] >> (64 - s));
+}
+
+__device__ __forceinline__ unsigned pq_code(const uint64_t *t2, uint64_t g)
+{
+    return (unsigned) (t2[g >> 5] >> (62 - 2 * (g & 31))) & 3u;
+}
+
+// Read containing text position g.
+__device__ __forceinline__ uint64_t pq_read_of(const Geom &G, uint64_t g)
+{
+    if (G.L)
+        return g / G.L;
+    uint64_t w = g >> 6;
+    uint64_t below = G.ends[w] & ((1ull << (g & 63)) - 1ull);
+    return G.end_rank[w] + (uint64_t) __popcll(below);
+}
+
+__device__ __forceinline__ uint64_t pq_read_start(const Geom &G, uint64_t r)
+{
+    return G.L ? r * (uint64_t) G.L : G.off[r];
+}
+
+__device__ __forceinline__ uint32_t pq_read_len(const Geom &G, uint64_t r)
+{
+    return G.L ? G.L : (uint32_t) (G.off[r + 1] - G.off[r]);
+}
+
+// Codes remaining from g to the end of its read (>= 1).
+__device__ __forceinline__ uint64_t pq_rem(const Geom &G, uint64_t g)
+{
+    if (G.L)
+        return G.L - (uint32_t) (g % G.L);
+    uint64_t r = pq_read_of(G, g);
+    return G.off[r + 1] - g;
+}
+
+__device__ __forceinline__ bool pq_is_start(const Geom &G, uint64_t g)
+{
+    if (G.L)
+        return g % G.L == 0;
+    if (g == 0)
+        return true;
+    uint64_t p = g - 1;
+    return (G.ends[p >> 6] >> (p & 63)) & 1ull;
+}
+
+// ---- occurrence lines -------------------------------------------------------
+
+struct Ranks {
+    uint32_t r[5];           // rank of $,A,C,G,T before the position
+};
+
+__device__ __forceinline__ void pq_line_ranks(const OccLine &ln, uint64_t line_idx,
+                                              unsigned j, Ranks &out)
+{
+    uint64_t m0 = j >= 64 ? ~0ull : ((1ull << j) - 1ull);
+    uint64_t m1 = j > 64 ? ((1ull << (j - 64)) - 1ull) : 0ull;
+    uint64_t a0 = ln.p0[0], a1 = ln.p0[1];
+    uint64_t b0 = ln.p1[0], b1 = ln.p1[1];
+    uint64_t c0 = ln.p2[0], c1 = ln.p2[1];
+    uint32_t nA = __popcll(a0 & ~b0 & ~c0 & m0) + __popcll(a1 & ~b1 & ~c1 & m1);
+    uint32_t nC = __popcll(~a0 & b0 & ~c0 & m0) + __popcll(~a1 & b1 & ~c1 & m1);
+    uint32_t nG = __popcll(a0 & b0 & ~c0 & m0) + __popcll(a1 & b1 & ~c1 & m1);
+    uint32_t nT = __popcll(~a0 & ~b0 & c0 & m0) + __popcll(~a1 & ~b1 & c1 & m1);
+    uint32_t nD = __popcll(~a0 & ~b0 & ~c0 & m0) + __popcll(~a1 & ~b1 & ~c1 & m1);
+    uint32_t base = (uint32_t) (line_idx * 128u);
+    uint32_t sum = ln.cnt[0] + ln.cnt[1] + ln.cnt[2] + ln.cnt[3];
+    out.r[0] = base - sum + nD;
+    out.r[1] = ln.cnt[0] + nA;
+    out.r[2] = ln.cnt[1] + nC;
+    out.r[3] = ln.cnt[2] + nG;
+    out.r[4] = ln.cnt[3] + nT;
+}
+
+__device__ __forceinline__ void pq_load_line(const OccLine *occ, uint64_t idx, OccLine &ln)
+{
+    const uint4 *p = reinterpret_cast<const uint4 *>(occ + idx);
+    uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+    ln.p0[0] = ((uint64_t) q0.y << 32) | q0.x;
+    ln.p0[1] = ((uint64_t) q0.w << 32) | q0.z;
+    ln.p1[0] = ((uint64_t) q1.y << 32) | q1.x;
+    ln.p1[1] = ((uint64_t) q1.w << 32) | q1.z;
+    ln.p2[0] = ((uint64_t) q2.y << 32) | q2.x;
+    ln.p2[1] = ((uint64_t) q2.w << 32) | q2.z;
+    ln.cnt[0] = q3.x;
+    ln.cnt[1] = q3.y;
+    ln.cnt[2] = q3.z;
+    ln.cnt[3] = q3.w;
+}
+
+// Ranks of all symbols before pos.
+__device__ __forceinline__ void pq_ranks(const OccLine *occ, uint64_t pos, Ranks &out)
+{
+    OccLine ln;
+    pq_load_line(occ, pos >> 7, ln);
+    pq_line_ranks(ln, pos >> 7, (unsigned) (pos & 127), out);
+}
+
+// Ranks before l and before u1 (= u+1); one line load when both share a line.
+__device__ __forceinline__ void pq_ranks2(const OccLine *occ, uint64_t l, uint64_t u1,
+                                          Ranks &rl, Ranks &ru)
+{
+    OccLine ln;
+    uint64_t wl = l >> 7, wu = u1 >> 7;
+    pq_load_line(occ, wl, ln);
+    pq_line_ranks(ln, wl, (unsigned) (l & 127), rl);
+    if (wu != wl)
+        pq_load_line(occ, wu, ln);
+    pq_line_ranks(ln, wu, (unsigned) (u1 & 127), ru);
+}

@@ -1,0 +1,655 @@
+// pq_search.hip -- FM-index overlap search (string_graph.c:162-731) on gfx950.
+//
+// Candidates (find_intervals, string_graph.c:162-383).  Four independent chains
+// per read, one lane each, four adjacent lanes per read:
+//   chain 0 FB  pattern n[L-o..L)      searched in the forward index
+//   chain 1 FF  comp(n[L-o..L))        searched in the backward index
+//   chain 2 BF  rev(n[0..o))           searched in the backward index
+//   chain 3 BB  rc(n[0..o))            searched in the forward index
+// Each chain is a bidirectional backward search from one code (or from the
+// learned seed) that keeps the partner interval in the other index; every step
+// costs the OccLines of l and u+1 only.  For t <= o < L the '$' rows of the
+// interval are reads that begin (end) with the pattern; they sit at the head of
+// the partner interval, and the chain records (o, partner start, count).
+// FB and BF stop when only the read itself is left (size < 2,
+// string_graph.c:226,311), FF and BB when the interval empties (269,354).
+//
+// Irreducible edges (extract_{fw,bw}_edges, string_graph.c:386-731), restated
+// on explicit strings.  The candidates of one side are expanded into items
+// (type, o, m) via the partner suffix array; each item's extension key is
+// m[o..) for FB/BB and rc(m)[o..) for FF/BF (2-bit codes, so comparing words
+// is comparing strings).  The reference's depth-first walk emits, in key order,
+// exactly the keys that have no other key as a proper prefix; at such a key it
+// takes the first candidate group (type order FB<FF, BF<BB, then o) and emits
+// its reads other than n in read order (reads_index order of identical
+// strings).  One wave per read evaluates this over all item pairs with the
+// keys in LDS.
+#include "pq_ctx.h"
+
+#include <chrono>
+
+#define KMAX_LDS 128
+#define KMAX_GLOBAL 65536
+#define EXT_WAVES 4
+
+struct ChainBuf {
+    uint2 *lc;               // (partner start, count)
+    uint16_t *o;             // pattern length
+    uint32_t *n;             // entries per (read, chain)
+    uint32_t cap;            // entries per chain
+};
+
+struct Seed {
+    uint32_t l, size, ll;    // search interval and partner start at depth k
+};
+
+__device__ __forceinline__ unsigned chain_sym(const uint64_t *fwt, uint64_t s, uint32_t L, int ch,
+                                              uint32_t o)
+{
+    bool from_end = ch < 2;
+    uint64_t idx = from_end ? (uint64_t) (L - o) : (uint64_t) (o - 1);
+    unsigned c = pq_code(fwt, s + idx);
+    if (ch == 1 || ch == 3)
+        c = 3u - c;
+    return c + 1u;
+}
+
+__global__ void __launch_bounds__(256) k_find(Geom G, DirView fw, DirView bw, uint64_t r0,
+                                              uint64_t nr, uint32_t t, uint4 Clo, uint32_t C5,
+                                              const Seed *seeds, uint32_t kseed, ChainBuf out)
+{
+    uint64_t tid = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t rl = tid >> 2;
+    int ch = (int) (tid & 3);
+    if (rl >= nr)
+        return;
+    uint64_t r = r0 + rl;
+    uint64_t s = pq_read_start(G, r);
+    uint32_t L = pq_read_len(G, r);
+    const bool fwsearch = (ch == 0 || ch == 3);
+    const OccLine *occ = fwsearch ? fw.occ : bw.occ;
+    const uint32_t *cf = fwsearch ? fw.cf : bw.cf;
+    const uint32_t minsize = (ch == 0 || ch == 2) ? 2u : 1u;
+    // C[0..5] = 0, 0 (A), C[2], C[3], C[4], C[5] (index.c:367-372)
+    const uint32_t Cc[6] = {0u, 0u, Clo.y, Clo.z, Clo.w, C5};
+    uint32_t o, l, size, ll;
+    if (seeds && kseed > 0 && kseed < L) {
+        Seed sd = seeds[rl * 4 + ch];
+        o = kseed;
+        l = sd.l;
+        size = sd.size;
+        ll = sd.ll;
+    } else {
+        unsigned c = chain_sym(fw.text2, s, L, ch, 1);
+        o = 1;
+        l = Cc[c];
+        size = Cc[c + 1] - Cc[c];
+        ll = l;
+    }
+    uint32_t nout = 0;
+    uint64_t base = (rl * 4 + (uint64_t) ch) * out.cap;
+    while (o < L && size >= minsize) {
+        Ranks a, b;
+        pq_ranks2(occ, l, (uint64_t) l + size, a, b);
+        if (o >= t) {
+            uint32_t cnt = b.r[0] - a.r[0];
+            if (cnt && nout < out.cap) {
+                out.lc[base + nout] = make_uint2(ll, cnt);
+                out.o[base + nout] = (uint16_t) o;
+                nout++;
+            }
+        }
+        if (o + 1 >= L)
+            break;
+        unsigned c = chain_sym(fw.text2, s, L, ch, o + 1);
+        uint32_t lt = 0;
+        for (unsigned q = 0; q < c; q++)
+            lt += b.r[q] - a.r[q];
+        ll += lt;
+        l = cf[c] + a.r[c];
+        size = b.r[c] - a.r[c];
+        o++;
+    }
+    out.n[rl * 4 + ch] = nout;
+}
+
+// ---- extraction ----------------------------------------------------------------------------
+
+struct ItemMeta {
+    uint32_t m;
+    uint16_t o;
+    uint16_t klen;
+    uint8_t rank, type, valid, emit;
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// -1/+1: a sorts before/after b with a difference inside the common length;
+// -2: a is a proper prefix of b; +2: b is a proper prefix of a; 0: equal.
+__device__ __forceinline__ int key_cmp(const uint64_t *a, uint32_t la, const uint64_t *b,
+                                       uint32_t lb)
+{
+    uint32_t mn = la < lb ? la : lb;
+    uint32_t full = mn >> 5, rem = mn & 31;
+    for (uint32_t w = 0; w < full; w++) {
+        uint64_t x = a[w], y = b[w];
+        if (x != y)
+            return x < y ? -1 : 1;
+    }
+    if (rem) {
+        uint64_t mask = ~0ull << (64 - 2 * rem);
+        uint64_t x = a[full] & mask, y = b[full] & mask;
+        if (x != y)
+            return x < y ? -1 : 1;
+    }
+    return la < lb ? -2 : (la > lb ? 2 : 0);
+}
+
+struct ExtOut {
+    int32_t *ovB, *ovF;
+    uint32_t *numB, *numF;
+    uint64_t *recB, *recF;   // append positions of the read's B and F edges
+    uint64_t *edges;
+    uint64_t edge_cap;
+    unsigned long long *edge_count;
+    uint32_t *overflow_reads;     // reads left for the global-memory pass
+    uint32_t *n_overflow;
+    uint32_t *flags;              // bit0: edge buffer full, bit1: item limit
+};
+
+// Process one read with one wave.  keys/meta/emitf/pre: this wave's storage
+// (LDS or global) for kmax items.  Returns false, writing nothing, if a side
+// has more than kmax items.
+__device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw,
+                             const ChainBuf &cb, uint64_t rl, uint64_t r, uint32_t kw,
+                             uint64_t *keys, ItemMeta *meta, uint8_t *emitf, uint32_t *pre,
+                             uint32_t kmax, const ExtOut &out)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t cnt4[4];
+    for (int c = 0; c < 4; c++)
+        cnt4[c] = cb.n[rl * 4 + c];
+    // items per side first, so that an oversized read is left untouched
+    for (int side = 0; side < 2; side++) {
+        const int chA = side ? 0 : 2, chB = side ? 1 : 3;
+        uint32_t k = 0;
+        for (uint32_t e = lane; e < cnt4[chA] + cnt4[chB]; e += 64) {
+            uint64_t at = (e < cnt4[chA]) ? (rl * 4 + chA) * cb.cap + e
+                                          : (rl * 4 + chB) * cb.cap + (e - cnt4[chA]);
+            k += cb.lc[at].y;
+        }
+        for (int d = 32; d > 0; d >>= 1)
+            k += __shfl_xor(k, d, 64);
+        if (k > kmax)
+            return false;
+    }
+    // side 0 = B (chains 2: BF, 3: BB), side 1 = F (chains 0: FB, 1: FF)
+    for (int side = 0; side < 2; side++) {
+        const int chA = side ? 0 : 2, chB = side ? 1 : 3;
+        const uint32_t nA = cnt4[chA], nB = cnt4[chB], ne = nA + nB;
+        // exclusive prefix of the counts over the side's entries
+        uint32_t carry = 0;
+        for (uint32_t e0 = 0; e0 < ne; e0 += 64) {
+            uint32_t e = e0 + lane;
+            uint32_t v = 0;
+            if (e < ne) {
+                uint64_t at = (e < nA) ? (rl * 4 + chA) * cb.cap + e
+                                       : (rl * 4 + chB) * cb.cap + (e - nA);
+                v = cb.lc[at].y;
+            }
+            uint32_t inc = v;
+            for (int d = 1; d < 64; d <<= 1) {
+                uint32_t x = __shfl_up(inc, d, 64);
+                if (lane >= d)
+                    inc += x;
+            }
+            if (e < ne)
+                pre[e] = carry + inc - v;
+            carry += __shfl(inc, 63, 64);
+        }
+        const uint32_t k = carry;
+        wave_sync();
+        // expand the candidate intervals into items with their extension keys
+        for (uint32_t q = lane; q < k; q += 64) {
+            uint32_t lo = 0, hi = ne;        // entry e with pre[e] <= q < pre[e+1]
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (pre[mid] <= q)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            uint32_t e = lo;
+            bool first = e < nA;
+            int ch = first ? chA : chB;
+            uint64_t at = first ? (rl * 4 + chA) * cb.cap + e : (rl * 4 + chB) * cb.cap + (e - nA);
+            uint2 lc = cb.lc[at];
+            uint32_t o = cb.o[at];
+            // FB, BB: partner is the backward index; FF, BF: the forward index
+            bool pbw = (ch == 0 || ch == 3);
+            uint32_t g = (pbw ? bw.sa : fw.sa)[lc.x + (q - pre[e])];
+            uint64_t m = pq_read_of(G, g);
+            uint32_t Lm = pq_read_len(G, m);
+            ItemMeta mt;
+            mt.m = (uint32_t) m;
+            mt.o = (uint16_t) o;
+            mt.valid = o < Lm;                 // contained reads are no candidates
+            mt.klen = (uint16_t) (mt.valid ? Lm - o : 0);
+            mt.rank = first ? 0 : 1;
+            mt.type = ch == 0 ? 1 : ch == 1 ? 2 : ch == 2 ? 0 : 3;   // FB FF BF BB
+            mt.emit = 0;
+            // key: FB/BB m[o..) from the forward text; FF/BF rc(m)[o..) as the
+            // complement of the backward text
+            bool plain = (ch == 0 || ch == 3);
+            const uint64_t *tx = plain ? fw.text2 : bw.text2;
+            uint64_t ks = pq_read_start(G, m) + o;
+            uint64_t *kq = keys + (uint64_t) q * kw;
+            for (uint32_t w = 0; w < kw; w++) {
+                uint64_t x = 0;
+                if (32 * w < mt.klen) {
+                    x = pq_window(tx, ks + 32 * w);
+                    if (!plain)
+                        x = ~x;
+                    uint32_t left = mt.klen - 32 * w;
+                    if (left < 32)
+                        x &= ~0ull << (64 - 2 * left);
+                }
+                kq[w] = x;
+            }
+            meta[q] = mt;
+        }
+        wave_sync();
+        // an item is emitted if no other key is a proper prefix of its key, its
+        // group is the first among equal keys, and it is not the read itself
+        uint32_t valid_items = 0, emitted = 0;
+        for (uint32_t q = lane; q < k + 63; q += 64) {      // uniform trip count
+            bool mine = q < k;
+            ItemMeta mi;
+            if (mine)
+                mi = meta[q];
+            bool act = mine && mi.valid;
+            bool keep = act && mi.m != (uint32_t) r;
+            if (keep) {
+                const uint64_t *ki = keys + (uint64_t) q * kw;
+                for (uint32_t j = 0; j < k && keep; j++) {
+                    if (j == q)
+                        continue;
+                    ItemMeta mj = meta[j];
+                    if (!mj.valid)
+                        continue;
+                    int c = key_cmp(ki, mi.klen, keys + (uint64_t) j * kw, mj.klen);
+                    if (c == 2)
+                        keep = false;                    // a shorter key is a prefix
+                    else if (c == 0 &&
+                             (mj.rank < mi.rank || (mj.rank == mi.rank && mj.o < mi.o)))
+                        keep = false;                    // an earlier group wins
+                }
+            }
+            valid_items += (uint32_t) __popcll(__ballot(act));
+            emitted += (uint32_t) __popcll(__ballot(keep));
+            if (mine)
+                emitf[q] = (uint8_t) keep;
+        }
+        uint64_t base = 0;
+        if (lane == 0) {
+            if (emitted)
+                base = atomicAdd(out.edge_count, (unsigned long long) emitted);
+            if (base + emitted > out.edge_cap)
+                atomicOr(out.flags, 1u);
+            if (side) {
+                out.ovF[r] = (int32_t) valid_items;
+                out.numF[r] = emitted;
+                out.recF[r] = base;
+            } else {
+                out.ovB[r] = (int32_t) valid_items;
+                out.numB[r] = emitted;
+                out.recB[r] = base;
+            }
+        }
+        base = __shfl(base, 0, 64);
+        wave_sync();
+        // output position = rank among the emitted items by (key, rank, o, m)
+        if (emitted) {
+            for (uint32_t q = lane; q < k; q += 64) {
+                if (!emitf[q])
+                    continue;
+                ItemMeta mi = meta[q];
+                const uint64_t *ki = keys + (uint64_t) q * kw;
+                uint32_t pos = 0;
+                for (uint32_t j = 0; j < k; j++) {
+                    if (j == q || !emitf[j])
+                        continue;
+                    ItemMeta mj = meta[j];
+                    int c = key_cmp(keys + (uint64_t) j * kw, mj.klen, ki, mi.klen);
+                    bool before = c < 0 ||
+                                  (c == 0 && (mj.rank < mi.rank ||
+                                              (mj.rank == mi.rank &&
+                                               (mj.o < mi.o || (mj.o == mi.o && mj.m < mi.m)))));
+                    pos += before ? 1u : 0u;
+                }
+                uint64_t at = base + pos;
+                if (at < out.edge_cap)
+                    out.edges[at] = ((uint64_t) mi.m << 32) | ((uint64_t) mi.o << 8) | mi.type;
+            }
+        }
+        wave_sync();
+    }
+    return true;
+}
+
+__host__ __device__ static inline size_t ext_wave_bytes(uint32_t kmax, uint32_t kw, uint32_t cap)
+{
+    size_t b = (size_t) kmax * kw * 8 + (size_t) kmax * sizeof(ItemMeta) + kmax +
+               (size_t) (2 * cap + 2) * 4;
+    return (b + 15) & ~(size_t) 15;
+}
+
+__global__ void __launch_bounds__(64 * EXT_WAVES) k_extract_lds(Geom G, DirView fw, DirView bw,
+                                                                ChainBuf cb, uint64_t r0,
+                                                                uint64_t nr, uint32_t kw,
+                                                                ExtOut out)
+{
+    extern __shared__ uint64_t lds[];
+    const int wid = threadIdx.x >> 6;
+    uint64_t rl = (uint64_t) blockIdx.x * EXT_WAVES + wid;
+    if (rl >= nr)
+        return;
+    size_t per = ext_wave_bytes(KMAX_LDS, kw, cb.cap);
+    char *base = (char *) lds + per * wid;
+    uint64_t *keys = (uint64_t *) base;
+    ItemMeta *meta = (ItemMeta *) (base + (size_t) KMAX_LDS * kw * 8);
+    uint32_t *pre = (uint32_t *) ((char *) meta + (size_t) KMAX_LDS * sizeof(ItemMeta));
+    uint8_t *emitf = (uint8_t *) (pre + 2 * cb.cap + 2);
+    bool ok = extract_read(G, fw, bw, cb, rl, r0 + rl, kw, keys, meta, emitf, pre, KMAX_LDS, out);
+    if (!ok && (threadIdx.x & 63) == 0) {
+        uint32_t at = atomicAdd(out.n_overflow, 1u);
+        out.overflow_reads[at] = (uint32_t) rl;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_extract_global(Geom G, DirView fw, DirView bw, ChainBuf cb,
+                                                       uint64_t r0, const uint32_t *list,
+                                                       uint32_t nlist, uint32_t kw, char *gscratch,
+                                                       ExtOut out)
+{
+    size_t per = ext_wave_bytes(KMAX_GLOBAL, kw, cb.cap);
+    char *base = gscratch + per * blockIdx.x;
+    uint64_t *keys = (uint64_t *) base;
+    ItemMeta *meta = (ItemMeta *) (base + (size_t) KMAX_GLOBAL * kw * 8);
+    uint32_t *pre = (uint32_t *) ((char *) meta + (size_t) KMAX_GLOBAL * sizeof(ItemMeta));
+    uint8_t *emitf = (uint8_t *) (pre + 2 * cb.cap + 2);
+    for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
+        uint64_t rl = list[i];
+        bool ok = extract_read(G, fw, bw, cb, rl, r0 + rl, kw, keys, meta, emitf, pre, KMAX_GLOBAL,
+                               out);
+        if (!ok && threadIdx.x == 0)
+            atomicOr(out.flags, 2u);
+    }
+}
+
+// tot[0..n) = edges per read, tot[n] = 0 so that the exclusive scan of n+1
+// entries ends with the total
+__global__ void k_edge_totals(const uint32_t *nb, const uint32_t *nf, uint64_t n, uint64_t *tot)
+{
+    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        tot[i] = (uint64_t) nb[i] + nf[i];
+    else if (i == n)
+        tot[i] = 0;
+}
+
+__global__ void k_csr(const uint64_t *recB, const uint64_t *recF, const uint32_t *numB,
+                      const uint64_t *off, const uint64_t *src, uint64_t n, uint64_t *dst)
+{
+    uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    uint64_t a = off[r], b = off[r + 1], nb = numB[r];
+    for (uint64_t i = 0; i < nb; i++)
+        dst[a + i] = src[recB[r] + i];
+    for (uint64_t i = a + nb; i < b; i++)
+        dst[i] = src[recF[r] + (i - a - nb)];
+}
+
+static double ev_ms(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0;
+    (void) hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, Seed *d_seeds, uint32_t *kseed);
+
+extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (!ctx->dir[0].built || !ctx->dir[1].built)
+        return ctx->fail(PQ_ERR_STATE, "index not built");
+    if (mode != PQ_SEARCH_PLAIN && mode != PQ_SEARCH_LEARNED)
+        return ctx->fail(PQ_ERR_ARG, "unknown search mode");
+    if (mode == PQ_SEARCH_LEARNED && !ctx->learned)
+        return ctx->fail(PQ_ERR_STATE, "learned index not trained");
+    const uint64_t n = ctx->n;
+    const uint32_t t = ctx->t;
+    ctx->searched = false;
+    ctx->timing.find_ms = ctx->timing.extract_ms = ctx->timing.compact_ms = 0;
+    ctx->timing.learned_ms = 0;
+    ctx->timing.find_kernel_ms = 0;
+    ctx->timing.find_launches = 0;
+    pq_free(ctx->d_ovB);
+    pq_free(ctx->d_ovF);
+    pq_free(ctx->d_numB);
+    pq_free(ctx->d_numF);
+    pq_free(ctx->d_edge_off);
+    pq_free(ctx->d_edges);
+    ctx->d_ovB = ctx->d_ovF = nullptr;
+    ctx->d_numB = ctx->d_numF = nullptr;
+    ctx->d_edge_off = ctx->d_edges = nullptr;
+    ctx->n_edges = 0;
+    int rc;
+    if ((rc = pq_alloc(ctx, (void **) &ctx->d_ovB, (n + 1) * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &ctx->d_ovF, (n + 1) * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &ctx->d_numB, (n + 1) * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &ctx->d_numF, (n + 1) * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &ctx->d_edge_off, (n + 1) * 8)))
+        return rc;
+    if (n == 0) {
+        PQ_CHECK(hipMemsetAsync(ctx->d_edge_off, 0, 8, ctx->stream));
+        ctx->searched = true;
+        return PQ_OK;
+    }
+    Geom G = ctx->geom();
+    DirView fw = ctx->view(0), bw = ctx->view(1);
+    ChainBuf cb;
+    cb.cap = ctx->Lmax > t ? ctx->Lmax - t : 1;
+    uint32_t kw = (cb.cap + 31) / 32;
+    if (kw > 8)
+        return ctx->fail(PQ_ERR_LIMIT, "extension keys longer than 256 codes");
+    // batch so the candidate buffer stays within ~2 GiB
+    uint64_t per_read = 4ull * cb.cap * (sizeof(uint2) + sizeof(uint16_t)) + 16;
+    uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(n, (2ull << 30) / per_read));
+    uint64_t *rec = nullptr, *recF = nullptr, *app = nullptr, *tot = nullptr;
+    uint32_t *ovl = nullptr, *misc = nullptr;
+    uint64_t edge_cap = std::max<uint64_t>(1 << 20, 8 * n);
+    Seed *seeds = nullptr;
+    char *gscratch = nullptr;
+    hipEvent_t e0, e1, e2;
+    (void) hipEventCreate(&e0);
+    (void) hipEventCreate(&e1);
+    (void) hipEventCreate(&e2);
+    auto cleanup = [&]() {
+        pq_free(cb.lc);
+        pq_free(cb.o);
+        pq_free(cb.n);
+        pq_free(rec);
+        pq_free(recF);
+        pq_free(app);
+        pq_free(tot);
+        pq_free(ovl);
+        pq_free(misc);
+        pq_free(seeds);
+        pq_free(gscratch);
+        (void) hipEventDestroy(e0);
+        (void) hipEventDestroy(e1);
+        (void) hipEventDestroy(e2);
+    };
+    cb.lc = nullptr;
+    cb.o = nullptr;
+    cb.n = nullptr;
+    if ((rc = pq_alloc(ctx, (void **) &cb.lc, batch * 4 * cb.cap * sizeof(uint2))) ||
+        (rc = pq_alloc(ctx, (void **) &cb.o, batch * 4 * cb.cap * sizeof(uint16_t))) ||
+        (rc = pq_alloc(ctx, (void **) &cb.n, batch * 4 * sizeof(uint32_t))) ||
+        (rc = pq_alloc(ctx, (void **) &rec, (n + 1) * 8)) ||
+        (rc = pq_alloc(ctx, (void **) &recF, (n + 1) * 8)) ||
+        (rc = pq_alloc(ctx, (void **) &app, edge_cap * 8)) ||
+        (rc = pq_alloc(ctx, (void **) &tot, (n + 1) * 8)) ||
+        (rc = pq_alloc(ctx, (void **) &ovl, (batch + 1) * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &misc, 64)) ||
+        (mode == PQ_SEARCH_LEARNED &&
+         (rc = pq_alloc(ctx, (void **) &seeds, batch * 4 * sizeof(Seed))))) {
+        cleanup();
+        return rc;
+    }
+    ExtOut eo;
+    eo.ovB = ctx->d_ovB;
+    eo.ovF = ctx->d_ovF;
+    eo.numB = ctx->d_numB;
+    eo.numF = ctx->d_numF;
+    eo.recB = rec;
+    eo.recF = recF;
+    eo.edges = app;
+    eo.edge_cap = edge_cap;
+    eo.edge_count = (unsigned long long *) misc;          // misc[0..1]
+    eo.n_overflow = misc + 2;
+    eo.flags = misc + 3;
+    eo.overflow_reads = ovl;
+    (void) hipMemsetAsync(misc, 0, 64, ctx->stream);
+    // C[2..4] travel in Clo.y..w, C[5] on its own (C[0] = C[1] = 0)
+    uint4 Clo = make_uint4(0, (uint32_t) ctx->C[2], (uint32_t) ctx->C[3], (uint32_t) ctx->C[4]);
+    uint32_t C5 = (uint32_t) ctx->C[5];
+    size_t lds_bytes = ext_wave_bytes(KMAX_LDS, kw, cb.cap) * EXT_WAVES;
+    for (uint64_t r0 = 0; r0 < n && rc == PQ_OK; r0 += batch) {
+        uint64_t nr = std::min(batch, n - r0);
+        uint32_t kseed = 0;
+        if (mode == PQ_SEARCH_LEARNED) {
+            auto tl0 = std::chrono::steady_clock::now();
+            if ((rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed)))
+                break;
+            PQ_CHECK(hipStreamSynchronize(ctx->stream));
+            ctx->timing.learned_ms += std::chrono::duration<double, std::milli>(
+                                          std::chrono::steady_clock::now() - tl0)
+                                          .count();
+        }
+        (void) hipEventRecord(e0, ctx->stream);
+        hipLaunchKernelGGL(k_find, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0, ctx->stream, G, fw,
+                           bw, r0, nr, t, Clo, C5, seeds, kseed, cb);
+        (void) hipEventRecord(e1, ctx->stream);
+        hipLaunchKernelGGL(k_extract_lds, dim3(pq_blocks(nr, EXT_WAVES)), dim3(64 * EXT_WAVES),
+                           lds_bytes, ctx->stream, G, fw, bw, cb, r0, nr, kw, eo);
+        PQ_CHECK(hipGetLastError());
+        uint32_t nov = 0;
+        PQ_CHECK(hipMemcpyAsync(&nov, eo.n_overflow, 4, hipMemcpyDeviceToHost, ctx->stream));
+        PQ_CHECK(hipStreamSynchronize(ctx->stream));
+        if (nov) {
+            const uint32_t slots = 64;
+            if (!gscratch &&
+                (rc = pq_alloc(ctx, (void **) &gscratch,
+                               slots * ext_wave_bytes(KMAX_GLOBAL, kw, cb.cap))))
+                break;
+            hipLaunchKernelGGL(k_extract_global, dim3(std::min<uint32_t>(slots, nov)), dim3(64), 0,
+                               ctx->stream, G, fw, bw, cb, r0, ovl, nov, kw, gscratch, eo);
+            PQ_CHECK(hipGetLastError());
+            PQ_CHECK(hipMemsetAsync(eo.n_overflow, 0, 4, ctx->stream));
+        }
+        (void) hipEventRecord(e2, ctx->stream);
+        PQ_CHECK(hipEventSynchronize(e2));
+        double fms = ev_ms(e0, e1);
+        ctx->timing.find_ms += fms;
+        ctx->timing.find_kernel_ms += fms;
+        ctx->timing.find_launches++;
+        ctx->timing.extract_ms += ev_ms(e1, e2);
+    }
+    uint32_t flags = 0;
+    unsigned long long used = 0;
+    if (rc == PQ_OK) {
+        (void) hipMemcpy(&flags, eo.flags, 4, hipMemcpyDeviceToHost);
+        (void) hipMemcpy(&used, eo.edge_count, 8, hipMemcpyDeviceToHost);
+        if (flags & 2u)
+            rc = ctx->fail(PQ_ERR_LIMIT, "a read has more than 65536 overlap candidates on one side");
+        else if (flags & 1u)
+            rc = ctx->fail(PQ_ERR_LIMIT, "edge buffer exhausted");
+    }
+    if (rc == PQ_OK) {
+        auto tc0 = std::chrono::steady_clock::now();
+        hipLaunchKernelGGL(k_edge_totals, dim3(pq_blocks(n + 1, 256)), dim3(256), 0, ctx->stream,
+                           ctx->d_numB, ctx->d_numF, n, tot);
+        uint64_t total = 0;
+        rc = pq_scan_u64(ctx, tot, ctx->d_edge_off, n + 1, &total);
+        if (rc == PQ_OK) {
+            if ((rc = pq_alloc(ctx, (void **) &ctx->d_edges, (total + 1) * 8)) == PQ_OK) {
+                hipLaunchKernelGGL(k_csr, dim3(pq_blocks(n, 256)), dim3(256), 0, ctx->stream, rec,
+                                   recF, ctx->d_numB, ctx->d_edge_off, app, n, ctx->d_edges);
+                PQ_CHECK(hipStreamSynchronize(ctx->stream));
+                ctx->n_edges = total;
+                ctx->searched = true;
+            }
+        }
+        ctx->timing.compact_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc0).count();
+    }
+    cleanup();
+    return rc;
+}
+
+extern "C" int pq_graph_sizes(pq_ctx_t *ctx, uint64_t *n_reads, uint64_t *n_edges)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (!ctx->searched)
+        return ctx->fail(PQ_ERR_STATE, "no search result");
+    if (n_reads)
+        *n_reads = ctx->n;
+    if (n_edges)
+        *n_edges = ctx->n_edges;
+    return PQ_OK;
+}
+
+extern "C" int pq_graph_copy(pq_ctx_t *ctx, int32_t *overlaps_B, int32_t *overlaps_F,
+                             uint32_t *num_B, uint32_t *num_F, uint64_t *edge_off, uint64_t *dest,
+                             uint16_t *len, uint8_t *type)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (!ctx->searched)
+        return ctx->fail(PQ_ERR_STATE, "no search result");
+    const uint64_t n = ctx->n, E = ctx->n_edges;
+    if (overlaps_B && n)
+        PQ_CHECK(hipMemcpy(overlaps_B, ctx->d_ovB, n * 4, hipMemcpyDeviceToHost));
+    if (overlaps_F && n)
+        PQ_CHECK(hipMemcpy(overlaps_F, ctx->d_ovF, n * 4, hipMemcpyDeviceToHost));
+    if (num_B && n)
+        PQ_CHECK(hipMemcpy(num_B, ctx->d_numB, n * 4, hipMemcpyDeviceToHost));
+    if (num_F && n)
+        PQ_CHECK(hipMemcpy(num_F, ctx->d_numF, n * 4, hipMemcpyDeviceToHost));
+    if (edge_off)
+        PQ_CHECK(hipMemcpy(edge_off, ctx->d_edge_off, (n + 1) * 8, hipMemcpyDeviceToHost));
+    if ((dest || len || type) && E) {
+        std::vector<uint64_t> pk(E);
+        PQ_CHECK(hipMemcpy(pk.data(), ctx->d_edges, E * 8, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < E; i++) {
+            if (dest)
+                dest[i] = pk[i] >> 32;
+            if (len)
+                len[i] = (uint16_t) ((pk[i] >> 8) & 0xFFFF);
+            if (type)
+                type[i] = (uint8_t) (pk[i] & 0xFF);
+        }
+    }
+    return PQ_OK;
+}
