@@ -483,20 +483,6 @@ static int fm_extract(fm_t *f, int side, uint64_t base, uint64_t size,
     return 0;
 }
 
-static int fm_push(fm_t *f, uint64_t at, uint64_t ll, uint64_t tmp, uint32_t len,
-                   uint8_t type)
-{
-    if (fm_reserve(f, at + 1))
-        return -1;
-    fm_iv_t *e = &f->arena[at];
-    e->ll = ll;
-    e->uu = ll + tmp;
-    e->len = len;
-    e->type = type;
-    e->remain = tmp + 1;
-    return 0;
-}
-
 int orc_search_fm(const orc_fmindex_t *ix, orc_graph_t *out)
 {
     fm_t f;
@@ -861,4 +847,48 @@ fail:
     free(all.len);
     free(all.type);
     return -1;
+}
+
+/* ---- property check ------------------------------------------------------------ */
+
+int64_t orc_check_sa(const uint8_t *dtext, const uint64_t *off, uint64_t n,
+                     const uint64_t *sa)
+{
+    uint64_t N = off[n];
+    uint8_t *seen = calloc(N / 8 + 1, 1);
+    uint64_t *end = malloc(sizeof(uint64_t) * (N ? N : 1));
+    if (!seen || !end) {
+        free(seen);
+        free(end);
+        return (int64_t) N;
+    }
+    for (uint64_t r = 0; r < n; r++)
+        for (uint64_t g = off[r]; g < off[r + 1]; g++)
+            end[g] = off[r + 1];
+    int64_t bad = -1;
+    for (uint64_t i = 0; i < N && bad < 0; i++) {
+        uint64_t g = sa[i];
+        if (g >= N || (seen[g / 8] >> (g % 8)) & 1) {
+            bad = (int64_t) N;
+            break;
+        }
+        seen[g / 8] |= (uint8_t) (1u << (g % 8));
+        if (i == 0)
+            continue;
+        uint64_t a = sa[i - 1], ra = end[a] - a, rb = end[g] - g, m = ra < rb ? ra : rb, k = 0;
+        while (k < m && dtext[a + k] == dtext[g + k])
+            k++;
+        int ok;
+        if (k < m)
+            ok = dtext[a + k] < dtext[g + k];
+        else if (ra != rb)
+            ok = ra < rb;
+        else
+            ok = a < g;
+        if (!ok)
+            bad = (int64_t) i;
+    }
+    free(seen);
+    free(end);
+    return bad;
 }

@@ -105,6 +105,12 @@ int orc_search_explicit(const uint8_t *text, const uint64_t *off, uint64_t n,
 
 void orc_graph_free(orc_graph_t *g);
 
+/* Property check of a suffix array at any size: a permutation of 0..N-1 in
+ * strictly increasing (suffix, position) order.  Returns -1 when it holds,
+ * else the first offending SA index (or N if not a permutation). */
+int64_t orc_check_sa(const uint8_t *dtext, const uint64_t *off, uint64_t n,
+                     const uint64_t *sa);
+
 #ifdef __cplusplus
 }
 #endif

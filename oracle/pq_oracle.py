@@ -52,6 +52,8 @@ def lib():
         _lib.orc_search_fm.argtypes = [C.POINTER(OrcFm), C.POINTER(OrcGraph)]
         _lib.orc_search_explicit.argtypes = [P, P, C.c_uint64, C.c_uint32, C.POINTER(OrcGraph)]
         _lib.orc_graph_free.argtypes = [C.POINTER(OrcGraph)]
+        _lib.orc_check_sa.argtypes = [P, P, C.c_uint64, P]
+        _lib.orc_check_sa.restype = C.c_int64
     return _lib
 
 
@@ -70,6 +72,12 @@ def suffix_array(dtext, off):
     if lib().orc_sa(_p(dtext), _p(off), len(off) - 1, _p(sa)) != 0:
         raise MemoryError("oracle suffix sort")
     return sa[:int(off[-1])]
+
+
+def check_sa(dtext, off, sa):
+    """-1 if sa is the canonical suffix array of dtext, else the first bad index."""
+    sa = np.ascontiguousarray(sa, np.uint64)
+    return int(lib().orc_check_sa(_p(dtext), _p(off), len(off) - 1, _p(sa)))
 
 
 def bwt(dtext, off, sa):

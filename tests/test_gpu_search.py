@@ -1,0 +1,103 @@
+"""Device overlap search (candidate intervals + irreducible edges) against the
+reference fixtures, the oracle's restatement of string_graph.c, and the
+explicit-string oracle for reads of unequal length."""
+import numpy as np
+import pytest
+
+import pq_oracle as orc
+from conftest import golden_cases, load_golden
+from test_oracle import _random_reads
+
+pytestmark = pytest.mark.gpu
+CASES = sorted(golden_cases())
+
+
+def _same_graph(g, want, where=""):
+    for k in ("overlaps_B", "overlaps_F", "num_B", "num_F", "edge_off"):
+        a, b = np.asarray(g[k]).astype(np.int64), np.asarray(want[k]).astype(np.int64)
+        if not np.array_equal(a, b):
+            bad = np.nonzero(a != b)[0][:5]
+            raise AssertionError(f"{where} {k} differs at {bad.tolist()}: {a[bad]} vs {b[bad]}")
+    for k in ("dest", "len", "type"):
+        a, b = np.asarray(g[k]).astype(np.int64), np.asarray(want[k]).astype(np.int64)
+        if not np.array_equal(a, b):
+            bad = np.nonzero(a != b)[0][:5]
+            raise AssertionError(f"{where} edge {k} differs at {bad.tolist()}")
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_graph_matches_reference_fixture(ctx, name):
+    G = load_golden(name, "u32")
+    meta = golden_cases()[name]
+    n = int(G["n"])
+    off = np.arange(n + 1, dtype=np.uint64) * np.uint64(meta["L"])
+    ctx.upload(G["reads"], off, meta["t"])
+    ctx.build_index()
+    ctx.find_overlaps()
+    g = ctx.graph()
+    nf = G["node_fields"]
+    assert np.array_equal(g["num_B"], nf[:, 0]) and np.array_equal(g["num_F"], nf[:, 1])
+    assert np.array_equal(g["edge_off"], G["edge_off"])
+    E = np.stack([g["dest"], g["len"], g["type"]], 1).astype(np.uint64)
+    assert np.array_equal(E, G["edges"])
+    ex = orc.search_explicit(G["reads"], off, meta["t"])
+    _same_graph(g, ex, name)
+
+
+@pytest.mark.parametrize("L,t,seed", [(50, 31, 1), (100, 63, 2), (80, 25, 3), (150, 63, 4)])
+def test_graph_matches_oracle_fixed_length(ctx, L, t, seed):
+    rng = np.random.default_rng(seed)
+    codes, off, _ = _random_reads(rng, 5000, int(25 * 5000 / L), L, L, err=0.004)
+    ctx.upload(codes, off, t)
+    ctx.build_index()
+    ctx.find_overlaps()
+    g = ctx.graph()
+    ix = orc.build_index(codes, off, t)
+    _same_graph(g, orc.search_fm(ix), f"L={L}")
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_graph_matches_oracle_uneven_lengths(ctx, seed):
+    rng = np.random.default_rng(seed)
+    codes, off, _ = _random_reads(rng, 8000, 1500, 36, 250, err=0.003)
+    ctx.upload(codes, off, 31)
+    ctx.build_index()
+    ctx.find_overlaps()
+    _same_graph(ctx.graph(), orc.search_explicit(codes, off, 31), "uneven")
+
+
+def test_graph_with_many_candidates(ctx):
+    # a short genome sampled deeply: hundreds of candidates per read side, so the
+    # extraction spills past its LDS capacity into the global-memory path
+    rng = np.random.default_rng(9)
+    codes, off, _ = _random_reads(rng, 400, 1500, 40, 40, err=0.0)
+    ctx.upload(codes, off, 21)
+    ctx.build_index()
+    ctx.find_overlaps()
+    g = ctx.graph()
+    ex = orc.search_explicit(codes, off, 21)
+    assert max(ex["overlaps_F"].max(), ex["overlaps_B"].max()) > 128
+    _same_graph(g, ex, "deep")
+
+
+def _identical_copies():
+    # three copies of one read overlapped by another read: the copies form one
+    # candidate group whose members are emitted in read order
+    g = np.random.default_rng(3).integers(1, 5, 30).astype(np.uint8)
+    reads = [g[0:16], g[0:16], g[8:24], g[0:16], g[14:30]]
+    off = np.zeros(len(reads) + 1, np.uint64)
+    off[1:] = np.cumsum([len(r) for r in reads])
+    return np.concatenate(reads), off, 6
+
+
+def test_graph_edge_cases(ctx):
+    # one read; reads shorter than t; identical reads (no duplicate removal here)
+    for codes, off, t in [
+        (np.array([1, 2, 3, 4] * 10, np.uint8), np.array([0, 40], np.uint64), 12),
+        (np.array([1, 2, 3, 4, 1, 2, 3, 4, 1, 2], np.uint8), np.array([0, 5, 10], np.uint64), 12),
+        _identical_copies(),
+    ]:
+        ctx.upload(codes, off, t)
+        ctx.build_index()
+        ctx.find_overlaps()
+        _same_graph(ctx.graph(), orc.search_explicit(codes, off, t), "edge case")

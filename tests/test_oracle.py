@@ -1,0 +1,219 @@
+"""The oracle pinned: against the reference's own outputs (golden fixtures and,
+where the reference programs are built, fresh runs), against brute force on tiny
+inputs, and against its second, independent restatement of the search."""
+import os
+
+import numpy as np
+import pytest
+
+import pq_oracle as orc
+from conftest import golden_cases, load_golden
+
+CASES = sorted(golden_cases())
+
+
+def _off(n, L):
+    return np.arange(n + 1, dtype=np.uint64) * np.uint64(L)
+
+
+@pytest.fixture(scope="module")
+def built():
+    cache = {}
+
+    def get(name, width):
+        if (name, width) not in cache:
+            G = load_golden(name, width)
+            meta = golden_cases()[name]
+            n = int(G["n"])
+            off = _off(n, meta["L"])
+            ix = orc.build_index(G["reads"], off, meta["t"])
+            cache[(name, width)] = (G, meta, off, ix)
+        return cache[(name, width)]
+    return get
+
+
+@pytest.mark.parametrize("width", ["u32", "u64"])
+@pytest.mark.parametrize("name", CASES)
+def test_index_matches_reference(built, name, width):
+    G, meta, off, ix = built(name, width)
+    assert np.array_equal(ix["C"], G["C"])
+    assert np.array_equal(ix["suffix_count"], G["suffix_count"])
+    assert np.array_equal(ix["prefix_count"], G["prefix_count"])
+    for d in (0, 1):
+        assert np.array_equal(ix[d]["bitmap"], G[f"bitmap{d}"]), d
+        assert np.array_equal(ix[d]["sample"], G[f"sample{d}"]), d
+        assert np.array_equal(ix[d]["reads_index"], G[f"reads_index{d}"]), d
+        st, en, dp = ix[d]["shortcut"]
+        assert np.array_equal(st, G[f"istart{d}"]) and np.array_equal(en, G[f"iend{d}"])
+        assert np.array_equal(dp, G[f"idepth{d}"])
+
+
+@pytest.mark.parametrize("width", ["u32", "u64"])
+@pytest.mark.parametrize("name", CASES)
+def test_search_matches_reference(built, name, width):
+    G, meta, off, ix = built(name, width)
+    fm = orc.search_fm(ix)
+    ex = orc.search_explicit(G["reads"], off, meta["t"])
+    nf = G["node_fields"]
+    for g in (fm, ex):
+        assert np.array_equal(g["num_B"], nf[:, 0]) and np.array_equal(g["num_F"], nf[:, 1])
+        assert np.array_equal(g["edge_off"], G["edge_off"])
+        E = np.stack([g["dest"], g["len"], g["type"]], 1).astype(np.uint64)
+        assert np.array_equal(E, G["edges"])
+    assert np.array_equal(fm["overlaps_B"], ex["overlaps_B"])
+    assert np.array_equal(fm["overlaps_F"], ex["overlaps_F"])
+    # SUSPECT marks follow from the candidate counts (string_graph.c:843-854)
+    a, b = fm["overlaps_B"].astype(float), fm["overlaps_F"].astype(float)
+    mx = np.maximum(a, b)
+    ratio = np.where(mx == 0, 0.0, np.abs(a - b) / np.where(mx == 0, 1, mx))
+    assert np.array_equal((ratio > 0.6).astype(np.uint64) * 4, nf[:, 7])
+    tot = int(G["edge_off"][-1])
+    if tot:
+        assert np.isclose(G["edges"][:, 1].sum() / tot, float(G["average_overlap"]), rtol=0,
+                          atol=0)
+
+
+# ---- brute force on tiny inputs ------------------------------------------------------
+
+def _random_reads(rng, G, n, lmin, lmax, err=0.0):
+    genome = rng.integers(1, 5, G).astype(np.uint8)
+    reads = []
+    for _ in range(n):
+        L = int(rng.integers(lmin, lmax + 1))
+        s = int(rng.integers(0, G - L + 1))
+        r = genome[s:s + L].copy()
+        if rng.random() < 0.5:
+            r = (5 - r[::-1]).astype(np.uint8)
+        flip = rng.random(L) < err
+        r[flip] = ((r[flip] - 1 + rng.integers(1, 4, flip.sum())) % 4 + 1).astype(np.uint8)
+        reads.append(r)
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum([len(r) for r in reads])
+    return np.concatenate(reads).astype(np.uint8), off, reads
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_sa_bruteforce(seed):
+    rng = np.random.default_rng(seed)
+    varlen = seed % 2 == 1
+    codes, off, reads = _random_reads(rng, 60, 25, 8, 20 if varlen else 8)
+    for d in (0, 1):
+        dt = orc.dir_text(codes, off, d)
+        sa = orc.suffix_array(dt, off)
+        keys = []
+        for r in range(len(reads)):
+            a, b = int(off[r]), int(off[r + 1])
+            for g in range(a, b):
+                keys.append((list(dt[g:b]), g))
+        want = [g for _, g in sorted(keys)]
+        assert sa.tolist() == want
+        assert orc.check_sa(dt, off, sa) == -1
+        # a planted mistake: swapping two suffixes must be caught
+        bad = sa.copy()
+        bad[[3, 4]] = bad[[4, 3]]
+        assert orc.check_sa(dt, off, bad) != -1
+        b, ri = orc.bwt(dt, off, sa)
+        starts = set(off[:-1].tolist())
+        for i, g in enumerate(sa.tolist()):
+            assert b[i] == (0 if g in starts else dt[g - 1])
+        assert sorted(ri.tolist()) == list(range(len(reads)))
+
+
+def _dfs_graph(reads, t):
+    """The reference's depth-first extraction, simulated on explicit read sets."""
+    def rc(s):
+        return [5 - c for c in reversed(s)]
+    out = []
+    for q, s in enumerate(reads):
+        s = list(s)
+        node = {}
+        for side, query in (("B", rc(s)), ("F", s)):
+            Lq = len(query)
+            items = []
+            for o in range(t, Lq):
+                for m, M in enumerate(reads):
+                    M = list(M)
+                    if o >= len(M):
+                        continue
+                    R = rc(M)
+                    plain_hit = M[:o] == query[Lq - o:]
+                    rc_hit = R[:o] == query[Lq - o:]
+                    if side == "F":
+                        if plain_hit:
+                            items.append((0, o, m, M[o:], 1))
+                        if rc_hit:
+                            items.append((1, o, m, R[o:], 2))
+                    else:
+                        if rc_hit:
+                            items.append((0, o, m, R[o:], 0))
+                        if plain_hit:
+                            items.append((1, o, m, M[o:], 3))
+            edges = []
+
+            def dfs(S, level):
+                if level > 0:
+                    done = [it for it in S if len(it[3]) == level]
+                    if done:
+                        rank, o = min((it[0], it[1]) for it in done)
+                        grp = sorted(it[2] for it in done if it[0] == rank and it[1] == o)
+                        ty = [it[4] for it in done if it[0] == rank][0]
+                        edges.extend((m, o, ty) for m in grp if m != q)
+                        return
+                for a in (1, 2, 3, 4):
+                    sub = [it for it in S if len(it[3]) > level and it[3][level] == a]
+                    if sub:
+                        dfs(sub, level + 1)
+            dfs(items, 0)
+            node[side] = (len(items), edges)
+        out.append(node)
+    return out
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_search_bruteforce(seed):
+    rng = np.random.default_rng(100 + seed)
+    varlen = seed % 2 == 1
+    t = 6
+    codes, off, reads = _random_reads(rng, 90, 40, 10, 24 if varlen else 10,
+                                      err=0.02 if seed % 4 >= 2 else 0.0)
+    ex = orc.search_explicit(codes, off, t)
+    want = _dfs_graph(reads, t)
+    for q, node in enumerate(want):
+        a, b = int(ex["edge_off"][q]), int(ex["edge_off"][q + 1])
+        got = list(zip(ex["dest"][a:b].tolist(), ex["len"][a:b].tolist(),
+                       ex["type"][a:b].tolist()))
+        assert got == node["B"][1] + node["F"][1], q
+        assert ex["overlaps_B"][q] == node["B"][0] and ex["overlaps_F"][q] == node["F"][0]
+    if not varlen:
+        ix = orc.build_index(codes, off, t)
+        fm = orc.search_fm(ix)
+        for k in ("edge_off", "dest", "len", "type", "overlaps_B", "overlaps_F"):
+            assert np.array_equal(fm[k], ex[k]), k
+
+
+# ---- fresh runs of the reference (where its programs are built) ---------------------------
+
+@pytest.mark.skipif(not orc.ref_available(), reason="reference programs not built")
+@pytest.mark.parametrize("seed", range(3))
+def test_against_live_reference(tmp_path, seed):
+    import pasqual_amd as pq
+    c = pq.SynthCfg()
+    c.genome_len, c.len_min, c.len_max, c.coverage = 3000 + 1000 * seed, 40, 40, 15
+    c.sub_rate, c.seed = 0.004 * seed, 900 + seed
+    fa = tmp_path / "r.fa"
+    pq.synth_fasta(c, fa)
+    t = 21
+    orc.run_ref_dump(fa, 40, t, tmp_path / "d.bin")
+    R = orc.read_ref_dump(tmp_path / "d.bin")
+    off = _off(R["n"], 40)
+    ix = orc.build_index(R["reads"], off, t)
+    for d in (0, 1):
+        assert np.array_equal(ix[d]["bitmap"], R[d]["bitmap"])
+        assert np.array_equal(ix[d]["sample"], R[d]["sample"])
+        assert np.array_equal(ix[d]["reads_index"], R[d]["reads_index"])
+        assert np.array_equal(ix[d]["shortcut"][2], R[d]["idepth"])
+    fm = orc.search_fm(ix)
+    for i, nd in enumerate(R["nodes"]):
+        a, b = int(fm["edge_off"][i]), int(fm["edge_off"][i + 1])
+        E = np.stack([fm["dest"][a:b], fm["len"][a:b], fm["type"][a:b]], 1).astype(np.uint64)
+        assert np.array_equal(E, nd["edges"]), i
