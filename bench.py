@@ -1,0 +1,184 @@
+"""Benchmark of the hot path on one MI355X: suffix-array + FM-index build of
+both directions and the FM-index overlap search, on the seeded synthetic
+"repeats and errors" workload (5 Mbp genome with 100 interspersed copies of ten
+1 kbp repeat units at 1% divergence, 100 bp reads at 40x with 0.5%
+substitutions, t = 63, 32-bit index).
+
+A step = one index build (SA, BWT occurrence lines, reads_index for the forward
+and the backward text) + one overlap search (candidate intervals, irreducible
+edges, edge CSR) over the deduplicated reads, which are resident in HBM before
+the timed region.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-baseline full|off]
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+LINE_BYTES = 64
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(fasta, L, t, n_expected):
+    """The reference OpenMP program (oracle/_ref) on the same reads file, timed
+    by its own timers (index.c:643-680, string_graph.c:737-1044)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pq_oracle as orc
+    exe = os.path.join(orc.REFDIR, "pasqual_omp_u32")
+    if not os.path.exists(exe):
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread")
+    with tempfile.TemporaryDirectory() as wd:
+        t0 = time.time()
+        r = subprocess.run([exe, "--singlereads", fasta, "-l", str(L), "-t", str(t)], cwd=wd,
+                           capture_output=True, text=True, env=env, timeout=600)
+        wall = time.time() - t0
+    if r.returncode != 0:
+        log("reference run failed: " + r.stderr[-500:])
+        return None
+    out = r.stdout
+    takes = [float(x) for x in re.findall(r"takes ([0-9.]+) ms", out)]
+    n = int(re.search(r"has imported (\d+) reads", out).group(1))
+    # takes: import, forward sort, backward sort, overlap graph, assembly
+    fw, bw, graph = takes[1], takes[2], takes[3]
+    return {"reads": n, "fw_ms": fw, "bw_ms": bw, "graph_ms": graph, "wall_s": wall,
+            "threads": threads, "reads_match_gpu": n == n_expected}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--cpu-baseline", default="full", choices=["full", "off"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the overlap search shards by read ranges with no exchange; see the "
+              "design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import pasqual_amd as pq
+    cfg = pq.synth_preset(args.config)
+    L, t = cfg.len_min, cfg.min_overlap
+    tmp = tempfile.mkdtemp(prefix="pqbench_")
+    fasta = os.path.join(tmp, "reads.fa")
+    t0 = time.time()
+    pq.synth_fasta(cfg, fasta)
+    imp = pq.import_fasta(fasta, L, index_bits=32)
+    n = imp["n"]
+    import numpy as np
+    off = np.arange(n + 1, dtype=np.uint64) * np.uint64(L)
+    N = n * L
+    log(f"workload config {args.config}: {n} reads after duplicate removal, N={N} per direction,"
+        f" prepared in {time.time() - t0:.1f}s")
+
+    ctx = pq.Context(0)
+    ctx.upload(imp["reads"], off, t)
+    for _ in range(args.warmup):
+        ctx.build_index()
+        ctx.find_overlaps()
+    ctx.sync()
+    build_ms, search_ms, find_ms, find_launch_ms = [], [], [], []
+    positions = lines = launches = 0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        ctx.build_index()
+        ctx.sync()
+        b = time.perf_counter()
+        ctx.find_overlaps()
+        ctx.sync()
+        c = time.perf_counter()
+        tm = ctx.timing()
+        build_ms.append((b - a) * 1e3)
+        search_ms.append((c - b) * 1e3)
+        find_ms.append(tm["find_ms"])
+        find_launch_ms.append(tm["find_kernel_ms"])
+        launches += tm["find_launches"]
+        positions += tm["find_positions"]
+        lines += tm["find_lines"]
+    ctx.sync()
+    total_s = time.perf_counter() - t_start
+    g = ctx.graph()
+    edges = int(g["edge_off"][-1])
+    ms_per_step = total_s * 1e3 / args.steps
+    avg_search = sum(search_ms) / len(search_ms)
+    avg_build = sum(build_ms) / len(build_ms)
+    reads_per_s = n / (avg_search / 1e3)
+    gsuffix_per_s = 2 * N / (avg_build / 1e3) / 1e9
+
+    # dominant kernel: the candidate search (bidirectional backward search)
+    avg_launch_ms = sum(find_launch_ms) / max(1, launches)
+    per_launch_positions = positions / max(1, launches)
+    alg_bytes = per_launch_positions * LINE_BYTES
+    achieved = alg_bytes / (avg_launch_ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_find", "alg_bytes_per_launch": int(alg_bytes),
+                "rank_positions_per_read": round(per_launch_positions / n, 1),
+                "lines_per_read": round(lines / max(1, launches) / n, 1),
+                "avg_launch_ms": round(avg_launch_ms, 3)}
+
+    base = None
+    if args.cpu_baseline == "full":
+        log("timing the reference OpenMP program on the same reads ...")
+        ref = cpu_baseline(fasta, L, t, n)
+        if ref:
+            base = {"value": round(ref["reads"] / (ref["graph_ms"] / 1e3), 1),
+                    "unit": "reads/s", "cores": ref["threads"], "kind": "reference",
+                    "sample": f"the full workload ({ref['reads']} reads), reference "
+                              f"pasqual_omp_u32, OMP_NUM_THREADS={ref['threads']}, one run, "
+                              "overlap-graph timer (string_graph.c:737-1044)",
+                    "sa_build_gsuffix_per_s": round(2 * ref["reads"] * L /
+                                                    ((ref["fw_ms"] + ref["bw_ms"]) / 1e3) / 1e9,
+                                                    5),
+                    "wall_s": round(ref["wall_s"], 1)}
+    line = {
+        "metric": "overlap-search reads/s (plain FM-index search); SA build Gsuffix/s",
+        "value": round(reads_per_s, 1),
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded generator, pasqual_amd/csrc/host/pq_synth.c)",
+        "config": {"workload": f"config {args.config}: 5 Mbp genome + 100 copies of ten 1 kbp "
+                               "repeat units (1% div), 100 bp reads at 40x, 0.5% substitutions, "
+                               "t=63, U32 index",
+                   "reads": n, "suffixes_per_direction": N, "read_len": L, "min_overlap": t,
+                   "search": "plain"},
+        "sa_build_gsuffix_per_s": round(gsuffix_per_s, 3),
+        "search_ms": round(avg_search, 3),
+        "build_ms": round(avg_build, 3),
+        "edges": edges,
+        "roofline": roofline,
+        "cpu_baseline": base,
+    }
+    if base:
+        line["speedup_vs_cpu_search"] = round(reads_per_s / base["value"], 1)
+        line["speedup_vs_cpu_sa_build"] = round(gsuffix_per_s / base["sa_build_gsuffix_per_s"], 1)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

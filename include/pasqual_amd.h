@@ -132,6 +132,8 @@ typedef struct {
     double learned_ms;       /* learned seeding (learned mode only) */
     uint64_t find_launches;
     double find_kernel_ms;   /* summed duration of the candidate-search kernel */
+    uint64_t find_positions; /* rank positions evaluated by it (2 per step) */
+    uint64_t find_lines;     /* occurrence lines it loaded (1 or 2 per step) */
 } pq_timing_t;
 int pq_get_timing(pq_ctx_t *ctx, pq_timing_t *t);
 int pq_sync(pq_ctx_t *ctx);

@@ -84,14 +84,16 @@ class Timing(C.Structure):
     _fields_ = [("upload_ms", C.c_double), ("sa_ms", C.c_double * 2), ("occ_ms", C.c_double * 2),
                 ("find_ms", C.c_double), ("extract_ms", C.c_double), ("compact_ms", C.c_double),
                 ("learned_ms", C.c_double), ("find_launches", C.c_uint64),
-                ("find_kernel_ms", C.c_double)]
+                ("find_kernel_ms", C.c_double), ("find_positions", C.c_uint64),
+                ("find_lines", C.c_uint64)]
 
     def as_dict(self):
         return {"upload_ms": self.upload_ms, "sa_ms": list(self.sa_ms),
                 "occ_ms": list(self.occ_ms), "find_ms": self.find_ms,
                 "extract_ms": self.extract_ms, "compact_ms": self.compact_ms,
                 "learned_ms": self.learned_ms, "find_launches": self.find_launches,
-                "find_kernel_ms": self.find_kernel_ms}
+                "find_kernel_ms": self.find_kernel_ms, "find_positions": self.find_positions,
+                "find_lines": self.find_lines}
 
 
 _LIBS = {}

@@ -56,13 +56,14 @@ __device__ __forceinline__ unsigned chain_sym(const uint64_t *fwt, uint64_t s, u
 
 __global__ void __launch_bounds__(256) k_find(Geom G, DirView fw, DirView bw, uint64_t r0,
                                               uint64_t nr, uint32_t t, uint4 Clo, uint32_t C5,
-                                              const Seed *seeds, uint32_t kseed, ChainBuf out)
+                                              const Seed *seeds, uint32_t kseed, ChainBuf out,
+                                              unsigned long long *stats)
 {
     uint64_t tid = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t rl = tid >> 2;
     int ch = (int) (tid & 3);
-    if (rl >= nr)
-        return;
+    uint32_t steps = 0, lines = 0;
+    if (rl < nr) {
     uint64_t r = r0 + rl;
     uint64_t s = pq_read_start(G, r);
     uint32_t L = pq_read_len(G, r);
@@ -91,6 +92,8 @@ __global__ void __launch_bounds__(256) k_find(Geom G, DirView fw, DirView bw, ui
     while (o < L && size >= minsize) {
         Ranks a, b;
         pq_ranks2(occ, l, (uint64_t) l + size, a, b);
+        steps++;
+        lines += ((uint64_t) l >> 7) == (((uint64_t) l + size) >> 7) ? 1u : 2u;
         if (o >= t) {
             uint32_t cnt = b.r[0] - a.r[0];
             if (cnt && nout < out.cap) {
@@ -111,6 +114,18 @@ __global__ void __launch_bounds__(256) k_find(Geom G, DirView fw, DirView bw, ui
         o++;
     }
     out.n[rl * 4 + ch] = nout;
+    }
+    if (stats) {
+        uint64_t s2 = steps, ln = lines;
+        for (int d = 32; d > 0; d >>= 1) {
+            s2 += __shfl_xor(s2, d, 64);
+            ln += __shfl_xor(ln, d, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&stats[0], (unsigned long long) (2 * s2));
+            atomicAdd(&stats[1], (unsigned long long) ln);
+        }
+    }
 }
 
 // ---- extraction ----------------------------------------------------------------------------
@@ -442,6 +457,7 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     ctx->timing.learned_ms = 0;
     ctx->timing.find_kernel_ms = 0;
     ctx->timing.find_launches = 0;
+    ctx->timing.find_positions = ctx->timing.find_lines = 0;
     pq_free(ctx->d_ovB);
     pq_free(ctx->d_ovF);
     pq_free(ctx->d_numB);
@@ -529,6 +545,7 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     eo.n_overflow = misc + 2;
     eo.flags = misc + 3;
     eo.overflow_reads = ovl;
+    unsigned long long *fstats = (unsigned long long *) (misc + 8);   // misc[8..11]
     (void) hipMemsetAsync(misc, 0, 64, ctx->stream);
     // C[2..4] travel in Clo.y..w, C[5] on its own (C[0] = C[1] = 0)
     uint4 Clo = make_uint4(0, (uint32_t) ctx->C[2], (uint32_t) ctx->C[3], (uint32_t) ctx->C[4]);
@@ -548,7 +565,7 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         }
         (void) hipEventRecord(e0, ctx->stream);
         hipLaunchKernelGGL(k_find, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0, ctx->stream, G, fw,
-                           bw, r0, nr, t, Clo, C5, seeds, kseed, cb);
+                           bw, r0, nr, t, Clo, C5, seeds, kseed, cb, fstats);
         (void) hipEventRecord(e1, ctx->stream);
         hipLaunchKernelGGL(k_extract_lds, dim3(pq_blocks(nr, EXT_WAVES)), dim3(64 * EXT_WAVES),
                            lds_bytes, ctx->stream, G, fw, bw, cb, r0, nr, kw, eo);
@@ -576,8 +593,11 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         ctx->timing.extract_ms += ev_ms(e1, e2);
     }
     uint32_t flags = 0;
-    unsigned long long used = 0;
+    unsigned long long used = 0, fst[2] = {0, 0};
     if (rc == PQ_OK) {
+        (void) hipMemcpy(fst, fstats, 16, hipMemcpyDeviceToHost);
+        ctx->timing.find_positions = fst[0];
+        ctx->timing.find_lines = fst[1];
         (void) hipMemcpy(&flags, eo.flags, 4, hipMemcpyDeviceToHost);
         (void) hipMemcpy(&used, eo.edge_count, 8, hipMemcpyDeviceToHost);
         if (flags & 2u)

@@ -70,7 +70,7 @@ def test_graph_with_many_candidates(ctx):
     # a short genome sampled deeply: hundreds of candidates per read side, so the
     # extraction spills past its LDS capacity into the global-memory path
     rng = np.random.default_rng(9)
-    codes, off, _ = _random_reads(rng, 400, 1500, 40, 40, err=0.0)
+    codes, off, _ = _random_reads(rng, 400, 4000, 40, 40, err=0.0)
     ctx.upload(codes, off, 21)
     ctx.build_index()
     ctx.find_overlaps()
