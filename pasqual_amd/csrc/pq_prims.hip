@@ -1,6 +1,6 @@
 // pq_prims.hip -- device-wide primitives written for wave64 / gfx950:
 // exclusive scan (reduce-then-scan, one wave per 1024-element strip) and one
-// stable LSD radix pass over (uint64 key, uint32 value) pairs.
+// stable LSD radix pass over (uint32 or uint64 key, uint32 value) pairs.
 //
 // Radix pass: a block owns a tile of 4096 pairs; each of its 4 waves owns 1024
 // contiguous pairs and walks them in 16 coalesced rounds of 64.  Within a round
@@ -203,7 +203,8 @@ int pq_scan_u4(pq_ctx_t *ctx, const void *in, void *out, uint64_t count, void *t
 #define RADIX_ROUNDS 16
 #define RADIX_TILE (RADIX_THREADS * RADIX_ROUNDS)
 
-__global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(const uint64_t *keys, uint64_t count,
+template <typename KT>
+__global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(const KT *keys, uint64_t count,
                                                              unsigned shift, uint32_t *hist,
                                                              uint32_t nblocks)
 {
@@ -220,13 +221,14 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(const uint64_t *ke
     hist[(uint64_t) threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+template <typename KT>
 __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
-    const uint64_t *kin, const uint32_t *vin, uint64_t *kout, uint32_t *vout, uint64_t count,
+    const KT *kin, const uint32_t *vin, KT *kout, uint32_t *vout, uint64_t count,
     unsigned shift, const uint32_t *offs, uint32_t nblocks)
 {
     __shared__ uint32_t wcnt[4][256];
     __shared__ uint32_t tstart[256];
-    __shared__ uint64_t sk[RADIX_TILE];
+    __shared__ KT sk[RADIX_TILE];
     __shared__ uint32_t sv[RADIX_TILE];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < 4 * 256; i += RADIX_THREADS)
@@ -236,7 +238,7 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
     const uint64_t tile = (uint64_t) blockIdx.x * RADIX_TILE;
     const uint64_t wbase = tile + (uint64_t) wid * (RADIX_TILE / 4);
     const uint64_t lt = (1ull << lane) - 1ull;
-    uint64_t key[RADIX_ROUNDS];
+    KT key[RADIX_ROUNDS];
     uint32_t val[RADIX_ROUNDS];
     uint32_t rnk[RADIX_ROUNDS];
 #pragma unroll
@@ -300,7 +302,7 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
     __syncthreads();
     uint64_t valid = count - tile < RADIX_TILE ? count - tile : RADIX_TILE;
     for (uint32_t p = threadIdx.x; p < valid; p += RADIX_THREADS) {
-        uint64_t k = sk[p];
+        KT k = sk[p];
         unsigned d = (unsigned) (k >> shift) & 255u;
         uint64_t o = (uint64_t) offs[(uint64_t) d * nblocks + blockIdx.x] + (p - tstart[d]);
         kout[o] = k;
@@ -308,9 +310,10 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
     }
 }
 
-// One stable pass by the 8-bit digit at `shift`.  Scratch slots 1 (histogram).
-int pq_radix_pass(pq_ctx_t *ctx, const uint64_t *kin, const uint32_t *vin, uint64_t *kout,
-                  uint32_t *vout, uint64_t count, unsigned shift)
+// One stable pass by the 8-bit digit at `shift`.  Scratch slot 1 (histogram).
+template <typename KT>
+static int radix_pass_t(pq_ctx_t *ctx, const KT *kin, const uint32_t *vin, KT *kout,
+                        uint32_t *vout, uint64_t count, unsigned shift)
 {
     if (count == 0)
         return PQ_OK;
@@ -321,13 +324,25 @@ int pq_radix_pass(pq_ctx_t *ctx, const uint64_t *kin, const uint32_t *vin, uint6
     if (!hist)
         return ctx->fail(PQ_ERR_NOMEM, "radix histogram");
     uint32_t *offs = hist + (size_t) nb * 256;
-    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RADIX_THREADS), 0, ctx->stream, kin, count,
-                       shift, hist, nb);
+    hipLaunchKernelGGL(k_radix_hist<KT>, dim3(nb), dim3(RADIX_THREADS), 0, ctx->stream, kin,
+                       count, shift, hist, nb);
     int rc = pq_scan_u32(ctx, hist, offs, (uint64_t) nb * 256, nullptr);
     if (rc)
         return rc;
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RADIX_THREADS), 0, ctx->stream, kin, vin,
-                       kout, vout, count, shift, offs, nb);
+    hipLaunchKernelGGL(k_radix_scatter<KT>, dim3(nb), dim3(RADIX_THREADS), 0, ctx->stream, kin,
+                       vin, kout, vout, count, shift, offs, nb);
     PQ_CHECK(hipGetLastError());
     return PQ_OK;
+}
+
+int pq_radix_pass(pq_ctx_t *ctx, const uint64_t *kin, const uint32_t *vin, uint64_t *kout,
+                  uint32_t *vout, uint64_t count, unsigned shift)
+{
+    return radix_pass_t<uint64_t>(ctx, kin, vin, kout, vout, count, shift);
+}
+
+int pq_radix_pass32(pq_ctx_t *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                    uint32_t *vout, uint64_t count, unsigned shift)
+{
+    return radix_pass_t<uint32_t>(ctx, kin, vin, kout, vout, count, shift);
 }

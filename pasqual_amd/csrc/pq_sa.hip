@@ -2,24 +2,23 @@
 //
 // Suffix order (the reference's, sufsort.c:154-383 at one thread): suffixes
 // run to the end of their read, end of read sorts first, ties by text
-// position.  The sort is an MSD refinement over 29-code chunks:
+// position.
 //
-//   key_r(g) = the 29 codes starting at g + 29r (2 bits each, zero padding past
-//              the read end) << 5  |  min(codes remaining from g + 29r, 30)
+//  1. Bucket all N suffixes by a 2K+4-bit key: their first K codes (zero
+//     padding past the read end) and min(length, K), K ~ log4(N/128) <= 12.
+//     The key is order-consistent, and a bucket whose length field is below K
+//     holds identical complete suffixes, already in position order because the
+//     LSD radix passes (from the identity permutation) are stable.
+//  2. Find the bucket runs and split them into size classes with scans (no
+//     global atomics): <= 16 and <= 64 elements are sorted by a group of lanes
+//     with a bitonic network over registers, <= 2048 by one block in LDS.  The
+//     comparator caches the first two 29-code chunk keys after the common
+//     prefix and reads further chunks from the packed text only for suffixes
+//     still tied (overlapping reads of one genome region).
+//  3. Rare larger buckets are refined level by level: one chunk key per
+//     element, global radix sort, and the tied sub-runs go back to step 2 at
+//     the next chunk offset.
 //
-// Two suffixes with equal key_r and a length field below 30 are identical
-// strings (ordered by position); equal keys with field 30 share the chunk and
-// go on to chunk r+1.  Round 0 first buckets all N (key_0, g) pairs by the top
-// 16 or 24 key bits with stable LSD radix passes, then every bucket and, in
-// later rounds, every unresolved run is sorted on its own by (key_r, g):
-// runs of <= 16 / <= 64 elements inside a wave (bitonic network over lanes),
-// runs of <= 4096 in LDS by one block, larger runs by global radix passes.
-// Each sort emits the unresolved sub-runs of the next round, so a round only
-// touches suffixes still tied.
-//
-// The BWT is never stored as bytes: one pass over the SA writes the 3-bit
-// symbol planes of every 128-row OccLine with wave ballots, a scan over lines
-// adds the counts, and a last pass writes reads_index from the '$' rows.
 #include "pq_ctx.h"
 
 #include <algorithm>
@@ -34,48 +33,91 @@ int pq_scan_u4(pq_ctx_t *ctx, const void *in, void *out, uint64_t count, void *t
 
 // ---- keys ------------------------------------------------------------------------
 
-__device__ __forceinline__ uint64_t sa_key(const Geom &G, const uint64_t *t2, uint64_t g,
-                                           unsigned r)
+// Codes remaining from g to the end of its read.
+__device__ __forceinline__ uint32_t suf_rem(const Geom &G, uint64_t g)
 {
-    uint64_t s = g + (uint64_t) CHUNK * r;
-    uint64_t f;
-    if (G.L) {
-        f = G.L - (s % G.L);
-    } else {
-        uint64_t w = s >> 6;
-        unsigned b = (unsigned) (s & 63);
-        uint64_t e = G.ends[w] >> b;
-        if (b)
-            e |= G.ends[w + 1] << (64 - b);
-        e &= (1ull << 30) - 1ull;
-        f = e ? (uint64_t) (__ffsll((long long) e)) : 30;  // ffs = index of end + 1
-    }
+    return (uint32_t) pq_rem(G, g);
+}
+
+// The 29 codes of suffix g from offset `off`, bounded by the read end:
+// (codes << 5) | min(codes remaining from g+off, 30).  Equal keys with a field
+// below 30 mean the two suffixes end inside this chunk as the same string.
+__device__ __forceinline__ uint64_t chunk_key(const uint64_t *t2, uint64_t g, uint32_t rem,
+                                              uint32_t off)
+{
+    if (rem <= off)
+        return 0;
+    uint32_t f = rem - off;
     if (f > 30)
         f = 30;
-    uint64_t chars = pq_window(t2, s) >> 6;
+    uint64_t chars = pq_window(t2, g + off) >> 6;
     if (f < CHUNK)
         chars &= ~((1ull << (2 * (CHUNK - f))) - 1ull);
     return (chars << 5) | f;
 }
 
-__global__ void k_key0(Geom G, const uint64_t *t2, uint64_t N, uint64_t *K, uint32_t *V)
+// Bucket key of the first K codes (K <= 12): (codes << 4) | min(rem, K).  Keys
+// with a field below K are complete suffixes; equal ones are equal strings.
+__global__ void k_bucket_keys(Geom G, const uint64_t *t2, uint64_t N, unsigned K, uint32_t *BK,
+                              uint32_t *V)
 {
-    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) {
-        K[i] = sa_key(G, t2, i, 0);
-        V[i] = (uint32_t) i;
+    uint64_t g = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N)
+        return;
+    uint32_t rem = suf_rem(G, g);
+    uint32_t f = rem < K ? rem : K;
+    uint32_t chars = (uint32_t) (pq_window(t2, g) >> (64 - 2 * K));
+    if (f < K)
+        chars &= ~((1u << (2 * (K - f))) - 1u);
+    BK[g] = (chars << 4) | f;
+    V[g] = (uint32_t) g;
+}
+
+// One suffix of a run being sorted: position, remaining codes, and the first
+// two chunk keys from the run's common offset.
+struct SufKey {
+    uint32_t g, rem;
+    uint64_t k0, k1;
+};
+
+#define SUF_NONE 0xFFFFFFFFu
+
+__device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, uint32_t g,
+                                         uint32_t base, SufKey &s)
+{
+    s.g = g;
+    s.rem = suf_rem(G, g);
+    s.k0 = chunk_key(t2, g, s.rem, base);
+    s.k1 = (s.k0 & 31ull) == 30ull ? chunk_key(t2, g, s.rem, base + CHUNK) : 0ull;
+}
+
+// Canonical order of two suffixes sharing their first `base` codes: string
+// order (end first) and then position.  Chunks past the cached two are read
+// from the text.
+__device__ __forceinline__ bool suf_less(const uint64_t *t2, uint32_t ga, uint32_t ra,
+                                         uint64_t a0, uint64_t a1, uint32_t gb, uint32_t rb,
+                                         uint64_t b0, uint64_t b1, uint32_t base)
+{
+    if (ga == SUF_NONE || gb == SUF_NONE)
+        return ga < gb;
+    if (a0 != b0)
+        return a0 < b0;
+    if ((a0 & 31ull) < 30ull)
+        return ga < gb;
+    if (a1 != b1)
+        return a1 < b1;
+    if ((a1 & 31ull) < 30ull)
+        return ga < gb;
+    for (uint32_t off = base + 2 * CHUNK;; off += CHUNK) {
+        uint64_t x = chunk_key(t2, ga, ra, off), y = chunk_key(t2, gb, rb, off);
+        if (x != y)
+            return x < y;
+        if ((x & 31ull) < 30ull)
+            return ga < gb;
     }
 }
 
-__global__ void k_keys_range(Geom G, const uint64_t *t2, const uint32_t *sa, uint64_t n,
-                             unsigned r, uint64_t *K)
-{
-    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        K[i] = sa_key(G, t2, sa[i], r);
-}
-
-// ---- segment lists -------------------------------------------------------------------
+// ---- run lists -----------------------------------------------------------------------
 
 struct SegLists {
     uint2 *items[4];
@@ -84,12 +126,17 @@ struct SegLists {
     uint32_t *overflow;
 };
 
+#define RUN_SMALL 16
+#define RUN_WAVE 64
+#define RUN_LDS 2048
+
 __device__ __forceinline__ int seg_class(uint32_t size)
 {
-    return size <= 16 ? 0 : size <= 64 ? 1 : size <= 4096 ? 2 : 3;
+    return size <= RUN_SMALL ? 0 : size <= RUN_WAVE ? 1 : size <= RUN_LDS ? 2 : 3;
 }
 
-// Wave-aggregated append; every lane of the wave must call it.
+// Wave-aggregated append (used only for the runs split off oversized runs);
+// every lane of the wave must call it.
 __device__ __forceinline__ void push_seg(const SegLists &S, bool valid, uint32_t start,
                                          uint32_t size)
 {
@@ -119,77 +166,71 @@ __device__ __forceinline__ void push_seg(const SegLists &S, bool valid, uint32_t
 
 // ---- sorts of one run ---------------------------------------------------------------
 
-__device__ __forceinline__ bool kv_less(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb)
-{
-    return ka < kb || (ka == kb && va < vb);
-}
-
 // One group of G lanes per run (G = 16 or 64); bitonic network over lanes.
 template <int G>
 __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2, uint32_t *sa,
-                                                     const uint64_t *K, unsigned r, int round0,
-                                                     const uint2 *segs, uint32_t nseg,
-                                                     SegLists out)
+                                                     uint32_t base, const uint2 *segs,
+                                                     uint32_t nseg)
 {
     uint64_t tid = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t sid = tid / G;
     int li = (int) (threadIdx.x % G);
-    int lane = threadIdx.x & 63;
     bool have = sid < nseg;
     uint2 sg = have ? segs[sid] : make_uint2(0, 0);
     bool e = have && (uint32_t) li < sg.y;
-    uint32_t g = e ? sa[sg.x + li] : 0xFFFFFFFFu;
-    uint64_t key = e ? (round0 ? K[sg.x + li] : sa_key(Gm, t2, g, r)) : ~0ull;
+    SufKey s;
+    if (e) {
+        suf_load(Gm, t2, sa[sg.x + li], base, s);
+    } else {
+        s.g = SUF_NONE;
+        s.rem = 0;
+        s.k0 = s.k1 = ~0ull;
+    }
 #pragma unroll
     for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            uint64_t pk = __shfl_xor(key, j, 64);
-            uint32_t pg = __shfl_xor(g, j, 64);
+            uint32_t pg = __shfl_xor(s.g, j, 64), pr = __shfl_xor(s.rem, j, 64);
+            uint64_t p0 = __shfl_xor(s.k0, j, 64), p1 = __shfl_xor(s.k1, j, 64);
             bool up = (li & k) == 0;
             bool lower = (li & j) == 0;
-            bool pless = kv_less(pk, pg, key, g);
-            bool swap = (lower == up) ? pless : kv_less(key, g, pk, pg);
+            bool pless = suf_less(t2, pg, pr, p0, p1, s.g, s.rem, s.k0, s.k1, base);
+            bool swap = (lower == up) ? pless : (!pless && pg != s.g);
             if (swap) {
-                key = pk;
-                g = pg;
+                s.g = pg;
+                s.rem = pr;
+                s.k0 = p0;
+                s.k1 = p1;
             }
         }
     }
     if (e)
-        sa[sg.x + li] = g;
-    uint64_t prev = __shfl_up(key, 1, 64);
-    bool head = e && (li == 0 || prev != key);
-    uint64_t hm = __ballot(head);
-    int gbase = lane - li;
-    uint64_t gm = hm >> gbase;
-    if (G < 64)
-        gm &= (1ull << G) - 1ull;
-    uint64_t above = (li + 1 < 64) ? (gm >> (li + 1)) : 0ull;
-    uint32_t next = above ? (uint32_t) (li + 1 + __ffsll((long long) above) - 1) : sg.y;
-    bool unres = head && (key & 31ull) == 30ull && next - (uint32_t) li > 1;
-    push_seg(out, unres, sg.x + (uint32_t) li, next - (uint32_t) li);
+        sa[sg.x + li] = s.g;
 }
 
-// One block per run of 65..4096 elements: bitonic sort in LDS.
-__global__ void __launch_bounds__(256) k_sort_block(Geom Gm, const uint64_t *t2, uint32_t *sa,
-                                                    const uint64_t *K, unsigned r, int round0,
-                                                    const uint2 *segs, SegLists out)
+// One block per run of 65..2048 elements: bitonic sort in LDS.
+__global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, uint32_t *sa,
+                                                  uint32_t base, const uint2 *segs)
 {
-    __shared__ uint64_t sk[4096];
-    __shared__ uint32_t sv[4096];
+    __shared__ uint64_t s0[RUN_LDS], s1[RUN_LDS];
+    __shared__ uint32_t sg_[RUN_LDS];
+    __shared__ uint16_t sr[RUN_LDS];
     uint2 sg = segs[blockIdx.x];
     uint32_t n = sg.y, P = 1;
     while (P < n)
         P <<= 1;
     for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
         if (i < n) {
-            uint32_t g = sa[sg.x + i];
-            sv[i] = g;
-            sk[i] = round0 ? K[sg.x + i] : sa_key(Gm, t2, g, r);
+            SufKey s;
+            suf_load(Gm, t2, sa[sg.x + i], base, s);
+            sg_[i] = s.g;
+            sr[i] = (uint16_t) s.rem;
+            s0[i] = s.k0;
+            s1[i] = s.k1;
         } else {
-            sv[i] = 0xFFFFFFFFu;
-            sk[i] = ~0ull;
+            sg_[i] = SUF_NONE;
+            sr[i] = 0;
+            s0[i] = s1[i] = ~0ull;
         }
     }
     __syncthreads();
@@ -198,41 +239,37 @@ __global__ void __launch_bounds__(256) k_sort_block(Geom Gm, const uint64_t *t2,
             for (uint32_t p = threadIdx.x; p < P / 2; p += blockDim.x) {
                 uint32_t i = 2 * j * (p / j) + (p % j), ip = i + j;
                 bool up = (i & k) == 0;
-                uint64_t ka = sk[i], kb = sk[ip];
-                uint32_t va = sv[i], vb = sv[ip];
-                bool gt = kv_less(kb, vb, ka, va);
+                bool gt = suf_less(t2, sg_[ip], sr[ip], s0[ip], s1[ip], sg_[i], sr[i], s0[i],
+                                   s1[i], base);
                 if (gt == up) {
-                    sk[i] = kb;
-                    sk[ip] = ka;
-                    sv[i] = vb;
-                    sv[ip] = va;
+                    uint64_t x0 = s0[i], x1 = s1[i];
+                    uint32_t xg = sg_[i];
+                    uint16_t xr = sr[i];
+                    s0[i] = s0[ip];
+                    s1[i] = s1[ip];
+                    sg_[i] = sg_[ip];
+                    sr[i] = sr[ip];
+                    s0[ip] = x0;
+                    s1[ip] = x1;
+                    sg_[ip] = xg;
+                    sr[ip] = xr;
                 }
             }
             __syncthreads();
         }
     }
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-        sa[sg.x + i] = sv[i];
-    for (uint32_t base = 0; base < n; base += blockDim.x) {
-        uint32_t i = base + threadIdx.x;
-        bool valid = i < n;
-        bool head = valid && (i == 0 || sk[i] != sk[i - 1]);
-        uint32_t next = i + 1;
-        if (head)
-            while (next < n && sk[next] == sk[i])
-                next++;
-        bool unres = head && (sk[i] & 31ull) == 30ull && next - i > 1;
-        push_seg(out, unres, sg.x + i, next - i);
-    }
+        sa[sg.x + i] = sg_[i];
 }
 
-// ---- run detection for bucket boundaries and large runs ------------------------------
+// ---- runs of equal keys ---------------------------------------------------------------
 
-__global__ void k_head_flags(const uint64_t *K, uint64_t n, unsigned shift, uint32_t *flags)
+template <typename KT>
+__global__ void k_head_flags(const KT *K, uint64_t n, uint32_t *flags)
 {
     uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
-        flags[i] = (i == 0 || (K[i] >> shift) != (K[i - 1] >> shift)) ? 1u : 0u;
+        flags[i] = (i == 0 || K[i] != K[i - 1]) ? 1u : 0u;
 }
 
 __global__ void k_compact_heads(const uint32_t *flags, const uint32_t *pos, uint64_t n,
@@ -241,78 +278,108 @@ __global__ void k_compact_heads(const uint32_t *flags, const uint32_t *pos, uint
     uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && flags[i])
         starts[pos[i]] = (uint32_t) i;
+    if (i == 0)
+        starts[pos[n - 1] + flags[n - 1]] = (uint32_t) n;   // sentinel
 }
 
-// Push runs [starts[k], starts[k+1]) of a range at offset `base`.  all_unresolved:
-// bucket runs (every run of size > 1 must be sorted); otherwise a run is
-// unresolved when its key has length field 30.
-__global__ void k_push_runs(const uint64_t *K, const uint32_t *starts, uint32_t nruns,
-                            uint64_t n, uint32_t base, int all_unresolved, SegLists out)
+// Size class of every bucket run, or 4 when it needs no sort (one element, or
+// complete identical suffixes whose order is already the position order).
+__global__ void k_bucket_class(const uint32_t *BK, const uint32_t *starts, uint32_t nruns,
+                               unsigned K, uint8_t *cls)
+{
+    uint64_t k = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nruns)
+        return;
+    uint32_t a = starts[k], size = starts[k + 1] - a;
+    bool done = size <= 1 || (BK[a] & 15u) < K;
+    cls[k] = done ? 4 : (uint8_t) seg_class(size);
+}
+
+__global__ void k_class_flags(const uint8_t *cls, uint32_t nruns, uint8_t c, uint32_t *flags)
+{
+    uint64_t k = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nruns)
+        flags[k] = cls[k] == c ? 1u : 0u;
+}
+
+__global__ void k_class_compact(const uint32_t *flags, const uint32_t *pos, const uint32_t *starts,
+                                uint32_t nruns, uint2 *out)
+{
+    uint64_t k = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nruns && flags[k])
+        out[pos[k]] = make_uint2(starts[k], starts[k + 1] - starts[k]);
+}
+
+// Runs of equal chunk keys inside an oversized run (offset `at` in the SA):
+// those still tied past the chunk go to the next level.
+__global__ void k_push_subruns(const uint64_t *K, const uint32_t *starts, uint32_t nruns,
+                               uint32_t at, SegLists out)
 {
     uint64_t k = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = k < nruns;
     uint32_t a = valid ? starts[k] : 0;
-    uint32_t b = valid ? (k + 1 < nruns ? starts[k + 1] : (uint32_t) n) : 0;
-    bool unres = valid && b - a > 1 && (all_unresolved || (K[a] & 31ull) == 30ull);
-    push_seg(out, unres, base + a, b - a);
+    uint32_t b = valid ? starts[k + 1] : 0;
+    bool unres = valid && b - a > 1 && (K[a] & 31ull) == 30ull;
+    push_seg(out, unres, at + a, b - a);
+}
+
+__global__ void k_chunk_keys(Geom G, const uint64_t *t2, const uint32_t *sa, uint64_t n,
+                             uint32_t base, uint64_t *K)
+{
+    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        uint32_t g = sa[i];
+        K[i] = chunk_key(t2, g, suf_rem(G, g), base);
+    }
 }
 
 // ---- driver ---------------------------------------------------------------------------
 
-struct SegBufs {
-    uint2 *items[2][4] = {{nullptr}};
-    uint32_t *counts = nullptr;   // [2][4] + overflow
-    uint32_t cap[4] = {0};
-};
+int pq_radix_pass32(pq_ctx_t *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                    uint32_t *vout, uint64_t count, unsigned shift);
 
-static SegLists lists_of(SegBufs &B, int which)
+// Run starts of equal keys in K[0..n): returns the run count; starts gets
+// nruns+1 entries (sentinel n).  flags/pos/starts: n+1 entries each.
+template <typename KT>
+static int find_runs(pq_ctx_t *ctx, const KT *K, uint64_t n, uint32_t *flags, uint32_t *pos,
+                     uint32_t *starts, uint32_t *nruns)
 {
-    SegLists S;
-    for (int c = 0; c < 4; c++) {
-        S.items[c] = B.items[which][c];
-        S.cap[c] = B.cap[c];
-    }
-    S.count = B.counts + 4 * which;
-    S.overflow = B.counts + 8;
-    return S;
-}
-
-// Find runs of equal (key >> shift) in K[0..n) and push them (offset base).
-static int push_runs_of(pq_ctx_t *ctx, const uint64_t *K, uint64_t n, unsigned shift,
-                        uint32_t base, int all_unres, SegLists out)
-{
-    uint32_t *flags = (uint32_t *) ctx->tmp(3, n * sizeof(uint32_t) * 3 + 64);
-    if (!flags)
-        return ctx->fail(PQ_ERR_NOMEM, "run scratch");
-    uint32_t *pos = flags + n, *starts = pos + n;
     unsigned nb = pq_blocks(n, 256);
-    hipLaunchKernelGGL(k_head_flags, dim3(nb), dim3(256), 0, ctx->stream, K, n, shift, flags);
-    uint32_t nruns = 0;
-    int rc = pq_scan_u32(ctx, flags, pos, n, &nruns);
+    hipLaunchKernelGGL(k_head_flags<KT>, dim3(nb), dim3(256), 0, ctx->stream, K, n, flags);
+    int rc = pq_scan_u32(ctx, flags, pos, n, nruns);
     if (rc)
         return rc;
     hipLaunchKernelGGL(k_compact_heads, dim3(nb), dim3(256), 0, ctx->stream, flags, pos, n,
                        starts);
-    hipLaunchKernelGGL(k_push_runs, dim3(pq_blocks(nruns, 256)), dim3(256), 0, ctx->stream, K,
-                       starts, nruns, n, base, all_unres, out);
     PQ_CHECK(hipGetLastError());
     return PQ_OK;
 }
 
-// Sort pairs (K[0..n), V[0..n)) by full key with 8 stable passes (result in place).
-static int radix_full(pq_ctx_t *ctx, uint64_t *K, uint32_t *V, uint64_t n, uint64_t *K2,
-                      uint32_t *V2)
+static int sort_lists(pq_ctx_t *ctx, const Geom &G, DirIndex &D, uint32_t base,
+                      uint2 *const items[3], const uint32_t cnt[3])
 {
-    for (unsigned p = 0; p < 8; p++) {
-        const uint64_t *ki = (p & 1) ? K2 : K;
-        const uint32_t *vi = (p & 1) ? V2 : V;
-        uint64_t *ko = (p & 1) ? K : K2;
-        uint32_t *vo = (p & 1) ? V : V2;
-        int rc = pq_radix_pass(ctx, ki, vi, ko, vo, n, 8 * p);
-        if (rc)
-            return rc;
-    }
+    if (cnt[0])
+        hipLaunchKernelGGL(k_sort_groups<RUN_SMALL>,
+                           dim3(pq_blocks((uint64_t) cnt[0] * RUN_SMALL, 256)), dim3(256), 0,
+                           ctx->stream, G, D.text2, D.sa, base, items[0], cnt[0]);
+    if (cnt[1])
+        hipLaunchKernelGGL(k_sort_groups<RUN_WAVE>,
+                           dim3(pq_blocks((uint64_t) cnt[1] * RUN_WAVE, 256)), dim3(256), 0,
+                           ctx->stream, G, D.text2, D.sa, base, items[1], cnt[1]);
+    if (cnt[2])
+        hipLaunchKernelGGL(k_sort_lds, dim3(cnt[2]), dim3(256), 0, ctx->stream, G, D.text2, D.sa,
+                           base, items[2]);
+    PQ_CHECK(hipGetLastError());
     return PQ_OK;
+}
+
+// Bucket width for N suffixes: about 128 suffixes per bucket of K codes.
+static unsigned bucket_codes(uint64_t N)
+{
+    unsigned K = 4;
+    while (K < 12 && (N >> (2 * K)) > 128)
+        K++;
+    return K;
 }
 
 int pq_build_sa_dir(pq_ctx_t *ctx, int d)
@@ -322,111 +389,148 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d)
     if (N == 0)
         return PQ_OK;
     Geom G = ctx->geom();
-    uint64_t *K = nullptr, *K2 = nullptr;
-    uint32_t *V2 = nullptr;
-    SegBufs B;
+    const unsigned K = bucket_codes(N);
+    uint32_t *BK = nullptr, *BK2 = nullptr, *V2 = nullptr, *flags = nullptr, *pos = nullptr,
+             *starts = nullptr;
+    uint8_t *cls = nullptr;
+    uint2 *items[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t *counts = nullptr;
     int rc = PQ_OK;
-    const uint64_t capdiv[4] = {2, 17, 65, 4097};
-    if ((rc = pq_alloc(ctx, (void **) &K, N * 8)) || (rc = pq_alloc(ctx, (void **) &K2, N * 8)) ||
+    if ((rc = pq_alloc(ctx, (void **) &BK, N * 4)) || (rc = pq_alloc(ctx, (void **) &BK2, N * 4)) ||
         (rc = pq_alloc(ctx, (void **) &V2, N * 4)) ||
-        (rc = pq_alloc(ctx, (void **) &B.counts, 16 * sizeof(uint32_t))))
+        (rc = pq_alloc(ctx, (void **) &flags, (N + 1) * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &pos, (N + 1) * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &starts, (N + 2) * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &cls, N + 1)) ||
+        (rc = pq_alloc(ctx, (void **) &counts, 64)))
         goto done;
-    for (int c = 0; c < 4; c++) {
-        B.cap[c] = (uint32_t) (N / capdiv[c] + 2);
-        for (int w = 0; w < 2; w++)
-            if ((rc = pq_alloc(ctx, (void **) &B.items[w][c], (size_t) B.cap[c] * sizeof(uint2))))
-                goto done;
-    }
     {
-        // round 0: keys, bucketing by the top bits
-        hipLaunchKernelGGL(k_key0, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, G,
-                           D.text2, N, K, D.sa);
-        unsigned topbits = N > (1ull << 26) ? 24 : 16;
-        unsigned passes = topbits / 8;
-        uint64_t *kc = K, *ka = K2;
-        uint32_t *vc = D.sa, *va = V2;
+        // 1. bucket by the first K codes (and the length below K)
+        hipLaunchKernelGGL(k_bucket_keys, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, G,
+                           D.text2, N, K, BK, D.sa);
+        unsigned bits = 2 * K + 4, passes = (bits + 7) / 8;
+        uint32_t *kc = BK, *ka = BK2, *vc = D.sa, *va = V2;
         for (unsigned p = 0; p < passes; p++) {
-            unsigned shift = 63 - topbits + 8 * p;
-            if ((rc = pq_radix_pass(ctx, kc, vc, ka, va, N, shift)))
+            if ((rc = pq_radix_pass32(ctx, kc, vc, ka, va, N, 8 * p)))
                 goto done;
             std::swap(kc, ka);
             std::swap(vc, va);
         }
-        // the sorted pairs are in (kc, vc); make them (K, D.sa)
-        K = kc;
-        K2 = ka;
+        BK = kc;
+        BK2 = ka;
         D.sa = vc;
         V2 = va;
-        PQ_CHECK(hipMemsetAsync(B.counts, 0, 16 * sizeof(uint32_t), ctx->stream));
-        if ((rc = push_runs_of(ctx, K, N, 63 - topbits, 0, 1, lists_of(B, 0))))
+        // 2. bucket runs, classified by size with scans
+        uint32_t nruns = 0;
+        if ((rc = find_runs<uint32_t>(ctx, BK, N, flags, pos, starts, &nruns)))
             goto done;
-
-        int cur = 0;
-        uint32_t maxround = (ctx->Lmax + CHUNK - 1) / CHUNK + 1;
-        for (unsigned r = 0; r <= maxround; r++) {
-            uint32_t cnt[9];
-            PQ_CHECK(hipMemcpyAsync(cnt, B.counts, 9 * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                    ctx->stream));
+        hipLaunchKernelGGL(k_bucket_class, dim3(pq_blocks(nruns, 256)), dim3(256), 0, ctx->stream,
+                           BK, starts, nruns, K, cls);
+        uint32_t cnt[4] = {0, 0, 0, 0};
+        std::vector<uint2 *> lists(4);
+        uint2 *lbuf = nullptr;
+        if ((rc = pq_alloc(ctx, (void **) &lbuf, (size_t) (nruns + 1) * 4 * sizeof(uint2))))
+            goto done;
+        for (int c = 0; c < 4; c++) {
+            lists[c] = lbuf + (size_t) c * (nruns + 1);
+            hipLaunchKernelGGL(k_class_flags, dim3(pq_blocks(nruns, 256)), dim3(256), 0,
+                               ctx->stream, cls, nruns, (uint8_t) c, flags);
+            if ((rc = pq_scan_u32(ctx, flags, pos, nruns, &cnt[c]))) {
+                pq_free(lbuf);
+                goto done;
+            }
+            hipLaunchKernelGGL(k_class_compact, dim3(pq_blocks(nruns, 256)), dim3(256), 0,
+                               ctx->stream, flags, pos, starts, nruns, lists[c]);
+        }
+        // 3. sort every bucket past its first K codes
+        uint2 *l3[3] = {lists[0], lists[1], lists[2]};
+        if ((rc = sort_lists(ctx, G, D, K, l3, cnt))) {
+            pq_free(lbuf);
+            goto done;
+        }
+        // 4. oversized buckets: MSD over 29-code chunks, level by level
+        std::vector<uint2> big(cnt[3]);
+        if (cnt[3]) {
+            PQ_CHECK(hipMemcpyAsync(big.data(), lists[3], cnt[3] * sizeof(uint2),
+                                    hipMemcpyDeviceToHost, ctx->stream));
             PQ_CHECK(hipStreamSynchronize(ctx->stream));
-            if (cnt[8]) {
-                rc = ctx->fail(PQ_ERR_LIMIT, "suffix sort run list overflow");
-                goto done;
-            }
-            uint32_t *c = cnt + 4 * cur;
-            if (c[0] + c[1] + c[2] + c[3] == 0)
+        }
+        pq_free(lbuf);
+        uint32_t base = K;
+        SegLists S;
+        uint2 *sbuf = nullptr;
+        const uint32_t scap = (uint32_t) (N / 2 + 2);
+        while (!big.empty() && rc == PQ_OK) {
+            uint64_t maxn = 0;
+            for (const uint2 &s : big)
+                maxn = std::max<uint64_t>(maxn, s.y);
+            uint64_t *KT = (uint64_t *) ctx->tmp(5, maxn * 8 * 2 + 64);
+            uint32_t *VT = (uint32_t *) ctx->tmp(6, maxn * 4 + 64);
+            if (!KT || !VT) {
+                rc = ctx->fail(PQ_ERR_NOMEM, "oversized run scratch");
                 break;
-            if (r == maxround) {
-                rc = ctx->fail(PQ_ERR_STATE, "suffix sort did not converge");
-                goto done;
             }
-            int nxt = 1 - cur;
-            PQ_CHECK(hipMemsetAsync(B.counts + 4 * nxt, 0, 4 * sizeof(uint32_t), ctx->stream));
-            SegLists out = lists_of(B, nxt);
-            int round0 = r == 0;
-            if (c[0])
-                hipLaunchKernelGGL(k_sort_groups<16>, dim3(pq_blocks((uint64_t) c[0] * 16, 256)),
-                                   dim3(256), 0, ctx->stream, G, D.text2, D.sa, K, r, round0,
-                                   B.items[cur][0], c[0], out);
-            if (c[1])
-                hipLaunchKernelGGL(k_sort_groups<64>, dim3(pq_blocks((uint64_t) c[1] * 64, 256)),
-                                   dim3(256), 0, ctx->stream, G, D.text2, D.sa, K, r, round0,
-                                   B.items[cur][1], c[1], out);
-            if (c[2])
-                hipLaunchKernelGGL(k_sort_block, dim3(c[2]), dim3(256), 0, ctx->stream, G,
-                                   D.text2, D.sa, K, r, round0, B.items[cur][2], out);
-            PQ_CHECK(hipGetLastError());
-            if (c[3]) {
-                std::vector<uint2> big(c[3]);
-                PQ_CHECK(hipMemcpyAsync(big.data(), B.items[cur][3], c[3] * sizeof(uint2),
+            if (!sbuf && (rc = pq_alloc(ctx, (void **) &sbuf, (size_t) scap * 4 * sizeof(uint2))))
+                break;
+            for (int c = 0; c < 4; c++) {
+                S.items[c] = sbuf + (size_t) c * scap;
+                S.cap[c] = scap;
+            }
+            S.count = counts;
+            S.overflow = counts + 4;
+            PQ_CHECK(hipMemsetAsync(counts, 0, 64, ctx->stream));
+            for (const uint2 &s : big) {
+                uint64_t *keys = KT, *keys2 = KT + maxn;
+                hipLaunchKernelGGL(k_chunk_keys, dim3(pq_blocks(s.y, 256)), dim3(256), 0,
+                                   ctx->stream, G, D.text2, D.sa + s.x, (uint64_t) s.y, base, keys);
+                for (unsigned p = 0; p < 8 && rc == PQ_OK; p++) {
+                    const uint64_t *ki = (p & 1) ? keys2 : keys;
+                    const uint32_t *vi = (p & 1) ? VT : D.sa + s.x;
+                    uint64_t *ko = (p & 1) ? keys : keys2;
+                    uint32_t *vo = (p & 1) ? D.sa + s.x : VT;
+                    rc = pq_radix_pass(ctx, ki, vi, ko, vo, s.y, 8 * p);
+                }
+                uint32_t nsub = 0;
+                if (rc || (rc = find_runs<uint64_t>(ctx, keys, s.y, flags, pos, starts, &nsub)))
+                    break;
+                hipLaunchKernelGGL(k_push_subruns, dim3(pq_blocks(nsub, 256)), dim3(256), 0,
+                                   ctx->stream, keys, starts, nsub, s.x, S);
+            }
+            if (rc)
+                break;
+            uint32_t c5[5];
+            PQ_CHECK(hipMemcpyAsync(c5, counts, 5 * 4, hipMemcpyDeviceToHost, ctx->stream));
+            PQ_CHECK(hipStreamSynchronize(ctx->stream));
+            if (c5[4]) {
+                rc = ctx->fail(PQ_ERR_LIMIT, "suffix sort run list overflow");
+                break;
+            }
+            base += CHUNK;
+            uint2 *sl[3] = {S.items[0], S.items[1], S.items[2]};
+            if ((rc = sort_lists(ctx, G, D, base, sl, c5)))
+                break;
+            big.assign(c5[3], make_uint2(0, 0));
+            if (c5[3]) {
+                PQ_CHECK(hipMemcpyAsync(big.data(), S.items[3], c5[3] * sizeof(uint2),
                                         hipMemcpyDeviceToHost, ctx->stream));
                 PQ_CHECK(hipStreamSynchronize(ctx->stream));
-                for (const uint2 &s : big) {
-                    uint64_t *keys;
-                    if (round0) {
-                        keys = K + s.x;
-                    } else {
-                        keys = K;   // K is free after round 0; use its head as key scratch
-                        hipLaunchKernelGGL(k_keys_range, dim3(pq_blocks(s.y, 256)), dim3(256), 0,
-                                           ctx->stream, G, D.text2, D.sa + s.x, (uint64_t) s.y, r,
-                                           keys);
-                    }
-                    if ((rc = radix_full(ctx, keys, D.sa + s.x, s.y, K2, V2)))
-                        goto done;
-                    if ((rc = push_runs_of(ctx, keys, s.y, 0, s.x, 0, out)))
-                        goto done;
-                }
             }
-            cur = nxt;
+            if (base > ctx->Lmax + CHUNK) {
+                rc = ctx->fail(PQ_ERR_STATE, "suffix sort did not converge");
+                break;
+            }
         }
+        pq_free(sbuf);
     }
 done:
-    pq_free(K);
-    pq_free(K2);
+    pq_free(BK);
+    pq_free(BK2);
     pq_free(V2);
-    pq_free(B.counts);
-    for (int w = 0; w < 2; w++)
-        for (int c = 0; c < 4; c++)
-            pq_free(B.items[w][c]);
+    pq_free(flags);
+    pq_free(pos);
+    pq_free(starts);
+    pq_free(cls);
+    pq_free(counts);
     return rc;
 }
 
