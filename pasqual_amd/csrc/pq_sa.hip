@@ -73,48 +73,82 @@ __global__ void k_bucket_keys(Geom G, const uint64_t *t2, uint64_t N, unsigned K
     V[g] = (uint32_t) g;
 }
 
-// One suffix of a run being sorted: position, remaining codes, and the first
-// two chunk keys from the run's common offset.
-struct SufKey {
-    uint32_t g, rem;
-    uint64_t k0, k1;
-};
-
+// One suffix of a run being sorted: position, codes left after the run's
+// common prefix of `base` codes, and those codes packed W words (32 codes
+// each, most significant first, zero past the end).  All of it is loaded up
+// front with independent loads, so comparisons inside the sorting networks
+// never wait on memory.
 #define SUF_NONE 0xFFFFFFFFu
 
+template <int W> struct SufW {
+    uint32_t g, len;
+    uint64_t w[W];
+};
+
+template <int W>
 __device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, uint32_t g,
-                                         uint32_t base, SufKey &s)
+                                         uint32_t base, SufW<W> &s)
 {
+    uint32_t rem = suf_rem(G, g);
     s.g = g;
-    s.rem = suf_rem(G, g);
-    s.k0 = chunk_key(t2, g, s.rem, base);
-    s.k1 = (s.k0 & 31ull) == 30ull ? chunk_key(t2, g, s.rem, base + CHUNK) : 0ull;
+    s.len = rem > base ? rem - base : 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        uint64_t x = 0;
+        if ((uint32_t) (32 * i) < s.len) {
+            x = pq_window(t2, (uint64_t) g + base + 32 * i);
+            uint32_t left = s.len - 32 * i;
+            if (left < 32)
+                x &= ~0ull << (64 - 2 * left);
+        }
+        s.w[i] = x;
+    }
+}
+
+template <int W> __device__ __forceinline__ void suf_none(SufW<W> &s)
+{
+    s.g = SUF_NONE;
+    s.len = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++)
+        s.w[i] = 0;
 }
 
 // Canonical order of two suffixes sharing their first `base` codes: string
-// order (end first) and then position.  Chunks past the cached two are read
-// from the text.
-__device__ __forceinline__ bool suf_less(const uint64_t *t2, uint32_t ga, uint32_t ra,
-                                         uint64_t a0, uint64_t a1, uint32_t gb, uint32_t rb,
-                                         uint64_t b0, uint64_t b1, uint32_t base)
+// order with the end of the read first, then position.
+template <int W> __device__ __forceinline__ bool suf_less(const SufW<W> &a, const SufW<W> &b)
 {
-    if (ga == SUF_NONE || gb == SUF_NONE)
-        return ga < gb;
-    if (a0 != b0)
-        return a0 < b0;
-    if ((a0 & 31ull) < 30ull)
-        return ga < gb;
-    if (a1 != b1)
-        return a1 < b1;
-    if ((a1 & 31ull) < 30ull)
-        return ga < gb;
-    for (uint32_t off = base + 2 * CHUNK;; off += CHUNK) {
-        uint64_t x = chunk_key(t2, ga, ra, off), y = chunk_key(t2, gb, rb, off);
-        if (x != y)
-            return x < y;
-        if ((x & 31ull) < 30ull)
-            return ga < gb;
+    if (a.g == SUF_NONE || b.g == SUF_NONE)
+        return a.g < b.g;
+    uint32_t m = a.len < b.len ? a.len : b.len;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        if ((uint32_t) (32 * i) < m) {
+            uint64_t x = a.w[i], y = b.w[i];
+            uint32_t left = m - 32 * i;
+            if (left < 32) {
+                uint64_t mask = ~0ull << (64 - 2 * left);
+                x &= mask;
+                y &= mask;
+            }
+            if (x != y)
+                return x < y;
+        }
     }
+    if (a.len != b.len)
+        return a.len < b.len;
+    return a.g < b.g;
+}
+
+template <int W> __device__ __forceinline__ SufW<W> suf_shfl_xor(const SufW<W> &s, int j)
+{
+    SufW<W> o;
+    o.g = __shfl_xor(s.g, j, 64);
+    o.len = __shfl_xor(s.len, j, 64);
+#pragma unroll
+    for (int i = 0; i < W; i++)
+        o.w[i] = __shfl_xor(s.w[i], j, 64);
+    return o;
 }
 
 // ---- run lists -----------------------------------------------------------------------
@@ -124,15 +158,19 @@ struct SegLists {
     uint32_t *count;         // 4 counters
     uint32_t cap[4];
     uint32_t *overflow;
+    uint32_t lcap;           // largest run of the LDS class (0: words too long)
 };
 
 #define RUN_SMALL 16
 #define RUN_WAVE 64
-#define RUN_LDS 2048
 
-__device__ __forceinline__ int seg_class(uint32_t size)
+// 0: <= 16 elements, 1: <= 64, 2: <= lcap (LDS), 3: larger.  lcap 0 means the
+// remaining suffix length exceeds the cached words: everything goes to class 3.
+__device__ __forceinline__ int seg_class(uint32_t size, uint32_t lcap)
 {
-    return size <= RUN_SMALL ? 0 : size <= RUN_WAVE ? 1 : size <= RUN_LDS ? 2 : 3;
+    if (lcap == 0)
+        return 3;
+    return size <= RUN_SMALL ? 0 : size <= RUN_WAVE ? 1 : size <= lcap ? 2 : 3;
 }
 
 // Wave-aggregated append (used only for the runs split off oversized runs);
@@ -141,7 +179,7 @@ __device__ __forceinline__ void push_seg(const SegLists &S, bool valid, uint32_t
                                          uint32_t size)
 {
     int lane = threadIdx.x & 63;
-    int cls = seg_class(size);
+    int cls = seg_class(size, S.lcap);
     uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
@@ -167,7 +205,7 @@ __device__ __forceinline__ void push_seg(const SegLists &S, bool valid, uint32_t
 // ---- sorts of one run ---------------------------------------------------------------
 
 // One group of G lanes per run (G = 16 or 64); bitonic network over lanes.
-template <int G>
+template <int G, int W>
 __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2, uint32_t *sa,
                                                      uint32_t base, const uint2 *segs,
                                                      uint32_t nseg)
@@ -178,60 +216,52 @@ __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2
     bool have = sid < nseg;
     uint2 sg = have ? segs[sid] : make_uint2(0, 0);
     bool e = have && (uint32_t) li < sg.y;
-    SufKey s;
-    if (e) {
-        suf_load(Gm, t2, sa[sg.x + li], base, s);
-    } else {
-        s.g = SUF_NONE;
-        s.rem = 0;
-        s.k0 = s.k1 = ~0ull;
-    }
+    SufW<W> s;
+    if (e)
+        suf_load<W>(Gm, t2, sa[sg.x + li], base, s);
+    else
+        suf_none<W>(s);
 #pragma unroll
     for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            uint32_t pg = __shfl_xor(s.g, j, 64), pr = __shfl_xor(s.rem, j, 64);
-            uint64_t p0 = __shfl_xor(s.k0, j, 64), p1 = __shfl_xor(s.k1, j, 64);
+            SufW<W> p = suf_shfl_xor<W>(s, j);
             bool up = (li & k) == 0;
             bool lower = (li & j) == 0;
-            bool pless = suf_less(t2, pg, pr, p0, p1, s.g, s.rem, s.k0, s.k1, base);
-            bool swap = (lower == up) ? pless : (!pless && pg != s.g);
-            if (swap) {
-                s.g = pg;
-                s.rem = pr;
-                s.k0 = p0;
-                s.k1 = p1;
-            }
+            bool pless = suf_less<W>(p, s);
+            bool swap = (lower == up) ? pless : (!pless && p.g != s.g);
+            if (swap)
+                s = p;
         }
     }
     if (e)
         sa[sg.x + li] = s.g;
 }
 
-// One block per run of 65..2048 elements: bitonic sort in LDS.
+// Largest run one 256-thread block sorts in LDS for W words per suffix.
+__host__ __device__ constexpr uint32_t lds_cap(int W)
+{
+    return W <= 2 ? 2048u : W <= 4 ? 1024u : 512u;
+}
+
+// One block per run of 65..lds_cap(W) elements: bitonic sort in LDS.
+template <int W>
 __global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, uint32_t *sa,
                                                   uint32_t base, const uint2 *segs)
 {
-    __shared__ uint64_t s0[RUN_LDS], s1[RUN_LDS];
-    __shared__ uint32_t sg_[RUN_LDS];
-    __shared__ uint16_t sr[RUN_LDS];
+    constexpr uint32_t CAP = lds_cap(W);
+    __shared__ SufW<W> sh[CAP];
     uint2 sg = segs[blockIdx.x];
     uint32_t n = sg.y, P = 1;
     while (P < n)
         P <<= 1;
     for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
-        if (i < n) {
-            SufKey s;
-            suf_load(Gm, t2, sa[sg.x + i], base, s);
-            sg_[i] = s.g;
-            sr[i] = (uint16_t) s.rem;
-            s0[i] = s.k0;
-            s1[i] = s.k1;
-        } else {
-            sg_[i] = SUF_NONE;
-            sr[i] = 0;
-            s0[i] = s1[i] = ~0ull;
-        }
+        SufW<W> s;
+        if (i < n)
+            suf_load<W>(Gm, t2, sa[sg.x + i], base, s);
+        else
+            suf_none<W>(s);
+        sh[i] = s;
     }
     __syncthreads();
     for (uint32_t k = 2; k <= P; k <<= 1) {
@@ -239,27 +269,17 @@ __global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, u
             for (uint32_t p = threadIdx.x; p < P / 2; p += blockDim.x) {
                 uint32_t i = 2 * j * (p / j) + (p % j), ip = i + j;
                 bool up = (i & k) == 0;
-                bool gt = suf_less(t2, sg_[ip], sr[ip], s0[ip], s1[ip], sg_[i], sr[i], s0[i],
-                                   s1[i], base);
-                if (gt == up) {
-                    uint64_t x0 = s0[i], x1 = s1[i];
-                    uint32_t xg = sg_[i];
-                    uint16_t xr = sr[i];
-                    s0[i] = s0[ip];
-                    s1[i] = s1[ip];
-                    sg_[i] = sg_[ip];
-                    sr[i] = sr[ip];
-                    s0[ip] = x0;
-                    s1[ip] = x1;
-                    sg_[ip] = xg;
-                    sr[ip] = xr;
+                SufW<W> x = sh[i], y = sh[ip];
+                if (suf_less<W>(y, x) == up) {
+                    sh[i] = y;
+                    sh[ip] = x;
                 }
             }
             __syncthreads();
         }
     }
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-        sa[sg.x + i] = sg_[i];
+        sa[sg.x + i] = sh[i].g;
 }
 
 // ---- runs of equal keys ---------------------------------------------------------------
@@ -285,14 +305,14 @@ __global__ void k_compact_heads(const uint32_t *flags, const uint32_t *pos, uint
 // Size class of every bucket run, or 4 when it needs no sort (one element, or
 // complete identical suffixes whose order is already the position order).
 __global__ void k_bucket_class(const uint32_t *BK, const uint32_t *starts, uint32_t nruns,
-                               unsigned K, uint8_t *cls)
+                               unsigned K, uint32_t lcap, uint8_t *cls)
 {
     uint64_t k = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nruns)
         return;
     uint32_t a = starts[k], size = starts[k + 1] - a;
     bool done = size <= 1 || (BK[a] & 15u) < K;
-    cls[k] = done ? 4 : (uint8_t) seg_class(size);
+    cls[k] = done ? 4 : (uint8_t) seg_class(size, lcap);
 }
 
 __global__ void k_class_flags(const uint8_t *cls, uint32_t nruns, uint8_t c, uint32_t *flags)
@@ -355,29 +375,58 @@ static int find_runs(pq_ctx_t *ctx, const KT *K, uint64_t n, uint32_t *flags, ui
     return PQ_OK;
 }
 
-static int sort_lists(pq_ctx_t *ctx, const Geom &G, DirIndex &D, uint32_t base,
-                      uint2 *const items[3], const uint32_t cnt[3])
+// Words of 32 codes cached per suffix for runs whose suffixes have at most
+// `left` codes after their common prefix (0: too long, sort by chunk levels).
+static int words_for(uint32_t left)
+{
+    return left <= 32 ? 1 : left <= 64 ? 2 : left <= 96 ? 3 : left <= 128 ? 4 : left <= 256 ? 8 : 0;
+}
+
+static uint32_t lcap_for(int W)
+{
+    return W == 0 ? 0u : lds_cap(W);
+}
+
+template <int W>
+static void launch_sorts(pq_ctx_t *ctx, const Geom &G, DirIndex &D, uint32_t base,
+                         uint2 *const items[3], const uint32_t cnt[3])
 {
     if (cnt[0])
-        hipLaunchKernelGGL(k_sort_groups<RUN_SMALL>,
+        hipLaunchKernelGGL((k_sort_groups<RUN_SMALL, W>),
                            dim3(pq_blocks((uint64_t) cnt[0] * RUN_SMALL, 256)), dim3(256), 0,
                            ctx->stream, G, D.text2, D.sa, base, items[0], cnt[0]);
     if (cnt[1])
-        hipLaunchKernelGGL(k_sort_groups<RUN_WAVE>,
+        hipLaunchKernelGGL((k_sort_groups<RUN_WAVE, W>),
                            dim3(pq_blocks((uint64_t) cnt[1] * RUN_WAVE, 256)), dim3(256), 0,
                            ctx->stream, G, D.text2, D.sa, base, items[1], cnt[1]);
     if (cnt[2])
-        hipLaunchKernelGGL(k_sort_lds, dim3(cnt[2]), dim3(256), 0, ctx->stream, G, D.text2, D.sa,
-                           base, items[2]);
+        hipLaunchKernelGGL(k_sort_lds<W>, dim3(cnt[2]), dim3(256), 0, ctx->stream, G, D.text2,
+                           D.sa, base, items[2]);
+}
+
+static int sort_lists(pq_ctx_t *ctx, const Geom &G, DirIndex &D, uint32_t base,
+                      uint2 *const items[3], const uint32_t cnt[3])
+{
+    uint32_t left = ctx->Lmax > base ? ctx->Lmax - base : 0;
+    switch (words_for(left)) {
+    case 1: launch_sorts<1>(ctx, G, D, base, items, cnt); break;
+    case 2: launch_sorts<2>(ctx, G, D, base, items, cnt); break;
+    case 3: launch_sorts<3>(ctx, G, D, base, items, cnt); break;
+    case 4: launch_sorts<4>(ctx, G, D, base, items, cnt); break;
+    case 8: launch_sorts<8>(ctx, G, D, base, items, cnt); break;
+    default:
+        if (cnt[0] || cnt[1] || cnt[2])
+            return ctx->fail(PQ_ERR_STATE, "run classified for cached words it cannot hold");
+    }
     PQ_CHECK(hipGetLastError());
     return PQ_OK;
 }
 
-// Bucket width for N suffixes: about 128 suffixes per bucket of K codes.
+// Bucket width for N suffixes: about 32 suffixes per bucket of K codes.
 static unsigned bucket_codes(uint64_t N)
 {
     unsigned K = 4;
-    while (K < 12 && (N >> (2 * K)) > 128)
+    while (K < 12 && (N >> (2 * K)) > 32)
         K++;
     return K;
 }
@@ -424,8 +473,9 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d)
         uint32_t nruns = 0;
         if ((rc = find_runs<uint32_t>(ctx, BK, N, flags, pos, starts, &nruns)))
             goto done;
+        uint32_t lcap0 = lcap_for(words_for(ctx->Lmax > K ? ctx->Lmax - K : 0));
         hipLaunchKernelGGL(k_bucket_class, dim3(pq_blocks(nruns, 256)), dim3(256), 0, ctx->stream,
-                           BK, starts, nruns, K, cls);
+                           BK, starts, nruns, K, lcap0, cls);
         uint32_t cnt[4] = {0, 0, 0, 0};
         std::vector<uint2 *> lists(4);
         uint2 *lbuf = nullptr;
@@ -478,6 +528,8 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d)
             }
             S.count = counts;
             S.overflow = counts + 4;
+            uint32_t nb = base + CHUNK;
+            S.lcap = lcap_for(words_for(ctx->Lmax > nb ? ctx->Lmax - nb : 0));
             PQ_CHECK(hipMemsetAsync(counts, 0, 64, ctx->stream));
             for (const uint2 &s : big) {
                 uint64_t *keys = KT, *keys2 = KT + maxn;
