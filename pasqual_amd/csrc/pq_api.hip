@@ -80,13 +80,7 @@ extern "C" void pq_ctx_destroy(pq_ctx_t *ctx)
     (void) hipSetDevice(ctx->device);
     (void) hipStreamSynchronize(ctx->stream);
     pq_index_release(ctx);
-    pq_free(ctx->d_ovB);
-    pq_free(ctx->d_ovF);
-    pq_free(ctx->d_numB);
-    pq_free(ctx->d_numF);
-    pq_free(ctx->d_edge_off);
-    pq_free(ctx->d_edges);
-    pq_learned_free(ctx);
+    pq_learned_free(ctx);   // search results live in scratch slots, freed below
     for (DevBuf &b : ctx->scratch)
         pq_free(b.p);
     (void) hipStreamDestroy(ctx->stream);

@@ -96,6 +96,27 @@ struct pq_ctx {
     void *tmp(int slot, size_t bytes);
 };
 
+// Scratch slots of pq_ctx::tmp (grow-only device buffers kept for the life of
+// the context, so repeated builds and searches allocate nothing).
+enum PqSlot {
+    SLOT_SCAN = 0, SLOT_RADIX, SLOT_OCC, SLOT_RUNS, SLOT_EXPORT, SLOT_BIGK, SLOT_BIGV,
+    SLOT_BK = 10, SLOT_BK2, SLOT_V2, SLOT_FLAGS, SLOT_POS, SLOT_STARTS, SLOT_CLS,
+    SLOT_COUNTS, SLOT_LISTS, SLOT_SUBLISTS,
+    SLOT_CAND_LC = 20, SLOT_CAND_O, SLOT_CAND_N, SLOT_RECB, SLOT_RECF, SLOT_APPEND,
+    SLOT_TOT, SLOT_OVL, SLOT_MISC, SLOT_SEEDS, SLOT_GSCRATCH, SLOT_STAGED,
+    SLOT_OVB = 34, SLOT_OVF, SLOT_NUMB, SLOT_NUMF, SLOT_EDGE_OFF, SLOT_EDGES,
+    SLOT_LEARNED = 40,
+};
+
+// Grow-only slot buffer of at least `bytes` bytes into *p, or PQ_ERR_NOMEM.
+template <typename T> static inline int pq_slot(pq_ctx_t *ctx, int slot, size_t bytes, T **p)
+{
+    *p = (T *) ctx->tmp(slot, bytes);
+    if (!*p)
+        return ctx->fail(PQ_ERR_NOMEM, "device scratch of " + std::to_string(bytes) + " bytes");
+    return PQ_OK;
+}
+
 // helpers shared across translation units
 int pq_alloc(pq_ctx_t *ctx, void **p, size_t bytes);
 void pq_free(void *p);

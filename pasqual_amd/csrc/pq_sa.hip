@@ -153,24 +153,31 @@ template <int W> __device__ __forceinline__ SufW<W> suf_shfl_xor(const SufW<W> &
 
 // ---- run lists -----------------------------------------------------------------------
 
+#define NCLS 5              // run size classes, see seg_class
+
 struct SegLists {
-    uint2 *items[4];
-    uint32_t *count;         // 4 counters
-    uint32_t cap[4];
+    uint2 *items[NCLS];
+    uint32_t *count;         // NCLS counters
+    uint32_t cap[NCLS];
     uint32_t *overflow;
     uint32_t lcap;           // largest run of the LDS class (0: words too long)
 };
 
 #define RUN_SMALL 16
+#define RUN_HALF 32
 #define RUN_WAVE 64
+#define CLS_BIG (NCLS - 1)
+#define CLS_DONE 255
 
-// 0: <= 16 elements, 1: <= 64, 2: <= lcap (LDS), 3: larger.  lcap 0 means the
-// remaining suffix length exceeds the cached words: everything goes to class 3.
+// 0: <= 16 elements, 1: <= 32, 2: <= 64 (lanes of one wave), 3: <= lcap (LDS),
+// 4: larger.  lcap 0 means the remaining suffix length exceeds the cached
+// words: everything goes to the chunk-level path.
 __device__ __forceinline__ int seg_class(uint32_t size, uint32_t lcap)
 {
     if (lcap == 0)
-        return 3;
-    return size <= RUN_SMALL ? 0 : size <= RUN_WAVE ? 1 : size <= lcap ? 2 : 3;
+        return CLS_BIG;
+    return size <= RUN_SMALL ? 0 : size <= RUN_HALF ? 1 : size <= RUN_WAVE ? 2
+                                 : size <= lcap ? 3 : CLS_BIG;
 }
 
 // Wave-aggregated append (used only for the runs split off oversized runs);
@@ -182,7 +189,7 @@ __device__ __forceinline__ void push_seg(const SegLists &S, bool valid, uint32_t
     int cls = seg_class(size, S.lcap);
     uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < NCLS; c++) {
         bool mine = valid && cls == c;
         uint64_t m = __ballot(mine);
         if (m) {
@@ -302,8 +309,8 @@ __global__ void k_compact_heads(const uint32_t *flags, const uint32_t *pos, uint
         starts[pos[n - 1] + flags[n - 1]] = (uint32_t) n;   // sentinel
 }
 
-// Size class of every bucket run, or 4 when it needs no sort (one element, or
-// complete identical suffixes whose order is already the position order).
+// Size class of every bucket run, or CLS_DONE when it needs no sort (one
+// element, or complete identical suffixes already in position order).
 __global__ void k_bucket_class(const uint32_t *BK, const uint32_t *starts, uint32_t nruns,
                                unsigned K, uint32_t lcap, uint8_t *cls)
 {
@@ -312,7 +319,7 @@ __global__ void k_bucket_class(const uint32_t *BK, const uint32_t *starts, uint3
         return;
     uint32_t a = starts[k], size = starts[k + 1] - a;
     bool done = size <= 1 || (BK[a] & 15u) < K;
-    cls[k] = done ? 4 : (uint8_t) seg_class(size, lcap);
+    cls[k] = done ? CLS_DONE : (uint8_t) seg_class(size, lcap);
 }
 
 __global__ void k_class_flags(const uint8_t *cls, uint32_t nruns, uint8_t c, uint32_t *flags)
@@ -389,23 +396,27 @@ static uint32_t lcap_for(int W)
 
 template <int W>
 static void launch_sorts(pq_ctx_t *ctx, const Geom &G, DirIndex &D, uint32_t base,
-                         uint2 *const items[3], const uint32_t cnt[3])
+                         uint2 *const items[CLS_BIG], const uint32_t cnt[CLS_BIG])
 {
     if (cnt[0])
         hipLaunchKernelGGL((k_sort_groups<RUN_SMALL, W>),
                            dim3(pq_blocks((uint64_t) cnt[0] * RUN_SMALL, 256)), dim3(256), 0,
                            ctx->stream, G, D.text2, D.sa, base, items[0], cnt[0]);
     if (cnt[1])
-        hipLaunchKernelGGL((k_sort_groups<RUN_WAVE, W>),
-                           dim3(pq_blocks((uint64_t) cnt[1] * RUN_WAVE, 256)), dim3(256), 0,
+        hipLaunchKernelGGL((k_sort_groups<RUN_HALF, W>),
+                           dim3(pq_blocks((uint64_t) cnt[1] * RUN_HALF, 256)), dim3(256), 0,
                            ctx->stream, G, D.text2, D.sa, base, items[1], cnt[1]);
     if (cnt[2])
-        hipLaunchKernelGGL(k_sort_lds<W>, dim3(cnt[2]), dim3(256), 0, ctx->stream, G, D.text2,
-                           D.sa, base, items[2]);
+        hipLaunchKernelGGL((k_sort_groups<RUN_WAVE, W>),
+                           dim3(pq_blocks((uint64_t) cnt[2] * RUN_WAVE, 256)), dim3(256), 0,
+                           ctx->stream, G, D.text2, D.sa, base, items[2], cnt[2]);
+    if (cnt[3])
+        hipLaunchKernelGGL(k_sort_lds<W>, dim3(cnt[3]), dim3(256), 0, ctx->stream, G, D.text2,
+                           D.sa, base, items[3]);
 }
 
 static int sort_lists(pq_ctx_t *ctx, const Geom &G, DirIndex &D, uint32_t base,
-                      uint2 *const items[3], const uint32_t cnt[3])
+                      uint2 *const items[CLS_BIG], const uint32_t cnt[CLS_BIG])
 {
     uint32_t left = ctx->Lmax > base ? ctx->Lmax - base : 0;
     switch (words_for(left)) {
@@ -415,7 +426,7 @@ static int sort_lists(pq_ctx_t *ctx, const Geom &G, DirIndex &D, uint32_t base,
     case 4: launch_sorts<4>(ctx, G, D, base, items, cnt); break;
     case 8: launch_sorts<8>(ctx, G, D, base, items, cnt); break;
     default:
-        if (cnt[0] || cnt[1] || cnt[2])
+        if (cnt[0] || cnt[1] || cnt[2] || cnt[3])
             return ctx->fail(PQ_ERR_STATE, "run classified for cached words it cannot hold");
     }
     PQ_CHECK(hipGetLastError());
@@ -445,30 +456,28 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d)
     uint2 *items[4] = {nullptr, nullptr, nullptr, nullptr};
     uint32_t *counts = nullptr;
     int rc = PQ_OK;
-    if ((rc = pq_alloc(ctx, (void **) &BK, N * 4)) || (rc = pq_alloc(ctx, (void **) &BK2, N * 4)) ||
-        (rc = pq_alloc(ctx, (void **) &V2, N * 4)) ||
-        (rc = pq_alloc(ctx, (void **) &flags, (N + 1) * 4)) ||
-        (rc = pq_alloc(ctx, (void **) &pos, (N + 1) * 4)) ||
-        (rc = pq_alloc(ctx, (void **) &starts, (N + 2) * 4)) ||
-        (rc = pq_alloc(ctx, (void **) &cls, N + 1)) ||
-        (rc = pq_alloc(ctx, (void **) &counts, 64)))
+    if ((rc = pq_slot(ctx, SLOT_BK, N * 4, &BK)) || (rc = pq_slot(ctx, SLOT_BK2, N * 4, &BK2)) ||
+        (rc = pq_slot(ctx, SLOT_V2, N * 4, &V2)) ||
+        (rc = pq_slot(ctx, SLOT_FLAGS, (N + 1) * 4, &flags)) ||
+        (rc = pq_slot(ctx, SLOT_POS, (N + 1) * 4, &pos)) ||
+        (rc = pq_slot(ctx, SLOT_STARTS, (N + 2) * 4, &starts)) ||
+        (rc = pq_slot(ctx, SLOT_CLS, N + 1, &cls)) ||
+        (rc = pq_slot(ctx, SLOT_COUNTS, 64, &counts)))
         goto done;
     {
         // 1. bucket by the first K codes (and the length below K)
-        hipLaunchKernelGGL(k_bucket_keys, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, G,
-                           D.text2, N, K, BK, D.sa);
         unsigned bits = 2 * K + 4, passes = (bits + 7) / 8;
-        uint32_t *kc = BK, *ka = BK2, *vc = D.sa, *va = V2;
+        // start in the pair that makes the last pass write (BK, D.sa)
+        uint32_t *kc = (passes & 1) ? BK2 : BK, *ka = (passes & 1) ? BK : BK2;
+        uint32_t *vc = (passes & 1) ? V2 : D.sa, *va = (passes & 1) ? D.sa : V2;
+        hipLaunchKernelGGL(k_bucket_keys, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, G,
+                           D.text2, N, K, kc, vc);
         for (unsigned p = 0; p < passes; p++) {
             if ((rc = pq_radix_pass32(ctx, kc, vc, ka, va, N, 8 * p)))
                 goto done;
             std::swap(kc, ka);
             std::swap(vc, va);
         }
-        BK = kc;
-        BK2 = ka;
-        D.sa = vc;
-        V2 = va;
         // 2. bucket runs, classified by size with scans
         uint32_t nruns = 0;
         if ((rc = find_runs<uint32_t>(ctx, BK, N, flags, pos, starts, &nruns)))
@@ -476,36 +485,31 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d)
         uint32_t lcap0 = lcap_for(words_for(ctx->Lmax > K ? ctx->Lmax - K : 0));
         hipLaunchKernelGGL(k_bucket_class, dim3(pq_blocks(nruns, 256)), dim3(256), 0, ctx->stream,
                            BK, starts, nruns, K, lcap0, cls);
-        uint32_t cnt[4] = {0, 0, 0, 0};
-        std::vector<uint2 *> lists(4);
+        uint32_t cnt[NCLS] = {0, 0, 0, 0, 0};
+        std::vector<uint2 *> lists(NCLS);
         uint2 *lbuf = nullptr;
-        if ((rc = pq_alloc(ctx, (void **) &lbuf, (size_t) (nruns + 1) * 4 * sizeof(uint2))))
+        if ((rc = pq_slot(ctx, SLOT_LISTS, (size_t) (nruns + 1) * NCLS * sizeof(uint2), &lbuf)))
             goto done;
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < NCLS; c++) {
             lists[c] = lbuf + (size_t) c * (nruns + 1);
             hipLaunchKernelGGL(k_class_flags, dim3(pq_blocks(nruns, 256)), dim3(256), 0,
                                ctx->stream, cls, nruns, (uint8_t) c, flags);
-            if ((rc = pq_scan_u32(ctx, flags, pos, nruns, &cnt[c]))) {
-                pq_free(lbuf);
+            if ((rc = pq_scan_u32(ctx, flags, pos, nruns, &cnt[c])))
                 goto done;
-            }
             hipLaunchKernelGGL(k_class_compact, dim3(pq_blocks(nruns, 256)), dim3(256), 0,
                                ctx->stream, flags, pos, starts, nruns, lists[c]);
         }
         // 3. sort every bucket past its first K codes
-        uint2 *l3[3] = {lists[0], lists[1], lists[2]};
-        if ((rc = sort_lists(ctx, G, D, K, l3, cnt))) {
-            pq_free(lbuf);
+        uint2 *l3[CLS_BIG] = {lists[0], lists[1], lists[2], lists[3]};
+        if ((rc = sort_lists(ctx, G, D, K, l3, cnt)))
             goto done;
-        }
         // 4. oversized buckets: MSD over 29-code chunks, level by level
-        std::vector<uint2> big(cnt[3]);
-        if (cnt[3]) {
-            PQ_CHECK(hipMemcpyAsync(big.data(), lists[3], cnt[3] * sizeof(uint2),
+        std::vector<uint2> big(cnt[CLS_BIG]);
+        if (cnt[CLS_BIG]) {
+            PQ_CHECK(hipMemcpyAsync(big.data(), lists[CLS_BIG], cnt[CLS_BIG] * sizeof(uint2),
                                     hipMemcpyDeviceToHost, ctx->stream));
             PQ_CHECK(hipStreamSynchronize(ctx->stream));
         }
-        pq_free(lbuf);
         uint32_t base = K;
         SegLists S;
         uint2 *sbuf = nullptr;
@@ -514,20 +518,21 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d)
             uint64_t maxn = 0;
             for (const uint2 &s : big)
                 maxn = std::max<uint64_t>(maxn, s.y);
-            uint64_t *KT = (uint64_t *) ctx->tmp(5, maxn * 8 * 2 + 64);
-            uint32_t *VT = (uint32_t *) ctx->tmp(6, maxn * 4 + 64);
+            uint64_t *KT = (uint64_t *) ctx->tmp(SLOT_BIGK, maxn * 8 * 2 + 64);
+            uint32_t *VT = (uint32_t *) ctx->tmp(SLOT_BIGV, maxn * 4 + 64);
             if (!KT || !VT) {
                 rc = ctx->fail(PQ_ERR_NOMEM, "oversized run scratch");
                 break;
             }
-            if (!sbuf && (rc = pq_alloc(ctx, (void **) &sbuf, (size_t) scap * 4 * sizeof(uint2))))
+            if (!sbuf &&
+                (rc = pq_slot(ctx, SLOT_SUBLISTS, (size_t) scap * NCLS * sizeof(uint2), &sbuf)))
                 break;
-            for (int c = 0; c < 4; c++) {
+            for (int c = 0; c < NCLS; c++) {
                 S.items[c] = sbuf + (size_t) c * scap;
                 S.cap[c] = scap;
             }
             S.count = counts;
-            S.overflow = counts + 4;
+            S.overflow = counts + NCLS;
             uint32_t nb = base + CHUNK;
             S.lcap = lcap_for(words_for(ctx->Lmax > nb ? ctx->Lmax - nb : 0));
             PQ_CHECK(hipMemsetAsync(counts, 0, 64, ctx->stream));
@@ -550,20 +555,21 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d)
             }
             if (rc)
                 break;
-            uint32_t c5[5];
-            PQ_CHECK(hipMemcpyAsync(c5, counts, 5 * 4, hipMemcpyDeviceToHost, ctx->stream));
+            uint32_t c5[NCLS + 1];
+            PQ_CHECK(hipMemcpyAsync(c5, counts, (NCLS + 1) * 4, hipMemcpyDeviceToHost,
+                                    ctx->stream));
             PQ_CHECK(hipStreamSynchronize(ctx->stream));
-            if (c5[4]) {
+            if (c5[NCLS]) {
                 rc = ctx->fail(PQ_ERR_LIMIT, "suffix sort run list overflow");
                 break;
             }
             base += CHUNK;
-            uint2 *sl[3] = {S.items[0], S.items[1], S.items[2]};
+            uint2 *sl[CLS_BIG] = {S.items[0], S.items[1], S.items[2], S.items[3]};
             if ((rc = sort_lists(ctx, G, D, base, sl, c5)))
                 break;
-            big.assign(c5[3], make_uint2(0, 0));
-            if (c5[3]) {
-                PQ_CHECK(hipMemcpyAsync(big.data(), S.items[3], c5[3] * sizeof(uint2),
+            big.assign(c5[CLS_BIG], make_uint2(0, 0));
+            if (c5[CLS_BIG]) {
+                PQ_CHECK(hipMemcpyAsync(big.data(), S.items[CLS_BIG], c5[CLS_BIG] * sizeof(uint2),
                                         hipMemcpyDeviceToHost, ctx->stream));
                 PQ_CHECK(hipStreamSynchronize(ctx->stream));
             }
@@ -572,17 +578,8 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d)
                 break;
             }
         }
-        pq_free(sbuf);
     }
 done:
-    pq_free(BK);
-    pq_free(BK2);
-    pq_free(V2);
-    pq_free(flags);
-    pq_free(pos);
-    pq_free(starts);
-    pq_free(cls);
-    pq_free(counts);
     return rc;
 }
 
@@ -664,7 +661,7 @@ int pq_build_occ_dir(pq_ctx_t *ctx, int d)
         return rc;
     if (!D.reads_index && (rc = pq_alloc(ctx, (void **) &D.reads_index, (ctx->n + 1) * 4)))
         return rc;
-    uint4 *cnts = (uint4 *) ctx->tmp(2, D.nlines * sizeof(uint4) * 2);
+    uint4 *cnts = (uint4 *) ctx->tmp(SLOT_OCC, D.nlines * sizeof(uint4) * 2);
     if (!cnts)
         return ctx->fail(PQ_ERR_NOMEM, "occ scratch");
     uint4 *excl = cnts + D.nlines;

@@ -165,11 +165,15 @@ __device__ __forceinline__ int key_cmp(const uint64_t *a, uint32_t la, const uin
     return la < lb ? -2 : (la > lb ? 2 : 0);
 }
 
+#define EDGE_SLOT 16          // edges per read side staged without any atomic
+
 struct ExtOut {
     int32_t *ovB, *ovF;
     uint32_t *numB, *numF;
-    uint64_t *recB, *recF;   // append positions of the read's B and F edges
-    uint64_t *edges;
+    uint64_t *recB, *recF;   // pool position of a side's edges when it has more
+                             // than EDGE_SLOT of them
+    uint64_t *staged;        // EDGE_SLOT edges per (read, side)
+    uint64_t *edges;         // pool for larger sides
     uint64_t edge_cap;
     unsigned long long *edge_count;
     uint32_t *overflow_reads;     // reads left for the global-memory pass
@@ -311,11 +315,13 @@ __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw
                 emitf[q] = (uint8_t) keep;
         }
         uint64_t base = 0;
+        uint64_t *dst = out.staged + (r * 2 + side) * EDGE_SLOT;
         if (lane == 0) {
-            if (emitted)
+            if (emitted > EDGE_SLOT) {
                 base = atomicAdd(out.edge_count, (unsigned long long) emitted);
-            if (base + emitted > out.edge_cap)
-                atomicOr(out.flags, 1u);
+                if (base + emitted > out.edge_cap)
+                    atomicOr(out.flags, 1u);
+            }
             if (side) {
                 out.ovF[r] = (int32_t) valid_items;
                 out.numF[r] = emitted;
@@ -327,6 +333,11 @@ __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw
             }
         }
         base = __shfl(base, 0, 64);
+        uint64_t cap = EDGE_SLOT;
+        if (emitted > EDGE_SLOT) {
+            dst = out.edges + base;
+            cap = base < out.edge_cap ? out.edge_cap - base : 0;
+        }
         wave_sync();
         // output position = rank among the emitted items by (key, rank, o, m)
         if (emitted) {
@@ -347,9 +358,8 @@ __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw
                                                (mj.o < mi.o || (mj.o == mi.o && mj.m < mi.m)))));
                     pos += before ? 1u : 0u;
                 }
-                uint64_t at = base + pos;
-                if (at < out.edge_cap)
-                    out.edges[at] = ((uint64_t) mi.m << 32) | ((uint64_t) mi.o << 8) | mi.type;
+                if (pos < cap)
+                    dst[pos] = ((uint64_t) mi.m << 32) | ((uint64_t) mi.o << 8) | mi.type;
             }
         }
         wave_sync();
@@ -419,16 +429,19 @@ __global__ void k_edge_totals(const uint32_t *nb, const uint32_t *nf, uint64_t n
 }
 
 __global__ void k_csr(const uint64_t *recB, const uint64_t *recF, const uint32_t *numB,
-                      const uint64_t *off, const uint64_t *src, uint64_t n, uint64_t *dst)
+                      const uint32_t *numF, const uint64_t *off, const uint64_t *staged,
+                      const uint64_t *pool, uint64_t n, uint64_t *dst)
 {
     uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n)
         return;
-    uint64_t a = off[r], b = off[r + 1], nb = numB[r];
+    uint64_t a = off[r], nb = numB[r], nf = numF[r];
+    const uint64_t *sb = nb > EDGE_SLOT ? pool + recB[r] : staged + (r * 2) * EDGE_SLOT;
+    const uint64_t *sf = nf > EDGE_SLOT ? pool + recF[r] : staged + (r * 2 + 1) * EDGE_SLOT;
     for (uint64_t i = 0; i < nb; i++)
-        dst[a + i] = src[recB[r] + i];
-    for (uint64_t i = a + nb; i < b; i++)
-        dst[i] = src[recF[r] + (i - a - nb)];
+        dst[a + i] = sb[i];
+    for (uint64_t i = 0; i < nf; i++)
+        dst[a + nb + i] = sf[i];
 }
 
 static double ev_ms(hipEvent_t a, hipEvent_t b)
@@ -458,22 +471,13 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     ctx->timing.find_kernel_ms = 0;
     ctx->timing.find_launches = 0;
     ctx->timing.find_positions = ctx->timing.find_lines = 0;
-    pq_free(ctx->d_ovB);
-    pq_free(ctx->d_ovF);
-    pq_free(ctx->d_numB);
-    pq_free(ctx->d_numF);
-    pq_free(ctx->d_edge_off);
-    pq_free(ctx->d_edges);
-    ctx->d_ovB = ctx->d_ovF = nullptr;
-    ctx->d_numB = ctx->d_numF = nullptr;
-    ctx->d_edge_off = ctx->d_edges = nullptr;
     ctx->n_edges = 0;
     int rc;
-    if ((rc = pq_alloc(ctx, (void **) &ctx->d_ovB, (n + 1) * 4)) ||
-        (rc = pq_alloc(ctx, (void **) &ctx->d_ovF, (n + 1) * 4)) ||
-        (rc = pq_alloc(ctx, (void **) &ctx->d_numB, (n + 1) * 4)) ||
-        (rc = pq_alloc(ctx, (void **) &ctx->d_numF, (n + 1) * 4)) ||
-        (rc = pq_alloc(ctx, (void **) &ctx->d_edge_off, (n + 1) * 8)))
+    if ((rc = pq_slot(ctx, SLOT_OVB, (n + 1) * 4, &ctx->d_ovB)) ||
+        (rc = pq_slot(ctx, SLOT_OVF, (n + 1) * 4, &ctx->d_ovF)) ||
+        (rc = pq_slot(ctx, SLOT_NUMB, (n + 1) * 4, &ctx->d_numB)) ||
+        (rc = pq_slot(ctx, SLOT_NUMF, (n + 1) * 4, &ctx->d_numF)) ||
+        (rc = pq_slot(ctx, SLOT_EDGE_OFF, (n + 1) * 8, &ctx->d_edge_off)))
         return rc;
     if (n == 0) {
         PQ_CHECK(hipMemsetAsync(ctx->d_edge_off, 0, 8, ctx->stream));
@@ -490,9 +494,9 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     // batch so the candidate buffer stays within ~2 GiB
     uint64_t per_read = 4ull * cb.cap * (sizeof(uint2) + sizeof(uint16_t)) + 16;
     uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(n, (2ull << 30) / per_read));
-    uint64_t *rec = nullptr, *recF = nullptr, *app = nullptr, *tot = nullptr;
+    uint64_t *rec = nullptr, *recF = nullptr, *app = nullptr, *tot = nullptr, *staged = nullptr;
     uint32_t *ovl = nullptr, *misc = nullptr;
-    uint64_t edge_cap = std::max<uint64_t>(1 << 20, 8 * n);
+    uint64_t edge_cap = std::max<uint64_t>(1 << 20, n / 4);
     Seed *seeds = nullptr;
     char *gscratch = nullptr;
     hipEvent_t e0, e1, e2;
@@ -500,17 +504,6 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     (void) hipEventCreate(&e1);
     (void) hipEventCreate(&e2);
     auto cleanup = [&]() {
-        pq_free(cb.lc);
-        pq_free(cb.o);
-        pq_free(cb.n);
-        pq_free(rec);
-        pq_free(recF);
-        pq_free(app);
-        pq_free(tot);
-        pq_free(ovl);
-        pq_free(misc);
-        pq_free(seeds);
-        pq_free(gscratch);
         (void) hipEventDestroy(e0);
         (void) hipEventDestroy(e1);
         (void) hipEventDestroy(e2);
@@ -518,17 +511,18 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     cb.lc = nullptr;
     cb.o = nullptr;
     cb.n = nullptr;
-    if ((rc = pq_alloc(ctx, (void **) &cb.lc, batch * 4 * cb.cap * sizeof(uint2))) ||
-        (rc = pq_alloc(ctx, (void **) &cb.o, batch * 4 * cb.cap * sizeof(uint16_t))) ||
-        (rc = pq_alloc(ctx, (void **) &cb.n, batch * 4 * sizeof(uint32_t))) ||
-        (rc = pq_alloc(ctx, (void **) &rec, (n + 1) * 8)) ||
-        (rc = pq_alloc(ctx, (void **) &recF, (n + 1) * 8)) ||
-        (rc = pq_alloc(ctx, (void **) &app, edge_cap * 8)) ||
-        (rc = pq_alloc(ctx, (void **) &tot, (n + 1) * 8)) ||
-        (rc = pq_alloc(ctx, (void **) &ovl, (batch + 1) * 4)) ||
-        (rc = pq_alloc(ctx, (void **) &misc, 64)) ||
+    if ((rc = pq_slot(ctx, SLOT_CAND_LC, batch * 4 * cb.cap * sizeof(uint2), &cb.lc)) ||
+        (rc = pq_slot(ctx, SLOT_CAND_O, batch * 4 * cb.cap * sizeof(uint16_t), &cb.o)) ||
+        (rc = pq_slot(ctx, SLOT_CAND_N, batch * 4 * sizeof(uint32_t), &cb.n)) ||
+        (rc = pq_slot(ctx, SLOT_RECB, (n + 1) * 8, &rec)) ||
+        (rc = pq_slot(ctx, SLOT_RECF, (n + 1) * 8, &recF)) ||
+        (rc = pq_slot(ctx, SLOT_APPEND, edge_cap * 8, &app)) ||
+        (rc = pq_slot(ctx, SLOT_STAGED, (n + 1) * 2 * EDGE_SLOT * 8, &staged)) ||
+        (rc = pq_slot(ctx, SLOT_TOT, (n + 1) * 8, &tot)) ||
+        (rc = pq_slot(ctx, SLOT_OVL, (batch + 1) * 4, &ovl)) ||
+        (rc = pq_slot(ctx, SLOT_MISC, 64, &misc)) ||
         (mode == PQ_SEARCH_LEARNED &&
-         (rc = pq_alloc(ctx, (void **) &seeds, batch * 4 * sizeof(Seed))))) {
+         (rc = pq_slot(ctx, SLOT_SEEDS, batch * 4 * sizeof(Seed), &seeds)))) {
         cleanup();
         return rc;
     }
@@ -540,6 +534,7 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     eo.recB = rec;
     eo.recF = recF;
     eo.edges = app;
+    eo.staged = staged;
     eo.edge_cap = edge_cap;
     eo.edge_count = (unsigned long long *) misc;          // misc[0..1]
     eo.n_overflow = misc + 2;
@@ -575,9 +570,9 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         PQ_CHECK(hipStreamSynchronize(ctx->stream));
         if (nov) {
             const uint32_t slots = 64;
-            if (!gscratch &&
-                (rc = pq_alloc(ctx, (void **) &gscratch,
-                               slots * ext_wave_bytes(KMAX_GLOBAL, kw, cb.cap))))
+            if (!gscratch && (rc = pq_slot(ctx, SLOT_GSCRATCH,
+                                           slots * ext_wave_bytes(KMAX_GLOBAL, kw, cb.cap),
+                                           &gscratch)))
                 break;
             hipLaunchKernelGGL(k_extract_global, dim3(std::min<uint32_t>(slots, nov)), dim3(64), 0,
                                ctx->stream, G, fw, bw, cb, r0, ovl, nov, kw, gscratch, eo);
@@ -612,9 +607,10 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         uint64_t total = 0;
         rc = pq_scan_u64(ctx, tot, ctx->d_edge_off, n + 1, &total);
         if (rc == PQ_OK) {
-            if ((rc = pq_alloc(ctx, (void **) &ctx->d_edges, (total + 1) * 8)) == PQ_OK) {
+            if ((rc = pq_slot(ctx, SLOT_EDGES, (total + 1) * 8, &ctx->d_edges)) == PQ_OK) {
                 hipLaunchKernelGGL(k_csr, dim3(pq_blocks(n, 256)), dim3(256), 0, ctx->stream, rec,
-                                   recF, ctx->d_numB, ctx->d_edge_off, app, n, ctx->d_edges);
+                                   recF, ctx->d_numB, ctx->d_numF, ctx->d_edge_off, staged, app, n,
+                                   ctx->d_edges);
                 PQ_CHECK(hipStreamSynchronize(ctx->stream));
                 ctx->n_edges = total;
                 ctx->searched = true;

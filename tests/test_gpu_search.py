@@ -80,11 +80,12 @@ def test_graph_with_many_candidates(ctx):
     _same_graph(g, ex, "deep")
 
 
-def _identical_copies():
-    # three copies of one read overlapped by another read: the copies form one
-    # candidate group whose members are emitted in read order
+def _identical_copies(copies=3):
+    # copies of one read overlapped by another read: the copies form one
+    # candidate group whose members are emitted in read order (more than 16 of
+    # them take the shared edge pool instead of the per-read slot)
     g = np.random.default_rng(3).integers(1, 5, 30).astype(np.uint8)
-    reads = [g[0:16], g[0:16], g[8:24], g[0:16], g[14:30]]
+    reads = [g[0:16], g[0:16], g[8:24]] + [g[0:16]] * (copies - 2) + [g[14:30]]
     off = np.zeros(len(reads) + 1, np.uint64)
     off[1:] = np.cumsum([len(r) for r in reads])
     return np.concatenate(reads), off, 6
@@ -96,6 +97,7 @@ def test_graph_edge_cases(ctx):
         (np.array([1, 2, 3, 4] * 10, np.uint8), np.array([0, 40], np.uint64), 12),
         (np.array([1, 2, 3, 4, 1, 2, 3, 4, 1, 2], np.uint8), np.array([0, 5, 10], np.uint64), 12),
         _identical_copies(),
+        _identical_copies(40),
     ]:
         ctx.upload(codes, off, t)
         ctx.build_index()
