@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--cpu-baseline", default="full", choices=["full", "off"])
+    ap.add_argument("--probe-copy", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -114,6 +116,10 @@ def main():
         lines += tm["find_lines"]
     ctx.sync()
     total_s = time.perf_counter() - t_start
+    if args.probe_copy:
+        pms = ctx.probe_copy(args.probe_copy << 20)
+        log(f"copy probe: {args.probe_copy} MiB in {pms:.3f} ms "
+            f"({2 * (args.probe_copy << 20) / pms / 1e6:.1f} GB/s read+write)")
     g = ctx.graph()
     edges = int(g["edge_off"][-1])
     ms_per_step = total_s * 1e3 / args.steps
@@ -130,8 +136,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_find", "alg_bytes_per_launch": int(alg_bytes),
-                "rank_positions_per_read": round(per_launch_positions / n, 1),
-                "lines_per_read": round(lines / max(1, launches) / n, 1),
+                "rank_positions_per_read": round(positions / (n * args.steps), 1),
+                "lines_per_read": round(lines / (n * args.steps), 1),
                 "avg_launch_ms": round(avg_launch_ms, 3)}
 
     base = None

@@ -137,6 +137,10 @@ typedef struct {
 } pq_timing_t;
 int pq_get_timing(pq_ctx_t *ctx, pq_timing_t *t);
 int pq_sync(pq_ctx_t *ctx);
+/* Measurement helper: one plain device-to-device copy of `bytes` bytes by a
+ * streaming kernel (k_copy_probe), the known transfer against which hardware
+ * traffic counters are scaled; *ms gets its duration. */
+int pq_probe_copy(pq_ctx_t *ctx, uint64_t bytes, double *ms);
 
 /* ---- host utilities (no device use) --------------------------------------------------- */
 /* Seeded synthetic workloads (see pq_synth.c for the generator contract). */

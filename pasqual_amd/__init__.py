@@ -32,7 +32,7 @@ PQ_SYMBOLS = [
     "pq_get_sa", "pq_get_bwt", "pq_get_reads_index", "pq_get_counts", "pq_get_occ_ref",
     "pq_get_shortcut", "pq_num_reads", "pq_text_len", "pq_find_overlaps", "pq_graph_sizes",
     "pq_graph_copy", "pq_learned_default", "pq_learned_train", "pq_learned_stats",
-    "pq_get_timing", "pq_sync", "pq_synth_preset", "pq_synth_num_reads", "pq_synth_genome",
+    "pq_get_timing", "pq_sync", "pq_probe_copy", "pq_synth_preset", "pq_synth_num_reads", "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
 ]
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
@@ -134,6 +134,7 @@ def lib(width="u32"):
         "pq_learned_stats": (I32, [P, P]),
         "pq_get_timing": (I32, [P, P]),
         "pq_sync": (I32, [P]),
+        "pq_probe_copy": (I32, [P, U64, P]),
         "pq_synth_preset": (I32, [I32, P]),
         "pq_synth_num_reads": (U64, [P]),
         "pq_synth_genome": (I32, [P, P]),
@@ -329,3 +330,8 @@ class Context:
 
     def sync(self):
         self._check(self._L.pq_sync(self._h), "sync")
+
+    def probe_copy(self, nbytes):
+        ms = C.c_double()
+        self._check(self._L.pq_probe_copy(self._h, nbytes, C.byref(ms)), "probe_copy")
+        return ms.value

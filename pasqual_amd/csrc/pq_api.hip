@@ -332,6 +332,41 @@ extern "C" int pq_get_shortcut(pq_ctx_t *ctx, int dir, uint64_t *start, uint64_t
     return PQ_OK;
 }
 
+// A plain streaming copy of `bytes` bytes between two device buffers: the
+// reference transfer for scaling hardware traffic counters.
+__global__ void k_copy_probe(const uint4 *src, uint4 *dst, uint64_t n16)
+{
+    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t) gridDim.x * blockDim.x;
+    for (; i < n16; i += stride)
+        dst[i] = src[i];
+}
+
+extern "C" int pq_probe_copy(pq_ctx_t *ctx, uint64_t bytes, double *ms)
+{
+    if (!ctx || bytes < 16)
+        return PQ_ERR_ARG;
+    uint64_t n16 = bytes / 16;
+    char *buf = (char *) ctx->tmp(SLOT_PROBE, n16 * 32);
+    if (!buf)
+        return ctx->fail(PQ_ERR_NOMEM, "copy probe buffers");
+    hipEvent_t a, b;
+    PQ_CHECK(hipEventCreate(&a));
+    PQ_CHECK(hipEventCreate(&b));
+    PQ_CHECK(hipEventRecord(a, ctx->stream));
+    hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(256), 0, ctx->stream, (const uint4 *) buf,
+                       (uint4 *) (buf + n16 * 16), n16);
+    PQ_CHECK(hipEventRecord(b, ctx->stream));
+    PQ_CHECK(hipEventSynchronize(b));
+    float t = 0;
+    (void) hipEventElapsedTime(&t, a, b);
+    if (ms)
+        *ms = t;
+    (void) hipEventDestroy(a);
+    (void) hipEventDestroy(b);
+    return PQ_OK;
+}
+
 extern "C" int pq_get_timing(pq_ctx_t *ctx, pq_timing_t *t)
 {
     if (!ctx || !t)

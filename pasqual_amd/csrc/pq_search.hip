@@ -491,9 +491,9 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     uint32_t kw = (cb.cap + 31) / 32;
     if (kw > 8)
         return ctx->fail(PQ_ERR_LIMIT, "extension keys longer than 256 codes");
-    // batch so the candidate buffer stays within ~2 GiB
+    // batch so the candidate buffer stays within ~8 GiB of the 288 GB
     uint64_t per_read = 4ull * cb.cap * (sizeof(uint2) + sizeof(uint16_t)) + 16;
-    uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(n, (2ull << 30) / per_read));
+    uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(n, (8ull << 30) / per_read));
     uint64_t *rec = nullptr, *recF = nullptr, *app = nullptr, *tot = nullptr, *staged = nullptr;
     uint32_t *ovl = nullptr, *misc = nullptr;
     uint64_t edge_cap = std::max<uint64_t>(1 << 20, n / 4);
