@@ -133,8 +133,18 @@ def main():
     per_launch_positions = positions / max(1, launches)
     alg_bytes = per_launch_positions * LINE_BYTES
     achieved = alg_bytes / (avg_launch_ms / 1e3) / 1e9
+    # HBM/fabric bytes per k_find launch from the committed counter runs of this
+    # same workload (profiles/config2_traffic.md): FETCH_SIZE and WRITE_SIZE,
+    # each scaled by what it reads for the plain copy kernel
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "config2_traffic.json")
+    if args.config == 2 and os.path.exists(tpath):
+        with open(tpath) as f:
+            tk = json.load(f)["kernels"].get("k_find")
+        if tk:
+            traffic = tk["traffic_bytes"][0]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_find", "alg_bytes_per_launch": int(alg_bytes),
                 "rank_positions_per_read": round(positions / (n * args.steps), 1),
                 "lines_per_read": round(lines / (n * args.steps), 1),
