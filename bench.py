@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--cpu-baseline", default="full", choices=["full", "off"])
+    ap.add_argument("--learned", type=int, default=1,
+                    help="also time the learned-index seeded search (1) or not (0)")
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="also run one plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -74,6 +76,7 @@ def main():
               "design notes)", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import pasqual_amd as pq
     cfg = pq.synth_preset(args.config)
     L, t = cfg.len_min, cfg.min_overlap
@@ -83,7 +86,6 @@ def main():
     pq.synth_fasta(cfg, fasta)
     imp = pq.import_fasta(fasta, L, index_bits=32)
     n = imp["n"]
-    import numpy as np
     off = np.arange(n + 1, dtype=np.uint64) * np.uint64(L)
     N = n * L
     log(f"workload config {args.config}: {n} reads after duplicate removal, N={N} per direction,"
@@ -116,12 +118,47 @@ def main():
         lines += tm["find_lines"]
     ctx.sync()
     total_s = time.perf_counter() - t_start
+    g = ctx.graph()
+    edges = int(g["edge_off"][-1])
+
+    # learned-index variant on the same index: train once on the built SA,
+    # then the same number of timed searches seeded by it
+    learned = None
+    if args.learned:
+        ctx.build_index()
+        ctx.learned_train()
+        ctx.find_overlaps(pq.SEARCH_LEARNED)
+        gl = ctx.graph()
+        same = all(np.array_equal(gl[k], g[k]) for k in g)
+        ctx.sync()
+        l_ms, l_seed, l_find, l_pos = [], [], [], 0
+        for _ in range(args.steps):
+            a = time.perf_counter()
+            ctx.find_overlaps(pq.SEARCH_LEARNED)
+            ctx.sync()
+            l_ms.append((time.perf_counter() - a) * 1e3)
+            tm = ctx.timing()
+            l_seed.append(tm["learned_ms"])
+            l_find.append(tm["find_kernel_ms"])
+            l_pos += tm["find_positions"]
+        st = ctx.learned_stats()
+        avg_l = sum(l_ms) / len(l_ms)
+        lookups = sum(st["lookups"])
+        learned = {"reads_per_s": round(n / (avg_l / 1e3), 1), "search_ms": round(avg_l, 3),
+                   "seed_kernel_ms": round(sum(l_seed) / len(l_seed), 3),
+                   "find_kernel_ms": round(sum(l_find) / len(l_find), 3),
+                   "rank_positions_per_read": round(l_pos / (n * args.steps), 1),
+                   "seed_depth": st["depth"], "model_k": st["k"],
+                   "mean_window": [round(x, 2) for x in st["mean_window"]],
+                   "max_window": list(st["max_window"]),
+                   "probes_per_lookup": round(sum(st["probes"]) / max(1, lookups), 2),
+                   "train_ms": round(st["train_ms"], 1),
+                   "train_loss": [float(f"{x:.3g}") for x in st["train_loss"]],
+                   "edges_identical_to_plain": bool(same)}
     if args.probe_copy:
         pms = ctx.probe_copy(args.probe_copy << 20)
         log(f"copy probe: {args.probe_copy} MiB in {pms:.3f} ms "
             f"({2 * (args.probe_copy << 20) / pms / 1e6:.1f} GB/s read+write)")
-    g = ctx.graph()
-    edges = int(g["edge_off"][-1])
     ms_per_step = total_s * 1e3 / args.steps
     avg_search = sum(search_ms) / len(search_ms)
     avg_build = sum(build_ms) / len(build_ms)
@@ -188,7 +225,10 @@ def main():
         "edges": edges,
         "roofline": roofline,
         "cpu_baseline": base,
+        "learned": learned,
     }
+    if learned:
+        line["learned_vs_plain_search"] = round(learned["reads_per_s"] / reads_per_s, 3)
     if base:
         line["speedup_vs_cpu_search"] = round(reads_per_s / base["value"], 1)
         line["speedup_vs_cpu_sa_build"] = round(gsuffix_per_s / base["sa_build_gsuffix_per_s"], 1)

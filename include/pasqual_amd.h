@@ -99,27 +99,59 @@ int pq_graph_copy(pq_ctx_t *ctx, int32_t *overlaps_B, int32_t *overlaps_F,
                   uint64_t *dest, uint16_t *len, uint8_t *type);
 
 /* ---- learned index --------------------------------------------------------------------- */
+/* Counterpart of the reference's interval shortcut (sufsort.c:179-189, used at
+ * string_graph.c:203-212, 289-298): every chain of the overlap search resumes at
+ * depth D = min(63, t) from intervals that a learned index finds.  Per direction
+ * the index holds the distinct D-code prefixes of the suffixes (each with its
+ * first suffix-array row and row count) in suffix-array order.  An FP32 MLP
+ * (k-code prefix -> hidden ReLU layer -> 1), fitted on the device to the table
+ * built from the freshly built suffix array, routes a key to one of `leaves`
+ * piecewise-linear leaves (`segments` pieces with exact end points) that predict
+ * its table entry; an exact galloping + binary search corrects the prediction.
+ * Results equal PQ_SEARCH_PLAIN's for any model.  The model is dropped by
+ * pq_upload_reads and pq_build_index. */
 typedef struct {
-    uint32_t k;              /* prefix length fed to the model (<= 31), 21 */
-    uint32_t hidden;         /* MLP hidden width, 64 */
+    uint32_t k;              /* prefix length fed to the model (<= 21; capped at D), 21 */
+    uint32_t hidden;         /* MLP hidden width (<= 64), 64 */
     uint32_t leaves;         /* piecewise-linear leaves, 4096 */
-    uint32_t epochs;         /* SGD passes over the sampled training set */
-    uint32_t batch;          /* minibatch size */
-    uint32_t samples;        /* training samples drawn from each SA */
-    float lr;                /* learning rate */
+    uint32_t segments;       /* linear pieces per leaf (<= 1024), 64 */
+    uint32_t epochs;         /* Adam passes over the sampled training set, 1 */
+    uint32_t batch;          /* minibatch size (multiple of 64), 4096 */
+    uint32_t samples;        /* training entries drawn from each table (multiple of 64) */
+    float lr;                /* Adam step size (decays linearly to 1%) */
+    uint32_t closed_form;    /* 1: solve the output layer by least squares before Adam */
     uint64_t seed;
 } pq_learned_cfg_t;
 
 typedef struct {
-    double mean_window[2];   /* mean |predicted row - true row| bound used, fw/bw */
-    uint64_t max_window[2];  /* largest correction window, fw/bw */
-    double train_loss[2];    /* final MSE of the MLP on normalised rows */
-    double train_ms;
+    double mean_window[2];   /* mean |exact entry - predicted entry| over the last learned
+                                search's lookups, per index (fw, bw) */
+    uint64_t max_window[2];  /* largest such correction window */
+    double train_loss[2];    /* mean squared error of the MLP on entry / table size */
+    double fit_loss[2];      /* the same before Adam (after the closed-form layer) */
+    double train_ms;         /* wall time of pq_learned_train (both directions) */
+    uint64_t lookups[2];     /* lookups in the last learned search, per index */
+    uint64_t probes[2];      /* key reads of those lookups */
+    uint64_t table_size[2];  /* distinct D-code prefixes per index */
+    uint64_t absent;         /* chains whose depth-D pattern does not occur */
+    uint64_t fallback;       /* chains left to the plain start (tables disagree) */
+    uint32_t depth;          /* D: depth the backward search resumes from */
+    uint32_t k;              /* model prefix length used */
 } pq_learned_stats_t;
 
 void pq_learned_default(pq_learned_cfg_t *cfg);
 int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg);
 int pq_learned_stats(pq_ctx_t *ctx, pq_learned_stats_t *st);
+/* Host copy of the trained parameters of direction dir (0 fw, 1 bw): W1[hidden][k],
+ * b1[hidden], w2[hidden], b2.  *count in: capacity, out: number of parameters
+ * (params may be NULL to query it). */
+int pq_learned_params(pq_ctx_t *ctx, int dir, float *params, uint32_t *count);
+/* Gradient of (1/n) sum (MLP(x_i) - y_i)^2 over n samples, by the device training
+ * kernels; x_i are k-code prefix values (first code most significant), params as
+ * above.  grad receives one float per parameter, *loss the mean squared error. */
+int pq_mlp_gradients(pq_ctx_t *ctx, uint32_t k, uint32_t hidden, const float *params,
+                     const uint64_t *xs, const float *targets, uint32_t n, float *grad,
+                     float *loss);
 
 /* ---- timing ---------------------------------------------------------------------------- */
 typedef struct {

@@ -892,3 +892,50 @@ int64_t orc_check_sa(const uint8_t *dtext, const uint64_t *off, uint64_t n,
     free(end);
     return bad;
 }
+
+/* ---- learned index model ------------------------------------------------------ */
+
+double orc_mlp_loss_grad(uint32_t k, uint32_t h, const double *params, const uint64_t *xs,
+                         const double *targets, uint64_t n, double *grad)
+{
+    const double *W1 = params, *b1 = params + (size_t) h * k, *w2 = b1 + h, *b2 = w2 + h;
+    const size_t P = (size_t) h * k + 2 * (size_t) h + 1;
+    double *f = malloc(sizeof(double) * k), *pre = malloc(sizeof(double) * h);
+    double loss = 0;
+    if (grad)
+        memset(grad, 0, sizeof(double) * P);
+    for (uint64_t s = 0; s < n; s++) {
+        for (uint32_t j = 0; j < k; j++) {
+            unsigned c = (unsigned) (xs[s] >> (2 * (k - 1 - j))) & 3u;
+            f[j] = (double) c * 2.0 / 3.0 - 1.0;
+        }
+        double out = b2[0];
+        for (uint32_t i = 0; i < h; i++) {
+            double a = b1[i];
+            for (uint32_t j = 0; j < k; j++)
+                a += W1[(size_t) i * k + j] * f[j];
+            pre[i] = a;
+            if (a > 0)
+                out += w2[i] * a;
+        }
+        double r = out - targets[s];
+        loss += r * r;
+        if (!grad)
+            continue;
+        double e = 2.0 * r / (double) n;   /* d loss / d out */
+        double *gW1 = grad, *gb1 = grad + (size_t) h * k, *gw2 = gb1 + h, *gb2 = gw2 + h;
+        gb2[0] += e;
+        for (uint32_t i = 0; i < h; i++) {
+            if (pre[i] <= 0)
+                continue;
+            gw2[i] += e * pre[i];
+            double dp = e * w2[i];
+            gb1[i] += dp;
+            for (uint32_t j = 0; j < k; j++)
+                gW1[(size_t) i * k + j] += dp * f[j];
+        }
+    }
+    free(f);
+    free(pre);
+    return loss / (double) n;
+}

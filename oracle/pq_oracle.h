@@ -111,6 +111,16 @@ void orc_graph_free(orc_graph_t *g);
 int64_t orc_check_sa(const uint8_t *dtext, const uint64_t *off, uint64_t n,
                      const uint64_t *sa);
 
+/* Learned index model (no counterpart in the reference): the FP32 MLP of the
+ * device learned index evaluated in double.  x_i is a k-code prefix value (first
+ * code most significant); code c enters as c * 2/3 - 1.  params = W1[h][k],
+ * b1[h], w2[h], b2; out = b2 + sum_i w2[i] relu(b1[i] + sum_j W1[i][j] f_j).
+ * Computes loss = (1/n) sum (out_i - y_i)^2 and its gradient with respect to
+ * every parameter (grad may be NULL).  Pinned in tests by central differences
+ * of the loss, which this function also gives. */
+double orc_mlp_loss_grad(uint32_t k, uint32_t h, const double *params, const uint64_t *xs,
+                         const double *targets, uint64_t n, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,8 @@ def lib():
         _lib.orc_graph_free.argtypes = [C.POINTER(OrcGraph)]
         _lib.orc_check_sa.argtypes = [P, P, C.c_uint64, P]
         _lib.orc_check_sa.restype = C.c_int64
+        _lib.orc_mlp_loss_grad.argtypes = [C.c_uint32, C.c_uint32, P, P, P, C.c_uint64, P]
+        _lib.orc_mlp_loss_grad.restype = C.c_double
     return _lib
 
 
@@ -266,3 +268,14 @@ def run_reference(fasta, L, t, workdir, width="u32", threads=None, extra=()):
         raise RuntimeError(f"reference failed: {r.stderr}")
     with open(os.path.join(workdir, "contigs.fa"), "rb") as f:
         return r.stdout, f.read()
+
+
+def mlp_loss_grad(k, h, params, xs, targets, want_grad=True):
+    """Loss and gradient of the learned-index MLP in double (orc_mlp_loss_grad)."""
+    params = np.ascontiguousarray(params, np.float64)
+    xs = np.ascontiguousarray(xs, np.uint64)
+    targets = np.ascontiguousarray(targets, np.float64)
+    grad = np.zeros(params.size, np.float64) if want_grad else None
+    loss = lib().orc_mlp_loss_grad(k, h, _p(params), _p(xs), _p(targets), xs.size,
+                                   _p(grad) if want_grad else None)
+    return loss, grad

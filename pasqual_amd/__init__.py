@@ -32,6 +32,7 @@ PQ_SYMBOLS = [
     "pq_get_sa", "pq_get_bwt", "pq_get_reads_index", "pq_get_counts", "pq_get_occ_ref",
     "pq_get_shortcut", "pq_num_reads", "pq_text_len", "pq_find_overlaps", "pq_graph_sizes",
     "pq_graph_copy", "pq_learned_default", "pq_learned_train", "pq_learned_stats",
+    "pq_learned_params", "pq_mlp_gradients",
     "pq_get_timing", "pq_sync", "pq_probe_copy", "pq_synth_preset", "pq_synth_num_reads", "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
 ]
@@ -71,13 +72,18 @@ class ImportResult(C.Structure):
 
 class LearnedCfg(C.Structure):
     _fields_ = [("k", C.c_uint32), ("hidden", C.c_uint32), ("leaves", C.c_uint32),
-                ("epochs", C.c_uint32), ("batch", C.c_uint32), ("samples", C.c_uint32),
-                ("lr", C.c_float), ("seed", C.c_uint64)]
+                ("segments", C.c_uint32), ("epochs", C.c_uint32), ("batch", C.c_uint32),
+                ("samples", C.c_uint32), ("lr", C.c_float), ("closed_form", C.c_uint32),
+                ("seed", C.c_uint64)]
 
 
 class LearnedStats(C.Structure):
     _fields_ = [("mean_window", C.c_double * 2), ("max_window", C.c_uint64 * 2),
-                ("train_loss", C.c_double * 2), ("train_ms", C.c_double)]
+                ("train_loss", C.c_double * 2), ("fit_loss", C.c_double * 2),
+                ("train_ms", C.c_double), ("lookups", C.c_uint64 * 2),
+                ("probes", C.c_uint64 * 2), ("table_size", C.c_uint64 * 2),
+                ("absent", C.c_uint64), ("fallback", C.c_uint64),
+                ("depth", C.c_uint32), ("k", C.c_uint32)]
 
 
 class Timing(C.Structure):
@@ -132,6 +138,8 @@ def lib(width="u32"):
         "pq_learned_default": (None, [P]),
         "pq_learned_train": (I32, [P, P]),
         "pq_learned_stats": (I32, [P, P]),
+        "pq_learned_params": (I32, [P, I32, P, P]),
+        "pq_mlp_gradients": (I32, [P, U32, U32, P, P, P, U32, P, P]),
         "pq_get_timing": (I32, [P, P]),
         "pq_sync": (I32, [P]),
         "pq_probe_copy": (I32, [P, U64, P]),
@@ -321,7 +329,29 @@ class Context:
         st = LearnedStats()
         self._check(self._L.pq_learned_stats(self._h, C.byref(st)), "learned_stats")
         return {"mean_window": list(st.mean_window), "max_window": list(st.max_window),
-                "train_loss": list(st.train_loss), "train_ms": st.train_ms}
+                "train_loss": list(st.train_loss), "fit_loss": list(st.fit_loss),
+                "train_ms": st.train_ms, "lookups": list(st.lookups), "probes": list(st.probes),
+                "table_size": list(st.table_size), "absent": st.absent,
+                "fallback": st.fallback, "depth": st.depth, "k": st.k}
+
+    def learned_params(self, d):
+        cnt = C.c_uint32(0)
+        self._check(self._L.pq_learned_params(self._h, d, None, C.byref(cnt)), "learned_params")
+        out = np.zeros(cnt.value, np.float32)
+        self._check(self._L.pq_learned_params(self._h, d, out.ctypes.data, C.byref(cnt)),
+                    "learned_params")
+        return out
+
+    def mlp_gradients(self, k, hidden, params, xs, targets):
+        params = np.ascontiguousarray(params, np.float32)
+        xs = np.ascontiguousarray(xs, np.uint64)
+        targets = np.ascontiguousarray(targets, np.float32)
+        grad = np.zeros(params.size, np.float32)
+        loss = C.c_float(0)
+        self._check(self._L.pq_mlp_gradients(self._h, k, hidden, params.ctypes.data,
+                                              xs.ctypes.data, targets.ctypes.data, xs.size,
+                                              grad.ctypes.data, C.byref(loss)), "mlp_gradients")
+        return grad, float(loss.value)
 
     def timing(self):
         t = Timing()

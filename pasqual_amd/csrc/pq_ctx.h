@@ -127,8 +127,14 @@ int pq_scan_u64(pq_ctx_t *ctx, const uint64_t *in, uint64_t *out, uint64_t count
 int pq_build_sa_dir(pq_ctx_t *ctx, int d);
 int pq_build_occ_dir(pq_ctx_t *ctx, int d);
 void pq_learned_free(pq_ctx_t *ctx);
-int pq_learned_seed(pq_ctx_t *ctx, int chain_dir, const uint64_t *d_kmers, uint64_t count,
-                    uint32_t *d_lo, uint32_t *d_hi);
+// Seed of one (read, chain) of the candidate search at the learned depth:
+// search interval [l, l + size) and partner interval start ll; size SEED_NONE
+// makes the chain start from depth 1 instead.
+struct Seed {
+    uint32_t l, size, ll;
+};
+#define SEED_NONE 0xFFFFFFFFu
+int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, Seed *d_seeds, uint32_t *kseed);
 
 static inline unsigned pq_blocks(uint64_t work, unsigned per)
 {

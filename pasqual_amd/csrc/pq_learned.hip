@@ -1,17 +1,673 @@
-// pq_learned.hip -- learned index over the suffix arrays (placeholder until the
-// model kernels land): training is refused, so learned search is unavailable.
+// pq_learned.hip -- learned index over the suffix arrays.
+//
+// Role: the reference starts two of its four backward searches from a
+// precomputed "interval shortcut" (sufsort.c:179-189, consumed at
+// string_graph.c:203-212, 289-298) and walks the rest from one code.  Here
+// every chain of the candidate search (k_find) can start at depth D = min(63, t)
+// from intervals found by a learned index instead of D-1 backward steps, each of
+// which loads one or two random 64-byte occurrence lines.  Candidates are only
+// recorded from depth t on, so no step below D records anything.
+//
+// Per direction:
+//   table   = the distinct D-code prefixes of the suffixes with at least D codes,
+//             in suffix-array order: key (128 bits, first code in the top bits,
+//             bit 0 set), first SA row and number of rows.  The SA rows of a
+//             D-code pattern P are exactly the rows of its table entry.
+//   model   = FP32 MLP on the k-code prefix of a key (k = 21 codes mapped to
+//             -1..1) -> 64 ReLU units -> 1 output, fitted to the entry's
+//             position / size of the table, which routes the key to one of 4096
+//             leaves.
+//   leaves  = each leaf is piecewise linear in the k-code prefix value x: over
+//             the x range of the training keys routed to it, S equal-width
+//             pieces whose end points are exact (x, first entry >= x) pairs.
+//   query   = predicted entry -> exact lower bound of the key in the table by
+//             galloping outward from the prediction until bracketed, then
+//             binary search; the entry gives the SA interval.  The prediction
+//             only narrows the search: the result is exact for any model,
+//             including an untrained one.
+//
+// Training, all on the device with the kernels below: rows sampled from the
+// table; the output layer first solved in closed form (least squares over the
+// hidden activations, Gram matrix accumulated by a device kernel, 65 x 65
+// system solved on the host), then every parameter refined by minibatch Adam
+// with the forward/backward kernels (block-ordered deterministic reductions).
 #include "pq_ctx.h"
 
-struct Seed;
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#define MLP_K_MAX 21
+#define MLP_H_MAX 64
+#define MLP_P_MAX (MLP_H_MAX * MLP_K_MAX + 2 * MLP_H_MAX + 1)
+#define TRAIN_BLOCK 64      // samples per block of the gradient kernel (one wave)
+#define SEG_MAX 1024
+#define REROUTE_MAX 64      // leaf corrections per query
+#define GRAM_SUB 128        // samples staged in LDS per round of the Gram kernel
+#define GRAM_CHUNK 1024     // samples per block of the Gram kernel
+#define GRAM_DIM (MLP_H_MAX + 2)   // activations, 1, target
+
+typedef unsigned __int128 u128;
+// constant address space: wave-uniform reads through it become scalar loads
+typedef __attribute__((address_space(4))) const float cfloat;
+
+struct MlpDims {
+    uint32_t k, h;
+};
+
+// parameter vector layout: W1[h][k], b1[h], w2[h], b2
+static __host__ __device__ inline uint32_t mlp_nparams(uint32_t k, uint32_t h)
+{
+    return h * k + h + h + 1;
+}
+
+// search statistics per index direction: lookups, key probes, sum and max of
+// |exact entry - predicted entry|; patterns absent from the table; chains left
+// to the plain start
+struct QStats {
+    unsigned long long lookups[2], probes[2], win_sum[2], win_max[2], absent, fallback;
+};
+
+struct DirModel {
+    u128 *key = nullptr;          // nU distinct keys
+    uint2 *rows = nullptr;        // (first SA row, rows) per key
+    uint64_t nU = 0;
+    float *params = nullptr;
+    float *iparams = nullptr;     // inference form (rows padded to MLP_K_MAX)
+    uint64_t *range = nullptr;    // [xmin, xmax] per leaf
+    uint32_t *knots = nullptr;    // (S + 1) entries per leaf
+    double loss = 0, fit_loss = 0;
+};
+
+struct Learned {
+    uint32_t k = 21, h = 64, leaves = 4096, segs = 64, depth = 21;
+    DirModel dir[2];
+    QStats *qstats = nullptr;     // device
+    double train_ms = 0;
+};
+
+// ---- forward pass ----------------------------------------------------------------------
+
+__device__ __forceinline__ float mlp_feature(uint64_t x, uint32_t k, uint32_t j)
+{
+    // code j of the k-code prefix x (first code most significant) mapped to -1..1
+    uint32_t c = (uint32_t) (x >> (2 * (k - 1 - j))) & 3u;
+    return (float) c * (2.0f / 3.0f) - 1.0f;
+}
+
+// Output of the MLP; p may live in LDS or global memory.  With act != nullptr the
+// ReLU activations are stored (stride act_stride) for the backward pass.
+__device__ __forceinline__ float mlp_forward(const float *p, MlpDims d, uint64_t x, float *act,
+                                             uint32_t act_stride)
+{
+    float f[MLP_K_MAX];
+#pragma unroll
+    for (uint32_t j = 0; j < MLP_K_MAX; j++)
+        f[j] = j < d.k ? mlp_feature(x, d.k, j) : 0.0f;
+    const float *W1 = p, *b1 = p + d.h * d.k, *w2 = b1 + d.h;
+    float out = w2[d.h];   // b2
+    for (uint32_t i = 0; i < d.h; i++) {
+        float s = b1[i];
+        const float *row = W1 + i * d.k;
+#pragma unroll
+        for (uint32_t j = 0; j < MLP_K_MAX; j++)
+            if (j < d.k)
+                s = fmaf(row[j], f[j], s);
+        float a = s > 0.0f ? s : 0.0f;
+        if (act)
+            act[i * act_stride] = a;
+        out = fmaf(w2[i], a, out);
+    }
+    return out;
+}
+
+// Inference form used by the search: W1 rows padded to MLP_K_MAX columns with
+// zero weights (features past k are 0 too), then b1, w2, b2.  Fixed trip counts
+// and wave-uniform parameters let every weight come from a scalar load.  Gives
+// the same output as mlp_forward on the unpadded parameters.
+__device__ __forceinline__ float mlp_forward_padded(const cfloat *p, MlpDims d, uint64_t x)
+{
+    float f[MLP_K_MAX];
+#pragma unroll
+    for (uint32_t j = 0; j < MLP_K_MAX; j++) {
+        uint32_t sh = j < d.k ? 2 * (d.k - 1 - j) : 0;
+        float v = (float) ((uint32_t) (x >> sh) & 3u) * (2.0f / 3.0f) - 1.0f;
+        f[j] = j < d.k ? v : 0.0f;
+    }
+    const cfloat *b1 = p + d.h * MLP_K_MAX, *w2 = b1 + d.h;
+    float out = w2[d.h];
+    for (uint32_t i = 0; i < d.h; i++) {
+        const cfloat *row = p + i * MLP_K_MAX;
+        float s = b1[i];
+#pragma unroll
+        for (uint32_t j = 0; j < MLP_K_MAX; j++)
+            s = fmaf(row[j], f[j], s);
+        float a = s > 0.0f ? s : 0.0f;
+        out = fmaf(w2[i], a, out);
+    }
+    return out;
+}
+
+// ---- training ----------------------------------------------------------------------
+
+// One wave per block, one sample per lane: forward, backward, and the block's
+// summed gradient of inv_batch * sum (out - target)^2, written to part[block][P];
+// the block's summed squared residual goes to loss_part[block].
+__global__ void __launch_bounds__(TRAIN_BLOCK) k_mlp_grad(const float *params, MlpDims d,
+                                                          const uint64_t *xs,
+                                                          const float *targets, uint32_t n,
+                                                          float inv_batch, float *part,
+                                                          float *loss_part)
+{
+    __shared__ float sp[MLP_P_MAX];
+    __shared__ float sf[TRAIN_BLOCK][MLP_K_MAX + 1];
+    __shared__ float sa[TRAIN_BLOCK][MLP_H_MAX + 1];   // ReLU activations
+    __shared__ float sd[TRAIN_BLOCK][MLP_H_MAX + 1];   // dLoss/dpre
+    __shared__ float se[TRAIN_BLOCK];                  // dLoss/dout
+    __shared__ float sl[TRAIN_BLOCK];                  // squared residual
+    const uint32_t P = mlp_nparams(d.k, d.h);
+    const uint32_t tx = threadIdx.x;
+    for (uint32_t i = tx; i < P; i += blockDim.x)
+        sp[i] = params[i];
+    __syncthreads();
+    const uint32_t s = blockIdx.x * TRAIN_BLOCK + tx;
+    const bool ok = s < n;
+    const uint64_t x = ok ? xs[s] : 0;
+    float out = mlp_forward(sp, d, x, &sa[tx][0], 1);
+    float r = ok ? out - targets[s] : 0.0f;
+    float e = 2.0f * r * inv_batch;
+    se[tx] = e;
+    sl[tx] = r * r;
+    for (uint32_t j = 0; j < d.k; j++)
+        sf[tx][j] = mlp_feature(x, d.k, j);
+    const float *w2 = sp + d.h * d.k + d.h;
+    for (uint32_t i = 0; i < d.h; i++)
+        sd[tx][i] = sa[tx][i] > 0.0f ? e * w2[i] : 0.0f;
+    __syncthreads();
+    // dW1[i][j] = sum_t dpre_i f_j; db1[i] = sum_t dpre_i; dw2[i] = sum_t e a_i; db2 = sum_t e
+    float *o = part + (uint64_t) blockIdx.x * P;
+    const uint32_t nw = d.h * d.k;
+    for (uint32_t q = tx; q < P; q += blockDim.x) {
+        float g = 0.0f;
+        if (q < nw) {
+            uint32_t i = q / d.k, j = q - i * d.k;
+            for (uint32_t t = 0; t < TRAIN_BLOCK; t++)
+                g = fmaf(sd[t][i], sf[t][j], g);
+        } else if (q < nw + d.h) {
+            uint32_t i = q - nw;
+            for (uint32_t t = 0; t < TRAIN_BLOCK; t++)
+                g += sd[t][i];
+        } else if (q < nw + 2 * d.h) {
+            uint32_t i = q - nw - d.h;
+            for (uint32_t t = 0; t < TRAIN_BLOCK; t++)
+                g = fmaf(se[t], sa[t][i], g);
+        } else {
+            for (uint32_t t = 0; t < TRAIN_BLOCK; t++)
+                g += se[t];
+        }
+        o[q] = g;
+    }
+    if (tx == 0) {
+        float acc = 0.0f;
+        for (uint32_t t = 0; t < TRAIN_BLOCK; t++)
+            acc += sl[t];
+        loss_part[blockIdx.x] = acc;
+    }
+}
+
+// Block partials summed in block order (deterministic).
+template <typename T>
+__global__ void k_sum_parts(const T *part, uint32_t nblocks, uint32_t P, T *sum)
+{
+    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P)
+        return;
+    T g = 0;
+    for (uint32_t b = 0; b < nblocks; b++)
+        g += part[(uint64_t) b * P + q];
+    sum[q] = g;
+}
+
+// Adam (beta1 0.9, beta2 0.999, eps 1e-8) on the block-summed gradient.
+__global__ void k_mlp_adam(float *params, float *m, float *v, const float *part, uint32_t nblocks,
+                           uint32_t P, float lr, float b1t, float b2t)
+{
+    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P)
+        return;
+    float g = 0.0f;
+    for (uint32_t b = 0; b < nblocks; b++)
+        g += part[(uint64_t) b * P + q];
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    float mq = beta1 * m[q] + (1.0f - beta1) * g;
+    float vq = beta2 * v[q] + (1.0f - beta2) * g * g;
+    m[q] = mq;
+    v[q] = vq;
+    float mh = mq / (1.0f - b1t), vh = vq / (1.0f - b2t);
+    params[q] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// Upper triangle (in double) of the Gram matrix of z = (a_0 .. a_{h-1}, 1, y)
+// over a block's GRAM_CHUNK samples, a = hidden activations of the forward pass:
+// the normal equations of the least-squares output layer.
+__global__ void __launch_bounds__(256) k_ls_gram(const float *params, MlpDims d,
+                                                 const uint64_t *xs, const float *targets,
+                                                 uint32_t n, double *part)
+{
+    __shared__ float z[GRAM_SUB][GRAM_DIM + 1];
+    __shared__ uint8_t pi[GRAM_DIM * (GRAM_DIM + 1) / 2], pj[GRAM_DIM * (GRAM_DIM + 1) / 2];
+    const uint32_t m = d.h + 2, ne = m * (m + 1) / 2;
+    if (threadIdx.x == 0) {
+        uint32_t e = 0;
+        for (uint32_t i = 0; i < m; i++)
+            for (uint32_t j = i; j < m; j++, e++) {
+                pi[e] = (uint8_t) i;
+                pj[e] = (uint8_t) j;
+            }
+    }
+    constexpr uint32_t per = (GRAM_DIM * (GRAM_DIM + 1) / 2 + 255) / 256;
+    double acc[per];
+    for (uint32_t q = 0; q < per; q++)
+        acc[q] = 0.0;
+    const uint64_t base = (uint64_t) blockIdx.x * GRAM_CHUNK;
+    for (uint32_t sub = 0; sub < GRAM_CHUNK; sub += GRAM_SUB) {
+        __syncthreads();
+        if (threadIdx.x < GRAM_SUB) {
+            uint64_t s = base + sub + threadIdx.x;
+            float *zr = &z[threadIdx.x][0];
+            if (s < n) {
+                mlp_forward(params, d, xs[s], zr, 1);
+                zr[d.h] = 1.0f;
+                zr[d.h + 1] = targets[s];
+            } else {
+                for (uint32_t i = 0; i < m; i++)
+                    zr[i] = 0.0f;
+            }
+        }
+        __syncthreads();
+        for (uint32_t q = 0; q < per; q++) {
+            uint32_t e = threadIdx.x + q * 256;
+            if (e < ne) {
+                uint32_t i = pi[e], j = pj[e];
+                double a = 0.0;
+                for (uint32_t t = 0; t < GRAM_SUB; t++)
+                    a += (double) z[t][i] * (double) z[t][j];
+                acc[q] += a;
+            }
+        }
+    }
+    for (uint32_t q = 0; q < per; q++) {
+        uint32_t e = threadIdx.x + q * 256;
+        if (e < ne)
+            part[(uint64_t) blockIdx.x * ne + e] = acc[q];
+    }
+}
+
+// ---- keys, table, samples, leaves ------------------------------------------------------
+
+// 64 codes from text position g, first code in the top bits.
+__device__ __forceinline__ u128 text_codes(const uint64_t *t2, uint64_t g)
+{
+    return ((u128) pq_window(t2, g) << 64) | (u128) pq_window(t2, g + 32);
+}
+
+// Key bits kept for D codes: the top 2D bits.
+__device__ __forceinline__ u128 top_mask(uint32_t D)
+{
+    return ~(u128) 0 << (128 - 2 * D);
+}
+
+// K[i] = first D codes of suffix SA[i] (codes past the end of its read read as 0)
+// | (suffix has at least D codes).  Non-decreasing along the SA.
+__global__ void k_learned_keys(Geom G, const uint64_t *t2, const uint32_t *sa, uint64_t N,
+                               uint32_t D, u128 *K)
+{
+    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    uint32_t g = sa[i];
+    uint64_t rem = pq_rem(G, g);
+    u128 w = text_codes(t2, g) & top_mask(D);
+    if (rem < D)
+        w &= ~(~(u128) 0 >> (2 * rem));
+    K[i] = w | (u128) (rem >= D ? 1u : 0u);
+}
+
+// First row of each run of equal full-length keys.
+__global__ void k_table_flags(const u128 *K, uint64_t N, uint32_t *flag)
+{
+    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    u128 k = K[i];
+    flag[i] = ((uint32_t) k & 1u) && (i == 0 || K[i - 1] != k) ? 1u : 0u;
+}
+
+__global__ void k_table_starts(const u128 *K, uint64_t N, const uint32_t *flag,
+                               const uint32_t *pos, u128 *key, uint2 *rows)
+{
+    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || !flag[i])
+        return;
+    key[pos[i]] = K[i];
+    rows[pos[i]] = make_uint2((uint32_t) i, 0u);
+}
+
+// Last row of a run: the run's entry is pos[i] + flag[i] - 1.
+__global__ void k_table_counts(const u128 *K, uint64_t N, const uint32_t *flag,
+                               const uint32_t *pos, uint2 *rows)
+{
+    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    u128 k = K[i];
+    if (!((uint32_t) k & 1u) || (i + 1 < N && K[i + 1] == k))
+        return;
+    uint32_t u = pos[i] + flag[i] - 1u;
+    rows[u].y = (uint32_t) (i + 1) - rows[u].x;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Training rows drawn uniformly from the table: the k-code prefix of the key and
+// entry / nU.
+__global__ void k_sample_rows(const u128 *key, uint64_t nU, uint32_t k, uint32_t n, uint64_t seed,
+                              uint64_t *xs, float *targets)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    uint64_t u = (uint64_t) (((u128) mix64(seed + 0x9E3779B97F4A7C15ull * (s + 1)) * nU) >> 64);
+    xs[s] = (uint64_t) (key[u] >> (128 - 2 * k));
+    targets[s] = (float) ((double) u / (double) nU);
+}
+
+__device__ __forceinline__ uint32_t leaf_of(float y, uint32_t leaves)
+{
+    float z = floorf(y * (float) leaves);
+    if (!(z >= 0.0f))
+        return 0;
+    return z >= (float) leaves ? leaves - 1 : (uint32_t) z;
+}
+
+__global__ void k_mlp_route(const float *params, MlpDims d, const uint64_t *xs, uint32_t n,
+                            uint32_t leaves, uint32_t *leaf)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    leaf[s] = leaf_of(mlp_forward(params, d, xs[s], nullptr, 0), leaves);
+}
+
+// First entry u with key[u] >= q (plain binary search over [0, n)).
+__device__ __forceinline__ uint64_t lower_bound_all(const u128 *key, uint64_t n, u128 q)
+{
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        if (key[mid] < q)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t knot_x(uint64_t xmin, uint64_t xmax, uint32_t S, uint32_t s)
+{
+    return xmin + ((xmax - xmin) * s) / S;
+}
+
+// Exact end points of every piece: the first table entry at or after the knot's
+// k-code prefix.
+__global__ void k_leaf_knots(const u128 *key, uint64_t nU, uint32_t k, const uint64_t *range,
+                             uint32_t leaves, uint32_t S, uint32_t *knots)
+{
+    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t) leaves * (S + 1))
+        return;
+    uint32_t j = (uint32_t) (i / (S + 1)), s = (uint32_t) (i % (S + 1));
+    uint64_t x = knot_x(range[2 * j], range[2 * j + 1], S, s);
+    knots[i] = (uint32_t) lower_bound_all(key, nU, (u128) x << (128 - 2 * k));
+}
+
+// ---- query ------------------------------------------------------------------------------
+
+struct ModelView {
+    const u128 *key;
+    const uint2 *rows;
+    uint64_t nU;
+    const float *iparams;
+    const uint64_t *range;
+    const uint32_t *knots;
+};
+
+// Parameters read by every lane of a wave at the same addresses: the pointer is
+// made wave-uniform and moved to the constant address space (scalar loads).
+__device__ __forceinline__ const cfloat *wave_params(const float *p)
+{
+    uint64_t v = (uint64_t) p;
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t) v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t) (v >> 32));
+    return (const cfloat *) (((uint64_t) hi << 32) | lo);
+}
+
+__device__ __forceinline__ uint64_t model_entry(const cfloat *params, const ModelView &M, MlpDims d,
+                                                uint32_t leaves, uint32_t S, uint64_t x)
+{
+    uint32_t j = leaf_of(mlp_forward_padded(params, d, x), leaves);
+    // the network routes approximately; step to the neighbouring leaf whose
+    // training range holds x (bounded)
+    for (int q = 0; q < REROUTE_MAX && j > 0 && x < M.range[2 * j]; q++)
+        j--;
+    for (int q = 0; q < REROUTE_MAX && j + 1 < leaves && x > M.range[2 * j + 1]; q++)
+        j++;
+    uint64_t xmin = M.range[2 * j], xmax = M.range[2 * j + 1];
+    const uint32_t *kn = M.knots + (uint64_t) j * (S + 1);
+    if (x <= xmin)
+        return kn[0];
+    if (x >= xmax)
+        return kn[S];
+    uint64_t span = xmax - xmin;
+    uint32_t s = (uint32_t) (((x - xmin) * S) / span);
+    s = s >= S ? S - 1 : s;
+    uint64_t xa = knot_x(xmin, xmax, S, s), xb = knot_x(xmin, xmax, S, s + 1);
+    uint32_t ra = kn[s], rb = kn[s + 1];
+    if (xb <= xa || x <= xa)
+        return ra;
+    float frac = (float) (x - xa) / (float) (xb - xa);
+    return ra + (uint64_t) (frac * (float) (rb - ra));
+}
+
+// First entry u >= 0 with key[u] >= q, searched outward from guess: gallop until
+// bracketed, then binary search.  Exact for any guess.
+__device__ __forceinline__ uint64_t lower_bound_from(const u128 *key, uint64_t n, u128 q,
+                                                     uint64_t guess, uint32_t &probes)
+{
+    if (n == 0)
+        return 0;
+    if (guess >= n)
+        guess = n - 1;
+    uint64_t lo, hi;   // answer in [lo, hi]; key[hi] >= q or hi == n
+    probes++;
+    if (key[guess] >= q) {
+        hi = guess;
+        lo = 0;
+        for (uint64_t step = 1; hi > 0; step <<= 1) {
+            uint64_t c = hi > step ? hi - step : 0;
+            probes++;
+            if (key[c] < q) {
+                lo = c + 1;
+                break;
+            }
+            hi = c;
+        }
+    } else {
+        lo = guess + 1;
+        hi = n;
+        for (uint64_t step = 1;; step <<= 1) {
+            uint64_t c = guess + step;
+            if (c >= n)
+                break;
+            probes++;
+            if (key[c] >= q) {
+                hi = c;
+                break;
+            }
+            lo = c + 1;
+        }
+    }
+    while (lo < hi) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        probes++;
+        if (key[mid] < q)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t rev_codes64(uint64_t x)
+{
+    // reverse the order of the 32 two-bit codes of x
+    x = ((x >> 2) & 0x3333333333333333ull) | ((x & 0x3333333333333333ull) << 2);
+    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
+    return __builtin_bswap64(x);
+}
+
+// The first D codes a chain has matched at depth D, for its search index and for
+// its partner index, left-aligned in 128 bits.  Chains as in k_find: FB
+// n[L-D..L) in fw, FF comp(n[L-D..L)) in bw, BF rev(n[0..D)) in bw, BB
+// rc(n[0..D)) in fw; the partner holds the reversed pattern.
+__device__ __forceinline__ void chain_patterns(const uint64_t *fwt, uint64_t s, uint32_t L, int ch,
+                                               uint32_t D, u128 &search, u128 &partner)
+{
+    const u128 mask = top_mask(D);
+    u128 fwd = text_codes(fwt, (ch < 2) ? s + L - D : s) & mask;
+    u128 r = ((u128) rev_codes64((uint64_t) fwd) << 64) | rev_codes64((uint64_t) (fwd >> 64));
+    u128 rev = r << (128 - 2 * D);
+    switch (ch) {
+    case 0: search = fwd; partner = rev; break;
+    case 1: search = fwd ^ mask; partner = rev ^ mask; break;
+    case 2: search = rev; partner = fwd; break;
+    default: search = rev ^ mask; partner = fwd ^ mask; break;
+    }
+}
+
+// Block of 256 threads = 64 reads; wave q handles chain q of those reads, so the
+// model of a wave is uniform.  Writes Seed{search interval, partner start} at
+// depth D for chains of reads longer than D (size 0 when the pattern does not
+// occur; SEED_NONE, the plain start, when the two tables disagree).
+__global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fwt, uint64_t r0,
+                                                      uint64_t nr, uint32_t D, MlpDims d,
+                                                      uint32_t leaves, uint32_t S, ModelView fw,
+                                                      ModelView bw, Seed *seeds, QStats *qs)
+{
+    const int ch = (int) (threadIdx.x >> 6);
+    const uint64_t rl = (uint64_t) blockIdx.x * 64 + (threadIdx.x & 63);
+    const bool fwsearch = (ch == 0 || ch == 3);
+    const int sdir = fwsearch ? 0 : 1;
+    uint32_t probes[2] = {0, 0}, looks[2] = {0, 0};
+    uint64_t win[2] = {0, 0};
+    uint32_t absent = 0, fallback = 0;
+    if (rl < nr) {
+        uint64_t r = r0 + rl;
+        uint64_t s = pq_read_start(G, r);
+        uint32_t L = pq_read_len(G, r);
+        Seed sd = {0, SEED_NONE, 0};
+        const ModelView &Sv = fwsearch ? fw : bw;
+        const ModelView &Pv = fwsearch ? bw : fw;
+        if (D < L && Sv.nU > 0 && Pv.nU > 0) {
+            u128 pat, par;
+            chain_patterns(fwt, s, L, ch, D, pat, par);
+            const uint32_t sh = 128 - 2 * d.k;
+            const u128 key = pat | 1u, pkey = par | 1u;
+            uint64_t g1 = model_entry(wave_params(Sv.iparams), Sv, d, leaves, S,
+                                      (uint64_t) (pat >> sh));
+            uint64_t u = lower_bound_from(Sv.key, Sv.nU, key, g1, probes[sdir]);
+            looks[sdir]++;
+            win[sdir] = u > g1 ? u - g1 : g1 - u;
+            if (u < Sv.nU && Sv.key[u] == key) {
+                uint2 iv = Sv.rows[u];
+                uint64_t g2 = model_entry(wave_params(Pv.iparams), Pv, d, leaves, S,
+                                          (uint64_t) (par >> sh));
+                uint64_t u2 = lower_bound_from(Pv.key, Pv.nU, pkey, g2, probes[1 - sdir]);
+                looks[1 - sdir]++;
+                win[1 - sdir] = u2 > g2 ? u2 - g2 : g2 - u2;
+                if (u2 < Pv.nU && Pv.key[u2] == pkey && Pv.rows[u2].y == iv.y) {
+                    sd.l = iv.x;
+                    sd.size = iv.y;
+                    sd.ll = Pv.rows[u2].x;
+                } else {
+                    fallback = 1;
+                }
+            } else {
+                sd.size = 0;   // no suffix starts with the pattern: the chain is empty
+                absent = 1;
+            }
+        }
+        seeds[rl * 4 + ch] = sd;
+    }
+    // block reduction of the statistics, one set of atomics per block
+    __shared__ unsigned long long red[4][10];
+    unsigned long long v[10] = {looks[0], looks[1], probes[0], probes[1], win[0],
+                                win[1],   win[0],   win[1],    absent,    fallback};
+    for (int o = 32; o > 0; o >>= 1)
+        for (int q = 0; q < 10; q++) {
+            unsigned long long w = __shfl_xor(v[q], o, 64);
+            v[q] = (q == 6 || q == 7) ? (w > v[q] ? w : v[q]) : v[q] + w;
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 10; q++)
+            red[ch][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x < 10) {
+        int q = threadIdx.x;
+        bool mx = (q == 6 || q == 7);
+        unsigned long long a = red[0][q];
+        for (int w = 1; w < 4; w++)
+            a = mx ? (red[w][q] > a ? red[w][q] : a) : a + red[w][q];
+        unsigned long long *dst = q < 2   ? &qs->lookups[q]
+                                  : q < 4 ? &qs->probes[q - 2]
+                                  : q < 6 ? &qs->win_sum[q - 4]
+                                  : q < 8 ? &qs->win_max[q - 6]
+                                  : q == 8 ? &qs->absent
+                                           : &qs->fallback;
+        if (a) {
+            if (mx)
+                atomicMax(dst, a);
+            else
+                atomicAdd(dst, a);
+        }
+    }
+}
+
+// ---- host driver -------------------------------------------------------------------------
 
 void pq_learned_free(pq_ctx_t *ctx)
 {
+    if (!ctx->learned)
+        return;
+    for (int d = 0; d < 2; d++) {
+        DirModel &M = ctx->learned->dir[d];
+        pq_free(M.key);
+        pq_free(M.rows);
+        pq_free(M.params);
+        pq_free(M.iparams);
+        pq_free(M.range);
+        pq_free(M.knots);
+    }
+    pq_free(ctx->learned->qstats);
+    delete ctx->learned;
     ctx->learned = nullptr;
-}
-
-int pq_learned_seeds(pq_ctx_t *ctx, uint64_t, uint64_t, Seed *, uint32_t *)
-{
-    return ctx->fail(PQ_ERR_STATE, "learned index not available");
 }
 
 extern "C" void pq_learned_default(pq_learned_cfg_t *cfg)
@@ -21,23 +677,442 @@ extern "C" void pq_learned_default(pq_learned_cfg_t *cfg)
     cfg->k = 21;
     cfg->hidden = 64;
     cfg->leaves = 4096;
-    cfg->epochs = 4;
+    cfg->segments = 64;
+    cfg->epochs = 1;
     cfg->batch = 4096;
-    cfg->samples = 1 << 20;
-    cfg->lr = 1e-3f;
+    cfg->samples = 1u << 20;
+    cfg->lr = 1e-5f;
+    cfg->closed_form = 1;
     cfg->seed = 12345;
 }
 
-extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *)
+static uint64_t host_mix(uint64_t z)
 {
-    if (!ctx)
-        return PQ_ERR_ARG;
-    return ctx->fail(PQ_ERR_STATE, "learned index not available");
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
 }
 
-extern "C" int pq_learned_stats(pq_ctx_t *ctx, pq_learned_stats_t *)
+static float host_uniform(uint64_t seed, uint64_t i)
+{
+    // uniform in [-1, 1)
+    return (float) (2.0 * (double) (host_mix(seed + 0x9E3779B97F4A7C15ull * (i + 1)) >> 11) *
+                        0x1.0p-53 -
+                    1.0);
+}
+
+// Solve (A + lambda I) w = b for symmetric positive semi-definite A (m x m, row
+// major) by Cholesky in double; lambda = 1e-10 * max diagonal keeps it definite.
+static bool solve_spd(std::vector<double> A, const std::vector<double> &b, uint32_t m,
+                      std::vector<double> &w)
+{
+    double dmax = 0;
+    for (uint32_t i = 0; i < m; i++)
+        dmax = std::max(dmax, A[(size_t) i * m + i]);
+    const double lam = 1e-10 * (dmax > 0 ? dmax : 1.0);
+    for (uint32_t i = 0; i < m; i++)
+        A[(size_t) i * m + i] += lam;
+    for (uint32_t j = 0; j < m; j++) {
+        double s = A[(size_t) j * m + j];
+        for (uint32_t q = 0; q < j; q++)
+            s -= A[(size_t) j * m + q] * A[(size_t) j * m + q];
+        if (!(s > 0))
+            return false;
+        double djj = std::sqrt(s);
+        A[(size_t) j * m + j] = djj;
+        for (uint32_t i = j + 1; i < m; i++) {
+            double t = A[(size_t) i * m + j];
+            for (uint32_t q = 0; q < j; q++)
+                t -= A[(size_t) i * m + q] * A[(size_t) j * m + q];
+            A[(size_t) i * m + j] = t / djj;
+        }
+    }
+    w.assign(m, 0.0);
+    for (uint32_t i = 0; i < m; i++) {   // L y = b
+        double t = b[i];
+        for (uint32_t q = 0; q < i; q++)
+            t -= A[(size_t) i * m + q] * w[q];
+        w[i] = t / A[(size_t) i * m + i];
+    }
+    for (int i = (int) m - 1; i >= 0; i--) {   // L^T w = y
+        double t = w[i];
+        for (uint32_t q = i + 1; q < m; q++)
+            t -= A[(size_t) q * m + i] * w[q];
+        w[i] = t / A[(size_t) i * m + i];
+    }
+    return true;
+}
+
+// Mean squared error of the current parameters over the n samples.
+static int sample_loss(pq_ctx_t *ctx, const float *params, MlpDims dims, const uint64_t *xs,
+                       const float *targets, uint32_t n, float *part, float *lpart, double *loss)
+{
+    const uint32_t nb = (n + TRAIN_BLOCK - 1) / TRAIN_BLOCK;
+    hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(TRAIN_BLOCK), 0, ctx->stream, params, dims, xs,
+                       targets, n, 1.0f / n, part, lpart);
+    std::vector<float> hl(nb);
+    PQ_CHECK(hipGetLastError());
+    PQ_CHECK(hipMemcpyAsync(hl.data(), lpart, (size_t) nb * 4, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    double s = 0;
+    for (float v : hl)
+        s += v;
+    *loss = s / n;
+    return PQ_OK;
+}
+
+// Inference copy of the parameters with W1 rows padded to MLP_K_MAX columns.
+static int upload_inference_params(pq_ctx_t *ctx, const Learned &LM, DirModel &M)
+{
+    const uint32_t P = mlp_nparams(LM.k, LM.h);
+    std::vector<float> p(P), q(MLP_P_MAX, 0.0f);
+    PQ_CHECK(hipMemcpyAsync(p.data(), M.params, P * 4, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    for (uint32_t i = 0; i < LM.h; i++)
+        for (uint32_t j = 0; j < LM.k; j++)
+            q[i * MLP_K_MAX + j] = p[i * LM.k + j];
+    for (uint32_t i = 0; i < 2 * LM.h + 1; i++)
+        q[LM.h * MLP_K_MAX + i] = p[LM.h * LM.k + i];
+    PQ_CHECK(hipMemcpyAsync(M.iparams, q.data(), MLP_P_MAX * 4, hipMemcpyHostToDevice,
+                            ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    return PQ_OK;
+}
+
+// Table of distinct full-length D-code prefixes of direction d, from its SA.
+static int build_table(pq_ctx_t *ctx, Learned &LM, int d)
+{
+    DirModel &M = LM.dir[d];
+    const uint64_t N = ctx->N;
+    u128 *K = (u128 *) ctx->tmp(SLOT_LEARNED + 3, N * sizeof(u128) + 16);
+    uint32_t *flag = (uint32_t *) ctx->tmp(SLOT_LEARNED + 4, N * 4 + 16);
+    uint32_t *pos = (uint32_t *) ctx->tmp(SLOT_LEARNED + 5, N * 4 + 16);
+    if (!K || !flag || !pos)
+        return ctx->fail(PQ_ERR_NOMEM, "learned index table scratch");
+    Geom G = ctx->geom();
+    hipLaunchKernelGGL(k_learned_keys, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, G,
+                       ctx->dir[d].text2, ctx->dir[d].sa, N, LM.depth, K);
+    hipLaunchKernelGGL(k_table_flags, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, K, N,
+                       flag);
+    PQ_CHECK(hipGetLastError());
+    uint32_t nU = 0;
+    int rc = pq_scan_u32(ctx, flag, pos, N, &nU);
+    if (rc)
+        return rc;
+    M.nU = nU;
+    if ((rc = pq_alloc(ctx, (void **) &M.key, ((size_t) nU + 1) * sizeof(u128))) ||
+        (rc = pq_alloc(ctx, (void **) &M.rows, ((size_t) nU + 1) * sizeof(uint2))))
+        return rc;
+    hipLaunchKernelGGL(k_table_starts, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, K, N,
+                       flag, pos, M.key, M.rows);
+    hipLaunchKernelGGL(k_table_counts, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, K, N,
+                       flag, pos, M.rows);
+    PQ_CHECK(hipGetLastError());
+    return PQ_OK;
+}
+
+static int train_dir(pq_ctx_t *ctx, Learned &LM, int d, const pq_learned_cfg_t &cfg)
+{
+    DirModel &M = LM.dir[d];
+    const MlpDims dims = {LM.k, LM.h};
+    const uint32_t P = mlp_nparams(LM.k, LM.h);
+    const uint32_t S = cfg.samples;
+    const uint32_t NL = LM.leaves, SG = LM.segs;
+    int rc = build_table(ctx, LM, d);
+    if (rc)
+        return rc;
+    if ((rc = pq_alloc(ctx, (void **) &M.params, P * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &M.iparams, MLP_P_MAX * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &M.range, (size_t) NL * 16)) ||
+        (rc = pq_alloc(ctx, (void **) &M.knots, (size_t) NL * (SG + 1) * 4)))
+        return rc;
+    // seeded initial parameters: W1 ~ U(-1/sqrt(k), 1/sqrt(k)); b1 = 5 for the
+    // first half of the units (always active: affine features), 0 for the rest
+    // (ReLU features); w2 ~ U(-1/sqrt(h), 1/sqrt(h)); b2 = 0.5
+    std::vector<float> hp(P, 0.0f);
+    const float s1 = 1.0f / std::sqrt((float) LM.k), s2 = 1.0f / std::sqrt((float) LM.h);
+    const uint64_t ps = cfg.seed * 2 + d;
+    for (uint32_t i = 0; i < LM.h * LM.k; i++)
+        hp[i] = s1 * host_uniform(ps, i);
+    for (uint32_t i = 0; i < LM.h; i++) {
+        hp[LM.h * LM.k + i] = i < LM.h / 2 ? 5.0f : 0.0f;
+        hp[LM.h * LM.k + LM.h + i] = s2 * host_uniform(ps, LM.h * LM.k + i);
+    }
+    hp[P - 1] = 0.5f;
+    PQ_CHECK(hipMemcpyAsync(M.params, hp.data(), P * 4, hipMemcpyHostToDevice, ctx->stream));
+    if (M.nU == 0) {
+        if ((rc = upload_inference_params(ctx, LM, M)))
+            return rc;
+        // no suffix reaches the seed depth: every chain keeps the plain start
+        PQ_CHECK(hipMemsetAsync(M.range, 0, (size_t) NL * 16, ctx->stream));
+        PQ_CHECK(hipMemsetAsync(M.knots, 0, (size_t) NL * (SG + 1) * 4, ctx->stream));
+        return PQ_OK;
+    }
+    const uint32_t nb = cfg.batch / TRAIN_BLOCK, nball = S / TRAIN_BLOCK;
+    const uint32_t ngram = (S + GRAM_CHUNK - 1) / GRAM_CHUNK, m = LM.h + 2, ne = m * (m + 1) / 2;
+    size_t need = (size_t) S * 8 + (size_t) S * 4 * 2 + (size_t) P * 4 * 2 +
+                  (size_t) nball * P * 4 + (size_t) nball * 4 + (size_t) ngram * ne * 8 +
+                  (size_t) ne * 8 + 4096;
+    char *w = (char *) ctx->tmp(SLOT_LEARNED, need);
+    if (!w)
+        return ctx->fail(PQ_ERR_NOMEM, "learned index scratch");
+    uint64_t *xs = (uint64_t *) w;
+    float *targets = (float *) (xs + S);
+    uint32_t *leaf = (uint32_t *) (targets + S);
+    float *mom = (float *) (leaf + S);
+    float *vel = mom + P;
+    float *part = vel + P;
+    float *lpart = part + (size_t) nball * P;
+    double *gpart = (double *) (((uintptr_t) (lpart + nball) + 15) & ~(uintptr_t) 15);
+    double *gram = gpart + (size_t) ngram * ne;
+    hipLaunchKernelGGL(k_sample_rows, dim3(pq_blocks(S, 256)), dim3(256), 0, ctx->stream, M.key,
+                       M.nU, LM.k, S, cfg.seed ^ (0xD1B54A32D192ED03ull * (d + 1)), xs, targets);
+    PQ_CHECK(hipGetLastError());
+    if (cfg.closed_form) {
+        // output layer: least squares over the hidden activations of all samples
+        hipLaunchKernelGGL(k_ls_gram, dim3(ngram), dim3(256), 0, ctx->stream, M.params, dims, xs,
+                           targets, S, gpart);
+        hipLaunchKernelGGL(k_sum_parts<double>, dim3(pq_blocks(ne, 256)), dim3(256), 0,
+                           ctx->stream, gpart, ngram, ne, gram);
+        PQ_CHECK(hipGetLastError());
+        std::vector<double> hg(ne);
+        PQ_CHECK(hipMemcpyAsync(hg.data(), gram, ne * 8, hipMemcpyDeviceToHost, ctx->stream));
+        PQ_CHECK(hipStreamSynchronize(ctx->stream));
+        const uint32_t q = LM.h + 1;   // unknowns: w2[0..h), b2
+        auto g = [&](uint32_t i, uint32_t j) {
+            if (i > j)
+                std::swap(i, j);
+            return hg[(size_t) i * m - (size_t) i * (i - 1) / 2 + (j - i)];   // row i of upper tri
+        };
+        std::vector<double> A((size_t) q * q), b(q), sol;
+        for (uint32_t i = 0; i < q; i++) {
+            for (uint32_t j = 0; j < q; j++)
+                A[(size_t) i * q + j] = g(i, j);
+            b[i] = g(i, m - 1);
+        }
+        if (solve_spd(A, b, q, sol)) {
+            for (uint32_t i = 0; i < LM.h; i++)
+                hp[LM.h * LM.k + LM.h + i] = (float) sol[i];
+            hp[P - 1] = (float) sol[LM.h];
+            PQ_CHECK(hipMemcpyAsync(M.params, hp.data(), P * 4, hipMemcpyHostToDevice,
+                                    ctx->stream));
+        }
+    }
+    if ((rc = sample_loss(ctx, M.params, dims, xs, targets, S, part, lpart, &M.fit_loss)))
+        return rc;
+    // minibatch Adam over the samples in a fixed order, step size decaying
+    // linearly to 1% over the run
+    PQ_CHECK(hipMemsetAsync(mom, 0, (size_t) P * 8, ctx->stream));
+    const uint32_t steps = S / cfg.batch, total = steps * cfg.epochs;
+    double b1t = 1.0, b2t = 1.0;
+    for (uint32_t t = 0; t < total; t++) {
+        uint32_t b = t % steps;
+        b1t *= 0.9;
+        b2t *= 0.999;
+        float lr = (float) (cfg.lr * (1.0 - 0.99 * (double) t / (double) total));
+        hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(TRAIN_BLOCK), 0, ctx->stream, M.params, dims,
+                           xs + (size_t) b * cfg.batch, targets + (size_t) b * cfg.batch,
+                           cfg.batch, 1.0f / cfg.batch, part, lpart);
+        hipLaunchKernelGGL(k_mlp_adam, dim3(pq_blocks(P, 256)), dim3(256), 0, ctx->stream,
+                           M.params, mom, vel, part, nb, P, lr, (float) b1t, (float) b2t);
+    }
+    PQ_CHECK(hipGetLastError());
+    if ((rc = sample_loss(ctx, M.params, dims, xs, targets, S, part, lpart, &M.loss)))
+        return rc;
+    if ((rc = upload_inference_params(ctx, LM, M)))
+        return rc;
+    // leaves: route every sample, x range per leaf
+    hipLaunchKernelGGL(k_mlp_route, dim3(pq_blocks(S, 256)), dim3(256), 0, ctx->stream, M.params,
+                       dims, xs, S, NL, leaf);
+    PQ_CHECK(hipGetLastError());
+    std::vector<uint64_t> hx(S);
+    std::vector<uint32_t> hleaf(S);
+    PQ_CHECK(hipMemcpyAsync(hx.data(), xs, (size_t) S * 8, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipMemcpyAsync(hleaf.data(), leaf, (size_t) S * 4, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    std::vector<uint64_t> range(2 * (size_t) NL);
+    std::vector<uint8_t> seen(NL, 0);
+    for (uint32_t i = 0; i < S; i++) {
+        uint32_t j = hleaf[i];
+        if (!seen[j]) {
+            seen[j] = 1;
+            range[2 * j] = range[2 * j + 1] = hx[i];
+        } else {
+            range[2 * j] = std::min(range[2 * j], hx[i]);
+            range[2 * j + 1] = std::max(range[2 * j + 1], hx[i]);
+        }
+    }
+    // a leaf no sample reached keeps the single point where the previous one ends
+    uint64_t prev = 0;
+    for (uint32_t j = 0; j < NL; j++) {
+        if (!seen[j])
+            range[2 * j] = range[2 * j + 1] = prev;
+        prev = range[2 * j + 1];
+    }
+    PQ_CHECK(hipMemcpyAsync(M.range, range.data(), range.size() * 8, hipMemcpyHostToDevice,
+                            ctx->stream));
+    hipLaunchKernelGGL(k_leaf_knots, dim3(pq_blocks((uint64_t) NL * (SG + 1), 256)), dim3(256), 0,
+                       ctx->stream, M.key, M.nU, LM.k, M.range, NL, SG, M.knots);
+    PQ_CHECK(hipGetLastError());
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    return PQ_OK;
+}
+
+extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
 {
     if (!ctx)
         return PQ_ERR_ARG;
-    return ctx->fail(PQ_ERR_STATE, "learned index not trained");
+    if (!ctx->dir[0].built || !ctx->dir[1].built)
+        return ctx->fail(PQ_ERR_STATE, "index not built");
+    pq_learned_cfg_t cfg;
+    pq_learned_default(&cfg);
+    if (cfg_in)
+        cfg = *cfg_in;
+    if (cfg.k == 0 || cfg.k > MLP_K_MAX || cfg.hidden == 0 || cfg.hidden > MLP_H_MAX ||
+        cfg.leaves == 0 || cfg.leaves > (1u << 20) || cfg.segments == 0 ||
+        cfg.segments > SEG_MAX || cfg.batch == 0 || cfg.batch % TRAIN_BLOCK ||
+        cfg.samples < cfg.batch || cfg.samples % TRAIN_BLOCK)
+        return ctx->fail(PQ_ERR_ARG, "learned index configuration out of range");
+    if (ctx->N == 0 || ctx->t < 2)
+        return ctx->fail(PQ_ERR_STATE, "no suffixes to learn (empty text or t < 2)");
+    PQ_CHECK(hipSetDevice(ctx->device));
+    auto t0 = std::chrono::steady_clock::now();
+    pq_learned_free(ctx);
+    Learned *LM = new Learned();
+    // seed depth: up to 63 codes (one 128-bit key) but never past the minimum
+    // overlap, since candidates are recorded from depth t on; the model sees the
+    // first k <= D codes
+    LM->depth = std::min<uint32_t>(63, ctx->t);
+    LM->k = std::min<uint32_t>(cfg.k, LM->depth);
+    LM->h = cfg.hidden;
+    LM->leaves = cfg.leaves;
+    LM->segs = cfg.segments;
+    ctx->learned = LM;
+    int rc = pq_alloc(ctx, (void **) &LM->qstats, sizeof(QStats));
+    for (int d = 0; d < 2 && rc == PQ_OK; d++)
+        rc = train_dir(ctx, *LM, d, cfg);
+    if (rc == PQ_OK && hipMemsetAsync(LM->qstats, 0, sizeof(QStats), ctx->stream) != hipSuccess)
+        rc = ctx->fail(PQ_ERR_HIP, "clearing learned statistics");
+    if (rc == PQ_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
+        rc = ctx->fail(PQ_ERR_HIP, "learned index training");
+    if (rc) {
+        pq_learned_free(ctx);
+        return rc;
+    }
+    LM->train_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return PQ_OK;
+}
+
+extern "C" int pq_learned_stats(pq_ctx_t *ctx, pq_learned_stats_t *st)
+{
+    if (!ctx || !st)
+        return PQ_ERR_ARG;
+    Learned *LM = ctx->learned;
+    if (!LM)
+        return ctx->fail(PQ_ERR_STATE, "learned index not trained");
+    QStats q;
+    PQ_CHECK(hipMemcpyAsync(&q, LM->qstats, sizeof(q), hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    memset(st, 0, sizeof(*st));
+    for (int d = 0; d < 2; d++) {
+        st->lookups[d] = q.lookups[d];
+        st->probes[d] = q.probes[d];
+        st->mean_window[d] = q.lookups[d] ? (double) q.win_sum[d] / (double) q.lookups[d] : 0.0;
+        st->max_window[d] = q.win_max[d];
+        st->train_loss[d] = LM->dir[d].loss;
+        st->fit_loss[d] = LM->dir[d].fit_loss;
+        st->table_size[d] = LM->dir[d].nU;
+    }
+    st->absent = q.absent;
+    st->fallback = q.fallback;
+    st->train_ms = LM->train_ms;
+    st->depth = LM->depth;
+    st->k = LM->k;
+    return PQ_OK;
+}
+
+// Launch the seed kernel for reads [r0, r0 + nr) of a search batch (stream
+// ordered, no synchronisation).  Statistics accumulate over one search.
+int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, Seed *d_seeds, uint32_t *kseed)
+{
+    Learned *LM = ctx->learned;
+    if (!LM)
+        return ctx->fail(PQ_ERR_STATE, "learned index not trained");
+    if (r0 == 0)
+        PQ_CHECK(hipMemsetAsync(LM->qstats, 0, sizeof(QStats), ctx->stream));
+    ModelView v[2];
+    for (int d = 0; d < 2; d++) {
+        const DirModel &M = LM->dir[d];
+        v[d] = {M.key, M.rows, M.nU, M.iparams, M.range, M.knots};
+    }
+    MlpDims dims = {LM->k, LM->h};
+    hipLaunchKernelGGL(k_learned_seed, dim3(pq_blocks(nr, 64)), dim3(256), 0, ctx->stream,
+                       ctx->geom(), ctx->dir[0].text2, r0, nr, LM->depth, dims, LM->leaves,
+                       LM->segs, v[0], v[1], d_seeds, LM->qstats);
+    PQ_CHECK(hipGetLastError());
+    *kseed = LM->depth;
+    return PQ_OK;
+}
+
+// Gradient of (1/n) sum (MLP(x) - y)^2 and the loss, by the training kernels, for
+// n host samples given as k-code prefix values (checked against the CPU oracle).
+extern "C" int pq_mlp_gradients(pq_ctx_t *ctx, uint32_t k, uint32_t hidden, const float *params,
+                                const uint64_t *xs, const float *targets, uint32_t n, float *grad,
+                                float *loss)
+{
+    if (!ctx || !params || !xs || !targets || !grad || n == 0 || k == 0 || k > MLP_K_MAX ||
+        hidden == 0 || hidden > MLP_H_MAX)
+        return PQ_ERR_ARG;
+    PQ_CHECK(hipSetDevice(ctx->device));
+    const uint32_t P = mlp_nparams(k, hidden);
+    const uint32_t nb = (n + TRAIN_BLOCK - 1) / TRAIN_BLOCK;
+    size_t need = (size_t) n * 8 + (size_t) P * 4 * 2 + (size_t) n * 4 + (size_t) nb * P * 4 +
+                  (size_t) nb * 4 + 1024;
+    char *w = (char *) ctx->tmp(SLOT_LEARNED + 1, need);
+    if (!w)
+        return ctx->fail(PQ_ERR_NOMEM, "gradient scratch");
+    uint64_t *dx = (uint64_t *) w;
+    float *dp = (float *) (dx + n), *dg = dp + P, *dt = dg + P;
+    float *part = dt + n, *lpart = part + (size_t) nb * P;
+    PQ_CHECK(hipMemcpyAsync(dp, params, P * 4, hipMemcpyHostToDevice, ctx->stream));
+    PQ_CHECK(hipMemcpyAsync(dx, xs, (size_t) n * 8, hipMemcpyHostToDevice, ctx->stream));
+    PQ_CHECK(hipMemcpyAsync(dt, targets, (size_t) n * 4, hipMemcpyHostToDevice, ctx->stream));
+    MlpDims dims = {k, hidden};
+    hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(TRAIN_BLOCK), 0, ctx->stream, dp, dims, dx, dt, n,
+                       1.0f / n, part, lpart);
+    hipLaunchKernelGGL(k_sum_parts<float>, dim3(pq_blocks(P, 256)), dim3(256), 0, ctx->stream,
+                       part, nb, P, dg);
+    PQ_CHECK(hipGetLastError());
+    std::vector<float> lp(nb);
+    PQ_CHECK(hipMemcpyAsync(grad, dg, P * 4, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipMemcpyAsync(lp.data(), lpart, nb * 4, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    if (loss) {
+        double s = 0;
+        for (float v : lp)
+            s += v;
+        *loss = (float) (s / n);
+    }
+    return PQ_OK;
+}
+
+// Parameters of the trained model of direction dir (0 fw, 1 bw), host copy.
+extern "C" int pq_learned_params(pq_ctx_t *ctx, int dir, float *params, uint32_t *count)
+{
+    if (!ctx || !count || dir < 0 || dir > 1)
+        return PQ_ERR_ARG;
+    Learned *LM = ctx->learned;
+    if (!LM)
+        return ctx->fail(PQ_ERR_STATE, "learned index not trained");
+    uint32_t P = mlp_nparams(LM->k, LM->h);
+    if (params) {
+        if (*count < P)
+            return ctx->fail(PQ_ERR_ARG, "parameter buffer too small");
+        PQ_CHECK(hipMemcpyAsync(params, LM->dir[dir].params, P * 4, hipMemcpyDeviceToHost,
+                                ctx->stream));
+        PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    *count = P;
+    return PQ_OK;
 }

@@ -764,6 +764,7 @@ extern "C" int pq_upload_reads(pq_ctx_t *ctx, const uint8_t *codes, const uint64
         return PQ_ERR_ARG;
     auto t0 = std::chrono::steady_clock::now();
     pq_index_release(ctx);
+    pq_learned_free(ctx);
     ctx->searched = false;
     ctx->n = n;
     ctx->N = off[n];
@@ -792,7 +793,9 @@ extern "C" int pq_upload_reads(pq_ctx_t *ctx, const uint8_t *codes, const uint64
     ctx->Lmax = lmax;
     ctx->L = (n && lmin == lmax) ? lmin : 0;
     const uint64_t N = ctx->N;
-    uint64_t nwords64 = N / 64 + 2, nwords32 = N / 32 + 2;
+    // two words of slack past the last code: a 64-code window (two 32-code
+    // reads) may start at any text position
+    uint64_t nwords64 = N / 64 + 2, nwords32 = N / 32 + 4;
     uint8_t *d_codes = nullptr;
     uint32_t *pc = nullptr;
     int rc;

@@ -39,10 +39,6 @@ struct ChainBuf {
     uint32_t cap;            // entries per chain
 };
 
-struct Seed {
-    uint32_t l, size, ll;    // search interval and partner start at depth k
-};
-
 __device__ __forceinline__ unsigned chain_sym(const uint64_t *fwt, uint64_t s, uint32_t L, int ch,
                                               uint32_t o)
 {
@@ -74,8 +70,10 @@ __global__ void __launch_bounds__(256) k_find(Geom G, DirView fw, DirView bw, ui
     // C[0..5] = 0, 0 (A), C[2], C[3], C[4], C[5] (index.c:367-372)
     const uint32_t Cc[6] = {0u, 0u, Clo.y, Clo.z, Clo.w, C5};
     uint32_t o, l, size, ll;
-    if (seeds && kseed > 0 && kseed < L) {
-        Seed sd = seeds[rl * 4 + ch];
+    Seed sd = {0, SEED_NONE, 0};
+    if (seeds && kseed > 0 && kseed < L)
+        sd = seeds[rl * 4 + ch];
+    if (sd.size != SEED_NONE) {
         o = kseed;
         l = sd.l;
         size = sd.size;
@@ -451,8 +449,6 @@ static double ev_ms(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
-int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, Seed *d_seeds, uint32_t *kseed);
-
 extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
 {
     if (!ctx)
@@ -499,11 +495,13 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     uint64_t edge_cap = std::max<uint64_t>(1 << 20, n / 4);
     Seed *seeds = nullptr;
     char *gscratch = nullptr;
-    hipEvent_t e0, e1, e2;
+    hipEvent_t es, e0, e1, e2;
+    (void) hipEventCreate(&es);
     (void) hipEventCreate(&e0);
     (void) hipEventCreate(&e1);
     (void) hipEventCreate(&e2);
     auto cleanup = [&]() {
+        (void) hipEventDestroy(es);
         (void) hipEventDestroy(e0);
         (void) hipEventDestroy(e1);
         (void) hipEventDestroy(e2);
@@ -549,15 +547,9 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     for (uint64_t r0 = 0; r0 < n && rc == PQ_OK; r0 += batch) {
         uint64_t nr = std::min(batch, n - r0);
         uint32_t kseed = 0;
-        if (mode == PQ_SEARCH_LEARNED) {
-            auto tl0 = std::chrono::steady_clock::now();
-            if ((rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed)))
-                break;
-            PQ_CHECK(hipStreamSynchronize(ctx->stream));
-            ctx->timing.learned_ms += std::chrono::duration<double, std::milli>(
-                                          std::chrono::steady_clock::now() - tl0)
-                                          .count();
-        }
+        (void) hipEventRecord(es, ctx->stream);
+        if (mode == PQ_SEARCH_LEARNED && (rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed)))
+            break;
         (void) hipEventRecord(e0, ctx->stream);
         hipLaunchKernelGGL(k_find, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0, ctx->stream, G, fw,
                            bw, r0, nr, t, Clo, C5, seeds, kseed, cb, fstats);
@@ -581,6 +573,8 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         }
         (void) hipEventRecord(e2, ctx->stream);
         PQ_CHECK(hipEventSynchronize(e2));
+        if (mode == PQ_SEARCH_LEARNED)
+            ctx->timing.learned_ms += ev_ms(es, e0);
         double fms = ev_ms(e0, e1);
         ctx->timing.find_ms += fms;
         ctx->timing.find_kernel_ms += fms;

@@ -217,3 +217,31 @@ def test_against_live_reference(tmp_path, seed):
         a, b = int(fm["edge_off"][i]), int(fm["edge_off"][i + 1])
         E = np.stack([fm["dest"][a:b], fm["len"][a:b], fm["type"][a:b]], 1).astype(np.uint64)
         assert np.array_equal(E, nd["edges"]), i
+
+
+@pytest.mark.parametrize("k,h,seed", [(21, 64, 1), (5, 7, 2), (1, 1, 3)])
+def test_mlp_gradient_oracle_central_differences(k, h, seed):
+    # the learned-index MLP oracle: its analytic gradient against central
+    # differences of its own loss (ReLU kinks avoided: pre-activations kept away
+    # from zero by the draw below)
+    rng = np.random.default_rng(seed)
+    P = h * k + 2 * h + 1
+    params = rng.uniform(-1, 1, P)
+    xs = rng.integers(0, 1 << (2 * k), 48, dtype=np.uint64)
+    ys = rng.uniform(0, 1, 48)
+    loss, g = orc.mlp_loss_grad(k, h, params, xs, ys)
+    eps = 1e-6
+    num = np.zeros(P)
+    for q in range(P):
+        a = params.copy()
+        a[q] += eps
+        b = params.copy()
+        b[q] -= eps
+        num[q] = (orc.mlp_loss_grad(k, h, a, xs, ys, False)[0] -
+                  orc.mlp_loss_grad(k, h, b, xs, ys, False)[0]) / (2 * eps)
+    assert np.allclose(g, num, rtol=1e-5, atol=1e-7), np.abs(g - num).max()
+    # closed form of the loss for a one-unit net: out = b2 + w2 relu(b1 + W1 f)
+    if k == 1 and h == 1:
+        f = (xs & 3).astype(np.float64) * 2 / 3 - 1
+        out = params[3] + params[2] * np.maximum(0, params[1] + params[0] * f)
+        assert np.isclose(loss, np.mean((out - ys) ** 2), rtol=1e-12)
