@@ -1,15 +1,21 @@
 """Benchmark of the hot path on one MI355X: suffix-array + FM-index build of
-both directions and the FM-index overlap search, on the seeded synthetic
-"repeats and errors" workload (5 Mbp genome with 100 interspersed copies of ten
-1 kbp repeat units at 1% divergence, 100 bp reads at 40x with 0.5%
-substitutions, t = 63, 32-bit index).
+both directions and the FM-index overlap search, plain and seeded by the
+learned index, on a seeded synthetic workload.
+
+Default workload ("repeats and errors"): 5 Mbp genome with 100 interspersed
+copies of ten 1 kbp repeat units at 1% divergence, 100 bp reads at 40x with 0.5%
+substitutions, t = 63, 32-bit index.  --config 3 / 4: 50 Mbp genome, read
+lengths uniform in 36..250 at 30x, 0.3% substitutions, t = 31 (4 = the same
+reads with the learned index as the headline search).
 
 A step = one index build (SA, BWT occurrence lines, reads_index for the forward
 and the backward text) + one overlap search (candidate intervals, irreducible
-edges, edge CSR) over the deduplicated reads, which are resident in HBM before
-the timed region.  Prints one JSON line.
+edges, edge CSR) over the reads, which are resident in HBM before the timed
+region.  The learned variant trains its model once on the built index and then
+times the same number of seeded searches.  Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-baseline full|off]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--config 2]
+                    [--cpu-baseline full|off] [--learned 1|0]
 """
 import argparse
 import json
@@ -26,12 +32,22 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 LINE_BYTES = 64
 
+WORKLOADS = {
+    1: "1 Mbp uniform genome, 50 bp error-free reads at 20x, t=31, U32 index",
+    2: "5 Mbp genome + 100 copies of ten 1 kbp repeat units (1% div), 100 bp reads at 40x, "
+       "0.5% substitutions, t=63, U32 index",
+    3: "50 Mbp uniform genome, read lengths U{36..250} at 30x, 0.3% substitutions, t=31, "
+       "U64 index (device rows 32-bit: text < 2^32 codes)",
+    4: "50 Mbp uniform genome, read lengths U{36..250} at 30x, 0.3% substitutions, t=31, "
+       "U64 index (device rows 32-bit), learned index k=21, 64 hidden, 4096 leaves",
+}
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(fasta, L, t, n_expected):
+def cpu_baseline_reference(fasta, L, t, n_expected):
     """The reference OpenMP program (oracle/_ref) on the same reads file, timed
     by its own timers (index.c:643-680, string_graph.c:737-1044)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -54,8 +70,54 @@ def cpu_baseline(fasta, L, t, n_expected):
     n = int(re.search(r"has imported (\d+) reads", out).group(1))
     # takes: import, forward sort, backward sort, overlap graph, assembly
     fw, bw, graph = takes[1], takes[2], takes[3]
-    return {"reads": n, "fw_ms": fw, "bw_ms": bw, "graph_ms": graph, "wall_s": wall,
-            "threads": threads, "reads_match_gpu": n == n_expected}
+    return {"value": round(n / (graph / 1e3), 1), "unit": "reads/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"the full workload ({n} reads), reference pasqual_omp_u32, "
+                      f"OMP_NUM_THREADS={threads}, one run, overlap-graph timer "
+                      "(string_graph.c:737-1044)",
+            "sa_build_gsuffix_per_s": round(2 * n * L / ((fw + bw) / 1e3) / 1e9, 5),
+            "wall_s": round(wall, 1), "reads_match_gpu": n == n_expected}
+
+
+def cpu_baseline_port(pq, config, t):
+    """Reads of uneven length are rejected by the reference's import, so the
+    baseline is the oracle's single-threaded restatement (explicit-string overlap
+    search and qsort suffix arrays) on a bounded sample: the same generator and
+    coverage over a 0.5 Mbp genome."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pq_oracle as orc
+    cfg = pq.synth_preset(config)
+    cfg.genome_len = 500_000
+    codes, off = pq.synth_reads(cfg)
+    n = len(off) - 1
+    t0 = time.perf_counter()
+    orc.search_explicit(codes, off, t)
+    ts = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for d in (0, 1):
+        orc.suffix_array(orc.dir_text(codes, off, d), off)
+    tb = time.perf_counter() - t0
+    return {"value": round(n / ts, 1), "unit": "reads/s", "cores": 1, "kind": "port",
+            "sample": f"{n} reads of the same generator over a 0.5 Mbp genome (same coverage "
+                      "and lengths); oracle orc_search_explicit and orc_sa, one thread",
+            "sa_build_gsuffix_per_s": round(2 * int(off[-1]) / tb / 1e9, 5),
+            "wall_s": round(ts + tb, 1)}
+
+
+def load_workload(pq, config, np):
+    cfg = pq.synth_preset(config)
+    t = cfg.min_overlap
+    if cfg.len_min == cfg.len_max:
+        # fixed-length reads go through the reference's import (duplicate removal)
+        L = cfg.len_min
+        fasta = os.path.join(tempfile.mkdtemp(prefix="pqbench_"), "reads.fa")
+        pq.synth_fasta(cfg, fasta)
+        imp = pq.import_fasta(fasta, L, index_bits=32)
+        n = imp["n"]
+        off = np.arange(n + 1, dtype=np.uint64) * np.uint64(L)
+        return imp["reads"], off, t, L, fasta, "after duplicate removal"
+    codes, off = pq.synth_reads(cfg)
+    return codes, off, t, None, None, "as generated (uneven lengths: no duplicate removal)"
 
 
 def main():
@@ -63,7 +125,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2, choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-baseline", default="full", choices=["full", "off"])
     ap.add_argument("--learned", type=int, default=1,
                     help="also time the learned-index seeded search (1) or not (0)")
@@ -75,29 +137,26 @@ def main():
               "(the overlap search shards by read ranges with no exchange; see the "
               "design notes)", file=sys.stderr)
         sys.exit(2)
+    if args.config == 4:
+        args.learned = 1
 
     import numpy as np
     import pasqual_amd as pq
-    cfg = pq.synth_preset(args.config)
-    L, t = cfg.len_min, cfg.min_overlap
-    tmp = tempfile.mkdtemp(prefix="pqbench_")
-    fasta = os.path.join(tmp, "reads.fa")
     t0 = time.time()
-    pq.synth_fasta(cfg, fasta)
-    imp = pq.import_fasta(fasta, L, index_bits=32)
-    n = imp["n"]
-    off = np.arange(n + 1, dtype=np.uint64) * np.uint64(L)
-    N = n * L
-    log(f"workload config {args.config}: {n} reads after duplicate removal, N={N} per direction,"
-        f" prepared in {time.time() - t0:.1f}s")
+    codes, off, t, L, fasta, how = load_workload(pq, args.config, np)
+    n = len(off) - 1
+    N = int(off[-1])
+    log(f"workload config {args.config}: {n} reads {how}, N={N} per direction, "
+        f"prepared in {time.time() - t0:.1f}s")
 
     ctx = pq.Context(0)
-    ctx.upload(imp["reads"], off, t)
+    ctx.upload(codes, off, t)
     for _ in range(args.warmup):
         ctx.build_index()
         ctx.find_overlaps()
     ctx.sync()
-    build_ms, search_ms, find_ms, find_launch_ms = [], [], [], []
+    log("warm-up done")
+    build_ms, search_ms, find_launch_ms = [], [], []
     positions = lines = launches = 0
     t_start = time.perf_counter()
     for _ in range(args.steps):
@@ -111,7 +170,6 @@ def main():
         tm = ctx.timing()
         build_ms.append((b - a) * 1e3)
         search_ms.append((c - b) * 1e3)
-        find_ms.append(tm["find_ms"])
         find_launch_ms.append(tm["find_kernel_ms"])
         launches += tm["find_launches"]
         positions += tm["find_positions"]
@@ -120,6 +178,7 @@ def main():
     total_s = time.perf_counter() - t_start
     g = ctx.graph()
     edges = int(g["edge_off"][-1])
+    log("plain steps done")
 
     # learned-index variant on the same index: train once on the built SA,
     # then the same number of timed searches seeded by it
@@ -149,12 +208,16 @@ def main():
                    "find_kernel_ms": round(sum(l_find) / len(l_find), 3),
                    "rank_positions_per_read": round(l_pos / (n * args.steps), 1),
                    "seed_depth": st["depth"], "model_k": st["k"],
+                   "table_size": list(st["table_size"]),
                    "mean_window": [round(x, 2) for x in st["mean_window"]],
                    "max_window": list(st["max_window"]),
                    "probes_per_lookup": round(sum(st["probes"]) / max(1, lookups), 2),
+                   "absent_chains": st["absent"], "fallback_chains": st["fallback"],
                    "train_ms": round(st["train_ms"], 1),
+                   "fit_loss": [float(f"{x:.3g}") for x in st["fit_loss"]],
                    "train_loss": [float(f"{x:.3g}") for x in st["train_loss"]],
                    "edges_identical_to_plain": bool(same)}
+        log("learned steps done")
     if args.probe_copy:
         pms = ctx.probe_copy(args.probe_copy << 20)
         log(f"copy probe: {args.probe_copy} MiB in {pms:.3f} ms "
@@ -171,11 +234,11 @@ def main():
     alg_bytes = per_launch_positions * LINE_BYTES
     achieved = alg_bytes / (avg_launch_ms / 1e3) / 1e9
     # HBM/fabric bytes per k_find launch from the committed counter runs of this
-    # same workload (profiles/config2_traffic.md): FETCH_SIZE and WRITE_SIZE,
-    # each scaled by what it reads for the plain copy kernel
+    # same workload: FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for
+    # the plain copy kernel
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "config2_traffic.json")
-    if args.config == 2 and os.path.exists(tpath):
+    tpath = os.path.join(ROOT, "profiles", f"config{args.config}_traffic.json")
+    if os.path.exists(tpath):
         with open(tpath) as f:
             tk = json.load(f)["kernels"].get("k_find")
         if tk:
@@ -185,25 +248,21 @@ def main():
                 "kernel": "k_find", "alg_bytes_per_launch": int(alg_bytes),
                 "rank_positions_per_read": round(positions / (n * args.steps), 1),
                 "lines_per_read": round(lines / (n * args.steps), 1),
-                "avg_launch_ms": round(avg_launch_ms, 3)}
+                "avg_launch_ms": round(avg_launch_ms, 3),
+                "launches_per_search": round(launches / args.steps, 2)}
 
     base = None
     if args.cpu_baseline == "full":
-        log("timing the reference OpenMP program on the same reads ...")
-        ref = cpu_baseline(fasta, L, t, n)
-        if ref:
-            base = {"value": round(ref["reads"] / (ref["graph_ms"] / 1e3), 1),
-                    "unit": "reads/s", "cores": ref["threads"], "kind": "reference",
-                    "sample": f"the full workload ({ref['reads']} reads), reference "
-                              f"pasqual_omp_u32, OMP_NUM_THREADS={ref['threads']}, one run, "
-                              "overlap-graph timer (string_graph.c:737-1044)",
-                    "sa_build_gsuffix_per_s": round(2 * ref["reads"] * L /
-                                                    ((ref["fw_ms"] + ref["bw_ms"]) / 1e3) / 1e9,
-                                                    5),
-                    "wall_s": round(ref["wall_s"], 1)}
+        log("timing the CPU baseline ...")
+        if L is not None:
+            base = cpu_baseline_reference(fasta, L, t, n)
+        else:
+            base = cpu_baseline_port(pq, args.config, t)
+    headline_learned = args.config == 4
+    value = learned["reads_per_s"] if headline_learned else reads_per_s
     line = {
-        "metric": "overlap-search reads/s (plain FM-index search); SA build Gsuffix/s",
-        "value": round(reads_per_s, 1),
+        "metric": "overlap-search reads/s + SA build Gsuffix/s, learned vs plain index, 1 GPU",
+        "value": round(value, 1),
         "unit": "reads/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -214,11 +273,11 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (seeded generator, pasqual_amd/csrc/host/pq_synth.c)",
-        "config": {"workload": f"config {args.config}: 5 Mbp genome + 100 copies of ten 1 kbp "
-                               "repeat units (1% div), 100 bp reads at 40x, 0.5% substitutions, "
-                               "t=63, U32 index",
-                   "reads": n, "suffixes_per_direction": N, "read_len": L, "min_overlap": t,
-                   "search": "plain"},
+        "config": {"workload": f"config {args.config}: {WORKLOADS[args.config]}",
+                   "reads": n, "suffixes_per_direction": N,
+                   "read_len": L if L is not None else "36..250", "min_overlap": t,
+                   "search": "learned" if headline_learned else "plain"},
+        "plain_search_reads_per_s": round(reads_per_s, 1),
         "sa_build_gsuffix_per_s": round(gsuffix_per_s, 3),
         "search_ms": round(avg_search, 3),
         "build_ms": round(avg_build, 3),
@@ -230,7 +289,7 @@ def main():
     if learned:
         line["learned_vs_plain_search"] = round(learned["reads_per_s"] / reads_per_s, 3)
     if base:
-        line["speedup_vs_cpu_search"] = round(reads_per_s / base["value"], 1)
+        line["speedup_vs_cpu_search"] = round(value / base["value"], 1)
         line["speedup_vs_cpu_sa_build"] = round(gsuffix_per_s / base["sa_build_gsuffix_per_s"], 1)
     ctx.close()
     print(json.dumps(line), flush=True)

@@ -146,6 +146,13 @@ int pq_learned_stats(pq_ctx_t *ctx, pq_learned_stats_t *st);
  * b1[hidden], w2[hidden], b2.  *count in: capacity, out: number of parameters
  * (params may be NULL to query it). */
 int pq_learned_params(pq_ctx_t *ctx, int dir, float *params, uint32_t *count);
+/* Host copy of the exact part of the learned index of direction dir: the distinct
+ * D-code prefixes of the suffixes with at least D codes, in suffix-array order.
+ * keys: two uint64 per entry (high half first; first code in the top bits, 2 bits
+ * per code, bit 0 of the low half set); first: first SA row; rows: row count.
+ * *count in: capacity, out: number of entries (outputs may be NULL to query). */
+int pq_learned_table(pq_ctx_t *ctx, int dir, uint64_t *keys, uint32_t *first, uint32_t *rows,
+                     uint64_t *count);
 /* Gradient of (1/n) sum (MLP(x_i) - y_i)^2 over n samples, by the device training
  * kernels; x_i are k-code prefix values (first code most significant), params as
  * above.  grad receives one float per parameter, *loss the mean squared error. */

@@ -32,7 +32,7 @@ PQ_SYMBOLS = [
     "pq_get_sa", "pq_get_bwt", "pq_get_reads_index", "pq_get_counts", "pq_get_occ_ref",
     "pq_get_shortcut", "pq_num_reads", "pq_text_len", "pq_find_overlaps", "pq_graph_sizes",
     "pq_graph_copy", "pq_learned_default", "pq_learned_train", "pq_learned_stats",
-    "pq_learned_params", "pq_mlp_gradients",
+    "pq_learned_params", "pq_learned_table", "pq_mlp_gradients",
     "pq_get_timing", "pq_sync", "pq_probe_copy", "pq_synth_preset", "pq_synth_num_reads", "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
 ]
@@ -139,6 +139,7 @@ def lib(width="u32"):
         "pq_learned_train": (I32, [P, P]),
         "pq_learned_stats": (I32, [P, P]),
         "pq_learned_params": (I32, [P, I32, P, P]),
+        "pq_learned_table": (I32, [P, I32, P, P, P, P]),
         "pq_mlp_gradients": (I32, [P, U32, U32, P, P, P, U32, P, P]),
         "pq_get_timing": (I32, [P, P]),
         "pq_sync": (I32, [P]),
@@ -341,6 +342,19 @@ class Context:
         self._check(self._L.pq_learned_params(self._h, d, out.ctypes.data, C.byref(cnt)),
                     "learned_params")
         return out
+
+    def learned_table(self, d):
+        """(keys uint64[n, 2] high/low halves, first SA row, row count) of direction d."""
+        cnt = C.c_uint64(0)
+        self._check(self._L.pq_learned_table(self._h, d, None, None, None, C.byref(cnt)),
+                    "learned_table")
+        n = cnt.value
+        keys = np.zeros((n, 2), np.uint64)
+        first = np.zeros(n, np.uint32)
+        rows = np.zeros(n, np.uint32)
+        self._check(self._L.pq_learned_table(self._h, d, keys.ctypes.data, first.ctypes.data,
+                                             rows.ctypes.data, C.byref(cnt)), "learned_table")
+        return keys, first, rows
 
     def mlp_gradients(self, k, hidden, params, xs, targets):
         params = np.ascontiguousarray(params, np.float32)

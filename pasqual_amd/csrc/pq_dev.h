@@ -53,13 +53,15 @@ struct Geom {
 
 __device__ __forceinline__ uint64_t pq_window(const uint64_t *t2, uint64_t g)
 {
-    // 32 codes starting at text position g, most significant first
+    // 32 codes starting at text position g, most significant first.  Both words
+    // are always loaded and joined by a branch-free funnel shift (the packed
+    // text carries slack words past its end): a variant that loaded the second
+    // word only for unaligned g, under an exec mask into registers the first
+    // word had just been moved into, returned wrong windows now and then on
+    // gfx950.
     uint64_t w = g >> 5;
     unsigned s = (unsigned) (g & 31) * 2;
-    uint64_t a = t2[w];
-    if (s == 0)
-        return a;
-    return (a << s) | (t2[w + 1] >> (64 - s));
+    return (t2[w] << s) | ((t2[w + 1] >> 1) >> (63 - s));
 }
 
 __device__ __forceinline__ unsigned pq_code(const uint64_t *t2, uint64_t g)

@@ -329,7 +329,9 @@ __global__ void k_learned_keys(Geom G, const uint64_t *t2, const uint32_t *sa, u
         return;
     uint32_t g = sa[i];
     uint64_t rem = pq_rem(G, g);
-    u128 w = text_codes(t2, g) & top_mask(D);
+    // up to 32 codes come from one 64-bit window (two words instead of three)
+    u128 w = D <= 32 ? (u128) pq_window(t2, g) << 64 : text_codes(t2, g);
+    w &= top_mask(D);
     if (rem < D)
         w &= ~(~(u128) 0 >> (2 * rem));
     K[i] = w | (u128) (rem >= D ? 1u : 0u);
@@ -1114,5 +1116,48 @@ extern "C" int pq_learned_params(pq_ctx_t *ctx, int dir, float *params, uint32_t
         PQ_CHECK(hipStreamSynchronize(ctx->stream));
     }
     *count = P;
+    return PQ_OK;
+}
+
+// Host copy of the table of direction dir: per entry the key as two 64-bit
+// halves (high first; first code in the top bits, bit 0 of the low half set),
+// its first SA row and its row count.  *count in: capacity (entries), out: table
+// size (any output may be NULL to query it).
+extern "C" int pq_learned_table(pq_ctx_t *ctx, int dir, uint64_t *keys, uint32_t *first,
+                                uint32_t *rows, uint64_t *count)
+{
+    if (!ctx || !count || dir < 0 || dir > 1)
+        return PQ_ERR_ARG;
+    Learned *LM = ctx->learned;
+    if (!LM)
+        return ctx->fail(PQ_ERR_STATE, "learned index not trained");
+    const DirModel &M = LM->dir[dir];
+    if ((keys || first || rows) && *count < M.nU)
+        return ctx->fail(PQ_ERR_ARG, "table buffer too small");
+    *count = M.nU;
+    if (M.nU == 0)
+        return PQ_OK;
+    if (keys) {
+        std::vector<u128> k(M.nU);
+        PQ_CHECK(hipMemcpyAsync(k.data(), M.key, M.nU * sizeof(u128), hipMemcpyDeviceToHost,
+                                ctx->stream));
+        PQ_CHECK(hipStreamSynchronize(ctx->stream));
+        for (uint64_t i = 0; i < M.nU; i++) {
+            keys[2 * i] = (uint64_t) (k[i] >> 64);
+            keys[2 * i + 1] = (uint64_t) k[i];
+        }
+    }
+    if (first || rows) {
+        std::vector<uint2> r(M.nU);
+        PQ_CHECK(hipMemcpyAsync(r.data(), M.rows, M.nU * sizeof(uint2), hipMemcpyDeviceToHost,
+                                ctx->stream));
+        PQ_CHECK(hipStreamSynchronize(ctx->stream));
+        for (uint64_t i = 0; i < M.nU; i++) {
+            if (first)
+                first[i] = r[i].x;
+            if (rows)
+                rows[i] = r[i].y;
+        }
+    }
     return PQ_OK;
 }

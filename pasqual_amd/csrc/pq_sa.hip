@@ -5,7 +5,7 @@
 // position.
 //
 //  1. Bucket all N suffixes by a 2K+4-bit key: their first K codes (zero
-//     padding past the read end) and min(length, K), K ~ log4(N/128) <= 12.
+//     padding past the read end) and min(length, K), K <= 14 (a 32-bit key).
 //     The key is order-consistent, and a bucket whose length field is below K
 //     holds identical complete suffixes, already in position order because the
 //     LSD radix passes (from the identity permutation) are stable.
@@ -56,7 +56,7 @@ __device__ __forceinline__ uint64_t chunk_key(const uint64_t *t2, uint64_t g, ui
     return (chars << 5) | f;
 }
 
-// Bucket key of the first K codes (K <= 12): (codes << 4) | min(rem, K).  Keys
+// Bucket key of the first K codes (K <= 14): (codes << 4) | min(rem, K).  Keys
 // with a field below K are complete suffixes; equal ones are equal strings.
 __global__ void k_bucket_keys(Geom G, const uint64_t *t2, uint64_t N, unsigned K, uint32_t *BK,
                               uint32_t *V)
@@ -433,13 +433,16 @@ static int sort_lists(pq_ctx_t *ctx, const Geom &G, DirIndex &D, uint32_t base,
     return PQ_OK;
 }
 
-// Bucket width for N suffixes: about 32 suffixes per bucket of K codes.
+// Bucket width for N suffixes: enough codes for about 8 suffixes per bucket,
+// then as many more as fit in the same number of 8-bit radix passes over the
+// 2K+4-bit key (finer buckets cost nothing there and make the sorts cheaper).
 static unsigned bucket_codes(uint64_t N)
 {
     unsigned K = 4;
-    while (K < 12 && (N >> (2 * K)) > 32)
+    while (K < 14 && (N >> (2 * K)) > 8)
         K++;
-    return K;
+    unsigned passes = (2 * K + 4 + 7) / 8;
+    return std::min(14u, (8 * passes - 4) / 2);
 }
 
 int pq_build_sa_dir(pq_ctx_t *ctx, int d)
