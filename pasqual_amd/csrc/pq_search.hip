@@ -28,9 +28,14 @@
 
 #include <chrono>
 
-#define KMAX_LDS 128
-#define KMAX_GLOBAL 65536
+// extraction tiers by items per read side: most reads fit the first (small LDS
+// footprint, high occupancy); the few larger ones go to one wave per block with a
+// large LDS share, and the rest to global-memory scratch
+#define KMAX_LDS 64
 #define EXT_WAVES 4
+#define KMAX_LDS2 512
+#define KMAX_GLOBAL 65536
+#define GLOBAL_SLOTS 256
 
 struct ChainBuf {
     uint2 *lc;               // (partner start, count)
@@ -365,30 +370,50 @@ __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw
     return true;
 }
 
-__host__ __device__ static inline size_t ext_wave_bytes(uint32_t kmax, uint32_t kw, uint32_t cap)
+// Per-wave storage: keys (kw words per item), metadata, emit flags, and the
+// prefix of entry counts (an entry holds at least one item, so at most kmax).
+__host__ __device__ static inline size_t ext_wave_bytes(uint32_t kmax, uint32_t kw)
 {
-    size_t b = (size_t) kmax * kw * 8 + (size_t) kmax * sizeof(ItemMeta) + kmax +
-               (size_t) (2 * cap + 2) * 4;
+    size_t b = (size_t) kmax * kw * 8 + (size_t) kmax * sizeof(ItemMeta) +
+               (size_t) (kmax + 2) * 4 + kmax;
     return (b + 15) & ~(size_t) 15;
 }
 
-__global__ void __launch_bounds__(64 * EXT_WAVES) k_extract_lds(Geom G, DirView fw, DirView bw,
-                                                                ChainBuf cb, uint64_t r0,
-                                                                uint64_t nr, uint32_t kw,
-                                                                ExtOut out)
+struct ExtWave {
+    uint64_t *keys;
+    ItemMeta *meta;
+    uint32_t *pre;
+    uint8_t *emitf;
+};
+
+__device__ __forceinline__ ExtWave ext_wave(char *base, uint32_t kmax, uint32_t kw)
+{
+    ExtWave w;
+    w.keys = (uint64_t *) base;
+    w.meta = (ItemMeta *) (base + (size_t) kmax * kw * 8);
+    w.pre = (uint32_t *) ((char *) w.meta + (size_t) kmax * sizeof(ItemMeta));
+    w.emitf = (uint8_t *) (w.pre + kmax + 2);
+    return w;
+}
+
+// One wave per read, storage in LDS.  Reads [0, nr) of the batch, or the nr
+// reads of `list`; a read with more than KMAX items on a side is appended to
+// out.overflow_reads for the next tier.
+template <uint32_t KMAX, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirView fw, DirView bw,
+                                                           ChainBuf cb, uint64_t r0, uint64_t nr,
+                                                           const uint32_t *list, uint32_t kw,
+                                                           ExtOut out)
 {
     extern __shared__ uint64_t lds[];
     const int wid = threadIdx.x >> 6;
-    uint64_t rl = (uint64_t) blockIdx.x * EXT_WAVES + wid;
-    if (rl >= nr)
+    uint64_t i = (uint64_t) blockIdx.x * WAVES + wid;
+    if (i >= nr)
         return;
-    size_t per = ext_wave_bytes(KMAX_LDS, kw, cb.cap);
-    char *base = (char *) lds + per * wid;
-    uint64_t *keys = (uint64_t *) base;
-    ItemMeta *meta = (ItemMeta *) (base + (size_t) KMAX_LDS * kw * 8);
-    uint32_t *pre = (uint32_t *) ((char *) meta + (size_t) KMAX_LDS * sizeof(ItemMeta));
-    uint8_t *emitf = (uint8_t *) (pre + 2 * cb.cap + 2);
-    bool ok = extract_read(G, fw, bw, cb, rl, r0 + rl, kw, keys, meta, emitf, pre, KMAX_LDS, out);
+    uint64_t rl = list ? list[i] : i;
+    ExtWave w = ext_wave((char *) lds + ext_wave_bytes(KMAX, kw) * wid, KMAX, kw);
+    bool ok = extract_read(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf, w.pre, KMAX,
+                           out);
     if (!ok && (threadIdx.x & 63) == 0) {
         uint32_t at = atomicAdd(out.n_overflow, 1u);
         out.overflow_reads[at] = (uint32_t) rl;
@@ -400,16 +425,11 @@ __global__ void __launch_bounds__(64) k_extract_global(Geom G, DirView fw, DirVi
                                                        uint32_t nlist, uint32_t kw, char *gscratch,
                                                        ExtOut out)
 {
-    size_t per = ext_wave_bytes(KMAX_GLOBAL, kw, cb.cap);
-    char *base = gscratch + per * blockIdx.x;
-    uint64_t *keys = (uint64_t *) base;
-    ItemMeta *meta = (ItemMeta *) (base + (size_t) KMAX_GLOBAL * kw * 8);
-    uint32_t *pre = (uint32_t *) ((char *) meta + (size_t) KMAX_GLOBAL * sizeof(ItemMeta));
-    uint8_t *emitf = (uint8_t *) (pre + 2 * cb.cap + 2);
+    ExtWave w = ext_wave(gscratch + ext_wave_bytes(KMAX_GLOBAL, kw) * blockIdx.x, KMAX_GLOBAL, kw);
     for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
         uint64_t rl = list[i];
-        bool ok = extract_read(G, fw, bw, cb, rl, r0 + rl, kw, keys, meta, emitf, pre, KMAX_GLOBAL,
-                               out);
+        bool ok = extract_read(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf, w.pre,
+                               KMAX_GLOBAL, out);
         if (!ok && threadIdx.x == 0)
             atomicOr(out.flags, 2u);
     }
@@ -517,7 +537,7 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         (rc = pq_slot(ctx, SLOT_APPEND, edge_cap * 8, &app)) ||
         (rc = pq_slot(ctx, SLOT_STAGED, (n + 1) * 2 * EDGE_SLOT * 8, &staged)) ||
         (rc = pq_slot(ctx, SLOT_TOT, (n + 1) * 8, &tot)) ||
-        (rc = pq_slot(ctx, SLOT_OVL, (batch + 1) * 4, &ovl)) ||
+        (rc = pq_slot(ctx, SLOT_OVL, (2 * batch + 2) * 4, &ovl)) ||
         (rc = pq_slot(ctx, SLOT_MISC, 64, &misc)) ||
         (mode == PQ_SEARCH_LEARNED &&
          (rc = pq_slot(ctx, SLOT_SEEDS, batch * 4 * sizeof(Seed), &seeds)))) {
@@ -543,7 +563,10 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     // C[2..4] travel in Clo.y..w, C[5] on its own (C[0] = C[1] = 0)
     uint4 Clo = make_uint4(0, (uint32_t) ctx->C[2], (uint32_t) ctx->C[3], (uint32_t) ctx->C[4]);
     uint32_t C5 = (uint32_t) ctx->C[5];
-    size_t lds_bytes = ext_wave_bytes(KMAX_LDS, kw, cb.cap) * EXT_WAVES;
+    const size_t lds1 = ext_wave_bytes(KMAX_LDS, kw) * EXT_WAVES, lds2 = ext_wave_bytes(KMAX_LDS2, kw);
+    ExtOut eo2 = eo;                   // second tier: overflow goes to the tail list
+    eo2.overflow_reads = ovl + batch;
+    eo2.n_overflow = misc + 4;
     for (uint64_t r0 = 0; r0 < n && rc == PQ_OK; r0 += batch) {
         uint64_t nr = std::min(batch, n - r0);
         uint32_t kseed = 0;
@@ -554,22 +577,32 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         hipLaunchKernelGGL(k_find, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0, ctx->stream, G, fw,
                            bw, r0, nr, t, Clo, C5, seeds, kseed, cb, fstats);
         (void) hipEventRecord(e1, ctx->stream);
-        hipLaunchKernelGGL(k_extract_lds, dim3(pq_blocks(nr, EXT_WAVES)), dim3(64 * EXT_WAVES),
-                           lds_bytes, ctx->stream, G, fw, bw, cb, r0, nr, kw, eo);
+        hipLaunchKernelGGL((k_extract_lds<KMAX_LDS, EXT_WAVES>), dim3(pq_blocks(nr, EXT_WAVES)),
+                           dim3(64 * EXT_WAVES), lds1, ctx->stream, G, fw, bw, cb, r0, nr,
+                           (const uint32_t *) nullptr, kw, eo);
         PQ_CHECK(hipGetLastError());
-        uint32_t nov = 0;
-        PQ_CHECK(hipMemcpyAsync(&nov, eo.n_overflow, 4, hipMemcpyDeviceToHost, ctx->stream));
+        uint32_t nov[3] = {0, 0, 0};   // misc[2..4]: tier-1 overflow, flags, tier-2 overflow
+        PQ_CHECK(hipMemcpyAsync(nov, eo.n_overflow, 12, hipMemcpyDeviceToHost, ctx->stream));
         PQ_CHECK(hipStreamSynchronize(ctx->stream));
-        if (nov) {
-            const uint32_t slots = 64;
+        if (nov[0]) {
+            hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1>), dim3(nov[0]), dim3(64), lds2,
+                               ctx->stream, G, fw, bw, cb, r0, (uint64_t) nov[0], ovl, kw, eo2);
+            PQ_CHECK(hipGetLastError());
+            PQ_CHECK(hipMemcpyAsync(&nov[2], eo2.n_overflow, 4, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+            PQ_CHECK(hipMemsetAsync(eo.n_overflow, 0, 4, ctx->stream));
+            PQ_CHECK(hipStreamSynchronize(ctx->stream));
+        }
+        if (nov[2]) {
             if (!gscratch && (rc = pq_slot(ctx, SLOT_GSCRATCH,
-                                           slots * ext_wave_bytes(KMAX_GLOBAL, kw, cb.cap),
+                                           GLOBAL_SLOTS * ext_wave_bytes(KMAX_GLOBAL, kw),
                                            &gscratch)))
                 break;
-            hipLaunchKernelGGL(k_extract_global, dim3(std::min<uint32_t>(slots, nov)), dim3(64), 0,
-                               ctx->stream, G, fw, bw, cb, r0, ovl, nov, kw, gscratch, eo);
+            hipLaunchKernelGGL(k_extract_global, dim3(std::min<uint32_t>(GLOBAL_SLOTS, nov[2])),
+                               dim3(64), 0, ctx->stream, G, fw, bw, cb, r0, eo2.overflow_reads,
+                               nov[2], kw, gscratch, eo);
             PQ_CHECK(hipGetLastError());
-            PQ_CHECK(hipMemsetAsync(eo.n_overflow, 0, 4, ctx->stream));
+            PQ_CHECK(hipMemsetAsync(eo2.n_overflow, 0, 4, ctx->stream));
         }
         (void) hipEventRecord(e2, ctx->stream);
         PQ_CHECK(hipEventSynchronize(e2));
