@@ -31,7 +31,8 @@
 // extraction tiers by items per read side: most reads fit the first (small LDS
 // footprint, high occupancy); the few larger ones go to one wave per block with a
 // large LDS share, and the rest to global-memory scratch
-#define KMAX_LDS 64
+#define KMAX_LDS 32          // first tier: a quarter wave per read
+#define EXT_GW 16
 #define EXT_WAVES 4
 #define KMAX_LDS2 512
 #define KMAX_GLOBAL 65536
@@ -184,15 +185,27 @@ struct ExtOut {
     uint32_t *flags;              // bit0: edge buffer full, bit1: item limit
 };
 
-// Process one read with one wave.  keys/meta/emitf/pre: this wave's storage
-// (LDS or global) for kmax items.  Returns false, writing nothing, if a side
-// has more than kmax items.
+// Items of this group of GW lanes set in a wave-wide ballot.
+template <int GW> __device__ __forceinline__ uint32_t group_count(bool x)
+{
+    uint64_t b = __ballot(x);
+    if (GW == 64)
+        return (uint32_t) __popcll(b);
+    const int shift = (int) (threadIdx.x % 64) & ~(GW - 1);
+    return (uint32_t) __popcll((b >> shift) & ((1ull << GW) - 1ull));
+}
+
+// Process one read with one group of GW lanes (a wave or a quarter wave; the
+// groups of a wave work on different reads).  keys/meta/emitf/pre: this group's
+// storage (LDS or global) for kmax items.  Returns false, writing nothing, if a
+// side has more than kmax items.
+template <int GW>
 __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw,
                              const ChainBuf &cb, uint64_t rl, uint64_t r, uint32_t kw,
                              uint64_t *keys, ItemMeta *meta, uint8_t *emitf, uint32_t *pre,
                              uint32_t kmax, const ExtOut &out)
 {
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & (GW - 1);
     uint32_t cnt4[4];
     for (int c = 0; c < 4; c++)
         cnt4[c] = cb.n[rl * 4 + c];
@@ -200,13 +213,13 @@ __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw
     for (int side = 0; side < 2; side++) {
         const int chA = side ? 0 : 2, chB = side ? 1 : 3;
         uint32_t k = 0;
-        for (uint32_t e = lane; e < cnt4[chA] + cnt4[chB]; e += 64) {
+        for (uint32_t e = lane; e < cnt4[chA] + cnt4[chB]; e += GW) {
             uint64_t at = (e < cnt4[chA]) ? (rl * 4 + chA) * cb.cap + e
                                           : (rl * 4 + chB) * cb.cap + (e - cnt4[chA]);
             k += cb.lc[at].y;
         }
-        for (int d = 32; d > 0; d >>= 1)
-            k += __shfl_xor(k, d, 64);
+        for (int d = GW / 2; d > 0; d >>= 1)
+            k += __shfl_xor(k, d, GW);
         if (k > kmax)
             return false;
     }
@@ -216,7 +229,7 @@ __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw
         const uint32_t nA = cnt4[chA], nB = cnt4[chB], ne = nA + nB;
         // exclusive prefix of the counts over the side's entries
         uint32_t carry = 0;
-        for (uint32_t e0 = 0; e0 < ne; e0 += 64) {
+        for (uint32_t e0 = 0; e0 < ne; e0 += GW) {
             uint32_t e = e0 + lane;
             uint32_t v = 0;
             if (e < ne) {
@@ -225,19 +238,19 @@ __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw
                 v = cb.lc[at].y;
             }
             uint32_t inc = v;
-            for (int d = 1; d < 64; d <<= 1) {
-                uint32_t x = __shfl_up(inc, d, 64);
+            for (int d = 1; d < GW; d <<= 1) {
+                uint32_t x = __shfl_up(inc, d, GW);
                 if (lane >= d)
                     inc += x;
             }
             if (e < ne)
                 pre[e] = carry + inc - v;
-            carry += __shfl(inc, 63, 64);
+            carry += __shfl(inc, GW - 1, GW);
         }
         const uint32_t k = carry;
         wave_sync();
         // expand the candidate intervals into items with their extension keys
-        for (uint32_t q = lane; q < k; q += 64) {
+        for (uint32_t q = lane; q < k; q += GW) {
             uint32_t lo = 0, hi = ne;        // entry e with pre[e] <= q < pre[e+1]
             while (hi - lo > 1) {
                 uint32_t mid = (lo + hi) >> 1;
@@ -289,7 +302,7 @@ __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw
         // an item is emitted if no other key is a proper prefix of its key, its
         // group is the first among equal keys, and it is not the read itself
         uint32_t valid_items = 0, emitted = 0;
-        for (uint32_t q = lane; q < k + 63; q += 64) {      // uniform trip count
+        for (uint32_t q = lane; q < k + GW - 1; q += GW) {  // same trip count in the group
             bool mine = q < k;
             ItemMeta mi;
             if (mine)
@@ -312,8 +325,8 @@ __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw
                         keep = false;                    // an earlier group wins
                 }
             }
-            valid_items += (uint32_t) __popcll(__ballot(act));
-            emitted += (uint32_t) __popcll(__ballot(keep));
+            valid_items += group_count<GW>(act);
+            emitted += group_count<GW>(keep);
             if (mine)
                 emitf[q] = (uint8_t) keep;
         }
@@ -335,7 +348,7 @@ __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw
                 out.recB[r] = base;
             }
         }
-        base = __shfl(base, 0, 64);
+        base = __shfl(base, 0, GW);
         uint64_t cap = EDGE_SLOT;
         if (emitted > EDGE_SLOT) {
             dst = out.edges + base;
@@ -344,7 +357,7 @@ __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw
         wave_sync();
         // output position = rank among the emitted items by (key, rank, o, m)
         if (emitted) {
-            for (uint32_t q = lane; q < k; q += 64) {
+            for (uint32_t q = lane; q < k; q += GW) {
                 if (!emitf[q])
                     continue;
                 ItemMeta mi = meta[q];
@@ -396,25 +409,25 @@ __device__ __forceinline__ ExtWave ext_wave(char *base, uint32_t kmax, uint32_t 
     return w;
 }
 
-// One wave per read, storage in LDS.  Reads [0, nr) of the batch, or the nr
-// reads of `list`; a read with more than KMAX items on a side is appended to
-// out.overflow_reads for the next tier.
-template <uint32_t KMAX, int WAVES>
+// One group of GW lanes per read, storage in LDS, WAVES waves per block.  Reads
+// [0, nr) of the batch, or the nr reads of `list`; a read with more than KMAX
+// items on a side is appended to out.overflow_reads for the next tier.
+template <uint32_t KMAX, int WAVES, int GW>
 __global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirView fw, DirView bw,
                                                            ChainBuf cb, uint64_t r0, uint64_t nr,
                                                            const uint32_t *list, uint32_t kw,
                                                            ExtOut out)
 {
     extern __shared__ uint64_t lds[];
-    const int wid = threadIdx.x >> 6;
-    uint64_t i = (uint64_t) blockIdx.x * WAVES + wid;
+    const int grp = threadIdx.x / GW;
+    uint64_t i = (uint64_t) blockIdx.x * (WAVES * 64 / GW) + grp;
     if (i >= nr)
         return;
     uint64_t rl = list ? list[i] : i;
-    ExtWave w = ext_wave((char *) lds + ext_wave_bytes(KMAX, kw) * wid, KMAX, kw);
-    bool ok = extract_read(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf, w.pre, KMAX,
-                           out);
-    if (!ok && (threadIdx.x & 63) == 0) {
+    ExtWave w = ext_wave((char *) lds + ext_wave_bytes(KMAX, kw) * grp, KMAX, kw);
+    bool ok = extract_read<GW>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf, w.pre,
+                               KMAX, out);
+    if (!ok && (threadIdx.x & (GW - 1)) == 0) {
         uint32_t at = atomicAdd(out.n_overflow, 1u);
         out.overflow_reads[at] = (uint32_t) rl;
     }
@@ -428,8 +441,8 @@ __global__ void __launch_bounds__(64) k_extract_global(Geom G, DirView fw, DirVi
     ExtWave w = ext_wave(gscratch + ext_wave_bytes(KMAX_GLOBAL, kw) * blockIdx.x, KMAX_GLOBAL, kw);
     for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
         uint64_t rl = list[i];
-        bool ok = extract_read(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf, w.pre,
-                               KMAX_GLOBAL, out);
+        bool ok = extract_read<64>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
+                                   w.pre, KMAX_GLOBAL, out);
         if (!ok && threadIdx.x == 0)
             atomicOr(out.flags, 2u);
     }
@@ -563,7 +576,8 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     // C[2..4] travel in Clo.y..w, C[5] on its own (C[0] = C[1] = 0)
     uint4 Clo = make_uint4(0, (uint32_t) ctx->C[2], (uint32_t) ctx->C[3], (uint32_t) ctx->C[4]);
     uint32_t C5 = (uint32_t) ctx->C[5];
-    const size_t lds1 = ext_wave_bytes(KMAX_LDS, kw) * EXT_WAVES, lds2 = ext_wave_bytes(KMAX_LDS2, kw);
+    const size_t lds1 = ext_wave_bytes(KMAX_LDS, kw) * (EXT_WAVES * 64 / EXT_GW);
+    const size_t lds2 = ext_wave_bytes(KMAX_LDS2, kw);
     ExtOut eo2 = eo;                   // second tier: overflow goes to the tail list
     eo2.overflow_reads = ovl + batch;
     eo2.n_overflow = misc + 4;
@@ -577,15 +591,15 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         hipLaunchKernelGGL(k_find, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0, ctx->stream, G, fw,
                            bw, r0, nr, t, Clo, C5, seeds, kseed, cb, fstats);
         (void) hipEventRecord(e1, ctx->stream);
-        hipLaunchKernelGGL((k_extract_lds<KMAX_LDS, EXT_WAVES>), dim3(pq_blocks(nr, EXT_WAVES)),
-                           dim3(64 * EXT_WAVES), lds1, ctx->stream, G, fw, bw, cb, r0, nr,
-                           (const uint32_t *) nullptr, kw, eo);
+        hipLaunchKernelGGL((k_extract_lds<KMAX_LDS, EXT_WAVES, EXT_GW>),
+                           dim3(pq_blocks(nr, EXT_WAVES * 64 / EXT_GW)), dim3(64 * EXT_WAVES), lds1,
+                           ctx->stream, G, fw, bw, cb, r0, nr, (const uint32_t *) nullptr, kw, eo);
         PQ_CHECK(hipGetLastError());
         uint32_t nov[3] = {0, 0, 0};   // misc[2..4]: tier-1 overflow, flags, tier-2 overflow
         PQ_CHECK(hipMemcpyAsync(nov, eo.n_overflow, 12, hipMemcpyDeviceToHost, ctx->stream));
         PQ_CHECK(hipStreamSynchronize(ctx->stream));
         if (nov[0]) {
-            hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1>), dim3(nov[0]), dim3(64), lds2,
+            hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64>), dim3(nov[0]), dim3(64), lds2,
                                ctx->stream, G, fw, bw, cb, r0, (uint64_t) nov[0], ovl, kw, eo2);
             PQ_CHECK(hipGetLastError());
             PQ_CHECK(hipMemcpyAsync(&nov[2], eo2.n_overflow, 4, hipMemcpyDeviceToHost,
