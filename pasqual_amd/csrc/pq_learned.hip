@@ -78,8 +78,11 @@ struct DirModel {
     float *iparams = nullptr;     // inference form (rows padded to MLP_K_MAX)
     uint64_t *range = nullptr;    // [xmin, xmax] per leaf
     uint32_t *knots = nullptr;    // (S + 1) entries per leaf
+    uint32_t *link = nullptr;     // entry of the reversed pattern in the other table
     double loss = 0, fit_loss = 0;
 };
+
+#define LINK_NONE 0xFFFFFFFFu
 
 struct Learned {
     uint32_t k = 21, h = 64, leaves = 4096, segs = 64, depth = 21;
@@ -449,6 +452,7 @@ struct ModelView {
     const float *iparams;
     const uint64_t *range;
     const uint32_t *knots;
+    const uint32_t *link;
 };
 
 // Parameters read by every lane of a wave at the same addresses: the pointer is
@@ -545,22 +549,50 @@ __device__ __forceinline__ uint64_t rev_codes64(uint64_t x)
     return __builtin_bswap64(x);
 }
 
-// The first D codes a chain has matched at depth D, for its search index and for
-// its partner index, left-aligned in 128 bits.  Chains as in k_find: FB
-// n[L-D..L) in fw, FF comp(n[L-D..L)) in bw, BF rev(n[0..D)) in bw, BB
-// rc(n[0..D)) in fw; the partner holds the reversed pattern.
-__device__ __forceinline__ void chain_patterns(const uint64_t *fwt, uint64_t s, uint32_t L, int ch,
-                                               uint32_t D, u128 &search, u128 &partner)
+// The D codes of a left-aligned pattern in reverse order, left-aligned.
+__device__ __forceinline__ u128 rev_pattern(u128 p, uint32_t D)
+{
+    u128 r = ((u128) rev_codes64((uint64_t) p) << 64) | rev_codes64((uint64_t) (p >> 64));
+    return r << (128 - 2 * D);
+}
+
+// The first D codes a chain has matched at depth D, left-aligned in 128 bits, as
+// searched in its index.  Chains as in k_find: FB n[L-D..L) in fw, FF
+// comp(n[L-D..L)) in bw, BF rev(n[0..D)) in bw, BB rc(n[0..D)) in fw.
+__device__ __forceinline__ u128 chain_pattern(const uint64_t *fwt, uint64_t s, uint32_t L, int ch,
+                                              uint32_t D)
 {
     const u128 mask = top_mask(D);
     u128 fwd = text_codes(fwt, (ch < 2) ? s + L - D : s) & mask;
-    u128 r = ((u128) rev_codes64((uint64_t) fwd) << 64) | rev_codes64((uint64_t) (fwd >> 64));
-    u128 rev = r << (128 - 2 * D);
     switch (ch) {
-    case 0: search = fwd; partner = rev; break;
-    case 1: search = fwd ^ mask; partner = rev ^ mask; break;
-    case 2: search = rev; partner = fwd; break;
-    default: search = rev ^ mask; partner = fwd ^ mask; break;
+    case 0: return fwd;
+    case 1: return fwd ^ mask;
+    case 2: return rev_pattern(fwd, D);
+    default: return rev_pattern(fwd, D) ^ mask;
+    }
+}
+
+// Link every entry of table A to the entry of its reversed pattern in table B
+// (the partner interval of a bidirectional search), found through B's model.
+// An entry whose reversed pattern is missing (never, for tables of one read
+// set) keeps LINK_NONE.
+__global__ void k_table_links(ModelView A, ModelView B, uint32_t D, MlpDims d, uint32_t leaves,
+                              uint32_t S, uint32_t *linkA, uint32_t *linkB)
+{
+    uint64_t u = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= A.nU)
+        return;
+    u128 pat = A.key[u] & ~(u128) 1;
+    u128 q = rev_pattern(pat, D) | 1u;
+    uint64_t g = model_entry(wave_params(B.iparams), B, d, leaves, S,
+                             (uint64_t) ((q & ~(u128) 1) >> (128 - 2 * d.k)));
+    uint32_t probes = 0;
+    uint64_t v = lower_bound_from(B.key, B.nU, q, g, probes);
+    if (v < B.nU && B.key[v] == q) {
+        linkA[u] = (uint32_t) v;
+        linkB[v] = (uint32_t) u;
+    } else {
+        linkA[u] = LINK_NONE;
     }
 }
 
@@ -588,23 +620,19 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
         const ModelView &Sv = fwsearch ? fw : bw;
         const ModelView &Pv = fwsearch ? bw : fw;
         if (D < L && Sv.nU > 0 && Pv.nU > 0) {
-            u128 pat, par;
-            chain_patterns(fwt, s, L, ch, D, pat, par);
-            const uint32_t sh = 128 - 2 * d.k;
-            const u128 key = pat | 1u, pkey = par | 1u;
+            const u128 pat = chain_pattern(fwt, s, L, ch, D);
+            const u128 key = pat | 1u;
             uint64_t g1 = model_entry(wave_params(Sv.iparams), Sv, d, leaves, S,
-                                      (uint64_t) (pat >> sh));
+                                      (uint64_t) (pat >> (128 - 2 * d.k)));
             uint64_t u = lower_bound_from(Sv.key, Sv.nU, key, g1, probes[sdir]);
             looks[sdir]++;
             win[sdir] = u > g1 ? u - g1 : g1 - u;
             if (u < Sv.nU && Sv.key[u] == key) {
+                // partner: the reversed pattern's entry in the other table,
+                // linked when the index was learned
                 uint2 iv = Sv.rows[u];
-                uint64_t g2 = model_entry(wave_params(Pv.iparams), Pv, d, leaves, S,
-                                          (uint64_t) (par >> sh));
-                uint64_t u2 = lower_bound_from(Pv.key, Pv.nU, pkey, g2, probes[1 - sdir]);
-                looks[1 - sdir]++;
-                win[1 - sdir] = u2 > g2 ? u2 - g2 : g2 - u2;
-                if (u2 < Pv.nU && Pv.key[u2] == pkey && Pv.rows[u2].y == iv.y) {
+                uint32_t u2 = Sv.link[u];
+                if (u2 != LINK_NONE && Pv.rows[u2].y == iv.y) {
                     sd.l = iv.x;
                     sd.size = iv.y;
                     sd.ll = Pv.rows[u2].x;
@@ -666,6 +694,7 @@ void pq_learned_free(pq_ctx_t *ctx)
         pq_free(M.iparams);
         pq_free(M.range);
         pq_free(M.knots);
+        pq_free(M.link);
     }
     pq_free(ctx->learned->qstats);
     delete ctx->learned;
@@ -960,6 +989,30 @@ static int train_dir(pq_ctx_t *ctx, Learned &LM, int d, const pq_learned_cfg_t &
     return PQ_OK;
 }
 
+static int build_links(pq_ctx_t *ctx, Learned &LM)
+{
+    int rc;
+    for (int d = 0; d < 2; d++)
+        if ((rc = pq_alloc(ctx, (void **) &LM.dir[d].link, (LM.dir[d].nU + 1) * 4)))
+            return rc;
+    PQ_CHECK(hipMemsetAsync(LM.dir[1].link, 0xFF, (LM.dir[1].nU + 1) * 4, ctx->stream));
+    ModelView v[2];
+    for (int d = 0; d < 2; d++) {
+        const DirModel &M = LM.dir[d];
+        v[d] = {M.key, M.rows, M.nU, M.iparams, M.range, M.knots, M.link};
+    }
+    MlpDims dims = {LM.k, LM.h};
+    if (LM.dir[0].nU && LM.dir[1].nU)
+        hipLaunchKernelGGL(k_table_links, dim3(pq_blocks(LM.dir[0].nU, 256)), dim3(256), 0,
+                           ctx->stream, v[0], v[1], LM.depth, dims, LM.leaves, LM.segs,
+                           LM.dir[0].link, LM.dir[1].link);
+    else
+        PQ_CHECK(hipMemsetAsync(LM.dir[0].link, 0xFF, (LM.dir[0].nU + 1) * 4, ctx->stream));
+    PQ_CHECK(hipGetLastError());
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    return PQ_OK;
+}
+
 extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
 {
     if (!ctx)
@@ -993,6 +1046,8 @@ extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
     int rc = pq_alloc(ctx, (void **) &LM->qstats, sizeof(QStats));
     for (int d = 0; d < 2 && rc == PQ_OK; d++)
         rc = train_dir(ctx, *LM, d, cfg);
+    if (rc == PQ_OK)
+        rc = build_links(ctx, *LM);
     if (rc == PQ_OK && hipMemsetAsync(LM->qstats, 0, sizeof(QStats), ctx->stream) != hipSuccess)
         rc = ctx->fail(PQ_ERR_HIP, "clearing learned statistics");
     if (rc == PQ_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
@@ -1046,7 +1101,7 @@ int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, Seed *d_seeds, uin
     ModelView v[2];
     for (int d = 0; d < 2; d++) {
         const DirModel &M = LM->dir[d];
-        v[d] = {M.key, M.rows, M.nU, M.iparams, M.range, M.knots};
+        v[d] = {M.key, M.rows, M.nU, M.iparams, M.range, M.knots, M.link};
     }
     MlpDims dims = {LM->k, LM->h};
     hipLaunchKernelGGL(k_learned_seed, dim3(pq_blocks(nr, 64)), dim3(256), 0, ctx->stream,
