@@ -410,35 +410,41 @@ __device__ __forceinline__ ExtWave ext_wave(char *base, uint32_t kmax, uint32_t 
 }
 
 // One group of GW lanes per read, storage in LDS, WAVES waves per block.  Reads
-// [0, nr) of the batch, or the nr reads of `list`; a read with more than KMAX
-// items on a side is appended to out.overflow_reads for the next tier.
+// [0, nr) of the batch, or (list != nullptr) the *count reads of `list`, taken
+// grid-stride so that the grid need not match the count known only on the
+// device.  A read with more than KMAX items on a side is appended to
+// out.overflow_reads for the next tier.
 template <uint32_t KMAX, int WAVES, int GW>
 __global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirView fw, DirView bw,
                                                            ChainBuf cb, uint64_t r0, uint64_t nr,
-                                                           const uint32_t *list, uint32_t kw,
+                                                           const uint32_t *list,
+                                                           const uint32_t *count, uint32_t kw,
                                                            ExtOut out)
 {
     extern __shared__ uint64_t lds[];
+    constexpr int GROUPS = WAVES * 64 / GW;
     const int grp = threadIdx.x / GW;
-    uint64_t i = (uint64_t) blockIdx.x * (WAVES * 64 / GW) + grp;
-    if (i >= nr)
-        return;
-    uint64_t rl = list ? list[i] : i;
+    const uint64_t total = list ? (uint64_t) *count : nr;
     ExtWave w = ext_wave((char *) lds + ext_wave_bytes(KMAX, kw) * grp, KMAX, kw);
-    bool ok = extract_read<GW>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf, w.pre,
-                               KMAX, out);
-    if (!ok && (threadIdx.x & (GW - 1)) == 0) {
-        uint32_t at = atomicAdd(out.n_overflow, 1u);
-        out.overflow_reads[at] = (uint32_t) rl;
+    for (uint64_t i = (uint64_t) blockIdx.x * GROUPS + grp; i < total;
+         i += (uint64_t) gridDim.x * GROUPS) {
+        uint64_t rl = list ? list[i] : i;
+        bool ok = extract_read<GW>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
+                                   w.pre, KMAX, out);
+        if (!ok && (threadIdx.x & (GW - 1)) == 0) {
+            uint32_t at = atomicAdd(out.n_overflow, 1u);
+            out.overflow_reads[at] = (uint32_t) rl;
+        }
     }
 }
 
 __global__ void __launch_bounds__(64) k_extract_global(Geom G, DirView fw, DirView bw, ChainBuf cb,
                                                        uint64_t r0, const uint32_t *list,
-                                                       uint32_t nlist, uint32_t kw, char *gscratch,
-                                                       ExtOut out)
+                                                       const uint32_t *count, uint32_t kw,
+                                                       char *gscratch, ExtOut out)
 {
     ExtWave w = ext_wave(gscratch + ext_wave_bytes(KMAX_GLOBAL, kw) * blockIdx.x, KMAX_GLOBAL, kw);
+    const uint32_t nlist = *count;
     for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
         uint64_t rl = list[i];
         bool ok = extract_read<64>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
@@ -446,6 +452,15 @@ __global__ void __launch_bounds__(64) k_extract_global(Geom G, DirView fw, DirVi
         if (!ok && threadIdx.x == 0)
             atomicOr(out.flags, 2u);
     }
+}
+
+// Zero `count` words (a kernel rather than a memset node, so that the sequence
+// stays capturable).
+__global__ void k_zero_u32(uint32_t *p, uint32_t count)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        p[i] = 0;
 }
 
 // tot[0..n) = edges per read, tot[n] = 0 so that the exclusive scan of n+1
@@ -528,16 +543,14 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     uint64_t edge_cap = std::max<uint64_t>(1 << 20, n / 4);
     Seed *seeds = nullptr;
     char *gscratch = nullptr;
-    hipEvent_t es, e0, e1, e2;
-    (void) hipEventCreate(&es);
-    (void) hipEventCreate(&e0);
-    (void) hipEventCreate(&e1);
-    (void) hipEventCreate(&e2);
+    // four events per batch (seed, find, extract boundaries), read after the loop
+    const uint64_t nbatch = (n + batch - 1) / batch;
+    std::vector<hipEvent_t> ev(4 * nbatch);
+    for (auto &e : ev)
+        (void) hipEventCreate(&e);
     auto cleanup = [&]() {
-        (void) hipEventDestroy(es);
-        (void) hipEventDestroy(e0);
-        (void) hipEventDestroy(e1);
-        (void) hipEventDestroy(e2);
+        for (auto &e : ev)
+            (void) hipEventDestroy(e);
     };
     cb.lc = nullptr;
     cb.o = nullptr;
@@ -552,6 +565,8 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         (rc = pq_slot(ctx, SLOT_TOT, (n + 1) * 8, &tot)) ||
         (rc = pq_slot(ctx, SLOT_OVL, (2 * batch + 2) * 4, &ovl)) ||
         (rc = pq_slot(ctx, SLOT_MISC, 64, &misc)) ||
+        (rc = pq_slot(ctx, SLOT_GSCRATCH, GLOBAL_SLOTS * ext_wave_bytes(KMAX_GLOBAL, kw),
+                      &gscratch)) ||
         (mode == PQ_SEARCH_LEARNED &&
          (rc = pq_slot(ctx, SLOT_SEEDS, batch * 4 * sizeof(Seed), &seeds)))) {
         cleanup();
@@ -572,7 +587,7 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     eo.flags = misc + 3;
     eo.overflow_reads = ovl;
     unsigned long long *fstats = (unsigned long long *) (misc + 8);   // misc[8..11]
-    (void) hipMemsetAsync(misc, 0, 64, ctx->stream);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc, 16u);
     // C[2..4] travel in Clo.y..w, C[5] on its own (C[0] = C[1] = 0)
     uint4 Clo = make_uint4(0, (uint32_t) ctx->C[2], (uint32_t) ctx->C[3], (uint32_t) ctx->C[4]);
     uint32_t C5 = (uint32_t) ctx->C[5];
@@ -581,66 +596,60 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     ExtOut eo2 = eo;                   // second tier: overflow goes to the tail list
     eo2.overflow_reads = ovl + batch;
     eo2.n_overflow = misc + 4;
-    for (uint64_t r0 = 0; r0 < n && rc == PQ_OK; r0 += batch) {
-        uint64_t nr = std::min(batch, n - r0);
+    // per batch: seeds, candidates, then the extraction tiers; the later tiers
+    // take their read lists and counts from the device (no host round trip)
+    const uint32_t grid2 = (uint32_t) std::min<uint64_t>(batch, 4096);
+    for (uint64_t b = 0; b < nbatch && rc == PQ_OK; b++) {
+        const uint64_t r0 = b * batch, nr = std::min(batch, n - r0);
+        hipEvent_t *e = &ev[4 * b];
         uint32_t kseed = 0;
-        (void) hipEventRecord(es, ctx->stream);
+        (void) hipEventRecord(e[0], ctx->stream);
         if (mode == PQ_SEARCH_LEARNED && (rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed)))
             break;
-        (void) hipEventRecord(e0, ctx->stream);
+        (void) hipEventRecord(e[1], ctx->stream);
         hipLaunchKernelGGL(k_find, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0, ctx->stream, G, fw,
                            bw, r0, nr, t, Clo, C5, seeds, kseed, cb, fstats);
-        (void) hipEventRecord(e1, ctx->stream);
+        (void) hipEventRecord(e[2], ctx->stream);
         hipLaunchKernelGGL((k_extract_lds<KMAX_LDS, EXT_WAVES, EXT_GW>),
                            dim3(pq_blocks(nr, EXT_WAVES * 64 / EXT_GW)), dim3(64 * EXT_WAVES), lds1,
-                           ctx->stream, G, fw, bw, cb, r0, nr, (const uint32_t *) nullptr, kw, eo);
+                           ctx->stream, G, fw, bw, cb, r0, nr, (const uint32_t *) nullptr,
+                           (const uint32_t *) nullptr, kw, eo);
+        hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64>), dim3(grid2), dim3(64), lds2,
+                           ctx->stream, G, fw, bw, cb, r0, nr, ovl, eo.n_overflow, kw, eo2);
+        hipLaunchKernelGGL(k_extract_global, dim3(GLOBAL_SLOTS), dim3(64), 0, ctx->stream, G, fw,
+                           bw, cb, r0, eo2.overflow_reads, eo2.n_overflow, kw, gscratch, eo);
+        // both overflow counters back to zero for the next batch (misc[2], misc[4])
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eo.n_overflow, 1u);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eo2.n_overflow, 1u);
         PQ_CHECK(hipGetLastError());
-        uint32_t nov[3] = {0, 0, 0};   // misc[2..4]: tier-1 overflow, flags, tier-2 overflow
-        PQ_CHECK(hipMemcpyAsync(nov, eo.n_overflow, 12, hipMemcpyDeviceToHost, ctx->stream));
-        PQ_CHECK(hipStreamSynchronize(ctx->stream));
-        if (nov[0]) {
-            hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64>), dim3(nov[0]), dim3(64), lds2,
-                               ctx->stream, G, fw, bw, cb, r0, (uint64_t) nov[0], ovl, kw, eo2);
-            PQ_CHECK(hipGetLastError());
-            PQ_CHECK(hipMemcpyAsync(&nov[2], eo2.n_overflow, 4, hipMemcpyDeviceToHost,
-                                    ctx->stream));
-            PQ_CHECK(hipMemsetAsync(eo.n_overflow, 0, 4, ctx->stream));
-            PQ_CHECK(hipStreamSynchronize(ctx->stream));
-        }
-        if (nov[2]) {
-            if (!gscratch && (rc = pq_slot(ctx, SLOT_GSCRATCH,
-                                           GLOBAL_SLOTS * ext_wave_bytes(KMAX_GLOBAL, kw),
-                                           &gscratch)))
-                break;
-            hipLaunchKernelGGL(k_extract_global, dim3(std::min<uint32_t>(GLOBAL_SLOTS, nov[2])),
-                               dim3(64), 0, ctx->stream, G, fw, bw, cb, r0, eo2.overflow_reads,
-                               nov[2], kw, gscratch, eo);
-            PQ_CHECK(hipGetLastError());
-            PQ_CHECK(hipMemsetAsync(eo2.n_overflow, 0, 4, ctx->stream));
-        }
-        (void) hipEventRecord(e2, ctx->stream);
-        PQ_CHECK(hipEventSynchronize(e2));
-        if (mode == PQ_SEARCH_LEARNED)
-            ctx->timing.learned_ms += ev_ms(es, e0);
-        double fms = ev_ms(e0, e1);
-        ctx->timing.find_ms += fms;
-        ctx->timing.find_kernel_ms += fms;
-        ctx->timing.find_launches++;
-        ctx->timing.extract_ms += ev_ms(e1, e2);
+        (void) hipEventRecord(e[3], ctx->stream);
     }
     uint32_t flags = 0;
-    unsigned long long used = 0, fst[2] = {0, 0};
+    unsigned long long used = 0;
     if (rc == PQ_OK) {
-        (void) hipMemcpy(fst, fstats, 16, hipMemcpyDeviceToHost);
-        ctx->timing.find_positions = fst[0];
-        ctx->timing.find_lines = fst[1];
-        (void) hipMemcpy(&flags, eo.flags, 4, hipMemcpyDeviceToHost);
-        (void) hipMemcpy(&used, eo.edge_count, 8, hipMemcpyDeviceToHost);
+        uint32_t hm[16];
+        PQ_CHECK(hipMemcpyAsync(hm, misc, 64, hipMemcpyDeviceToHost, ctx->stream));
+        PQ_CHECK(hipStreamSynchronize(ctx->stream));
+        for (uint64_t b = 0; b < nbatch; b++) {
+            hipEvent_t *e = &ev[4 * b];
+            if (mode == PQ_SEARCH_LEARNED)
+                ctx->timing.learned_ms += ev_ms(e[0], e[1]);
+            double fms = ev_ms(e[1], e[2]);
+            ctx->timing.find_ms += fms;
+            ctx->timing.find_kernel_ms += fms;
+            ctx->timing.find_launches++;
+            ctx->timing.extract_ms += ev_ms(e[2], e[3]);
+        }
+        used = (unsigned long long) hm[0] | ((unsigned long long) hm[1] << 32);
+        flags = hm[3];
+        ctx->timing.find_positions = (uint64_t) hm[8] | ((uint64_t) hm[9] << 32);
+        ctx->timing.find_lines = (uint64_t) hm[10] | ((uint64_t) hm[11] << 32);
         if (flags & 2u)
             rc = ctx->fail(PQ_ERR_LIMIT, "a read has more than 65536 overlap candidates on one side");
         else if (flags & 1u)
             rc = ctx->fail(PQ_ERR_LIMIT, "edge buffer exhausted");
     }
+    (void) used;
     if (rc == PQ_OK) {
         auto tc0 = std::chrono::steady_clock::now();
         hipLaunchKernelGGL(k_edge_totals, dim3(pq_blocks(n + 1, 256)), dim3(256), 0, ctx->stream,
