@@ -30,6 +30,7 @@ struct DirIndex {
     uint32_t *reads_index = nullptr;
     uint64_t nlines = 0;
     uint32_t cf[5] = {0, 0, 0, 0, 0};
+    uint8_t *bwt8 = nullptr;       // BWT symbol per row, between the SA and occ builds
     bool built = false;
 };
 
@@ -105,7 +106,7 @@ enum PqSlot {
     SLOT_CAND_LC = 20, SLOT_CAND_O, SLOT_CAND_N, SLOT_RECB, SLOT_RECF, SLOT_APPEND,
     SLOT_TOT, SLOT_OVL, SLOT_MISC, SLOT_SEEDS, SLOT_GSCRATCH, SLOT_STAGED,
     SLOT_OVB = 34, SLOT_OVF, SLOT_NUMB, SLOT_NUMF, SLOT_EDGE_OFF, SLOT_EDGES,
-    SLOT_LEARNED = 40, SLOT_PROBE = 50,
+    SLOT_LEARNED = 40, SLOT_BWT8 = 47, SLOT_PROBE = 50,
 };
 
 // Grow-only slot buffer of at least `bytes` bytes into *p, or PQ_ERR_NOMEM.

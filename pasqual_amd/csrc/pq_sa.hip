@@ -81,9 +81,19 @@ __global__ void k_bucket_keys(Geom G, const uint64_t *t2, uint64_t N, unsigned K
 #define SUF_NONE 0xFFFFFFFFu
 
 template <int W> struct SufW {
-    uint32_t g, len;
+    uint32_t g, len, sym;    // sym: BWT symbol of the row the suffix lands in
     uint64_t w[W];
 };
+
+// BWT symbol of the row holding suffix g: the code before g, or '$' (0) at a
+// read start.  Loaded with the suffix's other words, carried through the sort
+// and written next to the row it finalises.
+__device__ __forceinline__ unsigned bwt_sym(const Geom &G, const uint64_t *t2, uint32_t g)
+{
+    return pq_is_start(G, g) ? 0u : pq_code(t2, (uint64_t) g - 1) + 1u;
+}
+
+#define BWT_UNSET 0xFFu
 
 template <int W>
 __device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, uint32_t g,
@@ -91,6 +101,7 @@ __device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, uint
 {
     uint32_t rem = suf_rem(G, g);
     s.g = g;
+    s.sym = bwt_sym(G, t2, g);
     s.len = rem > base ? rem - base : 0;
 #pragma unroll
     for (int i = 0; i < W; i++) {
@@ -109,6 +120,7 @@ template <int W> __device__ __forceinline__ void suf_none(SufW<W> &s)
 {
     s.g = SUF_NONE;
     s.len = 0;
+    s.sym = BWT_UNSET;
 #pragma unroll
     for (int i = 0; i < W; i++)
         s.w[i] = 0;
@@ -145,6 +157,7 @@ template <int W> __device__ __forceinline__ SufW<W> suf_shfl_xor(const SufW<W> &
     SufW<W> o;
     o.g = __shfl_xor(s.g, j, 64);
     o.len = __shfl_xor(s.len, j, 64);
+    o.sym = __shfl_xor(s.sym, j, 64);
 #pragma unroll
     for (int i = 0; i < W; i++)
         o.w[i] = __shfl_xor(s.w[i], j, 64);
@@ -214,8 +227,8 @@ __device__ __forceinline__ void push_seg(const SegLists &S, bool valid, uint32_t
 // One group of G lanes per run (G = 16 or 64); bitonic network over lanes.
 template <int G, int W>
 __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2, uint32_t *sa,
-                                                     uint32_t base, const uint2 *segs,
-                                                     uint32_t nseg)
+                                                     uint8_t *bwt8, uint32_t base,
+                                                     const uint2 *segs, uint32_t nseg)
 {
     uint64_t tid = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t sid = tid / G;
@@ -241,8 +254,10 @@ __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2
                 s = p;
         }
     }
-    if (e)
+    if (e) {
         sa[sg.x + li] = s.g;
+        bwt8[sg.x + li] = (uint8_t) s.sym;
+    }
 }
 
 // Largest run one 256-thread block sorts in LDS for W words per suffix.
@@ -254,7 +269,7 @@ __host__ __device__ constexpr uint32_t lds_cap(int W)
 // One block per run of 65..lds_cap(W) elements: bitonic sort in LDS.
 template <int W>
 __global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, uint32_t *sa,
-                                                  uint32_t base, const uint2 *segs)
+                                                  uint8_t *bwt8, uint32_t base, const uint2 *segs)
 {
     constexpr uint32_t CAP = lds_cap(W);
     __shared__ SufW<W> sh[CAP];
@@ -285,8 +300,10 @@ __global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, u
             __syncthreads();
         }
     }
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         sa[sg.x + i] = sh[i].g;
+        bwt8[sg.x + i] = (uint8_t) sh[i].sym;
+    }
 }
 
 // ---- runs of equal keys ---------------------------------------------------------------
@@ -401,18 +418,18 @@ static void launch_sorts(pq_ctx_t *ctx, const Geom &G, DirIndex &D, uint32_t bas
     if (cnt[0])
         hipLaunchKernelGGL((k_sort_groups<RUN_SMALL, W>),
                            dim3(pq_blocks((uint64_t) cnt[0] * RUN_SMALL, 256)), dim3(256), 0,
-                           ctx->stream, G, D.text2, D.sa, base, items[0], cnt[0]);
+                           ctx->stream, G, D.text2, D.sa, D.bwt8, base, items[0], cnt[0]);
     if (cnt[1])
         hipLaunchKernelGGL((k_sort_groups<RUN_HALF, W>),
                            dim3(pq_blocks((uint64_t) cnt[1] * RUN_HALF, 256)), dim3(256), 0,
-                           ctx->stream, G, D.text2, D.sa, base, items[1], cnt[1]);
+                           ctx->stream, G, D.text2, D.sa, D.bwt8, base, items[1], cnt[1]);
     if (cnt[2])
         hipLaunchKernelGGL((k_sort_groups<RUN_WAVE, W>),
                            dim3(pq_blocks((uint64_t) cnt[2] * RUN_WAVE, 256)), dim3(256), 0,
-                           ctx->stream, G, D.text2, D.sa, base, items[2], cnt[2]);
+                           ctx->stream, G, D.text2, D.sa, D.bwt8, base, items[2], cnt[2]);
     if (cnt[3])
         hipLaunchKernelGGL(k_sort_lds<W>, dim3(cnt[3]), dim3(256), 0, ctx->stream, G, D.text2,
-                           D.sa, base, items[3]);
+                           D.sa, D.bwt8, base, items[3]);
 }
 
 static int sort_lists(pq_ctx_t *ctx, const Geom &G, DirIndex &D, uint32_t base,
@@ -465,8 +482,10 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d)
         (rc = pq_slot(ctx, SLOT_POS, (N + 1) * 4, &pos)) ||
         (rc = pq_slot(ctx, SLOT_STARTS, (N + 2) * 4, &starts)) ||
         (rc = pq_slot(ctx, SLOT_CLS, N + 1, &cls)) ||
-        (rc = pq_slot(ctx, SLOT_COUNTS, 64, &counts)))
+        (rc = pq_slot(ctx, SLOT_COUNTS, 64, &counts)) ||
+        (rc = pq_slot(ctx, SLOT_BWT8, N + 1, &D.bwt8)))
         goto done;
+    PQ_CHECK(hipMemsetAsync(D.bwt8, BWT_UNSET, N, ctx->stream));
     {
         // 1. bucket by the first K codes (and the length below K)
         unsigned bits = 2 * K + 4, passes = (bits + 7) / 8;
@@ -588,20 +607,24 @@ done:
 
 // ---- BWT occurrence lines + reads_index ---------------------------------------------------
 
-__device__ __forceinline__ unsigned bwt_sym(const Geom &G, const uint64_t *t2, uint32_t g)
+// BWT symbols of the rows no sort wrote (one-suffix buckets, buckets of
+// identical complete suffixes, rows placed by the chunk-level radix passes).
+__global__ void k_bwt_fill(Geom G, const uint64_t *t2, const uint32_t *sa, uint64_t N,
+                           uint8_t *bwt8)
 {
-    return pq_is_start(G, g) ? 0u : pq_code(t2, (uint64_t) g - 1) + 1u;
+    uint64_t row = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (row < N && bwt8[row] == BWT_UNSET)
+        bwt8[row] = (uint8_t) bwt_sym(G, t2, sa[row]);
 }
 
 // 256 threads = 2 lines; each wave builds one 64-row half of a line.
-__global__ void __launch_bounds__(256) k_occ_planes(Geom G, const uint64_t *t2, const uint32_t *sa,
-                                                    uint64_t N, uint64_t nlines, OccLine *occ,
-                                                    uint4 *cnts)
+__global__ void __launch_bounds__(256) k_occ_planes(const uint8_t *bwt8, uint64_t N,
+                                                    uint64_t nlines, OccLine *occ, uint4 *cnts)
 {
     __shared__ uint32_t part[4][4];
     uint64_t row = (uint64_t) blockIdx.x * 256 + threadIdx.x;
     int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    unsigned sym = row < N ? bwt_sym(G, t2, sa[row]) : 7u;
+    unsigned sym = row < N ? bwt8[row] : 7u;
     uint64_t b0 = __ballot(sym & 1u), b1 = __ballot(sym & 2u), b2 = __ballot(sym & 4u);
     uint64_t line = row >> 7;
     int half = (int) ((row >> 6) & 1);
@@ -668,8 +691,12 @@ int pq_build_occ_dir(pq_ctx_t *ctx, int d)
     if (!cnts)
         return ctx->fail(PQ_ERR_NOMEM, "occ scratch");
     uint4 *excl = cnts + D.nlines;
+    if (!D.bwt8)
+        return ctx->fail(PQ_ERR_STATE, "BWT symbols missing: suffix array not built");
+    hipLaunchKernelGGL(k_bwt_fill, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, G, D.text2,
+                       D.sa, N, D.bwt8);
     hipLaunchKernelGGL(k_occ_planes, dim3(pq_blocks(D.nlines * 128, 256)), dim3(256), 0,
-                       ctx->stream, G, D.text2, D.sa, N, D.nlines, D.occ, cnts);
+                       ctx->stream, D.bwt8, N, D.nlines, D.occ, cnts);
     uint32_t tot[4];
     if ((rc = pq_scan_u4(ctx, cnts, excl, D.nlines, tot)))
         return rc;
@@ -678,6 +705,7 @@ int pq_build_occ_dir(pq_ctx_t *ctx, int d)
     hipLaunchKernelGGL(k_reads_index, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, G,
                        D.sa, D.occ, N, D.reads_index);
     PQ_CHECK(hipGetLastError());
+    D.bwt8 = nullptr;   // the byte array is scratch shared by both directions
     // C (index.c:366-372): code counts of the text = BWT counts + reads ending
     // (fw) / starting (bw) with the code, whose last/first code is not a BWT row.
     const uint64_t *fix = d == 0 ? ctx->sfx : ctx->pfx;
