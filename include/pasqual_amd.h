@@ -160,6 +160,23 @@ int pq_mlp_gradients(pq_ctx_t *ctx, uint32_t k, uint32_t hidden, const float *pa
                      const uint64_t *xs, const float *targets, uint32_t n, float *grad,
                      float *loss);
 
+/* ---- autotuner ------------------------------------------------------------------------- */
+/* The first extraction tier processes one read per group of lanes holding up to
+ * `items` overlap candidates per read side in LDS (larger sides fall through to
+ * wider tiers); the best shape depends on the read set.  pq_autotune runs the
+ * search (mode as pq_find_overlaps) once per shape on the uploaded, indexed reads,
+ * keeps the fastest for later searches of this context and leaves that shape's
+ * (identical) result in place.  pq_set_tuning forces a shape. */
+typedef struct {
+    uint32_t shape;          /* chosen shape index */
+    uint32_t group_lanes;    /* lanes per read of that shape (8, 16, 32) */
+    uint32_t items;          /* candidates per side held in LDS (16, 32, 64) */
+    uint32_t nshapes;
+    double extract_ms[4];    /* extraction time measured per shape */
+} pq_tune_t;
+int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out);
+int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape);
+
 /* ---- timing ---------------------------------------------------------------------------- */
 typedef struct {
     double upload_ms;

@@ -33,7 +33,7 @@ PQ_SYMBOLS = [
     "pq_get_shortcut", "pq_num_reads", "pq_text_len", "pq_find_overlaps", "pq_graph_sizes",
     "pq_graph_copy", "pq_learned_default", "pq_learned_train", "pq_learned_stats",
     "pq_learned_params", "pq_learned_table", "pq_mlp_gradients",
-    "pq_get_timing", "pq_sync", "pq_probe_copy", "pq_synth_preset", "pq_synth_num_reads", "pq_synth_genome",
+    "pq_get_timing", "pq_autotune", "pq_set_tuning", "pq_sync", "pq_probe_copy", "pq_synth_preset", "pq_synth_num_reads", "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
 ]
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
@@ -84,6 +84,11 @@ class LearnedStats(C.Structure):
                 ("probes", C.c_uint64 * 2), ("table_size", C.c_uint64 * 2),
                 ("absent", C.c_uint64), ("fallback", C.c_uint64),
                 ("depth", C.c_uint32), ("k", C.c_uint32)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("shape", C.c_uint32), ("group_lanes", C.c_uint32), ("items", C.c_uint32),
+                ("nshapes", C.c_uint32), ("extract_ms", C.c_double * 4)]
 
 
 class Timing(C.Structure):
@@ -142,6 +147,8 @@ def lib(width="u32"):
         "pq_learned_table": (I32, [P, I32, P, P, P, P]),
         "pq_mlp_gradients": (I32, [P, U32, U32, P, P, P, U32, P, P]),
         "pq_get_timing": (I32, [P, P]),
+        "pq_autotune": (I32, [P, I32, P]),
+        "pq_set_tuning": (I32, [P, U32]),
         "pq_sync": (I32, [P]),
         "pq_probe_copy": (I32, [P, U64, P]),
         "pq_synth_preset": (I32, [I32, P]),
@@ -318,6 +325,15 @@ class Context:
                                           _ptr(eo), _ptr(dst), _ptr(ln), _ptr(ty)), "graph_copy")
         return {"overlaps_B": ovB[:n], "overlaps_F": ovF[:n], "num_B": nB[:n], "num_F": nF[:n],
                 "edge_off": eo, "dest": dst[:E], "len": ln[:E], "type": ty[:E]}
+
+    def autotune(self, mode=SEARCH_PLAIN):
+        t = Tune()
+        self._check(self._L.pq_autotune(self._h, mode, C.byref(t)), "autotune")
+        return {"shape": t.shape, "group_lanes": t.group_lanes, "items": t.items,
+                "extract_ms": list(t.extract_ms)[:t.nshapes]}
+
+    def set_tuning(self, shape):
+        self._check(self._L.pq_set_tuning(self._h, shape), "set_tuning")
 
     def learned_train(self, **kw):
         cfg = LearnedCfg()

@@ -61,6 +61,7 @@ struct pq_ctx {
     bool searched = false;
 
     Learned *learned = nullptr;
+    uint32_t ext_shape = 1;        // first extraction tier: index into the tuned shapes
     pq_timing_t timing{};
 
     std::vector<DevBuf> scratch;   // reusable temporaries, by slot

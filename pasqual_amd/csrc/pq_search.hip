@@ -27,13 +27,15 @@
 #include "pq_ctx.h"
 
 #include <chrono>
+#include <cstring>
 
 // extraction tiers by items per read side: most reads fit the first (small LDS
 // footprint, high occupancy); the few larger ones go to one wave per block with a
 // large LDS share, and the rest to global-memory scratch
-#define KMAX_LDS 32          // first tier: a quarter wave per read
-#define EXT_GW 16
 #define EXT_WAVES 4
+// first-tier shapes the autotuner chooses from: lanes per read, items per side
+static const uint32_t EXT_SHAPES[3][2] = {{8, 16}, {16, 32}, {32, 64}};
+#define EXT_NSHAPES 3
 #define KMAX_LDS2 512
 #define KMAX_GLOBAL 65536
 #define GLOBAL_SLOTS 256
@@ -591,7 +593,10 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     // C[2..4] travel in Clo.y..w, C[5] on its own (C[0] = C[1] = 0)
     uint4 Clo = make_uint4(0, (uint32_t) ctx->C[2], (uint32_t) ctx->C[3], (uint32_t) ctx->C[4]);
     uint32_t C5 = (uint32_t) ctx->C[5];
-    const size_t lds1 = ext_wave_bytes(KMAX_LDS, kw) * (EXT_WAVES * 64 / EXT_GW);
+    const uint32_t shape = ctx->ext_shape < EXT_NSHAPES ? ctx->ext_shape : 1;
+    const uint32_t gw1 = EXT_SHAPES[shape][0], kmax1 = EXT_SHAPES[shape][1];
+    const uint32_t groups1 = EXT_WAVES * 64 / gw1;
+    const size_t lds1 = ext_wave_bytes(kmax1, kw) * groups1;
     const size_t lds2 = ext_wave_bytes(KMAX_LDS2, kw);
     ExtOut eo2 = eo;                   // second tier: overflow goes to the tail list
     eo2.overflow_reads = ovl + batch;
@@ -610,10 +615,21 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         hipLaunchKernelGGL(k_find, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0, ctx->stream, G, fw,
                            bw, r0, nr, t, Clo, C5, seeds, kseed, cb, fstats);
         (void) hipEventRecord(e[2], ctx->stream);
-        hipLaunchKernelGGL((k_extract_lds<KMAX_LDS, EXT_WAVES, EXT_GW>),
-                           dim3(pq_blocks(nr, EXT_WAVES * 64 / EXT_GW)), dim3(64 * EXT_WAVES), lds1,
-                           ctx->stream, G, fw, bw, cb, r0, nr, (const uint32_t *) nullptr,
-                           (const uint32_t *) nullptr, kw, eo);
+        const dim3 g1(pq_blocks(nr, groups1)), b1(64 * EXT_WAVES);
+        const uint32_t *nl = nullptr;
+        switch (shape) {
+        case 0:
+            hipLaunchKernelGGL((k_extract_lds<16, EXT_WAVES, 8>), g1, b1, lds1, ctx->stream, G, fw,
+                               bw, cb, r0, nr, nl, nl, kw, eo);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_extract_lds<64, EXT_WAVES, 32>), g1, b1, lds1, ctx->stream, G,
+                               fw, bw, cb, r0, nr, nl, nl, kw, eo);
+            break;
+        default:
+            hipLaunchKernelGGL((k_extract_lds<32, EXT_WAVES, 16>), g1, b1, lds1, ctx->stream, G,
+                               fw, bw, cb, r0, nr, nl, nl, kw, eo);
+        }
         hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64>), dim3(grid2), dim3(64), lds2,
                            ctx->stream, G, fw, bw, cb, r0, nr, ovl, eo.n_overflow, kw, eo2);
         hipLaunchKernelGGL(k_extract_global, dim3(GLOBAL_SLOTS), dim3(64), 0, ctx->stream, G, fw,
@@ -717,5 +733,49 @@ extern "C" int pq_graph_copy(pq_ctx_t *ctx, int32_t *overlaps_B, int32_t *overla
                 type[i] = (uint8_t) (pk[i] & 0xFF);
         }
     }
+    return PQ_OK;
+}
+
+// Autotuner: time the search once per first-tier extraction shape on the
+// uploaded reads and keep the fastest (every shape gives the same result; reads
+// that do not fit a shape go to the next tier).
+extern "C" int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    double best = 1e300;
+    uint32_t pick = 1;
+    pq_tune_t t;
+    memset(&t, 0, sizeof(t));
+    for (uint32_t sh = 0; sh < EXT_NSHAPES; sh++) {
+        ctx->ext_shape = sh;
+        int rc = pq_find_overlaps(ctx, mode);
+        if (rc)
+            return rc;
+        double ms = ctx->timing.extract_ms;
+        t.extract_ms[sh] = ms;
+        if (ms < best) {
+            best = ms;
+            pick = sh;
+        }
+    }
+    ctx->ext_shape = pick;
+    t.shape = pick;
+    t.group_lanes = EXT_SHAPES[pick][0];
+    t.items = EXT_SHAPES[pick][1];
+    t.nshapes = EXT_NSHAPES;
+    if (out)
+        *out = t;
+    // leave the search result of the chosen shape in place
+    return pq_find_overlaps(ctx, mode);
+}
+
+extern "C" int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (shape >= EXT_NSHAPES)
+        return ctx->fail(PQ_ERR_ARG, "no such extraction shape");
+    ctx->ext_shape = shape;
     return PQ_OK;
 }

@@ -103,3 +103,23 @@ def test_graph_edge_cases(ctx):
         ctx.build_index()
         ctx.find_overlaps()
         _same_graph(ctx.graph(), orc.search_explicit(codes, off, t), "edge case")
+
+
+def test_every_extraction_shape_gives_the_same_graph(ctx):
+    # the autotuner's first-tier shapes (8, 16, 32 lanes per read) only change
+    # where reads are processed; reads with more candidates than a shape holds
+    # fall through to the wider tiers
+    rng = np.random.default_rng(17)
+    codes, off, _ = _random_reads(rng, 3000, 2000, 30, 120, err=0.002)
+    ctx.upload(codes, off, 21)
+    ctx.build_index()
+    want = orc.search_explicit(codes, off, 21)
+    assert max(want["overlaps_F"].max(), want["overlaps_B"].max()) > 16
+    for shape in range(3):
+        ctx.set_tuning(shape)
+        ctx.find_overlaps()
+        _same_graph(ctx.graph(), want, f"shape {shape}")
+    t = ctx.autotune()
+    assert 0 <= t["shape"] < 3 and len(t["extract_ms"]) == 3
+    _same_graph(ctx.graph(), want, "autotuned")
+    ctx.set_tuning(1)
