@@ -565,7 +565,7 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         (rc = pq_slot(ctx, SLOT_APPEND, edge_cap * 8, &app)) ||
         (rc = pq_slot(ctx, SLOT_STAGED, (n + 1) * 2 * EDGE_SLOT * 8, &staged)) ||
         (rc = pq_slot(ctx, SLOT_TOT, (n + 1) * 8, &tot)) ||
-        (rc = pq_slot(ctx, SLOT_OVL, (2 * batch + 2) * 4, &ovl)) ||
+        (rc = pq_slot(ctx, SLOT_OVL, (3 * batch + 3) * 4, &ovl)) ||
         (rc = pq_slot(ctx, SLOT_MISC, 64, &misc)) ||
         (rc = pq_slot(ctx, SLOT_GSCRATCH, GLOBAL_SLOTS * ext_wave_bytes(KMAX_GLOBAL, kw),
                       &gscratch)) ||
@@ -598,9 +598,17 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     const uint32_t groups1 = EXT_WAVES * 64 / gw1;
     const size_t lds1 = ext_wave_bytes(kmax1, kw) * groups1;
     const size_t lds2 = ext_wave_bytes(KMAX_LDS2, kw);
-    ExtOut eo2 = eo;                   // second tier: overflow goes to the tail list
-    eo2.overflow_reads = ovl + batch;
-    eo2.n_overflow = misc + 4;
+    // tiers after the first: (32 lanes, 64 items, 8 reads per block) unless the
+    // first tier already has that shape, then (64 lanes, 512 items, one read per
+    // block), then global scratch; each takes the previous one's overflow list
+    // and count: list i at ovl + i * batch, counts misc[2], misc[4], misc[6]
+    ExtOut eoB = eo, eoC = eo;
+    eoB.overflow_reads = ovl + batch;
+    eoB.n_overflow = misc + 4;
+    eoC.overflow_reads = ovl + 2 * batch;
+    eoC.n_overflow = misc + 6;
+    const size_t ldsB = ext_wave_bytes(64, kw) * (EXT_WAVES * 64 / 32);
+    const uint32_t gridB = (uint32_t) std::min<uint64_t>(pq_blocks(batch, EXT_WAVES * 2), 4096);
     // per batch: seeds, candidates, then the extraction tiers; the later tiers
     // take their read lists and counts from the device (no host round trip)
     const uint32_t grid2 = (uint32_t) std::min<uint64_t>(batch, 4096);
@@ -630,13 +638,22 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
             hipLaunchKernelGGL((k_extract_lds<32, EXT_WAVES, 16>), g1, b1, lds1, ctx->stream, G,
                                fw, bw, cb, r0, nr, nl, nl, kw, eo);
         }
+        const uint32_t *inC = eo.overflow_reads, *cntC = eo.n_overflow;
+        if (shape < 2) {
+            hipLaunchKernelGGL((k_extract_lds<64, EXT_WAVES, 32>), dim3(gridB), b1, ldsB,
+                               ctx->stream, G, fw, bw, cb, r0, nr, eo.overflow_reads,
+                               eo.n_overflow, kw, eoB);
+            inC = eoB.overflow_reads;
+            cntC = eoB.n_overflow;
+        }
         hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64>), dim3(grid2), dim3(64), lds2,
-                           ctx->stream, G, fw, bw, cb, r0, nr, ovl, eo.n_overflow, kw, eo2);
+                           ctx->stream, G, fw, bw, cb, r0, nr, inC, cntC, kw, eoC);
         hipLaunchKernelGGL(k_extract_global, dim3(GLOBAL_SLOTS), dim3(64), 0, ctx->stream, G, fw,
-                           bw, cb, r0, eo2.overflow_reads, eo2.n_overflow, kw, gscratch, eo);
-        // both overflow counters back to zero for the next batch (misc[2], misc[4])
+                           bw, cb, r0, eoC.overflow_reads, eoC.n_overflow, kw, gscratch, eo);
+        // overflow counters back to zero for the next batch (misc[2], misc[4], misc[6])
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eo.n_overflow, 1u);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eo2.n_overflow, 1u);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eoB.n_overflow, 1u);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eoC.n_overflow, 1u);
         PQ_CHECK(hipGetLastError());
         (void) hipEventRecord(e[3], ctx->stream);
     }
