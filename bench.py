@@ -62,6 +62,7 @@ def cpu_baseline_reference(fasta, L, t, n_expected):
         r = subprocess.run([exe, "--singlereads", fasta, "-l", str(L), "-t", str(t)], cwd=wd,
                            capture_output=True, text=True, env=env, timeout=600)
         wall = time.time() - t0
+        digest = _digest(os.path.join(wd, "contigs.fa"))
     if r.returncode != 0:
         log("reference run failed: " + r.stderr[-500:])
         return None
@@ -76,7 +77,44 @@ def cpu_baseline_reference(fasta, L, t, n_expected):
                       f"OMP_NUM_THREADS={threads}, one run, overlap-graph timer "
                       "(string_graph.c:737-1044)",
             "sa_build_gsuffix_per_s": round(2 * n * L / ((fw + bw) / 1e3) / 1e9, 5),
-            "wall_s": round(wall, 1), "reads_match_gpu": n == n_expected}
+            "wall_s": round(wall, 1), "reads_match_gpu": n == n_expected,
+            "contigs_sha256": digest}
+
+
+def _digest(path):
+    import hashlib
+    if not os.path.exists(path):
+        return None
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def dropin_run(fasta, L, t):
+    """The reference's own driver, assembler and host files linked against this
+    library (oracle/_ref/pasqual_hip_u32, built by `make -C oracle dropin`) on
+    the same reads: its 'Building overlap graph' timer covers the device search,
+    the copy of the edges and the host node build, like the reference's
+    (string_graph.c:737-1044)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pq_oracle as orc
+    exe = os.path.join(orc.REFDIR, "pasqual_hip_u32")
+    if not os.path.exists(exe):
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    with tempfile.TemporaryDirectory() as wd:
+        r = subprocess.run([exe, "--singlereads", fasta, "-l", str(L), "-t", str(t)], cwd=wd,
+                           capture_output=True, text=True, env=env, timeout=600)
+        digest = _digest(os.path.join(wd, "contigs.fa"))
+    if r.returncode != 0:
+        log("drop-in run failed: " + r.stderr[-500:])
+        return None
+    takes = [float(x) for x in re.findall(r"takes ([0-9.]+) ms", r.stdout)]
+    n = int(re.search(r"has imported (\d+) reads", r.stdout).group(1))
+    return {"reads": n, "import_ms": takes[0], "fw_index_ms": takes[1], "bw_index_ms": takes[2],
+            "graph_ms": takes[3], "assembly_ms": takes[4],
+            "graph_reads_per_s": round(n / (takes[3] / 1e3), 1), "host_threads": threads,
+            "contigs_sha256": digest}
 
 
 def cpu_baseline_port(pq, config, t):
@@ -256,11 +294,17 @@ def main():
                 "avg_launch_ms": round(avg_launch_ms, 3),
                 "launches_per_search": round(launches / args.steps, 2)}
 
-    base = None
+    base = dropin = None
     if args.cpu_baseline == "full":
         log("timing the CPU baseline ...")
         if L is not None:
             base = cpu_baseline_reference(fasta, L, t, n)
+            log("timing the reference driver on this library ...")
+            dropin = dropin_run(fasta, L, t)
+            if dropin and base:
+                dropin["contigs_identical_to_reference"] = (
+                    dropin["contigs_sha256"] is not None and
+                    dropin["contigs_sha256"] == base["contigs_sha256"])
         else:
             base = cpu_baseline_port(pq, args.config, t)
     headline_learned = args.config == 4
@@ -291,6 +335,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": base,
         "learned": learned,
+        "dropin": dropin,
     }
     if learned:
         line["learned_vs_plain_search"] = round(learned["reads_per_s"] / reads_per_s, 3)
