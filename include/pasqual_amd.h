@@ -98,6 +98,12 @@ int pq_graph_copy(pq_ctx_t *ctx, int32_t *overlaps_B, int32_t *overlaps_F,
                   uint32_t *num_B, uint32_t *num_F, uint64_t *edge_off,
                   uint64_t *dest, uint16_t *len, uint8_t *type);
 
+/* The edges of the last search in the reference's edge_t layout (string_graph.h:
+ * INDEX_TYPE_T dest, unsigned short overlap_len, char type; 8 bytes per edge for
+ * index_bytes 4, 16 for 8), in edge_off order: what construct_string_graph copies
+ * into the node records. */
+int pq_graph_copy_edges(pq_ctx_t *ctx, void *edges, uint32_t index_bytes);
+
 /* ---- learned index --------------------------------------------------------------------- */
 /* Counterpart of the reference's interval shortcut (sufsort.c:179-189, used at
  * string_graph.c:203-212, 289-298): every chain of the overlap search resumes at

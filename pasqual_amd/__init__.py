@@ -33,7 +33,7 @@ PQ_SYMBOLS = [
     "pq_get_shortcut", "pq_num_reads", "pq_text_len", "pq_find_overlaps", "pq_graph_sizes",
     "pq_graph_copy", "pq_learned_default", "pq_learned_train", "pq_learned_stats",
     "pq_learned_params", "pq_learned_table", "pq_mlp_gradients",
-    "pq_get_timing", "pq_autotune", "pq_set_tuning", "pq_sync", "pq_probe_copy", "pq_synth_preset", "pq_synth_num_reads", "pq_synth_genome",
+    "pq_get_timing", "pq_autotune", "pq_set_tuning", "pq_graph_copy_edges", "pq_sync", "pq_probe_copy", "pq_synth_preset", "pq_synth_num_reads", "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
 ]
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
@@ -148,6 +148,7 @@ def lib(width="u32"):
         "pq_mlp_gradients": (I32, [P, U32, U32, P, P, P, U32, P, P]),
         "pq_get_timing": (I32, [P, P]),
         "pq_autotune": (I32, [P, I32, P]),
+        "pq_graph_copy_edges": (I32, [P, P, U32]),
         "pq_set_tuning": (I32, [P, U32]),
         "pq_sync": (I32, [P]),
         "pq_probe_copy": (I32, [P, U64, P]),

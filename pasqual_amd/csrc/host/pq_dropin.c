@@ -257,13 +257,13 @@ string_graph_t *construct_string_graph(index_t *ix)
     uint32_t *nB = malloc(sizeof(uint32_t) * (n + 1));
     uint32_t *nF = malloc(sizeof(uint32_t) * (n + 1));
     uint64_t *eo = malloc(sizeof(uint64_t) * (n + 1));
-    uint64_t *dst = malloc(sizeof(uint64_t) * (E + 1));
-    uint16_t *len = malloc(sizeof(uint16_t) * (E + 1));
-    uint8_t *typ = malloc(E + 1);
+    edge_t *all = malloc(sizeof(edge_t) * (E + 1));   /* every node's edges, CSR order */
     string_graph_t *g = malloc(sizeof(string_graph_t));
-    if (!ovB || !ovF || !nB || !nF || !eo || !dst || !len || !typ || !g)
+    if (!ovB || !ovF || !nB || !nF || !eo || !all || !g)
         dev_fail("graph buffers", PQ_ERR_NOMEM);
-    if ((rc = pq_graph_copy(g_ctx, ovB, ovF, nB, nF, eo, dst, len, typ)))
+    if ((rc = pq_graph_copy(g_ctx, ovB, ovF, nB, nF, eo, NULL, NULL, NULL)))
+        dev_fail("graph copy", rc);
+    if ((rc = pq_graph_copy_edges(g_ctx, all, (uint32_t) sizeof(INDEX_TYPE_T))))
         dev_fail("graph copy", rc);
     g->nodes = calloc(n ? n : 1, sizeof(node_t));
     if (!g->nodes)
@@ -278,7 +278,8 @@ string_graph_t *construct_string_graph(index_t *ix)
     unsigned long long overlaps = 0;
     double overlap_len = 0.0;
     const int RL = ix->read_len;
-    /* node records, string_graph.c:832-1021 */
+    /* node records, string_graph.c:832-1021; edges are malloc'ed per node because
+     * assemble.c frees them one node at a time */
 #pragma omp parallel for schedule(dynamic, 4096) reduction(+ : overlaps, overlap_len)
     for (long long ii = 0; ii < (long long) n; ii++) {
         uint64_t i = (uint64_t) ii;
@@ -289,6 +290,8 @@ string_graph_t *construct_string_graph(index_t *ix)
         nd->status = PQ_REF_DEFAULT;
         nd->path_q = NULL;
         nd->edges = malloc((nb + nf) * sizeof(edge_t));
+        if (nb + nf)
+            memcpy(nd->edges, all + eo[i], (nb + nf) * sizeof(edge_t));
         int a = ovB[i], b = ovF[i], mx = a > b ? a : b;
         double ratio = mx == 0 ? 0 : ((double) abs(a - b)) / mx;
         nd->marked = ratio > 0.6 ? PQ_REF_SUSPECT : 0;
@@ -309,13 +312,9 @@ string_graph_t *construct_string_graph(index_t *ix)
             pq->max_F_trans = pq->max_B_trans = 0;   /* never read by assemble.c */
             pq->path = malloc(sizeof(path_t) * (nb + nf));
         }
-        uint64_t e0 = eo[i];
         unsigned olen = 0;
         for (unsigned j = 0; j < nb + nf; j++) {
-            edge_t *ed = &nd->edges[j];
-            ed->dest = (INDEX_TYPE_T) dst[e0 + j];
-            ed->overlap_len = len[e0 + j];
-            ed->type = (char) typ[e0 + j];
+            const edge_t *ed = &nd->edges[j];
             olen += ed->overlap_len;
             if (pq) {
                 path_t *p = &pq->path[j];
@@ -337,9 +336,7 @@ string_graph_t *construct_string_graph(index_t *ix)
     free(nB);
     free(nF);
     free(eo);
-    free(dst);
-    free(len);
-    free(typ);
+    free(all);
     printf("  takes %.3lf ms\n", now_ms() - t0);   /* string_graph.c:1043 */
     printf("  average overlap length: %lf\n", g->average_overlap);
     fflush(stdout);

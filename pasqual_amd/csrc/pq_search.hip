@@ -796,3 +796,47 @@ extern "C" int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape)
     ctx->ext_shape = shape;
     return PQ_OK;
 }
+
+// Edges in the reference's edge_t layout (string_graph.h: INDEX_TYPE_T dest,
+// unsigned short overlap_len, char type, padded to the alignment of dest).
+__global__ void k_ref_edges(const uint64_t *pk, uint64_t E, uint32_t index_bytes, uint8_t *out)
+{
+    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E)
+        return;
+    const uint64_t e = pk[i];
+    const uint64_t dest = e >> 32;
+    const uint16_t len = (uint16_t) ((e >> 8) & 0xFFFF);
+    const uint8_t type = (uint8_t) (e & 0xFF);
+    if (index_bytes == 4) {
+        // 8 bytes: dest u32 | len u16 | type u8 | pad
+        uint2 v = make_uint2((uint32_t) dest, (uint32_t) len | ((uint32_t) type << 16));
+        ((uint2 *) out)[i] = v;
+    } else {
+        // 16 bytes: dest u64 | len u16 | type u8 | pad
+        uint4 v = make_uint4((uint32_t) dest, (uint32_t) (dest >> 32),
+                             (uint32_t) len | ((uint32_t) type << 16), 0u);
+        ((uint4 *) out)[i] = v;
+    }
+}
+
+extern "C" int pq_graph_copy_edges(pq_ctx_t *ctx, void *edges, uint32_t index_bytes)
+{
+    if (!ctx || !edges || (index_bytes != 4 && index_bytes != 8))
+        return PQ_ERR_ARG;
+    if (!ctx->searched)
+        return ctx->fail(PQ_ERR_STATE, "no search result");
+    const uint64_t E = ctx->n_edges;
+    if (E == 0)
+        return PQ_OK;
+    const size_t rec = index_bytes == 4 ? 8 : 16;
+    uint8_t *d = (uint8_t *) ctx->tmp(SLOT_EXPORT, E * rec);
+    if (!d)
+        return ctx->fail(PQ_ERR_NOMEM, "edge export scratch");
+    hipLaunchKernelGGL(k_ref_edges, dim3(pq_blocks(E, 256)), dim3(256), 0, ctx->stream,
+                       ctx->d_edges, E, index_bytes, d);
+    PQ_CHECK(hipGetLastError());
+    PQ_CHECK(hipMemcpyAsync(edges, d, E * rec, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    return PQ_OK;
+}
