@@ -221,6 +221,18 @@ def main():
     g = ctx.graph()
     edges = int(g["edge_off"][-1])
     log("plain steps done")
+    # the same search as one HIP graph (captured by the first, untimed call)
+    ctx.find_overlaps(pq.SEARCH_GRAPH)
+    graph_ms = []
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        ctx.find_overlaps(pq.SEARCH_GRAPH)
+        ctx.sync()
+        graph_ms.append((time.perf_counter() - a) * 1e3)
+    gg = ctx.graph()
+    graph = {"plain_search_ms": round(sum(graph_ms) / len(graph_ms), 3),
+             "plain_edges_identical": all(np.array_equal(gg[k], g[k]) for k in g)}
+    log("graph-mode steps done")
 
     # learned-index variant on the same index: train once on the built SA,
     # then the same number of timed searches seeded by it
@@ -243,6 +255,16 @@ def main():
             l_find.append(tm["find_kernel_ms"])
             l_pos += tm["find_positions"]
         st = ctx.learned_stats()
+        ctx.find_overlaps(pq.SEARCH_LEARNED | pq.SEARCH_GRAPH)
+        lg_ms = []
+        for _ in range(args.steps):
+            a = time.perf_counter()
+            ctx.find_overlaps(pq.SEARCH_LEARNED | pq.SEARCH_GRAPH)
+            ctx.sync()
+            lg_ms.append((time.perf_counter() - a) * 1e3)
+        gl = ctx.graph()
+        graph["learned_search_ms"] = round(sum(lg_ms) / len(lg_ms), 3)
+        graph["learned_edges_identical"] = all(np.array_equal(gl[k], g[k]) for k in g)
         avg_l = sum(l_ms) / len(l_ms)
         lookups = sum(st["lookups"])
         learned = {"reads_per_s": round(n / (avg_l / 1e3), 1), "search_ms": round(avg_l, 3),
@@ -335,6 +357,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": base,
         "learned": learned,
+        "graph_mode": graph,
         "dropin": dropin,
     }
     if learned:

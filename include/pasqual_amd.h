@@ -47,6 +47,10 @@ extern "C" {
 
 #define PQ_SEARCH_PLAIN   0  /* bidirectional backward search from depth 1 */
 #define PQ_SEARCH_LEARNED 1  /* search seeded by the learned index */
+#define PQ_SEARCH_GRAPH 0x100 /* or'ed into the mode: run the search as one HIP graph,
+                                 captured on first use and replayed while the uploaded
+                                 reads, index buffers, learned model and tuning stay
+                                 the same (same result; no per-kernel timings) */
 
 typedef struct pq_ctx pq_ctx_t;
 

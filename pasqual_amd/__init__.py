@@ -24,6 +24,7 @@ PQ_OK = 0
 ERRORS = {-1: "PQ_ERR_ARG", -2: "PQ_ERR_IO", -3: "PQ_ERR_FORMAT", -4: "PQ_ERR_HIP",
           -5: "PQ_ERR_NOMEM", -6: "PQ_ERR_STATE", -7: "PQ_ERR_LIMIT"}
 SEARCH_PLAIN, SEARCH_LEARNED = 0, 1
+SEARCH_GRAPH = 0x100   # or'ed into a mode: search captured once as a HIP graph, then replayed
 BF, FB, FF, BB = 0, 1, 2, 3
 
 # exported symbols per header; the ABI test checks the library against these

@@ -81,6 +81,7 @@ extern "C" void pq_ctx_destroy(pq_ctx_t *ctx)
     (void) hipStreamSynchronize(ctx->stream);
     pq_index_release(ctx);
     pq_learned_free(ctx);   // search results live in scratch slots, freed below
+    pq_search_graph_free(ctx);
     for (DevBuf &b : ctx->scratch)
         pq_free(b.p);
     (void) hipStreamDestroy(ctx->stream);

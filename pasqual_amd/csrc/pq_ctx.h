@@ -62,6 +62,11 @@ struct pq_ctx {
 
     Learned *learned = nullptr;
     uint32_t ext_shape = 1;        // first extraction tier: index into the tuned shapes
+    uint64_t learned_gen = 0;      // bumped by every learned-index training
+    // search captured as a graph, valid while its key (launch arguments) holds
+    hipGraph_t search_graph = nullptr;
+    hipGraphExec_t search_exec = nullptr;
+    std::vector<uint64_t> search_key;
     pq_timing_t timing{};
 
     std::vector<DevBuf> scratch;   // reusable temporaries, by slot
@@ -126,6 +131,8 @@ int pq_scan_u32(pq_ctx_t *ctx, const uint32_t *in, uint32_t *out, uint64_t count
                 uint32_t *total_host);
 int pq_scan_u64(pq_ctx_t *ctx, const uint64_t *in, uint64_t *out, uint64_t count,
                 uint64_t *total_host);
+size_t pq_scan_scratch_bytes(uint64_t count, size_t elem);
+void pq_search_graph_free(pq_ctx_t *ctx);
 int pq_build_sa_dir(pq_ctx_t *ctx, int d);
 int pq_build_occ_dir(pq_ctx_t *ctx, int d);
 void pq_learned_free(pq_ctx_t *ctx);

@@ -680,6 +680,13 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
     }
 }
 
+__global__ void k_zero_u64(unsigned long long *p, uint32_t count)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        p[i] = 0;
+}
+
 // ---- host driver -------------------------------------------------------------------------
 
 void pq_learned_free(pq_ctx_t *ctx)
@@ -1043,6 +1050,7 @@ extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
     LM->leaves = cfg.leaves;
     LM->segs = cfg.segments;
     ctx->learned = LM;
+    ctx->learned_gen++;
     int rc = pq_alloc(ctx, (void **) &LM->qstats, sizeof(QStats));
     for (int d = 0; d < 2 && rc == PQ_OK; d++)
         rc = train_dir(ctx, *LM, d, cfg);
@@ -1096,8 +1104,10 @@ int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, Seed *d_seeds, uin
     Learned *LM = ctx->learned;
     if (!LM)
         return ctx->fail(PQ_ERR_STATE, "learned index not trained");
-    if (r0 == 0)
-        PQ_CHECK(hipMemsetAsync(LM->qstats, 0, sizeof(QStats), ctx->stream));
+    if (r0 == 0)   // a kernel, not a memset, so that the search can be captured
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream,
+                           (unsigned long long *) LM->qstats,
+                           (uint32_t) (sizeof(QStats) / sizeof(unsigned long long)));
     ModelView v[2];
     for (int d = 0; d < 2; d++) {
         const DirModel &M = LM->dir[d];

@@ -179,6 +179,12 @@ static int scan_impl(pq_ctx_t *ctx, const T *in, T *out, uint64_t count, T *tota
     return PQ_OK;
 }
 
+// scratch of a scan of count elements of elem bytes (slot 0)
+size_t pq_scan_scratch_bytes(uint64_t count, size_t elem)
+{
+    return ((count + SCAN_TILE - 1) / SCAN_TILE + 1) * elem;
+}
+
 // scratch slot 0 is reserved for scan partial sums
 int pq_scan_u32(pq_ctx_t *ctx, const uint32_t *in, uint32_t *out, uint64_t count,
                 uint32_t *total_host)

@@ -499,12 +499,99 @@ static double ev_ms(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+void pq_search_graph_free(pq_ctx_t *ctx)
+{
+    if (ctx->search_exec)
+        (void) hipGraphExecDestroy(ctx->search_exec);
+    if (ctx->search_graph)
+        (void) hipGraphDestroy(ctx->search_graph);
+    ctx->search_exec = nullptr;
+    ctx->search_graph = nullptr;
+    ctx->search_key.clear();
+}
+
+// Graph mode of pq_find_overlaps: the search sequence plus the edge CSR (into an
+// edge array sized for the worst case) as one graph; recaptured when any launch
+// argument it depends on changes.
+template <typename Enqueue>
+static int run_search_graph(pq_ctx_t *ctx, int mode, uint64_t n, uint64_t batch, uint64_t edge_cap,
+                            uint32_t *misc, uint64_t *rec, uint64_t *recF, uint64_t *staged,
+                            uint64_t *app, uint64_t *tot, Enqueue &enqueue,
+                            std::initializer_list<uint64_t> args)
+{
+    int rc;
+    const uint64_t ub = 2ull * EDGE_SLOT * n + edge_cap + 1;
+    if ((rc = pq_slot(ctx, SLOT_EDGES, ub * 8, &ctx->d_edges)))
+        return rc;
+    if (!ctx->tmp(SLOT_SCAN, pq_scan_scratch_bytes(n + 1, 8)))
+        return ctx->fail(PQ_ERR_NOMEM, "scan scratch");
+    std::vector<uint64_t> key = {(uint64_t) mode, ctx->ext_shape, n, batch, edge_cap,
+                                 mode == PQ_SEARCH_LEARNED ? ctx->learned_gen : 0,
+                                 (uint64_t) (uintptr_t) misc, (uint64_t) (uintptr_t) rec,
+                                 (uint64_t) (uintptr_t) recF, (uint64_t) (uintptr_t) staged,
+                                 (uint64_t) (uintptr_t) app, (uint64_t) (uintptr_t) tot,
+                                 (uint64_t) (uintptr_t) ctx->d_edges,
+                                 (uint64_t) (uintptr_t) ctx->d_edge_off,
+                                 (uint64_t) (uintptr_t) ctx->d_ovB, (uint64_t) (uintptr_t) ctx->d_ovF,
+                                 (uint64_t) (uintptr_t) ctx->d_numB,
+                                 (uint64_t) (uintptr_t) ctx->d_numF,
+                                 (uint64_t) (uintptr_t) ctx->scratch[SLOT_SCAN].p};
+    key.insert(key.end(), args.begin(), args.end());
+    for (int d = 0; d < 2; d++) {
+        const DirIndex &D = ctx->dir[d];
+        key.insert(key.end(), {(uint64_t) (uintptr_t) D.text2, (uint64_t) (uintptr_t) D.sa,
+                               (uint64_t) (uintptr_t) D.occ, (uint64_t) (uintptr_t) D.reads_index});
+        for (int c = 0; c < 5; c++)
+            key.push_back(D.cf[c]);
+    }
+    if (!ctx->search_exec || key != ctx->search_key) {
+        pq_search_graph_free(ctx);
+        PQ_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        rc = enqueue(false);
+        if (rc == PQ_OK) {
+            hipLaunchKernelGGL(k_edge_totals, dim3(pq_blocks(n + 1, 256)), dim3(256), 0,
+                               ctx->stream, ctx->d_numB, ctx->d_numF, n, tot);
+            rc = pq_scan_u64(ctx, tot, ctx->d_edge_off, n + 1, nullptr);
+        }
+        if (rc == PQ_OK)
+            hipLaunchKernelGGL(k_csr, dim3(pq_blocks(n, 256)), dim3(256), 0, ctx->stream, rec,
+                               recF, ctx->d_numB, ctx->d_numF, ctx->d_edge_off, staged, app, n,
+                               ctx->d_edges);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc)
+            return rc;
+        if (e != hipSuccess)
+            return ctx->fail(PQ_ERR_HIP, std::string("search capture: ") + hipGetErrorString(e));
+        ctx->search_graph = g;
+        PQ_CHECK(hipGraphInstantiate(&ctx->search_exec, g, nullptr, nullptr, 0));
+        ctx->search_key = key;
+    }
+    PQ_CHECK(hipGraphLaunch(ctx->search_exec, ctx->stream));
+    uint32_t hm[16];
+    uint64_t total = 0;
+    PQ_CHECK(hipMemcpyAsync(hm, misc, 64, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipMemcpyAsync(&total, ctx->d_edge_off + n, 8, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->timing.find_positions = (uint64_t) hm[8] | ((uint64_t) hm[9] << 32);
+    ctx->timing.find_lines = (uint64_t) hm[10] | ((uint64_t) hm[11] << 32);
+    if (hm[3] & 2u)
+        return ctx->fail(PQ_ERR_LIMIT, "a read has more than 65536 overlap candidates on one side");
+    if (hm[3] & 1u)
+        return ctx->fail(PQ_ERR_LIMIT, "edge buffer exhausted");
+    ctx->n_edges = total;
+    ctx->searched = true;
+    return PQ_OK;
+}
+
 extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
 {
     if (!ctx)
         return PQ_ERR_ARG;
     if (!ctx->dir[0].built || !ctx->dir[1].built)
         return ctx->fail(PQ_ERR_STATE, "index not built");
+    const bool as_graph = (mode & PQ_SEARCH_GRAPH) != 0;
+    mode &= ~PQ_SEARCH_GRAPH;
     if (mode != PQ_SEARCH_PLAIN && mode != PQ_SEARCH_LEARNED)
         return ctx->fail(PQ_ERR_ARG, "unknown search mode");
     if (mode == PQ_SEARCH_LEARNED && !ctx->learned)
@@ -589,10 +676,14 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     eo.flags = misc + 3;
     eo.overflow_reads = ovl;
     unsigned long long *fstats = (unsigned long long *) (misc + 8);   // misc[8..11]
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc, 16u);
+    // the whole search as a sequence of stream operations (captured into a graph
+    // in graph mode: kernels only, no memsets, no host synchronisation)
     // C[2..4] travel in Clo.y..w, C[5] on its own (C[0] = C[1] = 0)
-    uint4 Clo = make_uint4(0, (uint32_t) ctx->C[2], (uint32_t) ctx->C[3], (uint32_t) ctx->C[4]);
-    uint32_t C5 = (uint32_t) ctx->C[5];
+    const uint4 Clo = make_uint4(0, (uint32_t) ctx->C[2], (uint32_t) ctx->C[3], (uint32_t) ctx->C[4]);
+    const uint32_t C5 = (uint32_t) ctx->C[5];
+    auto enqueue = [&](bool timed) -> int {
+    int rc = PQ_OK;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc, 16u);
     const uint32_t shape = ctx->ext_shape < EXT_NSHAPES ? ctx->ext_shape : 1;
     const uint32_t gw1 = EXT_SHAPES[shape][0], kmax1 = EXT_SHAPES[shape][1];
     const uint32_t groups1 = EXT_WAVES * 64 / gw1;
@@ -616,13 +707,16 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         const uint64_t r0 = b * batch, nr = std::min(batch, n - r0);
         hipEvent_t *e = &ev[4 * b];
         uint32_t kseed = 0;
-        (void) hipEventRecord(e[0], ctx->stream);
+        if (timed)
+            (void) hipEventRecord(e[0], ctx->stream);
         if (mode == PQ_SEARCH_LEARNED && (rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed)))
             break;
-        (void) hipEventRecord(e[1], ctx->stream);
+        if (timed)
+            (void) hipEventRecord(e[1], ctx->stream);
         hipLaunchKernelGGL(k_find, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0, ctx->stream, G, fw,
                            bw, r0, nr, t, Clo, C5, seeds, kseed, cb, fstats);
-        (void) hipEventRecord(e[2], ctx->stream);
+        if (timed)
+            (void) hipEventRecord(e[2], ctx->stream);
         const dim3 g1(pq_blocks(nr, groups1)), b1(64 * EXT_WAVES);
         const uint32_t *nl = nullptr;
         switch (shape) {
@@ -655,8 +749,25 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eoB.n_overflow, 1u);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eoC.n_overflow, 1u);
         PQ_CHECK(hipGetLastError());
-        (void) hipEventRecord(e[3], ctx->stream);
+        if (timed)
+            (void) hipEventRecord(e[3], ctx->stream);
     }
+    return rc;
+    };
+    if (as_graph) {
+        rc = run_search_graph(ctx, mode, n, batch, edge_cap, misc, rec, recF, staged, app, tot,
+                              enqueue, {(uint64_t) (uintptr_t) cb.lc, (uint64_t) (uintptr_t) cb.o,
+                                        (uint64_t) (uintptr_t) cb.n, (uint64_t) (uintptr_t) seeds,
+                                        (uint64_t) (uintptr_t) gscratch, (uint64_t) (uintptr_t) ovl,
+                                        (uint64_t) t, (uint64_t) kw, (uint64_t) Clo.y,
+                                        (uint64_t) Clo.z, (uint64_t) Clo.w, (uint64_t) C5,
+                                        (uint64_t) (uintptr_t) G.off, (uint64_t) (uintptr_t) G.ends,
+                                        (uint64_t) (uintptr_t) G.end_rank, (uint64_t) G.L,
+                                        (uint64_t) G.Lmax, (uint64_t) G.N, (uint64_t) cb.cap});
+        cleanup();
+        return rc;
+    }
+    rc = enqueue(true);
     uint32_t flags = 0;
     unsigned long long used = 0;
     if (rc == PQ_OK) {
