@@ -96,27 +96,33 @@ __global__ void __launch_bounds__(256) k_find(Geom G, DirView fw, DirView bw, ui
     uint32_t nout = 0;
     uint64_t base = (rl * 4 + (uint64_t) ch) * out.cap;
     while (o < L && size >= minsize) {
-        Ranks a, b;
-        pq_ranks2(occ, l, (uint64_t) l + size, a, b);
+        // the next pattern symbol is read with the interval's lines
+        const bool more = o + 1 < L;
+        const unsigned c = more ? chain_sym(fw.text2, s, L, ch, o + 1) : 1u;
+        const uint64_t u1 = (uint64_t) l + size;
+        const uint64_t wl = (uint64_t) l >> 7, wu = u1 >> 7;
+        OccLine ln;
+        pq_load_line(occ, wl, ln);
+        const StepRanks a = pq_step_ranks(ln, wl, (unsigned) (l & 127), c);
+        if (wu != wl)
+            pq_load_line(occ, wu, ln);
+        const StepRanks b = pq_step_ranks(ln, wu, (unsigned) (u1 & 127), c);
         steps++;
-        lines += ((uint64_t) l >> 7) == (((uint64_t) l + size) >> 7) ? 1u : 2u;
+        lines += wl == wu ? 1u : 2u;
         if (o >= t) {
-            uint32_t cnt = b.r[0] - a.r[0];
+            uint32_t cnt = b.d - a.d;
             if (cnt && nout < out.cap) {
                 out.lc[base + nout] = make_uint2(ll, cnt);
                 out.o[base + nout] = (uint16_t) o;
                 nout++;
             }
         }
-        if (o + 1 >= L)
+        if (!more)
             break;
-        unsigned c = chain_sym(fw.text2, s, L, ch, o + 1);
-        uint32_t lt = 0;
-        for (unsigned q = 0; q < c; q++)
-            lt += b.r[q] - a.r[q];
-        ll += lt;
-        l = cf[c] + a.r[c];
-        size = b.r[c] - a.r[c];
+        const uint32_t cfc = c == 1 ? cf[1] : c == 2 ? cf[2] : c == 3 ? cf[3] : cf[4];
+        ll += b.lt - a.lt;
+        l = cfc + a.c;
+        size = b.c - a.c;
         o++;
     }
     out.n[rl * 4 + ch] = nout;
