@@ -66,6 +66,18 @@ const char *pq_last_error(const pq_ctx_t *ctx);
 int pq_upload_reads(pq_ctx_t *ctx, const uint8_t *codes, const uint64_t *off,
                     uint64_t n, uint32_t min_overlap);
 
+/* Device index layout, the runtime counterpart of the reference's INDEX_TYPE_T
+ * build choice (U32 / U64, common.h): row_bytes 4 or 8 is the width of suffix
+ * positions and BWT rows on the device, 0 (default) takes 4 below 2^32 text
+ * codes and 8 from there on; applies from the next pq_upload_reads.
+ * slice_suffixes bounds the suffixes sorted at once (0: 2^31, also the
+ * maximum); a larger text is sorted in key-range slices, which bounds the
+ * build's scratch memory; applies from the next pq_build_index.  Results do
+ * not depend on either. */
+int pq_set_layout(pq_ctx_t *ctx, uint32_t row_bytes, uint64_t slice_suffixes);
+/* Row width in use and the number of slices of the last build. */
+int pq_get_layout(pq_ctx_t *ctx, uint32_t *row_bytes, uint32_t *slices);
+
 /* ---- index build (index.c:408-706 + sufsort.c) ------------------------------------ */
 /* Builds both directions: suffix arrays, BWT occurrence lines, reads_index,
  * C and the end-of-read counts. */
@@ -131,6 +143,8 @@ typedef struct {
     float lr;                /* Adam step size (decays linearly to 1%) */
     uint32_t closed_form;    /* 1: solve the output layer by least squares before Adam */
     uint64_t seed;
+    uint64_t table_chunk;    /* suffix-array rows per step of the table build (16 bytes of
+                                scratch per row), 0: 2^28 */
 } pq_learned_cfg_t;
 
 typedef struct {
@@ -161,7 +175,7 @@ int pq_learned_params(pq_ctx_t *ctx, int dir, float *params, uint32_t *count);
  * keys: two uint64 per entry (high half first; first code in the top bits, 2 bits
  * per code, bit 0 of the low half set); first: first SA row; rows: row count.
  * *count in: capacity, out: number of entries (outputs may be NULL to query). */
-int pq_learned_table(pq_ctx_t *ctx, int dir, uint64_t *keys, uint32_t *first, uint32_t *rows,
+int pq_learned_table(pq_ctx_t *ctx, int dir, uint64_t *keys, uint64_t *first, uint32_t *rows,
                      uint64_t *count);
 /* Gradient of (1/n) sum (MLP(x_i) - y_i)^2 over n samples, by the device training
  * kernels; x_i are k-code prefix values (first code most significant), params as

@@ -34,7 +34,9 @@ PQ_SYMBOLS = [
     "pq_get_shortcut", "pq_num_reads", "pq_text_len", "pq_find_overlaps", "pq_graph_sizes",
     "pq_graph_copy", "pq_learned_default", "pq_learned_train", "pq_learned_stats",
     "pq_learned_params", "pq_learned_table", "pq_mlp_gradients",
-    "pq_get_timing", "pq_autotune", "pq_set_tuning", "pq_graph_copy_edges", "pq_sync", "pq_probe_copy", "pq_synth_preset", "pq_synth_num_reads", "pq_synth_genome",
+    "pq_get_timing", "pq_autotune", "pq_set_tuning", "pq_graph_copy_edges", "pq_sync",
+    "pq_probe_copy", "pq_set_layout", "pq_get_layout", "pq_synth_preset", "pq_synth_num_reads",
+    "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
 ]
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
@@ -75,7 +77,7 @@ class LearnedCfg(C.Structure):
     _fields_ = [("k", C.c_uint32), ("hidden", C.c_uint32), ("leaves", C.c_uint32),
                 ("segments", C.c_uint32), ("epochs", C.c_uint32), ("batch", C.c_uint32),
                 ("samples", C.c_uint32), ("lr", C.c_float), ("closed_form", C.c_uint32),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("table_chunk", C.c_uint64)]
 
 
 class LearnedStats(C.Structure):
@@ -153,6 +155,8 @@ def lib(width="u32"):
         "pq_set_tuning": (I32, [P, U32]),
         "pq_sync": (I32, [P]),
         "pq_probe_copy": (I32, [P, U64, P]),
+        "pq_set_layout": (I32, [P, U32, U64]),
+        "pq_get_layout": (I32, [P, P, P]),
         "pq_synth_preset": (I32, [I32, P]),
         "pq_synth_num_reads": (U64, [P]),
         "pq_synth_genome": (I32, [P, P]),
@@ -269,6 +273,18 @@ class Context:
     def build_index(self):
         self._check(self._L.pq_build_index(self._h), "build_index")
 
+    def set_layout(self, row_bytes=0, slice_suffixes=0):
+        """Row width 4 or 8 (0: by text size; from the next upload) and the most
+        suffixes sorted at once (0: 2^31; from the next build)."""
+        self._check(self._L.pq_set_layout(self._h, int(row_bytes), int(slice_suffixes)),
+                    "set_layout")
+
+    def layout(self):
+        """(row bytes in use, slices of the last build)."""
+        rb, sl = C.c_uint32(0), C.c_uint32(0)
+        self._check(self._L.pq_get_layout(self._h, C.byref(rb), C.byref(sl)), "get_layout")
+        return rb.value, sl.value
+
     def sa(self, d):
         out = np.empty(max(1, self.N), dtype=np.uint64)
         self._check(self._L.pq_get_sa(self._h, d, _ptr(out)), "get_sa")
@@ -368,7 +384,7 @@ class Context:
                     "learned_table")
         n = cnt.value
         keys = np.zeros((n, 2), np.uint64)
-        first = np.zeros(n, np.uint32)
+        first = np.zeros(n, np.uint64)
         rows = np.zeros(n, np.uint32)
         self._check(self._L.pq_learned_table(self._h, d, keys.ctypes.data, first.ctypes.data,
                                              rows.ctypes.data, C.byref(cnt)), "learned_table")

@@ -111,6 +111,45 @@ extern "C" int pq_sync(pq_ctx_t *ctx)
     return PQ_OK;
 }
 
+extern "C" int pq_set_layout(pq_ctx_t *ctx, uint32_t row_bytes, uint64_t slice_suffixes)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (row_bytes != 0 && row_bytes != 4 && row_bytes != 8)
+        return ctx->fail(PQ_ERR_ARG, "row_bytes must be 0, 4 or 8");
+    ctx->opt_row_bytes = row_bytes;
+    ctx->opt_slice = slice_suffixes;
+    return PQ_OK;
+}
+
+extern "C" int pq_get_layout(pq_ctx_t *ctx, uint32_t *row_bytes, uint32_t *slices)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (row_bytes)
+        *row_bytes = ctx->wide ? 8 : 4;
+    if (slices)
+        *slices = ctx->last_slices;
+    return PQ_OK;
+}
+
+// Scratch of the suffix-array build (a third of the memory of a slice for
+// 64-bit rows): freed after a build with 64-bit rows, whose texts are large
+// enough that the learned tables and the search need that memory.
+void pq_release_build_scratch(pq_ctx_t *ctx)
+{
+    const int slots[] = {SLOT_RADIX, SLOT_BIGK, SLOT_BIGV, SLOT_BK, SLOT_BK2, SLOT_V2,
+                         SLOT_FLAGS, SLOT_POS, SLOT_STARTS, SLOT_CLS, SLOT_LISTS, SLOT_SUBLISTS,
+                         SLOT_BWT8, SLOT_SLICE_HIST, SLOT_SLICE_TILES, SLOT_OCC};
+    (void) hipStreamSynchronize(ctx->stream);
+    for (int s : slots) {
+        if ((size_t) s < ctx->scratch.size() && ctx->scratch[s].p) {
+            pq_free(ctx->scratch[s].p);
+            ctx->scratch[s] = DevBuf();
+        }
+    }
+}
+
 extern "C" int pq_build_index(pq_ctx_t *ctx)
 {
     if (!ctx)
@@ -143,6 +182,8 @@ extern "C" int pq_build_index(pq_ctx_t *ctx)
     }
     for (auto &e : ev)
         (void) hipEventDestroy(e);
+    if (ctx->wide)
+        pq_release_build_scratch(ctx);
     return rc;
 }
 
@@ -150,15 +191,15 @@ extern "C" int pq_build_index(pq_ctx_t *ctx)
 
 __global__ void k_bwt_bytes(const OccLine *occ, uint64_t N, uint8_t *out)
 {
-    uint64_t row = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= N)
-        return;
-    const OccLine &ln = occ[row >> 7];
-    int h = (int) ((row >> 6) & 1);
-    unsigned j = (unsigned) (row & 63);
-    unsigned s = (unsigned) ((ln.p0[h] >> j) & 1) | (unsigned) (((ln.p1[h] >> j) & 1) << 1) |
-                 (unsigned) (((ln.p2[h] >> j) & 1) << 2);
-    out[row] = (uint8_t) s;
+    PQ_GRID_STRIDE_FOR(row, N)
+    {
+        const OccLine &ln = occ[row >> 7];
+        int h = (int) ((row >> 6) & 1);
+        unsigned j = (unsigned) (row & 63);
+        unsigned s = (unsigned) ((ln.p0[h] >> j) & 1) | (unsigned) (((ln.p1[h] >> j) & 1) << 1) |
+                     (unsigned) (((ln.p2[h] >> j) & 1) << 2);
+        out[row] = (uint8_t) s;
+    }
 }
 
 __global__ void k_occ_ref(const OccLine *occ, uint64_t rows, uint4 fix, uint64_t *bitmap,
@@ -194,17 +235,16 @@ __global__ void k_occ_ref(const OccLine *occ, uint64_t rows, uint4 fix, uint64_t
     }
 }
 
-__global__ void k_shortcut(Geom G, const uint64_t *fwt, uint64_t n, int dir, uint4 Clo,
-                           uint32_t C5, uint64_t *start, uint64_t *size, uint16_t *depth)
+__global__ void k_shortcut(Geom G, const uint64_t *fwt, uint64_t n, int dir, CTab<uint64_t> C,
+                           uint64_t *start, uint64_t *size, uint16_t *depth)
 {
     uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n)
         return;
     uint64_t s = pq_read_start(G, r), L = pq_read_len(G, r);
     unsigned c = pq_code(fwt, dir == 0 ? s + L - 1 : s) + 1;
-    const uint32_t Cc[6] = {0u, 0u, Clo.y, Clo.z, Clo.w, C5};
-    start[r] = Cc[c];
-    size[r] = Cc[c + 1] - Cc[c];
+    start[r] = pq_pick6(C.c, c);
+    size[r] = pq_pick6(C.c, c + 1) - pq_pick6(C.c, c);
     depth[r] = 1;
 }
 
@@ -224,6 +264,11 @@ extern "C" int pq_get_sa(pq_ctx_t *ctx, int dir, uint64_t *sa)
     int rc = dir_ok(ctx, dir);
     if (rc)
         return rc;
+    if (ctx->wide) {
+        if (ctx->N)
+            PQ_CHECK(hipMemcpy(sa, ctx->dir[dir].sa, ctx->N * 8, hipMemcpyDeviceToHost));
+        return PQ_OK;
+    }
     std::vector<uint32_t> h(ctx->N);
     if (ctx->N)
         PQ_CHECK(hipMemcpy(h.data(), ctx->dir[dir].sa, ctx->N * 4, hipMemcpyDeviceToHost));
@@ -242,7 +287,7 @@ extern "C" int pq_get_bwt(pq_ctx_t *ctx, int dir, uint8_t *bwt)
     uint8_t *d = (uint8_t *) ctx->tmp(4, ctx->N);
     if (!d)
         return ctx->fail(PQ_ERR_NOMEM, "bwt scratch");
-    hipLaunchKernelGGL(k_bwt_bytes, dim3(pq_blocks(ctx->N, 256)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_bwt_bytes, dim3(pq_blocks_gs(ctx->N, 256)), dim3(256), 0, ctx->stream,
                        ctx->dir[dir].occ, ctx->N, d);
     PQ_CHECK(hipMemcpyAsync(bwt, d, ctx->N, hipMemcpyDeviceToHost, ctx->stream));
     PQ_CHECK(hipStreamSynchronize(ctx->stream));
@@ -318,10 +363,8 @@ extern "C" int pq_get_shortcut(pq_ctx_t *ctx, int dir, uint64_t *start, uint64_t
         return ctx->fail(PQ_ERR_NOMEM, "shortcut scratch");
     uint64_t *ds = (uint64_t *) d, *dz = ds + n;
     uint16_t *dd = (uint16_t *) (dz + n);
-    uint4 Clo = make_uint4(0, (uint32_t) ctx->C[2], (uint32_t) ctx->C[3], (uint32_t) ctx->C[4]);
     hipLaunchKernelGGL(k_shortcut, dim3(pq_blocks(n, 256)), dim3(256), 0, ctx->stream,
-                       ctx->geom(), ctx->dir[0].text2, n, dir, Clo, (uint32_t) ctx->C[5], ds, dz,
-                       dd);
+                       ctx->geom(), ctx->dir[0].text2, n, dir, ctx->ctab<uint64_t>(), ds, dz, dd);
     PQ_CHECK(hipGetLastError());
     if (start)
         PQ_CHECK(hipMemcpyAsync(start, ds, n * 8, hipMemcpyDeviceToHost, ctx->stream));

@@ -25,11 +25,11 @@ struct DevBuf {
 // Per-direction device index.
 struct DirIndex {
     uint64_t *text2 = nullptr;     // packed direction text
-    uint32_t *sa = nullptr;
+    void *sa = nullptr;            // uint32_t or uint64_t positions (pq_ctx::wide)
     OccLine *occ = nullptr;
     uint32_t *reads_index = nullptr;
     uint64_t nlines = 0;
-    uint32_t cf[5] = {0, 0, 0, 0, 0};
+    uint64_t cf[5] = {0, 0, 0, 0, 0};
     uint8_t *bwt8 = nullptr;       // BWT symbol per row, between the SA and occ builds
     bool built = false;
 };
@@ -49,6 +49,12 @@ struct pq_ctx {
     uint64_t *d_off = nullptr;
     uint64_t *d_ends = nullptr;    // end-of-read bitmap
     uint32_t *d_end_rank = nullptr;
+    // rows and positions: 64-bit when the text has 2^32 codes or more (or when
+    // asked for); suffixes are sorted in slices of at most slice_cap
+    bool wide = false;
+    uint32_t opt_row_bytes = 0;    // 0 automatic, 4 or 8
+    uint64_t opt_slice = 0;        // 0 automatic (2^31)
+    uint32_t last_slices = 0;      // slices of the last suffix-array build
 
     DirIndex dir[2];
 
@@ -88,16 +94,23 @@ struct pq_ctx {
         g.end_rank = d_end_rank;
         return g;
     }
-    DirView view(int d) const
+    template <typename R> DirViewT<R> view(int d) const
     {
-        DirView v;
+        DirViewT<R> v;
         v.text2 = dir[d].text2;
-        v.sa = dir[d].sa;
+        v.sa = (const R *) dir[d].sa;
         v.occ = dir[d].occ;
         v.reads_index = dir[d].reads_index;
         for (int c = 0; c < 5; c++)
-            v.cf[c] = dir[d].cf[c];
+            v.cf[c] = (R) dir[d].cf[c];
         return v;
+    }
+    template <typename R> CTab<R> ctab() const
+    {
+        CTab<R> t;
+        for (int c = 0; c < 6; c++)
+            t.c[c] = (R) C[c];
+        return t;
     }
     // Scratch slot `slot` with at least `bytes` bytes (grown, never shrunk).
     void *tmp(int slot, size_t bytes);
@@ -112,7 +125,7 @@ enum PqSlot {
     SLOT_CAND_LC = 20, SLOT_CAND_O, SLOT_CAND_N, SLOT_RECB, SLOT_RECF, SLOT_APPEND,
     SLOT_TOT, SLOT_OVL, SLOT_MISC, SLOT_SEEDS, SLOT_GSCRATCH, SLOT_STAGED,
     SLOT_OVB = 34, SLOT_OVF, SLOT_NUMB, SLOT_NUMF, SLOT_EDGE_OFF, SLOT_EDGES,
-    SLOT_LEARNED = 40, SLOT_BWT8 = 47, SLOT_PROBE = 50,
+    SLOT_LEARNED = 40, SLOT_BWT8 = 47, SLOT_SLICE_HIST, SLOT_SLICE_TILES, SLOT_PROBE = 50,
 };
 
 // Grow-only slot buffer of at least `bytes` bytes into *p, or PQ_ERR_NOMEM.
@@ -132,6 +145,15 @@ int pq_scan_u32(pq_ctx_t *ctx, const uint32_t *in, uint32_t *out, uint64_t count
 int pq_scan_u64(pq_ctx_t *ctx, const uint64_t *in, uint64_t *out, uint64_t count,
                 uint64_t *total_host);
 size_t pq_scan_scratch_bytes(uint64_t count, size_t elem);
+// one stable 8-bit LSD radix pass of (key, value) pairs by the digit at shift
+int pq_radix_pass(pq_ctx_t *ctx, const uint64_t *kin, const uint32_t *vin, uint64_t *kout,
+                  uint32_t *vout, uint64_t count, unsigned shift);
+int pq_radix_pass(pq_ctx_t *ctx, const uint64_t *kin, const uint64_t *vin, uint64_t *kout,
+                  uint64_t *vout, uint64_t count, unsigned shift);
+int pq_radix_pass(pq_ctx_t *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                  uint32_t *vout, uint64_t count, unsigned shift);
+int pq_radix_pass(pq_ctx_t *ctx, const uint32_t *kin, const uint64_t *vin, uint32_t *kout,
+                  uint64_t *vout, uint64_t count, unsigned shift);
 void pq_search_graph_free(pq_ctx_t *ctx);
 int pq_build_sa_dir(pq_ctx_t *ctx, int d);
 int pq_build_occ_dir(pq_ctx_t *ctx, int d);
@@ -139,13 +161,24 @@ void pq_learned_free(pq_ctx_t *ctx);
 // Seed of one (read, chain) of the candidate search at the learned depth:
 // search interval [l, l + size) and partner interval start ll; size SEED_NONE
 // makes the chain start from depth 1 instead.
-struct Seed {
-    uint32_t l, size, ll;
+template <typename R> struct SeedT {
+    R l;
+    uint32_t size;
+    R ll;
 };
 #define SEED_NONE 0xFFFFFFFFu
-int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, Seed *d_seeds, uint32_t *kseed);
+// d_seeds: SeedT<uint32_t> or SeedT<uint64_t> per (read, chain) as ctx->wide says
+int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed);
+void pq_release_build_scratch(pq_ctx_t *ctx);
 
 static inline unsigned pq_blocks(uint64_t work, unsigned per)
 {
     return (unsigned) ((work + per - 1) / per);
+}
+
+// Blocks for a grid-stride kernel over `work` items: at most 2^20 blocks.
+static inline unsigned pq_blocks_gs(uint64_t work, unsigned per)
+{
+    uint64_t b = (work + per - 1) / per;
+    return (unsigned) (b < (1u << 20) ? (b ? b : 1) : (1u << 20));
 }

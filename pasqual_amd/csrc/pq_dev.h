@@ -12,7 +12,9 @@
 //   ends   : 1 bit per text position, set on the last code of every read
 //            (shared by both directions: a reversed read occupies the same
 //            range as the read).
-//   sa     : suffix array, uint32 positions.
+//   sa     : suffix array: positions of row type R, uint32_t while the text
+//            has fewer than 2^32 codes, uint64_t beyond (rows of the BWT and
+//            occ tables take the same type).
 //   occ    : one 64-byte OccLine per 128 BWT rows -- three bit planes of the
 //            3-bit row symbol ($=0 A=1 C=2 G=3 T=4, padding 7) plus the counts
 //            of A, C, G, T before the line.  A backward-search step touches the
@@ -31,14 +33,52 @@ struct __attribute__((aligned(64))) OccLine {
 };
 static_assert(sizeof(OccLine) == 64, "OccLine must be one 64-byte line");
 
-// Everything a search kernel needs about one direction.
-struct DirView {
+// Everything a search kernel needs about one direction, for row type R.
+template <typename R> struct DirViewT {
     const uint64_t *text2;
-    const uint32_t *sa;
+    const R *sa;
     const OccLine *occ;
     const uint32_t *reads_index;
-    uint32_t cf[5];          // backward-step base: C[c] + fix[c]; cf[0] = 0
+    R cf[5];                 // backward-step base: C[c] + fix[c]; cf[0] = 0
 };
+
+// A row with a count (an interval start and its size, a table entry's first
+// row and run length): 8 bytes for 32-bit rows, 16 for 64-bit rows, one load.
+template <typename R> struct __attribute__((aligned(2 * sizeof(R)))) RowCount {
+    R x;
+    uint32_t y;
+};
+
+// C[0..5] of index.c:366-372 in the row type.
+template <typename R> struct CTab {
+    R c[6];
+};
+
+// Element of C selected by a per-lane symbol without indexing (which would
+// move the table to LDS or scratch).
+template <typename R> __device__ __forceinline__ R pq_pick6(const R (&v)[6], unsigned c)
+{
+    R x = v[0];
+#pragma unroll
+    for (unsigned q = 1; q < 6; q++)
+        x = c == q ? v[q] : x;
+    return x;
+}
+
+template <typename R> __device__ __forceinline__ R pq_pick5(const R (&v)[5], unsigned c)
+{
+    R x = v[0];
+#pragma unroll
+    for (unsigned q = 1; q < 5; q++)
+        x = c == q ? v[q] : x;
+    return x;
+}
+
+// Grid-stride loop bounds for kernels over 2^32 elements or more: a launch
+// covers at most 2^32 work-items per dimension.
+#define PQ_GRID_STRIDE_FOR(i, count)                                                      \
+    for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < (count);   \
+         i += (uint64_t) gridDim.x * blockDim.x)
 
 // Read geometry: either a fixed read length or per-read offsets + end bitmap.
 struct Geom {

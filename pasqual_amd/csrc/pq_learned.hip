@@ -72,7 +72,7 @@ struct QStats {
 
 struct DirModel {
     u128 *key = nullptr;          // nU distinct keys
-    uint2 *rows = nullptr;        // (first SA row, rows) per key
+    void *rows = nullptr;         // RowCount<R>: (first SA row, rows) per key
     uint64_t nU = 0;
     float *params = nullptr;
     float *iparams = nullptr;     // inference form (rows padded to MLP_K_MAX)
@@ -322,56 +322,67 @@ __device__ __forceinline__ u128 top_mask(uint32_t D)
     return ~(u128) 0 << (128 - 2 * D);
 }
 
-// K[i] = first D codes of suffix SA[i] (codes past the end of its read read as 0)
-// | (suffix has at least D codes).  Non-decreasing along the SA.
-__global__ void k_learned_keys(Geom G, const uint64_t *t2, const uint32_t *sa, uint64_t N,
-                               uint32_t D, u128 *K)
+// K[j] = first D codes of suffix SA[r0 + j] (codes past the end of its read read
+// as 0) | (suffix has at least D codes), j < cnt.  Non-decreasing along the SA.
+template <typename R>
+__global__ void k_learned_keys(Geom G, const uint64_t *t2, const R *sa, uint64_t r0,
+                               uint64_t cnt, uint32_t D, u128 *K)
 {
-    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N)
+    uint64_t j = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cnt)
         return;
-    uint32_t g = sa[i];
+    uint64_t g = sa[r0 + j];
     uint64_t rem = pq_rem(G, g);
     // up to 32 codes come from one 64-bit window (two words instead of three)
     u128 w = D <= 32 ? (u128) pq_window(t2, g) << 64 : text_codes(t2, g);
     w &= top_mask(D);
     if (rem < D)
         w &= ~(~(u128) 0 >> (2 * rem));
-    K[i] = w | (u128) (rem >= D ? 1u : 0u);
+    K[j] = w | (u128) (rem >= D ? 1u : 0u);
 }
+
+// The table is built over chunks of SA rows [a, a + m): K holds the keys of
+// rows a - off .. a + m (one row before the chunk when a > 0, one after it
+// when it exists), so row a + i has key K[i + off].
 
 // First row of each run of equal full-length keys.
-__global__ void k_table_flags(const u128 *K, uint64_t N, uint32_t *flag)
+__global__ void k_table_flags(const u128 *K, uint64_t a, uint64_t m, uint32_t off, uint32_t *flag)
 {
     uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N)
+    if (i >= m)
         return;
-    u128 k = K[i];
-    flag[i] = ((uint32_t) k & 1u) && (i == 0 || K[i - 1] != k) ? 1u : 0u;
+    const u128 *k = K + off + i;
+    flag[i] = ((uint32_t) k[0] & 1u) && (a + i == 0 || k[-1] != k[0]) ? 1u : 0u;
 }
 
-__global__ void k_table_starts(const u128 *K, uint64_t N, const uint32_t *flag,
-                               const uint32_t *pos, u128 *key, uint2 *rows)
+template <typename R>
+__global__ void k_table_starts(const u128 *K, uint64_t a, uint64_t m, uint32_t off,
+                               const uint32_t *flag, const uint32_t *pos, uint64_t base, u128 *key,
+                               RowCount<R> *rows)
 {
     uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || !flag[i])
+    if (i >= m || !flag[i])
         return;
-    key[pos[i]] = K[i];
-    rows[pos[i]] = make_uint2((uint32_t) i, 0u);
+    const uint64_t u = base + pos[i];
+    key[u] = K[off + i];
+    rows[u] = RowCount<R>{(R) (a + i), 0u};
 }
 
-// Last row of a run: the run's entry is pos[i] + flag[i] - 1.
-__global__ void k_table_counts(const u128 *K, uint64_t N, const uint32_t *flag,
-                               const uint32_t *pos, uint2 *rows)
+// Last row of a run: the run's entry is base + pos[i] + flag[i] - 1 (an entry
+// of an earlier chunk when the run began there).
+template <typename R>
+__global__ void k_table_counts(const u128 *K, uint64_t a, uint64_t m, uint32_t off, uint64_t N,
+                               const uint32_t *flag, const uint32_t *pos, uint64_t base,
+                               RowCount<R> *rows)
 {
     uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N)
+    if (i >= m)
         return;
-    u128 k = K[i];
-    if (!((uint32_t) k & 1u) || (i + 1 < N && K[i + 1] == k))
+    const u128 *k = K + off + i;
+    if (!((uint32_t) k[0] & 1u) || (a + i + 1 < N && k[1] == k[0]))
         return;
-    uint32_t u = pos[i] + flag[i] - 1u;
-    rows[u].y = (uint32_t) (i + 1) - rows[u].x;
+    const uint64_t u = base + pos[i] + flag[i] - 1u;
+    rows[u].y = (uint32_t) ((R) (a + i + 1) - rows[u].x);
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
@@ -445,9 +456,9 @@ __global__ void k_leaf_knots(const u128 *key, uint64_t nU, uint32_t k, const uin
 
 // ---- query ------------------------------------------------------------------------------
 
-struct ModelView {
+template <typename R> struct ModelViewT {
     const u128 *key;
-    const uint2 *rows;
+    const RowCount<R> *rows;
     uint64_t nU;
     const float *iparams;
     const uint64_t *range;
@@ -465,7 +476,9 @@ __device__ __forceinline__ const cfloat *wave_params(const float *p)
     return (const cfloat *) (((uint64_t) hi << 32) | lo);
 }
 
-__device__ __forceinline__ uint64_t model_entry(const cfloat *params, const ModelView &M, MlpDims d,
+template <typename R>
+__device__ __forceinline__ uint64_t model_entry(const cfloat *params, const ModelViewT<R> &M,
+                                                MlpDims d,
                                                 uint32_t leaves, uint32_t S, uint64_t x)
 {
     uint32_t j = leaf_of(mlp_forward_padded(params, d, x), leaves);
@@ -576,8 +589,9 @@ __device__ __forceinline__ u128 chain_pattern(const uint64_t *fwt, uint64_t s, u
 // (the partner interval of a bidirectional search), found through B's model.
 // An entry whose reversed pattern is missing (never, for tables of one read
 // set) keeps LINK_NONE.
-__global__ void k_table_links(ModelView A, ModelView B, uint32_t D, MlpDims d, uint32_t leaves,
-                              uint32_t S, uint32_t *linkA, uint32_t *linkB)
+template <typename R>
+__global__ void k_table_links(ModelViewT<R> A, ModelViewT<R> B, uint32_t D, MlpDims d,
+                              uint32_t leaves, uint32_t S, uint32_t *linkA, uint32_t *linkB)
 {
     uint64_t u = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= A.nU)
@@ -600,10 +614,12 @@ __global__ void k_table_links(ModelView A, ModelView B, uint32_t D, MlpDims d, u
 // model of a wave is uniform.  Writes Seed{search interval, partner start} at
 // depth D for chains of reads longer than D (size 0 when the pattern does not
 // occur; SEED_NONE, the plain start, when the two tables disagree).
+template <typename R>
 __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fwt, uint64_t r0,
                                                       uint64_t nr, uint32_t D, MlpDims d,
-                                                      uint32_t leaves, uint32_t S, ModelView fw,
-                                                      ModelView bw, Seed *seeds, QStats *qs)
+                                                      uint32_t leaves, uint32_t S,
+                                                      ModelViewT<R> fw, ModelViewT<R> bw,
+                                                      SeedT<R> *seeds, QStats *qs)
 {
     const int ch = (int) (threadIdx.x >> 6);
     const uint64_t rl = (uint64_t) blockIdx.x * 64 + (threadIdx.x & 63);
@@ -616,9 +632,9 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
         uint64_t r = r0 + rl;
         uint64_t s = pq_read_start(G, r);
         uint32_t L = pq_read_len(G, r);
-        Seed sd = {0, SEED_NONE, 0};
-        const ModelView &Sv = fwsearch ? fw : bw;
-        const ModelView &Pv = fwsearch ? bw : fw;
+        SeedT<R> sd = {0, SEED_NONE, 0};
+        const ModelViewT<R> &Sv = fwsearch ? fw : bw;
+        const ModelViewT<R> &Pv = fwsearch ? bw : fw;
         if (D < L && Sv.nU > 0 && Pv.nU > 0) {
             const u128 pat = chain_pattern(fwt, s, L, ch, D);
             const u128 key = pat | 1u;
@@ -630,7 +646,7 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
             if (u < Sv.nU && Sv.key[u] == key) {
                 // partner: the reversed pattern's entry in the other table,
                 // linked when the index was learned
-                uint2 iv = Sv.rows[u];
+                RowCount<R> iv = Sv.rows[u];
                 uint32_t u2 = Sv.link[u];
                 if (u2 != LINK_NONE && Pv.rows[u2].y == iv.y) {
                     sd.l = iv.x;
@@ -722,6 +738,7 @@ extern "C" void pq_learned_default(pq_learned_cfg_t *cfg)
     cfg->lr = 1e-5f;
     cfg->closed_form = 1;
     cfg->seed = 12345;
+    cfg->table_chunk = 0;
 }
 
 static uint64_t host_mix(uint64_t z)
@@ -817,36 +834,73 @@ static int upload_inference_params(pq_ctx_t *ctx, const Learned &LM, DirModel &M
     return PQ_OK;
 }
 
-// Table of distinct full-length D-code prefixes of direction d, from its SA.
-static int build_table(pq_ctx_t *ctx, Learned &LM, int d)
+// Table of distinct full-length D-prefixes of direction d, from its SA, in
+// chunks of SA rows (the keys take 16 bytes per row): entries counted per
+// chunk, then written at the chunk's base (one pass when one chunk holds N).
+template <typename R>
+static int build_table_t(pq_ctx_t *ctx, Learned &LM, int d, uint64_t chunk)
 {
     DirModel &M = LM.dir[d];
     const uint64_t N = ctx->N;
-    u128 *K = (u128 *) ctx->tmp(SLOT_LEARNED + 3, N * sizeof(u128) + 16);
-    uint32_t *flag = (uint32_t *) ctx->tmp(SLOT_LEARNED + 4, N * 4 + 16);
-    uint32_t *pos = (uint32_t *) ctx->tmp(SLOT_LEARNED + 5, N * 4 + 16);
+    const uint64_t CH = std::min<uint64_t>(N, chunk ? chunk : (1ull << 28));
+    const uint64_t nch = (N + CH - 1) / CH;
+    u128 *K = (u128 *) ctx->tmp(SLOT_LEARNED + 3, (CH + 2) * sizeof(u128) + 16);
+    uint32_t *flag = (uint32_t *) ctx->tmp(SLOT_LEARNED + 4, CH * 4 + 16);
+    uint32_t *pos = (uint32_t *) ctx->tmp(SLOT_LEARNED + 5, CH * 4 + 16);
     if (!K || !flag || !pos)
         return ctx->fail(PQ_ERR_NOMEM, "learned index table scratch");
     Geom G = ctx->geom();
-    hipLaunchKernelGGL(k_learned_keys, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, G,
-                       ctx->dir[d].text2, ctx->dir[d].sa, N, LM.depth, K);
-    hipLaunchKernelGGL(k_table_flags, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, K, N,
-                       flag);
-    PQ_CHECK(hipGetLastError());
-    uint32_t nU = 0;
-    int rc = pq_scan_u32(ctx, flag, pos, N, &nU);
-    if (rc)
-        return rc;
+    const R *sa = (const R *) ctx->dir[d].sa;
+    auto chunk_keys = [&](uint64_t c, uint64_t &a, uint64_t &m, uint32_t &off) -> int {
+        a = c * CH;
+        m = std::min(CH, N - a);
+        off = a > 0 ? 1u : 0u;
+        const uint64_t r0 = a - off, cnt = std::min(a + m + 1, N) - r0;
+        hipLaunchKernelGGL(k_learned_keys<R>, dim3(pq_blocks(cnt, 256)), dim3(256), 0, ctx->stream,
+                           G, ctx->dir[d].text2, sa, r0, cnt, LM.depth, K);
+        hipLaunchKernelGGL(k_table_flags, dim3(pq_blocks(m, 256)), dim3(256), 0, ctx->stream, K,
+                           a, m, off, flag);
+        PQ_CHECK(hipGetLastError());
+        return PQ_OK;
+    };
+    std::vector<uint64_t> base(nch + 1, 0);
+    int rc;
+    uint64_t a = 0, m = 0;
+    uint32_t off = 0;
+    for (uint64_t c = 0; c < nch; c++) {
+        uint32_t cnt = 0;
+        if ((rc = chunk_keys(c, a, m, off)) || (rc = pq_scan_u32(ctx, flag, pos, m, &cnt)))
+            return rc;
+        base[c + 1] = base[c] + cnt;
+    }
+    const uint64_t nU = base[nch];
+    if (nU >= 0xFFFFFFFFull)
+        return ctx->fail(PQ_ERR_LIMIT, "learned table of 2^32 entries or more");
     M.nU = nU;
-    if ((rc = pq_alloc(ctx, (void **) &M.key, ((size_t) nU + 1) * sizeof(u128))) ||
-        (rc = pq_alloc(ctx, (void **) &M.rows, ((size_t) nU + 1) * sizeof(uint2))))
+    if ((rc = pq_alloc(ctx, (void **) &M.key, (nU + 1) * sizeof(u128))) ||
+        (rc = pq_alloc(ctx, (void **) &M.rows, (nU + 1) * sizeof(RowCount<R>))))
         return rc;
-    hipLaunchKernelGGL(k_table_starts, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, K, N,
-                       flag, pos, M.key, M.rows);
-    hipLaunchKernelGGL(k_table_counts, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, K, N,
-                       flag, pos, M.rows);
-    PQ_CHECK(hipGetLastError());
+    RowCount<R> *rows = (RowCount<R> *) M.rows;
+    for (uint64_t c = 0; c < nch; c++) {
+        // the last chunk's keys, flags and positions are still in place
+        if (c + 1 < nch || nch > 1) {
+            uint32_t cnt = 0;
+            if ((rc = chunk_keys(c, a, m, off)) || (rc = pq_scan_u32(ctx, flag, pos, m, &cnt)))
+                return rc;
+        }
+        hipLaunchKernelGGL(k_table_starts<R>, dim3(pq_blocks(m, 256)), dim3(256), 0, ctx->stream,
+                           K, a, m, off, flag, pos, base[c], M.key, rows);
+        hipLaunchKernelGGL(k_table_counts<R>, dim3(pq_blocks(m, 256)), dim3(256), 0, ctx->stream,
+                           K, a, m, off, N, flag, pos, base[c], rows);
+        PQ_CHECK(hipGetLastError());
+    }
     return PQ_OK;
+}
+
+static int build_table(pq_ctx_t *ctx, Learned &LM, int d, uint64_t chunk)
+{
+    return ctx->wide ? build_table_t<uint64_t>(ctx, LM, d, chunk)
+                     : build_table_t<uint32_t>(ctx, LM, d, chunk);
 }
 
 static int train_dir(pq_ctx_t *ctx, Learned &LM, int d, const pq_learned_cfg_t &cfg)
@@ -856,7 +910,7 @@ static int train_dir(pq_ctx_t *ctx, Learned &LM, int d, const pq_learned_cfg_t &
     const uint32_t P = mlp_nparams(LM.k, LM.h);
     const uint32_t S = cfg.samples;
     const uint32_t NL = LM.leaves, SG = LM.segs;
-    int rc = build_table(ctx, LM, d);
+    int rc = build_table(ctx, LM, d, cfg.table_chunk);
     if (rc)
         return rc;
     if ((rc = pq_alloc(ctx, (void **) &M.params, P * 4)) ||
@@ -996,21 +1050,22 @@ static int train_dir(pq_ctx_t *ctx, Learned &LM, int d, const pq_learned_cfg_t &
     return PQ_OK;
 }
 
-static int build_links(pq_ctx_t *ctx, Learned &LM)
+template <typename R> static ModelViewT<R> model_view(const DirModel &M)
+{
+    return {M.key, (const RowCount<R> *) M.rows, M.nU, M.iparams, M.range, M.knots, M.link};
+}
+
+template <typename R> static int build_links_t(pq_ctx_t *ctx, Learned &LM)
 {
     int rc;
     for (int d = 0; d < 2; d++)
         if ((rc = pq_alloc(ctx, (void **) &LM.dir[d].link, (LM.dir[d].nU + 1) * 4)))
             return rc;
     PQ_CHECK(hipMemsetAsync(LM.dir[1].link, 0xFF, (LM.dir[1].nU + 1) * 4, ctx->stream));
-    ModelView v[2];
-    for (int d = 0; d < 2; d++) {
-        const DirModel &M = LM.dir[d];
-        v[d] = {M.key, M.rows, M.nU, M.iparams, M.range, M.knots, M.link};
-    }
+    ModelViewT<R> v[2] = {model_view<R>(LM.dir[0]), model_view<R>(LM.dir[1])};
     MlpDims dims = {LM.k, LM.h};
     if (LM.dir[0].nU && LM.dir[1].nU)
-        hipLaunchKernelGGL(k_table_links, dim3(pq_blocks(LM.dir[0].nU, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_table_links<R>, dim3(pq_blocks(LM.dir[0].nU, 256)), dim3(256), 0,
                            ctx->stream, v[0], v[1], LM.depth, dims, LM.leaves, LM.segs,
                            LM.dir[0].link, LM.dir[1].link);
     else
@@ -1018,6 +1073,11 @@ static int build_links(pq_ctx_t *ctx, Learned &LM)
     PQ_CHECK(hipGetLastError());
     PQ_CHECK(hipStreamSynchronize(ctx->stream));
     return PQ_OK;
+}
+
+static int build_links(pq_ctx_t *ctx, Learned &LM)
+{
+    return ctx->wide ? build_links_t<uint64_t>(ctx, LM) : build_links_t<uint32_t>(ctx, LM);
 }
 
 extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
@@ -1099,7 +1159,18 @@ extern "C" int pq_learned_stats(pq_ctx_t *ctx, pq_learned_stats_t *st)
 
 // Launch the seed kernel for reads [r0, r0 + nr) of a search batch (stream
 // ordered, no synchronisation).  Statistics accumulate over one search.
-int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, Seed *d_seeds, uint32_t *kseed)
+template <typename R>
+static void launch_seeds(pq_ctx_t *ctx, const Learned *LM, uint64_t r0, uint64_t nr,
+                         SeedT<R> *d_seeds)
+{
+    ModelViewT<R> v[2] = {model_view<R>(LM->dir[0]), model_view<R>(LM->dir[1])};
+    MlpDims dims = {LM->k, LM->h};
+    hipLaunchKernelGGL(k_learned_seed<R>, dim3(pq_blocks(nr, 64)), dim3(256), 0, ctx->stream,
+                       ctx->geom(), ctx->dir[0].text2, r0, nr, LM->depth, dims, LM->leaves,
+                       LM->segs, v[0], v[1], d_seeds, LM->qstats);
+}
+
+int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed)
 {
     Learned *LM = ctx->learned;
     if (!LM)
@@ -1108,15 +1179,10 @@ int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, Seed *d_seeds, uin
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream,
                            (unsigned long long *) LM->qstats,
                            (uint32_t) (sizeof(QStats) / sizeof(unsigned long long)));
-    ModelView v[2];
-    for (int d = 0; d < 2; d++) {
-        const DirModel &M = LM->dir[d];
-        v[d] = {M.key, M.rows, M.nU, M.iparams, M.range, M.knots, M.link};
-    }
-    MlpDims dims = {LM->k, LM->h};
-    hipLaunchKernelGGL(k_learned_seed, dim3(pq_blocks(nr, 64)), dim3(256), 0, ctx->stream,
-                       ctx->geom(), ctx->dir[0].text2, r0, nr, LM->depth, dims, LM->leaves,
-                       LM->segs, v[0], v[1], d_seeds, LM->qstats);
+    if (ctx->wide)
+        launch_seeds<uint64_t>(ctx, LM, r0, nr, (SeedT<uint64_t> *) d_seeds);
+    else
+        launch_seeds<uint32_t>(ctx, LM, r0, nr, (SeedT<uint32_t> *) d_seeds);
     PQ_CHECK(hipGetLastError());
     *kseed = LM->depth;
     return PQ_OK;
@@ -1188,7 +1254,7 @@ extern "C" int pq_learned_params(pq_ctx_t *ctx, int dir, float *params, uint32_t
 // halves (high first; first code in the top bits, bit 0 of the low half set),
 // its first SA row and its row count.  *count in: capacity (entries), out: table
 // size (any output may be NULL to query it).
-extern "C" int pq_learned_table(pq_ctx_t *ctx, int dir, uint64_t *keys, uint32_t *first,
+extern "C" int pq_learned_table(pq_ctx_t *ctx, int dir, uint64_t *keys, uint64_t *first,
                                 uint32_t *rows, uint64_t *count)
 {
     if (!ctx || !count || dir < 0 || dir > 1)
@@ -1212,9 +1278,10 @@ extern "C" int pq_learned_table(pq_ctx_t *ctx, int dir, uint64_t *keys, uint32_t
             keys[2 * i + 1] = (uint64_t) k[i];
         }
     }
-    if (first || rows) {
-        std::vector<uint2> r(M.nU);
-        PQ_CHECK(hipMemcpyAsync(r.data(), M.rows, M.nU * sizeof(uint2), hipMemcpyDeviceToHost,
+    auto copy_rows = [&](auto tag) -> int {
+        using RC = decltype(tag);
+        std::vector<RC> r(M.nU);
+        PQ_CHECK(hipMemcpyAsync(r.data(), M.rows, M.nU * sizeof(RC), hipMemcpyDeviceToHost,
                                 ctx->stream));
         PQ_CHECK(hipStreamSynchronize(ctx->stream));
         for (uint64_t i = 0; i < M.nU; i++) {
@@ -1223,6 +1290,12 @@ extern "C" int pq_learned_table(pq_ctx_t *ctx, int dir, uint64_t *keys, uint32_t
             if (rows)
                 rows[i] = r[i].y;
         }
+        return PQ_OK;
+    };
+    if (first || rows) {
+        int rc = ctx->wide ? copy_rows(RowCount<uint64_t>{}) : copy_rows(RowCount<uint32_t>{});
+        if (rc)
+            return rc;
     }
     return PQ_OK;
 }

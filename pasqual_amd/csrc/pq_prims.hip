@@ -206,19 +206,25 @@ int pq_scan_u4(pq_ctx_t *ctx, const void *in, void *out, uint64_t count, void *t
 // ---- LSD radix pass ------------------------------------------------------------
 
 #define RADIX_THREADS 256
-#define RADIX_ROUNDS 16
-#define RADIX_TILE (RADIX_THREADS * RADIX_ROUNDS)
 
-template <typename KT>
+// Items per thread of a radix tile: 16, or 8 when key + value exceed 12 bytes
+// (the tile is staged in LDS).
+template <typename KT, typename VT> constexpr int radix_rounds()
+{
+    return sizeof(KT) + sizeof(VT) > 12 ? 8 : 16;
+}
+
+template <typename KT, int ROUNDS>
 __global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(const KT *keys, uint64_t count,
                                                              unsigned shift, uint32_t *hist,
                                                              uint32_t nblocks)
 {
+    constexpr int RADIX_TILE = RADIX_THREADS * ROUNDS;
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
     uint64_t base = (uint64_t) blockIdx.x * RADIX_TILE;
-    for (int r = 0; r < RADIX_ROUNDS; r++) {
+    for (int r = 0; r < ROUNDS; r++) {
         uint64_t i = base + (uint64_t) r * RADIX_THREADS + threadIdx.x;
         if (i < count)
             atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
@@ -227,15 +233,17 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(const KT *keys, ui
     hist[(uint64_t) threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-template <typename KT>
+template <typename KT, typename VT, int ROUNDS>
 __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
-    const KT *kin, const uint32_t *vin, KT *kout, uint32_t *vout, uint64_t count,
+    const KT *kin, const VT *vin, KT *kout, VT *vout, uint64_t count,
     unsigned shift, const uint32_t *offs, uint32_t nblocks)
 {
+    constexpr int RADIX_TILE = RADIX_THREADS * ROUNDS;
+    constexpr int RADIX_ROUNDS = ROUNDS;
     __shared__ uint32_t wcnt[4][256];
     __shared__ uint32_t tstart[256];
     __shared__ KT sk[RADIX_TILE];
-    __shared__ uint32_t sv[RADIX_TILE];
+    __shared__ VT sv[RADIX_TILE];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < 4 * 256; i += RADIX_THREADS)
         (&wcnt[0][0])[i] = 0;
@@ -245,7 +253,7 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
     const uint64_t wbase = tile + (uint64_t) wid * (RADIX_TILE / 4);
     const uint64_t lt = (1ull << lane) - 1ull;
     KT key[RADIX_ROUNDS];
-    uint32_t val[RADIX_ROUNDS];
+    VT val[RADIX_ROUNDS];
     uint32_t rnk[RADIX_ROUNDS];
 #pragma unroll
     for (int r = 0; r < RADIX_ROUNDS; r++) {
@@ -317,10 +325,12 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
 }
 
 // One stable pass by the 8-bit digit at `shift`.  Scratch slot 1 (histogram).
-template <typename KT>
-static int radix_pass_t(pq_ctx_t *ctx, const KT *kin, const uint32_t *vin, KT *kout,
-                        uint32_t *vout, uint64_t count, unsigned shift)
+template <typename KT, typename VT>
+static int radix_pass_t(pq_ctx_t *ctx, const KT *kin, const VT *vin, KT *kout, VT *vout,
+                        uint64_t count, unsigned shift)
 {
+    constexpr int ROUNDS = radix_rounds<KT, VT>();
+    constexpr int RADIX_TILE = RADIX_THREADS * ROUNDS;
     if (count == 0)
         return PQ_OK;
     if (count > 0xFFFFFFFFull)
@@ -330,13 +340,13 @@ static int radix_pass_t(pq_ctx_t *ctx, const KT *kin, const uint32_t *vin, KT *k
     if (!hist)
         return ctx->fail(PQ_ERR_NOMEM, "radix histogram");
     uint32_t *offs = hist + (size_t) nb * 256;
-    hipLaunchKernelGGL(k_radix_hist<KT>, dim3(nb), dim3(RADIX_THREADS), 0, ctx->stream, kin,
-                       count, shift, hist, nb);
+    hipLaunchKernelGGL((k_radix_hist<KT, ROUNDS>), dim3(nb), dim3(RADIX_THREADS), 0, ctx->stream,
+                       kin, count, shift, hist, nb);
     int rc = pq_scan_u32(ctx, hist, offs, (uint64_t) nb * 256, nullptr);
     if (rc)
         return rc;
-    hipLaunchKernelGGL(k_radix_scatter<KT>, dim3(nb), dim3(RADIX_THREADS), 0, ctx->stream, kin,
-                       vin, kout, vout, count, shift, offs, nb);
+    hipLaunchKernelGGL((k_radix_scatter<KT, VT, ROUNDS>), dim3(nb), dim3(RADIX_THREADS), 0,
+                       ctx->stream, kin, vin, kout, vout, count, shift, offs, nb);
     PQ_CHECK(hipGetLastError());
     return PQ_OK;
 }
@@ -344,11 +354,23 @@ static int radix_pass_t(pq_ctx_t *ctx, const KT *kin, const uint32_t *vin, KT *k
 int pq_radix_pass(pq_ctx_t *ctx, const uint64_t *kin, const uint32_t *vin, uint64_t *kout,
                   uint32_t *vout, uint64_t count, unsigned shift)
 {
-    return radix_pass_t<uint64_t>(ctx, kin, vin, kout, vout, count, shift);
+    return radix_pass_t<uint64_t, uint32_t>(ctx, kin, vin, kout, vout, count, shift);
 }
 
-int pq_radix_pass32(pq_ctx_t *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
-                    uint32_t *vout, uint64_t count, unsigned shift)
+int pq_radix_pass(pq_ctx_t *ctx, const uint64_t *kin, const uint64_t *vin, uint64_t *kout,
+                  uint64_t *vout, uint64_t count, unsigned shift)
 {
-    return radix_pass_t<uint32_t>(ctx, kin, vin, kout, vout, count, shift);
+    return radix_pass_t<uint64_t, uint64_t>(ctx, kin, vin, kout, vout, count, shift);
+}
+
+int pq_radix_pass(pq_ctx_t *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                  uint32_t *vout, uint64_t count, unsigned shift)
+{
+    return radix_pass_t<uint32_t, uint32_t>(ctx, kin, vin, kout, vout, count, shift);
+}
+
+int pq_radix_pass(pq_ctx_t *ctx, const uint32_t *kin, const uint64_t *vin, uint32_t *kout,
+                  uint64_t *vout, uint64_t count, unsigned shift)
+{
+    return radix_pass_t<uint32_t, uint64_t>(ctx, kin, vin, kout, vout, count, shift);
 }

@@ -25,8 +25,6 @@
 #include <cstring>
 #include <chrono>
 
-int pq_radix_pass(pq_ctx_t *ctx, const uint64_t *kin, const uint32_t *vin, uint64_t *kout,
-                  uint32_t *vout, uint64_t count, unsigned shift);
 int pq_scan_u4(pq_ctx_t *ctx, const void *in, void *out, uint64_t count, void *total_host);
 
 #define CHUNK 29
@@ -58,19 +56,115 @@ __device__ __forceinline__ uint64_t chunk_key(const uint64_t *t2, uint64_t g, ui
 
 // Bucket key of the first K codes (K <= 14): (codes << 4) | min(rem, K).  Keys
 // with a field below K are complete suffixes; equal ones are equal strings.
-__global__ void k_bucket_keys(Geom G, const uint64_t *t2, uint64_t N, unsigned K, uint32_t *BK,
-                              uint32_t *V)
+__device__ __forceinline__ uint32_t bucket_key(const Geom &G, const uint64_t *t2, uint64_t g,
+                                               unsigned K)
 {
-    uint64_t g = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= N)
-        return;
     uint32_t rem = suf_rem(G, g);
     uint32_t f = rem < K ? rem : K;
     uint32_t chars = (uint32_t) (pq_window(t2, g) >> (64 - 2 * K));
     if (f < K)
         chars &= ~((1u << (2 * (K - f))) - 1u);
-    BK[g] = (chars << 4) | f;
-    V[g] = (uint32_t) g;
+    return (chars << 4) | f;
+}
+
+// Keys of all N suffixes in position order (one slice holds every suffix).
+template <typename R>
+__global__ void k_bucket_keys(Geom G, const uint64_t *t2, uint64_t N, unsigned K, uint32_t *BK,
+                              R *V)
+{
+    PQ_GRID_STRIDE_FOR(g, N)
+    {
+        BK[g] = bucket_key(G, t2, g, K);
+        V[g] = (R) g;
+    }
+}
+
+// ---- slices of the suffix set ---------------------------------------------------------
+//
+// For large texts the suffixes are sorted a slice at a time: a slice is a range
+// of bucket keys, so it is a contiguous range of the suffix array, and its
+// scratch is bounded by the slice size instead of N.
+
+#define SLICE_BINS 4096      // histogram over the top 12 key bits
+#define SLICE_TILE 4096      // positions per block of the compaction (16 x 256)
+
+__global__ void __launch_bounds__(256) k_key_hist(Geom G, const uint64_t *t2, uint64_t N,
+                                                  unsigned K, unsigned shift,
+                                                  unsigned long long *hist)
+{
+    __shared__ uint32_t h[SLICE_BINS];
+    for (uint32_t i = threadIdx.x; i < SLICE_BINS; i += blockDim.x)
+        h[i] = 0;
+    __syncthreads();
+    PQ_GRID_STRIDE_FOR(g, N)
+    atomicAdd(&h[bucket_key(G, t2, g, K) >> shift], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < SLICE_BINS; i += blockDim.x)
+        if (h[i])
+            atomicAdd(&hist[i], (unsigned long long) h[i]);
+}
+
+// Suffixes of the slice [lo, hi) of keys per tile of SLICE_TILE positions.
+__global__ void __launch_bounds__(256) k_slice_count(Geom G, const uint64_t *t2, uint64_t N,
+                                                     unsigned K, uint64_t lo, uint64_t hi,
+                                                     uint32_t *bcnt)
+{
+    __shared__ uint32_t ws[4];
+    const uint64_t tile = (uint64_t) blockIdx.x * SLICE_TILE;
+    uint32_t c = 0;
+    for (int r = 0; r < SLICE_TILE / 256; r++) {
+        uint64_t g = tile + (uint64_t) r * 256 + threadIdx.x;
+        if (g < N) {
+            uint64_t k = bucket_key(G, t2, g, K);
+            c += (k >= lo && k < hi) ? 1u : 0u;
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1)
+        c += __shfl_xor(c, d, 64);
+    if ((threadIdx.x & 63) == 0)
+        ws[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        bcnt[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// Keys and positions of the slice's suffixes, in position order (the radix
+// passes that follow are stable, and runs of identical complete suffixes rely
+// on position order).
+template <typename R>
+__global__ void __launch_bounds__(256) k_slice_emit(Geom G, const uint64_t *t2, uint64_t N,
+                                                    unsigned K, uint64_t lo, uint64_t hi,
+                                                    const uint32_t *boff, uint32_t *BK, R *V)
+{
+    __shared__ uint32_t ws[4];
+    const uint64_t tile = (uint64_t) blockIdx.x * SLICE_TILE;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t base = boff[blockIdx.x];
+    for (int r = 0; r < SLICE_TILE / 256; r++) {
+        uint64_t g = tile + (uint64_t) r * 256 + threadIdx.x;
+        uint32_t k = 0;
+        bool in = false;
+        if (g < N) {
+            k = bucket_key(G, t2, g, K);
+            in = (uint64_t) k >= lo && (uint64_t) k < hi;
+        }
+        uint64_t m = __ballot(in);
+        if (lane == 0)
+            ws[wid] = (uint32_t) __popcll(m);
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (int w = 0; w < 4; w++) {
+            pre += w < wid ? ws[w] : 0u;
+            tot += ws[w];
+        }
+        if (in) {
+            uint32_t at = base + pre + (uint32_t) __popcll(m & ((1ull << lane) - 1ull));
+            BK[at] = k;
+            V[at] = (R) g;
+        }
+        base += tot;
+        __syncthreads();
+    }
 }
 
 // One suffix of a run being sorted: position, codes left after the run's
@@ -78,26 +172,25 @@ __global__ void k_bucket_keys(Geom G, const uint64_t *t2, uint64_t N, unsigned K
 // each, most significant first, zero past the end).  All of it is loaded up
 // front with independent loads, so comparisons inside the sorting networks
 // never wait on memory.
-#define SUF_NONE 0xFFFFFFFFu
-
-template <int W> struct SufW {
-    uint32_t g, len, sym;    // sym: BWT symbol of the row the suffix lands in
+template <int W, typename R> struct SufW {
+    R g;                     // position; all ones for a padding element
+    uint32_t len, sym;       // sym: BWT symbol of the row the suffix lands in
     uint64_t w[W];
 };
 
 // BWT symbol of the row holding suffix g: the code before g, or '$' (0) at a
 // read start.  Loaded with the suffix's other words, carried through the sort
 // and written next to the row it finalises.
-__device__ __forceinline__ unsigned bwt_sym(const Geom &G, const uint64_t *t2, uint32_t g)
+__device__ __forceinline__ unsigned bwt_sym(const Geom &G, const uint64_t *t2, uint64_t g)
 {
-    return pq_is_start(G, g) ? 0u : pq_code(t2, (uint64_t) g - 1) + 1u;
+    return pq_is_start(G, g) ? 0u : pq_code(t2, g - 1) + 1u;
 }
 
 #define BWT_UNSET 0xFFu
 
-template <int W>
-__device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, uint32_t g,
-                                         uint32_t base, SufW<W> &s)
+template <int W, typename R>
+__device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, R g, uint32_t base,
+                                         SufW<W, R> &s)
 {
     uint32_t rem = suf_rem(G, g);
     s.g = g;
@@ -116,9 +209,9 @@ __device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, uint
     }
 }
 
-template <int W> __device__ __forceinline__ void suf_none(SufW<W> &s)
+template <int W, typename R> __device__ __forceinline__ void suf_none(SufW<W, R> &s)
 {
-    s.g = SUF_NONE;
+    s.g = (R) ~(R) 0;
     s.len = 0;
     s.sym = BWT_UNSET;
 #pragma unroll
@@ -128,9 +221,11 @@ template <int W> __device__ __forceinline__ void suf_none(SufW<W> &s)
 
 // Canonical order of two suffixes sharing their first `base` codes: string
 // order with the end of the read first, then position.
-template <int W> __device__ __forceinline__ bool suf_less(const SufW<W> &a, const SufW<W> &b)
+template <int W, typename R>
+__device__ __forceinline__ bool suf_less(const SufW<W, R> &a, const SufW<W, R> &b)
 {
-    if (a.g == SUF_NONE || b.g == SUF_NONE)
+    const R none = (R) ~(R) 0;
+    if (a.g == none || b.g == none)
         return a.g < b.g;
     uint32_t m = a.len < b.len ? a.len : b.len;
 #pragma unroll
@@ -152,9 +247,10 @@ template <int W> __device__ __forceinline__ bool suf_less(const SufW<W> &a, cons
     return a.g < b.g;
 }
 
-template <int W> __device__ __forceinline__ SufW<W> suf_shfl_xor(const SufW<W> &s, int j)
+template <int W, typename R>
+__device__ __forceinline__ SufW<W, R> suf_shfl_xor(const SufW<W, R> &s, int j)
 {
-    SufW<W> o;
+    SufW<W, R> o;
     o.g = __shfl_xor(s.g, j, 64);
     o.len = __shfl_xor(s.len, j, 64);
     o.sym = __shfl_xor(s.sym, j, 64);
@@ -225,8 +321,8 @@ __device__ __forceinline__ void push_seg(const SegLists &S, bool valid, uint32_t
 // ---- sorts of one run ---------------------------------------------------------------
 
 // One group of G lanes per run (G = 16 or 64); bitonic network over lanes.
-template <int G, int W>
-__global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2, uint32_t *sa,
+template <int G, int W, typename R>
+__global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2, R *sa,
                                                      uint8_t *bwt8, uint32_t base,
                                                      const uint2 *segs, uint32_t nseg)
 {
@@ -236,19 +332,19 @@ __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2
     bool have = sid < nseg;
     uint2 sg = have ? segs[sid] : make_uint2(0, 0);
     bool e = have && (uint32_t) li < sg.y;
-    SufW<W> s;
+    SufW<W, R> s;
     if (e)
-        suf_load<W>(Gm, t2, sa[sg.x + li], base, s);
+        suf_load<W, R>(Gm, t2, sa[sg.x + li], base, s);
     else
-        suf_none<W>(s);
+        suf_none<W, R>(s);
 #pragma unroll
     for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            SufW<W> p = suf_shfl_xor<W>(s, j);
+            SufW<W, R> p = suf_shfl_xor<W, R>(s, j);
             bool up = (li & k) == 0;
             bool lower = (li & j) == 0;
-            bool pless = suf_less<W>(p, s);
+            bool pless = suf_less<W, R>(p, s);
             bool swap = (lower == up) ? pless : (!pless && p.g != s.g);
             if (swap)
                 s = p;
@@ -267,22 +363,22 @@ __host__ __device__ constexpr uint32_t lds_cap(int W)
 }
 
 // One block per run of 65..lds_cap(W) elements: bitonic sort in LDS.
-template <int W>
-__global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, uint32_t *sa,
+template <int W, typename R>
+__global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, R *sa,
                                                   uint8_t *bwt8, uint32_t base, const uint2 *segs)
 {
     constexpr uint32_t CAP = lds_cap(W);
-    __shared__ SufW<W> sh[CAP];
+    __shared__ SufW<W, R> sh[CAP];
     uint2 sg = segs[blockIdx.x];
     uint32_t n = sg.y, P = 1;
     while (P < n)
         P <<= 1;
     for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
-        SufW<W> s;
+        SufW<W, R> s;
         if (i < n)
-            suf_load<W>(Gm, t2, sa[sg.x + i], base, s);
+            suf_load<W, R>(Gm, t2, sa[sg.x + i], base, s);
         else
-            suf_none<W>(s);
+            suf_none<W, R>(s);
         sh[i] = s;
     }
     __syncthreads();
@@ -291,8 +387,8 @@ __global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, u
             for (uint32_t p = threadIdx.x; p < P / 2; p += blockDim.x) {
                 uint32_t i = 2 * j * (p / j) + (p % j), ip = i + j;
                 bool up = (i & k) == 0;
-                SufW<W> x = sh[i], y = sh[ip];
-                if (suf_less<W>(y, x) == up) {
+                SufW<W, R> x = sh[i], y = sh[ip];
+                if (suf_less<W, R>(y, x) == up) {
                     sh[i] = y;
                     sh[ip] = x;
                 }
@@ -367,20 +463,18 @@ __global__ void k_push_subruns(const uint64_t *K, const uint32_t *starts, uint32
     push_seg(out, unres, at + a, b - a);
 }
 
-__global__ void k_chunk_keys(Geom G, const uint64_t *t2, const uint32_t *sa, uint64_t n,
-                             uint32_t base, uint64_t *K)
+template <typename R>
+__global__ void k_chunk_keys(Geom G, const uint64_t *t2, const R *sa, uint64_t n, uint32_t base,
+                             uint64_t *K)
 {
     uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        uint32_t g = sa[i];
+        uint64_t g = sa[i];
         K[i] = chunk_key(t2, g, suf_rem(G, g), base);
     }
 }
 
 // ---- driver ---------------------------------------------------------------------------
-
-int pq_radix_pass32(pq_ctx_t *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
-                    uint32_t *vout, uint64_t count, unsigned shift);
 
 // Run starts of equal keys in K[0..n): returns the run count; starts gets
 // nruns+1 entries (sentinel n).  flags/pos/starts: n+1 entries each.
@@ -411,37 +505,38 @@ static uint32_t lcap_for(int W)
     return W == 0 ? 0u : lds_cap(W);
 }
 
-template <int W>
-static void launch_sorts(pq_ctx_t *ctx, const Geom &G, DirIndex &D, uint32_t base,
-                         uint2 *const items[CLS_BIG], const uint32_t cnt[CLS_BIG])
+template <int W, typename R>
+static void launch_sorts(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, R *sa, uint8_t *bwt8,
+                         uint32_t base, uint2 *const items[CLS_BIG], const uint32_t cnt[CLS_BIG])
 {
     if (cnt[0])
-        hipLaunchKernelGGL((k_sort_groups<RUN_SMALL, W>),
+        hipLaunchKernelGGL((k_sort_groups<RUN_SMALL, W, R>),
                            dim3(pq_blocks((uint64_t) cnt[0] * RUN_SMALL, 256)), dim3(256), 0,
-                           ctx->stream, G, D.text2, D.sa, D.bwt8, base, items[0], cnt[0]);
+                           ctx->stream, G, t2, sa, bwt8, base, items[0], cnt[0]);
     if (cnt[1])
-        hipLaunchKernelGGL((k_sort_groups<RUN_HALF, W>),
+        hipLaunchKernelGGL((k_sort_groups<RUN_HALF, W, R>),
                            dim3(pq_blocks((uint64_t) cnt[1] * RUN_HALF, 256)), dim3(256), 0,
-                           ctx->stream, G, D.text2, D.sa, D.bwt8, base, items[1], cnt[1]);
+                           ctx->stream, G, t2, sa, bwt8, base, items[1], cnt[1]);
     if (cnt[2])
-        hipLaunchKernelGGL((k_sort_groups<RUN_WAVE, W>),
+        hipLaunchKernelGGL((k_sort_groups<RUN_WAVE, W, R>),
                            dim3(pq_blocks((uint64_t) cnt[2] * RUN_WAVE, 256)), dim3(256), 0,
-                           ctx->stream, G, D.text2, D.sa, D.bwt8, base, items[2], cnt[2]);
+                           ctx->stream, G, t2, sa, bwt8, base, items[2], cnt[2]);
     if (cnt[3])
-        hipLaunchKernelGGL(k_sort_lds<W>, dim3(cnt[3]), dim3(256), 0, ctx->stream, G, D.text2,
-                           D.sa, D.bwt8, base, items[3]);
+        hipLaunchKernelGGL((k_sort_lds<W, R>), dim3(cnt[3]), dim3(256), 0, ctx->stream, G, t2, sa,
+                           bwt8, base, items[3]);
 }
 
-static int sort_lists(pq_ctx_t *ctx, const Geom &G, DirIndex &D, uint32_t base,
-                      uint2 *const items[CLS_BIG], const uint32_t cnt[CLS_BIG])
+template <typename R>
+static int sort_lists(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, R *sa, uint8_t *bwt8,
+                      uint32_t base, uint2 *const items[CLS_BIG], const uint32_t cnt[CLS_BIG])
 {
     uint32_t left = ctx->Lmax > base ? ctx->Lmax - base : 0;
     switch (words_for(left)) {
-    case 1: launch_sorts<1>(ctx, G, D, base, items, cnt); break;
-    case 2: launch_sorts<2>(ctx, G, D, base, items, cnt); break;
-    case 3: launch_sorts<3>(ctx, G, D, base, items, cnt); break;
-    case 4: launch_sorts<4>(ctx, G, D, base, items, cnt); break;
-    case 8: launch_sorts<8>(ctx, G, D, base, items, cnt); break;
+    case 1: launch_sorts<1, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
+    case 2: launch_sorts<2, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
+    case 3: launch_sorts<3, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
+    case 4: launch_sorts<4, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
+    case 8: launch_sorts<8, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
     default:
         if (cnt[0] || cnt[1] || cnt[2] || cnt[3])
             return ctx->fail(PQ_ERR_STATE, "run classified for cached words it cannot hold");
@@ -462,188 +557,294 @@ static unsigned bucket_codes(uint64_t N)
     return std::min(14u, (8 * passes - 4) / 2);
 }
 
-int pq_build_sa_dir(pq_ctx_t *ctx, int d)
+// Buffers of one slice sort: keys BK (sorted), the slice's range of the suffix
+// array and of the BWT bytes, and run scratch (flags and pos alias the radix
+// ping-pong buffers, free once the passes are done).
+template <typename R> struct SliceSort {
+    uint32_t *BK, *flags, *pos, *starts, *counts;
+    uint8_t *cls;
+    R *sa;
+    uint8_t *bwt8;
+};
+
+// Sort the ns suffixes of one slice whose bucket keys are sorted in BK and
+// whose positions sit in key order in S.sa: runs of equal keys by size class,
+// then oversized runs level by level over 29-code chunks.
+template <typename R>
+static int sort_runs(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned K, uint64_t ns,
+                     const SliceSort<R> &S)
 {
-    const uint64_t N = ctx->N;
-    DirIndex &D = ctx->dir[d];
-    if (N == 0)
-        return PQ_OK;
-    Geom G = ctx->geom();
-    const unsigned K = bucket_codes(N);
-    uint32_t *BK = nullptr, *BK2 = nullptr, *V2 = nullptr, *flags = nullptr, *pos = nullptr,
-             *starts = nullptr;
-    uint8_t *cls = nullptr;
-    uint2 *items[4] = {nullptr, nullptr, nullptr, nullptr};
-    uint32_t *counts = nullptr;
     int rc = PQ_OK;
-    if ((rc = pq_slot(ctx, SLOT_BK, N * 4, &BK)) || (rc = pq_slot(ctx, SLOT_BK2, N * 4, &BK2)) ||
-        (rc = pq_slot(ctx, SLOT_V2, N * 4, &V2)) ||
-        (rc = pq_slot(ctx, SLOT_FLAGS, (N + 1) * 4, &flags)) ||
-        (rc = pq_slot(ctx, SLOT_POS, (N + 1) * 4, &pos)) ||
-        (rc = pq_slot(ctx, SLOT_STARTS, (N + 2) * 4, &starts)) ||
-        (rc = pq_slot(ctx, SLOT_CLS, N + 1, &cls)) ||
-        (rc = pq_slot(ctx, SLOT_COUNTS, 64, &counts)) ||
-        (rc = pq_slot(ctx, SLOT_BWT8, N + 1, &D.bwt8)))
-        goto done;
-    PQ_CHECK(hipMemsetAsync(D.bwt8, BWT_UNSET, N, ctx->stream));
-    {
-        // 1. bucket by the first K codes (and the length below K)
-        unsigned bits = 2 * K + 4, passes = (bits + 7) / 8;
-        // start in the pair that makes the last pass write (BK, D.sa)
-        uint32_t *kc = (passes & 1) ? BK2 : BK, *ka = (passes & 1) ? BK : BK2;
-        uint32_t *vc = (passes & 1) ? V2 : D.sa, *va = (passes & 1) ? D.sa : V2;
-        hipLaunchKernelGGL(k_bucket_keys, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, G,
-                           D.text2, N, K, kc, vc);
-        for (unsigned p = 0; p < passes; p++) {
-            if ((rc = pq_radix_pass32(ctx, kc, vc, ka, va, N, 8 * p)))
-                goto done;
-            std::swap(kc, ka);
-            std::swap(vc, va);
+    // bucket runs, classified by size with scans
+    uint32_t nruns = 0;
+    if ((rc = find_runs<uint32_t>(ctx, S.BK, ns, S.flags, S.pos, S.starts, &nruns)))
+        return rc;
+    uint32_t lcap0 = lcap_for(words_for(ctx->Lmax > K ? ctx->Lmax - K : 0));
+    hipLaunchKernelGGL(k_bucket_class, dim3(pq_blocks(nruns, 256)), dim3(256), 0, ctx->stream,
+                       S.BK, S.starts, nruns, K, lcap0, S.cls);
+    // the class lists one after the other in one buffer (together at most nruns)
+    uint32_t cnt[NCLS] = {0, 0, 0, 0, 0};
+    uint2 *lists[NCLS];
+    uint2 *lbuf = nullptr;
+    if ((rc = pq_slot(ctx, SLOT_LISTS, ((size_t) nruns + NCLS) * sizeof(uint2), &lbuf)))
+        return rc;
+    uint64_t at = 0;
+    for (int c = 0; c < NCLS; c++) {
+        lists[c] = lbuf + at;
+        hipLaunchKernelGGL(k_class_flags, dim3(pq_blocks(nruns, 256)), dim3(256), 0, ctx->stream,
+                           S.cls, nruns, (uint8_t) c, S.flags);
+        if ((rc = pq_scan_u32(ctx, S.flags, S.pos, nruns, &cnt[c])))
+            return rc;
+        hipLaunchKernelGGL(k_class_compact, dim3(pq_blocks(nruns, 256)), dim3(256), 0,
+                           ctx->stream, S.flags, S.pos, S.starts, nruns, lists[c]);
+        at += cnt[c];
+    }
+    // sort every bucket past its first K codes
+    uint2 *l3[CLS_BIG] = {lists[0], lists[1], lists[2], lists[3]};
+    if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, K, l3, cnt)))
+        return rc;
+    // oversized buckets: MSD over 29-code chunks, level by level
+    std::vector<uint2> big(cnt[CLS_BIG]);
+    if (cnt[CLS_BIG]) {
+        PQ_CHECK(hipMemcpyAsync(big.data(), lists[CLS_BIG], cnt[CLS_BIG] * sizeof(uint2),
+                                hipMemcpyDeviceToHost, ctx->stream));
+        PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    uint32_t base = K;
+    SegLists SL;
+    while (!big.empty() && rc == PQ_OK) {
+        uint64_t maxn = 0, bigsum = 0;
+        for (const uint2 &s : big) {
+            maxn = std::max<uint64_t>(maxn, s.y);
+            bigsum += s.y;
         }
-        // 2. bucket runs, classified by size with scans
-        uint32_t nruns = 0;
-        if ((rc = find_runs<uint32_t>(ctx, BK, N, flags, pos, starts, &nruns)))
-            goto done;
-        uint32_t lcap0 = lcap_for(words_for(ctx->Lmax > K ? ctx->Lmax - K : 0));
-        hipLaunchKernelGGL(k_bucket_class, dim3(pq_blocks(nruns, 256)), dim3(256), 0, ctx->stream,
-                           BK, starts, nruns, K, lcap0, cls);
-        uint32_t cnt[NCLS] = {0, 0, 0, 0, 0};
-        std::vector<uint2 *> lists(NCLS);
-        uint2 *lbuf = nullptr;
-        if ((rc = pq_slot(ctx, SLOT_LISTS, (size_t) (nruns + 1) * NCLS * sizeof(uint2), &lbuf)))
-            goto done;
+        uint64_t *KT = (uint64_t *) ctx->tmp(SLOT_BIGK, maxn * 8 * 2 + 64);
+        R *VT = (R *) ctx->tmp(SLOT_BIGV, maxn * sizeof(R) + 64);
+        if (!KT || !VT)
+            return ctx->fail(PQ_ERR_NOMEM, "oversized run scratch");
+        // every run pushed from this level holds at least two elements of it
+        const uint32_t scap = (uint32_t) (bigsum / 2 + 2);
+        uint2 *sbuf = nullptr;
+        if ((rc = pq_slot(ctx, SLOT_SUBLISTS, (size_t) scap * NCLS * sizeof(uint2), &sbuf)))
+            return rc;
         for (int c = 0; c < NCLS; c++) {
-            lists[c] = lbuf + (size_t) c * (nruns + 1);
-            hipLaunchKernelGGL(k_class_flags, dim3(pq_blocks(nruns, 256)), dim3(256), 0,
-                               ctx->stream, cls, nruns, (uint8_t) c, flags);
-            if ((rc = pq_scan_u32(ctx, flags, pos, nruns, &cnt[c])))
-                goto done;
-            hipLaunchKernelGGL(k_class_compact, dim3(pq_blocks(nruns, 256)), dim3(256), 0,
-                               ctx->stream, flags, pos, starts, nruns, lists[c]);
+            SL.items[c] = sbuf + (size_t) c * scap;
+            SL.cap[c] = scap;
         }
-        // 3. sort every bucket past its first K codes
-        uint2 *l3[CLS_BIG] = {lists[0], lists[1], lists[2], lists[3]};
-        if ((rc = sort_lists(ctx, G, D, K, l3, cnt)))
-            goto done;
-        // 4. oversized buckets: MSD over 29-code chunks, level by level
-        std::vector<uint2> big(cnt[CLS_BIG]);
-        if (cnt[CLS_BIG]) {
-            PQ_CHECK(hipMemcpyAsync(big.data(), lists[CLS_BIG], cnt[CLS_BIG] * sizeof(uint2),
+        SL.count = S.counts;
+        SL.overflow = S.counts + NCLS;
+        uint32_t nb = base + CHUNK;
+        SL.lcap = lcap_for(words_for(ctx->Lmax > nb ? ctx->Lmax - nb : 0));
+        PQ_CHECK(hipMemsetAsync(S.counts, 0, 64, ctx->stream));
+        for (const uint2 &s : big) {
+            uint64_t *keys = KT, *keys2 = KT + maxn;
+            R *run = S.sa + s.x;
+            hipLaunchKernelGGL(k_chunk_keys<R>, dim3(pq_blocks(s.y, 256)), dim3(256), 0,
+                               ctx->stream, G, t2, run, (uint64_t) s.y, base, keys);
+            for (unsigned p = 0; p < 8 && rc == PQ_OK; p++) {
+                const uint64_t *ki = (p & 1) ? keys2 : keys;
+                const R *vi = (p & 1) ? VT : run;
+                uint64_t *ko = (p & 1) ? keys : keys2;
+                R *vo = (p & 1) ? run : VT;
+                rc = pq_radix_pass(ctx, ki, vi, ko, vo, s.y, 8 * p);
+            }
+            uint32_t nsub = 0;
+            if (rc || (rc = find_runs<uint64_t>(ctx, keys, s.y, S.flags, S.pos, S.starts, &nsub)))
+                return rc;
+            hipLaunchKernelGGL(k_push_subruns, dim3(pq_blocks(nsub, 256)), dim3(256), 0,
+                               ctx->stream, keys, S.starts, nsub, s.x, SL);
+        }
+        uint32_t c5[NCLS + 1];
+        PQ_CHECK(hipMemcpyAsync(c5, S.counts, (NCLS + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+        PQ_CHECK(hipStreamSynchronize(ctx->stream));
+        if (c5[NCLS])
+            return ctx->fail(PQ_ERR_LIMIT, "suffix sort run list overflow");
+        base += CHUNK;
+        uint2 *sl[CLS_BIG] = {SL.items[0], SL.items[1], SL.items[2], SL.items[3]};
+        if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, base, sl, c5)))
+            return rc;
+        big.assign(c5[CLS_BIG], make_uint2(0, 0));
+        if (c5[CLS_BIG]) {
+            PQ_CHECK(hipMemcpyAsync(big.data(), SL.items[CLS_BIG], c5[CLS_BIG] * sizeof(uint2),
                                     hipMemcpyDeviceToHost, ctx->stream));
             PQ_CHECK(hipStreamSynchronize(ctx->stream));
         }
-        uint32_t base = K;
-        SegLists S;
-        uint2 *sbuf = nullptr;
-        const uint32_t scap = (uint32_t) (N / 2 + 2);
-        while (!big.empty() && rc == PQ_OK) {
-            uint64_t maxn = 0;
-            for (const uint2 &s : big)
-                maxn = std::max<uint64_t>(maxn, s.y);
-            uint64_t *KT = (uint64_t *) ctx->tmp(SLOT_BIGK, maxn * 8 * 2 + 64);
-            uint32_t *VT = (uint32_t *) ctx->tmp(SLOT_BIGV, maxn * 4 + 64);
-            if (!KT || !VT) {
-                rc = ctx->fail(PQ_ERR_NOMEM, "oversized run scratch");
-                break;
-            }
-            if (!sbuf &&
-                (rc = pq_slot(ctx, SLOT_SUBLISTS, (size_t) scap * NCLS * sizeof(uint2), &sbuf)))
-                break;
-            for (int c = 0; c < NCLS; c++) {
-                S.items[c] = sbuf + (size_t) c * scap;
-                S.cap[c] = scap;
-            }
-            S.count = counts;
-            S.overflow = counts + NCLS;
-            uint32_t nb = base + CHUNK;
-            S.lcap = lcap_for(words_for(ctx->Lmax > nb ? ctx->Lmax - nb : 0));
-            PQ_CHECK(hipMemsetAsync(counts, 0, 64, ctx->stream));
-            for (const uint2 &s : big) {
-                uint64_t *keys = KT, *keys2 = KT + maxn;
-                hipLaunchKernelGGL(k_chunk_keys, dim3(pq_blocks(s.y, 256)), dim3(256), 0,
-                                   ctx->stream, G, D.text2, D.sa + s.x, (uint64_t) s.y, base, keys);
-                for (unsigned p = 0; p < 8 && rc == PQ_OK; p++) {
-                    const uint64_t *ki = (p & 1) ? keys2 : keys;
-                    const uint32_t *vi = (p & 1) ? VT : D.sa + s.x;
-                    uint64_t *ko = (p & 1) ? keys : keys2;
-                    uint32_t *vo = (p & 1) ? D.sa + s.x : VT;
-                    rc = pq_radix_pass(ctx, ki, vi, ko, vo, s.y, 8 * p);
-                }
-                uint32_t nsub = 0;
-                if (rc || (rc = find_runs<uint64_t>(ctx, keys, s.y, flags, pos, starts, &nsub)))
-                    break;
-                hipLaunchKernelGGL(k_push_subruns, dim3(pq_blocks(nsub, 256)), dim3(256), 0,
-                                   ctx->stream, keys, starts, nsub, s.x, S);
-            }
-            if (rc)
-                break;
-            uint32_t c5[NCLS + 1];
-            PQ_CHECK(hipMemcpyAsync(c5, counts, (NCLS + 1) * 4, hipMemcpyDeviceToHost,
-                                    ctx->stream));
-            PQ_CHECK(hipStreamSynchronize(ctx->stream));
-            if (c5[NCLS]) {
-                rc = ctx->fail(PQ_ERR_LIMIT, "suffix sort run list overflow");
-                break;
-            }
-            base += CHUNK;
-            uint2 *sl[CLS_BIG] = {S.items[0], S.items[1], S.items[2], S.items[3]};
-            if ((rc = sort_lists(ctx, G, D, base, sl, c5)))
-                break;
-            big.assign(c5[CLS_BIG], make_uint2(0, 0));
-            if (c5[CLS_BIG]) {
-                PQ_CHECK(hipMemcpyAsync(big.data(), S.items[CLS_BIG], c5[CLS_BIG] * sizeof(uint2),
-                                        hipMemcpyDeviceToHost, ctx->stream));
-                PQ_CHECK(hipStreamSynchronize(ctx->stream));
-            }
-            if (base > ctx->Lmax + CHUNK) {
-                rc = ctx->fail(PQ_ERR_STATE, "suffix sort did not converge");
-                break;
-            }
-        }
+        if (base > ctx->Lmax + CHUNK)
+            return ctx->fail(PQ_ERR_STATE, "suffix sort did not converge");
     }
-done:
     return rc;
+}
+
+// Slices of the suffix set: contiguous key ranges [lo, hi) of at most cap
+// suffixes each, cut at SLICE_BINS-histogram boundaries; one slice when N fits.
+struct Slice {
+    uint64_t lo, hi, count;
+};
+
+static int plan_slices(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned K, uint64_t cap,
+                       std::vector<Slice> &out)
+{
+    const uint64_t N = ctx->N;
+    out.clear();
+    if (N <= cap) {
+        out.push_back({0, 1ull << 32, N});
+        return PQ_OK;
+    }
+    const unsigned shift = 2 * K + 4 - 12;
+    unsigned long long *hist = nullptr;
+    int rc;
+    if ((rc = pq_slot(ctx, SLOT_SLICE_HIST, SLICE_BINS * 8, &hist)))
+        return rc;
+    PQ_CHECK(hipMemsetAsync(hist, 0, SLICE_BINS * 8, ctx->stream));
+    hipLaunchKernelGGL(k_key_hist, dim3(std::min<unsigned>(1024, pq_blocks(N, 256))), dim3(256), 0,
+                       ctx->stream, G, t2, N, K, shift, hist);
+    PQ_CHECK(hipGetLastError());
+    std::vector<uint64_t> h(SLICE_BINS);
+    PQ_CHECK(hipMemcpyAsync(h.data(), hist, SLICE_BINS * 8, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    uint64_t lo = 0, cur = 0, total = 0;
+    for (uint64_t b = 0; b < SLICE_BINS; b++) {
+        if (h[b] > cap)
+            return ctx->fail(PQ_ERR_LIMIT, "one bucket range holds more suffixes than a slice");
+        if (cur + h[b] > cap) {
+            out.push_back({lo << shift, b << shift, cur});
+            lo = b;
+            cur = 0;
+        }
+        cur += h[b];
+        total += h[b];
+    }
+    if (cur)
+        out.push_back({lo << shift, 1ull << 32, cur});
+    if (total != N)
+        return ctx->fail(PQ_ERR_STATE, "slice histogram does not cover the text");
+    return PQ_OK;
+}
+
+template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
+{
+    const uint64_t N = ctx->N;
+    DirIndex &D = ctx->dir[d];
+    Geom G = ctx->geom();
+    const unsigned K = bucket_codes(N);
+    const unsigned passes = (2 * K + 4 + 7) / 8;
+    uint64_t cap = ctx->opt_slice ? ctx->opt_slice : (1ull << 31);
+    cap = std::min<uint64_t>(cap, 1ull << 31);
+    std::vector<Slice> slices;
+    int rc;
+    if ((rc = plan_slices(ctx, G, D.text2, K, cap, slices)))
+        return rc;
+    ctx->last_slices = (uint32_t) slices.size();
+    uint64_t smax = 0;
+    for (const Slice &sl : slices)
+        smax = std::max(smax, sl.count);
+    const uint64_t ntiles = (N + SLICE_TILE - 1) / SLICE_TILE;
+    SliceSort<R> S;
+    uint32_t *BK2 = nullptr, *bcnt = nullptr;
+    R *V2 = nullptr;
+    const size_t vbytes = (smax + 1) * std::max<size_t>(4, sizeof(R));
+    if ((rc = pq_slot(ctx, SLOT_BK, smax * 4 + 4, &S.BK)) ||
+        (rc = pq_slot(ctx, SLOT_BK2, (smax + 1) * 4, &BK2)) ||
+        (rc = pq_slot(ctx, SLOT_V2, vbytes, &V2)) ||
+        (rc = pq_slot(ctx, SLOT_STARTS, (smax + 2) * 4, &S.starts)) ||
+        (rc = pq_slot(ctx, SLOT_CLS, smax + 1, &S.cls)) ||
+        (rc = pq_slot(ctx, SLOT_COUNTS, 64, &S.counts)) ||
+        (rc = pq_slot(ctx, SLOT_BWT8, N + 1, &D.bwt8)) ||
+        (slices.size() > 1 && (rc = pq_slot(ctx, SLOT_SLICE_TILES, ntiles * 8 + 8, &bcnt))))
+        return rc;
+    S.flags = BK2;
+    S.pos = (uint32_t *) V2;
+    PQ_CHECK(hipMemsetAsync(D.bwt8, BWT_UNSET, N, ctx->stream));
+    uint64_t off = 0;
+    for (const Slice &sl : slices) {
+        const uint64_t ns = sl.count;
+        S.sa = (R *) D.sa + off;
+        S.bwt8 = D.bwt8 + off;
+        // start in the pair that makes the last radix pass write (BK, slice of sa)
+        uint32_t *kc = (passes & 1) ? BK2 : S.BK, *ka = (passes & 1) ? S.BK : BK2;
+        R *vc = (passes & 1) ? V2 : S.sa, *va = (passes & 1) ? S.sa : V2;
+        // 1. bucket keys by the first K codes (and the length below K)
+        if (slices.size() == 1) {
+            hipLaunchKernelGGL(k_bucket_keys<R>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
+                               ctx->stream, G, D.text2, N, K, kc, vc);
+        } else {
+            uint32_t *boff = bcnt + ntiles + 1;
+            hipLaunchKernelGGL(k_slice_count, dim3((unsigned) ntiles), dim3(256), 0, ctx->stream,
+                               G, D.text2, N, K, sl.lo, sl.hi, bcnt);
+            uint32_t got = 0;
+            if ((rc = pq_scan_u32(ctx, bcnt, boff, ntiles, &got)))
+                return rc;
+            if (got != ns)
+                return ctx->fail(PQ_ERR_STATE, "slice count disagrees with its histogram");
+            hipLaunchKernelGGL(k_slice_emit<R>, dim3((unsigned) ntiles), dim3(256), 0,
+                               ctx->stream, G, D.text2, N, K, sl.lo, sl.hi, boff, kc, vc);
+        }
+        for (unsigned p = 0; p < passes; p++) {
+            if ((rc = pq_radix_pass(ctx, kc, vc, ka, va, ns, 8 * p)))
+                return rc;
+            std::swap(kc, ka);
+            std::swap(vc, va);
+        }
+        // 2-4. runs of equal keys
+        if ((rc = sort_runs<R>(ctx, G, D.text2, K, ns, S)))
+            return rc;
+        off += ns;
+    }
+    if (off != N)
+        return ctx->fail(PQ_ERR_STATE, "slices do not cover the text");
+    return PQ_OK;
+}
+
+int pq_build_sa_dir(pq_ctx_t *ctx, int d)
+{
+    if (ctx->N == 0)
+        return PQ_OK;
+    return ctx->wide ? build_sa_dir_t<uint64_t>(ctx, d) : build_sa_dir_t<uint32_t>(ctx, d);
 }
 
 // ---- BWT occurrence lines + reads_index ---------------------------------------------------
 
 // BWT symbols of the rows no sort wrote (one-suffix buckets, buckets of
 // identical complete suffixes, rows placed by the chunk-level radix passes).
-__global__ void k_bwt_fill(Geom G, const uint64_t *t2, const uint32_t *sa, uint64_t N,
-                           uint8_t *bwt8)
+template <typename R>
+__global__ void k_bwt_fill(Geom G, const uint64_t *t2, const R *sa, uint64_t N, uint8_t *bwt8)
 {
-    uint64_t row = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (row < N && bwt8[row] == BWT_UNSET)
-        bwt8[row] = (uint8_t) bwt_sym(G, t2, sa[row]);
+    PQ_GRID_STRIDE_FOR(row, N)
+    if (bwt8[row] == BWT_UNSET)
+        bwt8[row] = (uint8_t) bwt_sym(G, t2, (uint64_t) sa[row]);
 }
 
-// 256 threads = 2 lines; each wave builds one 64-row half of a line.
+// 256 threads = 2 lines per tile; each wave builds one 64-row half of a line.
 __global__ void __launch_bounds__(256) k_occ_planes(const uint8_t *bwt8, uint64_t N,
                                                     uint64_t nlines, OccLine *occ, uint4 *cnts)
 {
     __shared__ uint32_t part[4][4];
-    uint64_t row = (uint64_t) blockIdx.x * 256 + threadIdx.x;
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    unsigned sym = row < N ? bwt8[row] : 7u;
-    uint64_t b0 = __ballot(sym & 1u), b1 = __ballot(sym & 2u), b2 = __ballot(sym & 4u);
-    uint64_t line = row >> 7;
-    int half = (int) ((row >> 6) & 1);
-    if (lane == 0 && line < nlines) {
-        occ[line].p0[half] = b0;
-        occ[line].p1[half] = b1;
-        occ[line].p2[half] = b2;
-        part[wid][0] = __popcll(b0 & ~b1 & ~b2);
-        part[wid][1] = __popcll(~b0 & b1 & ~b2);
-        part[wid][2] = __popcll(b0 & b1 & ~b2);
-        part[wid][3] = __popcll(~b0 & ~b1 & b2);
-    }
-    __syncthreads();
-    if (threadIdx.x < 2) {
-        uint64_t ln = (uint64_t) blockIdx.x * 2 + threadIdx.x;
-        int w0 = 2 * threadIdx.x;
-        if (ln < nlines)
-            cnts[ln] = make_uint4(part[w0][0] + part[w0 + 1][0], part[w0][1] + part[w0 + 1][1],
-                                  part[w0][2] + part[w0 + 1][2], part[w0][3] + part[w0 + 1][3]);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t ntiles = (nlines + 1) / 2;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t row = tile * 256 + threadIdx.x;
+        unsigned sym = row < N ? bwt8[row] : 7u;
+        uint64_t b0 = __ballot(sym & 1u), b1 = __ballot(sym & 2u), b2 = __ballot(sym & 4u);
+        uint64_t line = row >> 7;
+        int half = (int) ((row >> 6) & 1);
+        if (lane == 0 && line < nlines) {
+            occ[line].p0[half] = b0;
+            occ[line].p1[half] = b1;
+            occ[line].p2[half] = b2;
+            part[wid][0] = __popcll(b0 & ~b1 & ~b2);
+            part[wid][1] = __popcll(~b0 & b1 & ~b2);
+            part[wid][2] = __popcll(b0 & b1 & ~b2);
+            part[wid][3] = __popcll(~b0 & ~b1 & b2);
+        }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            uint64_t ln = tile * 2 + threadIdx.x;
+            int w0 = 2 * threadIdx.x;
+            if (ln < nlines)
+                cnts[ln] = make_uint4(part[w0][0] + part[w0 + 1][0], part[w0][1] + part[w0 + 1][1],
+                                      part[w0][2] + part[w0 + 1][2], part[w0][3] + part[w0 + 1][3]);
+        }
+        __syncthreads();
     }
 }
 
@@ -659,21 +860,21 @@ __global__ void k_occ_counts(OccLine *occ, const uint4 *excl, uint64_t nlines)
     }
 }
 
-__global__ void k_reads_index(Geom G, const uint32_t *sa, const OccLine *occ, uint64_t N,
-                              uint32_t *ri)
+template <typename R>
+__global__ void k_reads_index(Geom G, const R *sa, const OccLine *occ, uint64_t N, uint32_t *ri)
 {
-    uint64_t row = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= N)
-        return;
-    uint64_t line = row >> 7;
-    int half = (int) ((row >> 6) & 1);
-    uint64_t a = occ[line].p0[half], b = occ[line].p1[half], c = occ[line].p2[half];
-    unsigned j = (unsigned) (row & 63);
-    if ((a >> j) & 1 || (b >> j) & 1 || (c >> j) & 1)
-        return;
-    Ranks rk;
-    pq_ranks(occ, row, rk);
-    ri[rk.r[0]] = (uint32_t) pq_read_of(G, sa[row]);
+    PQ_GRID_STRIDE_FOR(row, N)
+    {
+        uint64_t line = row >> 7;
+        int half = (int) ((row >> 6) & 1);
+        uint64_t a = occ[line].p0[half], b = occ[line].p1[half], c = occ[line].p2[half];
+        unsigned j = (unsigned) (row & 63);
+        if ((a >> j) & 1 || (b >> j) & 1 || (c >> j) & 1)
+            continue;
+        Ranks rk;
+        pq_ranks(occ, row, rk);
+        ri[rk.r[0]] = (uint32_t) pq_read_of(G, (uint64_t) sa[row]);
+    }
 }
 
 int pq_build_occ_dir(pq_ctx_t *ctx, int d)
@@ -693,25 +894,40 @@ int pq_build_occ_dir(pq_ctx_t *ctx, int d)
     uint4 *excl = cnts + D.nlines;
     if (!D.bwt8)
         return ctx->fail(PQ_ERR_STATE, "BWT symbols missing: suffix array not built");
-    hipLaunchKernelGGL(k_bwt_fill, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, G, D.text2,
-                       D.sa, N, D.bwt8);
-    hipLaunchKernelGGL(k_occ_planes, dim3(pq_blocks(D.nlines * 128, 256)), dim3(256), 0,
+    if (ctx->wide)
+        hipLaunchKernelGGL(k_bwt_fill<uint64_t>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
+                           ctx->stream, G, D.text2, (const uint64_t *) D.sa, N, D.bwt8);
+    else
+        hipLaunchKernelGGL(k_bwt_fill<uint32_t>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
+                           ctx->stream, G, D.text2, (const uint32_t *) D.sa, N, D.bwt8);
+    hipLaunchKernelGGL(k_occ_planes, dim3(pq_blocks_gs((D.nlines + 1) / 2, 1)), dim3(256), 0,
                        ctx->stream, D.bwt8, N, D.nlines, D.occ, cnts);
     uint32_t tot[4];
     if ((rc = pq_scan_u4(ctx, cnts, excl, D.nlines, tot)))
         return rc;
     hipLaunchKernelGGL(k_occ_counts, dim3(pq_blocks(D.nlines, 256)), dim3(256), 0, ctx->stream,
                        D.occ, excl, D.nlines);
-    hipLaunchKernelGGL(k_reads_index, dim3(pq_blocks(N, 256)), dim3(256), 0, ctx->stream, G,
-                       D.sa, D.occ, N, D.reads_index);
+    if (ctx->wide)
+        hipLaunchKernelGGL(k_reads_index<uint64_t>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
+                           ctx->stream, G, (const uint64_t *) D.sa, D.occ, N, D.reads_index);
+    else
+        hipLaunchKernelGGL(k_reads_index<uint32_t>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
+                           ctx->stream, G, (const uint32_t *) D.sa, D.occ, N, D.reads_index);
     PQ_CHECK(hipGetLastError());
     D.bwt8 = nullptr;   // the byte array is scratch shared by both directions
+    // the per-line counts are 32-bit: every symbol must fill fewer than 2^32
+    // rows (the '$' rows are the n read starts)
+    if ((uint64_t) tot[0] + tot[1] + tot[2] + tot[3] + ctx->n != N)
+        return ctx->fail(PQ_ERR_LIMIT, "a symbol fills 2^32 rows or more of the BWT");
     // C (index.c:366-372): code counts of the text = BWT counts + reads ending
     // (fw) / starting (bw) with the code, whose last/first code is not a BWT row.
     const uint64_t *fix = d == 0 ? ctx->sfx : ctx->pfx;
     uint64_t cnt[4];
-    for (int c = 0; c < 4; c++)
+    for (int c = 0; c < 4; c++) {
         cnt[c] = (uint64_t) tot[c] + fix[c];
+        if (cnt[c] >= 0xFFFFFFFFull)   // interval sizes are 32-bit
+            return ctx->fail(PQ_ERR_LIMIT, "a symbol occurs 2^32 times or more in the text");
+    }
     if (d == 0) {
         ctx->C[0] = 0;
         ctx->C[1] = 0;
@@ -724,7 +940,7 @@ int pq_build_occ_dir(pq_ctx_t *ctx, int d)
     }
     D.cf[0] = 0;
     for (int c = 1; c <= 4; c++)
-        D.cf[c] = (uint32_t) (ctx->C[c] + fix[c - 1]);
+        D.cf[c] = ctx->C[c] + fix[c - 1];
     D.built = true;
     return PQ_OK;
 }
@@ -800,8 +1016,10 @@ extern "C" int pq_upload_reads(pq_ctx_t *ctx, const uint8_t *codes, const uint64
     ctx->n = n;
     ctx->N = off[n];
     ctx->t = min_overlap;
-    if (ctx->N >= 0xFFFFFFF0ull || n >= 0xFFFFFFF0ull)
-        return ctx->fail(PQ_ERR_LIMIT, "text of 2^32 codes or more needs 64-bit positions");
+    if (n >= 0xFFFFFFF0ull)
+        return ctx->fail(PQ_ERR_LIMIT, "2^32 reads or more");
+    // 64-bit rows and positions from 2^32 codes on (or when asked for)
+    ctx->wide = ctx->opt_row_bytes == 8 || (ctx->opt_row_bytes == 0 && ctx->N >= 0xFFFFFFF0ull);
     ctx->h_off.assign(off, off + n + 1);
     uint32_t lmin = UINT32_MAX, lmax = 0;
     memset(ctx->sfx, 0, sizeof(ctx->sfx));
@@ -837,8 +1055,8 @@ extern "C" int pq_upload_reads(pq_ctx_t *ctx, const uint8_t *codes, const uint64
         (rc = pq_alloc(ctx, (void **) &pc, nwords64 * 4)) ||
         (rc = pq_alloc(ctx, (void **) &ctx->dir[0].text2, nwords32 * 8)) ||
         (rc = pq_alloc(ctx, (void **) &ctx->dir[1].text2, nwords32 * 8)) ||
-        (rc = pq_alloc(ctx, (void **) &ctx->dir[0].sa, N * 4 + 4)) ||
-        (rc = pq_alloc(ctx, (void **) &ctx->dir[1].sa, N * 4 + 4))) {
+        (rc = pq_alloc(ctx, (void **) &ctx->dir[0].sa, (N + 1) * (ctx->wide ? 8 : 4))) ||
+        (rc = pq_alloc(ctx, (void **) &ctx->dir[1].sa, (N + 1) * (ctx->wide ? 8 : 4)))) {
         pq_free(d_codes);
         pq_free(pc);
         return rc;

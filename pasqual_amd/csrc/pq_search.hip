@@ -40,8 +40,8 @@ static const uint32_t EXT_SHAPES[3][2] = {{8, 16}, {16, 32}, {32, 64}};
 #define KMAX_GLOBAL 65536
 #define GLOBAL_SLOTS 256
 
-struct ChainBuf {
-    uint2 *lc;               // (partner start, count)
+template <typename R> struct ChainBufT {
+    RowCount<R> *lc;         // (partner start, count)
     uint16_t *o;             // pattern length
     uint32_t *n;             // entries per (read, chain)
     uint32_t cap;            // entries per chain
@@ -58,10 +58,11 @@ __device__ __forceinline__ unsigned chain_sym(const uint64_t *fwt, uint64_t s, u
     return c + 1u;
 }
 
-__global__ void __launch_bounds__(256) k_find(Geom G, DirView fw, DirView bw, uint64_t r0,
-                                              uint64_t nr, uint32_t t, uint4 Clo, uint32_t C5,
-                                              const Seed *seeds, uint32_t kseed, ChainBuf out,
-                                              unsigned long long *stats)
+template <typename R>
+__global__ void __launch_bounds__(256) k_find(Geom G, DirViewT<R> fw, DirViewT<R> bw, uint64_t r0,
+                                              uint64_t nr, uint32_t t, CTab<R> C,
+                                              const SeedT<R> *seeds, uint32_t kseed,
+                                              ChainBufT<R> out, unsigned long long *stats)
 {
     uint64_t tid = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t rl = tid >> 2;
@@ -73,12 +74,14 @@ __global__ void __launch_bounds__(256) k_find(Geom G, DirView fw, DirView bw, ui
     uint32_t L = pq_read_len(G, r);
     const bool fwsearch = (ch == 0 || ch == 3);
     const OccLine *occ = fwsearch ? fw.occ : bw.occ;
-    const uint32_t *cf = fwsearch ? fw.cf : bw.cf;
+    R cf[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+        cf[q] = fwsearch ? fw.cf[q] : bw.cf[q];
     const uint32_t minsize = (ch == 0 || ch == 2) ? 2u : 1u;
-    // C[0..5] = 0, 0 (A), C[2], C[3], C[4], C[5] (index.c:367-372)
-    const uint32_t Cc[6] = {0u, 0u, Clo.y, Clo.z, Clo.w, C5};
-    uint32_t o, l, size, ll;
-    Seed sd = {0, SEED_NONE, 0};
+    uint32_t o, size;
+    R l, ll;
+    SeedT<R> sd = {0, SEED_NONE, 0};
     if (seeds && kseed > 0 && kseed < L)
         sd = seeds[rl * 4 + ch];
     if (sd.size != SEED_NONE) {
@@ -87,10 +90,11 @@ __global__ void __launch_bounds__(256) k_find(Geom G, DirView fw, DirView bw, ui
         size = sd.size;
         ll = sd.ll;
     } else {
+        // C[0..5] = 0, 0 (A), C[2], C[3], C[4], C[5] (index.c:367-372)
         unsigned c = chain_sym(fw.text2, s, L, ch, 1);
         o = 1;
-        l = Cc[c];
-        size = Cc[c + 1] - Cc[c];
+        l = pq_pick6(C.c, c);
+        size = (uint32_t) (pq_pick6(C.c, c + 1) - l);
         ll = l;
     }
     uint32_t nout = 0;
@@ -112,16 +116,15 @@ __global__ void __launch_bounds__(256) k_find(Geom G, DirView fw, DirView bw, ui
         if (o >= t) {
             uint32_t cnt = b.d - a.d;
             if (cnt && nout < out.cap) {
-                out.lc[base + nout] = make_uint2(ll, cnt);
+                out.lc[base + nout] = RowCount<R>{ll, cnt};
                 out.o[base + nout] = (uint16_t) o;
                 nout++;
             }
         }
         if (!more)
             break;
-        const uint32_t cfc = c == 1 ? cf[1] : c == 2 ? cf[2] : c == 3 ? cf[3] : cf[4];
         ll += b.lt - a.lt;
-        l = cfc + a.c;
+        l = pq_pick5(cf, c) + a.c;
         size = b.c - a.c;
         o++;
     }
@@ -207,9 +210,9 @@ template <int GW> __device__ __forceinline__ uint32_t group_count(bool x)
 // groups of a wave work on different reads).  keys/meta/emitf/pre: this group's
 // storage (LDS or global) for kmax items.  Returns false, writing nothing, if a
 // side has more than kmax items.
-template <int GW>
-__device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw,
-                             const ChainBuf &cb, uint64_t rl, uint64_t r, uint32_t kw,
+template <int GW, typename R>
+__device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirViewT<R> &bw,
+                             const ChainBufT<R> &cb, uint64_t rl, uint64_t r, uint32_t kw,
                              uint64_t *keys, ItemMeta *meta, uint8_t *emitf, uint32_t *pre,
                              uint32_t kmax, const ExtOut &out)
 {
@@ -271,11 +274,11 @@ __device__ bool extract_read(const Geom &G, const DirView &fw, const DirView &bw
             bool first = e < nA;
             int ch = first ? chA : chB;
             uint64_t at = first ? (rl * 4 + chA) * cb.cap + e : (rl * 4 + chB) * cb.cap + (e - nA);
-            uint2 lc = cb.lc[at];
+            RowCount<R> lc = cb.lc[at];
             uint32_t o = cb.o[at];
             // FB, BB: partner is the backward index; FF, BF: the forward index
             bool pbw = (ch == 0 || ch == 3);
-            uint32_t g = (pbw ? bw.sa : fw.sa)[lc.x + (q - pre[e])];
+            uint64_t g = (uint64_t) (pbw ? bw.sa : fw.sa)[(uint64_t) lc.x + (q - pre[e])];
             uint64_t m = pq_read_of(G, g);
             uint32_t Lm = pq_read_len(G, m);
             ItemMeta mt;
@@ -422,9 +425,10 @@ __device__ __forceinline__ ExtWave ext_wave(char *base, uint32_t kmax, uint32_t 
 // grid-stride so that the grid need not match the count known only on the
 // device.  A read with more than KMAX items on a side is appended to
 // out.overflow_reads for the next tier.
-template <uint32_t KMAX, int WAVES, int GW>
-__global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirView fw, DirView bw,
-                                                           ChainBuf cb, uint64_t r0, uint64_t nr,
+template <uint32_t KMAX, int WAVES, int GW, typename R>
+__global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirViewT<R> fw,
+                                                           DirViewT<R> bw, ChainBufT<R> cb,
+                                                           uint64_t r0, uint64_t nr,
                                                            const uint32_t *list,
                                                            const uint32_t *count, uint32_t kw,
                                                            ExtOut out)
@@ -437,8 +441,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirView fw, 
     for (uint64_t i = (uint64_t) blockIdx.x * GROUPS + grp; i < total;
          i += (uint64_t) gridDim.x * GROUPS) {
         uint64_t rl = list ? list[i] : i;
-        bool ok = extract_read<GW>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
-                                   w.pre, KMAX, out);
+        bool ok = extract_read<GW, R>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
+                                      w.pre, KMAX, out);
         if (!ok && (threadIdx.x & (GW - 1)) == 0) {
             uint32_t at = atomicAdd(out.n_overflow, 1u);
             out.overflow_reads[at] = (uint32_t) rl;
@@ -446,8 +450,10 @@ __global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirView fw, 
     }
 }
 
-__global__ void __launch_bounds__(64) k_extract_global(Geom G, DirView fw, DirView bw, ChainBuf cb,
-                                                       uint64_t r0, const uint32_t *list,
+template <typename R>
+__global__ void __launch_bounds__(64) k_extract_global(Geom G, DirViewT<R> fw, DirViewT<R> bw,
+                                                       ChainBufT<R> cb, uint64_t r0,
+                                                       const uint32_t *list,
                                                        const uint32_t *count, uint32_t kw,
                                                        char *gscratch, ExtOut out)
 {
@@ -455,8 +461,8 @@ __global__ void __launch_bounds__(64) k_extract_global(Geom G, DirView fw, DirVi
     const uint32_t nlist = *count;
     for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
         uint64_t rl = list[i];
-        bool ok = extract_read<64>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
-                                   w.pre, KMAX_GLOBAL, out);
+        bool ok = extract_read<64, R>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
+                                      w.pre, KMAX_GLOBAL, out);
         if (!ok && threadIdx.x == 0)
             atomicOr(out.flags, 2u);
     }
@@ -590,18 +596,8 @@ static int run_search_graph(pq_ctx_t *ctx, int mode, uint64_t n, uint64_t batch,
     return PQ_OK;
 }
 
-extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
+template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph)
 {
-    if (!ctx)
-        return PQ_ERR_ARG;
-    if (!ctx->dir[0].built || !ctx->dir[1].built)
-        return ctx->fail(PQ_ERR_STATE, "index not built");
-    const bool as_graph = (mode & PQ_SEARCH_GRAPH) != 0;
-    mode &= ~PQ_SEARCH_GRAPH;
-    if (mode != PQ_SEARCH_PLAIN && mode != PQ_SEARCH_LEARNED)
-        return ctx->fail(PQ_ERR_ARG, "unknown search mode");
-    if (mode == PQ_SEARCH_LEARNED && !ctx->learned)
-        return ctx->fail(PQ_ERR_STATE, "learned index not trained");
     const uint64_t n = ctx->n;
     const uint32_t t = ctx->t;
     ctx->searched = false;
@@ -624,19 +620,20 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         return PQ_OK;
     }
     Geom G = ctx->geom();
-    DirView fw = ctx->view(0), bw = ctx->view(1);
-    ChainBuf cb;
+    const DirViewT<R> fw = ctx->view<R>(0), bw = ctx->view<R>(1);
+    const CTab<R> Ct = ctx->ctab<R>();
+    ChainBufT<R> cb;
     cb.cap = ctx->Lmax > t ? ctx->Lmax - t : 1;
     uint32_t kw = (cb.cap + 31) / 32;
     if (kw > 8)
         return ctx->fail(PQ_ERR_LIMIT, "extension keys longer than 256 codes");
     // batch so the candidate buffer stays within ~8 GiB of the 288 GB
-    uint64_t per_read = 4ull * cb.cap * (sizeof(uint2) + sizeof(uint16_t)) + 16;
+    uint64_t per_read = 4ull * cb.cap * (sizeof(RowCount<R>) + sizeof(uint16_t)) + 16;
     uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(n, (8ull << 30) / per_read));
     uint64_t *rec = nullptr, *recF = nullptr, *app = nullptr, *tot = nullptr, *staged = nullptr;
     uint32_t *ovl = nullptr, *misc = nullptr;
     uint64_t edge_cap = std::max<uint64_t>(1 << 20, n / 4);
-    Seed *seeds = nullptr;
+    SeedT<R> *seeds = nullptr;
     char *gscratch = nullptr;
     // four events per batch (seed, find, extract boundaries), read after the loop
     const uint64_t nbatch = (n + batch - 1) / batch;
@@ -650,7 +647,7 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     cb.lc = nullptr;
     cb.o = nullptr;
     cb.n = nullptr;
-    if ((rc = pq_slot(ctx, SLOT_CAND_LC, batch * 4 * cb.cap * sizeof(uint2), &cb.lc)) ||
+    if ((rc = pq_slot(ctx, SLOT_CAND_LC, batch * 4 * cb.cap * sizeof(RowCount<R>), &cb.lc)) ||
         (rc = pq_slot(ctx, SLOT_CAND_O, batch * 4 * cb.cap * sizeof(uint16_t), &cb.o)) ||
         (rc = pq_slot(ctx, SLOT_CAND_N, batch * 4 * sizeof(uint32_t), &cb.n)) ||
         (rc = pq_slot(ctx, SLOT_RECB, (n + 1) * 8, &rec)) ||
@@ -663,7 +660,7 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         (rc = pq_slot(ctx, SLOT_GSCRATCH, GLOBAL_SLOTS * ext_wave_bytes(KMAX_GLOBAL, kw),
                       &gscratch)) ||
         (mode == PQ_SEARCH_LEARNED &&
-         (rc = pq_slot(ctx, SLOT_SEEDS, batch * 4 * sizeof(Seed), &seeds)))) {
+         (rc = pq_slot(ctx, SLOT_SEEDS, batch * 4 * sizeof(SeedT<R>), &seeds)))) {
         cleanup();
         return rc;
     }
@@ -684,9 +681,6 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     unsigned long long *fstats = (unsigned long long *) (misc + 8);   // misc[8..11]
     // the whole search as a sequence of stream operations (captured into a graph
     // in graph mode: kernels only, no memsets, no host synchronisation)
-    // C[2..4] travel in Clo.y..w, C[5] on its own (C[0] = C[1] = 0)
-    const uint4 Clo = make_uint4(0, (uint32_t) ctx->C[2], (uint32_t) ctx->C[3], (uint32_t) ctx->C[4]);
-    const uint32_t C5 = (uint32_t) ctx->C[5];
     auto enqueue = [&](bool timed) -> int {
     int rc = PQ_OK;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc, 16u);
@@ -719,36 +713,36 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
             break;
         if (timed)
             (void) hipEventRecord(e[1], ctx->stream);
-        hipLaunchKernelGGL(k_find, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0, ctx->stream, G, fw,
-                           bw, r0, nr, t, Clo, C5, seeds, kseed, cb, fstats);
+        hipLaunchKernelGGL(k_find<R>, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0, ctx->stream, G,
+                           fw, bw, r0, nr, t, Ct, seeds, kseed, cb, fstats);
         if (timed)
             (void) hipEventRecord(e[2], ctx->stream);
         const dim3 g1(pq_blocks(nr, groups1)), b1(64 * EXT_WAVES);
         const uint32_t *nl = nullptr;
         switch (shape) {
         case 0:
-            hipLaunchKernelGGL((k_extract_lds<16, EXT_WAVES, 8>), g1, b1, lds1, ctx->stream, G, fw,
+            hipLaunchKernelGGL((k_extract_lds<16, EXT_WAVES, 8, R>), g1, b1, lds1, ctx->stream, G, fw,
                                bw, cb, r0, nr, nl, nl, kw, eo);
             break;
         case 2:
-            hipLaunchKernelGGL((k_extract_lds<64, EXT_WAVES, 32>), g1, b1, lds1, ctx->stream, G,
+            hipLaunchKernelGGL((k_extract_lds<64, EXT_WAVES, 32, R>), g1, b1, lds1, ctx->stream, G,
                                fw, bw, cb, r0, nr, nl, nl, kw, eo);
             break;
         default:
-            hipLaunchKernelGGL((k_extract_lds<32, EXT_WAVES, 16>), g1, b1, lds1, ctx->stream, G,
+            hipLaunchKernelGGL((k_extract_lds<32, EXT_WAVES, 16, R>), g1, b1, lds1, ctx->stream, G,
                                fw, bw, cb, r0, nr, nl, nl, kw, eo);
         }
         const uint32_t *inC = eo.overflow_reads, *cntC = eo.n_overflow;
         if (shape < 2) {
-            hipLaunchKernelGGL((k_extract_lds<64, EXT_WAVES, 32>), dim3(gridB), b1, ldsB,
+            hipLaunchKernelGGL((k_extract_lds<64, EXT_WAVES, 32, R>), dim3(gridB), b1, ldsB,
                                ctx->stream, G, fw, bw, cb, r0, nr, eo.overflow_reads,
                                eo.n_overflow, kw, eoB);
             inC = eoB.overflow_reads;
             cntC = eoB.n_overflow;
         }
-        hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64>), dim3(grid2), dim3(64), lds2,
+        hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64, R>), dim3(grid2), dim3(64), lds2,
                            ctx->stream, G, fw, bw, cb, r0, nr, inC, cntC, kw, eoC);
-        hipLaunchKernelGGL(k_extract_global, dim3(GLOBAL_SLOTS), dim3(64), 0, ctx->stream, G, fw,
+        hipLaunchKernelGGL(k_extract_global<R>, dim3(GLOBAL_SLOTS), dim3(64), 0, ctx->stream, G, fw,
                            bw, cb, r0, eoC.overflow_reads, eoC.n_overflow, kw, gscratch, eo);
         // overflow counters back to zero for the next batch (misc[2], misc[4], misc[6])
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eo.n_overflow, 1u);
@@ -765,8 +759,9 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
                               enqueue, {(uint64_t) (uintptr_t) cb.lc, (uint64_t) (uintptr_t) cb.o,
                                         (uint64_t) (uintptr_t) cb.n, (uint64_t) (uintptr_t) seeds,
                                         (uint64_t) (uintptr_t) gscratch, (uint64_t) (uintptr_t) ovl,
-                                        (uint64_t) t, (uint64_t) kw, (uint64_t) Clo.y,
-                                        (uint64_t) Clo.z, (uint64_t) Clo.w, (uint64_t) C5,
+                                        (uint64_t) t, (uint64_t) kw, (uint64_t) Ct.c[2],
+                                        (uint64_t) Ct.c[3], (uint64_t) Ct.c[4], (uint64_t) Ct.c[5],
+                                        (uint64_t) sizeof(R),
                                         (uint64_t) (uintptr_t) G.off, (uint64_t) (uintptr_t) G.ends,
                                         (uint64_t) (uintptr_t) G.end_rank, (uint64_t) G.L,
                                         (uint64_t) G.Lmax, (uint64_t) G.N, (uint64_t) cb.cap});
@@ -821,6 +816,22 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     }
     cleanup();
     return rc;
+}
+
+extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (!ctx->dir[0].built || !ctx->dir[1].built)
+        return ctx->fail(PQ_ERR_STATE, "index not built");
+    const bool as_graph = (mode & PQ_SEARCH_GRAPH) != 0;
+    mode &= ~PQ_SEARCH_GRAPH;
+    if (mode != PQ_SEARCH_PLAIN && mode != PQ_SEARCH_LEARNED)
+        return ctx->fail(PQ_ERR_ARG, "unknown search mode");
+    if (mode == PQ_SEARCH_LEARNED && !ctx->learned)
+        return ctx->fail(PQ_ERR_STATE, "learned index not trained");
+    return ctx->wide ? find_overlaps_t<uint64_t>(ctx, mode, as_graph)
+                     : find_overlaps_t<uint32_t>(ctx, mode, as_graph);
 }
 
 extern "C" int pq_graph_sizes(pq_ctx_t *ctx, uint64_t *n_reads, uint64_t *n_edges)
