@@ -6,7 +6,10 @@ Default workload ("repeats and errors"): 5 Mbp genome with 100 interspersed
 copies of ten 1 kbp repeat units at 1% divergence, 100 bp reads at 40x with 0.5%
 substitutions, t = 63, 32-bit index.  --config 3 / 4: 50 Mbp genome, read
 lengths uniform in 36..250 at 30x, 0.3% substitutions, t = 31 (4 = the same
-reads with the learned index as the headline search).
+reads with the learned index as the headline search).  --config 5: 200 Mbp
+genome with 5% repeat families, 150 bp reads at 30x (40 M reads, 6 G suffixes
+per direction: 64-bit device rows, suffix sort in slices), t = 63, learned index
+as the headline search.
 
 A step = one index build (SA, BWT occurrence lines, reads_index for the forward
 and the backward text) + one overlap search (candidate intervals, irreducible
@@ -40,19 +43,24 @@ WORKLOADS = {
        "U64 index (device rows 32-bit: text < 2^32 codes)",
     4: "50 Mbp uniform genome, read lengths U{36..250} at 30x, 0.3% substitutions, t=31, "
        "U64 index (device rows 32-bit), learned index k=21, 64 hidden, 4096 leaves",
+    5: "200 Mbp genome (uniform + 5% repeat families of 1-10 kbp at 0.5-3% divergence), "
+       "150 bp reads at 30x, 0.3% substitutions, t=63, U64 index (device rows 64-bit), "
+       "learned index k=21, 64 hidden, 4096 leaves",
 }
+# configurations whose headline search is the learned one
+LEARNED_HEADLINE = (4, 5)
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline_reference(fasta, L, t, n_expected):
+def cpu_baseline_reference(fasta, L, t, n_expected, exe_name="pasqual_omp_u32"):
     """The reference OpenMP program (oracle/_ref) on the same reads file, timed
     by its own timers (index.c:643-680, string_graph.c:737-1044)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pq_oracle as orc
-    exe = os.path.join(orc.REFDIR, "pasqual_omp_u32")
+    exe = os.path.join(orc.REFDIR, exe_name)
     if not os.path.exists(exe):
         return None
     threads = min(16, os.cpu_count() or 1)
@@ -60,7 +68,7 @@ def cpu_baseline_reference(fasta, L, t, n_expected):
     with tempfile.TemporaryDirectory() as wd:
         t0 = time.time()
         r = subprocess.run([exe, "--singlereads", fasta, "-l", str(L), "-t", str(t)], cwd=wd,
-                           capture_output=True, text=True, env=env, timeout=600)
+                           capture_output=True, text=True, env=env, timeout=900)
         wall = time.time() - t0
         digest = _digest(os.path.join(wd, "contigs.fa"))
     if r.returncode != 0:
@@ -73,11 +81,12 @@ def cpu_baseline_reference(fasta, L, t, n_expected):
     fw, bw, graph = takes[1], takes[2], takes[3]
     return {"value": round(n / (graph / 1e3), 1), "unit": "reads/s", "cores": threads,
             "kind": "reference",
-            "sample": f"the full workload ({n} reads), reference pasqual_omp_u32, "
+            "sample": f"the full workload ({n} reads), reference {exe_name}, "
                       f"OMP_NUM_THREADS={threads}, one run, overlap-graph timer "
-                      "(string_graph.c:737-1044)",
+                      "(string_graph.c:737-1044)", "reads": n,
             "sa_build_gsuffix_per_s": round(2 * n * L / ((fw + bw) / 1e3) / 1e9, 5),
-            "wall_s": round(wall, 1), "reads_match_gpu": n == n_expected,
+            "wall_s": round(wall, 1),
+            "reads_match_gpu": None if n_expected is None else n == n_expected,
             "contigs_sha256": digest}
 
 
@@ -142,9 +151,33 @@ def cpu_baseline_port(pq, config, t):
             "wall_s": round(ts + tb, 1)}
 
 
+def cpu_baseline_reference_sample(pq, config, t):
+    """The reference OpenMP program (pasqual_omp_u64) on a bounded sample of a
+    workload too large for it: the same generator over a 10 Mbp genome."""
+    cfg = pq.synth_preset(config)
+    cfg.genome_len = 10_000_000
+    L = cfg.len_min
+    wd = tempfile.mkdtemp(prefix="pqbase_")
+    fasta = os.path.join(wd, "sample.fa")
+    pq.synth_fasta(cfg, fasta)
+    base = cpu_baseline_reference(fasta, L, t, None, exe_name="pasqual_omp_u64")
+    os.remove(fasta)
+    if base:
+        base["sample"] = (f"{base['reads']} reads of the same generator over a 10 Mbp genome "
+                          "(same repeat families, coverage, length and errors), reference "
+                          f"pasqual_omp_u64, OMP_NUM_THREADS={base['cores']}, one run, "
+                          "overlap-graph timer (string_graph.c:737-1044)")
+    return base
+
+
 def load_workload(pq, config, np):
     cfg = pq.synth_preset(config)
     t = cfg.min_overlap
+    if config == 5:
+        # 40 M reads: the generator's reads as they are (the 6.8 GB reads file
+        # and its import are not part of the measured path)
+        codes, off = pq.synth_reads(cfg)
+        return codes, off, t, None, None, "as generated (no duplicate removal)"
     if cfg.len_min == cfg.len_max:
         # fixed-length reads go through the reference's import (duplicate removal)
         L = cfg.len_min
@@ -175,7 +208,7 @@ def main():
               "(the overlap search shards by read ranges with no exchange; see the "
               "design notes)", file=sys.stderr)
         sys.exit(2)
-    if args.config == 4:
+    if args.config in LEARNED_HEADLINE:
         args.learned = 1
 
     import numpy as np
@@ -287,6 +320,10 @@ def main():
         pms = ctx.probe_copy(args.probe_copy << 20)
         log(f"copy probe: {args.probe_copy} MiB in {pms:.3f} ms "
             f"({2 * (args.probe_copy << 20) / pms / 1e6:.1f} GB/s read+write)")
+    row_bytes, slices = ctx.layout()
+    lens = np.diff(off)
+    read_len = int(lens[0]) if int(lens.min()) == int(lens.max()) else \
+        f"{int(lens.min())}..{int(lens.max())}"
     ms_per_step = total_s * 1e3 / args.steps
     avg_search = sum(search_ms) / len(search_ms)
     avg_build = sum(build_ms) / len(build_ms)
@@ -327,9 +364,11 @@ def main():
                 dropin["contigs_identical_to_reference"] = (
                     dropin["contigs_sha256"] is not None and
                     dropin["contigs_sha256"] == base["contigs_sha256"])
+        elif args.config == 5:
+            base = cpu_baseline_reference_sample(pq, args.config, t)
         else:
             base = cpu_baseline_port(pq, args.config, t)
-    headline_learned = args.config == 4
+    headline_learned = args.config in LEARNED_HEADLINE
     value = learned["reads_per_s"] if headline_learned else reads_per_s
     line = {
         "metric": "overlap-search reads/s + SA build Gsuffix/s, learned vs plain index, 1 GPU",
@@ -342,11 +381,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32",
+        "dtype": "u64" if row_bytes == 8 else "u32",
         "data": "synthetic (seeded generator, pasqual_amd/csrc/host/pq_synth.c)",
         "config": {"workload": f"config {args.config}: {WORKLOADS[args.config]}",
                    "reads": n, "suffixes_per_direction": N,
-                   "read_len": L if L is not None else "36..250", "min_overlap": t,
+                   "read_len": read_len, "min_overlap": t,
+                   "device_rows": f"{8 * row_bytes}-bit", "sa_slices_per_direction": slices,
                    "search": "learned" if headline_learned else "plain"},
         "plain_search_reads_per_s": round(reads_per_s, 1),
         "sa_build_gsuffix_per_s": round(gsuffix_per_s, 3),

@@ -450,28 +450,78 @@ __global__ void k_class_compact(const uint32_t *flags, const uint32_t *pos, cons
         out[pos[k]] = make_uint2(starts[k], starts[k + 1] - starts[k]);
 }
 
-// Runs of equal chunk keys inside an oversized run (offset `at` in the SA):
-// those still tied past the chunk go to the next level.
-__global__ void k_push_subruns(const uint64_t *K, const uint32_t *starts, uint32_t nruns,
-                               uint32_t at, SegLists out)
-{
-    uint64_t k = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = k < nruns;
-    uint32_t a = valid ? starts[k] : 0;
-    uint32_t b = valid ? starts[k + 1] : 0;
-    bool unres = valid && b - a > 1 && (K[a] & 31ull) == 30ull;
-    push_seg(out, unres, at + a, b - a);
-}
+// ---- oversized runs: every run of one level at once -------------------------------
+//
+// The elements of all oversized runs of a level are concatenated (run j from
+// offset off_j on), keyed by their 29-code chunk at the level's offset, sorted
+// stably by that key and then stably by run: every run's elements end up in
+// chunk-key order, ties in their previous order, as a radix sort of each run
+// alone would leave them.  They go back to the run's range of the suffix array;
+// the runs of equal chunk keys still tied past the chunk go to the next level.
+// run[j] = (start in the slice, size, off_j, 0).
 
 template <typename R>
-__global__ void k_chunk_keys(Geom G, const uint64_t *t2, const R *sa, uint64_t n, uint32_t base,
-                             uint64_t *K)
+__global__ void __launch_bounds__(256) k_big_load(Geom G, const uint64_t *t2, const R *sa,
+                                                  uint32_t base, const uint4 *run, uint32_t nrun,
+                                                  uint32_t *rid, R *pos, uint64_t *korig,
+                                                  uint64_t *K, uint32_t *E)
 {
-    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        uint64_t g = sa[i];
-        K[i] = chunk_key(t2, g, suf_rem(G, g), base);
+    for (uint32_t j = blockIdx.x; j < nrun; j += gridDim.x) {
+        const uint4 r = run[j];
+        for (uint32_t i = threadIdx.x; i < r.y; i += blockDim.x) {
+            const uint32_t e = r.z + i;
+            const uint64_t g = sa[r.x + i];
+            const uint64_t k = chunk_key(t2, g, suf_rem(G, g), base);
+            rid[e] = j;
+            pos[e] = (R) g;
+            korig[e] = k;
+            K[e] = k;
+            E[e] = e;
+        }
     }
+}
+
+// Run of every element in chunk-key order: the key of the stable sort by run.
+__global__ void k_big_rids(const uint32_t *E, const uint32_t *rid, uint64_t T, uint32_t *RK)
+{
+    uint64_t o = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (o < T)
+        RK[o] = rid[E[o]];
+}
+
+// Sorted element o back to its run's range of the suffix array; its chunk key;
+// head flag of the runs of equal (run, chunk key).
+template <typename R>
+__global__ void k_big_store(const uint32_t *E, const uint32_t *rid, const R *pos,
+                            const uint64_t *korig, const uint4 *run, uint64_t T, R *sa,
+                            uint64_t *KS, uint32_t *flags)
+{
+    uint64_t o = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= T)
+        return;
+    const uint32_t e = E[o];
+    const uint4 r = run[rid[e]];
+    sa[r.x + (o - r.z)] = pos[e];
+    const uint64_t k = korig[e];
+    KS[o] = k;
+    flags[o] = (o == r.z || korig[E[o - 1]] != k) ? 1u : 0u;
+}
+
+__global__ void k_big_subruns(const uint32_t *starts, uint32_t nsub, const uint32_t *E,
+                              const uint32_t *rid, const uint4 *run, const uint64_t *KS,
+                              SegLists out)
+{
+    uint64_t k = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = k < nsub;
+    uint32_t a = valid ? starts[k] : 0;
+    uint32_t b = valid ? starts[k + 1] : 0;
+    uint32_t at = 0;
+    if (valid) {
+        const uint4 r = run[rid[E[a]]];
+        at = r.x + (a - r.z);
+    }
+    bool unres = valid && b - a > 1 && (KS[a] & 31ull) == 30ull;
+    push_seg(out, unres, at, b - a);
 }
 
 // ---- driver ---------------------------------------------------------------------------
@@ -505,25 +555,37 @@ static uint32_t lcap_for(int W)
     return W == 0 ? 0u : lds_cap(W);
 }
 
+// Runs per launch: a dispatch covers fewer than 2^32 work-items (64 lanes per
+// run for the widest group sort, 256 per run for the LDS sort).
+#define SORT_RUNS_PER_LAUNCH (1u << 22)
+
 template <int W, typename R>
 static void launch_sorts(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, R *sa, uint8_t *bwt8,
                          uint32_t base, uint2 *const items[CLS_BIG], const uint32_t cnt[CLS_BIG])
 {
-    if (cnt[0])
+    for (uint32_t at = 0; at < cnt[0]; at += SORT_RUNS_PER_LAUNCH) {
+        const uint32_t m = std::min(cnt[0] - at, SORT_RUNS_PER_LAUNCH);
         hipLaunchKernelGGL((k_sort_groups<RUN_SMALL, W, R>),
-                           dim3(pq_blocks((uint64_t) cnt[0] * RUN_SMALL, 256)), dim3(256), 0,
-                           ctx->stream, G, t2, sa, bwt8, base, items[0], cnt[0]);
-    if (cnt[1])
+                           dim3(pq_blocks((uint64_t) m * RUN_SMALL, 256)), dim3(256), 0,
+                           ctx->stream, G, t2, sa, bwt8, base, items[0] + at, m);
+    }
+    for (uint32_t at = 0; at < cnt[1]; at += SORT_RUNS_PER_LAUNCH) {
+        const uint32_t m = std::min(cnt[1] - at, SORT_RUNS_PER_LAUNCH);
         hipLaunchKernelGGL((k_sort_groups<RUN_HALF, W, R>),
-                           dim3(pq_blocks((uint64_t) cnt[1] * RUN_HALF, 256)), dim3(256), 0,
-                           ctx->stream, G, t2, sa, bwt8, base, items[1], cnt[1]);
-    if (cnt[2])
+                           dim3(pq_blocks((uint64_t) m * RUN_HALF, 256)), dim3(256), 0,
+                           ctx->stream, G, t2, sa, bwt8, base, items[1] + at, m);
+    }
+    for (uint32_t at = 0; at < cnt[2]; at += SORT_RUNS_PER_LAUNCH) {
+        const uint32_t m = std::min(cnt[2] - at, SORT_RUNS_PER_LAUNCH);
         hipLaunchKernelGGL((k_sort_groups<RUN_WAVE, W, R>),
-                           dim3(pq_blocks((uint64_t) cnt[2] * RUN_WAVE, 256)), dim3(256), 0,
-                           ctx->stream, G, t2, sa, bwt8, base, items[2], cnt[2]);
-    if (cnt[3])
-        hipLaunchKernelGGL((k_sort_lds<W, R>), dim3(cnt[3]), dim3(256), 0, ctx->stream, G, t2, sa,
-                           bwt8, base, items[3]);
+                           dim3(pq_blocks((uint64_t) m * RUN_WAVE, 256)), dim3(256), 0,
+                           ctx->stream, G, t2, sa, bwt8, base, items[2] + at, m);
+    }
+    for (uint32_t at = 0; at < cnt[3]; at += SORT_RUNS_PER_LAUNCH) {
+        const uint32_t m = std::min(cnt[3] - at, SORT_RUNS_PER_LAUNCH);
+        hipLaunchKernelGGL((k_sort_lds<W, R>), dim3(m), dim3(256), 0, ctx->stream, G, t2, sa,
+                           bwt8, base, items[3] + at);
+    }
 }
 
 template <typename R>
@@ -612,18 +674,56 @@ static int sort_runs(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned 
     }
     uint32_t base = K;
     SegLists SL;
+    std::vector<uint4> hrun;
     while (!big.empty() && rc == PQ_OK) {
-        uint64_t maxn = 0, bigsum = 0;
-        for (const uint2 &s : big) {
-            maxn = std::max<uint64_t>(maxn, s.y);
-            bigsum += s.y;
+        const uint32_t nb = (uint32_t) big.size();
+        hrun.resize(nb);
+        uint64_t T = 0;
+        for (uint32_t j = 0; j < nb; j++) {
+            hrun[j] = make_uint4(big[j].x, big[j].y, (uint32_t) T, 0u);
+            T += big[j].y;
         }
-        uint64_t *KT = (uint64_t *) ctx->tmp(SLOT_BIGK, maxn * 8 * 2 + 64);
-        R *VT = (R *) ctx->tmp(SLOT_BIGV, maxn * sizeof(R) + 64);
-        if (!KT || !VT)
-            return ctx->fail(PQ_ERR_NOMEM, "oversized run scratch");
-        // every run pushed from this level holds at least two elements of it
-        const uint32_t scap = (uint32_t) (bigsum / 2 + 2);
+        uint4 *run = nullptr;
+        uint64_t *KA = nullptr;
+        char *VA = nullptr;
+        if ((rc = pq_slot(ctx, SLOT_BIGRUNS, (size_t) nb * sizeof(uint4), &run)) ||
+            (rc = pq_slot(ctx, SLOT_BIGK, T * 8 * 4 + 64, &KA)) ||
+            (rc = pq_slot(ctx, SLOT_BIGV, T * (5 * 4 + sizeof(R)) + 64, &VA)))
+            return rc;
+        uint64_t *Kc = KA, *Ka = KA + T, *korig = KA + 2 * T, *KS = KA + 3 * T;
+        R *pos = (R *) VA;   // first: 8-byte aligned for 64-bit positions
+        uint32_t *rid = (uint32_t *) (pos + T), *Ec = rid + T, *Ea = Ec + T, *RKc = Ea + T,
+                 *RKa = RKc + T;
+        PQ_CHECK(hipMemcpyAsync(run, hrun.data(), (size_t) nb * sizeof(uint4),
+                                hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_big_load<R>, dim3(std::min<uint32_t>(nb, 65536)), dim3(256), 0,
+                           ctx->stream, G, t2, (const R *) S.sa, base, run, nb, rid, pos, korig,
+                           Kc, Ec);
+        // stable by chunk key, then stable by run
+        for (unsigned p = 0; p < 8 && rc == PQ_OK; p++) {
+            rc = pq_radix_pass(ctx, Kc, Ec, Ka, Ea, T, 8 * p);
+            std::swap(Kc, Ka);
+            std::swap(Ec, Ea);
+        }
+        if (rc)
+            return rc;
+        hipLaunchKernelGGL(k_big_rids, dim3(pq_blocks(T, 256)), dim3(256), 0, ctx->stream, Ec,
+                           rid, T, RKc);
+        for (unsigned p = 0; p < 4 && (p == 0 || ((nb - 1) >> (8 * p))); p++) {
+            if ((rc = pq_radix_pass(ctx, RKc, Ec, RKa, Ea, T, 8 * p)))
+                return rc;
+            std::swap(RKc, RKa);
+            std::swap(Ec, Ea);
+        }
+        hipLaunchKernelGGL(k_big_store<R>, dim3(pq_blocks(T, 256)), dim3(256), 0, ctx->stream, Ec,
+                           rid, (const R *) pos, korig, run, T, S.sa, KS, S.flags);
+        uint32_t nsub = 0;
+        if ((rc = pq_scan_u32(ctx, S.flags, S.pos, T, &nsub)))
+            return rc;
+        hipLaunchKernelGGL(k_compact_heads, dim3(pq_blocks(T, 256)), dim3(256), 0, ctx->stream,
+                           S.flags, S.pos, T, S.starts);
+        // every run pushed from this level holds at least two of its elements
+        const uint32_t scap = (uint32_t) (T / 2 + 2);
         uint2 *sbuf = nullptr;
         if ((rc = pq_slot(ctx, SLOT_SUBLISTS, (size_t) scap * NCLS * sizeof(uint2), &sbuf)))
             return rc;
@@ -633,33 +733,18 @@ static int sort_runs(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned 
         }
         SL.count = S.counts;
         SL.overflow = S.counts + NCLS;
-        uint32_t nb = base + CHUNK;
-        SL.lcap = lcap_for(words_for(ctx->Lmax > nb ? ctx->Lmax - nb : 0));
+        const uint32_t nbase = base + CHUNK;
+        SL.lcap = lcap_for(words_for(ctx->Lmax > nbase ? ctx->Lmax - nbase : 0));
         PQ_CHECK(hipMemsetAsync(S.counts, 0, 64, ctx->stream));
-        for (const uint2 &s : big) {
-            uint64_t *keys = KT, *keys2 = KT + maxn;
-            R *run = S.sa + s.x;
-            hipLaunchKernelGGL(k_chunk_keys<R>, dim3(pq_blocks(s.y, 256)), dim3(256), 0,
-                               ctx->stream, G, t2, run, (uint64_t) s.y, base, keys);
-            for (unsigned p = 0; p < 8 && rc == PQ_OK; p++) {
-                const uint64_t *ki = (p & 1) ? keys2 : keys;
-                const R *vi = (p & 1) ? VT : run;
-                uint64_t *ko = (p & 1) ? keys : keys2;
-                R *vo = (p & 1) ? run : VT;
-                rc = pq_radix_pass(ctx, ki, vi, ko, vo, s.y, 8 * p);
-            }
-            uint32_t nsub = 0;
-            if (rc || (rc = find_runs<uint64_t>(ctx, keys, s.y, S.flags, S.pos, S.starts, &nsub)))
-                return rc;
-            hipLaunchKernelGGL(k_push_subruns, dim3(pq_blocks(nsub, 256)), dim3(256), 0,
-                               ctx->stream, keys, S.starts, nsub, s.x, SL);
-        }
+        hipLaunchKernelGGL(k_big_subruns, dim3(pq_blocks(nsub, 256)), dim3(256), 0, ctx->stream,
+                           S.starts, nsub, Ec, rid, run, KS, SL);
+        PQ_CHECK(hipGetLastError());
         uint32_t c5[NCLS + 1];
         PQ_CHECK(hipMemcpyAsync(c5, S.counts, (NCLS + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
         PQ_CHECK(hipStreamSynchronize(ctx->stream));
         if (c5[NCLS])
             return ctx->fail(PQ_ERR_LIMIT, "suffix sort run list overflow");
-        base += CHUNK;
+        base = nbase;
         uint2 *sl[CLS_BIG] = {SL.items[0], SL.items[1], SL.items[2], SL.items[3]};
         if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, base, sl, c5)))
             return rc;

@@ -97,3 +97,35 @@ def test_layout_edge_cases(wctx):
         _check_against_oracle(wctx, codes, off, t)
         wctx.find_overlaps(pq.SEARCH_PLAIN)
         _same_graph(wctx.graph(), orc.search_explicit(codes, off, t), "wide edge case")
+
+
+def test_repeat_families_wide_sliced(wctx, ctx):
+    # the large workload's generator over a 1 Mbp genome (200k reads of 150,
+    # 30M suffixes per direction): repeat families give many buckets too large
+    # for one sort, refined level by level; 64-bit rows, slices of 6M suffixes
+    cfg = pq.synth_preset(5)
+    cfg.genome_len = 1_000_000
+    codes, off = pq.synth_reads(cfg)
+    wctx.set_layout(8, 6_000_000)
+    wctx.upload(codes, off, 63)
+    wctx.build_index()
+    rb, slices = wctx.layout()
+    assert rb == 8 and slices >= 5
+    Cc, sfx, pfx = orc.counts(codes, off)
+    C2, s2, p2 = wctx.counts()
+    assert np.array_equal(C2, Cc) and np.array_equal(s2, sfx) and np.array_equal(p2, pfx)
+    ctx.upload(codes, off, 63)   # default layout: 32-bit rows, one slice
+    ctx.build_index()
+    n = len(off) - 1
+    for d in (0, 1):
+        dt = orc.dir_text(codes, off, d)
+        sa = wctx.sa(d)
+        assert orc.check_sa(dt, off, sa) == -1, d
+        assert np.array_equal(sa, ctx.sa(d)), d
+        bo, ri = orc.bwt(dt, off, sa)
+        assert np.array_equal(wctx.bwt(d), bo), d
+        assert np.array_equal(wctx.reads_index(d), ri), d
+        assert np.array_equal(np.sort(ri), np.arange(n, dtype=np.uint64))
+    wctx.find_overlaps(pq.SEARCH_PLAIN)
+    ctx.find_overlaps(pq.SEARCH_PLAIN)
+    _same_graph(wctx.graph(), ctx.graph(), "repeat families, wide vs default")
