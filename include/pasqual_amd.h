@@ -158,7 +158,7 @@ typedef struct {
     uint64_t probes[2];      /* key reads of those lookups */
     uint64_t table_size[2];  /* distinct D-code prefixes per index */
     uint64_t absent;         /* chains whose depth-D pattern does not occur */
-    uint64_t fallback;       /* chains left to the plain start (tables disagree) */
+    uint64_t fallback;       /* always 0: a seed is the searched index's own table entry */
     uint32_t depth;          /* D: depth the backward search resumes from */
     uint32_t k;              /* model prefix length used */
 } pq_learned_stats_t;

@@ -98,7 +98,6 @@ struct pq_ctx {
     {
         DirViewT<R> v;
         v.text2 = dir[d].text2;
-        v.sa = (const R *) dir[d].sa;
         v.occ = dir[d].occ;
         v.reads_index = dir[d].reads_index;
         for (int c = 0; c < 5; c++)
@@ -160,12 +159,11 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d);
 int pq_build_occ_dir(pq_ctx_t *ctx, int d);
 void pq_learned_free(pq_ctx_t *ctx);
 // Seed of one (read, chain) of the candidate search at the learned depth:
-// search interval [l, l + size) and partner interval start ll; size SEED_NONE
-// makes the chain start from depth 1 instead.
+// search interval [l, l + size); size SEED_NONE makes the chain start from
+// depth 1 instead.
 template <typename R> struct SeedT {
     R l;
     uint32_t size;
-    R ll;
 };
 #define SEED_NONE 0xFFFFFFFFu
 // d_seeds: SeedT<uint32_t> or SeedT<uint64_t> per (read, chain) as ctx->wide says

@@ -33,10 +33,11 @@ struct __attribute__((aligned(64))) OccLine {
 };
 static_assert(sizeof(OccLine) == 64, "OccLine must be one 64-byte line");
 
-// Everything a search kernel needs about one direction, for row type R.
+// Everything a search kernel needs about one direction, for row type R (the
+// search reads the packed text, the occurrence lines and reads_index; not the
+// suffix array).
 template <typename R> struct DirViewT {
     const uint64_t *text2;
-    const R *sa;
     const OccLine *occ;
     const uint32_t *reads_index;
     R cf[5];                 // backward-step base: C[c] + fix[c]; cf[0] = 0
@@ -177,10 +178,10 @@ __device__ __forceinline__ void pq_line_ranks(const OccLine &ln, uint64_t line_i
 }
 
 // What one backward step by symbol c (1..4) needs from a line, before row j of
-// it: rows holding c, rows holding a symbol below c ('$' included) and '$'
-// rows.  Masks by selects on c, so nothing is indexed per lane.
+// it: rows holding c and '$' rows.  Masks by selects on c, so nothing is
+// indexed per lane.
 struct StepRanks {
-    uint32_t c, lt, d;
+    uint32_t c, d;
 };
 
 __device__ __forceinline__ StepRanks pq_step_ranks(const OccLine &ln, uint64_t line_idx,
@@ -191,23 +192,18 @@ __device__ __forceinline__ StepRanks pq_step_ranks(const OccLine &ln, uint64_t l
     const uint64_t a0 = ln.p0[0], a1 = ln.p0[1];
     const uint64_t b0 = ln.p1[0], b1 = ln.p1[1];
     const uint64_t c0 = ln.p2[0], c1 = ln.p2[1];
-    // symbol c: plane bits (c & 1, c & 2, c & 4); below c: '$' (c = 1),
-    // {$, A} (c = 2), {$, A, C} (c = 3), {$, A, C, G} (c = 4)
+    // symbol c: plane bits (c & 1, c & 2, c & 4)
     const bool s0 = c & 1u, s1 = c & 2u, s2 = c & 4u;
     uint64_t e0 = (s0 ? a0 : ~a0) & (s1 ? b0 : ~b0) & (s2 ? c0 : ~c0);
     uint64_t e1 = (s0 ? a1 : ~a1) & (s1 ? b1 : ~b1) & (s2 ? c1 : ~c1);
-    uint64_t l0 = ~c0 & (c == 4 ? ~0ull : c == 3 ? ~(a0 & b0) : c == 2 ? ~b0 : ~(a0 | b0));
-    uint64_t l1 = ~c1 & (c == 4 ? ~0ull : c == 3 ? ~(a1 & b1) : c == 2 ? ~b1 : ~(a1 | b1));
     uint64_t d0 = ~(a0 | b0 | c0), d1 = ~(a1 | b1 | c1);
     const uint32_t k0 = ln.cnt[0], k1 = ln.cnt[1], k2 = ln.cnt[2], k3 = ln.cnt[3];
     // '$' rows before the line: rows before it minus A..T rows before it
     const uint32_t dbase = (uint32_t) (line_idx * 128u) - (k0 + k1 + k2 + k3);
     const uint32_t kc = c == 1 ? k0 : c == 2 ? k1 : c == 3 ? k2 : k3;
-    const uint32_t kl = c == 1 ? 0u : c == 2 ? k0 : c == 3 ? k0 + k1 : k0 + k1 + k2;
     StepRanks r;
     r.c = kc + __popcll(e0 & m0) + __popcll(e1 & m1);
     r.d = dbase + __popcll(d0 & m0) + __popcll(d1 & m1);
-    r.lt = dbase + kl + __popcll(l0 & m0) + __popcll(l1 & m1);
     return r;
 }
 

@@ -64,10 +64,9 @@ static __host__ __device__ inline uint32_t mlp_nparams(uint32_t k, uint32_t h)
 }
 
 // search statistics per index direction: lookups, key probes, sum and max of
-// |exact entry - predicted entry|; patterns absent from the table; chains left
-// to the plain start
+// |exact entry - predicted entry|; patterns absent from the table
 struct QStats {
-    unsigned long long lookups[2], probes[2], win_sum[2], win_max[2], absent, fallback;
+    unsigned long long lookups[2], probes[2], win_sum[2], win_max[2], absent;
 };
 
 struct DirModel {
@@ -78,11 +77,8 @@ struct DirModel {
     float *iparams = nullptr;     // inference form (rows padded to MLP_K_MAX)
     uint64_t *range = nullptr;    // [xmin, xmax] per leaf
     uint32_t *knots = nullptr;    // (S + 1) entries per leaf
-    uint32_t *link = nullptr;     // entry of the reversed pattern in the other table
     double loss = 0, fit_loss = 0;
 };
-
-#define LINK_NONE 0xFFFFFFFFu
 
 struct Learned {
     uint32_t k = 21, h = 64, leaves = 4096, segs = 64, depth = 21;
@@ -463,7 +459,6 @@ template <typename R> struct ModelViewT {
     const float *iparams;
     const uint64_t *range;
     const uint32_t *knots;
-    const uint32_t *link;
 };
 
 // Parameters read by every lane of a wave at the same addresses: the pointer is
@@ -585,35 +580,9 @@ __device__ __forceinline__ u128 chain_pattern(const uint64_t *fwt, uint64_t s, u
     }
 }
 
-// Link every entry of table A to the entry of its reversed pattern in table B
-// (the partner interval of a bidirectional search), found through B's model.
-// An entry whose reversed pattern is missing (never, for tables of one read
-// set) keeps LINK_NONE.
-template <typename R>
-__global__ void k_table_links(ModelViewT<R> A, ModelViewT<R> B, uint32_t D, MlpDims d,
-                              uint32_t leaves, uint32_t S, uint32_t *linkA, uint32_t *linkB)
-{
-    uint64_t u = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= A.nU)
-        return;
-    u128 pat = A.key[u] & ~(u128) 1;
-    u128 q = rev_pattern(pat, D) | 1u;
-    uint64_t g = model_entry(wave_params(B.iparams), B, d, leaves, S,
-                             (uint64_t) ((q & ~(u128) 1) >> (128 - 2 * d.k)));
-    uint32_t probes = 0;
-    uint64_t v = lower_bound_from(B.key, B.nU, q, g, probes);
-    if (v < B.nU && B.key[v] == q) {
-        linkA[u] = (uint32_t) v;
-        linkB[v] = (uint32_t) u;
-    } else {
-        linkA[u] = LINK_NONE;
-    }
-}
-
 // Block of 256 threads = 64 reads; wave q handles chain q of those reads, so the
-// model of a wave is uniform.  Writes Seed{search interval, partner start} at
-// depth D for chains of reads longer than D (size 0 when the pattern does not
-// occur; SEED_NONE, the plain start, when the two tables disagree).
+// model of a wave is uniform.  Writes Seed{search interval} at depth D for
+// chains of reads longer than D (size 0 when the pattern does not occur).
 template <typename R>
 __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fwt, uint64_t r0,
                                                       uint64_t nr, uint32_t D, MlpDims d,
@@ -627,15 +596,14 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
     const int sdir = fwsearch ? 0 : 1;
     uint32_t probes[2] = {0, 0}, looks[2] = {0, 0};
     uint64_t win[2] = {0, 0};
-    uint32_t absent = 0, fallback = 0;
+    uint32_t absent = 0;
     if (rl < nr) {
         uint64_t r = r0 + rl;
         uint64_t s = pq_read_start(G, r);
         uint32_t L = pq_read_len(G, r);
-        SeedT<R> sd = {0, SEED_NONE, 0};
+        SeedT<R> sd = {0, SEED_NONE};
         const ModelViewT<R> &Sv = fwsearch ? fw : bw;
-        const ModelViewT<R> &Pv = fwsearch ? bw : fw;
-        if (D < L && Sv.nU > 0 && Pv.nU > 0) {
+        if (D < L && Sv.nU > 0) {
             const u128 pat = chain_pattern(fwt, s, L, ch, D);
             const u128 key = pat | 1u;
             uint64_t g1 = model_entry(wave_params(Sv.iparams), Sv, d, leaves, S,
@@ -644,17 +612,9 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
             looks[sdir]++;
             win[sdir] = u > g1 ? u - g1 : g1 - u;
             if (u < Sv.nU && Sv.key[u] == key) {
-                // partner: the reversed pattern's entry in the other table,
-                // linked when the index was learned
-                RowCount<R> iv = Sv.rows[u];
-                uint32_t u2 = Sv.link[u];
-                if (u2 != LINK_NONE && Pv.rows[u2].y == iv.y) {
-                    sd.l = iv.x;
-                    sd.size = iv.y;
-                    sd.ll = Pv.rows[u2].x;
-                } else {
-                    fallback = 1;
-                }
+                const RowCount<R> iv = Sv.rows[u];
+                sd.l = iv.x;
+                sd.size = iv.y;
             } else {
                 sd.size = 0;   // no suffix starts with the pattern: the chain is empty
                 absent = 1;
@@ -663,19 +623,19 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
         seeds[rl * 4 + ch] = sd;
     }
     // block reduction of the statistics, one set of atomics per block
-    __shared__ unsigned long long red[4][10];
-    unsigned long long v[10] = {looks[0], looks[1], probes[0], probes[1], win[0],
-                                win[1],   win[0],   win[1],    absent,    fallback};
+    __shared__ unsigned long long red[4][9];
+    unsigned long long v[9] = {looks[0], looks[1], probes[0], probes[1], win[0],
+                               win[1],   win[0],   win[1],    absent};
     for (int o = 32; o > 0; o >>= 1)
-        for (int q = 0; q < 10; q++) {
+        for (int q = 0; q < 9; q++) {
             unsigned long long w = __shfl_xor(v[q], o, 64);
             v[q] = (q == 6 || q == 7) ? (w > v[q] ? w : v[q]) : v[q] + w;
         }
     if ((threadIdx.x & 63) == 0)
-        for (int q = 0; q < 10; q++)
+        for (int q = 0; q < 9; q++)
             red[ch][q] = v[q];
     __syncthreads();
-    if (threadIdx.x < 10) {
+    if (threadIdx.x < 9) {
         int q = threadIdx.x;
         bool mx = (q == 6 || q == 7);
         unsigned long long a = red[0][q];
@@ -685,8 +645,7 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
                                   : q < 4 ? &qs->probes[q - 2]
                                   : q < 6 ? &qs->win_sum[q - 4]
                                   : q < 8 ? &qs->win_max[q - 6]
-                                  : q == 8 ? &qs->absent
-                                           : &qs->fallback;
+                                          : &qs->absent;
         if (a) {
             if (mx)
                 atomicMax(dst, a);
@@ -717,7 +676,6 @@ void pq_learned_free(pq_ctx_t *ctx)
         pq_free(M.iparams);
         pq_free(M.range);
         pq_free(M.knots);
-        pq_free(M.link);
     }
     pq_free(ctx->learned->qstats);
     delete ctx->learned;
@@ -1052,32 +1010,7 @@ static int train_dir(pq_ctx_t *ctx, Learned &LM, int d, const pq_learned_cfg_t &
 
 template <typename R> static ModelViewT<R> model_view(const DirModel &M)
 {
-    return {M.key, (const RowCount<R> *) M.rows, M.nU, M.iparams, M.range, M.knots, M.link};
-}
-
-template <typename R> static int build_links_t(pq_ctx_t *ctx, Learned &LM)
-{
-    int rc;
-    for (int d = 0; d < 2; d++)
-        if ((rc = pq_alloc(ctx, (void **) &LM.dir[d].link, (LM.dir[d].nU + 1) * 4)))
-            return rc;
-    PQ_CHECK(hipMemsetAsync(LM.dir[1].link, 0xFF, (LM.dir[1].nU + 1) * 4, ctx->stream));
-    ModelViewT<R> v[2] = {model_view<R>(LM.dir[0]), model_view<R>(LM.dir[1])};
-    MlpDims dims = {LM.k, LM.h};
-    if (LM.dir[0].nU && LM.dir[1].nU)
-        hipLaunchKernelGGL(k_table_links<R>, dim3(pq_blocks(LM.dir[0].nU, 256)), dim3(256), 0,
-                           ctx->stream, v[0], v[1], LM.depth, dims, LM.leaves, LM.segs,
-                           LM.dir[0].link, LM.dir[1].link);
-    else
-        PQ_CHECK(hipMemsetAsync(LM.dir[0].link, 0xFF, (LM.dir[0].nU + 1) * 4, ctx->stream));
-    PQ_CHECK(hipGetLastError());
-    PQ_CHECK(hipStreamSynchronize(ctx->stream));
-    return PQ_OK;
-}
-
-static int build_links(pq_ctx_t *ctx, Learned &LM)
-{
-    return ctx->wide ? build_links_t<uint64_t>(ctx, LM) : build_links_t<uint32_t>(ctx, LM);
+    return {M.key, (const RowCount<R> *) M.rows, M.nU, M.iparams, M.range, M.knots};
 }
 
 extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
@@ -1114,8 +1047,6 @@ extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
     int rc = pq_alloc(ctx, (void **) &LM->qstats, sizeof(QStats));
     for (int d = 0; d < 2 && rc == PQ_OK; d++)
         rc = train_dir(ctx, *LM, d, cfg);
-    if (rc == PQ_OK)
-        rc = build_links(ctx, *LM);
     if (rc == PQ_OK && hipMemsetAsync(LM->qstats, 0, sizeof(QStats), ctx->stream) != hipSuccess)
         rc = ctx->fail(PQ_ERR_HIP, "clearing learned statistics");
     if (rc == PQ_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
@@ -1150,7 +1081,7 @@ extern "C" int pq_learned_stats(pq_ctx_t *ctx, pq_learned_stats_t *st)
         st->table_size[d] = LM->dir[d].nU;
     }
     st->absent = q.absent;
-    st->fallback = q.fallback;
+    st->fallback = 0;   // every seed comes from its own table: none falls back
     st->train_ms = LM->train_ms;
     st->depth = LM->depth;
     st->k = LM->k;

@@ -6,17 +6,18 @@
 //   chain 1 FF  comp(n[L-o..L))        searched in the backward index
 //   chain 2 BF  rev(n[0..o))           searched in the backward index
 //   chain 3 BB  rc(n[0..o))            searched in the forward index
-// Each chain is a bidirectional backward search from one code (or from the
-// learned seed) that keeps the partner interval in the other index; every step
-// costs the OccLines of l and u+1 only.  For t <= o < L the '$' rows of the
-// interval are reads that begin (end) with the pattern; they sit at the head of
-// the partner interval, and the chain records (o, partner start, count).
-// FB and BF stop when only the read itself is left (size < 2,
-// string_graph.c:226,311), FF and BB when the interval empties (269,354).
+// Each chain is a backward search from one code (or from the learned seed);
+// every step costs the OccLines of l and u+1 only.  For t <= o < L the '$' rows
+// of the interval are the reads whose text in the searched index begins with
+// the pattern (string_graph.c:220-239 and the like take them by the same '$'
+// step), and the chain records (o, '$' rank of the interval, count): reads
+// reads_index[rank .. rank + count) of the searched index.  FB and BF stop when
+// only the read itself is left (size < 2, string_graph.c:226,311), FF and BB
+// when the interval empties (269,354).
 //
 // Irreducible edges (extract_{fw,bw}_edges, string_graph.c:386-731), restated
 // on explicit strings.  The candidates of one side are expanded into items
-// (type, o, m) via the partner suffix array; each item's extension key is
+// (type, o, m) through reads_index; each item's extension key is
 // m[o..) for FB/BB and rc(m)[o..) for FF/BF (2-bit codes, so comparing words
 // is comparing strings).  The reference's depth-first walk emits, in key order,
 // exactly the keys that have no other key as a proper prefix; at such a key it
@@ -40,8 +41,8 @@ static const uint32_t EXT_SHAPES[3][2] = {{8, 16}, {16, 32}, {32, 64}};
 #define KMAX_GLOBAL 65536
 #define GLOBAL_SLOTS 256
 
-template <typename R> struct ChainBufT {
-    RowCount<R> *lc;         // (partner start, count)
+struct ChainBuf {
+    uint2 *lc;               // ('$' rank in the searched index, count)
     uint16_t *o;             // pattern length
     uint32_t *n;             // entries per (read, chain)
     uint32_t cap;            // entries per chain
@@ -59,10 +60,10 @@ __device__ __forceinline__ unsigned chain_sym(const uint64_t *fwt, uint64_t s, u
 }
 
 template <typename R>
-__global__ void __launch_bounds__(256) k_find(Geom G, DirViewT<R> fw, DirViewT<R> bw, uint64_t r0,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_find(Geom G, DirViewT<R> fw, DirViewT<R> bw, uint64_t r0,
                                               uint64_t nr, uint32_t t, CTab<R> C,
                                               const SeedT<R> *seeds, uint32_t kseed,
-                                              ChainBufT<R> out, unsigned long long *stats)
+                                              ChainBuf out, unsigned long long *stats)
 {
     uint64_t tid = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t rl = tid >> 2;
@@ -80,22 +81,20 @@ __global__ void __launch_bounds__(256) k_find(Geom G, DirViewT<R> fw, DirViewT<R
         cf[q] = fwsearch ? fw.cf[q] : bw.cf[q];
     const uint32_t minsize = (ch == 0 || ch == 2) ? 2u : 1u;
     uint32_t o, size;
-    R l, ll;
-    SeedT<R> sd = {0, SEED_NONE, 0};
+    R l;
+    SeedT<R> sd = {0, SEED_NONE};
     if (seeds && kseed > 0 && kseed < L)
         sd = seeds[rl * 4 + ch];
     if (sd.size != SEED_NONE) {
         o = kseed;
         l = sd.l;
         size = sd.size;
-        ll = sd.ll;
     } else {
         // C[0..5] = 0, 0 (A), C[2], C[3], C[4], C[5] (index.c:367-372)
         unsigned c = chain_sym(fw.text2, s, L, ch, 1);
         o = 1;
         l = pq_pick6(C.c, c);
         size = (uint32_t) (pq_pick6(C.c, c + 1) - l);
-        ll = l;
     }
     uint32_t nout = 0;
     uint64_t base = (rl * 4 + (uint64_t) ch) * out.cap;
@@ -116,14 +115,13 @@ __global__ void __launch_bounds__(256) k_find(Geom G, DirViewT<R> fw, DirViewT<R
         if (o >= t) {
             uint32_t cnt = b.d - a.d;
             if (cnt && nout < out.cap) {
-                out.lc[base + nout] = RowCount<R>{ll, cnt};
+                out.lc[base + nout] = make_uint2(a.d, cnt);
                 out.o[base + nout] = (uint16_t) o;
                 nout++;
             }
         }
         if (!more)
             break;
-        ll += b.lt - a.lt;
         l = pq_pick5(cf, c) + a.c;
         size = b.c - a.c;
         o++;
@@ -180,6 +178,21 @@ __device__ __forceinline__ int key_cmp(const uint64_t *a, uint32_t la, const uin
     return la < lb ? -2 : (la > lb ? 2 : 0);
 }
 
+// Leading codes two keys share (at most the shorter length).
+__device__ __forceinline__ uint32_t key_lcp(const uint64_t *a, uint32_t la, const uint64_t *b,
+                                            uint32_t lb)
+{
+    const uint32_t mn = la < lb ? la : lb;
+    for (uint32_t w = 0; 32 * w < mn; w++) {
+        uint64_t x = a[w] ^ b[w];
+        if (x) {
+            uint32_t l = 32 * w + (uint32_t) __clzll((long long) x) / 2;
+            return l < mn ? l : mn;
+        }
+    }
+    return mn;
+}
+
 #define EDGE_SLOT 16          // edges per read side staged without any atomic
 
 struct ExtOut {
@@ -206,15 +219,165 @@ template <int GW> __device__ __forceinline__ uint32_t group_count(bool x)
     return (uint32_t) __popcll((b >> shift) & ((1ull << GW) - 1ull));
 }
 
+// Edge emission of one read side evaluated by one wave (all 64 lanes) on items
+// sorted by (key, rank, o, m), the order the edges are emitted in.  The valid
+// items sort first; with lcp[i] the codes shared by sorted items i-1 and i:
+//  - a key has another key as a proper prefix iff the chain of keys before it
+//    that are prefixes of their successors reaches a shorter one: with
+//    M = the shortest key of the chain ending at the previous item, the chain
+//    continues iff M <= lcp[i], and the item is dropped iff also M < len[i];
+//  - among equal keys the first group by (rank, o) is kept (a block of equal
+//    keys starts where lcp[i] is below either length);
+//  - the read itself is never emitted;
+// which are the rules of the pairwise evaluation in extract_read.  perm reuses
+// the entry prefix array; emitf is indexed by sorted position.
+__device__ void extract_sorted(uint64_t r, int side, uint32_t kw, uint32_t k, const uint64_t *keys,
+                               const ItemMeta *meta, uint8_t *emitf, uint32_t *perm,
+                               uint16_t *lcpv, const ExtOut &out)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t P = 1;
+    while (P < k)
+        P <<= 1;
+    for (uint32_t i = lane; i < P; i += 64)
+        perm[i] = i;
+    wave_sync();
+    // strict order: valid items by (key, rank, o, m), then invalid ones, then
+    // the padding up to the power of two
+    auto less = [&](uint32_t a, uint32_t b) -> bool {
+        if (a >= k || b >= k)
+            return a < b;
+        const ItemMeta ma = meta[a], mb = meta[b];
+        if (ma.valid != mb.valid)
+            return ma.valid > mb.valid;
+        if (!ma.valid)
+            return a < b;
+        int c = key_cmp(keys + (uint64_t) a * kw, ma.klen, keys + (uint64_t) b * kw, mb.klen);
+        if (c != 0)
+            return c < 0;
+        if (ma.rank != mb.rank)
+            return ma.rank < mb.rank;
+        if (ma.o != mb.o)
+            return ma.o < mb.o;
+        return ma.m < mb.m;
+    };
+    for (uint32_t s2 = 2; s2 <= P; s2 <<= 1) {
+        for (uint32_t j2 = s2 >> 1; j2 > 0; j2 >>= 1) {
+            for (uint32_t p = lane; p < P / 2; p += 64) {
+                const uint32_t i = 2 * j2 * (p / j2) + (p % j2), ip = i + j2;
+                const bool up = (i & s2) == 0;
+                const uint32_t x = perm[i], y = perm[ip];
+                if (less(y, x) == up) {
+                    perm[i] = y;
+                    perm[ip] = x;
+                }
+            }
+            wave_sync();
+        }
+    }
+    uint32_t nv = 0;
+    for (uint32_t q = lane; q < k + 63; q += 64)
+        nv += (uint32_t) __popcll(__ballot(q < k && meta[q].valid));
+    for (uint32_t i = lane; i < nv; i += 64) {
+        uint32_t l = 0;
+        if (i > 0) {
+            const uint32_t a = perm[i - 1], b = perm[i];
+            l = key_lcp(keys + (uint64_t) a * kw, meta[a].klen, keys + (uint64_t) b * kw,
+                        meta[b].klen);
+        }
+        lcpv[i] = (uint16_t) l;
+    }
+    wave_sync();
+    // the ordered pass: 64 sorted items per round, one per lane, walked in
+    // order with the state in scalar registers
+    uint32_t M = 0, prevlen = 0, bf_rank = 0, bf_o = 0, emitted = 0;
+    for (uint32_t c0 = 0; c0 < nv; c0 += 64) {
+        const uint32_t i = c0 + lane;
+        uint32_t li = 0, ai = 0, rk = 0, oi = 0, self = 0;
+        if (i < nv) {
+            const ItemMeta mt = meta[perm[i]];
+            li = mt.klen;
+            ai = lcpv[i];
+            rk = mt.rank;
+            oi = mt.o;
+            self = mt.m == (uint32_t) r ? 1u : 0u;
+        }
+        uint32_t keepv = 0;
+        const uint32_t nt = nv - c0 < 64 ? nv - c0 : 64;
+        for (uint32_t t = 0; t < nt; t++) {
+            const uint32_t Lt = __builtin_amdgcn_readlane(li, t);
+            const uint32_t At = __builtin_amdgcn_readlane(ai, t);
+            const uint32_t Rt = __builtin_amdgcn_readlane(rk, t);
+            const uint32_t Ot = __builtin_amdgcn_readlane(oi, t);
+            const uint32_t St = __builtin_amdgcn_readlane(self, t);
+            const bool first = c0 + t == 0;
+            const bool chain = !first && M <= At;
+            const bool dropped = chain && M < Lt;
+            const bool same_key = !first && At == Lt && At == prevlen;
+            if (!same_key) {
+                bf_rank = Rt;
+                bf_o = Ot;
+            }
+            const bool keep = !dropped && Rt == bf_rank && Ot == bf_o && !St;
+            M = chain ? (M < Lt ? M : Lt) : Lt;
+            prevlen = Lt;
+            emitted += keep ? 1u : 0u;
+            keepv = (uint32_t) lane == t ? (keep ? 1u : 0u) : keepv;
+        }
+        if (i < nv)
+            emitf[i] = (uint8_t) keepv;
+    }
+    wave_sync();
+    uint64_t base = 0;
+    uint64_t *dst = out.staged + (r * 2 + side) * EDGE_SLOT;
+    if (lane == 0) {
+        if (emitted > EDGE_SLOT) {
+            base = atomicAdd(out.edge_count, (unsigned long long) emitted);
+            if (base + emitted > out.edge_cap)
+                atomicOr(out.flags, 1u);
+        }
+        if (side) {
+            out.ovF[r] = (int32_t) nv;
+            out.numF[r] = emitted;
+            out.recF[r] = base;
+        } else {
+            out.ovB[r] = (int32_t) nv;
+            out.numB[r] = emitted;
+            out.recB[r] = base;
+        }
+    }
+    base = __shfl(base, 0, 64);
+    uint64_t cap = EDGE_SLOT;
+    if (emitted > EDGE_SLOT) {
+        dst = out.edges + base;
+        cap = base < out.edge_cap ? out.edge_cap - base : 0;
+    }
+    // output position = number of kept items before it in sorted order
+    uint32_t carry = 0;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t c0 = 0; c0 < nv; c0 += 64) {
+        const uint32_t i = c0 + lane;
+        const bool kf = i < nv && emitf[i];
+        const uint64_t bal = __ballot(kf);
+        const uint32_t pos = carry + (uint32_t) __popcll(bal & lt);
+        if (kf && pos < cap) {
+            const ItemMeta mi = meta[perm[i]];
+            dst[pos] = ((uint64_t) mi.m << 32) | ((uint64_t) mi.o << 8) | mi.type;
+        }
+        carry += (uint32_t) __popcll(bal);
+    }
+    wave_sync();
+}
+
 // Process one read with one group of GW lanes (a wave or a quarter wave; the
 // groups of a wave work on different reads).  keys/meta/emitf/pre: this group's
 // storage (LDS or global) for kmax items.  Returns false, writing nothing, if a
 // side has more than kmax items.
 template <int GW, typename R>
 __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirViewT<R> &bw,
-                             const ChainBufT<R> &cb, uint64_t rl, uint64_t r, uint32_t kw,
+                             const ChainBuf &cb, uint64_t rl, uint64_t r, uint32_t kw,
                              uint64_t *keys, ItemMeta *meta, uint8_t *emitf, uint32_t *pre,
-                             uint32_t kmax, const ExtOut &out)
+                             uint16_t *lcpv, uint32_t kmax, const ExtOut &out)
 {
     const int lane = threadIdx.x & (GW - 1);
     uint32_t cnt4[4];
@@ -274,12 +437,11 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
             bool first = e < nA;
             int ch = first ? chA : chB;
             uint64_t at = first ? (rl * 4 + chA) * cb.cap + e : (rl * 4 + chB) * cb.cap + (e - nA);
-            RowCount<R> lc = cb.lc[at];
+            uint2 lc = cb.lc[at];
             uint32_t o = cb.o[at];
-            // FB, BB: partner is the backward index; FF, BF: the forward index
-            bool pbw = (ch == 0 || ch == 3);
-            uint64_t g = (uint64_t) (pbw ? bw.sa : fw.sa)[(uint64_t) lc.x + (q - pre[e])];
-            uint64_t m = pq_read_of(G, g);
+            // FB, BB searched the forward index; FF, BF the backward index
+            const uint32_t *ri = (ch == 0 || ch == 3) ? fw.reads_index : bw.reads_index;
+            uint64_t m = ri[lc.x + (q - pre[e])];
             uint32_t Lm = pq_read_len(G, m);
             ItemMeta mt;
             mt.m = (uint32_t) m;
@@ -310,6 +472,12 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
             meta[q] = mt;
         }
         wave_sync();
+        if constexpr (GW == 64) {
+            // one wave per read with up to kmax (512 or 65536) items: sort instead
+            // of comparing all pairs
+            extract_sorted(r, side, kw, k, keys, meta, emitf, pre, lcpv, out);
+            continue;
+        }
         // an item is emitted if no other key is a proper prefix of its key, its
         // group is the first among equal keys, and it is not the read itself
         uint32_t valid_items = 0, emitted = 0;
@@ -399,7 +567,7 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
 __host__ __device__ static inline size_t ext_wave_bytes(uint32_t kmax, uint32_t kw)
 {
     size_t b = (size_t) kmax * kw * 8 + (size_t) kmax * sizeof(ItemMeta) +
-               (size_t) (kmax + 2) * 4 + kmax;
+               (size_t) (kmax + 2) * 4 + (size_t) kmax * 2 + kmax;
     return (b + 15) & ~(size_t) 15;
 }
 
@@ -407,6 +575,7 @@ struct ExtWave {
     uint64_t *keys;
     ItemMeta *meta;
     uint32_t *pre;
+    uint16_t *lcp;       // one-wave tiers: codes shared with the previous sorted item
     uint8_t *emitf;
 };
 
@@ -416,7 +585,8 @@ __device__ __forceinline__ ExtWave ext_wave(char *base, uint32_t kmax, uint32_t 
     w.keys = (uint64_t *) base;
     w.meta = (ItemMeta *) (base + (size_t) kmax * kw * 8);
     w.pre = (uint32_t *) ((char *) w.meta + (size_t) kmax * sizeof(ItemMeta));
-    w.emitf = (uint8_t *) (w.pre + kmax + 2);
+    w.lcp = (uint16_t *) (w.pre + kmax + 2);
+    w.emitf = (uint8_t *) (w.lcp + kmax);
     return w;
 }
 
@@ -427,7 +597,7 @@ __device__ __forceinline__ ExtWave ext_wave(char *base, uint32_t kmax, uint32_t 
 // out.overflow_reads for the next tier.
 template <uint32_t KMAX, int WAVES, int GW, typename R>
 __global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirViewT<R> fw,
-                                                           DirViewT<R> bw, ChainBufT<R> cb,
+                                                           DirViewT<R> bw, ChainBuf cb,
                                                            uint64_t r0, uint64_t nr,
                                                            const uint32_t *list,
                                                            const uint32_t *count, uint32_t kw,
@@ -442,7 +612,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirViewT<R> 
          i += (uint64_t) gridDim.x * GROUPS) {
         uint64_t rl = list ? list[i] : i;
         bool ok = extract_read<GW, R>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
-                                      w.pre, KMAX, out);
+                                      w.pre, w.lcp, KMAX, out);
         if (!ok && (threadIdx.x & (GW - 1)) == 0) {
             uint32_t at = atomicAdd(out.n_overflow, 1u);
             out.overflow_reads[at] = (uint32_t) rl;
@@ -452,7 +622,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirViewT<R> 
 
 template <typename R>
 __global__ void __launch_bounds__(64) k_extract_global(Geom G, DirViewT<R> fw, DirViewT<R> bw,
-                                                       ChainBufT<R> cb, uint64_t r0,
+                                                       ChainBuf cb, uint64_t r0,
                                                        const uint32_t *list,
                                                        const uint32_t *count, uint32_t kw,
                                                        char *gscratch, ExtOut out)
@@ -462,7 +632,7 @@ __global__ void __launch_bounds__(64) k_extract_global(Geom G, DirViewT<R> fw, D
     for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
         uint64_t rl = list[i];
         bool ok = extract_read<64, R>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
-                                      w.pre, KMAX_GLOBAL, out);
+                                      w.pre, w.lcp, KMAX_GLOBAL, out);
         if (!ok && threadIdx.x == 0)
             atomicOr(out.flags, 2u);
     }
@@ -622,13 +792,13 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     Geom G = ctx->geom();
     const DirViewT<R> fw = ctx->view<R>(0), bw = ctx->view<R>(1);
     const CTab<R> Ct = ctx->ctab<R>();
-    ChainBufT<R> cb;
+    ChainBuf cb;
     cb.cap = ctx->Lmax > t ? ctx->Lmax - t : 1;
     uint32_t kw = (cb.cap + 31) / 32;
     if (kw > 8)
         return ctx->fail(PQ_ERR_LIMIT, "extension keys longer than 256 codes");
     // batch so the candidate buffer stays within ~8 GiB of the 288 GB
-    uint64_t per_read = 4ull * cb.cap * (sizeof(RowCount<R>) + sizeof(uint16_t)) + 16;
+    uint64_t per_read = 4ull * cb.cap * (sizeof(uint2) + sizeof(uint16_t)) + 16;
     uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(n, (8ull << 30) / per_read));
     uint64_t *rec = nullptr, *recF = nullptr, *app = nullptr, *tot = nullptr, *staged = nullptr;
     uint32_t *ovl = nullptr, *misc = nullptr;
@@ -647,7 +817,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     cb.lc = nullptr;
     cb.o = nullptr;
     cb.n = nullptr;
-    if ((rc = pq_slot(ctx, SLOT_CAND_LC, batch * 4 * cb.cap * sizeof(RowCount<R>), &cb.lc)) ||
+    if ((rc = pq_slot(ctx, SLOT_CAND_LC, batch * 4 * cb.cap * sizeof(uint2), &cb.lc)) ||
         (rc = pq_slot(ctx, SLOT_CAND_O, batch * 4 * cb.cap * sizeof(uint16_t), &cb.o)) ||
         (rc = pq_slot(ctx, SLOT_CAND_N, batch * 4 * sizeof(uint32_t), &cb.n)) ||
         (rc = pq_slot(ctx, SLOT_RECB, (n + 1) * 8, &rec)) ||

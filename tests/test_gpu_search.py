@@ -80,6 +80,25 @@ def test_graph_with_many_candidates(ctx):
     _same_graph(g, ex, "deep")
 
 
+@pytest.mark.parametrize("G,n,lmin,lmax,err,seed", [(2000, 6000, 60, 120, 0.01, 31),
+                                                    (600, 5000, 40, 90, 0.005, 32)])
+def test_graph_with_many_candidates_and_errors(ctx, G, n, lmin, lmax, err, seed):
+    # deep coverage with errors: hundreds of candidates per read side whose
+    # extension keys are prefixes of one another or equal; the one-wave tiers
+    # sort the items and read the emission rules off adjacent common prefixes
+    rng = np.random.default_rng(seed)
+    codes, off, _ = _random_reads(rng, G, n, lmin, lmax, err=err)
+    ctx.upload(codes, off, 21)
+    ctx.build_index()
+    ex = orc.search_explicit(codes, off, 21)
+    assert max(ex["overlaps_F"].max(), ex["overlaps_B"].max()) > 128
+    for shape in range(3):
+        ctx.set_tuning(shape)
+        ctx.find_overlaps()
+        _same_graph(ctx.graph(), ex, f"deep with errors, shape {shape}")
+    ctx.set_tuning(1)
+
+
 def _identical_copies(copies=3):
     # copies of one read overlapped by another read: the copies form one
     # candidate group whose members are emitted in read order (more than 16 of
