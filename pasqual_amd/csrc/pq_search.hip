@@ -1,7 +1,7 @@
 // pq_search.hip -- FM-index overlap search (string_graph.c:162-731) on gfx950.
 //
 // Candidates (find_intervals, string_graph.c:162-383).  Four independent chains
-// per read, one lane each, four adjacent lanes per read:
+// per read, two per lane (the two of one index), two adjacent lanes per read:
 //   chain 0 FB  pattern n[L-o..L)      searched in the forward index
 //   chain 1 FF  comp(n[L-o..L))        searched in the backward index
 //   chain 2 BF  rev(n[0..o))           searched in the backward index
@@ -59,74 +59,105 @@ __device__ __forceinline__ unsigned chain_sym(const uint64_t *fwt, uint64_t s, u
     return c + 1u;
 }
 
+// Two chains per lane: FB and BB search the forward index, FF and BF the
+// backward one, so a lane takes the two chains of one index of a read (same
+// occurrence lines and step bases) and keeps one backward step of each in
+// flight: two independent line loads where one chain has only one.
 template <typename R>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_find(Geom G, DirViewT<R> fw, DirViewT<R> bw, uint64_t r0,
-                                              uint64_t nr, uint32_t t, CTab<R> C,
-                                              const SeedT<R> *seeds, uint32_t kseed,
-                                              ChainBuf out, unsigned long long *stats)
+__global__ void __launch_bounds__(256) k_find(Geom G, DirViewT<R> fw, DirViewT<R> bw,
+                                               uint64_t r0, uint64_t nr, uint32_t t, CTab<R> C,
+                                               const SeedT<R> *seeds, uint32_t kseed,
+                                               ChainBuf out, unsigned long long *stats)
 {
-    uint64_t tid = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t rl = tid >> 2;
-    int ch = (int) (tid & 3);
+    const uint64_t tid = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t rl = tid >> 1;
+    const bool bwi = (tid & 1) != 0;
     uint32_t steps = 0, lines = 0;
     if (rl < nr) {
-    uint64_t r = r0 + rl;
-    uint64_t s = pq_read_start(G, r);
-    uint32_t L = pq_read_len(G, r);
-    const bool fwsearch = (ch == 0 || ch == 3);
-    const OccLine *occ = fwsearch ? fw.occ : bw.occ;
-    R cf[5];
+        const uint64_t r = r0 + rl;
+        const uint64_t s = pq_read_start(G, r);
+        const uint32_t L = pq_read_len(G, r);
+        const OccLine *occ = bwi ? bw.occ : fw.occ;
+        R cf[5];
 #pragma unroll
-    for (int q = 0; q < 5; q++)
-        cf[q] = fwsearch ? fw.cf[q] : bw.cf[q];
-    const uint32_t minsize = (ch == 0 || ch == 2) ? 2u : 1u;
-    uint32_t o, size;
-    R l;
-    SeedT<R> sd = {0, SEED_NONE};
-    if (seeds && kseed > 0 && kseed < L)
-        sd = seeds[rl * 4 + ch];
-    if (sd.size != SEED_NONE) {
-        o = kseed;
-        l = sd.l;
-        size = sd.size;
-    } else {
-        // C[0..5] = 0, 0 (A), C[2], C[3], C[4], C[5] (index.c:367-372)
-        unsigned c = chain_sym(fw.text2, s, L, ch, 1);
-        o = 1;
-        l = pq_pick6(C.c, c);
-        size = (uint32_t) (pq_pick6(C.c, c + 1) - l);
-    }
-    uint32_t nout = 0;
-    uint64_t base = (rl * 4 + (uint64_t) ch) * out.cap;
-    while (o < L && size >= minsize) {
-        // the next pattern symbol is read with the interval's lines
-        const bool more = o + 1 < L;
-        const unsigned c = more ? chain_sym(fw.text2, s, L, ch, o + 1) : 1u;
-        const uint64_t u1 = (uint64_t) l + size;
-        const uint64_t wl = (uint64_t) l >> 7, wu = u1 >> 7;
-        OccLine ln;
-        pq_load_line(occ, wl, ln);
-        const StepRanks a = pq_step_ranks(ln, wl, (unsigned) (l & 127), c);
-        if (wu != wl)
-            pq_load_line(occ, wu, ln);
-        const StepRanks b = pq_step_ranks(ln, wu, (unsigned) (u1 & 127), c);
-        steps++;
-        lines += wl == wu ? 1u : 2u;
-        if (o >= t) {
-            uint32_t cnt = b.d - a.d;
-            if (cnt && nout < out.cap) {
-                out.lc[base + nout] = make_uint2(a.d, cnt);
-                out.o[base + nout] = (uint16_t) o;
-                nout++;
+        for (int q = 0; q < 5; q++)
+            cf[q] = bwi ? bw.cf[q] : fw.cf[q];
+        // chains (FB, BB) on the forward index, (FF, BF) on the backward one
+        const int ch[2] = {bwi ? 1 : 0, bwi ? 2 : 3};
+        const uint32_t minsize[2] = {bwi ? 1u : 2u, bwi ? 2u : 1u};
+        uint32_t o[2], size[2], nout[2];
+        R l[2];
+        bool live[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            SeedT<R> sd = {0, SEED_NONE};
+            if (seeds && kseed > 0 && kseed < L)
+                sd = seeds[rl * 4 + ch[q]];
+            if (sd.size != SEED_NONE) {
+                o[q] = kseed;
+                l[q] = sd.l;
+                size[q] = sd.size;
+            } else {
+                // C[0..5] = 0, 0 (A), C[2], C[3], C[4], C[5] (index.c:367-372)
+                unsigned c = chain_sym(fw.text2, s, L, ch[q], 1);
+                o[q] = 1;
+                l[q] = pq_pick6(C.c, c);
+                size[q] = (uint32_t) (pq_pick6(C.c, c + 1) - l[q]);
+            }
+            nout[q] = 0;
+            live[q] = o[q] < L && size[q] >= minsize[q];
+        }
+        while (live[0] || live[1]) {
+            bool more[2];
+            unsigned c[2];
+            uint64_t wl[2], wu[2], u1[2];
+            OccLine la[2], lb[2];
+            // both chains' lines (and next symbols) requested before either is used
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                if (!live[q])
+                    continue;
+                more[q] = o[q] + 1 < L;
+                c[q] = more[q] ? chain_sym(fw.text2, s, L, ch[q], o[q] + 1) : 1u;
+                u1[q] = (uint64_t) l[q] + size[q];
+                wl[q] = (uint64_t) l[q] >> 7;
+                wu[q] = u1[q] >> 7;
+                pq_load_line(occ, wl[q], la[q]);
+                if (wu[q] != wl[q])
+                    pq_load_line(occ, wu[q], lb[q]);
+                else
+                    lb[q] = la[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                if (!live[q])
+                    continue;
+                const StepRanks a = pq_step_ranks(la[q], wl[q], (unsigned) (l[q] & 127), c[q]);
+                const StepRanks b = pq_step_ranks(lb[q], wu[q], (unsigned) (u1[q] & 127), c[q]);
+                steps++;
+                lines += wl[q] == wu[q] ? 1u : 2u;
+                if (o[q] >= t) {
+                    const uint32_t cnt = b.d - a.d;
+                    if (cnt && nout[q] < out.cap) {
+                        const uint64_t at = (rl * 4 + (uint64_t) ch[q]) * out.cap + nout[q];
+                        out.lc[at] = make_uint2(a.d, cnt);
+                        out.o[at] = (uint16_t) o[q];
+                        nout[q]++;
+                    }
+                }
+                if (!more[q]) {
+                    live[q] = false;
+                    continue;
+                }
+                l[q] = pq_pick5(cf, c[q]) + a.c;
+                size[q] = b.c - a.c;
+                o[q]++;
+                live[q] = o[q] < L && size[q] >= minsize[q];
             }
         }
-        if (!more)
-            break;
-        l = pq_pick5(cf, c) + a.c;
-        size = b.c - a.c;
-        o++;
-    }
-    out.n[rl * 4 + ch] = nout;
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            out.n[rl * 4 + ch[q]] = nout[q];
     }
     if (stats) {
         uint64_t s2 = steps, ln = lines;
@@ -883,7 +914,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
             break;
         if (timed)
             (void) hipEventRecord(e[1], ctx->stream);
-        hipLaunchKernelGGL(k_find<R>, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0, ctx->stream, G,
+        hipLaunchKernelGGL(k_find<R>, dim3(pq_blocks(nr * 2, 256)), dim3(256), 0, ctx->stream, G,
                            fw, bw, r0, nr, t, Ct, seeds, kseed, cb, fstats);
         if (timed)
             (void) hipEventRecord(e[2], ctx->stream);
