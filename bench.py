@@ -314,7 +314,8 @@ def main():
                    "fit_loss": [float(f"{x:.3g}") for x in st["fit_loss"]],
                    "train_loss": [float(f"{x:.3g}") for x in st["train_loss"]],
                    "edges_identical_to_plain": bool(same),
-                   "extraction_shape": {k: tune_l[k] for k in ("group_lanes", "items")}}
+                   "extraction_shape": {k: tune_l[k] for k in ("group_lanes", "items")},
+                   "find_chains_per_lane": tune_l["find_chains"]}
         log("learned steps done")
     if args.probe_copy:
         pms = ctx.probe_copy(args.probe_copy << 20)
@@ -394,6 +395,7 @@ def main():
         "build_ms": round(avg_build, 3),
         "edges": edges,
         "extraction_shape": {k: tune[k] for k in ("group_lanes", "items")},
+        "find_chains_per_lane": tune["find_chains"],
         "roofline": roofline,
         "cpu_baseline": base,
         "learned": learned,

@@ -197,9 +197,15 @@ typedef struct {
     uint32_t items;          /* candidates per side held in LDS (16, 32, 64) */
     uint32_t nshapes;
     double extract_ms[4];    /* extraction time measured per shape */
+    uint32_t find_chains;    /* candidate-search chains per lane chosen (1 or 2) */
+    double find_ms[2];       /* candidate-search time with 1 and 2 chains per lane */
 } pq_tune_t;
 int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out);
 int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape);
+/* The candidate search runs the four chains of a read on four lanes (1) or the
+ * two chains of each index interleaved on one lane (2, the default); same
+ * result.  pq_autotune times both. */
+int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains);
 
 /* ---- timing ---------------------------------------------------------------------------- */
 typedef struct {

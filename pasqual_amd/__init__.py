@@ -35,7 +35,7 @@ PQ_SYMBOLS = [
     "pq_graph_copy", "pq_learned_default", "pq_learned_train", "pq_learned_stats",
     "pq_learned_params", "pq_learned_table", "pq_mlp_gradients",
     "pq_get_timing", "pq_autotune", "pq_set_tuning", "pq_graph_copy_edges", "pq_sync",
-    "pq_probe_copy", "pq_set_layout", "pq_get_layout", "pq_synth_preset", "pq_synth_num_reads",
+    "pq_probe_copy", "pq_set_layout", "pq_get_layout", "pq_set_find_chains", "pq_synth_preset", "pq_synth_num_reads",
     "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
 ]
@@ -91,7 +91,8 @@ class LearnedStats(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("shape", C.c_uint32), ("group_lanes", C.c_uint32), ("items", C.c_uint32),
-                ("nshapes", C.c_uint32), ("extract_ms", C.c_double * 4)]
+                ("nshapes", C.c_uint32), ("extract_ms", C.c_double * 4),
+                ("find_chains", C.c_uint32), ("find_ms", C.c_double * 2)]
 
 
 class Timing(C.Structure):
@@ -156,6 +157,7 @@ def lib(width="u32"):
         "pq_sync": (I32, [P]),
         "pq_probe_copy": (I32, [P, U64, P]),
         "pq_set_layout": (I32, [P, U32, U64]),
+        "pq_set_find_chains": (I32, [P, U32]),
         "pq_get_layout": (I32, [P, P, P]),
         "pq_synth_preset": (I32, [I32, P]),
         "pq_synth_num_reads": (U64, [P]),
@@ -348,10 +350,14 @@ class Context:
         t = Tune()
         self._check(self._L.pq_autotune(self._h, mode, C.byref(t)), "autotune")
         return {"shape": t.shape, "group_lanes": t.group_lanes, "items": t.items,
-                "extract_ms": list(t.extract_ms)[:t.nshapes]}
+                "extract_ms": list(t.extract_ms)[:t.nshapes], "find_chains": t.find_chains,
+                "find_ms": list(t.find_ms)}
 
     def set_tuning(self, shape):
         self._check(self._L.pq_set_tuning(self._h, shape), "set_tuning")
+
+    def set_find_chains(self, chains):
+        self._check(self._L.pq_set_find_chains(self._h, chains), "set_find_chains")
 
     def learned_train(self, **kw):
         cfg = LearnedCfg()

@@ -59,12 +59,95 @@ __device__ __forceinline__ unsigned chain_sym(const uint64_t *fwt, uint64_t s, u
     return c + 1u;
 }
 
+// One chain per lane, four adjacent lanes per read.
+template <typename R>
+__global__ void __launch_bounds__(256) k_find1(Geom G, DirViewT<R> fw, DirViewT<R> bw, uint64_t r0,
+                                              uint64_t nr, uint32_t t, CTab<R> C,
+                                              const SeedT<R> *seeds, uint32_t kseed,
+                                              ChainBuf out, unsigned long long *stats)
+{
+    uint64_t tid = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t rl = tid >> 2;
+    int ch = (int) (tid & 3);
+    uint32_t steps = 0, lines = 0;
+    if (rl < nr) {
+    uint64_t r = r0 + rl;
+    uint64_t s = pq_read_start(G, r);
+    uint32_t L = pq_read_len(G, r);
+    const bool fwsearch = (ch == 0 || ch == 3);
+    const OccLine *occ = fwsearch ? fw.occ : bw.occ;
+    R cf[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+        cf[q] = fwsearch ? fw.cf[q] : bw.cf[q];
+    const uint32_t minsize = (ch == 0 || ch == 2) ? 2u : 1u;
+    uint32_t o, size;
+    R l;
+    SeedT<R> sd = {0, SEED_NONE};
+    if (seeds && kseed > 0 && kseed < L)
+        sd = seeds[rl * 4 + ch];
+    if (sd.size != SEED_NONE) {
+        o = kseed;
+        l = sd.l;
+        size = sd.size;
+    } else {
+        // C[0..5] = 0, 0 (A), C[2], C[3], C[4], C[5] (index.c:367-372)
+        unsigned c = chain_sym(fw.text2, s, L, ch, 1);
+        o = 1;
+        l = pq_pick6(C.c, c);
+        size = (uint32_t) (pq_pick6(C.c, c + 1) - l);
+    }
+    uint32_t nout = 0;
+    uint64_t base = (rl * 4 + (uint64_t) ch) * out.cap;
+    while (o < L && size >= minsize) {
+        // the next pattern symbol is read with the interval's lines
+        const bool more = o + 1 < L;
+        const unsigned c = more ? chain_sym(fw.text2, s, L, ch, o + 1) : 1u;
+        const uint64_t u1 = (uint64_t) l + size;
+        const uint64_t wl = (uint64_t) l >> 7, wu = u1 >> 7;
+        OccLine ln;
+        pq_load_line(occ, wl, ln);
+        const StepRanks a = pq_step_ranks(ln, wl, (unsigned) (l & 127), c);
+        if (wu != wl)
+            pq_load_line(occ, wu, ln);
+        const StepRanks b = pq_step_ranks(ln, wu, (unsigned) (u1 & 127), c);
+        steps++;
+        lines += wl == wu ? 1u : 2u;
+        if (o >= t) {
+            uint32_t cnt = b.d - a.d;
+            if (cnt && nout < out.cap) {
+                out.lc[base + nout] = make_uint2(a.d, cnt);
+                out.o[base + nout] = (uint16_t) o;
+                nout++;
+            }
+        }
+        if (!more)
+            break;
+        l = pq_pick5(cf, c) + a.c;
+        size = b.c - a.c;
+        o++;
+    }
+    out.n[rl * 4 + ch] = nout;
+    }
+    if (stats) {
+        uint64_t s2 = steps, ln = lines;
+        for (int d = 32; d > 0; d >>= 1) {
+            s2 += __shfl_xor(s2, d, 64);
+            ln += __shfl_xor(ln, d, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&stats[0], (unsigned long long) (2 * s2));
+            atomicAdd(&stats[1], (unsigned long long) ln);
+        }
+    }
+}
+
 // Two chains per lane: FB and BB search the forward index, FF and BF the
 // backward one, so a lane takes the two chains of one index of a read (same
 // occurrence lines and step bases) and keeps one backward step of each in
 // flight: two independent line loads where one chain has only one.
 template <typename R>
-__global__ void __launch_bounds__(256) k_find(Geom G, DirViewT<R> fw, DirViewT<R> bw,
+__global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<R> bw,
                                                uint64_t r0, uint64_t nr, uint32_t t, CTab<R> C,
                                                const SeedT<R> *seeds, uint32_t kseed,
                                                ChainBuf out, unsigned long long *stats)
@@ -882,6 +965,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     unsigned long long *fstats = (unsigned long long *) (misc + 8);   // misc[8..11]
     // the whole search as a sequence of stream operations (captured into a graph
     // in graph mode: kernels only, no memsets, no host synchronisation)
+    const uint32_t chains = ctx->find_chains == 1 ? 1u : 2u;
     auto enqueue = [&](bool timed) -> int {
     int rc = PQ_OK;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc, 16u);
@@ -914,8 +998,12 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
             break;
         if (timed)
             (void) hipEventRecord(e[1], ctx->stream);
-        hipLaunchKernelGGL(k_find<R>, dim3(pq_blocks(nr * 2, 256)), dim3(256), 0, ctx->stream, G,
-                           fw, bw, r0, nr, t, Ct, seeds, kseed, cb, fstats);
+        if (chains == 1)
+            hipLaunchKernelGGL(k_find1<R>, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0,
+                               ctx->stream, G, fw, bw, r0, nr, t, Ct, seeds, kseed, cb, fstats);
+        else
+            hipLaunchKernelGGL(k_find2<R>, dim3(pq_blocks(nr * 2, 256)), dim3(256), 0,
+                               ctx->stream, G, fw, bw, r0, nr, t, Ct, seeds, kseed, cb, fstats);
         if (timed)
             (void) hipEventRecord(e[2], ctx->stream);
         const dim3 g1(pq_blocks(nr, groups1)), b1(64 * EXT_WAVES);
@@ -962,7 +1050,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
                                         (uint64_t) (uintptr_t) gscratch, (uint64_t) (uintptr_t) ovl,
                                         (uint64_t) t, (uint64_t) kw, (uint64_t) Ct.c[2],
                                         (uint64_t) Ct.c[3], (uint64_t) Ct.c[4], (uint64_t) Ct.c[5],
-                                        (uint64_t) sizeof(R),
+                                        (uint64_t) sizeof(R), (uint64_t) chains,
                                         (uint64_t) (uintptr_t) G.off, (uint64_t) (uintptr_t) G.ends,
                                         (uint64_t) (uintptr_t) G.end_rank, (uint64_t) G.L,
                                         (uint64_t) G.Lmax, (uint64_t) G.N, (uint64_t) cb.cap});
@@ -1093,6 +1181,9 @@ extern "C" int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out)
     uint32_t pick = 1;
     pq_tune_t t;
     memset(&t, 0, sizeof(t));
+    // extraction shapes, with the candidate search at two chains per lane
+    ctx->find_chains = 2;
+    t.find_ms[1] = 1e300;
     for (uint32_t sh = 0; sh < EXT_NSHAPES; sh++) {
         ctx->ext_shape = sh;
         int rc = pq_find_overlaps(ctx, mode);
@@ -1100,12 +1191,23 @@ extern "C" int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out)
             return rc;
         double ms = ctx->timing.extract_ms;
         t.extract_ms[sh] = ms;
+        t.find_ms[1] = std::min(t.find_ms[1], ctx->timing.find_kernel_ms);
         if (ms < best) {
             best = ms;
             pick = sh;
         }
     }
     ctx->ext_shape = pick;
+    // the candidate search at one chain per lane (more waves in flight, each with
+    // one line load outstanding): better where lengths vary or the index is
+    // HBM-resident
+    ctx->find_chains = 1;
+    int rc = pq_find_overlaps(ctx, mode);
+    if (rc)
+        return rc;
+    t.find_ms[0] = ctx->timing.find_kernel_ms;
+    ctx->find_chains = t.find_ms[0] < t.find_ms[1] ? 1u : 2u;
+    t.find_chains = ctx->find_chains;
     t.shape = pick;
     t.group_lanes = EXT_SHAPES[pick][0];
     t.items = EXT_SHAPES[pick][1];
@@ -1123,6 +1225,16 @@ extern "C" int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape)
     if (shape >= EXT_NSHAPES)
         return ctx->fail(PQ_ERR_ARG, "no such extraction shape");
     ctx->ext_shape = shape;
+    return PQ_OK;
+}
+
+extern "C" int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (chains != 1 && chains != 2)
+        return ctx->fail(PQ_ERR_ARG, "chains per lane must be 1 or 2");
+    ctx->find_chains = chains;
     return PQ_OK;
 }
 

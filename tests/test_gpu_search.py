@@ -138,7 +138,13 @@ def test_every_extraction_shape_gives_the_same_graph(ctx):
         ctx.set_tuning(shape)
         ctx.find_overlaps()
         _same_graph(ctx.graph(), want, f"shape {shape}")
+    for chains in (1, 2):
+        ctx.set_find_chains(chains)
+        ctx.find_overlaps()
+        _same_graph(ctx.graph(), want, f"{chains} chains per lane")
     t = ctx.autotune()
     assert 0 <= t["shape"] < 3 and len(t["extract_ms"]) == 3
+    assert t["find_chains"] in (1, 2) and min(t["find_ms"]) > 0
     _same_graph(ctx.graph(), want, "autotuned")
     ctx.set_tuning(1)
+    ctx.set_find_chains(2)
