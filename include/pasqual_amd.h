@@ -136,7 +136,7 @@ typedef struct {
     uint32_t k;              /* prefix length fed to the model (<= 21; capped at D), 21 */
     uint32_t hidden;         /* MLP hidden width (<= 64), 64 */
     uint32_t leaves;         /* piecewise-linear leaves, 4096 */
-    uint32_t segments;       /* linear pieces per leaf (<= 1024), 64 */
+    uint32_t segments;       /* linear pieces per leaf (<= 1024), 256 */
     uint32_t epochs;         /* Adam passes over the sampled training set, 1 */
     uint32_t batch;          /* minibatch size (multiple of 64), 4096 */
     uint32_t samples;        /* training entries drawn from each table (multiple of 64) */

@@ -126,7 +126,8 @@ __device__ __forceinline__ float mlp_forward(const float *p, MlpDims d, uint64_t
 // zero weights (features past k are 0 too), then b1, w2, b2.  Fixed trip counts
 // and wave-uniform parameters let every weight come from a scalar load.  Gives
 // the same output as mlp_forward on the unpadded parameters.
-__device__ __forceinline__ float mlp_forward_padded(const cfloat *p, MlpDims d, uint64_t x)
+template <typename PT>
+__device__ __forceinline__ float mlp_forward_padded(const PT *p, MlpDims d, uint64_t x)
 {
     float f[MLP_K_MAX];
 #pragma unroll
@@ -135,10 +136,10 @@ __device__ __forceinline__ float mlp_forward_padded(const cfloat *p, MlpDims d, 
         float v = (float) ((uint32_t) (x >> sh) & 3u) * (2.0f / 3.0f) - 1.0f;
         f[j] = j < d.k ? v : 0.0f;
     }
-    const cfloat *b1 = p + d.h * MLP_K_MAX, *w2 = b1 + d.h;
+    const PT *b1 = p + d.h * MLP_K_MAX, *w2 = b1 + d.h;
     float out = w2[d.h];
     for (uint32_t i = 0; i < d.h; i++) {
-        const cfloat *row = p + i * MLP_K_MAX;
+        const PT *row = p + i * MLP_K_MAX;
         float s = b1[i];
 #pragma unroll
         for (uint32_t j = 0; j < MLP_K_MAX; j++)
@@ -471,12 +472,73 @@ __device__ __forceinline__ const cfloat *wave_params(const float *p)
     return (const cfloat *) (((uint64_t) hi << 32) | lo);
 }
 
-template <typename R>
-__device__ __forceinline__ uint64_t model_entry(const cfloat *params, const ModelViewT<R> &M,
-                                                MlpDims d,
-                                                uint32_t leaves, uint32_t S, uint64_t x)
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// Input j of the padded network for prefix value x of k codes: the code mapped
+// to -1..1 for j < k, 0 for k <= j < MLP_K_MAX, and 1 for j = MLP_K_MAX (the
+// input that carries b1 in the padded weight matrix below).
+__device__ __forceinline__ float mlp_input(uint64_t x, uint32_t k, uint32_t j)
 {
-    uint32_t j = leaf_of(mlp_forward_padded(params, d, x), leaves);
+    if (j == MLP_K_MAX)
+        return 1.0f;
+    if (j >= k)
+        return 0.0f;
+    return (float) ((uint32_t) (x >> (2 * (k - 1 - j))) & 3u) * (2.0f / 3.0f) - 1.0f;
+}
+
+// Network output for every lane's prefix value x (one key per lane, 64 per
+// wave) on the matrix cores, same function as mlp_forward_padded.  With W' =
+// [W1 | b1] (h x 22) and X' the 64 x 22 inputs, H^T = W' X'^T (h x 64) is four
+// 32 x 32 tiles (hidden half nt, key half mt) accumulated by 11 steps of
+// v_mfma_f32_32x32x2f32 each.  Operands per lane l (c = l % 32, g = l / 32):
+// A = W'[32 nt + c][k0 + g], B = X'[32 mt + c][k0 + g]; result register v holds
+// H^T[32 nt + 8 (v / 4) + 4 g + v % 4][32 mt + c], so a lane holds half the
+// hidden units of key 32 mt + c and the lane l ^ 32 the other half.  p: the
+// padded parameters (LDS).  Every lane of the wave must call it.
+__device__ __forceinline__ float mlp_forward_mfma(const float *p, MlpDims d, uint64_t x)
+{
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 5, c = lane & 31;
+    const uint64_t xp = __shfl_xor(x, 32, 64);   // the key of lane l ^ 32
+    const float *b1 = p + d.h * MLP_K_MAX, *w2 = b1 + d.h;
+    float part0 = 0.0f, part1 = 0.0f;
+    // one 32 x 32 tile at a time (16 accumulators live), keys half mt, hidden half nt
+#pragma unroll 1
+    for (int t4 = 0; t4 < 4; t4++) {
+        const int mt = t4 >> 1, nt = t4 & 1;
+        const uint64_t xb = g == mt ? x : xp;    // key 32 mt + c
+        const uint32_t n = 32 * nt + c;
+        floatx16 acc = {};
+#pragma unroll
+        for (uint32_t k0 = 0; k0 < MLP_K_MAX + 1; k0 += 2) {
+            const uint32_t kk = k0 + g;
+            const float a = n >= d.h ? 0.0f : kk < MLP_K_MAX ? p[n * MLP_K_MAX + kk] : b1[n];
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, mlp_input(xb, d.k, kk), acc, 0, 0, 0);
+        }
+        float part = 0.0f;
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const uint32_t r = 32 * nt + 8 * (v / 4) + 4 * g + (v % 4);
+            part = fmaf(r < d.h ? w2[r] : 0.0f, fmaxf(acc[v], 0.0f), part);
+        }
+        if (mt == 0)
+            part0 += part;
+        else
+            part1 += part;
+    }
+    part0 += __shfl_xor(part0, 32, 64);
+    part1 += __shfl_xor(part1, 32, 64);
+    // lane l < 32 is key c = 32 * 0 + c, lane l >= 32 key 32 * 1 + c
+    return w2[d.h] + (g == 0 ? part0 : part1);
+}
+
+// Predicted entry of prefix value x from the network output y: leaf, then the
+// leaf's piecewise-linear piece.
+template <typename R>
+__device__ __forceinline__ uint64_t leaf_entry(float y, const ModelViewT<R> &M, uint32_t leaves,
+                                               uint32_t S, uint64_t x)
+{
+    uint32_t j = leaf_of(y, leaves);
     // the network routes approximately; step to the neighbouring leaf whose
     // training range holds x (bounded)
     for (int q = 0; q < REROUTE_MAX && j > 0 && x < M.range[2 * j]; q++)
@@ -597,17 +659,32 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
     uint32_t probes[2] = {0, 0}, looks[2] = {0, 0};
     uint64_t win[2] = {0, 0};
     uint32_t absent = 0;
+    // both directions' inference parameters in LDS, read as wave-wide broadcasts
+    __shared__ float wsh[2][MLP_P_MAX];
+    for (uint32_t i = threadIdx.x; i < MLP_P_MAX; i += blockDim.x) {
+        wsh[0][i] = fw.iparams[i];
+        wsh[1][i] = bw.iparams[i];
+    }
+    __syncthreads();
+    // every lane's key (zero where the chain takes no seed), then the network for
+    // the whole wave on the matrix cores
+    const ModelViewT<R> &Sv = fwsearch ? fw : bw;
+    bool act = false;
+    u128 pat = 0;
     if (rl < nr) {
-        uint64_t r = r0 + rl;
-        uint64_t s = pq_read_start(G, r);
-        uint32_t L = pq_read_len(G, r);
+        const uint64_t r = r0 + rl;
+        const uint32_t L = pq_read_len(G, r);
+        act = D < L && Sv.nU > 0;
+        if (act)
+            pat = chain_pattern(fwt, pq_read_start(G, r), L, ch, D);
+    }
+    const uint64_t xk = (uint64_t) (pat >> (128 - 2 * d.k));
+    const float y = mlp_forward_mfma(&wsh[sdir][0], d, xk);
+    if (rl < nr) {
         SeedT<R> sd = {0, SEED_NONE};
-        const ModelViewT<R> &Sv = fwsearch ? fw : bw;
-        if (D < L && Sv.nU > 0) {
-            const u128 pat = chain_pattern(fwt, s, L, ch, D);
+        if (act) {
             const u128 key = pat | 1u;
-            uint64_t g1 = model_entry(wave_params(Sv.iparams), Sv, d, leaves, S,
-                                      (uint64_t) (pat >> (128 - 2 * d.k)));
+            uint64_t g1 = leaf_entry(y, Sv, leaves, S, xk);
             uint64_t u = lower_bound_from(Sv.key, Sv.nU, key, g1, probes[sdir]);
             looks[sdir]++;
             win[sdir] = u > g1 ? u - g1 : g1 - u;
@@ -689,7 +766,7 @@ extern "C" void pq_learned_default(pq_learned_cfg_t *cfg)
     cfg->k = 21;
     cfg->hidden = 64;
     cfg->leaves = 4096;
-    cfg->segments = 64;
+    cfg->segments = 256;
     cfg->epochs = 1;
     cfg->batch = 4096;
     cfg->samples = 1u << 20;
