@@ -333,7 +333,18 @@ template <int GW> __device__ __forceinline__ uint32_t group_count(bool x)
     return (uint32_t) __popcll((b >> shift) & ((1ull << GW) - 1ull));
 }
 
-// Edge emission of one read side evaluated by one wave (all 64 lanes) on items
+// Lanes of this group of GW lanes set in a wave-wide ballot, at the group's
+// lane positions.
+template <int GW> __device__ __forceinline__ uint64_t group_bits(bool x)
+{
+    uint64_t b = __ballot(x);
+    if (GW == 64)
+        return b;
+    const int shift = (int) (threadIdx.x % 64) & ~(GW - 1);
+    return (b >> shift) & ((1ull << GW) - 1ull);
+}
+
+// Edge emission of one read side evaluated by one group of GW lanes on items
 // sorted by (key, rank, o, m), the order the edges are emitted in.  The valid
 // items sort first; with lcp[i] the codes shared by sorted items i-1 and i:
 //  - a key has another key as a proper prefix iff the chain of keys before it
@@ -343,21 +354,22 @@ template <int GW> __device__ __forceinline__ uint32_t group_count(bool x)
 //  - among equal keys the first group by (rank, o) is kept (a block of equal
 //    keys starts where lcp[i] is below either length);
 //  - the read itself is never emitted;
-// which are the rules of the pairwise evaluation in extract_read.  perm reuses
-// the entry prefix array; emitf is indexed by sorted position.
-__device__ void extract_sorted(uint64_t r, int side, uint32_t kw, uint32_t k, const uint64_t *keys,
-                               const ItemMeta *meta, uint8_t *emitf, uint32_t *perm,
-                               uint16_t *lcpv, const ExtOut &out)
+// the rules of the reference's depth-first walk (string_graph.c:386-731) as
+// restated in the header.  P: a power of two >= k, the same for every group of
+// a wave (the tier's item limit when several groups share a wave), so that all
+// loops have the same trip count in every group.  perm reuses the entry prefix
+// array; emitf is indexed by sorted position.
+template <int GW>
+__device__ void extract_sorted(uint64_t r, int side, uint32_t kw, uint32_t k, uint32_t P,
+                               const uint64_t *keys, const ItemMeta *meta, uint8_t *emitf,
+                               uint32_t *perm, uint16_t *lcpv, const ExtOut &out)
 {
-    const int lane = threadIdx.x & 63;
-    uint32_t P = 1;
-    while (P < k)
-        P <<= 1;
-    for (uint32_t i = lane; i < P; i += 64)
+    const int lane = threadIdx.x & (GW - 1);
+    for (uint32_t i = lane; i < P; i += GW)
         perm[i] = i;
     wave_sync();
     // strict order: valid items by (key, rank, o, m), then invalid ones, then
-    // the padding up to the power of two
+    // the padding up to P
     auto less = [&](uint32_t a, uint32_t b) -> bool {
         if (a >= k || b >= k)
             return a < b;
@@ -377,7 +389,7 @@ __device__ void extract_sorted(uint64_t r, int side, uint32_t kw, uint32_t k, co
     };
     for (uint32_t s2 = 2; s2 <= P; s2 <<= 1) {
         for (uint32_t j2 = s2 >> 1; j2 > 0; j2 >>= 1) {
-            for (uint32_t p = lane; p < P / 2; p += 64) {
+            for (uint32_t p = lane; p < P / 2; p += GW) {
                 const uint32_t i = 2 * j2 * (p / j2) + (p % j2), ip = i + j2;
                 const bool up = (i & s2) == 0;
                 const uint32_t x = perm[i], y = perm[ip];
@@ -390,56 +402,87 @@ __device__ void extract_sorted(uint64_t r, int side, uint32_t kw, uint32_t k, co
         }
     }
     uint32_t nv = 0;
-    for (uint32_t q = lane; q < k + 63; q += 64)
-        nv += (uint32_t) __popcll(__ballot(q < k && meta[q].valid));
-    for (uint32_t i = lane; i < nv; i += 64) {
-        uint32_t l = 0;
-        if (i > 0) {
-            const uint32_t a = perm[i - 1], b = perm[i];
-            l = key_lcp(keys + (uint64_t) a * kw, meta[a].klen, keys + (uint64_t) b * kw,
-                        meta[b].klen);
+    for (uint32_t q0 = 0; q0 < P; q0 += GW) {
+        const uint32_t q = q0 + lane;
+        nv += (uint32_t) __popcll(group_bits<GW>(q < k && meta[q].valid));
+    }
+    for (uint32_t i = lane; i < P; i += GW) {
+        if (i < nv) {
+            uint32_t l = 0;
+            if (i > 0) {
+                const uint32_t a = perm[i - 1], b = perm[i];
+                l = key_lcp(keys + (uint64_t) a * kw, meta[a].klen, keys + (uint64_t) b * kw,
+                            meta[b].klen);
+            }
+            lcpv[i] = (uint16_t) l;
         }
-        lcpv[i] = (uint16_t) l;
     }
     wave_sync();
-    // the ordered pass: 64 sorted items per round, one per lane, walked in
-    // order with the state in scalar registers
+    // the ordered pass over the sorted items with the state in registers
+    // (group-uniform)
     uint32_t M = 0, prevlen = 0, bf_rank = 0, bf_o = 0, emitted = 0;
-    for (uint32_t c0 = 0; c0 < nv; c0 += 64) {
-        const uint32_t i = c0 + lane;
-        uint32_t li = 0, ai = 0, rk = 0, oi = 0, self = 0;
-        if (i < nv) {
-            const ItemMeta mt = meta[perm[i]];
-            li = mt.klen;
-            ai = lcpv[i];
-            rk = mt.rank;
-            oi = mt.o;
-            self = mt.m == (uint32_t) r ? 1u : 0u;
-        }
-        uint32_t keepv = 0;
-        const uint32_t nt = nv - c0 < 64 ? nv - c0 : 64;
-        for (uint32_t t = 0; t < nt; t++) {
-            const uint32_t Lt = __builtin_amdgcn_readlane(li, t);
-            const uint32_t At = __builtin_amdgcn_readlane(ai, t);
-            const uint32_t Rt = __builtin_amdgcn_readlane(rk, t);
-            const uint32_t Ot = __builtin_amdgcn_readlane(oi, t);
-            const uint32_t St = __builtin_amdgcn_readlane(self, t);
-            const bool first = c0 + t == 0;
-            const bool chain = !first && M <= At;
-            const bool dropped = chain && M < Lt;
-            const bool same_key = !first && At == Lt && At == prevlen;
-            if (!same_key) {
-                bf_rank = Rt;
-                bf_o = Ot;
+    if constexpr (GW == 64) {
+        // 64 sorted items per round, one per lane, read back lane by lane
+        for (uint32_t c0 = 0; c0 < nv; c0 += 64) {
+            const uint32_t i = c0 + lane;
+            uint32_t li = 0, ai = 0, rk = 0, oi = 0, self = 0;
+            if (i < nv) {
+                const ItemMeta mt = meta[perm[i]];
+                li = mt.klen;
+                ai = lcpv[i];
+                rk = mt.rank;
+                oi = mt.o;
+                self = mt.m == (uint32_t) r ? 1u : 0u;
             }
-            const bool keep = !dropped && Rt == bf_rank && Ot == bf_o && !St;
-            M = chain ? (M < Lt ? M : Lt) : Lt;
-            prevlen = Lt;
-            emitted += keep ? 1u : 0u;
-            keepv = (uint32_t) lane == t ? (keep ? 1u : 0u) : keepv;
+            uint32_t keepv = 0;
+            const uint32_t nt = nv - c0 < 64 ? nv - c0 : 64;
+            for (uint32_t t = 0; t < nt; t++) {
+                const uint32_t Lt = __builtin_amdgcn_readlane(li, t);
+                const uint32_t At = __builtin_amdgcn_readlane(ai, t);
+                const uint32_t Rt = __builtin_amdgcn_readlane(rk, t);
+                const uint32_t Ot = __builtin_amdgcn_readlane(oi, t);
+                const uint32_t St = __builtin_amdgcn_readlane(self, t);
+                const bool first = c0 + t == 0;
+                const bool chain = !first && M <= At;
+                const bool dropped = chain && M < Lt;
+                const bool same_key = !first && At == Lt && At == prevlen;
+                if (!same_key) {
+                    bf_rank = Rt;
+                    bf_o = Ot;
+                }
+                const bool keep = !dropped && Rt == bf_rank && Ot == bf_o && !St;
+                M = chain ? (M < Lt ? M : Lt) : Lt;
+                prevlen = Lt;
+                emitted += keep ? 1u : 0u;
+                keepv = (uint32_t) lane == t ? (keep ? 1u : 0u) : keepv;
+            }
+            if (i < nv)
+                emitf[i] = (uint8_t) keepv;
         }
-        if (i < nv)
-            emitf[i] = (uint8_t) keepv;
+    } else {
+        // a few items per group: walk them in order, every lane of the group
+        // reading the same LDS entries
+        for (uint32_t i = 0; i < P; i++) {
+            if (i < nv) {
+                const ItemMeta mt = meta[perm[i]];
+                const uint32_t Lt = mt.klen, At = lcpv[i];
+                const bool first = i == 0;
+                const bool chain = !first && M <= At;
+                const bool dropped = chain && M < Lt;
+                const bool same_key = !first && At == Lt && At == prevlen;
+                if (!same_key) {
+                    bf_rank = mt.rank;
+                    bf_o = mt.o;
+                }
+                const bool keep =
+                    !dropped && mt.rank == bf_rank && mt.o == bf_o && mt.m != (uint32_t) r;
+                M = chain ? (M < Lt ? M : Lt) : Lt;
+                prevlen = Lt;
+                emitted += keep ? 1u : 0u;
+                if (lane == 0)
+                    emitf[i] = keep ? 1u : 0u;
+            }
+        }
     }
     wave_sync();
     uint64_t base = 0;
@@ -460,7 +503,7 @@ __device__ void extract_sorted(uint64_t r, int side, uint32_t kw, uint32_t k, co
             out.recB[r] = base;
         }
     }
-    base = __shfl(base, 0, 64);
+    base = __shfl(base, 0, GW);
     uint64_t cap = EDGE_SLOT;
     if (emitted > EDGE_SLOT) {
         dst = out.edges + base;
@@ -469,10 +512,10 @@ __device__ void extract_sorted(uint64_t r, int side, uint32_t kw, uint32_t k, co
     // output position = number of kept items before it in sorted order
     uint32_t carry = 0;
     const uint64_t lt = (1ull << lane) - 1ull;
-    for (uint32_t c0 = 0; c0 < nv; c0 += 64) {
+    for (uint32_t c0 = 0; c0 < P; c0 += GW) {
         const uint32_t i = c0 + lane;
         const bool kf = i < nv && emitf[i];
-        const uint64_t bal = __ballot(kf);
+        const uint64_t bal = group_bits<GW>(kf);
         const uint32_t pos = carry + (uint32_t) __popcll(bal & lt);
         if (kf && pos < cap) {
             const ItemMeta mi = meta[perm[i]];
@@ -586,10 +629,13 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
             meta[q] = mt;
         }
         wave_sync();
-        if constexpr (GW == 64) {
-            // one wave per read with up to kmax (512 or 65536) items: sort instead
-            // of comparing all pairs
-            extract_sorted(r, side, kw, k, keys, meta, emitf, pre, lcpv, out);
+        {
+            // sorted evaluation: one read per wave sorts to the next power of two
+            // of its item count; groups sharing a wave sort to the tier's limit
+            uint32_t P = 1;
+            while (P < (GW == 64 ? k : kmax))
+                P <<= 1;
+            extract_sorted<GW>(r, side, kw, k, P, keys, meta, emitf, pre, lcpv, out);
             continue;
         }
         // an item is emitted if no other key is a proper prefix of its key, its
