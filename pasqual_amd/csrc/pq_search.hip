@@ -23,8 +23,8 @@
 // exactly the keys that have no other key as a proper prefix; at such a key it
 // takes the first candidate group (type order FB<FF, BF<BB, then o) and emits
 // its reads other than n in read order (reads_index order of identical
-// strings).  One wave per read evaluates this over all item pairs with the
-// keys in LDS.
+// strings).  A group of lanes per read sorts the items with their keys in LDS
+// and reads these rules off the sorted order (extract_sorted).
 #include "pq_ctx.h"
 
 #include <chrono>
@@ -261,7 +261,7 @@ struct ItemMeta {
     uint32_t m;
     uint16_t o;
     uint16_t klen;
-    uint8_t rank, type, valid, emit;
+    uint8_t rank, type, valid, pad;
 };
 
 __device__ __forceinline__ void wave_sync()
@@ -323,15 +323,6 @@ struct ExtOut {
     uint32_t *flags;              // bit0: edge buffer full, bit1: item limit
 };
 
-// Items of this group of GW lanes set in a wave-wide ballot.
-template <int GW> __device__ __forceinline__ uint32_t group_count(bool x)
-{
-    uint64_t b = __ballot(x);
-    if (GW == 64)
-        return (uint32_t) __popcll(b);
-    const int shift = (int) (threadIdx.x % 64) & ~(GW - 1);
-    return (uint32_t) __popcll((b >> shift) & ((1ull << GW) - 1ull));
-}
 
 // Lanes of this group of GW lanes set in a wave-wide ballot, at the group's
 // lane positions.
@@ -462,8 +453,8 @@ __device__ void extract_sorted(uint64_t r, int side, uint32_t kw, uint32_t k, ui
     } else {
         // a few items per group: walk them in order, every lane of the group
         // reading the same LDS entries
-        for (uint32_t i = 0; i < P; i++) {
-            if (i < nv) {
+        for (uint32_t i = 0; i < nv; i++) {
+            {
                 const ItemMeta mt = meta[perm[i]];
                 const uint32_t Lt = mt.klen, At = lcpv[i];
                 const bool first = i == 0;
@@ -607,7 +598,7 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
             mt.klen = (uint16_t) (mt.valid ? Lm - o : 0);
             mt.rank = first ? 0 : 1;
             mt.type = ch == 0 ? 1 : ch == 1 ? 2 : ch == 2 ? 0 : 3;   // FB FF BF BB
-            mt.emit = 0;
+            mt.pad = 0;
             // key: FB/BB m[o..) from the forward text; FF/BF rc(m)[o..) as the
             // complement of the backward text
             bool plain = (ch == 0 || ch == 3);
@@ -629,95 +620,12 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
             meta[q] = mt;
         }
         wave_sync();
-        {
-            // sorted evaluation: one read per wave sorts to the next power of two
-            // of its item count; groups sharing a wave sort to the tier's limit
-            uint32_t P = 1;
-            while (P < (GW == 64 ? k : kmax))
-                P <<= 1;
-            extract_sorted<GW>(r, side, kw, k, P, keys, meta, emitf, pre, lcpv, out);
-            continue;
-        }
-        // an item is emitted if no other key is a proper prefix of its key, its
-        // group is the first among equal keys, and it is not the read itself
-        uint32_t valid_items = 0, emitted = 0;
-        for (uint32_t q = lane; q < k + GW - 1; q += GW) {  // same trip count in the group
-            bool mine = q < k;
-            ItemMeta mi;
-            if (mine)
-                mi = meta[q];
-            bool act = mine && mi.valid;
-            bool keep = act && mi.m != (uint32_t) r;
-            if (keep) {
-                const uint64_t *ki = keys + (uint64_t) q * kw;
-                for (uint32_t j = 0; j < k && keep; j++) {
-                    if (j == q)
-                        continue;
-                    ItemMeta mj = meta[j];
-                    if (!mj.valid)
-                        continue;
-                    int c = key_cmp(ki, mi.klen, keys + (uint64_t) j * kw, mj.klen);
-                    if (c == 2)
-                        keep = false;                    // a shorter key is a prefix
-                    else if (c == 0 &&
-                             (mj.rank < mi.rank || (mj.rank == mi.rank && mj.o < mi.o)))
-                        keep = false;                    // an earlier group wins
-                }
-            }
-            valid_items += group_count<GW>(act);
-            emitted += group_count<GW>(keep);
-            if (mine)
-                emitf[q] = (uint8_t) keep;
-        }
-        uint64_t base = 0;
-        uint64_t *dst = out.staged + (r * 2 + side) * EDGE_SLOT;
-        if (lane == 0) {
-            if (emitted > EDGE_SLOT) {
-                base = atomicAdd(out.edge_count, (unsigned long long) emitted);
-                if (base + emitted > out.edge_cap)
-                    atomicOr(out.flags, 1u);
-            }
-            if (side) {
-                out.ovF[r] = (int32_t) valid_items;
-                out.numF[r] = emitted;
-                out.recF[r] = base;
-            } else {
-                out.ovB[r] = (int32_t) valid_items;
-                out.numB[r] = emitted;
-                out.recB[r] = base;
-            }
-        }
-        base = __shfl(base, 0, GW);
-        uint64_t cap = EDGE_SLOT;
-        if (emitted > EDGE_SLOT) {
-            dst = out.edges + base;
-            cap = base < out.edge_cap ? out.edge_cap - base : 0;
-        }
-        wave_sync();
-        // output position = rank among the emitted items by (key, rank, o, m)
-        if (emitted) {
-            for (uint32_t q = lane; q < k; q += GW) {
-                if (!emitf[q])
-                    continue;
-                ItemMeta mi = meta[q];
-                const uint64_t *ki = keys + (uint64_t) q * kw;
-                uint32_t pos = 0;
-                for (uint32_t j = 0; j < k; j++) {
-                    if (j == q || !emitf[j])
-                        continue;
-                    ItemMeta mj = meta[j];
-                    int c = key_cmp(keys + (uint64_t) j * kw, mj.klen, ki, mi.klen);
-                    bool before = c < 0 ||
-                                  (c == 0 && (mj.rank < mi.rank ||
-                                              (mj.rank == mi.rank &&
-                                               (mj.o < mi.o || (mj.o == mi.o && mj.m < mi.m)))));
-                    pos += before ? 1u : 0u;
-                }
-                if (pos < cap)
-                    dst[pos] = ((uint64_t) mi.m << 32) | ((uint64_t) mi.o << 8) | mi.type;
-            }
-        }
-        wave_sync();
+        // sorted evaluation: one read per wave sorts to the next power of two of
+        // its item count; groups sharing a wave sort to the tier's limit
+        uint32_t P = 1;
+        while (P < (GW == 64 ? k : kmax))
+            P <<= 1;
+        extract_sorted<GW>(r, side, kw, k, P, keys, meta, emitf, pre, lcpv, out);
     }
     return true;
 }
