@@ -339,16 +339,20 @@ def main():
     # HBM/fabric bytes per k_find launch from the committed counter runs of this
     # same workload: FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for
     # the plain copy kernel
+    # (the autotuner's choice: one or two search chains per lane)
+    kname = "k_find2" if tune["find_chains"] == 2 else "k_find1"
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"config{args.config}_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            tk = json.load(f)["kernels"].get("k_find")
+            kernels = json.load(f)["kernels"]
+        tk = next((v for k, v in kernels.items() if k == kname or k.startswith(kname + "<")),
+                  None)
         if tk:
             traffic = tk["traffic_bytes"][0]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_find", "alg_bytes_per_launch": int(alg_bytes),
+                "kernel": kname, "alg_bytes_per_launch": int(alg_bytes),
                 "rank_positions_per_read": round(positions / (n * args.steps), 1),
                 "lines_per_read": round(lines / (n * args.steps), 1),
                 "avg_launch_ms": round(avg_launch_ms, 3),
