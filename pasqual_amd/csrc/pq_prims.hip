@@ -1,16 +1,9 @@
 // pq_prims.hip -- device-wide primitives written for wave64 / gfx950:
-// exclusive scan (reduce-then-scan, one wave per 1024-element strip) and one
-// stable LSD radix pass over (uint32 or uint64 key, uint32 value) pairs.
-//
-// Radix pass: a block owns a tile of 4096 pairs; each of its 4 waves owns 1024
-// contiguous pairs and walks them in 16 coalesced rounds of 64.  Within a round
-// a lane finds the lanes holding the same 8-bit digit with 8 ballots, and its
-// rank among them with one popcount; a per-wave LDS counter per digit carries
-// the rank across rounds, so the ranking is stable without any block-wide
-// barrier until the tile is complete.  The tile is then staged in LDS in digit
-// order and written out so that lanes write consecutive addresses of each
-// digit's run.
+// exclusive scan (reduce-then-scan, one wave per 1024-element strip) and the
+// array forms of the stable LSD radix pass of pq_radix.h over (uint32 or uint64
+// key, uint32 or uint64 value) pairs.
 #include "pq_ctx.h"
+#include "pq_radix.h"
 
 #include <cstring>
 
@@ -203,174 +196,28 @@ int pq_scan_u4(pq_ctx_t *ctx, const void *in, void *out, uint64_t count, void *t
     return scan_impl<U4>(ctx, (const U4 *) in, (U4 *) out, count, (U4 *) total_host, 0);
 }
 
-// ---- LSD radix pass ------------------------------------------------------------
-
-#define RADIX_THREADS 256
-
-// Items per thread of a radix tile: 16, or 8 when key + value exceed 12 bytes
-// (the tile is staged in LDS).
-template <typename KT, typename VT> constexpr int radix_rounds()
-{
-    return sizeof(KT) + sizeof(VT) > 12 ? 8 : 16;
-}
-
-template <typename KT, int ROUNDS>
-__global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(const KT *keys, uint64_t count,
-                                                             unsigned shift, uint32_t *hist,
-                                                             uint32_t nblocks)
-{
-    constexpr int RADIX_TILE = RADIX_THREADS * ROUNDS;
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    uint64_t base = (uint64_t) blockIdx.x * RADIX_TILE;
-    for (int r = 0; r < ROUNDS; r++) {
-        uint64_t i = base + (uint64_t) r * RADIX_THREADS + threadIdx.x;
-        if (i < count)
-            atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    hist[(uint64_t) threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
-}
-
-template <typename KT, typename VT, int ROUNDS>
-__global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(
-    const KT *kin, const VT *vin, KT *kout, VT *vout, uint64_t count,
-    unsigned shift, const uint32_t *offs, uint32_t nblocks)
-{
-    constexpr int RADIX_TILE = RADIX_THREADS * ROUNDS;
-    constexpr int RADIX_ROUNDS = ROUNDS;
-    __shared__ uint32_t wcnt[4][256];
-    __shared__ uint32_t tstart[256];
-    __shared__ KT sk[RADIX_TILE];
-    __shared__ VT sv[RADIX_TILE];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < 4 * 256; i += RADIX_THREADS)
-        (&wcnt[0][0])[i] = 0;
-    __syncthreads();
-
-    const uint64_t tile = (uint64_t) blockIdx.x * RADIX_TILE;
-    const uint64_t wbase = tile + (uint64_t) wid * (RADIX_TILE / 4);
-    const uint64_t lt = (1ull << lane) - 1ull;
-    KT key[RADIX_ROUNDS];
-    VT val[RADIX_ROUNDS];
-    uint32_t rnk[RADIX_ROUNDS];
-#pragma unroll
-    for (int r = 0; r < RADIX_ROUNDS; r++) {
-        uint64_t i = wbase + (uint64_t) r * 64 + lane;
-        bool ok = i < count;
-        key[r] = ok ? kin[i] : 0;
-        val[r] = ok ? vin[i] : 0;
-        unsigned d = (unsigned) (key[r] >> shift) & 255u;
-        uint64_t peers = __ballot(ok);
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            uint64_t m = __ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? m : ~m;
-        }
-        uint32_t before = (uint32_t) __popcll(peers & lt);
-        uint32_t base = wcnt[wid][d];
-        rnk[r] = base + before;
-        // the highest peer lane updates the running count after all lanes read
-        if (ok && (peers >> lane) == 1ull)
-            wcnt[wid][d] = base + (uint32_t) __popcll(peers);
-    }
-    __syncthreads();
-    // digit totals of the tile and exclusive prefix over waves and digits
-    {
-        int d = threadIdx.x;
-        uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d], c3 = wcnt[3][d];
-        uint32_t tot = c0 + c1 + c2 + c3;
-        // block-wide exclusive scan over the 256 digit totals
-        uint32_t inc = tot;
-        for (int s = 1; s < 64; s <<= 1) {
-            uint32_t o = __shfl_up(inc, s, 64);
-            if (lane >= s)
-                inc += o;
-        }
-        __shared__ uint32_t wsum[4];
-        if (lane == 63)
-            wsum[wid] = inc;
-        __syncthreads();
-        uint32_t pre = 0;
-        for (int w = 0; w < wid; w++)
-            pre += wsum[w];
-        tstart[d] = pre + inc - tot;
-        __syncthreads();
-        wcnt[0][d] = 0;
-        wcnt[1][d] = c0;
-        wcnt[2][d] = c0 + c1;
-        wcnt[3][d] = c0 + c1 + c2;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < RADIX_ROUNDS; r++) {
-        uint64_t i = wbase + (uint64_t) r * 64 + lane;
-        if (i < count) {
-            unsigned d = (unsigned) (key[r] >> shift) & 255u;
-            uint32_t p = tstart[d] + wcnt[wid][d] + rnk[r];
-            sk[p] = key[r];
-            sv[p] = val[r];
-        }
-    }
-    __syncthreads();
-    uint64_t valid = count - tile < RADIX_TILE ? count - tile : RADIX_TILE;
-    for (uint32_t p = threadIdx.x; p < valid; p += RADIX_THREADS) {
-        KT k = sk[p];
-        unsigned d = (unsigned) (k >> shift) & 255u;
-        uint64_t o = (uint64_t) offs[(uint64_t) d * nblocks + blockIdx.x] + (p - tstart[d]);
-        kout[o] = k;
-        vout[o] = sv[p];
-    }
-}
-
-// One stable pass by the 8-bit digit at `shift`.  Scratch slot 1 (histogram).
-template <typename KT, typename VT>
-static int radix_pass_t(pq_ctx_t *ctx, const KT *kin, const VT *vin, KT *kout, VT *vout,
-                        uint64_t count, unsigned shift)
-{
-    constexpr int ROUNDS = radix_rounds<KT, VT>();
-    constexpr int RADIX_TILE = RADIX_THREADS * ROUNDS;
-    if (count == 0)
-        return PQ_OK;
-    if (count > 0xFFFFFFFFull)
-        return ctx->fail(PQ_ERR_LIMIT, "radix pass over more than 2^32 items");
-    uint32_t nb = (uint32_t) ((count + RADIX_TILE - 1) / RADIX_TILE);
-    uint32_t *hist = (uint32_t *) ctx->tmp(1, (size_t) nb * 256 * sizeof(uint32_t) * 2);
-    if (!hist)
-        return ctx->fail(PQ_ERR_NOMEM, "radix histogram");
-    uint32_t *offs = hist + (size_t) nb * 256;
-    hipLaunchKernelGGL((k_radix_hist<KT, ROUNDS>), dim3(nb), dim3(RADIX_THREADS), 0, ctx->stream,
-                       kin, count, shift, hist, nb);
-    int rc = pq_scan_u32(ctx, hist, offs, (uint64_t) nb * 256, nullptr);
-    if (rc)
-        return rc;
-    hipLaunchKernelGGL((k_radix_scatter<KT, VT, ROUNDS>), dim3(nb), dim3(RADIX_THREADS), 0,
-                       ctx->stream, kin, vin, kout, vout, count, shift, offs, nb);
-    PQ_CHECK(hipGetLastError());
-    return PQ_OK;
-}
+// ---- LSD radix pass (kernels in pq_radix.h) ------------------------------------
 
 int pq_radix_pass(pq_ctx_t *ctx, const uint64_t *kin, const uint32_t *vin, uint64_t *kout,
                   uint32_t *vout, uint64_t count, unsigned shift)
 {
-    return radix_pass_t<uint64_t, uint32_t>(ctx, kin, vin, kout, vout, count, shift);
+    return radix_pass_src(ctx, ArraySrc<uint64_t, uint32_t>{kin, vin}, kout, vout, count, shift);
 }
 
 int pq_radix_pass(pq_ctx_t *ctx, const uint64_t *kin, const uint64_t *vin, uint64_t *kout,
                   uint64_t *vout, uint64_t count, unsigned shift)
 {
-    return radix_pass_t<uint64_t, uint64_t>(ctx, kin, vin, kout, vout, count, shift);
+    return radix_pass_src(ctx, ArraySrc<uint64_t, uint64_t>{kin, vin}, kout, vout, count, shift);
 }
 
 int pq_radix_pass(pq_ctx_t *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                   uint32_t *vout, uint64_t count, unsigned shift)
 {
-    return radix_pass_t<uint32_t, uint32_t>(ctx, kin, vin, kout, vout, count, shift);
+    return radix_pass_src(ctx, ArraySrc<uint32_t, uint32_t>{kin, vin}, kout, vout, count, shift);
 }
 
 int pq_radix_pass(pq_ctx_t *ctx, const uint32_t *kin, const uint64_t *vin, uint32_t *kout,
                   uint64_t *vout, uint64_t count, unsigned shift)
 {
-    return radix_pass_t<uint32_t, uint64_t>(ctx, kin, vin, kout, vout, count, shift);
+    return radix_pass_src(ctx, ArraySrc<uint32_t, uint64_t>{kin, vin}, kout, vout, count, shift);
 }

@@ -20,6 +20,7 @@
 //     the next chunk offset.
 //
 #include "pq_ctx.h"
+#include "pq_radix.h"
 
 #include <algorithm>
 #include <cstring>
@@ -67,17 +68,20 @@ __device__ __forceinline__ uint32_t bucket_key(const Geom &G, const uint64_t *t2
     return (chars << 4) | f;
 }
 
-// Keys of all N suffixes in position order (one slice holds every suffix).
-template <typename R>
-__global__ void k_bucket_keys(Geom G, const uint64_t *t2, uint64_t N, unsigned K, uint32_t *BK,
-                              R *V)
-{
-    PQ_GRID_STRIDE_FOR(g, N)
+// The (bucket key, position) pairs of all N suffixes in position order, made
+// on the fly by the first radix pass of a build in one slice (no key array is
+// written and read back).
+template <typename R> struct BucketSrc {
+    Geom G;
+    const uint64_t *t2;
+    unsigned K;
+    __device__ __forceinline__ uint32_t key(uint64_t g) const { return bucket_key(G, t2, g, K); }
+    __device__ __forceinline__ void load(uint64_t g, uint32_t &k, R &v) const
     {
-        BK[g] = bucket_key(G, t2, g, K);
-        V[g] = (R) g;
+        k = bucket_key(G, t2, g, K);
+        v = (R) g;
     }
-}
+};
 
 // ---- slices of the suffix set ---------------------------------------------------------
 //
@@ -849,9 +853,14 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
         uint32_t *kc = (passes & 1) ? BK2 : S.BK, *ka = (passes & 1) ? S.BK : BK2;
         R *vc = (passes & 1) ? V2 : S.sa, *va = (passes & 1) ? S.sa : V2;
         // 1. bucket keys by the first K codes (and the length below K)
+        unsigned p0 = 0;
         if (slices.size() == 1) {
-            hipLaunchKernelGGL(k_bucket_keys<R>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
-                               ctx->stream, G, D.text2, N, K, kc, vc);
+            // the first pass makes the pairs itself
+            if ((rc = radix_pass_src(ctx, BucketSrc<R>{G, D.text2, K}, ka, va, N, 0)))
+                return rc;
+            std::swap(kc, ka);
+            std::swap(vc, va);
+            p0 = 1;
         } else {
             uint32_t *boff = bcnt + ntiles + 1;
             hipLaunchKernelGGL(k_slice_count, dim3((unsigned) ntiles), dim3(256), 0, ctx->stream,
@@ -864,7 +873,7 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
             hipLaunchKernelGGL(k_slice_emit<R>, dim3((unsigned) ntiles), dim3(256), 0,
                                ctx->stream, G, D.text2, N, K, sl.lo, sl.hi, boff, kc, vc);
         }
-        for (unsigned p = 0; p < passes; p++) {
+        for (unsigned p = p0; p < passes; p++) {
             if ((rc = pq_radix_pass(ctx, kc, vc, ka, va, ns, 8 * p)))
                 return rc;
             std::swap(kc, ka);
