@@ -251,17 +251,68 @@ __device__ __forceinline__ bool suf_less(const SufW<W, R> &a, const SufW<W, R> &
     return a.g < b.g;
 }
 
-template <int W, typename R>
-__device__ __forceinline__ SufW<W, R> suf_shfl_xor(const SufW<W, R> &s, int j)
+// The value of lane l ^ J.  J = 1, 2: a DPP quad permutation inside the VALU;
+// J < 32: ds_swizzle in bit-mask mode (xor within 32 lanes, no address
+// arithmetic); 32: ds_bpermute.
+template <int J> __device__ __forceinline__ uint32_t xor_lane(uint32_t v)
+{
+    if constexpr (J == 1)
+        return (uint32_t) __builtin_amdgcn_mov_dpp((int) v, 0xB1, 0xF, 0xF, true);
+    else if constexpr (J == 2)
+        return (uint32_t) __builtin_amdgcn_mov_dpp((int) v, 0x4E, 0xF, 0xF, true);
+    else if constexpr (J < 32)
+        return (uint32_t) __builtin_amdgcn_ds_swizzle((int) v, (J << 10) | 0x1F);
+    else
+        return (uint32_t) __shfl_xor((int) v, J, 64);
+}
+
+template <int J> __device__ __forceinline__ uint64_t xor_lane64(uint64_t v)
+{
+    return ((uint64_t) xor_lane<J>((uint32_t) (v >> 32)) << 32) | xor_lane<J>((uint32_t) v);
+}
+
+template <int J, typename R> __device__ __forceinline__ R xor_lane_r(R v)
+{
+    if constexpr (sizeof(R) == 8)
+        return (R) xor_lane64<J>((uint64_t) v);
+    else
+        return (R) xor_lane<J>((uint32_t) v);
+}
+
+template <int J, int W, typename R>
+__device__ __forceinline__ SufW<W, R> suf_xor(const SufW<W, R> &s)
 {
     SufW<W, R> o;
-    o.g = __shfl_xor(s.g, j, 64);
-    o.len = __shfl_xor(s.len, j, 64);
-    o.sym = __shfl_xor(s.sym, j, 64);
+    o.g = xor_lane_r<J, R>(s.g);
+    o.len = xor_lane<J>(s.len);
+    o.sym = xor_lane<J>(s.sym);
 #pragma unroll
     for (int i = 0; i < W; i++)
-        o.w[i] = __shfl_xor(s.w[i], j, 64);
+        o.w[i] = xor_lane64<J>(s.w[i]);
     return o;
+}
+
+// Bitonic network over the lanes of a group: stage K (merge width), step J.
+template <int K, int J, int W, typename R>
+__device__ __forceinline__ void bitonic_steps(SufW<W, R> &s, int li)
+{
+    SufW<W, R> p = suf_xor<J, W, R>(s);
+    const bool up = (li & K) == 0;
+    const bool lower = (li & J) == 0;
+    const bool pless = suf_less<W, R>(p, s);
+    const bool swap = (lower == up) ? pless : (!pless && p.g != s.g);
+    if (swap)
+        s = p;
+    if constexpr (J > 1)
+        bitonic_steps<K, J / 2, W, R>(s, li);
+}
+
+template <int G, int K, int W, typename R>
+__device__ __forceinline__ void bitonic_stages(SufW<W, R> &s, int li)
+{
+    bitonic_steps<K, K / 2, W, R>(s, li);
+    if constexpr (K < G)
+        bitonic_stages<G, 2 * K, W, R>(s, li);
 }
 
 // ---- run lists -----------------------------------------------------------------------
@@ -341,19 +392,7 @@ __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2
         suf_load<W, R>(Gm, t2, sa[sg.x + li], base, s);
     else
         suf_none<W, R>(s);
-#pragma unroll
-    for (int k = 2; k <= G; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            SufW<W, R> p = suf_shfl_xor<W, R>(s, j);
-            bool up = (li & k) == 0;
-            bool lower = (li & j) == 0;
-            bool pless = suf_less<W, R>(p, s);
-            bool swap = (lower == up) ? pless : (!pless && p.g != s.g);
-            if (swap)
-                s = p;
-        }
-    }
+    bitonic_stages<G, 2, W, R>(s, li);
     if (e) {
         sa[sg.x + li] = s.g;
         bwt8[sg.x + li] = (uint8_t) s.sym;
