@@ -80,6 +80,7 @@ def cpu_baseline_reference(fasta, L, t, n_expected, exe_name="pasqual_omp_u32"):
     # takes: import, forward sort, backward sort, overlap graph, assembly
     fw, bw, graph = takes[1], takes[2], takes[3]
     return {"value": round(n / (graph / 1e3), 1), "unit": "reads/s", "cores": threads,
+            "import_ms": takes[0],
             "kind": "reference",
             "sample": f"the full workload ({n} reads), reference {exe_name}, "
                       f"OMP_NUM_THREADS={threads}, one run, overlap-graph timer "
@@ -110,7 +111,7 @@ def dropin_run(fasta, L, t):
     if not os.path.exists(exe):
         return None
     threads = min(16, os.cpu_count() or 1)
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), PQ_TIMING="1")
     with tempfile.TemporaryDirectory() as wd:
         r = subprocess.run([exe, "--singlereads", fasta, "-l", str(L), "-t", str(t)], cwd=wd,
                            capture_output=True, text=True, env=env, timeout=600)
@@ -120,8 +121,11 @@ def dropin_run(fasta, L, t):
         return None
     takes = [float(x) for x in re.findall(r"takes ([0-9.]+) ms", r.stdout)]
     n = int(re.search(r"has imported (\d+) reads", r.stdout).group(1))
+    # phases inside the graph timer, printed by the library under PQ_TIMING=1
+    phases = {k.replace(" ", "_") + "_ms": float(v)
+              for k, v in re.findall(r"\[pq\] ([a-z ]+) ([0-9.]+) ms", r.stderr)}
     return {"reads": n, "import_ms": takes[0], "fw_index_ms": takes[1], "bw_index_ms": takes[2],
-            "graph_ms": takes[3], "assembly_ms": takes[4],
+            "graph_ms": takes[3], "graph_phases": phases, "assembly_ms": takes[4],
             "graph_reads_per_s": round(n / (takes[3] / 1e3), 1), "host_threads": threads,
             "contigs_sha256": digest}
 
@@ -170,6 +174,11 @@ def cpu_baseline_reference_sample(pq, config, t):
     return base
 
 
+# reads import of the fixed-length workloads (file read, records cut on the
+# host, duplicate filter on the device), filled by load_workload
+IMPORT_INFO = {}
+
+
 def load_workload(pq, config, np):
     cfg = pq.synth_preset(config)
     t = cfg.min_overlap
@@ -185,6 +194,7 @@ def load_workload(pq, config, np):
         pq.synth_fasta(cfg, fasta)
         imp = pq.import_fasta(fasta, L, index_bits=32)
         n = imp["n"]
+        IMPORT_INFO.update(records=n + imp["removed"], kept=n, ms=round(imp["ms"], 1))
         off = np.arange(n + 1, dtype=np.uint64) * np.uint64(L)
         return imp["reads"], off, t, L, fasta, "after duplicate removal"
     codes, off = pq.synth_reads(cfg)
@@ -405,9 +415,12 @@ def main():
         "learned": learned,
         "graph_mode": graph,
         "dropin": dropin,
+        "import": dict(IMPORT_INFO) or None,
     }
     if learned:
         line["learned_vs_plain_search"] = round(learned["reads_per_s"] / reads_per_s, 3)
+    if base and IMPORT_INFO and base.get("import_ms"):
+        line["import"]["reference_ms"] = base["import_ms"]
     if base:
         line["speedup_vs_cpu_search"] = round(value / base["value"], 1)
         line["speedup_vs_cpu_sa_build"] = round(gsuffix_per_s / base["sa_build_gsuffix_per_s"], 1)

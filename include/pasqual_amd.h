@@ -143,8 +143,9 @@ typedef struct {
     float lr;                /* Adam step size (decays linearly to 1%) */
     uint32_t closed_form;    /* 1: solve the output layer by least squares before Adam */
     uint64_t seed;
-    uint64_t table_chunk;    /* suffix-array rows per step of the table build (16 bytes of
-                                scratch per row), 0: 2^28 */
+    uint64_t table_chunk;    /* suffix-array rows per step of the table build (24 bytes of
+                                scratch per row), 0: as many as half the free device
+                                memory holds (one step when the whole array fits) */
 } pq_learned_cfg_t;
 
 typedef struct {
@@ -256,7 +257,13 @@ int pq_synth_fasta(const pq_synth_cfg_t *cfg, const char *path);
 
 /* Read import with duplicate removal, the rules of import_singlereads
  * (index.c:145-405) and bloom_filter (bloom.c:91-135) with hashes in
- * index_bits (32 or 64) arithmetic.  *reads is malloc'ed (n*read_len codes). */
+ * index_bits (32 or 64) arithmetic; replaces import_singlereads behind
+ * construct_index (index.c:709-745).  The file is split into records on the
+ * host in parallel (lines as the reference's 1000-byte fgets buffer cuts them,
+ * index.c:218), and the duplicate filter runs on ctx's device
+ * (pq_dedup_reads).  *reads is malloc'ed (n*read_len codes, input order); on
+ * PQ_ERR_FORMAT bad_line is the line number the reference reports
+ * (2 * reads kept before the bad record + 2, index.c:223). */
 typedef struct {
     uint8_t *reads;
     uint64_t n;
@@ -265,8 +272,19 @@ typedef struct {
     double ms;
     long bad_line;           /* line number reported on PQ_ERR_FORMAT */
 } pq_import_t;
-int pq_import_fasta(const char *path, int read_len, int bloom_m, int n_hashfunc,
-                    int index_bits, pq_import_t *out);
+int pq_import_fasta(pq_ctx_t *ctx, const char *path, int read_len, int bloom_m,
+                    int n_hashfunc, int index_bits, pq_import_t *out);
+
+/* The duplicate filter of import_singlereads (index.c:327-333, bloom.c:91-135)
+ * over nrec parsed records: codes = nrec*read_len codes 1..4 in file order,
+ * ok[r] = 0 for a record holding N (dropped, sets no bits).  keep[r] = 1 for
+ * the records the reference keeps, given a filter of bloom_bits bits (computed
+ * in the index width by the caller, index.c:206) and n_hashfunc hashes.  The
+ * order-dependent filter is resolved exactly by passes over all records until
+ * no verdict changes; *passes (may be NULL) reports how many ran. */
+int pq_dedup_reads(pq_ctx_t *ctx, const uint8_t *codes, const uint8_t *ok, uint64_t nrec,
+                   int read_len, uint64_t bloom_bits, int n_hashfunc, int index_bits,
+                   uint8_t *keep, uint32_t *passes);
 
 #ifdef __cplusplus
 }

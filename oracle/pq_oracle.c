@@ -12,6 +12,7 @@
  */
 #include "pq_oracle.h"
 
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -938,4 +939,163 @@ double orc_mlp_loss_grad(uint32_t k, uint32_t h, const double *params, const uin
     free(f);
     free(pre);
     return loss / (double) n;
+}
+
+/* ---- reads import with duplicate removal (index.c:145-405, bloom.c) ---------- */
+
+/* The hashes of bloom.c:18-86 over a NUL-terminated code string, in the index
+ * width: wide = 0 reduces every step modulo 2^32. */
+static uint64_t orc_hash(int f, const uint8_t *k, uint64_t size, int wide)
+{
+    const uint64_t mask = wide ? ~0ull : 0xFFFFFFFFull;
+    uint64_t h = 0;
+    for (; *k; k++) {
+        const uint64_t c = *k;
+        switch (f) {
+        case 0: h ^= ((h << 5) + (h >> 2) + c) & mask; break;
+        case 1: h = c + (h << 6) + (h << 16) - h; break;
+        case 2: h = h + c; break;
+        case 3: h ^= c; break;
+        default: h = 33 * h + c; break;
+        }
+        h &= mask;
+    }
+    return h % size;
+}
+
+/* bloom_filter, bloom.c:91-135: 1 = seen (read or its reverse complement),
+ * 0 = new, its own bits set. */
+static int orc_bloom(uint8_t *a, uint64_t size, int nf, const uint8_t *s,
+                     const uint8_t *rs, int wide)
+{
+    uint64_t hv[5];
+    int miss = 0;
+    for (int i = 0; i < nf; i++) {
+        hv[i] = orc_hash(i, s, size, wide);
+        if (!(a[hv[i] / 8] & (1u << (hv[i] % 8))))
+            miss = 1;
+    }
+    if (!miss)
+        return 1;
+    for (int i = 0; i < nf; i++) {
+        uint64_t r = orc_hash(i, rs, size, wide);
+        if (!(a[r / 8] & (1u << (r % 8)))) {
+            for (int j = 0; j < nf; j++)
+                a[hv[j] / 8] |= (uint8_t) (1u << (hv[j] % 8));
+            return 0;
+        }
+    }
+    return 1;
+}
+
+void orc_dedup(const uint8_t *codes, const uint8_t *ok, uint64_t nrec, int read_len,
+               uint64_t bloom_bits, int n_hashfunc, int index_bits, uint8_t *keep)
+{
+    const int L = read_len, wide = index_bits == 64;
+    uint8_t *a = calloc((size_t) (bloom_bits / 8 + 2), 1);
+    uint8_t *s = malloc((size_t) L + 1), *rs = malloc((size_t) L + 1);
+    for (uint64_t r = 0; r < nrec; r++) {
+        keep[r] = 0;
+        if (!ok[r])
+            continue;
+        for (int i = 0; i < L; i++) {
+            s[i] = codes[r * (uint64_t) L + i];
+            rs[L - 1 - i] = (uint8_t) (5 - s[i]);
+        }
+        s[L] = rs[L] = 0;
+        keep[r] = orc_bloom(a, bloom_bits, n_hashfunc, s, rs, wide) ? 0 : 1;
+    }
+    free(a);
+    free(s);
+    free(rs);
+}
+
+int orc_import_fasta(const char *path, int read_len, int bloom_m, int n_hashfunc,
+                     int index_bits, uint8_t **reads, uint64_t *n, uint64_t *removed,
+                     long *bad_line)
+{
+    char line[1000];
+    FILE *fp = fopen(path, "rb");
+    *reads = NULL;
+    *n = *removed = 0;
+    *bad_line = 0;
+    if (!fp)
+        return -2;
+    uint64_t records = 0;
+    while (fgets(line, 1000, fp))
+        records += line[0] == '>';
+    const int wide = index_bits == 64;
+    const uint64_t bits = wide ? records * (uint64_t) bloom_m
+                               : (uint64_t) (uint32_t) ((uint32_t) records * (uint32_t) bloom_m);
+    uint8_t *a = calloc((size_t) (bits / 8 + 2), 1);
+    uint8_t *s = malloc((size_t) read_len + 1), *rs = malloc((size_t) read_len + 1);
+    uint64_t cap = 1024, kept = 0, dropped = 0;
+    uint8_t *out = malloc(cap * (size_t) read_len);
+    int rc = (a && s && rs && out) ? 0 : -5;
+    rewind(fp);
+    while (rc == 0 && fgets(line, 1000, fp)) {
+        if (line[0] != '>' || !fgets(line, 1000, fp)) {
+            rc = -3;
+            break;
+        }
+        size_t len = strlen(line);
+        if (len == 0 || len - 1 != (size_t) read_len) {
+            rc = -3;
+            break;
+        }
+        int with_n = 0;
+        for (int i = 0; i < read_len && rc == 0; i++) {
+            uint8_t v = 0;
+            switch (line[i]) {
+            case 'a': case 'A': v = 1; break;
+            case 'c': case 'C': v = 2; break;
+            case 'g': case 'G': v = 3; break;
+            case 't': case 'T': v = 4; break;
+            case 'n': case 'N': with_n = 1; break;
+            default: rc = -3;
+            }
+            s[i] = v;
+            rs[read_len - 1 - i] = (uint8_t) (v ? 5 - v : 0);
+        }
+        if (rc)
+            break;
+        s[read_len] = rs[read_len] = 0;
+        if (with_n) {
+            dropped++;
+            continue;
+        }
+        if (bits == 0) {            /* h % 0: the reference faults */
+            rc = -7;
+            break;
+        }
+        if (orc_bloom(a, bits, n_hashfunc, s, rs, wide)) {
+            dropped++;
+            continue;
+        }
+        if (kept == cap) {
+            uint8_t *nb = realloc(out, 2 * cap * (size_t) read_len);
+            if (!nb) {
+                rc = -5;
+                break;
+            }
+            out = nb;
+            cap *= 2;
+        }
+        memcpy(out + kept * (uint64_t) read_len, s, (size_t) read_len);
+        kept++;
+    }
+    fclose(fp);
+    free(a);
+    free(s);
+    free(rs);
+    if (rc == -3)
+        *bad_line = (long) (kept * 2 + 2);
+    if (rc) {
+        free(out);
+        return rc;
+    }
+    *reads = out;
+    *n = kept;
+    *removed = dropped;
+    return 0;
 }

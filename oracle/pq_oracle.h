@@ -121,6 +121,26 @@ int64_t orc_check_sa(const uint8_t *dtext, const uint64_t *off, uint64_t n,
 double orc_mlp_loss_grad(uint32_t k, uint32_t h, const double *params, const uint64_t *xs,
                          const double *targets, uint64_t n, double *grad);
 
+/* Reads import with duplicate removal, sequential, as import_singlereads
+ * does it (index.c:145-405): a first pass counts the fgets(line, 1000) pieces
+ * starting with '>' (index.c:195-206), the filter has records * bloom_m bits
+ * computed in index_bits arithmetic, then pieces are taken two at a time
+ * (header, sequence; the sequence's last byte dropped, index.c:253), codes
+ * mapped (index.c:280-321), reads with N dropped, and bloom_filter
+ * (bloom.c:91-135) with the hashes of bloom.c:18-86 decides duplicates.
+ * Returns 0, or -2 (file cannot be opened), -3 (malformed: *bad_line = the
+ * line number the reference prints, index.c:223), -5 (no memory), -7 (filter
+ * of 0 bits).  *reads is malloc'ed (n * read_len codes); free with free(). */
+int orc_import_fasta(const char *path, int read_len, int bloom_m, int n_hashfunc,
+                     int index_bits, uint8_t **reads, uint64_t *n, uint64_t *removed,
+                     long *bad_line);
+
+/* bloom_filter (bloom.c:91-135) applied in record order to nrec parsed
+ * records (codes 1..4, read_len each); records with ok[r] = 0 (holding N) are
+ * skipped.  keep[r] = 1 when the record is kept. */
+void orc_dedup(const uint8_t *codes, const uint8_t *ok, uint64_t nrec, int read_len,
+               uint64_t bloom_bits, int n_hashfunc, int index_bits, uint8_t *keep);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,10 @@ def lib():
         _lib.orc_check_sa.restype = C.c_int64
         _lib.orc_mlp_loss_grad.argtypes = [C.c_uint32, C.c_uint32, P, P, P, C.c_uint64, P]
         _lib.orc_mlp_loss_grad.restype = C.c_double
+        _lib.orc_import_fasta.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(C.c_void_p), C.POINTER(C.c_uint64),
+                                          C.POINTER(C.c_uint64), C.POINTER(C.c_long)]
+        _lib.orc_dedup.argtypes = [P, P, C.c_uint64, C.c_int, C.c_uint64, C.c_int, C.c_int, P]
     return _lib
 
 
@@ -88,6 +92,34 @@ def bwt(dtext, off, sa):
     ri = np.empty(max(1, n), np.uint64)
     lib().orc_bwt(_p(dtext), _p(off), n, _p(sa), _p(b), _p(ri))
     return b[:N], ri[:n]
+
+
+def import_fasta(path, L, bloom_m=36, n_hashfunc=2, index_bits=32):
+    """Sequential reads import (orc_import_fasta): dict like the library's, or
+    {"rc": code, "bad_line": line} when it fails."""
+    rp, n, rem, bad = C.c_void_p(), C.c_uint64(), C.c_uint64(), C.c_long()
+    rc = lib().orc_import_fasta(str(path).encode(), L, bloom_m, n_hashfunc, index_bits,
+                                C.byref(rp), C.byref(n), C.byref(rem), C.byref(bad))
+    if rc != 0:
+        return {"rc": rc, "bad_line": bad.value}
+    k = int(n.value)
+    reads = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_uint8)), shape=(max(k * L, 1),))
+    reads = reads[:k * L].copy()
+    C.CDLL(None).free(rp)
+    off = np.arange(k + 1, dtype=np.uint64) * np.uint64(L)
+    Cc, sfx, pfx = counts(reads, off) if k else (np.zeros(6, np.uint64),) + (np.zeros(4, np.uint64),) * 2
+    return {"rc": 0, "reads": reads, "n": k, "removed": int(rem.value), "C": Cc.tolist(),
+            "suffix_count": sfx.tolist(), "prefix_count": pfx.tolist()}
+
+
+def dedup(codes, ok, L, bloom_bits, n_hashfunc=2, index_bits=32):
+    """bloom_filter applied in record order to parsed records (orc_dedup)."""
+    codes = np.ascontiguousarray(codes, dtype=np.uint8)
+    ok = np.ascontiguousarray(ok, dtype=np.uint8)
+    keep = np.zeros(max(len(ok), 1), np.uint8)
+    lib().orc_dedup(_p(codes), _p(ok), len(ok), L, int(bloom_bits), n_hashfunc, index_bits,
+                    _p(keep))
+    return keep[:len(ok)]
 
 
 def counts(codes, off):
@@ -201,10 +233,12 @@ def ref_available(width="u32"):
     return os.path.exists(os.path.join(REFDIR, f"ref_dump_{width}"))
 
 
-def run_ref_dump(fasta, L, t, out, width="u32", graph=True, threads=1):
+def run_ref_dump(fasta, L, t, out, width="u32", graph=True, threads=1, bloom_m=36,
+                 n_hashfunc=2):
     exe = os.path.join(REFDIR, f"ref_dump_{width}")
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     args = [exe, str(fasta), str(L), str(t), str(out)] + ([] if graph else ["nograph"])
+    args += [f"bloom={bloom_m}", f"nf={n_hashfunc}"]
     r = subprocess.run(args, capture_output=True, text=True, env=env)
     if r.returncode != 0:
         raise RuntimeError(f"ref_dump failed: {r.stderr}")

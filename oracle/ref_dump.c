@@ -5,10 +5,11 @@
  * to one binary file, so tests can compare the oracle and the HIP library with
  * the reference bit for bit.
  *
- *   ref_dump <reads.fa> <read_len> <min_overlap> <out.bin> [nograph]
+ *   ref_dump <reads.fa> <read_len> <min_overlap> <out.bin> [nograph] [bloom=M] [nf=K]
  *
  * Calls construct_index (index.h:64-68) with the CLI defaults bloom_m=36,
- * n_hashfunc=2 (test.c:136-148) and construct_string_graph (string_graph.h:139).
+ * n_hashfunc=2 (test.c:136-148) unless bloom=/nf= say otherwise, and
+ * construct_string_graph (string_graph.h:139).
  *
  * File layout: every integer is written as little-endian uint64.
  *   header  : magic 0x31464552514150 ("PQAREF1"), W, n, L, t, N
@@ -47,8 +48,16 @@ int main(int argc, char **argv)
         return 2;
     }
     int L = atoi(argv[2]), t = atoi(argv[3]);
-    int nograph = argc > 5 && strcmp(argv[5], "nograph") == 0;
-    index_t *ix = construct_index(argv[1], NULL, NULL, NULL, NULL, t, L, 36, 2);
+    int nograph = 0, bloom_m = 36, nf = 2;
+    for (int a = 5; a < argc; a++) {
+        if (strcmp(argv[a], "nograph") == 0)
+            nograph = 1;
+        else if (strncmp(argv[a], "bloom=", 6) == 0)
+            bloom_m = atoi(argv[a] + 6);
+        else if (strncmp(argv[a], "nf=", 3) == 0)
+            nf = atoi(argv[a] + 3);
+    }
+    index_t *ix = construct_index(argv[1], NULL, NULL, NULL, NULL, t, L, bloom_m, nf);
     size_t n = ix->num_reads, N = n * (size_t) L, rows = (N + 64) / 64;
 
     fo = fopen(argv[4], "wb");

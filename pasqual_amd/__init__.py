@@ -38,6 +38,7 @@ PQ_SYMBOLS = [
     "pq_probe_copy", "pq_set_layout", "pq_get_layout", "pq_set_find_chains", "pq_synth_preset", "pq_synth_num_reads",
     "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
+    "pq_dedup_reads",
 ]
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
                "time_importreads"]
@@ -165,7 +166,8 @@ def lib(width="u32"):
         "pq_synth_lengths": (I32, [P, P]),
         "pq_synth_reads": (I32, [P, P, P, P]),
         "pq_synth_fasta": (I32, [P, C.c_char_p]),
-        "pq_import_fasta": (I32, [C.c_char_p, I32, I32, I32, I32, P]),
+        "pq_import_fasta": (I32, [P, C.c_char_p, I32, I32, I32, I32, P]),
+        "pq_dedup_reads": (I32, [P, P, P, U64, I32, U64, I32, I32, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -208,13 +210,20 @@ def synth_fasta(cfg, path):
         raise PqError(f"writing {path} failed")
 
 
-def import_fasta(path, read_len, bloom_m=36, n_hashfunc=2, index_bits=32):
-    """Reads import with duplicate removal (index.c:145-405); returns a dict."""
+def import_fasta(path, read_len, bloom_m=36, n_hashfunc=2, index_bits=32, ctx=None):
+    """Reads import with duplicate removal (index.c:145-405), the duplicate
+    filter on ctx's device (a context of its own when ctx is None); returns a
+    dict."""
+    if ctx is None:
+        with Context() as c:
+            return import_fasta(path, read_len, bloom_m, n_hashfunc, index_bits, c)
     res = ImportResult()
-    rc = lib().pq_import_fasta(str(path).encode(), read_len, bloom_m, n_hashfunc, index_bits,
-                               C.byref(res))
+    rc = ctx._L.pq_import_fasta(ctx._h, str(path).encode(), read_len, bloom_m, n_hashfunc,
+                                index_bits, C.byref(res))
     if rc != 0:
-        raise PqError(f"import of {path} failed: {ERRORS.get(rc, rc)} line {res.bad_line}")
+        msg = ctx._L.pq_last_error(ctx._h) if rc not in (-2, -3) else b""
+        raise PqError(f"import of {path} failed: {ERRORS.get(rc, rc)} line {res.bad_line}"
+                      + (f": {msg.decode()}" if msg else ""))
     n = int(res.n)
     reads = np.ctypeslib.as_array(res.reads, shape=(n * read_len,)).copy() if n else \
         np.zeros(0, np.uint8)
@@ -257,6 +266,21 @@ class Context:
         if rc != 0:
             msg = self._L.pq_last_error(self._h)
             raise PqError(f"{what}: {ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def dedup_reads(self, codes, ok, read_len, bloom_bits, n_hashfunc=2, index_bits=32):
+        """The import's duplicate filter over parsed records (pq_dedup_reads):
+        codes = records * read_len codes 1..4, ok = 0 for records holding N.
+        Returns (keep flags, passes)."""
+        codes = np.ascontiguousarray(codes, dtype=np.uint8)
+        ok = np.ascontiguousarray(ok, dtype=np.uint8)
+        nrec = len(ok)
+        assert codes.size == nrec * read_len
+        keep = np.zeros(max(nrec, 1), np.uint8)
+        passes = C.c_uint32()
+        self._check(self._L.pq_dedup_reads(self._h, _ptr(codes), _ptr(ok), nrec, read_len,
+                                           int(bloom_bits), n_hashfunc, index_bits, _ptr(keep),
+                                           C.byref(passes)), "dedup_reads")
+        return keep[:nrec], int(passes.value)
 
     @property
     def n(self):

@@ -22,7 +22,8 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
 ARCH = os.environ.get("PQ_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["pq_prims.hip", "pq_sa.hip", "pq_search.hip", "pq_api.hip", "pq_learned.hip"]
+HIP_SOURCES = ["pq_prims.hip", "pq_sa.hip", "pq_search.hip", "pq_api.hip", "pq_learned.hip",
+               "pq_dedup.hip"]
 HOST_SOURCES = ["host/pq_synth.c", "host/pq_import.c"]
 
 HIP_FLAGS = [

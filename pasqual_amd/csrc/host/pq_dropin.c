@@ -6,7 +6,8 @@
  * Linked in place of index.o + sufsort.o + string_graph.o, the reference's CLI
  * (test.c), bloom.c, common.c and assemble.c run unchanged on top of it.
  *
- *   construct_index         index.c:709-745: reads import (pq_import.c), then
+ *   construct_index         index.c:709-745: reads import (pq_import.c, its
+ *                           duplicate filter on the device), then
  *                           both suffix arrays / BWTs / occurrence tables on the
  *                           device; the host-layout arrays of index_t are filled
  *                           from the device index unless PQ_HOST_INDEX=0.
@@ -22,6 +23,7 @@
  * search with the learned index), PQ_HOST_INDEX=0 (leave the host occurrence
  * arrays of index_t NULL; free(NULL) in the caller stays valid).
  */
+#include <malloc.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -162,7 +164,8 @@ index_t *construct_index(char *single_file, char *pair_file1, char *pair_file2,
         pq_import_t imp;
         printf("Importing reads from %s ... ", single_file);
         fflush(stdout);
-        int rc = pq_import_fasta(single_file, read_len, bloom_m, n_hashfunc,
+        ensure_ctx();
+        int rc = pq_import_fasta(g_ctx, single_file, read_len, bloom_m, n_hashfunc,
                                  (int) (8 * sizeof(INDEX_TYPE_T)), &imp);
         if (rc == PQ_ERR_IO)
             REF_ERROR_EXIT("open reads file %s failed.\n", single_file);
@@ -245,7 +248,11 @@ string_graph_t *construct_string_graph(index_t *ix)
             dev_fail("learned index training", rc);
         mode = PQ_SEARCH_LEARNED;
     }
+    const int timing = env_int("PQ_TIMING", 0);
+    double tp = now_ms();
     int rc = pq_find_overlaps(g_ctx, mode);
+    if (timing)
+        fprintf(stderr, "[pq] search %.3f ms\n", now_ms() - tp), tp = now_ms();
     if (rc == PQ_ERR_LIMIT)
         REF_ERROR_EXIT("assembly failed. Try a larger min overlap length.\n");
     if (rc)
@@ -265,6 +272,8 @@ string_graph_t *construct_string_graph(index_t *ix)
         dev_fail("graph copy", rc);
     if ((rc = pq_graph_copy_edges(g_ctx, all, (uint32_t) sizeof(INDEX_TYPE_T))))
         dev_fail("graph copy", rc);
+    if (timing)
+        fprintf(stderr, "[pq] graph copy %.3f ms\n", now_ms() - tp), tp = now_ms();
     g->nodes = calloc(n ? n : 1, sizeof(node_t));
     if (!g->nodes)
         dev_fail("graph nodes", PQ_ERR_NOMEM);
@@ -279,7 +288,11 @@ string_graph_t *construct_string_graph(index_t *ix)
     double overlap_len = 0.0;
     const int RL = ix->read_len;
     /* node records, string_graph.c:832-1021; edges are malloc'ed per node because
-     * assemble.c frees them one node at a time */
+     * assemble.c frees them one node at a time.  The heaps grow in steps of 64 MiB
+     * meanwhile (glibc's default step is 128 KiB): millions of small blocks from
+     * many threads otherwise spend most of the loop growing the heaps, which
+     * stalls the page faults of every other thread. */
+    mallopt(M_TOP_PAD, 64 << 20);
 #pragma omp parallel for schedule(dynamic, 4096) reduction(+ : overlaps, overlap_len)
     for (long long ii = 0; ii < (long long) n; ii++) {
         uint64_t i = (uint64_t) ii;
@@ -330,7 +343,10 @@ string_graph_t *construct_string_graph(index_t *ix)
         overlaps += nb + nf;
         overlap_len += (double) olen;
     }
+    mallopt(M_TOP_PAD, 128 * 1024);
     g->average_overlap = overlap_len / (double) overlaps;
+    if (timing)
+        fprintf(stderr, "[pq] node build %.3f ms\n", now_ms() - tp);
     free(ovB);
     free(ovF);
     free(nB);

@@ -1,213 +1,307 @@
 /*
  * pq_import.c -- reads-file import with duplicate removal.
  *
- * Host-side, sequential by nature (the duplicate filter depends on insertion
- * order).  Follows import_singlereads (index.c:145-405): a two-line FASTA read
- * with a 1000-byte line buffer, the last byte of each sequence line dropped as
+ * Follows import_singlereads (index.c:145-405): a two-line FASTA read through
+ * a 1000-byte fgets buffer (index.c:218; a longer line arrives as several
+ * pieces of at most 999 bytes), the last byte of each sequence line dropped as
  * its newline (index.c:253), exact read length required, A/C/G/T (either case)
  * mapped to 1..4, reads containing N dropped, and a Bloom filter of
- * bloom_m * #records bits probed with the read and its reverse complement
- * (bloom.c:91-135).  The hash functions are the five string hashes of
- * bloom.c:18-86, evaluated in the index width (INDEX_TYPE_T), which decides
- * which reads survive; kept reads get ids 0..n-1 in input order.
+ * bloom_m * #records bits (records = pieces starting with '>', index.c:195-206)
+ * probed with the read and its reverse complement (bloom.c:91-135); kept reads
+ * get ids 0..n-1 in input order.
+ *
+ * The reference does all of this in one sequential loop.  Here the file is
+ * read whole, cut into fgets pieces and records by all host threads, the
+ * order-dependent filter is resolved on the device (pq_dedup_reads, exact),
+ * and the kept reads are gathered by all threads again.
  */
 #include "pq_host.h"
 
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 #include <sys/time.h>
 
-#define LINE_BUF 1000
+#define PIECE_MAX 999     /* bytes fgets(line, 1000, fp) returns at most */
 
-/* ---- hashes over a NUL-terminated code string, in 32- or 64-bit arithmetic ---- */
-
-#define DEF_HASHES(T, S)                                                      \
-    static T h0_##S(const char *k, T size)                                    \
-    {                                                                         \
-        T h = 0;                                                              \
-        while (*k)                                                            \
-            h ^= (T) ((h << 5) + (h >> 2) + (unsigned char) *k++);            \
-        return h % size;                                                      \
-    }                                                                         \
-    static T h1_##S(const char *k, T size)                                    \
-    {                                                                         \
-        T h = 0;                                                              \
-        while (*k)                                                            \
-            h = (T) ((unsigned char) *k++ + (h << 6) + (h << 16) - h);        \
-        return h % size;                                                      \
-    }                                                                         \
-    static T h2_##S(const char *k, T size)                                    \
-    {                                                                         \
-        T h = 0;                                                              \
-        while (*k)                                                            \
-            h = (T) (h + (T) (signed char) *k++);                             \
-        return h % size;                                                      \
-    }                                                                         \
-    static T h3_##S(const char *k, T size)                                    \
-    {                                                                         \
-        T h = 0;                                                              \
-        while (*k)                                                            \
-            h ^= (T) (signed char) *k++;                                      \
-        return h % size;                                                      \
-    }                                                                         \
-    static T h4_##S(const char *k, T size)                                    \
-    {                                                                         \
-        T h = 0;                                                              \
-        while (*k)                                                            \
-            h = (T) (33 * h + (T) (signed char) *k++);                        \
-        return h % size;                                                      \
-    }                                                                         \
-    static T hash_##S(int f, const char *k, T size)                           \
-    {                                                                         \
-        switch (f) {                                                          \
-        case 0: return h0_##S(k, size);                                       \
-        case 1: return h1_##S(k, size);                                       \
-        case 2: return h2_##S(k, size);                                       \
-        case 3: return h3_##S(k, size);                                       \
-        default: return h4_##S(k, size);                                      \
-        }                                                                     \
-    }                                                                         \
-    /* 1 = seen before (read or its reverse complement), 0 = new (bits set) */\
-    static int seen_##S(unsigned char *a, T size, int nf, const char *s,      \
-                        const char *rs)                                       \
-    {                                                                         \
-        T hv[5];                                                              \
-        int miss = 0;                                                         \
-        for (int i = 0; i < nf; i++) {                                        \
-            hv[i] = hash_##S(i, s, size);                                     \
-            if (!(a[hv[i] / 8] & (1 << (hv[i] % 8))))                         \
-                miss = 1;                                                     \
-        }                                                                     \
-        if (!miss)                                                            \
-            return 1;                                                         \
-        for (int i = 0; i < nf; i++) {                                        \
-            T r = hash_##S(i, rs, size);                                      \
-            if (!(a[r / 8] & (1 << (r % 8)))) {                               \
-                for (int j = 0; j < nf; j++)                                  \
-                    a[hv[j] / 8] |= (unsigned char) (1 << (hv[j] % 8));       \
-                return 0;                                                     \
-            }                                                                 \
-        }                                                                     \
-        return 1;                                                             \
-    }
-
-DEF_HASHES(uint32_t, 32)
-DEF_HASHES(uint64_t, 64)
-
-int pq_import_fasta(const char *path, int read_len, int bloom_m, int n_hashfunc,
-                    int index_bits, pq_import_t *out)
+static double now_ms(void)
 {
-    struct timeval tv1, tv2;
-    gettimeofday(&tv1, NULL);
-    memset(out, 0, sizeof(*out));
-    if (read_len <= 0 || n_hashfunc < 1 || n_hashfunc > 5 ||
-        (index_bits != 32 && index_bits != 64))
-        return PQ_ERR_ARG;
-    FILE *fp = fopen(path, "r");
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    return tv.tv_sec * 1000.0 + tv.tv_usec / 1000.0;
+}
+
+/* The whole file in one buffer (size from fstat for a regular file, else read
+ * until end of file). */
+static int read_whole(const char *path, char **buf, uint64_t *size)
+{
+    FILE *fp = fopen(path, "rb");
     if (!fp)
         return PQ_ERR_IO;
-    char line[LINE_BUF];
-    uint64_t records = 0;
-    while (fgets(line, LINE_BUF, fp))
-        if (line[0] == '>')
-            records++;
-
-    /* filter size in the index width (index.c:206) */
-    uint64_t bsize = index_bits == 32 ? (uint64_t) (uint32_t) ((uint32_t) records * (uint32_t) bloom_m)
-                                      : records * (uint64_t) bloom_m;
-    unsigned char *bloom = calloc((size_t) ((bsize + 7) / 8) + 1, 1);
-    char *rline = malloc((size_t) read_len + 1);
+    struct stat st;
     uint64_t cap = 1 << 20;
-    uint8_t *reads = malloc(cap * (size_t) read_len);
-    if (!bloom || !rline || !reads) {
-        free(bloom);
-        free(rline);
-        free(reads);
-        fclose(fp);
-        return PQ_ERR_NOMEM;
-    }
-    uint64_t n = 0, removed = 0, cnt[4] = { 0, 0, 0, 0 };
-    int rc = PQ_OK;
-    fseek(fp, 0, SEEK_SET);
-    while (fgets(line, LINE_BUF, fp)) {
-        if (line[0] != '>') {
-            rc = PQ_ERR_FORMAT;
-            break;
-        }
-        if (!fgets(line, LINE_BUF, fp)) {
-            rc = PQ_ERR_FORMAT;
-            break;
-        }
-        size_t sl = strlen(line);
-        line[sl - 1] = '\0';      /* drops the newline (or the last code) */
-        if (strlen(line) != (size_t) read_len) {
-            rc = PQ_ERR_FORMAT;
-            break;
-        }
-        int with_n = 0;
-        for (int i = 0; i < read_len && rc == PQ_OK; i++) {
-            char c = line[i], v = 0, rv = 0;
-            switch (c) {
-            case 'a': case 'A': v = 1; rv = 4; break;
-            case 'c': case 'C': v = 2; rv = 3; break;
-            case 'g': case 'G': v = 3; rv = 2; break;
-            case 't': case 'T': v = 4; rv = 1; break;
-            case 'n': case 'N': with_n = 1; v = c; break;
-            default: rc = PQ_ERR_FORMAT;
-            }
-            line[i] = v;
-            rline[read_len - 1 - i] = rv;
-        }
-        if (rc != PQ_OK)
-            break;
-        rline[read_len] = '\0';
-        if (with_n) {
-            removed++;
-            continue;
-        }
-        int dup = index_bits == 32
-            ? seen_32(bloom, (uint32_t) bsize, n_hashfunc, line, rline)
-            : seen_64(bloom, bsize, n_hashfunc, line, rline);
-        if (dup) {
-            removed++;
-            continue;
-        }
-        out->prefix_count[line[0] - 1]++;
-        out->suffix_count[line[read_len - 1] - 1]++;
-        for (int i = 0; i < read_len; i++)
-            cnt[line[i] - 1]++;
-        if (n == cap) {
-            cap *= 2;
-            uint8_t *nr = realloc(reads, cap * (size_t) read_len);
-            if (!nr) {
+    if (fstat(fileno(fp), &st) == 0 && S_ISREG(st.st_mode))
+        cap = (uint64_t) st.st_size + 1;
+    char *b = malloc(cap);
+    uint64_t len = 0;
+    int rc = b ? PQ_OK : PQ_ERR_NOMEM;
+    while (rc == PQ_OK) {
+        if (len == cap) {
+            char *nb = realloc(b, cap * 2);
+            if (!nb) {
                 rc = PQ_ERR_NOMEM;
                 break;
             }
-            reads = nr;
+            b = nb;
+            cap *= 2;
         }
-        memcpy(reads + n * (uint64_t) read_len, line, (size_t) read_len);
-        n++;
+        size_t got = fread(b + len, 1, cap - len, fp);
+        len += got;
+        if (got == 0) {
+            if (ferror(fp))
+                rc = PQ_ERR_IO;
+            break;
+        }
     }
-    if (rc == PQ_ERR_FORMAT)
-        out->bad_line = (long) (n * 2 + 2);   /* as reported by index.c:223 */
     fclose(fp);
-    free(bloom);
-    free(rline);
     if (rc != PQ_OK) {
-        free(reads);
+        free(b);
         return rc;
     }
+    *buf = b;
+    *size = len;
+    return PQ_OK;
+}
+
+/* Start offsets of the pieces fgets would return, in file order; *np pieces,
+ * (*pieces)[*np] = size.  A line (up to and including its newline, or to the
+ * end of the file) of len bytes is ceil(len / 999) pieces. */
+static int cut_pieces(const char *b, uint64_t size, uint64_t **pieces, uint64_t *np)
+{
+    const int T = omp_get_max_threads();
+    uint64_t *cnt = calloc((size_t) T + 1, sizeof(uint64_t));
+    if (!cnt)
+        return PQ_ERR_NOMEM;
+    const uint64_t seg = (size + T - 1) / T;
+    /* pieces of the lines that start in each thread's byte range */
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        const uint64_t a = seg * t < size ? seg * t : size;
+        const uint64_t e = seg * (t + 1) < size ? seg * (t + 1) : size;
+        uint64_t c = 0;
+        for (uint64_t s = a; s < e;) {
+            if (s > 0 && b[s - 1] != '\n') {          /* not a line start: skip to one */
+                const char *nl = memchr(b + s, '\n', e - s);
+                if (!nl)
+                    break;
+                s = (uint64_t) (nl - b) + 1;
+                continue;
+            }
+            const char *nl = memchr(b + s, '\n', size - s);
+            const uint64_t end = nl ? (uint64_t) (nl - b) + 1 : size;
+            c += (end - s + PIECE_MAX - 1) / PIECE_MAX;
+            s = end;
+        }
+        cnt[t + 1] = c;
+    }
+    for (int t = 0; t < T; t++)
+        cnt[t + 1] += cnt[t];
+    const uint64_t n = cnt[T];
+    uint64_t *p = malloc(sizeof(uint64_t) * (n + 1));
+    if (!p) {
+        free(cnt);
+        return PQ_ERR_NOMEM;
+    }
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        const uint64_t a = seg * t < size ? seg * t : size;
+        const uint64_t e = seg * (t + 1) < size ? seg * (t + 1) : size;
+        uint64_t k = cnt[t];
+        for (uint64_t s = a; s < e;) {
+            if (s > 0 && b[s - 1] != '\n') {
+                const char *nl = memchr(b + s, '\n', e - s);
+                if (!nl)
+                    break;
+                s = (uint64_t) (nl - b) + 1;
+                continue;
+            }
+            const char *nl = memchr(b + s, '\n', size - s);
+            const uint64_t end = nl ? (uint64_t) (nl - b) + 1 : size;
+            for (uint64_t q = s; q < end; q += PIECE_MAX)
+                p[k++] = q;
+            s = end;
+        }
+    }
+    p[n] = size;
+    free(cnt);
+    *pieces = p;
+    *np = n;
+    return PQ_OK;
+}
+
+enum { REC_OK = 0, REC_N = 1, REC_BAD = 2 };
+
+/* Record r = pieces 2r (header) and 2r+1 (sequence): its codes and status,
+ * index.c:220-321. */
+static int parse_record(const char *b, const uint64_t *pc, uint64_t r, int L, uint8_t *codes)
+{
+    if (b[pc[2 * r]] != '>')
+        return REC_BAD;
+    const uint64_t s = pc[2 * r + 1], e = pc[2 * r + 2];
+    const char *z = memchr(b + s, '\0', e - s);     /* strlen stops at a NUL */
+    const uint64_t slen = z ? (uint64_t) (z - (b + s)) : e - s;
+    if (slen == 0 || slen - 1 != (uint64_t) L)
+        return REC_BAD;
+    int st = REC_OK;
+    for (int i = 0; i < L; i++) {
+        uint8_t v;
+        switch (b[s + i]) {
+        case 'a': case 'A': v = 1; break;
+        case 'c': case 'C': v = 2; break;
+        case 'g': case 'G': v = 3; break;
+        case 't': case 'T': v = 4; break;
+        case 'n': case 'N': v = 0; st = REC_N; break;
+        default: return REC_BAD;
+        }
+        codes[i] = v;
+    }
+    return st;
+}
+
+int pq_import_fasta(pq_ctx_t *ctx, const char *path, int read_len, int bloom_m, int n_hashfunc,
+                    int index_bits, pq_import_t *out)
+{
+    const double t0 = now_ms();
+    if (!out)
+        return PQ_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    if (!ctx || !path || read_len <= 0 || n_hashfunc < 1 || n_hashfunc > 5 ||
+        (index_bits != 32 && index_bits != 64))
+        return PQ_ERR_ARG;
+    const uint64_t L = (uint64_t) read_len;
+    char *b = NULL;
+    uint64_t size = 0, *pc = NULL, np = 0;
+    int rc = read_whole(path, &b, &size);
+    if (rc)
+        return rc;
+    if ((rc = cut_pieces(b, size, &pc, &np))) {
+        free(b);
+        return rc;
+    }
+    /* records for the filter size: pieces that start with '>' (index.c:195-206) */
+    uint64_t records = 0;
+#pragma omp parallel for reduction(+ : records)
+    for (long long i = 0; i < (long long) np; i++)
+        records += b[pc[i]] == '>';
+    const uint64_t bsize = index_bits == 32
+                               ? (uint64_t) (uint32_t) ((uint32_t) records * (uint32_t) bloom_m)
+                               : records * (uint64_t) bloom_m;
+
+    /* every record up to the first malformed one (the reference stops there;
+     * an odd last piece is a header without its sequence line) */
+    const uint64_t npairs = np / 2;
+    uint8_t *codes = malloc(npairs * L + 1), *ok = malloc(npairs + 1), *keep = malloc(npairs + 1);
+    if (!codes || !ok || !keep) {
+        rc = PQ_ERR_NOMEM;
+        goto done;
+    }
+    uint64_t first_bad = np & 1 ? npairs : UINT64_MAX;
+#pragma omp parallel for schedule(static, 4096) reduction(min : first_bad)
+    for (long long r = 0; r < (long long) npairs; r++) {
+        const int st = parse_record(b, pc, (uint64_t) r, read_len, codes + (uint64_t) r * L);
+        ok[r] = st == REC_OK;
+        if (st == REC_BAD && (uint64_t) r < first_bad)
+            first_bad = (uint64_t) r;
+    }
+    const uint64_t P = first_bad < npairs ? first_bad : npairs;
+    uint32_t passes = 0;
+    if ((rc = pq_dedup_reads(ctx, codes, ok, P, read_len, bsize, n_hashfunc, index_bits, keep,
+                             &passes)))
+        goto done;
+
+    /* kept reads in input order, gathered by thread ranges */
+    const int T = omp_get_max_threads();
+    uint64_t *kc = calloc((size_t) T + 1, sizeof(uint64_t));
+    uint64_t (*cnt)[14] = calloc((size_t) T, sizeof(*cnt));   /* nt[4] prefix[4] suffix[4] */
+    if (!kc || !cnt) {
+        free(kc);
+        free(cnt);
+        rc = PQ_ERR_NOMEM;
+        goto done;
+    }
+    const uint64_t seg = (P + T - 1) / T;
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        const uint64_t a = seg * t < P ? seg * t : P, e = seg * (t + 1) < P ? seg * (t + 1) : P;
+        uint64_t c = 0;
+        for (uint64_t r = a; r < e; r++)
+            c += keep[r];
+        kc[t + 1] = c;
+    }
+    for (int t = 0; t < T; t++)
+        kc[t + 1] += kc[t];
+    const uint64_t n = kc[T];
+    if (first_bad != UINT64_MAX) {
+        out->bad_line = (long) (n * 2 + 2);      /* index.c:223 */
+        rc = PQ_ERR_FORMAT;
+        free(kc);
+        free(cnt);
+        goto done;
+    }
+    uint8_t *reads = malloc(n * L + 1);
+    if (!reads) {
+        free(kc);
+        free(cnt);
+        rc = PQ_ERR_NOMEM;
+        goto done;
+    }
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        const uint64_t a = seg * t < P ? seg * t : P, e = seg * (t + 1) < P ? seg * (t + 1) : P;
+        uint64_t k = kc[t];
+        uint64_t *c = cnt[t];
+        for (uint64_t r = a; r < e; r++) {
+            if (!keep[r])
+                continue;
+            const uint8_t *s = codes + r * L;
+            memcpy(reads + k * L, s, L);
+            k++;
+            c[4 + s[0] - 1]++;
+            c[8 + s[L - 1] - 1]++;
+            for (uint64_t i = 0; i < L; i++)
+                c[s[i] - 1]++;
+        }
+    }
+    uint64_t nt[4] = { 0, 0, 0, 0 };
+    for (int t = 0; t < T; t++)
+        for (int c = 0; c < 4; c++) {
+            nt[c] += cnt[t][c];
+            out->prefix_count[c] += cnt[t][4 + c];
+            out->suffix_count[c] += cnt[t][8 + c];
+        }
+    free(kc);
+    free(cnt);
     out->reads = reads;
     out->n = n;
-    out->removed = removed;
+    out->removed = P - n;
     out->C[0] = 0;
     out->C[1] = 0;
-    out->C[2] = cnt[0];
-    out->C[3] = cnt[0] + cnt[1];
-    out->C[4] = cnt[0] + cnt[1] + cnt[2];
-    out->C[5] = cnt[0] + cnt[1] + cnt[2] + cnt[3];
-    gettimeofday(&tv2, NULL);
-    out->ms = (tv2.tv_sec - tv1.tv_sec) * 1000.0 + (tv2.tv_usec - tv1.tv_usec) / 1000.0;
-    return PQ_OK;
+    out->C[2] = nt[0];
+    out->C[3] = nt[0] + nt[1];
+    out->C[4] = nt[0] + nt[1] + nt[2];
+    out->C[5] = nt[0] + nt[1] + nt[2] + nt[3];
+done:
+    free(codes);
+    free(ok);
+    free(keep);
+    free(pc);
+    free(b);
+    out->ms = now_ms() - t0;
+    return rc;
 }

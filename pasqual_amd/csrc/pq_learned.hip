@@ -869,6 +869,17 @@ static int upload_inference_params(pq_ctx_t *ctx, const Learned &LM, DirModel &M
     return PQ_OK;
 }
 
+// Rows per table-build chunk when the caller leaves it to the library: what
+// half of the free device memory holds at 24 bytes per row (key, flag and
+// position), at least 2^24 rows.
+static uint64_t default_table_chunk(uint64_t N)
+{
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+        return 1ull << 28;
+    return std::max<uint64_t>(1ull << 24, std::min<uint64_t>(N, free_b / 2 / 24));
+}
+
 // Table of distinct full-length D-prefixes of direction d, from its SA, in
 // chunks of SA rows (the keys take 16 bytes per row): entries counted per
 // chunk, then written at the chunk's base (one pass when one chunk holds N).
@@ -877,7 +888,7 @@ static int build_table_t(pq_ctx_t *ctx, Learned &LM, int d, uint64_t chunk)
 {
     DirModel &M = LM.dir[d];
     const uint64_t N = ctx->N;
-    const uint64_t CH = std::min<uint64_t>(N, chunk ? chunk : (1ull << 28));
+    const uint64_t CH = std::min<uint64_t>(N, chunk ? chunk : default_table_chunk(N));
     const uint64_t nch = (N + CH - 1) / CH;
     u128 *K = (u128 *) ctx->tmp(SLOT_LEARNED + 3, (CH + 2) * sizeof(u128) + 16);
     uint32_t *flag = (uint32_t *) ctx->tmp(SLOT_LEARNED + 4, CH * 4 + 16);
@@ -917,8 +928,10 @@ static int build_table_t(pq_ctx_t *ctx, Learned &LM, int d, uint64_t chunk)
         return rc;
     RowCount<R> *rows = (RowCount<R> *) M.rows;
     for (uint64_t c = 0; c < nch; c++) {
-        // the last chunk's keys, flags and positions are still in place
-        if (c + 1 < nch || nch > 1) {
+        // in chunk order: a run's count is written where its last row lies, from
+        // the first row written where it began.  With one chunk its keys, flags
+        // and positions are still in place.
+        if (nch > 1) {
             uint32_t cnt = 0;
             if ((rc = chunk_keys(c, a, m, off)) || (rc = pq_scan_u32(ctx, flag, pos, m, &cnt)))
                 return rc;
