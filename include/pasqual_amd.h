@@ -144,8 +144,9 @@ typedef struct {
     uint32_t closed_form;    /* 1: solve the output layer by least squares before Adam */
     uint64_t seed;
     uint64_t table_chunk;    /* suffix-array rows per step of the table build (24 bytes of
-                                scratch per row), 0: as many as half the free device
-                                memory holds (one step when the whole array fits) */
+                                scratch per row), 0: the whole array in one step when its
+                                scratch and a table of one entry per row fit in 3/4 of the
+                                free device memory, else steps of 2^28 rows */
 } pq_learned_cfg_t;
 
 typedef struct {

@@ -869,15 +869,19 @@ static int upload_inference_params(pq_ctx_t *ctx, const Learned &LM, DirModel &M
     return PQ_OK;
 }
 
-// Rows per table-build chunk when the caller leaves it to the library: what
-// half of the free device memory holds at 24 bytes per row (key, flag and
-// position), at least 2^24 rows.
-static uint64_t default_table_chunk(uint64_t N)
+// Rows per table-build chunk when the caller leaves it to the library.  The
+// chunk's scratch (24 bytes per row: key, flag, position) is held while the
+// table itself is allocated (entry_bytes per entry, at most one entry per
+// row): all N rows in one chunk when both fit in three quarters of the free
+// device memory, else chunks of 2^28 rows (6 GiB of scratch).
+static uint64_t default_table_chunk(uint64_t N, uint64_t entry_bytes)
 {
+    const uint64_t fallback = std::min<uint64_t>(N, 1ull << 28);
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess)
-        return 1ull << 28;
-    return std::max<uint64_t>(1ull << 24, std::min<uint64_t>(N, free_b / 2 / 24));
+        return fallback;
+    const uint64_t budget = (uint64_t) free_b / 4 * 3;
+    return 24 * N + entry_bytes * (N + 1) <= budget ? N : fallback;
 }
 
 // Table of distinct full-length D-prefixes of direction d, from its SA, in
@@ -888,7 +892,8 @@ static int build_table_t(pq_ctx_t *ctx, Learned &LM, int d, uint64_t chunk)
 {
     DirModel &M = LM.dir[d];
     const uint64_t N = ctx->N;
-    const uint64_t CH = std::min<uint64_t>(N, chunk ? chunk : default_table_chunk(N));
+    const uint64_t CH = std::min<uint64_t>(
+        N, chunk ? chunk : default_table_chunk(N, sizeof(u128) + sizeof(RowCount<R>)));
     const uint64_t nch = (N + CH - 1) / CH;
     u128 *K = (u128 *) ctx->tmp(SLOT_LEARNED + 3, (CH + 2) * sizeof(u128) + 16);
     uint32_t *flag = (uint32_t *) ctx->tmp(SLOT_LEARNED + 4, CH * 4 + 16);
@@ -943,6 +948,18 @@ static int build_table_t(pq_ctx_t *ctx, Learned &LM, int d, uint64_t chunk)
         PQ_CHECK(hipGetLastError());
     }
     return PQ_OK;
+}
+
+// The table build's scratch (keys, flags, positions of one chunk: up to 24
+// bytes per suffix-array row) is freed once both tables exist.
+static void release_table_scratch(pq_ctx_t *ctx)
+{
+    (void) hipStreamSynchronize(ctx->stream);
+    for (int s = SLOT_LEARNED + 3; s <= SLOT_LEARNED + 5; s++)
+        if ((size_t) s < ctx->scratch.size() && ctx->scratch[s].p) {
+            pq_free(ctx->scratch[s].p);
+            ctx->scratch[s] = DevBuf();
+        }
 }
 
 static int build_table(pq_ctx_t *ctx, Learned &LM, int d, uint64_t chunk)
@@ -1141,6 +1158,7 @@ extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
         rc = ctx->fail(PQ_ERR_HIP, "clearing learned statistics");
     if (rc == PQ_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
         rc = ctx->fail(PQ_ERR_HIP, "learned index training");
+    release_table_scratch(ctx);
     if (rc) {
         pq_learned_free(ctx);
         return rc;
