@@ -120,6 +120,24 @@ int pq_graph_copy(pq_ctx_t *ctx, int32_t *overlaps_B, int32_t *overlaps_F,
  * into the node records. */
 int pq_graph_copy_edges(pq_ctx_t *ctx, void *edges, uint32_t index_bytes);
 
+/* Node record of one read as construct_string_graph sets it (string_graph.c:
+ * 832-889), with the reference's values (string_graph.h; PQ_REF_* of
+ * pasqual_refabi.h): B_type / F_type ONE 1, BRANCH 2, ENDING 4; status DEFAULT 0
+ * or REMOVED 8 (no edge on either side); marked SUSPECT 4 when
+ * |overlaps_B - overlaps_F| / max(overlaps_B, overlaps_F) > 0.6, else 0. */
+typedef struct {
+    uint16_t num_B, num_F;   /* irreducible edges per side (max_B/max_F start equal) */
+    int8_t B_type, F_type;
+    int8_t status, marked;
+} pq_node_t;
+
+/* Node records of the last search, computed on the device (8 bytes per read
+ * instead of the 24 of pq_graph_copy's per-read arrays), and the sum of the
+ * overlap lengths of all edges (average_overlap = sum / edges, string_graph.c:
+ * 1051).  PQ_ERR_LIMIT if a node has more than 65535 edges on one side
+ * (string_graph.c:469-472). */
+int pq_graph_nodes(pq_ctx_t *ctx, pq_node_t *nodes, uint64_t *overlap_len_sum);
+
 /* ---- learned index --------------------------------------------------------------------- */
 /* Counterpart of the reference's interval shortcut (sufsort.c:179-189, used at
  * string_graph.c:203-212, 289-298): every chain of the overlap search resumes at

@@ -38,7 +38,7 @@ PQ_SYMBOLS = [
     "pq_probe_copy", "pq_set_layout", "pq_get_layout", "pq_set_find_chains", "pq_synth_preset", "pq_synth_num_reads",
     "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
-    "pq_dedup_reads",
+    "pq_dedup_reads", "pq_graph_nodes",
 ]
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
                "time_importreads"]
@@ -168,6 +168,7 @@ def lib(width="u32"):
         "pq_synth_fasta": (I32, [P, C.c_char_p]),
         "pq_import_fasta": (I32, [P, C.c_char_p, I32, I32, I32, I32, P]),
         "pq_dedup_reads": (I32, [P, P, P, U64, I32, U64, I32, I32, P, P]),
+        "pq_graph_nodes": (I32, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -369,6 +370,19 @@ class Context:
                                           _ptr(eo), _ptr(dst), _ptr(ln), _ptr(ty)), "graph_copy")
         return {"overlaps_B": ovB[:n], "overlaps_F": ovF[:n], "num_B": nB[:n], "num_F": nF[:n],
                 "edge_off": eo, "dest": dst[:E], "len": ln[:E], "type": ty[:E]}
+
+    NODE_DTYPE = np.dtype([("num_B", "<u2"), ("num_F", "<u2"), ("B_type", "i1"),
+                           ("F_type", "i1"), ("status", "i1"), ("marked", "i1")])
+
+    def graph_nodes(self):
+        """Node records of the last search as construct_string_graph sets them
+        (pq_graph_nodes) and the sum of the edges' overlap lengths."""
+        n, E = C.c_uint64(), C.c_uint64()
+        self._check(self._L.pq_graph_sizes(self._h, C.byref(n), C.byref(E)), "graph_sizes")
+        rec = np.zeros(max(1, n.value), self.NODE_DTYPE)
+        total = C.c_uint64()
+        self._check(self._L.pq_graph_nodes(self._h, _ptr(rec), C.byref(total)), "graph_nodes")
+        return rec[:n.value], int(total.value)
 
     def autotune(self, mode=SEARCH_PLAIN):
         t = Tune()

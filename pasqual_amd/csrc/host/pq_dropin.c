@@ -24,6 +24,7 @@
  * arrays of index_t NULL; free(NULL) in the caller stays valid).
  */
 #include <malloc.h>
+#include <omp.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -251,24 +252,31 @@ string_graph_t *construct_string_graph(index_t *ix)
     const int timing = env_int("PQ_TIMING", 0);
     double tp = now_ms();
     int rc = pq_find_overlaps(g_ctx, mode);
-    if (timing)
-        fprintf(stderr, "[pq] search %.3f ms\n", now_ms() - tp), tp = now_ms();
+    if (timing) {
+        pq_timing_t st;
+        pq_get_timing(g_ctx, &st);
+        fprintf(stderr, "[pq] search %.3f ms\n", now_ms() - tp);
+        fprintf(stderr, "[pq] search find %.3f ms\n[pq] search extract %.3f ms\n"
+                "[pq] search compact %.3f ms\n", st.find_ms, st.extract_ms, st.compact_ms);
+        tp = now_ms();
+    }
     if (rc == PQ_ERR_LIMIT)
         REF_ERROR_EXIT("assembly failed. Try a larger min overlap length.\n");
     if (rc)
         dev_fail("overlap search", rc);
     uint64_t n = 0, E = 0;
     pq_graph_sizes(g_ctx, &n, &E);
-    int32_t *ovB = malloc(sizeof(int32_t) * (n + 1));
-    int32_t *ovF = malloc(sizeof(int32_t) * (n + 1));
-    uint32_t *nB = malloc(sizeof(uint32_t) * (n + 1));
-    uint32_t *nF = malloc(sizeof(uint32_t) * (n + 1));
-    uint64_t *eo = malloc(sizeof(uint64_t) * (n + 1));
+    pq_node_t *rec = malloc(sizeof(pq_node_t) * (n + 1));
     edge_t *all = malloc(sizeof(edge_t) * (E + 1));   /* every node's edges, CSR order */
     string_graph_t *g = malloc(sizeof(string_graph_t));
-    if (!ovB || !ovF || !nB || !nF || !eo || !all || !g)
+    if (!rec || !all || !g)
         dev_fail("graph buffers", PQ_ERR_NOMEM);
-    if ((rc = pq_graph_copy(g_ctx, ovB, ovF, nB, nF, eo, NULL, NULL, NULL)))
+    /* node records (types, marks, counts) are computed on the device */
+    uint64_t len_sum = 0;
+    rc = pq_graph_nodes(g_ctx, rec, &len_sum);
+    if (rc == PQ_ERR_LIMIT)
+        REF_ERROR_EXIT("assembly failed. Try a larger min overlap length.\n");
+    if (rc)
         dev_fail("graph copy", rc);
     if ((rc = pq_graph_copy_edges(g_ctx, all, (uint32_t) sizeof(INDEX_TYPE_T))))
         dev_fail("graph copy", rc);
@@ -280,56 +288,59 @@ string_graph_t *construct_string_graph(index_t *ix)
     g->num_reads = (INDEX_TYPE_T) n;
     g->read_len = ix->read_len;
     g->min_overlap = ix->min_overlap;
-    for (uint64_t i = 0; i < n; i++)
-        if (nB[i] > 65535 || nF[i] > 65535)
-            REF_ERROR_EXIT("assembly failed. Try a larger min overlap length.\n");
 
-    unsigned long long overlaps = 0;
-    double overlap_len = 0.0;
     const int RL = ix->read_len;
     /* node records, string_graph.c:832-1021; edges are malloc'ed per node because
      * assemble.c frees them one node at a time.  The heaps grow in steps of 64 MiB
      * meanwhile (glibc's default step is 128 KiB): millions of small blocks from
      * many threads otherwise spend most of the loop growing the heaps, which
-     * stalls the page faults of every other thread. */
+     * stalls the page faults of every other thread.  Each thread takes a
+     * contiguous range of nodes, so its edge offset is a running sum. */
     mallopt(M_TOP_PAD, 64 << 20);
-#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : overlaps, overlap_len)
-    for (long long ii = 0; ii < (long long) n; ii++) {
-        uint64_t i = (uint64_t) ii;
-        node_t *nd = &g->nodes[i];
-        unsigned nb = nB[i], nf = nF[i];
-        nd->max_B = nd->num_B = (unsigned short) nb;
-        nd->max_F = nd->num_F = (unsigned short) nf;
-        nd->status = PQ_REF_DEFAULT;
-        nd->path_q = NULL;
-        nd->edges = malloc((nb + nf) * sizeof(edge_t));
-        if (nb + nf)
-            memcpy(nd->edges, all + eo[i], (nb + nf) * sizeof(edge_t));
-        int a = ovB[i], b = ovF[i], mx = a > b ? a : b;
-        double ratio = mx == 0 ? 0 : ((double) abs(a - b)) / mx;
-        nd->marked = ratio > 0.6 ? PQ_REF_SUSPECT : 0;
-        if (nb != 0 || nf != 0) {
-            nd->B_type = nb == 1 ? PQ_REF_ONE : nb > 1 ? PQ_REF_BRANCH : PQ_REF_ENDING;
-            nd->F_type = nf == 1 ? PQ_REF_ONE : nf > 1 ? PQ_REF_BRANCH : PQ_REF_ENDING;
-        } else {
-            nd->B_type = nd->F_type = PQ_REF_ENDING;
-            nd->status = PQ_REF_REMOVED;
-        }
-        int with_paths = (nd->B_type == PQ_REF_BRANCH && nd->F_type != PQ_REF_ENDING) ||
-                         (nd->F_type == PQ_REF_BRANCH && nd->B_type != PQ_REF_ENDING);
-        path_q_t *pq = NULL;
-        if (with_paths) {
-            pq = nd->path_q = malloc(sizeof(path_q_t));
+    const int T = omp_get_max_threads();
+    uint64_t *tbase = calloc((size_t) T + 1, sizeof(uint64_t));
+    if (!tbase)
+        dev_fail("graph buffers", PQ_ERR_NOMEM);
+    const uint64_t seg = (n + T - 1) / T;
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        const uint64_t a = seg * t < n ? seg * t : n, e = seg * (t + 1) < n ? seg * (t + 1) : n;
+        uint64_t k = 0;
+        for (uint64_t i = a; i < e; i++)
+            k += (uint64_t) rec[i].num_B + rec[i].num_F;
+        tbase[t + 1] = k;
+#pragma omp barrier
+#pragma omp single
+        for (int q = 0; q < T; q++)
+            tbase[q + 1] += tbase[q];
+        uint64_t off = tbase[t];
+        for (uint64_t i = a; i < e; i++) {
+            node_t *nd = &g->nodes[i];
+            const pq_node_t r = rec[i];
+            const unsigned nb = r.num_B, nf = r.num_F;
+            nd->max_B = nd->num_B = (unsigned short) nb;
+            nd->max_F = nd->num_F = (unsigned short) nf;
+            nd->B_type = r.B_type;
+            nd->F_type = r.F_type;
+            nd->status = r.status;
+            nd->marked = r.marked;
+            nd->path_q = NULL;
+            nd->edges = malloc((nb + nf) * sizeof(edge_t));
+            if (nb + nf)
+                memcpy(nd->edges, all + off, (nb + nf) * sizeof(edge_t));
+            off += nb + nf;
+            int with_paths = (nd->B_type == PQ_REF_BRANCH && nd->F_type != PQ_REF_ENDING) ||
+                             (nd->F_type == PQ_REF_BRANCH && nd->B_type != PQ_REF_ENDING);
+            if (!with_paths)
+                continue;
+            path_q_t *pq = nd->path_q = malloc(sizeof(path_q_t));
             pq->max_B_endlen = pq->max_F_endlen = 0;
             pq->max_B_len = pq->max_F_len = 0;
             pq->max_F_trans = pq->max_B_trans = 0;   /* never read by assemble.c */
             pq->path = malloc(sizeof(path_t) * (nb + nf));
-        }
-        unsigned olen = 0;
-        for (unsigned j = 0; j < nb + nf; j++) {
-            const edge_t *ed = &nd->edges[j];
-            olen += ed->overlap_len;
-            if (pq) {
+            for (unsigned j = 0; j < nb + nf; j++) {
+                const edge_t *ed = &nd->edges[j];
                 path_t *p = &pq->path[j];
                 memset(p, 0, sizeof(*p));
                 p->end = p->hop = ed->dest;
@@ -340,18 +351,14 @@ string_graph_t *construct_string_graph(index_t *ix)
                 p->direction = j < nb ? (ed->type == PQ_REF_BB) : (ed->type == PQ_REF_FB);
             }
         }
-        overlaps += nb + nf;
-        overlap_len += (double) olen;
     }
     mallopt(M_TOP_PAD, 128 * 1024);
-    g->average_overlap = overlap_len / (double) overlaps;
+    /* string_graph.c:1051: the sum of edge lengths over the number of edges */
+    g->average_overlap = (double) len_sum / (double) E;
     if (timing)
         fprintf(stderr, "[pq] node build %.3f ms\n", now_ms() - tp);
-    free(ovB);
-    free(ovF);
-    free(nB);
-    free(nF);
-    free(eo);
+    free(tbase);
+    free(rec);
     free(all);
     printf("  takes %.3lf ms\n", now_ms() - t0);   /* string_graph.c:1043 */
     printf("  average overlap length: %lf\n", g->average_overlap);

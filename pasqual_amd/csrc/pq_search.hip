@@ -1235,3 +1235,81 @@ extern "C" int pq_graph_copy_edges(pq_ctx_t *ctx, void *edges, uint32_t index_by
     PQ_CHECK(hipStreamSynchronize(ctx->stream));
     return PQ_OK;
 }
+
+// Node records, string_graph.c:842-889: SUSPECT mark from the candidate counts
+// (ratio in double, as the reference computes it), B/F types from the edge
+// counts, REMOVED for a node without edges.  Flags a side of more than 65535
+// edges (the 16-bit counts of node_t).
+__global__ void k_node_records(const int32_t *ovB, const int32_t *ovF, const uint32_t *nB,
+                               const uint32_t *nF, uint64_t n, uint2 *out, uint32_t *over)
+{
+    PQ_GRID_STRIDE_FOR(i, n)
+    {
+        const uint32_t b = nB[i], f = nF[i];
+        if (b > 65535u || f > 65535u)
+            atomicOr(over, 1u);
+        const int32_t a = ovB[i], c = ovF[i], mx = a > c ? a : c;
+        const double ratio = mx == 0 ? 0.0 : (double) abs(a - c) / (double) mx;
+        const uint32_t marked = ratio > 0.6 ? 4u : 0u;
+        uint32_t bt = 4u, ft = 4u, status = 8u;          // ENDING, ENDING, REMOVED
+        if (b || f) {
+            bt = b == 1 ? 1u : b > 1 ? 2u : 4u;
+            ft = f == 1 ? 1u : f > 1 ? 2u : 4u;
+            status = 0u;
+        }
+        out[i] = make_uint2((b & 0xFFFFu) | (f << 16), bt | (ft << 8) | (status << 16) | (marked << 24));
+    }
+}
+
+// Sum of the overlap lengths of all edges (packed dest<<32 | len<<8 | type).
+__global__ void k_edge_len_sum(const uint64_t *pk, uint64_t E, unsigned long long *sum)
+{
+    uint64_t s = 0;
+    PQ_GRID_STRIDE_FOR(i, E)
+        s += (pk[i] >> 8) & 0xFFFFu;
+    for (int o = 32; o; o >>= 1)
+        s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63u) == 0 && s)
+        atomicAdd(sum, (unsigned long long) s);
+}
+
+__global__ void k_zero_words(unsigned long long *p, uint32_t count)
+{
+    if (threadIdx.x < count)
+        p[threadIdx.x] = 0ull;
+}
+
+extern "C" int pq_graph_nodes(pq_ctx_t *ctx, pq_node_t *nodes, uint64_t *overlap_len_sum)
+{
+    static_assert(sizeof(pq_node_t) == 8, "pq_node_t is 8 bytes");
+    if (!ctx || (!nodes && ctx->n))
+        return PQ_ERR_ARG;
+    if (!ctx->searched)
+        return ctx->fail(PQ_ERR_STATE, "no search result");
+    const uint64_t n = ctx->n, E = ctx->n_edges;
+    uint8_t *buf;
+    int rc = pq_slot(ctx, SLOT_NODES, n * 8 + 16, &buf);
+    if (rc)
+        return rc;
+    unsigned long long *words = (unsigned long long *) buf;   // [0] length sum, [1] over flag
+    uint2 *recs = (uint2 *) (buf + 16);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, ctx->stream, words, 2u);
+    if (n)
+        hipLaunchKernelGGL(k_node_records, dim3(pq_blocks_gs(n, 256)), dim3(256), 0, ctx->stream,
+                           ctx->d_ovB, ctx->d_ovF, ctx->d_numB, ctx->d_numF, n, recs,
+                           (uint32_t *) (words + 1));
+    if (E)
+        hipLaunchKernelGGL(k_edge_len_sum, dim3(std::min<unsigned>(pq_blocks_gs(E, 256), 2048u)),
+                           dim3(256), 0, ctx->stream, ctx->d_edges, E, words);
+    PQ_CHECK(hipGetLastError());
+    unsigned long long w[2];
+    PQ_CHECK(hipMemcpyAsync(w, words, 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (n)
+        PQ_CHECK(hipMemcpyAsync(nodes, recs, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    if (w[1])
+        return ctx->fail(PQ_ERR_LIMIT, "a node has more than 65535 edges on one side");
+    if (overlap_len_sum)
+        *overlap_len_sum = w[0];
+    return PQ_OK;
+}

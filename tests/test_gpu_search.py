@@ -38,6 +38,13 @@ def test_graph_matches_reference_fixture(ctx, name):
     nf = G["node_fields"]
     assert np.array_equal(g["num_B"], nf[:, 0]) and np.array_equal(g["num_F"], nf[:, 1])
     assert np.array_equal(g["edge_off"], G["edge_off"])
+    # node records computed on the device: fields num_B, num_F, B_type, F_type,
+    # status, marked of the reference's nodes, and its average overlap
+    rec, len_sum = ctx.graph_nodes()
+    for col, k in ((0, "num_B"), (1, "num_F"), (4, "B_type"), (5, "F_type"), (6, "status"),
+                   (7, "marked")):
+        assert np.array_equal(rec[k].astype(np.int64), nf[:, col].astype(np.int64)), k
+    assert len_sum / len(g["dest"]) == float(G["average_overlap"])
     E = np.stack([g["dest"], g["len"], g["type"]], 1).astype(np.uint64)
     assert np.array_equal(E, G["edges"])
     ex = orc.search_explicit(G["reads"], off, meta["t"])
