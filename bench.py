@@ -331,6 +331,15 @@ def main():
         pms = ctx.probe_copy(args.probe_copy << 20)
         log(f"copy probe: {args.probe_copy} MiB in {pms:.3f} ms "
             f"({2 * (args.probe_copy << 20) / pms / 1e6:.1f} GB/s read+write)")
+    # measured streaming ceilings of this GPU, reported beside the published HBM
+    # peak (untimed region, best of 3): a copy (1 GiB read + 1 GiB written) and a
+    # read of 1 GiB (the candidate search only reads)
+    copy_gbs = read_gbs = None
+    try:
+        copy_gbs = round(2 * (1 << 30) / min(ctx.probe_copy(1 << 30) for _ in range(3)) / 1e6, 1)
+        read_gbs = round((1 << 30) / min(ctx.probe_read(1 << 30) for _ in range(3)) / 1e6, 1)
+    except pq.PqError as e:
+        log(f"streaming ceilings not measured: {e}")
     row_bytes, slices = ctx.layout()
     lens = np.diff(off)
     read_len = int(lens[0]) if int(lens.min()) == int(lens.max()) else \
@@ -367,6 +376,9 @@ def main():
                 "lines_per_read": round(lines / (n * args.steps), 1),
                 "avg_launch_ms": round(avg_launch_ms, 3),
                 "launches_per_search": round(launches / args.steps, 2)}
+    roofline["copy_ceiling"] = copy_gbs
+    roofline["read_ceiling"] = read_gbs
+    roofline["frac_of_read_ceiling"] = round(achieved / read_gbs, 4) if read_gbs else None
 
     base = dropin = None
     if args.cpu_baseline == "full":

@@ -247,6 +247,9 @@ int pq_sync(pq_ctx_t *ctx);
  * streaming kernel (k_copy_probe), the known transfer against which hardware
  * traffic counters are scaled; *ms gets its duration. */
 int pq_probe_copy(pq_ctx_t *ctx, uint64_t bytes, double *ms);
+/* Measurement helper: one streaming read of `bytes` bytes (k_read_probe), the
+ * read-only ceiling of the device; *ms gets its duration. */
+int pq_probe_read(pq_ctx_t *ctx, uint64_t bytes, double *ms);
 
 /* ---- host utilities (no device use) --------------------------------------------------- */
 /* Seeded synthetic workloads (see pq_synth.c for the generator contract). */

@@ -38,7 +38,7 @@ PQ_SYMBOLS = [
     "pq_probe_copy", "pq_set_layout", "pq_get_layout", "pq_set_find_chains", "pq_synth_preset", "pq_synth_num_reads",
     "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
-    "pq_dedup_reads", "pq_graph_nodes",
+    "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read",
 ]
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
                "time_importreads"]
@@ -169,6 +169,7 @@ def lib(width="u32"):
         "pq_import_fasta": (I32, [P, C.c_char_p, I32, I32, I32, I32, P]),
         "pq_dedup_reads": (I32, [P, P, P, U64, I32, U64, I32, I32, P, P]),
         "pq_graph_nodes": (I32, [P, P, P]),
+        "pq_probe_read": (I32, [P, U64, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -456,4 +457,9 @@ class Context:
     def probe_copy(self, nbytes):
         ms = C.c_double()
         self._check(self._L.pq_probe_copy(self._h, nbytes, C.byref(ms)), "probe_copy")
+        return ms.value
+
+    def probe_read(self, nbytes):
+        ms = C.c_double()
+        self._check(self._L.pq_probe_read(self._h, nbytes, C.byref(ms)), "probe_read")
         return ms.value

@@ -398,8 +398,56 @@ extern "C" int pq_probe_copy(pq_ctx_t *ctx, uint64_t bytes, double *ms)
     PQ_CHECK(hipEventCreate(&a));
     PQ_CHECK(hipEventCreate(&b));
     PQ_CHECK(hipEventRecord(a, ctx->stream));
-    hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(256), 0, ctx->stream, (const uint4 *) buf,
+    // 1024 blocks of one 16-byte load in flight per lane: the fastest of grids
+    // 512-16384 with 1-8 loads in flight per lane on MI355X (1 GiB copies)
+    hipLaunchKernelGGL(k_copy_probe, dim3(1024), dim3(256), 0, ctx->stream, (const uint4 *) buf,
                        (uint4 *) (buf + n16 * 16), n16);
+    PQ_CHECK(hipEventRecord(b, ctx->stream));
+    PQ_CHECK(hipEventSynchronize(b));
+    float t = 0;
+    (void) hipEventElapsedTime(&t, a, b);
+    if (ms)
+        *ms = t;
+    (void) hipEventDestroy(a);
+    (void) hipEventDestroy(b);
+    return PQ_OK;
+}
+
+// Streaming read of n16 16-byte words.  The xor of what a lane read is
+// stored to the sink only when it equals one value, so the loads stay and the
+// kernel writes at most 4 bytes whatever the buffer holds.
+__global__ void k_read_probe(const uint4 *src, uint64_t n16, uint32_t *sink)
+{
+    uint32_t acc = 0;
+    for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (uint64_t) gridDim.x * blockDim.x) {
+        const uint4 v = src[i];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (acc == 0x9E3779B9u)
+        sink[0] = acc;
+}
+
+extern "C" int pq_probe_read(pq_ctx_t *ctx, uint64_t bytes, double *ms)
+{
+    if (!ctx || bytes < 16)
+        return PQ_ERR_ARG;
+    const uint64_t n16 = bytes / 16;
+    char *buf = (char *) ctx->tmp(SLOT_PROBE, n16 * 16 + 64);
+    if (!buf)
+        return ctx->fail(PQ_ERR_NOMEM, "read probe buffer");
+    // 4096 blocks: the fastest grid of streaming reads on MI355X (1 GiB).  One
+    // untimed pass first: lines a previous kernel left dirty in the caches would
+    // otherwise be written back during the timed one.
+    uint32_t *sink = (uint32_t *) (buf + n16 * 16);
+    hipLaunchKernelGGL(k_read_probe, dim3(4096), dim3(256), 0, ctx->stream, (const uint4 *) buf,
+                       n16, sink);
+    hipEvent_t a, b;
+    PQ_CHECK(hipEventCreate(&a));
+    PQ_CHECK(hipEventCreate(&b));
+    PQ_CHECK(hipEventRecord(a, ctx->stream));
+    hipLaunchKernelGGL(k_read_probe, dim3(4096), dim3(256), 0, ctx->stream, (const uint4 *) buf,
+                       n16, sink);
     PQ_CHECK(hipEventRecord(b, ctx->stream));
     PQ_CHECK(hipEventSynchronize(b));
     float t = 0;
