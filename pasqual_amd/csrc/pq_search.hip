@@ -865,9 +865,21 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     uint32_t kw = (cb.cap + 31) / 32;
     if (kw > 8)
         return ctx->fail(PQ_ERR_LIMIT, "extension keys longer than 256 codes");
-    // batch so the candidate buffer stays within ~8 GiB of the 288 GB
+    // batch so the candidate buffers take at most a third of the free device
+    // memory (at least 8 GiB): fewer, larger batches keep the GPU full to the
+    // end of each launch (chains of uneven reads finish at very different times)
     uint64_t per_read = 4ull * cb.cap * (sizeof(uint2) + sizeof(uint16_t)) + 16;
-    uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(n, (8ull << 30) / per_read));
+    uint64_t budget = 8ull << 30;
+    {
+        size_t free_b = 0, total_b = 0;
+        const size_t held = ctx->scratch.size() > SLOT_CAND_O
+                                ? ctx->scratch[SLOT_CAND_LC].bytes + ctx->scratch[SLOT_CAND_O].bytes
+                                : 0;
+        // buffers this search reuses count as free
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+            budget = std::max<uint64_t>(budget, ((uint64_t) free_b + held) / 3);
+    }
+    uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(n, budget / per_read));
     uint64_t *rec = nullptr, *recF = nullptr, *app = nullptr, *tot = nullptr, *staged = nullptr;
     uint32_t *ovl = nullptr, *misc = nullptr;
     uint64_t edge_cap = std::max<uint64_t>(1 << 20, n / 4);
