@@ -726,12 +726,15 @@ __global__ void k_edge_totals(const uint32_t *nb, const uint32_t *nf, uint64_t n
         tot[i] = 0;
 }
 
+// Nothing is copied when the search raised an error flag (edge buffer
+// exhausted: the counts then exceed what the buffers hold; candidate overflow):
+// the result is discarded, and the copy would read and write past the buffers.
 __global__ void k_csr(const uint64_t *recB, const uint64_t *recF, const uint32_t *numB,
                       const uint32_t *numF, const uint64_t *off, const uint64_t *staged,
-                      const uint64_t *pool, uint64_t n, uint64_t *dst)
+                      const uint64_t *pool, uint64_t n, const uint32_t *flags, uint64_t *dst)
 {
     uint64_t r = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n)
+    if (r >= n || (*flags & 3u))
         return;
     uint64_t a = off[r], nb = numB[r], nf = numF[r];
     const uint64_t *sb = nb > EDGE_SLOT ? pool + recB[r] : staged + (r * 2) * EDGE_SLOT;
@@ -806,7 +809,7 @@ static int run_search_graph(pq_ctx_t *ctx, int mode, uint64_t n, uint64_t batch,
         if (rc == PQ_OK)
             hipLaunchKernelGGL(k_csr, dim3(pq_blocks(n, 256)), dim3(256), 0, ctx->stream, rec,
                                recF, ctx->d_numB, ctx->d_numF, ctx->d_edge_off, staged, app, n,
-                               ctx->d_edges);
+                               misc + 3, ctx->d_edges);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         if (rc)
@@ -1023,52 +1026,69 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         cleanup();
         return rc;
     }
-    rc = enqueue(true);
-    uint32_t flags = 0;
-    unsigned long long used = 0;
-    if (rc == PQ_OK) {
-        uint32_t hm[16];
-        PQ_CHECK(hipMemcpyAsync(hm, misc, 64, hipMemcpyDeviceToHost, ctx->stream));
-        PQ_CHECK(hipStreamSynchronize(ctx->stream));
-        for (uint64_t b = 0; b < nbatch; b++) {
-            hipEvent_t *e = &ev[4 * b];
-            if (mode == PQ_SEARCH_LEARNED)
-                ctx->timing.learned_ms += ev_ms(e[0], e[1]);
-            double fms = ev_ms(e[1], e[2]);
-            ctx->timing.find_ms += fms;
-            ctx->timing.find_kernel_ms += fms;
-            ctx->timing.find_launches++;
-            ctx->timing.extract_ms += ev_ms(e[2], e[3]);
-        }
-        used = (unsigned long long) hm[0] | ((unsigned long long) hm[1] << 32);
-        flags = hm[3];
-        ctx->timing.find_positions = (uint64_t) hm[8] | ((uint64_t) hm[9] << 32);
-        ctx->timing.find_lines = (uint64_t) hm[10] | ((uint64_t) hm[11] << 32);
-        if (flags & 2u)
-            rc = ctx->fail(PQ_ERR_LIMIT, "a read has more than 65536 overlap candidates on one side");
-        else if (flags & 1u)
-            rc = ctx->fail(PQ_ERR_LIMIT, "edge buffer exhausted");
+    // the edge array at the bound the graph path uses, so that the whole search,
+    // edge compaction included, runs with one host wait at its end
+    const uint64_t ub = 2ull * EDGE_SLOT * n + edge_cap + 1;
+    if ((rc = pq_slot(ctx, SLOT_EDGES, ub * 8, &ctx->d_edges)) ||
+        (!ctx->tmp(SLOT_SCAN, pq_scan_scratch_bytes(n + 1, 8)) &&
+         (rc = ctx->fail(PQ_ERR_NOMEM, "scan scratch")))) {
+        cleanup();
+        return rc;
     }
-    (void) used;
+    hipEvent_t ce[2] = {nullptr, nullptr};
+    (void) hipEventCreate(&ce[0]);
+    (void) hipEventCreate(&ce[1]);
+    rc = enqueue(true);
     if (rc == PQ_OK) {
-        auto tc0 = std::chrono::steady_clock::now();
+        (void) hipEventRecord(ce[0], ctx->stream);
         hipLaunchKernelGGL(k_edge_totals, dim3(pq_blocks(n + 1, 256)), dim3(256), 0, ctx->stream,
                            ctx->d_numB, ctx->d_numF, n, tot);
+        rc = pq_scan_u64(ctx, tot, ctx->d_edge_off, n + 1, nullptr);
+    }
+    if (rc == PQ_OK) {
+        hipLaunchKernelGGL(k_csr, dim3(pq_blocks(n, 256)), dim3(256), 0, ctx->stream, rec, recF,
+                           ctx->d_numB, ctx->d_numF, ctx->d_edge_off, staged, app, n, misc + 3,
+                           ctx->d_edges);
+        (void) hipEventRecord(ce[1], ctx->stream);
+        if (hipGetLastError() != hipSuccess)
+            rc = ctx->fail(PQ_ERR_HIP, "edge compaction launch");
+    }
+    if (rc == PQ_OK) {
+        uint32_t hm[16];
         uint64_t total = 0;
-        rc = pq_scan_u64(ctx, tot, ctx->d_edge_off, n + 1, &total);
+        if (hipMemcpyAsync(hm, misc, 64, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(&total, ctx->d_edge_off + n, 8, hipMemcpyDeviceToHost, ctx->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = ctx->fail(PQ_ERR_HIP, "search result read-back");
         if (rc == PQ_OK) {
-            if ((rc = pq_slot(ctx, SLOT_EDGES, (total + 1) * 8, &ctx->d_edges)) == PQ_OK) {
-                hipLaunchKernelGGL(k_csr, dim3(pq_blocks(n, 256)), dim3(256), 0, ctx->stream, rec,
-                                   recF, ctx->d_numB, ctx->d_numF, ctx->d_edge_off, staged, app, n,
-                                   ctx->d_edges);
-                PQ_CHECK(hipStreamSynchronize(ctx->stream));
+            for (uint64_t b = 0; b < nbatch; b++) {
+                hipEvent_t *e = &ev[4 * b];
+                if (mode == PQ_SEARCH_LEARNED)
+                    ctx->timing.learned_ms += ev_ms(e[0], e[1]);
+                double fms = ev_ms(e[1], e[2]);
+                ctx->timing.find_ms += fms;
+                ctx->timing.find_kernel_ms += fms;
+                ctx->timing.find_launches++;
+                ctx->timing.extract_ms += ev_ms(e[2], e[3]);
+            }
+            ctx->timing.compact_ms = ev_ms(ce[0], ce[1]);
+            ctx->timing.find_positions = (uint64_t) hm[8] | ((uint64_t) hm[9] << 32);
+            ctx->timing.find_lines = (uint64_t) hm[10] | ((uint64_t) hm[11] << 32);
+            if (hm[3] & 2u)
+                rc = ctx->fail(PQ_ERR_LIMIT,
+                               "a read has more than 65536 overlap candidates on one side");
+            else if (hm[3] & 1u)
+                rc = ctx->fail(PQ_ERR_LIMIT, "edge buffer exhausted");
+            else {
                 ctx->n_edges = total;
                 ctx->searched = true;
             }
         }
-        ctx->timing.compact_ms =
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc0).count();
     }
+    for (auto &e : ce)
+        if (e)
+            (void) hipEventDestroy(e);
     cleanup();
     return rc;
 }
