@@ -226,6 +226,11 @@ int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape);
  * two chains of each index interleaved on one lane (2, the default); same
  * result.  pq_autotune times both. */
 int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains);
+/* Capacity of the search's shared edge buffer: edges of read sides with more
+ * than 16 irreducible edges (the first 16 of every side have slots of their
+ * own).  0 (the default): max(2^20, reads / 4).  A search that needs more
+ * fails with PQ_ERR_LIMIT "edge buffer exhausted" and leaves no result. */
+int pq_set_edge_capacity(pq_ctx_t *ctx, uint64_t edges);
 
 /* ---- timing ---------------------------------------------------------------------------- */
 typedef struct {

@@ -38,7 +38,7 @@ PQ_SYMBOLS = [
     "pq_probe_copy", "pq_set_layout", "pq_get_layout", "pq_set_find_chains", "pq_synth_preset", "pq_synth_num_reads",
     "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
-    "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read",
+    "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read", "pq_set_edge_capacity",
 ]
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
                "time_importreads"]
@@ -159,6 +159,7 @@ def lib(width="u32"):
         "pq_probe_copy": (I32, [P, U64, P]),
         "pq_set_layout": (I32, [P, U32, U64]),
         "pq_set_find_chains": (I32, [P, U32]),
+        "pq_set_edge_capacity": (I32, [P, U64]),
         "pq_get_layout": (I32, [P, P, P]),
         "pq_synth_preset": (I32, [I32, P]),
         "pq_synth_num_reads": (U64, [P]),
@@ -397,6 +398,9 @@ class Context:
 
     def set_find_chains(self, chains):
         self._check(self._L.pq_set_find_chains(self._h, chains), "set_find_chains")
+
+    def set_edge_capacity(self, edges):
+        self._check(self._L.pq_set_edge_capacity(self._h, int(edges)), "set_edge_capacity")
 
     def learned_train(self, **kw):
         cfg = LearnedCfg()

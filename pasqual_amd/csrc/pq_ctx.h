@@ -69,6 +69,7 @@ struct pq_ctx {
     Learned *learned = nullptr;
     uint32_t ext_shape = 1;        // first extraction tier: index into the tuned shapes
     uint32_t find_chains = 2;      // candidate-search chains per lane (1 or 2)
+    uint64_t edge_cap_set = 0;     // shared edge buffer capacity, 0 automatic
     uint64_t learned_gen = 0;      // bumped by every learned-index training
     // search captured as a graph, valid while its key (launch arguments) holds
     hipGraph_t search_graph = nullptr;

@@ -885,7 +885,8 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(n, budget / per_read));
     uint64_t *rec = nullptr, *recF = nullptr, *app = nullptr, *tot = nullptr, *staged = nullptr;
     uint32_t *ovl = nullptr, *misc = nullptr;
-    uint64_t edge_cap = std::max<uint64_t>(1 << 20, n / 4);
+    const uint64_t edge_cap =
+        ctx->edge_cap_set ? ctx->edge_cap_set : std::max<uint64_t>(1 << 20, n / 4);
     SeedT<R> *seeds = nullptr;
     char *gscratch = nullptr;
     // four events per batch (seed, find, extract boundaries), read after the loop
@@ -1221,6 +1222,14 @@ extern "C" int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains)
     if (chains != 1 && chains != 2)
         return ctx->fail(PQ_ERR_ARG, "chains per lane must be 1 or 2");
     ctx->find_chains = chains;
+    return PQ_OK;
+}
+
+extern "C" int pq_set_edge_capacity(pq_ctx_t *ctx, uint64_t edges)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    ctx->edge_cap_set = edges;
     return PQ_OK;
 }
 

@@ -4,6 +4,7 @@ explicit-string oracle for reads of unequal length."""
 import numpy as np
 import pytest
 
+import pasqual_amd as pq
 import pq_oracle as orc
 from conftest import golden_cases, load_golden
 from test_oracle import _random_reads
@@ -115,6 +116,27 @@ def _identical_copies(copies=3):
     off = np.zeros(len(reads) + 1, np.uint64)
     off[1:] = np.cumsum([len(r) for r in reads])
     return np.concatenate(reads), off, 6
+
+
+def test_edge_buffer_exhaustion_is_reported(ctx):
+    # read sides with more than 16 edges share one buffer; made too small, the
+    # search reports it (eager and graph mode) and the edge copy leaves every
+    # buffer alone; at its automatic size the graph is the oracle's again
+    codes, off, t = _identical_copies(40)
+    ctx.upload(codes, off, t)
+    ctx.build_index()
+    want = orc.search_explicit(codes, off, t)
+    assert max(want["num_B"].max(), want["num_F"].max()) > 16
+    ctx.set_edge_capacity(8)
+    try:
+        for mode in (pq.SEARCH_PLAIN, pq.SEARCH_PLAIN | pq.SEARCH_GRAPH):
+            with pytest.raises(pq.PqError, match="edge buffer exhausted"):
+                ctx.find_overlaps(mode)
+    finally:
+        ctx.set_edge_capacity(0)
+    for mode in (pq.SEARCH_PLAIN, pq.SEARCH_PLAIN | pq.SEARCH_GRAPH):
+        ctx.find_overlaps(mode)
+        _same_graph(ctx.graph(), want, f"edge capacity restored, mode {mode:#x}")
 
 
 def test_graph_edge_cases(ctx):
