@@ -94,8 +94,13 @@ int pq_get_counts(pq_ctx_t *ctx, uint64_t C[6], uint64_t suffix_count[4],
 int pq_get_occ_ref(pq_ctx_t *ctx, int dir, uint64_t *bitmap, void *sample,
                    int sample_bytes);
 /* Interval shortcut of every read for the reference's search entry
- * (index.h:31-37).  Depth-1 intervals: the SA interval of the last (fw) or
- * first (bw) code of the read. */
+ * (index.h:31-37): start, size and depth of the SA interval the reference's
+ * sort records for the read's last P codes (fw; first P codes reversed, bw):
+ * the deepest P in 9, 12, ... <= min(t, L) whose bucket the sort visits
+ * (sufsort.c:177-209, 322-334, 354-376, 594-611), else depth 1 (the interval of
+ * the last / first code, index.c:495-502, 548-555).  The deeper records need
+ * reads of one length (the reference's only input) and a text of fewer than
+ * 2^32 codes; otherwise every read gets the depth-1 interval. */
 int pq_get_shortcut(pq_ctx_t *ctx, int dir, uint64_t *start, uint64_t *size,
                     uint16_t *depth);
 uint64_t pq_num_reads(const pq_ctx_t *ctx);

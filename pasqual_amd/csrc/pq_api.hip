@@ -248,6 +248,164 @@ __global__ void k_shortcut(Geom G, const uint64_t *fwt, uint64_t n, int dir, CTa
     depth[r] = 1;
 }
 
+// ---- exact interval shortcuts (fixed read length L, N < 2^32) ----------------------------
+//
+// The reference's MSD sort records, for a suffix of length exactly P (the
+// read's last P codes of the direction text) with P = 9, 12, ... <= min(t, L),
+// the SA range of the bucket it lies in at depth P, when that bucket is one the
+// sort visits: a depth-9 bucket of more than one suffix goes to process_slot_1
+// (sufsort.c:594-611), and inside it a child bucket at depth Q + 3 of size f > 1
+// recurses into process_slot_1 when f * (L - Q) >= 3500 and is finished by
+// insert_sort otherwise (sufsort.c:354-376); both record suffixes of length
+// exactly their depth (sufsort.c:177-209, 322-334), the deepest record wins.
+// The bucket of row x at depth Q is the run of rows around x whose common
+// prefix with their predecessor is at least Q codes.
+
+// Common prefix (in codes, at most cap) of the suffixes of rows i-1 and i.
+template <typename R>
+__global__ void k_sc_lcp(const R *sa, const uint64_t *t2, uint64_t N, uint32_t L, uint32_t cap,
+                         uint16_t *lcp)
+{
+    PQ_GRID_STRIDE_FOR(i, N)
+    {
+        if (i == 0) {
+            lcp[0] = 0;
+            continue;
+        }
+        const uint64_t a = (uint64_t) sa[i - 1], b = (uint64_t) sa[i];
+        const uint32_t ra = L - (uint32_t) (a % L), rb = L - (uint32_t) (b % L);
+        uint32_t m = ra < rb ? ra : rb;
+        m = m < cap ? m : cap;
+        uint32_t k = 0;
+        while (k < m) {
+            const uint64_t x = pq_window(t2, a + k) ^ pq_window(t2, b + k);
+            const uint32_t same = x ? (uint32_t) __clzll(x) / 2 : 32u;
+            k += same;
+            if (same < 32)
+                break;
+        }
+        lcp[i] = (uint16_t) (k < m ? k : m);
+    }
+}
+
+// Row of every (read, P) suffix: rowP[r * nP + (P - 9) / 3].
+template <typename R>
+__global__ void k_sc_rows(const R *sa, uint64_t N, uint32_t L, uint32_t Pmax, uint32_t nP,
+                          uint32_t *rowP)
+{
+    PQ_GRID_STRIDE_FOR(i, N)
+    {
+        const uint64_t pos = (uint64_t) sa[i];
+        const uint32_t P = L - (uint32_t) (pos % L);
+        if (P >= 9 && P <= Pmax && (P - 9) % 3 == 0)
+            rowP[(pos / L) * nP + (P - 9) / 3] = (uint32_t) i;
+    }
+}
+
+// Run boundaries at depth Q.
+__global__ void k_sc_flags(const uint16_t *lcp, uint64_t N, uint32_t Q, uint32_t *flag)
+{
+    PQ_GRID_STRIDE_FOR(i, N)
+        flag[i] = (i == 0 || lcp[i] < Q) ? 1u : 0u;
+}
+
+// start[k] = first row of run k (runs numbered by the exclusive scan of the
+// flags); start[runs] = N.
+__global__ void k_sc_starts(const uint32_t *flag, const uint32_t *excl, uint64_t N, uint32_t runs,
+                            uint32_t *start)
+{
+    PQ_GRID_STRIDE_FOR(i, N)
+    {
+        if (flag[i])
+            start[excl[i]] = (uint32_t) i;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        start[runs] = (uint32_t) N;
+}
+
+// Depth Q for every (read, P >= Q): the visit state of its bucket chain
+// (vis[r * nP + p]) and, at Q == P, the record.
+__global__ void k_sc_level(const uint32_t *rowP, const uint32_t *flag, const uint32_t *excl,
+                           const uint32_t *start, uint64_t n, uint32_t nP, uint32_t Q, uint32_t L,
+                           uint8_t *vis, uint64_t *istart, uint64_t *isize, uint16_t *idepth)
+{
+    PQ_GRID_STRIDE_FOR(q, n * nP)
+    {
+        const uint32_t p = (uint32_t) (q % nP), P = 9 + 3 * p;
+        if (P < Q)
+            continue;
+        const uint32_t x = rowP[q];
+        const uint32_t run = excl[x] + flag[x] - 1u;
+        const uint32_t rs = start[run], size = start[run + 1] - rs;
+        bool v;
+        if (Q == 9) {
+            v = size > 1;
+            vis[q] = v ? 1 : 0;
+        } else {
+            v = vis[q] && (Q - 3) < L && size > 1;
+            vis[q] = (v && (uint64_t) size * (L - (Q - 3)) >= 3500u) ? 1 : 0;
+        }
+        if (Q == P && v) {            // one writer per read per level; deeper levels later
+            const uint64_t r = q / nP;
+            istart[r] = rs;
+            isize[r] = size;
+            idepth[r] = (uint16_t) P;
+        }
+    }
+}
+
+template <typename R>
+static int exact_shortcut(pq_ctx_t *ctx, int dir, uint64_t *ds, uint64_t *dz, uint16_t *dd)
+{
+    const uint64_t n = ctx->n, N = ctx->N;
+    const uint32_t L = ctx->L, t = ctx->t;
+    const uint32_t Pmax = t < L ? t : L;
+    if (Pmax < 9)
+        return PQ_OK;
+    const uint32_t nP = (Pmax - 9) / 3 + 1;
+    uint16_t *lcp = nullptr;
+    uint32_t *rowP = nullptr, *flag = nullptr, *excl = nullptr, *start = nullptr;
+    uint8_t *vis = nullptr;
+    int rc;
+    if ((rc = pq_alloc(ctx, (void **) &lcp, N * 2)) ||
+        (rc = pq_alloc(ctx, (void **) &rowP, n * nP * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &flag, N * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &excl, N * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &start, (N + 1) * 4)) ||
+        (rc = pq_alloc(ctx, (void **) &vis, n * nP))) {
+        pq_free(lcp), pq_free(rowP), pq_free(flag), pq_free(excl), pq_free(start), pq_free(vis);
+        return rc;
+    }
+    auto run = [&]() -> int {
+        const R *sa = (const R *) ctx->dir[dir].sa;
+        const uint64_t *t2 = ctx->dir[dir].text2;
+        hipLaunchKernelGGL(k_sc_lcp<R>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0, ctx->stream, sa,
+                           t2, N, L, Pmax, lcp);
+        hipLaunchKernelGGL(k_sc_rows<R>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0, ctx->stream,
+                           sa, N, L, Pmax, nP, rowP);
+        PQ_CHECK(hipGetLastError());
+        for (uint32_t Q = 9; Q <= Pmax; Q += 3) {
+            hipLaunchKernelGGL(k_sc_flags, dim3(pq_blocks_gs(N, 256)), dim3(256), 0, ctx->stream,
+                               lcp, N, Q, flag);
+            uint32_t runs = 0;
+            int rc2 = pq_scan_u32(ctx, flag, excl, N, &runs);
+            if (rc2)
+                return rc2;
+            hipLaunchKernelGGL(k_sc_starts, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
+                               ctx->stream, flag, excl, N, runs, start);
+            hipLaunchKernelGGL(k_sc_level, dim3(pq_blocks_gs(n * nP, 256)), dim3(256), 0,
+                               ctx->stream, rowP, flag, excl, start, n, nP, Q, L, vis, ds, dz,
+                               dd);
+            PQ_CHECK(hipGetLastError());
+        }
+        PQ_CHECK(hipStreamSynchronize(ctx->stream));
+        return PQ_OK;
+    };
+    rc = run();
+    pq_free(lcp), pq_free(rowP), pq_free(flag), pq_free(excl), pq_free(start), pq_free(vis);
+    return rc;
+}
+
 static int dir_ok(pq_ctx_t *ctx, int dir)
 {
     if (!ctx)
@@ -366,6 +524,14 @@ extern "C" int pq_get_shortcut(pq_ctx_t *ctx, int dir, uint64_t *start, uint64_t
     hipLaunchKernelGGL(k_shortcut, dim3(pq_blocks(n, 256)), dim3(256), 0, ctx->stream,
                        ctx->geom(), ctx->dir[0].text2, n, dir, ctx->ctab<uint64_t>(), ds, dz, dd);
     PQ_CHECK(hipGetLastError());
+    // deeper records where the reference's sort makes them: reads of one length
+    // (the reference's only input) and texts of fewer than 2^32 codes
+    if (ctx->L && ctx->N < 0xFFFFFFFFull) {
+        rc = ctx->wide ? exact_shortcut<uint64_t>(ctx, dir, ds, dz, dd)
+                       : exact_shortcut<uint32_t>(ctx, dir, ds, dz, dd);
+        if (rc)
+            return rc;
+    }
     if (start)
         PQ_CHECK(hipMemcpyAsync(start, ds, n * 8, hipMemcpyDeviceToHost, ctx->stream));
     if (size)

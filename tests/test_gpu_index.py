@@ -24,6 +24,14 @@ def _check_against_oracle(ctx, codes, off, t):
         assert np.array_equal(ctx.reads_index(d), ix[d]["reads_index"]), ("reads_index", d)
         bm, sm = ctx.occ_ref(d, 8)
         assert np.array_equal(bm, ix[d]["bitmap"]) and np.array_equal(sm, ix[d]["sample"])
+        lens = np.diff(off.astype(np.int64))
+        if len(lens) and lens.min() == lens.max():
+            # interval shortcuts at the depths the reference's sort records them
+            L = int(lens[0])
+            want = orc.shortcut(ix[d]["text"], len(lens), L, t, ix[d]["sa"], ix["C"])
+            got = ctx.shortcut(d)
+            for k, (a, b) in enumerate(zip(got, want)):
+                assert np.array_equal(a.astype(np.uint64), b.astype(np.uint64)), ("shortcut", d, k)
     return ix
 
 
@@ -43,11 +51,10 @@ def test_index_matches_reference_fixture(ctx, name):
         assert np.array_equal(bm, G[f"bitmap{d}"]), d
         assert np.array_equal(sm.astype(np.uint64), G[f"sample{d}"]), d
         assert np.array_equal(ctx.reads_index(d), G[f"reads_index{d}"]), d
-        # depth-1 shortcut: the C interval of the read's last (fw) / first (bw) code
+        # interval shortcuts: the reference's own start / size / depth per read
         st, sz, dp = ctx.shortcut(d)
-        last = G["reads"][np.arange(n) * meta["L"] + (meta["L"] - 1 if d == 0 else 0)]
-        assert np.array_equal(st, G["C"][last]) and np.all(dp == 1)
-        assert np.array_equal(sz, G["C"][last + 1] - G["C"][last])
+        assert np.array_equal(st, G[f"istart{d}"]) and np.array_equal(sz, G[f"iend{d}"])
+        assert np.array_equal(dp.astype(np.uint64), G[f"idepth{d}"]), d
 
 
 @pytest.mark.parametrize("kind,seed", [("fixed20", 1), ("fixed50", 2), ("fixed101", 3),
