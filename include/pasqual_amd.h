@@ -103,6 +103,13 @@ int pq_get_occ_ref(pq_ctx_t *ctx, int dir, uint64_t *bitmap, void *sample,
  * 2^32 codes; otherwise every read gets the depth-1 interval. */
 int pq_get_shortcut(pq_ctx_t *ctx, int dir, uint64_t *start, uint64_t *size,
                     uint16_t *depth);
+/* First-triplet slot sizes of index_t (size_fw / offset_fw / max_size_fw and the
+ * bw ones, index.h:52-57), as initial_sort computes them (sufsort.c:386-450):
+ * size[k] = text positions whose code of the (up to) three codes from there to
+ * the end of the read, 1 + 21 c0 + [rem >= 2] (1 + 5 c1 + [rem >= 3] (1 + c2)),
+ * lies in 17k .. 17k+16; offset = exclusive prefix; *max_size = largest size. */
+int pq_get_slot_sizes(pq_ctx_t *ctx, int dir, uint64_t size[5], uint64_t offset[5],
+                      uint64_t *max_size);
 uint64_t pq_num_reads(const pq_ctx_t *ctx);
 uint64_t pq_text_len(const pq_ctx_t *ctx);
 
@@ -303,6 +310,7 @@ typedef struct {
     uint64_t C[6], suffix_count[4], prefix_count[4];
     double ms;
     long bad_line;           /* line number reported on PQ_ERR_FORMAT */
+    uint64_t bloom_bits;     /* filter size: records * bloom_m in the index width */
 } pq_import_t;
 int pq_import_fasta(pq_ctx_t *ctx, const char *path, int read_len, int bloom_m,
                     int n_hashfunc, int index_bits, pq_import_t *out);

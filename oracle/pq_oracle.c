@@ -941,6 +941,34 @@ double orc_mlp_loss_grad(uint32_t k, uint32_t h, const double *params, const uin
     return loss / (double) n;
 }
 
+/* ---- first-triplet slot sizes (sufsort.c:386-450) ------------------------------ */
+
+void orc_slot_sizes(const uint8_t *dtext, const uint64_t *off, uint64_t n, uint64_t size[5],
+                    uint64_t offset[5], uint64_t *max_size)
+{
+    uint64_t freq[85] = { 0 };
+    for (uint64_t r = 0; r < n; r++)
+        for (uint64_t g = off[r]; g < off[r + 1]; g++) {
+            const uint64_t rem = off[r + 1] - g;
+            unsigned code = 1 + 21 * (dtext[g] - 1);
+            if (rem >= 2) {
+                code += 1 + 5 * (dtext[g + 1] - 1);
+                if (rem >= 3)
+                    code += 1 + (dtext[g + 2] - 1);
+            }
+            freq[code]++;
+        }
+    *max_size = 0;
+    for (int k = 0; k < 5; k++) {
+        size[k] = 0;
+        for (int j = 0; j < 17; j++)
+            size[k] += freq[17 * k + j];
+        offset[k] = k ? offset[k - 1] + size[k - 1] : 0;
+        if (size[k] > *max_size)
+            *max_size = size[k];
+    }
+}
+
 /* ---- reads import with duplicate removal (index.c:145-405, bloom.c) ---------- */
 
 /* The hashes of bloom.c:18-86 over a NUL-terminated code string, in the index

@@ -121,6 +121,15 @@ int64_t orc_check_sa(const uint8_t *dtext, const uint64_t *off, uint64_t n,
 double orc_mlp_loss_grad(uint32_t k, uint32_t h, const double *params, const uint64_t *xs,
                          const double *targets, uint64_t n, double *grad);
 
+/* First-triplet slot sizes of a direction text (initial_sort, sufsort.c:
+ * 386-450): the triplet code of position g is 1 + 21 c0 + [rem >= 2] (1 + 5 c1 +
+ * [rem >= 3] (1 + c2)) for the codes c0, c1, c2 (0..3) at g, g+1, g+2 and rem the
+ * codes from g to the end of its read (sufsort.c:21-40, 124-151); slot k holds
+ * codes 17k .. 17k+16.  size[k] = positions in slot k, offset = exclusive
+ * prefix of size, *max_size = the largest size. */
+void orc_slot_sizes(const uint8_t *dtext, const uint64_t *off, uint64_t n, uint64_t size[5],
+                    uint64_t offset[5], uint64_t *max_size);
+
 /* Reads import with duplicate removal, sequential, as import_singlereads
  * does it (index.c:145-405): a first pass counts the fgets(line, 1000) pieces
  * starting with '>' (index.c:195-206), the filter has records * bloom_m bits

@@ -60,6 +60,7 @@ def lib():
                                           C.POINTER(C.c_void_p), C.POINTER(C.c_uint64),
                                           C.POINTER(C.c_uint64), C.POINTER(C.c_long)]
         _lib.orc_dedup.argtypes = [P, P, C.c_uint64, C.c_int, C.c_uint64, C.c_int, C.c_int, P]
+        _lib.orc_slot_sizes.argtypes = [P, P, C.c_uint64, P, P, P]
     return _lib
 
 
@@ -110,6 +111,17 @@ def import_fasta(path, L, bloom_m=36, n_hashfunc=2, index_bits=32):
     Cc, sfx, pfx = counts(reads, off) if k else (np.zeros(6, np.uint64),) + (np.zeros(4, np.uint64),) * 2
     return {"rc": 0, "reads": reads, "n": k, "removed": int(rem.value), "C": Cc.tolist(),
             "suffix_count": sfx.tolist(), "prefix_count": pfx.tolist()}
+
+
+def slot_sizes(dtext, off):
+    """(size[5], offset[5], max_size) of a direction text (orc_slot_sizes)."""
+    size = np.zeros(5, np.uint64)
+    offs = np.zeros(5, np.uint64)
+    mx = np.zeros(1, np.uint64)
+    lib().orc_slot_sizes(_p(np.ascontiguousarray(dtext, np.uint8)),
+                         _p(np.ascontiguousarray(off, np.uint64)), len(off) - 1, _p(size),
+                         _p(offs), _p(mx))
+    return size.tolist(), offs.tolist(), int(mx[0])
 
 
 def dedup(codes, ok, L, bloom_bits, n_hashfunc=2, index_bits=32):

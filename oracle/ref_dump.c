@@ -7,6 +7,8 @@
  *
  *   ref_dump <reads.fa> <read_len> <min_overlap> <out.bin> [nograph] [bloom=M] [nf=K]
  *
+ * Prints the slot sizes of index_t (size/offset/max_size per direction) on a
+ * line "slots fw <5 sizes> <5 offsets> <max> bw <5 sizes> <5 offsets> <max>".
  * Calls construct_index (index.h:64-68) with the CLI defaults bloom_m=36,
  * n_hashfunc=2 (test.c:136-148) unless bloom=/nf= say otherwise, and
  * construct_string_graph (string_graph.h:139).
@@ -59,6 +61,18 @@ int main(int argc, char **argv)
     }
     index_t *ix = construct_index(argv[1], NULL, NULL, NULL, NULL, t, L, bloom_m, nf);
     size_t n = ix->num_reads, N = n * (size_t) L, rows = (N + 64) / 64;
+    /* the first-triplet slot sizes of initial_sort (sufsort.c:386-450), one line */
+    printf("slots fw");
+    for (int k = 0; k < 5; k++)
+        printf(" %llu", (unsigned long long) ix->size_fw[k]);
+    for (int k = 0; k < 5; k++)
+        printf(" %llu", (unsigned long long) ix->offset_fw[k]);
+    printf(" %llu bw", (unsigned long long) ix->max_size_fw);
+    for (int k = 0; k < 5; k++)
+        printf(" %llu", (unsigned long long) ix->size_bw[k]);
+    for (int k = 0; k < 5; k++)
+        printf(" %llu", (unsigned long long) ix->offset_bw[k]);
+    printf(" %llu\n", (unsigned long long) ix->max_size_bw);
 
     fo = fopen(argv[4], "wb");
     if (!fo) {

@@ -39,6 +39,7 @@ PQ_SYMBOLS = [
     "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
     "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read", "pq_set_edge_capacity",
+    "pq_get_slot_sizes",
 ]
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
                "time_importreads"]
@@ -70,7 +71,7 @@ class ImportResult(C.Structure):
     _fields_ = [
         ("reads", C.POINTER(C.c_uint8)), ("n", C.c_uint64), ("removed", C.c_uint64),
         ("C", C.c_uint64 * 6), ("suffix_count", C.c_uint64 * 4), ("prefix_count", C.c_uint64 * 4),
-        ("ms", C.c_double), ("bad_line", C.c_long),
+        ("ms", C.c_double), ("bad_line", C.c_long), ("bloom_bits", C.c_uint64),
     ]
 
 
@@ -160,6 +161,7 @@ def lib(width="u32"):
         "pq_set_layout": (I32, [P, U32, U64]),
         "pq_set_find_chains": (I32, [P, U32]),
         "pq_set_edge_capacity": (I32, [P, U64]),
+        "pq_get_slot_sizes": (I32, [P, I32, P, P, P]),
         "pq_get_layout": (I32, [P, P, P]),
         "pq_synth_preset": (I32, [I32, P]),
         "pq_synth_num_reads": (U64, [P]),
@@ -233,7 +235,7 @@ def import_fasta(path, read_len, bloom_m=36, n_hashfunc=2, index_bits=32, ctx=No
     C.CDLL(None).free(C.cast(res.reads, C.c_void_p))
     return {"reads": reads, "n": n, "removed": int(res.removed), "C": list(res.C),
             "suffix_count": list(res.suffix_count), "prefix_count": list(res.prefix_count),
-            "ms": res.ms}
+            "ms": res.ms, "bloom_bits": int(res.bloom_bits)}
 
 
 # ---- device context ------------------------------------------------------------------
@@ -352,6 +354,15 @@ class Context:
         self._check(self._L.pq_get_shortcut(self._h, d, _ptr(st), _ptr(sz), _ptr(dp)),
                     "get_shortcut")
         return st[:n], sz[:n], dp[:n]
+
+    def slot_sizes(self, d):
+        """index_t's first-triplet slot sizes of direction d: (size[5], offset[5], max)."""
+        size = np.zeros(5, np.uint64)
+        offs = np.zeros(5, np.uint64)
+        mx = C.c_uint64()
+        self._check(self._L.pq_get_slot_sizes(self._h, d, _ptr(size), _ptr(offs), C.byref(mx)),
+                    "get_slot_sizes")
+        return size.tolist(), offs.tolist(), int(mx.value)
 
     def find_overlaps(self, mode=SEARCH_PLAIN):
         self._check(self._L.pq_find_overlaps(self._h, mode), "find_overlaps")

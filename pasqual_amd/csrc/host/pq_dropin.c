@@ -176,6 +176,7 @@ index_t *construct_index(char *single_file, char *pair_file1, char *pair_file2,
             dev_fail("reads import", rc);
         ix->reads = (char *) imp.reads;
         ix->num_reads = (INDEX_TYPE_T) imp.n;
+        ix->bloom_size = (INDEX_TYPE_T) imp.bloom_bits;   /* index.c:205 */
         for (int c = 0; c < 6; c++)
             ix->C[c] = (INDEX_TYPE_T) imp.C[c];
         for (int c = 0; c < 4; c++) {
@@ -201,6 +202,22 @@ index_t *construct_index(char *single_file, char *pair_file1, char *pair_file2,
     printf("Sorting forward suffixes ... done\n  takes %.3lf ms\n", tm.sa_ms[0] + tm.occ_ms[0]);
     printf("Sorting backward suffixes ... done\n  takes %.3lf ms\n", tm.sa_ms[1] + tm.occ_ms[1]);
     fflush(stdout);
+    /* first-triplet slot sizes of initial_sort (sufsort.c:386-450) */
+    for (int d = 0; d < 2; d++) {
+        uint64_t sz[5], of[5], mx = 0;
+        int rc = pq_get_slot_sizes(g_ctx, d, sz, of, &mx);
+        if (rc)
+            dev_fail("slot sizes", rc);
+        INDEX_TYPE_T *S = d ? ix->size_bw : ix->size_fw, *O = d ? ix->offset_bw : ix->offset_fw;
+        for (int k = 0; k < 5; k++) {
+            S[k] = (INDEX_TYPE_T) sz[k];
+            O[k] = (INDEX_TYPE_T) of[k];
+        }
+        if (d)
+            ix->max_size_bw = (INDEX_TYPE_T) mx;
+        else
+            ix->max_size_fw = (INDEX_TYPE_T) mx;
+    }
     if (env_int("PQ_HOST_INDEX", 1))
         export_host_index(ix);
     return ix;

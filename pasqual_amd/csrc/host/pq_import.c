@@ -200,6 +200,7 @@ int pq_import_fasta(pq_ctx_t *ctx, const char *path, int read_len, int bloom_m, 
     const uint64_t bsize = index_bits == 32
                                ? (uint64_t) (uint32_t) ((uint32_t) records * (uint32_t) bloom_m)
                                : records * (uint64_t) bloom_m;
+    out->bloom_bits = bsize;
 
     /* every record up to the first malformed one (the reference stops there;
      * an odd last piece is a header without its sequence line) */

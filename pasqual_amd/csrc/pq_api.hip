@@ -406,6 +406,79 @@ static int exact_shortcut(pq_ctx_t *ctx, int dir, uint64_t *ds, uint64_t *dz, ui
     return rc;
 }
 
+// ---- first-triplet slot sizes (initial_sort, sufsort.c:386-450) ------------------------
+//
+// The reference codes position g by the (up to) three codes from g to the end of
+// its read, 1 + 21 c0 + [rem >= 2] (1 + 5 c1 + [rem >= 3] (1 + c2)) (sufsort.c:
+// 21-40, 124-151), and sizes its five sort slots by the codes 17k .. 17k+16.
+__global__ void k_slot_hist(Geom G, const uint64_t *t2, unsigned long long *cnt)
+{
+    __shared__ unsigned long long h[5];
+    if (threadIdx.x < 5)
+        h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t k0 = 0, k1 = 0, k2 = 0, k3 = 0, k4 = 0;
+    PQ_GRID_STRIDE_FOR(g, G.N)
+    {
+        const uint64_t rem = pq_rem(G, g);
+        const uint64_t w = pq_window(t2, g);
+        const uint32_t c0 = (uint32_t) (w >> 62), c1 = (uint32_t) (w >> 60) & 3u,
+                       c2 = (uint32_t) (w >> 58) & 3u;
+        const uint32_t code =
+            1u + 21u * c0 + (rem >= 2 ? 1u + 5u * c1 + (rem >= 3 ? 1u + c2 : 0u) : 0u);
+        const uint32_t k = code / 17u;
+        k0 += k == 0;
+        k1 += k == 1;
+        k2 += k == 2;
+        k3 += k == 3;
+        k4 += k == 4;
+    }
+    atomicAdd(&h[0], (unsigned long long) k0);
+    atomicAdd(&h[1], (unsigned long long) k1);
+    atomicAdd(&h[2], (unsigned long long) k2);
+    atomicAdd(&h[3], (unsigned long long) k3);
+    atomicAdd(&h[4], (unsigned long long) k4);
+    __syncthreads();
+    if (threadIdx.x < 5 && h[threadIdx.x])
+        atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_zero_slots(unsigned long long *cnt)
+{
+    if (threadIdx.x < 5)
+        cnt[threadIdx.x] = 0;
+}
+
+static int dir_ok(pq_ctx_t *ctx, int dir);
+
+extern "C" int pq_get_slot_sizes(pq_ctx_t *ctx, int dir, uint64_t size[5], uint64_t offset[5],
+                                 uint64_t *max_size)
+{
+    int rc = dir_ok(ctx, dir);
+    if (rc)
+        return rc;
+    if (!size || !offset || !max_size)
+        return ctx->fail(PQ_ERR_ARG, "pq_get_slot_sizes: null output");
+    unsigned long long *cnt = (unsigned long long *) ctx->tmp(SLOT_MISC2, 64);
+    if (!cnt)
+        return ctx->fail(PQ_ERR_NOMEM, "slot counters");
+    hipLaunchKernelGGL(k_zero_slots, dim3(1), dim3(64), 0, ctx->stream, cnt);
+    if (ctx->N)
+        hipLaunchKernelGGL(k_slot_hist, dim3(pq_blocks_gs(ctx->N, 256)), dim3(256), 0,
+                           ctx->stream, ctx->geom(), ctx->dir[dir].text2, cnt);
+    PQ_CHECK(hipGetLastError());
+    unsigned long long h[5];
+    PQ_CHECK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    *max_size = 0;
+    for (int k = 0; k < 5; k++) {
+        size[k] = h[k];
+        offset[k] = k ? offset[k - 1] + size[k - 1] : 0;
+        *max_size = size[k] > *max_size ? size[k] : *max_size;
+    }
+    return PQ_OK;
+}
+
 static int dir_ok(pq_ctx_t *ctx, int dir)
 {
     if (!ctx)

@@ -1,5 +1,8 @@
 """Device index build (SA, BWT, occurrence table, reads_index, C) against the
 reference fixtures and the oracle; size-independent properties at scale."""
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -10,6 +13,8 @@ from test_oracle import _random_reads
 
 pytestmark = pytest.mark.gpu
 CASES = sorted(golden_cases())
+with open(os.path.join(os.path.dirname(__file__), "golden", "slots.json")) as _f:
+    SLOTS = json.load(_f)
 
 
 def _check_against_oracle(ctx, codes, off, t):
@@ -24,6 +29,7 @@ def _check_against_oracle(ctx, codes, off, t):
         assert np.array_equal(ctx.reads_index(d), ix[d]["reads_index"]), ("reads_index", d)
         bm, sm = ctx.occ_ref(d, 8)
         assert np.array_equal(bm, ix[d]["bitmap"]) and np.array_equal(sm, ix[d]["sample"])
+        assert ctx.slot_sizes(d) == orc.slot_sizes(ix[d]["text"], off), ("slots", d)
         lens = np.diff(off.astype(np.int64))
         if len(lens) and lens.min() == lens.max():
             # interval shortcuts at the depths the reference's sort records them
@@ -51,6 +57,12 @@ def test_index_matches_reference_fixture(ctx, name):
         assert np.array_equal(bm, G[f"bitmap{d}"]), d
         assert np.array_equal(sm.astype(np.uint64), G[f"sample{d}"]), d
         assert np.array_equal(ctx.reads_index(d), G[f"reads_index{d}"]), d
+        # first-triplet slot sizes as the reference's construct_index sets them
+        size, offs, mx = ctx.slot_sizes(d)
+        tag = "fw" if d == 0 else "bw"
+        assert (size, offs, mx) == (SLOTS[name]["u32"][f"size_{tag}"],
+                                    SLOTS[name]["u32"][f"offset_{tag}"],
+                                    SLOTS[name]["u32"][f"max_size_{tag}"]), d
         # interval shortcuts: the reference's own start / size / depth per read
         st, sz, dp = ctx.shortcut(d)
         assert np.array_equal(st, G[f"istart{d}"]) and np.array_equal(sz, G[f"iend{d}"])

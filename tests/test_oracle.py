@@ -245,3 +245,21 @@ def test_mlp_gradient_oracle_central_differences(k, h, seed):
         f = (xs & 3).astype(np.float64) * 2 / 3 - 1
         out = params[3] + params[2] * np.maximum(0, params[1] + params[0] * f)
         assert np.isclose(loss, np.mean((out - ys) ** 2), rtol=1e-12)
+
+
+@pytest.mark.parametrize("width", ["u32", "u64"])
+@pytest.mark.parametrize("name", CASES)
+def test_slot_sizes_match_reference(name, width):
+    # index_t's first-triplet slot sizes as the reference's construct_index
+    # reports them (tests/golden/slots.json, written by make_slot_golden.py)
+    import json
+    want = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "slots.json")))
+    want = want[name][width]
+    G = load_golden(name, width)
+    L = golden_cases()[name]["L"]
+    n = int(G["n"])
+    off = _off(n, L)
+    for d, tag in ((0, "fw"), (1, "bw")):
+        size, offs, mx = orc.slot_sizes(orc.dir_text(G["reads"], off, d), off)
+        assert size == want[f"size_{tag}"] and offs == want[f"offset_{tag}"], (d, size)
+        assert mx == want[f"max_size_{tag}"] and sum(size) == n * L
