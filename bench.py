@@ -55,6 +55,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """Model name of the host CPU and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "usable_cores": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline_reference(fasta, L, t, n_expected, exe_name="pasqual_omp_u32"):
     """The reference OpenMP program (oracle/_ref) on the same reads file, timed
     by its own timers (index.c:643-680, string_graph.c:737-1044)."""
@@ -431,6 +445,8 @@ def main():
     }
     if learned:
         line["learned_vs_plain_search"] = round(learned["reads_per_s"] / reads_per_s, 3)
+    if base:
+        base["host"] = host_cpu()
     if base and IMPORT_INFO and base.get("import_ms"):
         line["import"]["reference_ms"] = base["import_ms"]
     if base:
