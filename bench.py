@@ -298,6 +298,7 @@ def main():
         ctx.build_index()
         ctx.learned_train()
         tune_l = ctx.autotune(pq.SEARCH_LEARNED)
+        log(f"learned search tuned: {tune_l}")
         gl = ctx.graph()
         same = all(np.array_equal(gl[k], g[k]) for k in g)
         ctx.sync()

@@ -48,6 +48,15 @@ struct ChainBuf {
     uint32_t cap;            // entries per chain
 };
 
+// Orders the LDS accesses of a wave: what one lane wrote before it is visible
+// to every lane after it.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ unsigned chain_sym(const uint64_t *fwt, uint64_t s, uint32_t L, int ch,
                                               uint32_t o)
 {
@@ -76,10 +85,6 @@ __global__ void __launch_bounds__(256) k_find1(Geom G, DirViewT<R> fw, DirViewT<
     uint32_t L = pq_read_len(G, r);
     const bool fwsearch = (ch == 0 || ch == 3);
     const OccLine *occ = fwsearch ? fw.occ : bw.occ;
-    R cf[5];
-#pragma unroll
-    for (int q = 0; q < 5; q++)
-        cf[q] = fwsearch ? fw.cf[q] : bw.cf[q];
     const uint32_t minsize = (ch == 0 || ch == 2) ? 2u : 1u;
     uint32_t o, size;
     R l;
@@ -123,7 +128,7 @@ __global__ void __launch_bounds__(256) k_find1(Geom G, DirViewT<R> fw, DirViewT<
         }
         if (!more)
             break;
-        l = pq_pick5(cf, c) + a.c;
+        l = (fwsearch ? pq_pick5(fw.cf, c) : pq_pick5(bw.cf, c)) + a.c;
         size = b.c - a.c;
         o++;
     }
@@ -161,10 +166,6 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
         const uint64_t s = pq_read_start(G, r);
         const uint32_t L = pq_read_len(G, r);
         const OccLine *occ = bwi ? bw.occ : fw.occ;
-        R cf[5];
-#pragma unroll
-        for (int q = 0; q < 5; q++)
-            cf[q] = bwi ? bw.cf[q] : fw.cf[q];
         // chains (FB, BB) on the forward index, (FF, BF) on the backward one
         const int ch[2] = {bwi ? 1 : 0, bwi ? 2 : 3};
         const uint32_t minsize[2] = {bwi ? 1u : 2u, bwi ? 2u : 1u};
@@ -232,7 +233,7 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
                     live[q] = false;
                     continue;
                 }
-                l[q] = pq_pick5(cf, c[q]) + a.c;
+                l[q] = (bwi ? pq_pick5(bw.cf, c[q]) : pq_pick5(fw.cf, c[q])) + a.c;
                 size[q] = b.c - a.c;
                 o[q]++;
                 live[q] = o[q] < L && size[q] >= minsize[q];
@@ -263,13 +264,6 @@ struct ItemMeta {
     uint16_t klen;
     uint8_t rank, type, valid, pad;
 };
-
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // -1/+1: a sorts before/after b with a difference inside the common length;
 // -2: a is a proper prefix of b; +2: b is a proper prefix of a; 0: equal.
