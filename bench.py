@@ -340,7 +340,7 @@ def main():
                    "train_loss": [float(f"{x:.3g}") for x in st["train_loss"]],
                    "edges_identical_to_plain": bool(same),
                    "extraction_shape": {k: tune_l[k] for k in ("group_lanes", "items")},
-                   "find_chains_per_lane": tune_l["find_chains"]}
+                   "find_kernel": tune_l["find_chains"]}
         log("learned steps done")
     if args.probe_copy:
         pms = ctx.probe_copy(args.probe_copy << 20)
@@ -355,6 +355,19 @@ def main():
         read_gbs = round((1 << 30) / min(ctx.probe_read(1 << 30) for _ in range(3)) / 1e6, 1)
     except pq.PqError as e:
         log(f"streaming ceilings not measured: {e}")
+    # scattered 64-byte line reads over a buffer the size of both indexes'
+    # occurrence lines (0.5 B per row and direction): independent positions, and
+    # chains of dependent loads as a backward search makes them
+    gather_gbs = chain_gbs = None
+    try:
+        occ_bytes = 2 * ((N + 127) // 128) * LINE_BYTES
+        nbytes = (1 << 25) * LINE_BYTES
+        gather_gbs = round(nbytes / min(ctx.probe_gather(occ_bytes, 0) for _ in range(3)) / 1e6, 1)
+        chain_gbs = round(nbytes / min(ctx.probe_gather(occ_bytes, 1) for _ in range(3)) / 1e6, 1)
+        log(f"scattered line reads over {occ_bytes / 1e9:.2f} GB: {gather_gbs} GB/s independent, "
+            f"{chain_gbs} GB/s in dependent chains")
+    except pq.PqError as e:
+        log(f"scattered-line ceilings not measured: {e}")
     row_bytes, slices = ctx.layout()
     lens = np.diff(off)
     read_len = int(lens[0]) if int(lens.min()) == int(lens.max()) else \
@@ -373,8 +386,8 @@ def main():
     # HBM/fabric bytes per k_find launch from the committed counter runs of this
     # same workload: FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for
     # the plain copy kernel
-    # (the autotuner's choice: one or two search chains per lane)
-    kname = "k_find2" if tune["find_chains"] == 2 else "k_find1"
+    # (the autotuner's choice of search kernel)
+    kname = {1: "k_find1", 2: "k_find2", 3: "k_find1q"}[tune["find_chains"]]
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"config{args.config}_traffic.json")
     if os.path.exists(tpath):
@@ -394,6 +407,14 @@ def main():
     roofline["copy_ceiling"] = copy_gbs
     roofline["read_ceiling"] = read_gbs
     roofline["frac_of_read_ceiling"] = round(achieved / read_gbs, 4) if read_gbs else None
+    # the search's distinct line loads (one or two per step) against the
+    # scattered-line ceilings of the same footprint
+    line_gbs = lines * LINE_BYTES / (sum(find_launch_ms) / 1e3) / 1e9 if sum(find_launch_ms) else None
+    roofline["line_gbs"] = round(line_gbs, 1) if line_gbs else None
+    roofline["gather_ceiling"] = gather_gbs
+    roofline["chain_ceiling"] = chain_gbs
+    roofline["lines_frac_of_gather_ceiling"] = round(line_gbs / gather_gbs, 4) \
+        if line_gbs and gather_gbs else None
 
     base = dropin = None
     if args.cpu_baseline == "full":
@@ -436,7 +457,7 @@ def main():
         "build_ms": round(avg_build, 3),
         "edges": edges,
         "extraction_shape": {k: tune[k] for k in ("group_lanes", "items")},
-        "find_chains_per_lane": tune["find_chains"],
+        "find_kernel": tune["find_chains"],
         "roofline": roofline,
         "cpu_baseline": base,
         "learned": learned,

@@ -229,14 +229,15 @@ typedef struct {
     uint32_t items;          /* candidates per side held in LDS (16, 32, 64) */
     uint32_t nshapes;
     double extract_ms[4];    /* extraction time measured per shape */
-    uint32_t find_chains;    /* candidate-search chains per lane chosen (1 or 2) */
-    double find_ms[2];       /* candidate-search time with 1 and 2 chains per lane */
+    uint32_t find_chains;    /* candidate-search kernel chosen (1, 2 or 3, below) */
+    double find_ms[3];       /* candidate-search time of kernels 1, 2 and 3 */
 } pq_tune_t;
 int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out);
 int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape);
-/* The candidate search runs the four chains of a read on four lanes (1) or the
- * two chains of each index interleaved on one lane (2, the default); same
- * result.  pq_autotune times both. */
+/* The candidate search runs the four chains of a read on four lanes (1), the
+ * two chains of each index interleaved on one lane (2, the default), or on four
+ * lanes that load each other's occurrence lines together, one 64-byte access
+ * per line (3); same result.  pq_autotune times all three. */
 int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains);
 /* Capacity of the search's shared edge buffer: edges of read sides with more
  * than 16 irreducible edges (the first 16 of every side have slots of their
@@ -267,6 +268,13 @@ int pq_probe_copy(pq_ctx_t *ctx, uint64_t bytes, double *ms);
 /* Measurement helper: one streaming read of `bytes` bytes (k_read_probe), the
  * read-only ceiling of the device; *ms gets its duration. */
 int pq_probe_read(pq_ctx_t *ctx, uint64_t bytes, double *ms);
+/* Measurement helper: 2^25 scattered 64-byte line reads over a buffer of
+ * `bytes` bytes (k_gather_probe).  mode 0: every lane loads whole lines as the
+ * search loads occurrence lines, independent positions (throughput ceiling of
+ * scattered lines); 1: each position depends on the previous line the lane
+ * read; 2: four lanes load one line together, 16 bytes each; 3: every lane
+ * loads 40 bytes of each line (three loads). */
+int pq_probe_gather(pq_ctx_t *ctx, uint64_t bytes, int mode, double *ms);
 
 /* ---- host utilities (no device use) --------------------------------------------------- */
 /* Seeded synthetic workloads (see pq_synth.c for the generator contract). */

@@ -38,7 +38,7 @@ PQ_SYMBOLS = [
     "pq_probe_copy", "pq_set_layout", "pq_get_layout", "pq_set_find_chains", "pq_synth_preset", "pq_synth_num_reads",
     "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
-    "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read", "pq_set_edge_capacity",
+    "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read", "pq_probe_gather", "pq_set_edge_capacity",
     "pq_get_slot_sizes",
 ]
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
@@ -94,7 +94,7 @@ class LearnedStats(C.Structure):
 class Tune(C.Structure):
     _fields_ = [("shape", C.c_uint32), ("group_lanes", C.c_uint32), ("items", C.c_uint32),
                 ("nshapes", C.c_uint32), ("extract_ms", C.c_double * 4),
-                ("find_chains", C.c_uint32), ("find_ms", C.c_double * 2)]
+                ("find_chains", C.c_uint32), ("find_ms", C.c_double * 3)]
 
 
 class Timing(C.Structure):
@@ -173,6 +173,7 @@ def lib(width="u32"):
         "pq_dedup_reads": (I32, [P, P, P, U64, I32, U64, I32, I32, P, P]),
         "pq_graph_nodes": (I32, [P, P, P]),
         "pq_probe_read": (I32, [P, U64, P]),
+        "pq_probe_gather": (I32, [P, U64, I32, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -472,6 +473,15 @@ class Context:
     def probe_copy(self, nbytes):
         ms = C.c_double()
         self._check(self._L.pq_probe_copy(self._h, nbytes, C.byref(ms)), "probe_copy")
+        return ms.value
+
+    def probe_gather(self, nbytes, mode):
+        """Milliseconds for 2^25 scattered 64-byte line reads over nbytes (mode:
+        0 whole lines per lane, 1 dependent chains, 2 four lanes per line,
+        3 40 bytes per line)."""
+        ms = C.c_double()
+        self._check(self._L.pq_probe_gather(self._h, nbytes, int(mode), C.byref(ms)),
+                    "probe_gather")
         return ms.value
 
     def probe_read(self, nbytes):

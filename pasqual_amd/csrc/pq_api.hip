@@ -698,6 +698,99 @@ extern "C" int pq_probe_read(pq_ctx_t *ctx, uint64_t bytes, double *ms)
     return PQ_OK;
 }
 
+// Scattered 64-byte line reads over a buffer of nlines OccLines, 2^25 lines
+// per launch at hashed positions (below nlines by construction: multiply-high).
+// Modes: 0 = every lane loads whole lines as a search step does (four 16-byte
+// loads), positions independent of each other (the throughput ceiling of
+// scattered lines); 1 = the same with each position hashed from the previous
+// line's contents (chains of dependent loads, as in a backward search); 2 =
+// the four lanes of a quad load one line together, 16 bytes each; 3 = every
+// lane loads the 40 bytes of a line that one rank needs (three loads: 16, 16
+// and 8 bytes from byte 0 or 24).  Same sink rule as k_read_probe.
+__device__ __forceinline__ uint64_t probe_hash(uint64_t h)
+{
+    h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+    h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+    return h ^ (h >> 31);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_gather_probe(const OccLine *src, uint64_t nlines,
+                                                      uint32_t k, uint32_t *sink)
+{
+    const uint64_t lane = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t who = MODE == 2 ? lane >> 2 : lane;
+    uint32_t acc = 0;
+    for (uint32_t i = 0; i < k; i++) {
+        const uint64_t h = probe_hash(who * 0x9E3779B97F4A7C15ull + i + (MODE == 1 ? acc : 0u));
+        const uint64_t idx = __umul64hi(h, nlines);
+        if (MODE == 2) {
+            const uint4 v = reinterpret_cast<const uint4 *>(src + idx)[lane & 3];
+            acc ^= v.x ^ v.y ^ v.z ^ v.w;
+        } else if (MODE == 3) {
+            const uint32_t *w = reinterpret_cast<const uint32_t *>(src + idx) + ((h & 1) ? 6 : 0);
+            const uint4 a = *reinterpret_cast<const uint4 *>(w);
+            const uint4 b = *reinterpret_cast<const uint4 *>(w + 4);
+            const uint2 c = *reinterpret_cast<const uint2 *>(w + 8);
+            acc ^= a.x ^ a.y ^ a.z ^ a.w ^ b.x ^ b.y ^ b.z ^ b.w ^ c.x ^ c.y;
+        } else {
+            OccLine ln;
+            pq_load_line(src, idx, ln);
+            acc ^= (uint32_t) (ln.p0[0] ^ ln.p0[1] ^ ln.p1[0] ^ ln.p1[1] ^ ln.p2[0] ^ ln.p2[1]) ^
+                   (uint32_t) ((ln.p0[0] ^ ln.p1[1] ^ ln.p2[0]) >> 32) ^ ln.cnt[0] ^ ln.cnt[1] ^
+                   ln.cnt[2] ^ ln.cnt[3];
+        }
+    }
+    if (acc == 0x9E3779B9u)
+        sink[0] = acc;
+}
+
+extern "C" int pq_probe_gather(pq_ctx_t *ctx, uint64_t bytes, int mode, double *ms)
+{
+    if (!ctx || bytes < 64 || mode < 0 || mode > 3)
+        return PQ_ERR_ARG;
+    const uint64_t nlines = bytes / 64;
+    char *buf = (char *) ctx->tmp(SLOT_PROBE, nlines * 64 + 64);
+    if (!buf)
+        return ctx->fail(PQ_ERR_NOMEM, "gather probe buffer");
+    // 2^20 lanes (4096 blocks: several rounds of the GPU's resident waves) x 32
+    // lines per lane (x 128 per quad in mode 2) = 2^25 lines; one untimed
+    // launch first
+    uint32_t *sink = (uint32_t *) (buf + nlines * 64);
+    const OccLine *src = (const OccLine *) buf;
+    auto launch = [&]() {
+        const dim3 g(4096), b(256);
+        switch (mode) {
+        case 0:
+            hipLaunchKernelGGL(k_gather_probe<0>, g, b, 0, ctx->stream, src, nlines, 32u, sink);
+            break;
+        case 1:
+            hipLaunchKernelGGL(k_gather_probe<1>, g, b, 0, ctx->stream, src, nlines, 32u, sink);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_gather_probe<2>, g, b, 0, ctx->stream, src, nlines, 128u, sink);
+            break;
+        default:
+            hipLaunchKernelGGL(k_gather_probe<3>, g, b, 0, ctx->stream, src, nlines, 32u, sink);
+        }
+    };
+    launch();
+    hipEvent_t a, b;
+    PQ_CHECK(hipEventCreate(&a));
+    PQ_CHECK(hipEventCreate(&b));
+    PQ_CHECK(hipEventRecord(a, ctx->stream));
+    launch();
+    PQ_CHECK(hipEventRecord(b, ctx->stream));
+    PQ_CHECK(hipEventSynchronize(b));
+    float t = 0;
+    (void) hipEventElapsedTime(&t, a, b);
+    if (ms)
+        *ms = t;
+    (void) hipEventDestroy(a);
+    (void) hipEventDestroy(b);
+    return PQ_OK;
+}
+
 extern "C" int pq_get_timing(pq_ctx_t *ctx, pq_timing_t *t)
 {
     if (!ctx || !t)

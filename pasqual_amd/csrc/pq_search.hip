@@ -147,6 +147,152 @@ __global__ void __launch_bounds__(256) k_find1(Geom G, DirViewT<R> fw, DirViewT<
     }
 }
 
+// Lane I of the quad (four adjacent lanes) broadcast to the whole quad.
+template <int I> __device__ __forceinline__ uint32_t quad_bcast(uint32_t v)
+{
+    return (uint32_t) __builtin_amdgcn_mov_dpp((int) v, I * 0x55, 0xF, 0xF, false);
+}
+
+template <int I> __device__ __forceinline__ uint64_t quad_bcast64(uint64_t v)
+{
+    return ((uint64_t) quad_bcast<I>((uint32_t) (v >> 32)) << 32) | quad_bcast<I>((uint32_t) v);
+}
+
+// Lane q of a quad reads bytes [16q, 16q+16) of the line of quad lane I, when
+// that lane needs one (quad-uniform): one 64-byte access per line.
+template <int I>
+__device__ __forceinline__ void quad_fetch(const OccLine *occ, uint64_t w, bool need, uint4 *xq,
+                                           unsigned q)
+{
+    const uint64_t wi = quad_bcast64<I>(w);
+    if (quad_bcast<I>(need ? 1u : 0u))
+        xq[I * 4 + q] = reinterpret_cast<const uint4 *>(occ + wi)[q];
+}
+
+// Line of this lane from the quarters its quad stored in xq (lane-major).
+__device__ __forceinline__ void quad_line(const uint4 *xq, unsigned q, OccLine &ln)
+{
+    const uint4 q0 = xq[q * 4 + 0], q1 = xq[q * 4 + 1], q2 = xq[q * 4 + 2], q3 = xq[q * 4 + 3];
+    ln.p0[0] = ((uint64_t) q0.y << 32) | q0.x;
+    ln.p0[1] = ((uint64_t) q0.w << 32) | q0.z;
+    ln.p1[0] = ((uint64_t) q1.y << 32) | q1.x;
+    ln.p1[1] = ((uint64_t) q1.w << 32) | q1.z;
+    ln.p2[0] = ((uint64_t) q2.y << 32) | q2.x;
+    ln.p2[1] = ((uint64_t) q2.w << 32) | q2.z;
+    ln.cnt[0] = q3.x;
+    ln.cnt[1] = q3.y;
+    ln.cnt[2] = q3.z;
+    ln.cnt[3] = q3.w;
+}
+
+// One chain per lane as k_find1, with the occurrence lines of a read's four
+// chains loaded by its four lanes together: lane q fetches quarter q of each
+// lane's line, one 64-byte access per line instead of four 16-byte accesses
+// from one lane, and the quarters are exchanged through LDS.  Where the lines
+// span more memory than the address translation caches cover (gigabytes of
+// lines), every access pays a translation: a quarter of the accesses then
+// doubles the rate of scattered line loads.  Chains 0 and 3 search the forward
+// index, 1 and 2 the backward one, so the index of each quad lane is known at
+// compile time.  The four lanes of a read iterate while any of them is live.
+template <typename R>
+__global__ void __launch_bounds__(256) k_find1q(Geom G, DirViewT<R> fw, DirViewT<R> bw,
+                                               uint64_t r0, uint64_t nr, uint32_t t, CTab<R> C,
+                                               const SeedT<R> *seeds, uint32_t kseed,
+                                               ChainBuf out, unsigned long long *stats)
+{
+    __shared__ uint4 xch[256 * 4];
+    uint64_t tid = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t rl = tid >> 2;
+    const int ch = (int) (tid & 3);
+    const unsigned q = threadIdx.x & 3;
+    uint4 *xq = xch + (threadIdx.x & ~3u) * 4;
+    uint32_t steps = 0, lines = 0;
+    const bool valid = rl < nr;
+    const uint64_t r = r0 + (valid ? rl : 0);
+    const uint64_t s = pq_read_start(G, r);
+    const uint32_t L = pq_read_len(G, r);
+    const bool fwsearch = (ch == 0 || ch == 3);
+    const uint32_t minsize = (ch == 0 || ch == 2) ? 2u : 1u;
+    uint32_t o = 1, size = 0;
+    R l = 0;
+    SeedT<R> sd = {0, SEED_NONE};
+    if (valid && seeds && kseed > 0 && kseed < L)
+        sd = seeds[rl * 4 + ch];
+    if (sd.size != SEED_NONE) {
+        o = kseed;
+        l = sd.l;
+        size = sd.size;
+    } else if (valid) {
+        unsigned c = chain_sym(fw.text2, s, L, ch, 1);
+        l = pq_pick6(C.c, c);
+        size = (uint32_t) (pq_pick6(C.c, c + 1) - l);
+    }
+    uint32_t nout = 0;
+    const uint64_t base = (rl * 4 + (uint64_t) ch) * out.cap;
+    bool live = valid && o < L && size >= minsize;
+    const unsigned qshift = threadIdx.x & 60u;
+    while ((__ballot(live) >> qshift) & 0xFull) {
+        const bool more = live && o + 1 < L;
+        const unsigned c = more ? chain_sym(fw.text2, s, L, ch, o + 1) : 1u;
+        const uint64_t u1 = (uint64_t) l + size;
+        const uint64_t wl = (uint64_t) l >> 7, wu = u1 >> 7;
+        const bool two = live && wu != wl;
+        quad_fetch<0>(fw.occ, wl, live, xq, q);
+        quad_fetch<1>(bw.occ, wl, live, xq, q);
+        quad_fetch<2>(bw.occ, wl, live, xq, q);
+        quad_fetch<3>(fw.occ, wl, live, xq, q);
+        wave_sync();
+        OccLine ln;
+        quad_line(xq, q, ln);
+        const StepRanks a = pq_step_ranks(ln, wl, (unsigned) (l & 127), c);
+        wave_sync();
+        if ((__ballot(two) >> qshift) & 0xFull) {
+            quad_fetch<0>(fw.occ, wu, two, xq, q);
+            quad_fetch<1>(bw.occ, wu, two, xq, q);
+            quad_fetch<2>(bw.occ, wu, two, xq, q);
+            quad_fetch<3>(fw.occ, wu, two, xq, q);
+            wave_sync();
+            if (two)
+                quad_line(xq, q, ln);
+            wave_sync();
+        }
+        const StepRanks b = pq_step_ranks(ln, wu, (unsigned) (u1 & 127), c);
+        if (live) {
+            steps++;
+            lines += two ? 2u : 1u;
+            if (o >= t) {
+                uint32_t cnt = b.d - a.d;
+                if (cnt && nout < out.cap) {
+                    out.lc[base + nout] = make_uint2(a.d, cnt);
+                    out.o[base + nout] = (uint16_t) o;
+                    nout++;
+                }
+            }
+            if (more) {
+                l = (fwsearch ? pq_pick5(fw.cf, c) : pq_pick5(bw.cf, c)) + a.c;
+                size = b.c - a.c;
+                o++;
+                live = size >= minsize;
+            } else {
+                live = false;
+            }
+        }
+    }
+    if (valid)
+        out.n[rl * 4 + ch] = nout;
+    if (stats) {
+        uint64_t s2 = steps, ln = lines;
+        for (int d = 32; d > 0; d >>= 1) {
+            s2 += __shfl_xor(s2, d, 64);
+            ln += __shfl_xor(ln, d, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&stats[0], (unsigned long long) (2 * s2));
+            atomicAdd(&stats[1], (unsigned long long) ln);
+        }
+    }
+}
+
 // Two chains per lane: FB and BB search the forward index, FF and BF the
 // backward one, so a lane takes the two chains of one index of a read (same
 // occurrence lines and step bases) and keeps one backward step of each in
@@ -929,7 +1075,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     unsigned long long *fstats = (unsigned long long *) (misc + 8);   // misc[8..11]
     // the whole search as a sequence of stream operations (captured into a graph
     // in graph mode: kernels only, no memsets, no host synchronisation)
-    const uint32_t chains = ctx->find_chains == 1 ? 1u : 2u;
+    const uint32_t chains = ctx->find_chains >= 1 && ctx->find_chains <= 3 ? ctx->find_chains : 2u;
     auto enqueue = [&](bool timed) -> int {
     int rc = PQ_OK;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc, 16u);
@@ -964,6 +1110,9 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
             (void) hipEventRecord(e[1], ctx->stream);
         if (chains == 1)
             hipLaunchKernelGGL(k_find1<R>, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0,
+                               ctx->stream, G, fw, bw, r0, nr, t, Ct, seeds, kseed, cb, fstats);
+        else if (chains == 3)
+            hipLaunchKernelGGL(k_find1q<R>, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0,
                                ctx->stream, G, fw, bw, r0, nr, t, Ct, seeds, kseed, cb, fstats);
         else
             hipLaunchKernelGGL(k_find2<R>, dim3(pq_blocks(nr * 2, 256)), dim3(256), 0,
@@ -1187,7 +1336,17 @@ extern "C" int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out)
     if (rc)
         return rc;
     t.find_ms[0] = ctx->timing.find_kernel_ms;
-    ctx->find_chains = t.find_ms[0] < t.find_ms[1] ? 1u : 2u;
+    // one chain per lane, the lines of a read's four lanes loaded together (wins
+    // where the lines span more than the translation caches cover)
+    ctx->find_chains = 3;
+    if ((rc = pq_find_overlaps(ctx, mode)))
+        return rc;
+    t.find_ms[2] = ctx->timing.find_kernel_ms;
+    uint32_t fk = 0;
+    for (uint32_t k = 1; k < 3; k++)
+        if (t.find_ms[k] < t.find_ms[fk])
+            fk = k;
+    ctx->find_chains = fk + 1;
     t.find_chains = ctx->find_chains;
     t.shape = pick;
     t.group_lanes = EXT_SHAPES[pick][0];
@@ -1213,8 +1372,8 @@ extern "C" int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains)
 {
     if (!ctx)
         return PQ_ERR_ARG;
-    if (chains != 1 && chains != 2)
-        return ctx->fail(PQ_ERR_ARG, "chains per lane must be 1 or 2");
+    if (chains < 1 || chains > 3)
+        return ctx->fail(PQ_ERR_ARG, "search kernel must be 1, 2 or 3");
     ctx->find_chains = chains;
     return PQ_OK;
 }

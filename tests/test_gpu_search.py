@@ -167,13 +167,13 @@ def test_every_extraction_shape_gives_the_same_graph(ctx):
         ctx.set_tuning(shape)
         ctx.find_overlaps()
         _same_graph(ctx.graph(), want, f"shape {shape}")
-    for chains in (1, 2):
-        ctx.set_find_chains(chains)
+    for kernel in (1, 2, 3):
+        ctx.set_find_chains(kernel)
         ctx.find_overlaps()
-        _same_graph(ctx.graph(), want, f"{chains} chains per lane")
+        _same_graph(ctx.graph(), want, f"search kernel {kernel}")
     t = ctx.autotune()
     assert 0 <= t["shape"] < 3 and len(t["extract_ms"]) == 3
-    assert t["find_chains"] in (1, 2) and min(t["find_ms"]) > 0
+    assert t["find_chains"] in (1, 2, 3) and min(t["find_ms"]) > 0
     _same_graph(ctx.graph(), want, "autotuned")
     ctx.set_tuning(1)
     ctx.set_find_chains(2)

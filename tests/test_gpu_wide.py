@@ -83,6 +83,13 @@ def test_layout_search_and_learned_match_oracle(wctx, row_bytes, slice_, lmin, l
         wctx.find_overlaps(mode)
         _same_graph(wctx.graph(), want, f"mode {mode:#x}")
     assert wctx.learned_stats()["fallback"] == 0
+    # every candidate-search kernel, plain and seeded, on these rows
+    for kernel in (1, 2, 3):
+        wctx.set_find_chains(kernel)
+        for mode in (pq.SEARCH_PLAIN, pq.SEARCH_LEARNED):
+            wctx.find_overlaps(mode)
+            _same_graph(wctx.graph(), want, f"kernel {kernel} mode {mode:#x}")
+    wctx.set_find_chains(2)
 
 
 def test_layout_edge_cases(wctx):
