@@ -358,14 +358,17 @@ def main():
     # scattered 64-byte line reads over a buffer the size of both indexes'
     # occurrence lines (0.5 B per row and direction): independent positions, and
     # chains of dependent loads as a backward search makes them
-    gather_gbs = chain_gbs = None
+    # (GB/s of lines, best of 3): whole lines per lane (positions independent, and
+    # in dependent chains) and four lanes per line, 16 bytes each
+    gather = None
     try:
         occ_bytes = 2 * ((N + 127) // 128) * LINE_BYTES
         nbytes = (1 << 25) * LINE_BYTES
-        gather_gbs = round(nbytes / min(ctx.probe_gather(occ_bytes, 0) for _ in range(3)) / 1e6, 1)
-        chain_gbs = round(nbytes / min(ctx.probe_gather(occ_bytes, 1) for _ in range(3)) / 1e6, 1)
-        log(f"scattered line reads over {occ_bytes / 1e9:.2f} GB: {gather_gbs} GB/s independent, "
-            f"{chain_gbs} GB/s in dependent chains")
+        gather = {name: round(nbytes / min(ctx.probe_gather(occ_bytes, mode) for _ in range(3))
+                              / 1e6, 1)
+                  for name, mode in (("lane", 0), ("lane_chained", 1), ("quad", 2))}
+        gather["footprint_bytes"] = occ_bytes
+        log(f"scattered line reads over {occ_bytes / 1e9:.2f} GB: {gather}")
     except pq.PqError as e:
         log(f"scattered-line ceilings not measured: {e}")
     row_bytes, slices = ctx.layout()
@@ -411,10 +414,10 @@ def main():
     # scattered-line ceilings of the same footprint
     line_gbs = lines * LINE_BYTES / (sum(find_launch_ms) / 1e3) / 1e9 if sum(find_launch_ms) else None
     roofline["line_gbs"] = round(line_gbs, 1) if line_gbs else None
-    roofline["gather_ceiling"] = gather_gbs
-    roofline["chain_ceiling"] = chain_gbs
-    roofline["lines_frac_of_gather_ceiling"] = round(line_gbs / gather_gbs, 4) \
-        if line_gbs and gather_gbs else None
+    roofline["scattered_line_ceilings"] = gather
+    best = max(gather["lane"], gather["quad"]) if gather else None
+    roofline["lines_frac_of_scattered_ceiling"] = round(line_gbs / best, 4) \
+        if line_gbs and best else None
 
     base = dropin = None
     if args.cpu_baseline == "full":

@@ -235,9 +235,11 @@ typedef struct {
 int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out);
 int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape);
 /* The candidate search runs the four chains of a read on four lanes (1), the
- * two chains of each index interleaved on one lane (2, the default), or on four
- * lanes that load each other's occurrence lines together, one 64-byte access
- * per line (3); same result.  pq_autotune times all three. */
+ * two chains of each index interleaved on one lane (2), or on four lanes that
+ * load each other's occurrence lines together, one 64-byte access per line (3);
+ * same result.  pq_autotune times all three and keeps the fastest; until then
+ * (or after chains = 0) the kernel follows the size of both indexes'
+ * occurrence lines: 2 below 1 GiB, 1 below 4 GiB, 3 from there on. */
 int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains);
 /* Capacity of the search's shared edge buffer: edges of read sides with more
  * than 16 irreducible edges (the first 16 of every side have slots of their

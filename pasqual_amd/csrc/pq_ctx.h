@@ -68,7 +68,7 @@ struct pq_ctx {
 
     Learned *learned = nullptr;
     uint32_t ext_shape = 1;        // first extraction tier: index into the tuned shapes
-    uint32_t find_chains = 2;      // candidate-search chains per lane (1 or 2)
+    uint32_t find_chains = 0;      // candidate-search kernel (1, 2, 3; 0: by index size)
     uint64_t edge_cap_set = 0;     // shared edge buffer capacity, 0 automatic
     uint64_t learned_gen = 0;      // bumped by every learned-index training
     // search captured as a graph, valid while its key (launch arguments) holds

@@ -312,6 +312,13 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
         const uint64_t s = pq_read_start(G, r);
         const uint32_t L = pq_read_len(G, r);
         const OccLine *occ = bwi ? bw.occ : fw.occ;
+        // the lane's step bases as an array (the compiler keeps it in LDS): with
+        // selects from the kernel arguments instead, the seeded search of the
+        // 100 bp workload took 6% longer per launch
+        R cf[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++)
+            cf[q] = bwi ? bw.cf[q] : fw.cf[q];
         // chains (FB, BB) on the forward index, (FF, BF) on the backward one
         const int ch[2] = {bwi ? 1 : 0, bwi ? 2 : 3};
         const uint32_t minsize[2] = {bwi ? 1u : 2u, bwi ? 2u : 1u};
@@ -379,7 +386,7 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
                     live[q] = false;
                     continue;
                 }
-                l[q] = (bwi ? pq_pick5(bw.cf, c[q]) : pq_pick5(fw.cf, c[q])) + a.c;
+                l[q] = pq_pick5(cf, c[q]) + a.c;
                 size[q] = b.c - a.c;
                 o[q]++;
                 live[q] = o[q] < L && size[q] >= minsize[q];
@@ -1075,7 +1082,16 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     unsigned long long *fstats = (unsigned long long *) (misc + 8);   // misc[8..11]
     // the whole search as a sequence of stream operations (captured into a graph
     // in graph mode: kernels only, no memsets, no host synchronisation)
-    const uint32_t chains = ctx->find_chains >= 1 && ctx->find_chains <= 3 ? ctx->find_chains : 2u;
+    // search kernel: the autotuner's or the caller's choice, else by the size of
+    // both indexes' occurrence lines (the autotuner's picks on this GPU: two
+    // chains per lane while the lines fit the last-level cache, one chain per
+    // lane up to a few GiB, quad-loaded lines beyond, where scattered loads of
+    // whole lines by single lanes lose half their rate)
+    const uint64_t occ_bytes = (ctx->dir[0].nlines + ctx->dir[1].nlines) * sizeof(OccLine);
+    const uint32_t chains = ctx->find_chains >= 1 && ctx->find_chains <= 3 ? ctx->find_chains
+                            : occ_bytes < (1ull << 30)                     ? 2u
+                            : occ_bytes < (4ull << 30)                     ? 1u
+                                                                           : 3u;
     auto enqueue = [&](bool timed) -> int {
     int rc = PQ_OK;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc, 16u);
@@ -1372,8 +1388,8 @@ extern "C" int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains)
 {
     if (!ctx)
         return PQ_ERR_ARG;
-    if (chains < 1 || chains > 3)
-        return ctx->fail(PQ_ERR_ARG, "search kernel must be 1, 2 or 3");
+    if (chains > 3)
+        return ctx->fail(PQ_ERR_ARG, "search kernel must be 0 (by index size), 1, 2 or 3");
     ctx->find_chains = chains;
     return PQ_OK;
 }

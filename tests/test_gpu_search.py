@@ -167,7 +167,7 @@ def test_every_extraction_shape_gives_the_same_graph(ctx):
         ctx.set_tuning(shape)
         ctx.find_overlaps()
         _same_graph(ctx.graph(), want, f"shape {shape}")
-    for kernel in (1, 2, 3):
+    for kernel in (0, 1, 2, 3):
         ctx.set_find_chains(kernel)
         ctx.find_overlaps()
         _same_graph(ctx.graph(), want, f"search kernel {kernel}")
@@ -176,4 +176,4 @@ def test_every_extraction_shape_gives_the_same_graph(ctx):
     assert t["find_chains"] in (1, 2, 3) and min(t["find_ms"]) > 0
     _same_graph(ctx.graph(), want, "autotuned")
     ctx.set_tuning(1)
-    ctx.set_find_chains(2)
+    ctx.set_find_chains(0)

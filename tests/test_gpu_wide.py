@@ -89,7 +89,7 @@ def test_layout_search_and_learned_match_oracle(wctx, row_bytes, slice_, lmin, l
         for mode in (pq.SEARCH_PLAIN, pq.SEARCH_LEARNED):
             wctx.find_overlaps(mode)
             _same_graph(wctx.graph(), want, f"kernel {kernel} mode {mode:#x}")
-    wctx.set_find_chains(2)
+    wctx.set_find_chains(0)
 
 
 def test_layout_edge_cases(wctx):
