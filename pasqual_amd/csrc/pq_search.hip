@@ -718,9 +718,11 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
         }
         const uint32_t k = carry;
         wave_sync();
-        // expand the candidate intervals into items with their extension keys
-        for (uint32_t q = lane; q < k; q += GW) {
-            uint32_t lo = 0, hi = ne;        // entry e with pre[e] <= q < pre[e+1]
+        // expand the candidate intervals into items with their extension keys:
+        // two items per lane at a time, so that each lane has two independent
+        // chains of loads (candidate record -> reads_index -> key words) in flight
+        auto entry_of = [&](uint32_t q) -> uint32_t {   // e with pre[e] <= q < pre[e+1]
+            uint32_t lo = 0, hi = ne;
             while (hi - lo > 1) {
                 uint32_t mid = (lo + hi) >> 1;
                 if (pre[mid] <= q)
@@ -728,43 +730,68 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
                 else
                     hi = mid;
             }
-            uint32_t e = lo;
-            bool first = e < nA;
-            int ch = first ? chA : chB;
-            uint64_t at = first ? (rl * 4 + chA) * cb.cap + e : (rl * 4 + chB) * cb.cap + (e - nA);
-            uint2 lc = cb.lc[at];
-            uint32_t o = cb.o[at];
-            // FB, BB searched the forward index; FF, BF the backward index
-            const uint32_t *ri = (ch == 0 || ch == 3) ? fw.reads_index : bw.reads_index;
-            uint64_t m = ri[lc.x + (q - pre[e])];
-            uint32_t Lm = pq_read_len(G, m);
-            ItemMeta mt;
-            mt.m = (uint32_t) m;
-            mt.o = (uint16_t) o;
-            mt.valid = o < Lm;                 // contained reads are no candidates
-            mt.klen = (uint16_t) (mt.valid ? Lm - o : 0);
-            mt.rank = first ? 0 : 1;
-            mt.type = ch == 0 ? 1 : ch == 1 ? 2 : ch == 2 ? 0 : 3;   // FB FF BF BB
-            mt.pad = 0;
-            // key: FB/BB m[o..) from the forward text; FF/BF rc(m)[o..) as the
-            // complement of the backward text
-            bool plain = (ch == 0 || ch == 3);
-            const uint64_t *tx = plain ? fw.text2 : bw.text2;
-            uint64_t ks = pq_read_start(G, m) + o;
-            uint64_t *kq = keys + (uint64_t) q * kw;
-            for (uint32_t w = 0; w < kw; w++) {
-                uint64_t x = 0;
-                if (32 * w < mt.klen) {
-                    x = pq_window(tx, ks + 32 * w);
-                    if (!plain)
-                        x = ~x;
-                    uint32_t left = mt.klen - 32 * w;
-                    if (left < 32)
-                        x &= ~0ull << (64 - 2 * left);
-                }
-                kq[w] = x;
+            return lo;
+        };
+        for (uint32_t q0 = lane; q0 < k; q0 += 2 * GW) {
+            uint32_t qq[2] = {q0, q0 + GW};
+            bool on[2] = {true, q0 + GW < k};
+            uint32_t ee[2], oo[2];
+            bool firsts[2];
+            uint2 lcs[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                ee[h] = on[h] ? entry_of(qq[h]) : 0u;
+                firsts[h] = ee[h] < nA;
+                const uint64_t at = firsts[h] ? (rl * 4 + chA) * cb.cap + ee[h]
+                                              : (rl * 4 + chB) * cb.cap + (ee[h] - nA);
+                lcs[h] = on[h] ? cb.lc[at] : make_uint2(0, 0);
+                oo[h] = on[h] ? (uint32_t) cb.o[at] : 0u;
             }
-            meta[q] = mt;
+            uint64_t mm[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                // FB, BB searched the forward index; FF, BF the backward index
+                const int ch = firsts[h] ? chA : chB;
+                const uint32_t *ri = (ch == 0 || ch == 3) ? fw.reads_index : bw.reads_index;
+                mm[h] = on[h] ? (uint64_t) ri[lcs[h].x + (qq[h] - pre[ee[h]])] : 0ull;
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (!on[h])
+                    continue;
+                const uint32_t q = qq[h];
+                const int ch = firsts[h] ? chA : chB;
+                const uint64_t m = mm[h];
+                const uint32_t o = oo[h];
+                uint32_t Lm = pq_read_len(G, m);
+                ItemMeta mt;
+                mt.m = (uint32_t) m;
+                mt.o = (uint16_t) o;
+                mt.valid = o < Lm;                 // contained reads are no candidates
+                mt.klen = (uint16_t) (mt.valid ? Lm - o : 0);
+                mt.rank = firsts[h] ? 0 : 1;
+                mt.type = ch == 0 ? 1 : ch == 1 ? 2 : ch == 2 ? 0 : 3;   // FB FF BF BB
+                mt.pad = 0;
+                // key: FB/BB m[o..) from the forward text; FF/BF rc(m)[o..) as the
+                // complement of the backward text
+                bool plain = (ch == 0 || ch == 3);
+                const uint64_t *tx = plain ? fw.text2 : bw.text2;
+                uint64_t ks = pq_read_start(G, m) + o;
+                uint64_t *kq = keys + (uint64_t) q * kw;
+                for (uint32_t w = 0; w < kw; w++) {
+                    uint64_t x = 0;
+                    if (32 * w < mt.klen) {
+                        x = pq_window(tx, ks + 32 * w);
+                        if (!plain)
+                            x = ~x;
+                        uint32_t left = mt.klen - 32 * w;
+                        if (left < 32)
+                            x &= ~0ull << (64 - 2 * left);
+                    }
+                    kq[w] = x;
+                }
+                meta[q] = mt;
+            }
         }
         wave_sync();
         // sorted evaluation: one read per wave sorts to the next power of two of
