@@ -22,6 +22,7 @@ times the same number of seeded searches.  Prints one JSON line.
 """
 import argparse
 import json
+import math
 import os
 import re
 import subprocess
@@ -380,6 +381,21 @@ def main():
     avg_build = sum(build_ms) / len(build_ms)
     reads_per_s = n / (avg_search / 1e3)
     gsuffix_per_s = 2 * N / (avg_build / 1e3) / 1e9
+    # the suffix-array build against the HBM peak by a prefix-doubling byte
+    # model: R = 1 + ceil(log2(Lmax / 29)) rounds from a 64-bit key of 29 codes,
+    # each one key + index pass (read and write) plus a rank read, gather and
+    # write, then the BWT gather and the occurrence write; W = the index width
+    # (4 bytes for 32-bit indexes, 8 for the 64-bit index of the larger
+    # workloads, whatever width the device rows have)
+    W = 8 if args.config >= 3 else 4
+    lmax = int(np.diff(off).max())
+    rounds = 1 + max(0, math.ceil(math.log2(lmax / 29)))
+    b_suffix = rounds * (2 * (8 + W) + 3 * W) + (W + 2 + (40 + 5 * W) / 64)
+    build_gbs = gsuffix_per_s * b_suffix
+    build_roofline = {"bound": "hbm", "achieved": round(build_gbs, 1), "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
+                      "model_bytes_per_suffix": round(b_suffix, 2), "rounds": rounds,
+                      "index_width": W}
 
     # dominant kernel: the candidate search (bidirectional backward search)
     avg_launch_ms = sum(find_launch_ms) / max(1, launches)
@@ -462,6 +478,7 @@ def main():
         "extraction_shape": {k: tune[k] for k in ("group_lanes", "items")},
         "find_kernel": tune["find_chains"],
         "roofline": roofline,
+        "build_roofline": build_roofline,
         "cpu_baseline": base,
         "learned": learned,
         "graph_mode": graph,
