@@ -216,6 +216,16 @@ def load_workload(pq, config, np):
     return codes, off, t, None, None, "as generated (uneven lengths: no duplicate removal)"
 
 
+def build_model_bytes(lmax, W):
+    """(rounds, bytes per suffix) of the prefix-doubling byte model of a suffix-array
+    build: R = 1 + ceil(log2(lmax / 29)) rounds from a 64-bit initial key of 29
+    two-bit codes and a clamped length, each round one key + index pass (read and
+    write, 2 * (8 + W)) and a rank read, gather and write (3 * W), then the BWT
+    gather and the occurrence write (W + 2 + (40 + 5 W) / 64); W = index width."""
+    rounds = 1 + max(0, math.ceil(math.log2(lmax / 29)))
+    return rounds, rounds * (2 * (8 + W) + 3 * W) + (W + 2 + (40 + 5 * W) / 64)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -388,9 +398,7 @@ def main():
     # (4 bytes for 32-bit indexes, 8 for the 64-bit index of the larger
     # workloads, whatever width the device rows have)
     W = 8 if args.config >= 3 else 4
-    lmax = int(np.diff(off).max())
-    rounds = 1 + max(0, math.ceil(math.log2(lmax / 29)))
-    b_suffix = rounds * (2 * (8 + W) + 3 * W) + (W + 2 + (40 + 5 * W) / 64)
+    rounds, b_suffix = build_model_bytes(int(np.diff(off).max()), W)
     build_gbs = gsuffix_per_s * b_suffix
     build_roofline = {"bound": "hbm", "achieved": round(build_gbs, 1), "peak": HBM_PEAK_GBS,
                       "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),

@@ -57,3 +57,22 @@ def test_synth_presets_and_determinism():
     small.seed = 4
     c, _ = pq.synth_reads(small)
     assert not np.array_equal(a[:1000], c[:1000])
+
+
+def test_build_byte_model_matches_its_worked_values():
+    # the prefix-doubling byte model the bench reports the suffix-array build
+    # against: 78.9 B per suffix for 50 bp reads and a 32-bit index (2 rounds),
+    # 235.3 B for 150 bp reads and a 64-bit index (4 rounds), exactly 78.9375 and
+    # 235.25; a dropped term, a wrong width or an off-by-one round count moves
+    # these
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "bench", os.path.join(os.path.dirname(__file__), "..", "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    r, b = bench.build_model_bytes(50, 4)
+    assert r == 2 and b == 78.9375
+    r, b = bench.build_model_bytes(150, 8)
+    assert r == 4 and b == 235.25
+    # 29 codes fit the initial key: one round
+    assert bench.build_model_bytes(29, 4)[0] == 1 and bench.build_model_bytes(30, 4)[0] == 2
