@@ -1,7 +1,9 @@
 // pq_search.hip -- FM-index overlap search (string_graph.c:162-731) on gfx950.
 //
 // Candidates (find_intervals, string_graph.c:162-383).  Four independent chains
-// per read, two per lane (the two of one index), two adjacent lanes per read:
+// per read: two per lane (the two of one index) on two adjacent lanes (k_find2),
+// or one per lane on four adjacent lanes, each lane loading its own lines
+// (k_find1) or the four lanes loading each line together (k_find1q):
 //   chain 0 FB  pattern n[L-o..L)      searched in the forward index
 //   chain 1 FF  comp(n[L-o..L))        searched in the backward index
 //   chain 2 BF  rev(n[0..o))           searched in the backward index
