@@ -2,29 +2,40 @@
 both directions and the FM-index overlap search, plain and seeded by the
 learned index, on a seeded synthetic workload.
 
-Default workload ("repeats and errors"): 5 Mbp genome with 100 interspersed
-copies of ten 1 kbp repeat units at 1% divergence, 100 bp reads at 40x with 0.5%
-substitutions, t = 63, 32-bit index.  --config 3 / 4: 50 Mbp genome, read
-lengths uniform in 36..250 at 30x, 0.3% substitutions, t = 31 (4 = the same
-reads with the learned index as the headline search).  --config 5: 200 Mbp
-genome with 5% repeat families, 150 bp reads at 30x (40 M reads, 6 G suffixes
-per direction: 64-bit device rows, suffix sort in slices), t = 63, learned index
-as the headline search.
+Default workload (the largest single-GPU one): 200 Mbp genome with 5% repeat
+families of 1-10 kbp at 0.5-3% divergence, 150 bp reads at 30x with 0.3%
+substitutions (40 M reads, 6 G suffixes per direction: 64-bit device rows,
+suffix sort in slices), t = 63.  The reads pass the reference's duplicate
+filter first (index.c:145-405, bloom.c:91-135: Bloom filter of 36 bits per
+record, 2 hashes, 64-bit index arithmetic), resolved on the device, so the
+read set is the one the reference's U64 build imports from the same reads.
+The headline search is the one seeded by the learned index.
 
-A step = one index build (SA, BWT occurrence lines, reads_index for the forward
-and the backward text) + one overlap search (candidate intervals, irreducible
-edges, edge CSR) over the reads, which are resident in HBM before the timed
-region.  The learned variant trains its model once on the built index and then
-times the same number of seeded searches.  Prints one JSON line.
+--config 2: 5 Mbp genome with 100 interspersed copies of ten 1 kbp repeat
+units at 1% divergence, 100 bp reads at 40x with 0.5% substitutions, t = 63,
+32-bit index, reads through the same import.  --config 3 / 4: 50 Mbp genome,
+read lengths uniform in 36..250 at 30x, 0.3% substitutions, t = 31 (4 = the
+same reads with the learned index as the headline search).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--config 2]
-                    [--cpu-baseline full|off] [--learned 1|0]
+A step = one index build (SA, BWT occurrence lines, reads_index for the
+forward and the backward text) + one plain overlap search + (where the learned
+search is the headline) the learned index trained on the freshly built
+suffix arrays + one search seeded by it.  A search ends with the node records
+and the edges (the reference's edge_t layout) in page-locked host memory: the
+reference's overlap-graph timer (string_graph.c:737-1044) ends with the
+graph in host memory too.  The reads are resident in HBM before the timed
+region.  Prints one JSON line; with the default workload it also carries the
+numbers of config 2 under "config2".
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--config 5]
+                    [--cpu-baseline full|off] [--learned 1|0] [--extra-config 2|0]
 """
 import argparse
 import json
 import math
 import os
 import re
+import statistics
 import subprocess
 import sys
 import tempfile
@@ -35,6 +46,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 LINE_BYTES = 64
+BLOOM_M, N_HASHFUNC = 36, 2          # the reference CLI's defaults (test.c:143-144)
 
 WORKLOADS = {
     1: "1 Mbp uniform genome, 50 bp error-free reads at 20x, t=31, U32 index",
@@ -50,6 +62,8 @@ WORKLOADS = {
 }
 # configurations whose headline search is the learned one
 LEARNED_HEADLINE = (4, 5)
+# index width of the reference build each configuration runs with
+INDEX_BITS = {1: 32, 2: 32, 3: 64, 4: 64, 5: 64}
 
 
 def log(msg):
@@ -70,40 +84,8 @@ def host_cpu():
     return {"model": model, "usable_cores": len(os.sched_getaffinity(0))}
 
 
-def cpu_baseline_reference(fasta, L, t, n_expected, exe_name="pasqual_omp_u32"):
-    """The reference OpenMP program (oracle/_ref) on the same reads file, timed
-    by its own timers (index.c:643-680, string_graph.c:737-1044)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pq_oracle as orc
-    exe = os.path.join(orc.REFDIR, exe_name)
-    if not os.path.exists(exe):
-        return None
-    threads = min(16, os.cpu_count() or 1)
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread")
-    with tempfile.TemporaryDirectory() as wd:
-        t0 = time.time()
-        r = subprocess.run([exe, "--singlereads", fasta, "-l", str(L), "-t", str(t)], cwd=wd,
-                           capture_output=True, text=True, env=env, timeout=900)
-        wall = time.time() - t0
-        digest = _digest(os.path.join(wd, "contigs.fa"))
-    if r.returncode != 0:
-        log("reference run failed: " + r.stderr[-500:])
-        return None
-    out = r.stdout
-    takes = [float(x) for x in re.findall(r"takes ([0-9.]+) ms", out)]
-    n = int(re.search(r"has imported (\d+) reads", out).group(1))
-    # takes: import, forward sort, backward sort, overlap graph, assembly
-    fw, bw, graph = takes[1], takes[2], takes[3]
-    return {"value": round(n / (graph / 1e3), 1), "unit": "reads/s", "cores": threads,
-            "import_ms": takes[0],
-            "kind": "reference",
-            "sample": f"the full workload ({n} reads), reference {exe_name}, "
-                      f"OMP_NUM_THREADS={threads}, one run, overlap-graph timer "
-                      "(string_graph.c:737-1044)", "reads": n,
-            "sa_build_gsuffix_per_s": round(2 * n * L / ((fw + bw) / 1e3) / 1e9, 5),
-            "wall_s": round(wall, 1),
-            "reads_match_gpu": None if n_expected is None else n == n_expected,
-            "contigs_sha256": digest}
+def usable_cores():
+    return len(os.sched_getaffinity(0))
 
 
 def _digest(path):
@@ -114,7 +96,58 @@ def _digest(path):
         return hashlib.sha256(f.read()).hexdigest()
 
 
-def dropin_run(fasta, L, t):
+def _reference_run(exe, fasta, L, t, threads, timeout):
+    """One run of a reference-driver program; (stdout timers, reads, contigs digest)."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread")
+    with tempfile.TemporaryDirectory() as wd:
+        t0 = time.time()
+        r = subprocess.run([exe, "--singlereads", fasta, "-l", str(L), "-t", str(t)], cwd=wd,
+                           capture_output=True, text=True, env=env, timeout=timeout)
+        wall = time.time() - t0
+        digest = _digest(os.path.join(wd, "contigs.fa"))
+    if r.returncode != 0:
+        log("reference run failed: " + r.stderr[-500:])
+        return None
+    takes = [float(x) for x in re.findall(r"takes ([0-9.]+) ms", r.stdout)]
+    n = int(re.search(r"has imported (\d+) reads", r.stdout).group(1))
+    return {"takes": takes, "n": n, "digest": digest, "wall": wall}
+
+
+def cpu_baseline_reference(fasta, L, t, n_expected, exe_name="pasqual_omp_u32", runs=3):
+    """The reference OpenMP program (oracle/_ref) on the same reads file, timed
+    by its own timers (index.c:643-680, string_graph.c:737-1044): the median of
+    `runs` runs with one OpenMP thread per usable core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pq_oracle as orc
+    exe = os.path.join(orc.REFDIR, exe_name)
+    if not os.path.exists(exe):
+        return None
+    threads = usable_cores()
+    res = []
+    for _ in range(runs):
+        r = _reference_run(exe, fasta, L, t, threads, 900)
+        if r is None:
+            return None
+        res.append(r)
+    n = res[0]["n"]
+    # takes: import, forward sort, backward sort, overlap graph, assembly
+    graph = statistics.median(r["takes"][3] for r in res)
+    build = statistics.median(r["takes"][1] + r["takes"][2] for r in res)
+    return {"value": round(n / (graph / 1e3), 1), "unit": "reads/s", "cores": threads,
+            "import_ms": statistics.median(r["takes"][0] for r in res),
+            "kind": "reference",
+            "sample": f"the full workload ({n} reads), reference {exe_name}, "
+                      f"OMP_NUM_THREADS={threads} (usable cores), median of {runs} runs, "
+                      "overlap-graph timer (string_graph.c:737-1044)", "reads": n,
+            "graph_ms_runs": [r["takes"][3] for r in res],
+            "sa_build_gsuffix_per_s": round(2 * n * L / (build / 1e3) / 1e9, 5),
+            "wall_s": round(sum(r["wall"] for r in res), 1),
+            "reads_match_gpu": None if n_expected is None else n == n_expected,
+            "contigs_sha256": res[0]["digest"],
+            "contigs_same_every_run": len({r["digest"] for r in res}) == 1}
+
+
+def dropin_run(fasta, L, t, width="u32"):
     """The reference's own driver, assembler and host files linked against this
     library (oracle/_ref/pasqual_hip_u32, built by `make -C oracle dropin`) on
     the same reads: its 'Building overlap graph' timer covers the device search,
@@ -122,10 +155,10 @@ def dropin_run(fasta, L, t):
     (string_graph.c:737-1044)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pq_oracle as orc
-    exe = os.path.join(orc.REFDIR, "pasqual_hip_u32")
+    exe = os.path.join(orc.REFDIR, f"pasqual_hip_{width}")
     if not os.path.exists(exe):
         return None
-    threads = min(16, os.cpu_count() or 1)
+    threads = usable_cores()
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), PQ_TIMING="1")
     with tempfile.TemporaryDirectory() as wd:
         r = subprocess.run([exe, "--singlereads", fasta, "-l", str(L), "-t", str(t)], cwd=wd,
@@ -170,50 +203,71 @@ def cpu_baseline_port(pq, config, t):
             "wall_s": round(ts + tb, 1)}
 
 
-def cpu_baseline_reference_sample(pq, config, t):
+def cpu_baseline_reference_sample(pq, config, t, genome_len=5_000_000):
     """The reference OpenMP program (pasqual_omp_u64) on a bounded sample of a
-    workload too large for it: the same generator over a 10 Mbp genome."""
+    workload too large for it: the same generator over a smaller genome."""
     cfg = pq.synth_preset(config)
-    cfg.genome_len = 10_000_000
+    cfg.genome_len = genome_len
     L = cfg.len_min
     wd = tempfile.mkdtemp(prefix="pqbase_")
     fasta = os.path.join(wd, "sample.fa")
     pq.synth_fasta(cfg, fasta)
     base = cpu_baseline_reference(fasta, L, t, None, exe_name="pasqual_omp_u64")
     os.remove(fasta)
+    os.rmdir(wd)
     if base:
-        base["sample"] = (f"{base['reads']} reads of the same generator over a 10 Mbp genome "
-                          "(same repeat families, coverage, length and errors), reference "
-                          f"pasqual_omp_u64, OMP_NUM_THREADS={base['cores']}, one run, "
+        base["sample"] = (f"{base['reads']} reads of the same generator over a "
+                          f"{genome_len // 1_000_000} Mbp genome (same repeat families, coverage, "
+                          "length and errors), reference pasqual_omp_u64, "
+                          f"OMP_NUM_THREADS={base['cores']} (usable cores), median of 3 runs, "
                           "overlap-graph timer (string_graph.c:737-1044)")
     return base
 
 
-# reads import of the fixed-length workloads (file read, records cut on the
-# host, duplicate filter on the device), filled by load_workload
-IMPORT_INFO = {}
+def dedup_in_memory(pq, ctx, codes, off, index_bits):
+    """The reads import's duplicate filter (index.c:327-333, bloom.c:91-135) over
+    generated fixed-length reads, on the device: the reads the reference keeps
+    when it imports the same reads from a 2-line FASTA (filter of 36 bits per
+    record, 2 hashes, hashes in the index width)."""
+    import numpy as np
+    n = len(off) - 1
+    L = int(off[1] - off[0])
+    t0 = time.perf_counter()
+    keep, passes = ctx.dedup_reads(codes, np.ones(n, np.uint8), L, n * BLOOM_M, N_HASHFUNC,
+                                   index_bits)
+    kept = np.flatnonzero(keep)
+    out = codes.reshape(n, L)[kept].reshape(-1)
+    ms = (time.perf_counter() - t0) * 1e3
+    return out, np.arange(len(kept) + 1, dtype=np.uint64) * np.uint64(L), {
+        "records": n, "kept": int(len(kept)), "passes": passes, "ms": round(ms, 1),
+        "filter_bits": n * BLOOM_M, "index_bits": index_bits}
 
 
 def load_workload(pq, config, np):
+    """(codes, off, t, L or None, FASTA path or None, import record, read set note)."""
     cfg = pq.synth_preset(config)
     t = cfg.min_overlap
     if config == 5:
-        # 40 M reads: the generator's reads as they are (the 6.8 GB reads file
-        # and its import are not part of the measured path)
+        # 40 M reads: the 6.8 GB reads file is not written; the generated reads
+        # pass the same duplicate filter on the device (dedup_in_memory)
         codes, off = pq.synth_reads(cfg)
-        return codes, off, t, None, None, "as generated (no duplicate removal)"
+        with pq.Context(0) as c:
+            codes, off, info = dedup_in_memory(pq, c, codes, off, INDEX_BITS[config])
+        return codes, off, t, None, None, info, \
+            "after the reference's duplicate filter (Bloom filter resolved on the device)"
     if cfg.len_min == cfg.len_max:
         # fixed-length reads go through the reference's import (duplicate removal)
         L = cfg.len_min
         fasta = os.path.join(tempfile.mkdtemp(prefix="pqbench_"), "reads.fa")
         pq.synth_fasta(cfg, fasta)
-        imp = pq.import_fasta(fasta, L, index_bits=32)
+        imp = pq.import_fasta(fasta, L, index_bits=INDEX_BITS[config])
         n = imp["n"]
-        IMPORT_INFO.update(records=n + imp["removed"], kept=n, ms=round(imp["ms"], 1))
         off = np.arange(n + 1, dtype=np.uint64) * np.uint64(L)
-        return imp["reads"], off, t, L, fasta, "after duplicate removal"
+        return imp["reads"], off, t, L, fasta, \
+            {"records": n + imp["removed"], "kept": n, "ms": round(imp["ms"], 1)}, \
+            "after duplicate removal (reads import from the FASTA file)"
     codes, off = pq.synth_reads(cfg)
-    return codes, off, t, None, None, "as generated (uneven lengths: no duplicate removal)"
+    return codes, off, t, None, None, None, "as generated (uneven lengths: no duplicate removal)"
 
 
 def build_model_bytes(lmax, W):
@@ -226,153 +280,199 @@ def build_model_bytes(lmax, W):
     return rounds, rounds * (2 * (8 + W) + 3 * W) + (W + 2 + (40 + 5 * W) / 64)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=2, choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-baseline", default="full", choices=["full", "off"])
-    ap.add_argument("--learned", type=int, default=1,
-                    help="also time the learned-index seeded search (1) or not (0)")
-    ap.add_argument("--probe-copy", type=int, default=0,
-                    help="also run one plain device copy of this many MiB (counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the overlap search shards by read ranges with no exchange; see the "
-              "design notes)", file=sys.stderr)
-        sys.exit(2)
-    if args.config in LEARNED_HEADLINE:
-        args.learned = 1
+def _digest_buf(arr):
+    """Digest of a host buffer (xxh3-128 when xxhash is importable, else SHA-256)."""
+    try:
+        import xxhash
+        return "xxh3_128:" + xxhash.xxh3_128(memoryview(arr)).hexdigest()
+    except ImportError:
+        import hashlib
+        return "sha256:" + hashlib.sha256(memoryview(arr)).hexdigest()
 
-    import numpy as np
-    import pasqual_amd as pq
+
+def _mean(xs):
+    return sum(xs) / len(xs) if xs else 0.0
+
+
+def search_roofline(positions, lines, kernel_ms, launches, copy_gbs, read_gbs, gather, kname,
+                    n, steps):
+    """Roofline of the candidate-search kernel from its live HIP-event launch times:
+    algorithmic bytes = 64 B per distinct occurrence line a backward-search step
+    needs (one line when the rows l and u+1 share a line, else two; counted by the
+    kernel, the same count as the oracle's restatement of the chains), against
+    the HBM peak; also the 64 B per rank position form and the measured
+    ceilings."""
+    avg_launch_ms = kernel_ms / max(1, launches)
+    alg = lines / max(1, launches) * LINE_BYTES
+    pos_bytes = positions / max(1, launches) * LINE_BYTES
+    achieved = alg / (avg_launch_ms / 1e3) / 1e9 if avg_launch_ms else 0.0
+    best = max(gather["lane"], gather["quad"]) if gather else None
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": kname, "alg_bytes_per_launch": int(alg),
+            "alg_bytes_basis": "64 B x distinct occurrence lines per backward-search step",
+            "lines_per_read": round(lines / (n * steps), 1),
+            "rank_positions_per_read": round(positions / (n * steps), 1),
+            "position_bytes_per_launch": int(pos_bytes),
+            "position_bytes_gbs": round(pos_bytes / (avg_launch_ms / 1e3) / 1e9, 1)
+            if avg_launch_ms else None,
+            "avg_launch_ms": round(avg_launch_ms, 3),
+            "launches_per_search": round(launches / steps, 2),
+            "copy_ceiling": copy_gbs, "read_ceiling": read_gbs,
+            "frac_of_read_ceiling": round(achieved / read_gbs, 4) if read_gbs else None,
+            "scattered_line_ceilings": gather,
+            "frac_of_scattered_ceiling": round(achieved / best, 4) if best else None}
+
+
+def committed_traffic(config, kname):
+    """HBM bytes per launch of kernel kname on this configuration from the committed
+    counter summary (profiles/config<N>_traffic.json: FETCH_SIZE and WRITE_SIZE
+    per dispatch, each scaled by what the counter reads for the plain copy
+    kernel), or None."""
+    tpath = os.path.join(ROOT, "profiles", f"config{config}_traffic.json")
+    if not os.path.exists(tpath):
+        return None
+    with open(tpath) as f:
+        kernels = json.load(f)["kernels"]
+    tk = next((v for k, v in kernels.items() if k == kname or k.startswith(kname + "<")), None)
+    return tk["traffic_bytes"][0] if tk else None
+
+
+def run_workload(pq, np, config, args, cpu_baseline):
     t0 = time.time()
-    codes, off, t, L, fasta, how = load_workload(pq, args.config, np)
+    codes, off, t, L, fasta, imp, how = load_workload(pq, config, np)
     n = len(off) - 1
     N = int(off[-1])
-    log(f"workload config {args.config}: {n} reads {how}, N={N} per direction, "
+    log(f"workload config {config}: {n} reads {how}, N={N} per direction, "
         f"prepared in {time.time() - t0:.1f}s")
+    headline_learned = config in LEARNED_HEADLINE
+    learned_on = bool(args.learned) or headline_learned
+    index_bytes = INDEX_BITS[config] // 8
+    rec_bytes = 8 if index_bytes == 4 else 16
 
     ctx = pq.Context(0)
     ctx.upload(codes, off, t)
+    del codes
     for _ in range(args.warmup):
         ctx.build_index()
         ctx.find_overlaps()
     if args.warmup == 0:
         ctx.build_index()
     ctx.sync()
-    # untimed: pick the extraction shape for this read set (identical results)
+    # untimed: the extraction shape and search kernel for this read set (identical
+    # results), for the plain search and, after a training, the learned one
     tune = ctx.autotune()
-    log(f"warm-up done; extraction shape {tune}")
-    build_ms, search_ms, find_launch_ms = [], [], []
-    positions = lines = launches = 0
+    tune_l = None
+    if learned_on:
+        ctx.learned_train()
+        tune_l = ctx.autotune(pq.SEARCH_LEARNED)
+    log(f"warm-up done; plain tuning {tune}; learned tuning {tune_l}")
+    _, E = ctx.graph_size()
+    nodes_buf = ctx.host_buffer(max(1, n) * 8)
+    edges_buf = ctx.host_buffer(max(1, E) * rec_bytes)
+
+    def use(tn):
+        ctx.set_tuning(tn["shape"])
+        ctx.set_find_chains(tn["find_chains"])
+
+    def export():
+        nonlocal edges_buf
+        _, e = ctx.graph_size()
+        if e * rec_bytes > edges_buf.nbytes:
+            edges_buf.free()
+            edges_buf = ctx.host_buffer(e * rec_bytes)
+        return ctx.graph_export(nodes_buf, edges_buf, index_bytes)
+
+    P = {"build": [], "search": [], "search_dev": [], "export": [], "kernel": [], "launches": 0,
+         "positions": 0, "lines": 0}
+    Lr = {"train": [], "search": [], "search_dev": [], "export": [], "kernel": [], "seed": [],
+          "launches": 0, "positions": 0, "lines": 0, "same": True}
+    digests = {}
     t_start = time.perf_counter()
-    for _ in range(args.steps):
+    for step in range(args.steps):
         a = time.perf_counter()
         ctx.build_index()
         ctx.sync()
         b = time.perf_counter()
+        use(tune)
         ctx.find_overlaps()
-        ctx.sync()
         c = time.perf_counter()
         tm = ctx.timing()
-        build_ms.append((b - a) * 1e3)
-        search_ms.append((c - b) * 1e3)
-        find_launch_ms.append(tm["find_kernel_ms"])
-        launches += tm["find_launches"]
-        positions += tm["find_positions"]
-        lines += tm["find_lines"]
+        len_sum = export()
+        d = time.perf_counter()
+        P["build"].append((b - a) * 1e3)
+        P["search"].append((d - b) * 1e3)
+        P["search_dev"].append((c - b) * 1e3)
+        P["export"].append((d - c) * 1e3)
+        P["kernel"].append(tm["find_kernel_ms"])
+        P["launches"] += tm["find_launches"]
+        P["positions"] += tm["find_positions"]
+        P["lines"] += tm["find_lines"]
+        if step == 0:
+            _, E = ctx.graph_size()
+            digests["plain"] = (_digest_buf(nodes_buf.array), _digest_buf(
+                edges_buf.array[:E * rec_bytes]), len_sum)
+        if learned_on:
+            e0 = time.perf_counter()
+            ctx.learned_train()
+            e1 = time.perf_counter()
+            use(tune_l)
+            ctx.find_overlaps(pq.SEARCH_LEARNED)
+            e2 = time.perf_counter()
+            tl = ctx.timing()
+            ls = export()
+            e3 = time.perf_counter()
+            Lr["train"].append((e1 - e0) * 1e3)
+            Lr["search"].append((e3 - e1) * 1e3)
+            Lr["search_dev"].append((e2 - e1) * 1e3)
+            Lr["export"].append((e3 - e2) * 1e3)
+            Lr["kernel"].append(tl["find_kernel_ms"])
+            Lr["seed"].append(tl["learned_ms"])
+            Lr["launches"] += tl["find_launches"]
+            Lr["positions"] += tl["find_positions"]
+            Lr["lines"] += tl["find_lines"]
+            if step == 0:
+                _, E = ctx.graph_size()
+                got = (_digest_buf(nodes_buf.array), _digest_buf(
+                    edges_buf.array[:E * rec_bytes]), ls)
+                Lr["same"] = got == digests["plain"]
     ctx.sync()
     total_s = time.perf_counter() - t_start
-    g = ctx.graph()
-    edges = int(g["edge_off"][-1])
-    log("plain steps done")
-    # the same search as one HIP graph (captured by the first, untimed call)
-    ctx.find_overlaps(pq.SEARCH_GRAPH)
-    graph_ms = []
-    for _ in range(args.steps):
-        a = time.perf_counter()
-        ctx.find_overlaps(pq.SEARCH_GRAPH)
-        ctx.sync()
-        graph_ms.append((time.perf_counter() - a) * 1e3)
-    gg = ctx.graph()
-    graph = {"plain_search_ms": round(sum(graph_ms) / len(graph_ms), 3),
-             "plain_edges_identical": all(np.array_equal(gg[k], g[k]) for k in g)}
-    log("graph-mode steps done")
+    _, edges = ctx.graph_size()
+    log("timed steps done")
+    st = ctx.learned_stats() if learned_on else None
 
-    # learned-index variant on the same index: train once on the built SA,
-    # then the same number of timed searches seeded by it
-    learned = None
-    if args.learned:
-        ctx.build_index()
-        ctx.learned_train()
-        tune_l = ctx.autotune(pq.SEARCH_LEARNED)
-        log(f"learned search tuned: {tune_l}")
-        gl = ctx.graph()
-        same = all(np.array_equal(gl[k], g[k]) for k in g)
-        ctx.sync()
-        l_ms, l_seed, l_find, l_pos = [], [], [], 0
-        for _ in range(args.steps):
+    # the same searches as one HIP graph each (captured by the first, untimed call)
+    reps = min(args.steps, 5)
+    graph = {}
+    for name, mode, tn in (("plain", pq.SEARCH_PLAIN, tune),
+                           ("learned", pq.SEARCH_LEARNED, tune_l)):
+        if tn is None:
+            continue
+        use(tn)
+        ctx.find_overlaps(mode | pq.SEARCH_GRAPH)
+        ms = []
+        for _ in range(reps):
             a = time.perf_counter()
-            ctx.find_overlaps(pq.SEARCH_LEARNED)
-            ctx.sync()
-            l_ms.append((time.perf_counter() - a) * 1e3)
-            tm = ctx.timing()
-            l_seed.append(tm["learned_ms"])
-            l_find.append(tm["find_kernel_ms"])
-            l_pos += tm["find_positions"]
-        st = ctx.learned_stats()
-        ctx.find_overlaps(pq.SEARCH_LEARNED | pq.SEARCH_GRAPH)
-        lg_ms = []
-        for _ in range(args.steps):
-            a = time.perf_counter()
-            ctx.find_overlaps(pq.SEARCH_LEARNED | pq.SEARCH_GRAPH)
-            ctx.sync()
-            lg_ms.append((time.perf_counter() - a) * 1e3)
-        gl = ctx.graph()
-        graph["learned_search_ms"] = round(sum(lg_ms) / len(lg_ms), 3)
-        graph["learned_edges_identical"] = all(np.array_equal(gl[k], g[k]) for k in g)
-        avg_l = sum(l_ms) / len(l_ms)
-        lookups = sum(st["lookups"])
-        learned = {"reads_per_s": round(n / (avg_l / 1e3), 1), "search_ms": round(avg_l, 3),
-                   "seed_kernel_ms": round(sum(l_seed) / len(l_seed), 3),
-                   "find_kernel_ms": round(sum(l_find) / len(l_find), 3),
-                   "rank_positions_per_read": round(l_pos / (n * args.steps), 1),
-                   "seed_depth": st["depth"], "model_k": st["k"],
-                   "table_size": list(st["table_size"]),
-                   "mean_window": [round(x, 2) for x in st["mean_window"]],
-                   "max_window": list(st["max_window"]),
-                   "probes_per_lookup": round(sum(st["probes"]) / max(1, lookups), 2),
-                   "absent_chains": st["absent"], "fallback_chains": st["fallback"],
-                   "train_ms": round(st["train_ms"], 1),
-                   "fit_loss": [float(f"{x:.3g}") for x in st["fit_loss"]],
-                   "train_loss": [float(f"{x:.3g}") for x in st["train_loss"]],
-                   "edges_identical_to_plain": bool(same),
-                   "extraction_shape": {k: tune_l[k] for k in ("group_lanes", "items")},
-                   "find_kernel": tune_l["find_chains"]}
-        log("learned steps done")
-    if args.probe_copy:
-        pms = ctx.probe_copy(args.probe_copy << 20)
-        log(f"copy probe: {args.probe_copy} MiB in {pms:.3f} ms "
-            f"({2 * (args.probe_copy << 20) / pms / 1e6:.1f} GB/s read+write)")
+            ctx.find_overlaps(mode | pq.SEARCH_GRAPH)
+            ms.append((time.perf_counter() - a) * 1e3)
+        export()
+        _, E = ctx.graph_size()
+        graph[f"{name}_search_device_ms"] = round(_mean(ms), 3)
+        graph[f"{name}_edges_identical"] = (
+            (_digest_buf(nodes_buf.array), _digest_buf(edges_buf.array[:E * rec_bytes]))
+            == digests["plain"][:2])
+    log("graph-mode searches done")
+
     # measured streaming ceilings of this GPU, reported beside the published HBM
-    # peak (untimed region, best of 3): a copy (1 GiB read + 1 GiB written) and a
-    # read of 1 GiB (the candidate search only reads)
-    copy_gbs = read_gbs = None
+    # peak (untimed, best of 3): a copy (1 GiB read + 1 GiB written) and a read of
+    # 1 GiB; scattered 64-byte line reads over a buffer the size of both indexes'
+    # occurrence lines (whole lines per lane, independent and in dependent
+    # chains, and four lanes per line)
+    copy_gbs = read_gbs = gather = None
     try:
         copy_gbs = round(2 * (1 << 30) / min(ctx.probe_copy(1 << 30) for _ in range(3)) / 1e6, 1)
         read_gbs = round((1 << 30) / min(ctx.probe_read(1 << 30) for _ in range(3)) / 1e6, 1)
-    except pq.PqError as e:
-        log(f"streaming ceilings not measured: {e}")
-    # scattered 64-byte line reads over a buffer the size of both indexes'
-    # occurrence lines (0.5 B per row and direction): independent positions, and
-    # chains of dependent loads as a backward search makes them
-    # (GB/s of lines, best of 3): whole lines per lane (positions independent, and
-    # in dependent chains) and four lanes per line, 16 bytes each
-    gather = None
-    try:
         occ_bytes = 2 * ((N + 127) // 128) * LINE_BYTES
         nbytes = (1 << 25) * LINE_BYTES
         gather = {name: round(nbytes / min(ctx.probe_gather(occ_bytes, mode) for _ in range(3))
@@ -381,85 +481,173 @@ def main():
         gather["footprint_bytes"] = occ_bytes
         log(f"scattered line reads over {occ_bytes / 1e9:.2f} GB: {gather}")
     except pq.PqError as e:
-        log(f"scattered-line ceilings not measured: {e}")
+        log(f"ceilings not measured: {e}")
+    if args.probe_copy:
+        pms = ctx.probe_copy(args.probe_copy << 20)
+        log(f"copy probe: {args.probe_copy} MiB in {pms:.3f} ms")
     row_bytes, slices = ctx.layout()
     lens = np.diff(off)
     read_len = int(lens[0]) if int(lens.min()) == int(lens.max()) else \
         f"{int(lens.min())}..{int(lens.max())}"
-    ms_per_step = total_s * 1e3 / args.steps
-    avg_search = sum(search_ms) / len(search_ms)
-    avg_build = sum(build_ms) / len(build_ms)
-    reads_per_s = n / (avg_search / 1e3)
+    kname = {1: "k_find1", 2: "k_find2", 3: "k_find1q"}
+    avg_build = _mean(P["build"])
+    avg_search = _mean(P["search"])
     gsuffix_per_s = 2 * N / (avg_build / 1e3) / 1e9
-    # the suffix-array build against the HBM peak by a prefix-doubling byte
-    # model: R = 1 + ceil(log2(Lmax / 29)) rounds from a 64-bit key of 29 codes,
-    # each one key + index pass (read and write) plus a rank read, gather and
-    # write, then the BWT gather and the occurrence write; W = the index width
-    # (4 bytes for 32-bit indexes, 8 for the 64-bit index of the larger
-    # workloads, whatever width the device rows have)
-    W = 8 if args.config >= 3 else 4
-    rounds, b_suffix = build_model_bytes(int(np.diff(off).max()), W)
+    # the suffix-array build against the HBM peak by the prefix-doubling byte model;
+    # W = the width of the device rows this build used
+    W = row_bytes
+    rounds, b_suffix = build_model_bytes(int(lens.max()), W)
     build_gbs = gsuffix_per_s * b_suffix
     build_roofline = {"bound": "hbm", "achieved": round(build_gbs, 1), "peak": HBM_PEAK_GBS,
                       "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
                       "model_bytes_per_suffix": round(b_suffix, 2), "rounds": rounds,
                       "index_width": W}
-
-    # dominant kernel: the candidate search (bidirectional backward search)
-    avg_launch_ms = sum(find_launch_ms) / max(1, launches)
-    per_launch_positions = positions / max(1, launches)
-    alg_bytes = per_launch_positions * LINE_BYTES
-    achieved = alg_bytes / (avg_launch_ms / 1e3) / 1e9
-    # HBM/fabric bytes per k_find launch from the committed counter runs of this
-    # same workload: FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for
-    # the plain copy kernel
-    # (the autotuner's choice of search kernel)
-    kname = {1: "k_find1", 2: "k_find2", 3: "k_find1q"}[tune["find_chains"]]
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"config{args.config}_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            kernels = json.load(f)["kernels"]
-        tk = next((v for k, v in kernels.items() if k == kname or k.startswith(kname + "<")),
-                  None)
-        if tk:
-            traffic = tk["traffic_bytes"][0]
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": kname, "alg_bytes_per_launch": int(alg_bytes),
-                "rank_positions_per_read": round(positions / (n * args.steps), 1),
-                "lines_per_read": round(lines / (n * args.steps), 1),
-                "avg_launch_ms": round(avg_launch_ms, 3),
-                "launches_per_search": round(launches / args.steps, 2)}
-    roofline["copy_ceiling"] = copy_gbs
-    roofline["read_ceiling"] = read_gbs
-    roofline["frac_of_read_ceiling"] = round(achieved / read_gbs, 4) if read_gbs else None
-    # the search's distinct line loads (one or two per step) against the
-    # scattered-line ceilings of the same footprint
-    line_gbs = lines * LINE_BYTES / (sum(find_launch_ms) / 1e3) / 1e9 if sum(find_launch_ms) else None
-    roofline["line_gbs"] = round(line_gbs, 1) if line_gbs else None
-    roofline["scattered_line_ceilings"] = gather
-    best = max(gather["lane"], gather["quad"]) if gather else None
-    roofline["lines_frac_of_scattered_ceiling"] = round(line_gbs / best, 4) \
-        if line_gbs and best else None
+    plain_roof = search_roofline(P["positions"], P["lines"], sum(P["kernel"]), P["launches"],
+                                 copy_gbs, read_gbs, gather, kname[tune["find_chains"]], n,
+                                 args.steps)
+    plain_roof["traffic"] = committed_traffic(config, plain_roof["kernel"])
+    res = {
+        "config": config, "n": n, "N": N, "t": t, "read_len": read_len, "row_bytes": row_bytes,
+        "slices": slices, "how": how, "import": imp, "steps": args.steps,
+        "ms_per_step": total_s * 1e3 / args.steps,
+        "plain_search_reads_per_s": round(n / (avg_search / 1e3), 1),
+        "plain_search_ms": round(avg_search, 3),
+        "plain_search_device_ms": round(_mean(P["search_dev"]), 3),
+        "plain_export_ms": round(_mean(P["export"]), 3),
+        "sa_build_gsuffix_per_s": round(gsuffix_per_s, 3),
+        "build_ms": round(avg_build, 3),
+        "edges": edges, "edge_record_bytes": rec_bytes,
+        "edges_digest": digests["plain"][1], "nodes_digest": digests["plain"][0],
+        "overlap_len_sum": digests["plain"][2],
+        "extraction_shape": {k: tune[k] for k in ("group_lanes", "items")},
+        "find_kernel": tune["find_chains"],
+        "plain_roofline": plain_roof, "build_roofline": build_roofline, "graph_mode": graph,
+    }
+    if learned_on:
+        lsearch = _mean(Lr["search"])
+        lookups = sum(st["lookups"])
+        lroof = search_roofline(Lr["positions"], Lr["lines"], sum(Lr["kernel"]),
+                                Lr["launches"], copy_gbs, read_gbs, gather,
+                                kname[tune_l["find_chains"]], n, args.steps)
+        lroof["traffic"] = committed_traffic(config, lroof["kernel"] + "_seeded")
+        train = _mean(Lr["train"])
+        res["learned"] = {
+            "reads_per_s": round(n / (lsearch / 1e3), 1), "search_ms": round(lsearch, 3),
+            "search_device_ms": round(_mean(Lr["search_dev"]), 3),
+            "export_ms": round(_mean(Lr["export"]), 3),
+            "seed_kernel_ms": round(_mean(Lr["seed"]), 3),
+            "find_kernel_ms": round(_mean(Lr["kernel"]), 3),
+            "train_ms": round(train, 1),
+            "train_ms_in_library": round(st["train_ms"], 1),
+            "seed_depth": st["depth"], "model_k": st["k"],
+            "table_size": list(st["table_size"]),
+            "mean_window": [round(x, 2) for x in st["mean_window"]],
+            "max_window": list(st["max_window"]),
+            "probes_per_lookup": round(sum(st["probes"]) / max(1, lookups), 2),
+            "absent_chains": st["absent"],
+            "fit_loss": [float(f"{x:.3g}") for x in st["fit_loss"]],
+            "train_loss": [float(f"{x:.3g}") for x in st["train_loss"]],
+            "edges_identical_to_plain": bool(Lr["same"]),
+            "extraction_shape": {k: tune_l[k] for k in ("group_lanes", "items")},
+            "find_kernel": tune_l["find_chains"], "roofline": lroof,
+        }
+        res["learned_vs_plain_search"] = round(avg_search / lsearch, 3)
+        # build-once / search-once pipeline: the learned search pays for its table
+        # and its training on every freshly built index
+        res["learned_end_to_end"] = {
+            "plain_ms": round(avg_search, 3), "learned_ms": round(train + lsearch, 3),
+            "train_ms": round(train, 1), "search_ms": round(lsearch, 3),
+            "learned_vs_plain": round(avg_search / (train + lsearch), 3)}
+    nodes_buf.free()
+    edges_buf.free()
+    ctx.close()
 
     base = dropin = None
-    if args.cpu_baseline == "full":
-        log("timing the CPU baseline ...")
+    if cpu_baseline:
+        log(f"timing the CPU baseline of config {config} ...")
         if L is not None:
-            base = cpu_baseline_reference(fasta, L, t, n)
+            base = cpu_baseline_reference(fasta, L, t, n,
+                                          exe_name=f"pasqual_omp_u{INDEX_BITS[config]}")
             log("timing the reference driver on this library ...")
-            dropin = dropin_run(fasta, L, t)
+            dropin = dropin_run(fasta, L, t, width=f"u{INDEX_BITS[config]}")
             if dropin and base:
                 dropin["contigs_identical_to_reference"] = (
                     dropin["contigs_sha256"] is not None and
                     dropin["contigs_sha256"] == base["contigs_sha256"])
-        elif args.config == 5:
-            base = cpu_baseline_reference_sample(pq, args.config, t)
+        elif config == 5:
+            base = cpu_baseline_reference_sample(pq, config, t)
         else:
-            base = cpu_baseline_port(pq, args.config, t)
-    headline_learned = args.config in LEARNED_HEADLINE
-    value = learned["reads_per_s"] if headline_learned else reads_per_s
+            base = cpu_baseline_port(pq, config, t)
+        if base:
+            base["host"] = host_cpu()
+            if imp and base.get("import_ms"):
+                imp["reference_ms"] = base["import_ms"]
+    if fasta:
+        os.remove(fasta)
+        os.rmdir(os.path.dirname(fasta))
+    res["cpu_baseline"] = base
+    res["dropin"] = dropin
+    return res
+
+
+def summary(res):
+    """The numbers of one configuration carried as extra keys of the line."""
+    keys = ("plain_search_reads_per_s", "plain_search_ms", "plain_search_device_ms",
+            "sa_build_gsuffix_per_s", "build_ms", "edges", "edges_digest", "find_kernel",
+            "extraction_shape", "learned_vs_plain_search", "learned_end_to_end", "dropin",
+            "import", "graph_mode")
+    out = {"workload": f"config {res['config']}: {WORKLOADS[res['config']]}", "reads": res["n"],
+           "suffixes_per_direction": res["N"], "ms_per_step": round(res["ms_per_step"], 3)}
+    out.update({k: res[k] for k in keys if k in res})
+    out["search_roofline"] = res["plain_roofline"]
+    out["build_roofline"] = res["build_roofline"]
+    if "learned" in res:
+        out["learned_reads_per_s"] = res["learned"]["reads_per_s"]
+        out["learned_search_ms"] = res["learned"]["search_ms"]
+        out["learned_edges_identical_to_plain"] = res["learned"]["edges_identical_to_plain"]
+    base = res.get("cpu_baseline")
+    if base:
+        out["cpu_baseline"] = base
+        out["speedup_vs_cpu_search"] = round(res["plain_search_reads_per_s"] / base["value"], 1)
+        out["speedup_vs_cpu_sa_build"] = round(
+            res["sa_build_gsuffix_per_s"] / base["sa_build_gsuffix_per_s"], 1)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=5, choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-baseline", default="full", choices=["full", "off"])
+    ap.add_argument("--learned", type=int, default=1,
+                    help="also time the learned-index seeded search (1) or not (0)")
+    ap.add_argument("--extra-config", type=int, default=2, choices=[0, 1, 2, 3, 4],
+                    help="also measure this configuration and report it under config<N> "
+                         "(0: none; only with the default workload)")
+    ap.add_argument("--probe-copy", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the overlap search shards by read ranges with no exchange; see the "
+              "design notes)", file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1:
+        print("--steps must be at least 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import pasqual_amd as pq
+    res = run_workload(pq, np, args.config, args, args.cpu_baseline == "full")
+    extra = None
+    if args.extra_config and args.extra_config != args.config and args.config == 5:
+        extra = run_workload(pq, np, args.extra_config, args, args.cpu_baseline == "full")
+
+    headline_learned = args.config in LEARNED_HEADLINE and "learned" in res
+    value = res["learned"]["reads_per_s"] if headline_learned else res["plain_search_reads_per_s"]
+    roofline = res["learned"]["roofline"] if headline_learned else res["plain_roofline"]
     line = {
         "metric": "overlap-search reads/s + SA build Gsuffix/s, learned vs plain index, 1 GPU",
         "value": round(value, 1),
@@ -467,42 +655,53 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(res["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u64" if row_bytes == 8 else "u32",
+        "dtype": "u64" if res["row_bytes"] == 8 else "u32",
         "data": "synthetic (seeded generator, pasqual_amd/csrc/host/pq_synth.c)",
         "config": {"workload": f"config {args.config}: {WORKLOADS[args.config]}",
-                   "reads": n, "suffixes_per_direction": N,
-                   "read_len": read_len, "min_overlap": t,
-                   "device_rows": f"{8 * row_bytes}-bit", "sa_slices_per_direction": slices,
-                   "search": "learned" if headline_learned else "plain"},
-        "plain_search_reads_per_s": round(reads_per_s, 1),
-        "sa_build_gsuffix_per_s": round(gsuffix_per_s, 3),
-        "search_ms": round(avg_search, 3),
-        "build_ms": round(avg_build, 3),
-        "edges": edges,
-        "extraction_shape": {k: tune[k] for k in ("group_lanes", "items")},
-        "find_kernel": tune["find_chains"],
+                   "reads": res["n"], "read_set": res["how"],
+                   "suffixes_per_direction": res["N"], "read_len": res["read_len"],
+                   "min_overlap": res["t"], "device_rows": f"{8 * res['row_bytes']}-bit",
+                   "sa_slices_per_direction": res["slices"],
+                   "search": "learned" if headline_learned else "plain",
+                   "step": "index build + plain search" +
+                           (" + learned-index training + learned search"
+                            if "learned" in res else "") +
+                           "; each search ends with node records and edge_t records in "
+                           "page-locked host memory"},
+        "sa_build_gsuffix_per_s": res["sa_build_gsuffix_per_s"],
+        "build_ms": res["build_ms"],
+        "plain_search_reads_per_s": res["plain_search_reads_per_s"],
+        "plain_search_ms": res["plain_search_ms"],
+        "plain_search_device_ms": res["plain_search_device_ms"],
+        "plain_export_ms": res["plain_export_ms"],
+        "edges": res["edges"],
+        "edges_digest": res["edges_digest"],
+        "nodes_digest": res["nodes_digest"],
+        "overlap_len_sum": res["overlap_len_sum"],
+        "extraction_shape": res["extraction_shape"],
+        "find_kernel": res["find_kernel"],
         "roofline": roofline,
-        "build_roofline": build_roofline,
-        "cpu_baseline": base,
-        "learned": learned,
-        "graph_mode": graph,
-        "dropin": dropin,
-        "import": dict(IMPORT_INFO) or None,
+        "plain_roofline": res["plain_roofline"],
+        "build_roofline": res["build_roofline"],
+        "cpu_baseline": res["cpu_baseline"],
+        "learned": res.get("learned"),
+        "learned_vs_plain_search": res.get("learned_vs_plain_search"),
+        "learned_end_to_end": res.get("learned_end_to_end"),
+        "graph_mode": res["graph_mode"],
+        "dropin": res["dropin"],
+        "import": res["import"],
     }
-    if learned:
-        line["learned_vs_plain_search"] = round(learned["reads_per_s"] / reads_per_s, 3)
-    if base:
-        base["host"] = host_cpu()
-    if base and IMPORT_INFO and base.get("import_ms"):
-        line["import"]["reference_ms"] = base["import_ms"]
+    base = res["cpu_baseline"]
     if base:
         line["speedup_vs_cpu_search"] = round(value / base["value"], 1)
-        line["speedup_vs_cpu_sa_build"] = round(gsuffix_per_s / base["sa_build_gsuffix_per_s"], 1)
-    ctx.close()
+        line["speedup_vs_cpu_sa_build"] = round(
+            res["sa_build_gsuffix_per_s"] / base["sa_build_gsuffix_per_s"], 1)
+    if extra:
+        line[f"config{args.extra_config}"] = summary(extra)
     print(json.dumps(line), flush=True)
 
 

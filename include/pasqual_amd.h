@@ -131,6 +131,8 @@ int pq_graph_copy(pq_ctx_t *ctx, int32_t *overlaps_B, int32_t *overlaps_F,
  * index_bytes 4, 16 for 8), in edge_off order: what construct_string_graph copies
  * into the node records. */
 int pq_graph_copy_edges(pq_ctx_t *ctx, void *edges, uint32_t index_bytes);
+/* Bytes of one edge_t record: 8 for index_bytes 4, 16 for 8. */
+#define PQ_EDGE_BYTES(index_bytes) ((index_bytes) == 4 ? 8u : 16u)
 
 /* Node record of one read as construct_string_graph sets it (string_graph.c:
  * 832-889), with the reference's values (string_graph.h; PQ_REF_* of
@@ -149,6 +151,19 @@ typedef struct {
  * 1051).  PQ_ERR_LIMIT if a node has more than 65535 edges on one side
  * (string_graph.c:469-472). */
 int pq_graph_nodes(pq_ctx_t *ctx, pq_node_t *nodes, uint64_t *overlap_len_sum);
+
+/* Node records (as pq_graph_nodes) and edges in the edge_t layout (as
+ * pq_graph_copy_edges) of the last search, with one host wait: everything
+ * construct_string_graph (string_graph.c:832-1021) needs in host memory.
+ * nodes or edges may be NULL.  Into memory from pq_host_alloc the copies run
+ * at the host link's rate. */
+int pq_graph_export(pq_ctx_t *ctx, pq_node_t *nodes, void *edges, uint32_t index_bytes,
+                    uint64_t *overlap_len_sum);
+/* Page-locked host memory (hipHostMalloc) for the transfers above; released
+ * by pq_host_free.  No counterpart in the reference, whose results never leave
+ * host memory. */
+int pq_host_alloc(pq_ctx_t *ctx, uint64_t bytes, void **p);
+void pq_host_free(void *p);
 
 /* ---- learned index --------------------------------------------------------------------- */
 /* Counterpart of the reference's interval shortcut (sufsort.c:179-189, used at

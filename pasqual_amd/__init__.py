@@ -39,7 +39,7 @@ PQ_SYMBOLS = [
     "pq_synth_genome",
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
     "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read", "pq_probe_gather", "pq_set_edge_capacity",
-    "pq_get_slot_sizes",
+    "pq_get_slot_sizes", "pq_graph_export", "pq_host_alloc", "pq_host_free",
 ]
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
                "time_importreads"]
@@ -172,6 +172,9 @@ def lib(width="u32"):
         "pq_import_fasta": (I32, [P, C.c_char_p, I32, I32, I32, I32, P]),
         "pq_dedup_reads": (I32, [P, P, P, U64, I32, U64, I32, I32, P, P]),
         "pq_graph_nodes": (I32, [P, P, P]),
+        "pq_graph_export": (I32, [P, P, P, U32, P]),
+        "pq_host_alloc": (I32, [P, U64, C.POINTER(P)]),
+        "pq_host_free": (None, [P]),
         "pq_probe_read": (I32, [P, U64, P]),
         "pq_probe_gather": (I32, [P, U64, I32, P]),
     }
@@ -240,6 +243,31 @@ def import_fasta(path, read_len, bloom_m=36, n_hashfunc=2, index_bits=32, ctx=No
 
 
 # ---- device context ------------------------------------------------------------------
+
+class PinnedBuffer:
+    """Page-locked host memory from pq_host_alloc; `.array` is a uint8 view."""
+
+    def __init__(self, ctx, nbytes):
+        self._L = ctx._L
+        self.ptr = C.c_void_p()
+        ctx._check(self._L.pq_host_alloc(ctx._h, max(16, int(nbytes)), C.byref(self.ptr)),
+                   "host_alloc")
+        self.nbytes = int(nbytes)
+        self.array = np.ctypeslib.as_array(C.cast(self.ptr, C.POINTER(C.c_uint8)),
+                                           shape=(max(16, self.nbytes),))[:self.nbytes]
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            self._L.pq_host_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
 
 class Context:
     """One device, one stream, one index (not thread-safe)."""
@@ -397,6 +425,28 @@ class Context:
         total = C.c_uint64()
         self._check(self._L.pq_graph_nodes(self._h, _ptr(rec), C.byref(total)), "graph_nodes")
         return rec[:n.value], int(total.value)
+
+    def host_buffer(self, nbytes):
+        """Page-locked host memory of nbytes (pq_host_alloc) as a uint8 array; freed
+        with the array's owner (PinnedBuffer.free or garbage collection)."""
+        return PinnedBuffer(self, nbytes)
+
+    def graph_export(self, nodes, edges, index_bytes):
+        """Node records and edge_t records of the last search into host buffers
+        (pq_graph_export, one host wait): nodes holds n * 8 bytes, edges E *
+        (8 if index_bytes == 4 else 16); either may be None.  Returns the sum of
+        the overlap lengths."""
+        total = C.c_uint64()
+        self._check(self._L.pq_graph_export(self._h, None if nodes is None else nodes.ptr,
+                                            None if edges is None else edges.ptr, index_bytes,
+                                            C.byref(total)), "graph_export")
+        return int(total.value)
+
+    def graph_size(self):
+        """(reads, edges) of the last search."""
+        n, E = C.c_uint64(), C.c_uint64()
+        self._check(self._L.pq_graph_sizes(self._h, C.byref(n), C.byref(E)), "graph_sizes")
+        return n.value, E.value
 
     def autotune(self, mode=SEARCH_PLAIN):
         t = Tune()

@@ -39,6 +39,26 @@ double time_importreads = 0.0;
 static pq_ctx_t *g_ctx;
 static const index_t *g_index;
 
+/* Page-locked staging for the search result (node records, edge_t records),
+ * set up by construct_index for about four edges per read and grown by
+ * construct_string_graph if the search finds more: device-to-host copies into
+ * pageable memory go through a bounce buffer at a fraction of the link's rate. */
+static void *g_stage_nodes, *g_stage_edges;
+static uint64_t g_stage_nodes_bytes, g_stage_edges_bytes;
+
+static void *stage(void **buf, uint64_t *have, uint64_t bytes)
+{
+    if (*buf && *have >= bytes)
+        return *buf;
+    pq_host_free(*buf);
+    *buf = NULL;
+    *have = 0;
+    if (pq_host_alloc(g_ctx, bytes, buf) != PQ_OK)
+        return NULL;
+    *have = bytes;
+    return *buf;
+}
+
 #define REF_ERROR_EXIT(...)                                                    \
     do {                                                                       \
         fprintf(stderr, "Error: ");                                            \
@@ -220,6 +240,10 @@ index_t *construct_index(char *single_file, char *pair_file1, char *pair_file2,
     }
     if (env_int("PQ_HOST_INDEX", 1))
         export_host_index(ix);
+    /* staging for construct_string_graph's copies (its size is what the search
+     * will need for about four edges per read) */
+    stage(&g_stage_nodes, &g_stage_nodes_bytes, sizeof(pq_node_t) * ((uint64_t) ix->num_reads + 1));
+    stage(&g_stage_edges, &g_stage_edges_bytes, sizeof(edge_t) * (4 * (uint64_t) ix->num_reads + 1));
     return ix;
 }
 
@@ -283,19 +307,18 @@ string_graph_t *construct_string_graph(index_t *ix)
         dev_fail("overlap search", rc);
     uint64_t n = 0, E = 0;
     pq_graph_sizes(g_ctx, &n, &E);
-    pq_node_t *rec = malloc(sizeof(pq_node_t) * (n + 1));
-    edge_t *all = malloc(sizeof(edge_t) * (E + 1));   /* every node's edges, CSR order */
+    /* node records (types, marks, counts, computed on the device) and every
+     * node's edges in CSR order, into page-locked staging */
+    pq_node_t *rec = stage(&g_stage_nodes, &g_stage_nodes_bytes, sizeof(pq_node_t) * (n + 1));
+    edge_t *all = stage(&g_stage_edges, &g_stage_edges_bytes, sizeof(edge_t) * (E + 1));
     string_graph_t *g = malloc(sizeof(string_graph_t));
     if (!rec || !all || !g)
         dev_fail("graph buffers", PQ_ERR_NOMEM);
-    /* node records (types, marks, counts) are computed on the device */
     uint64_t len_sum = 0;
-    rc = pq_graph_nodes(g_ctx, rec, &len_sum);
+    rc = pq_graph_export(g_ctx, rec, all, (uint32_t) sizeof(INDEX_TYPE_T), &len_sum);
     if (rc == PQ_ERR_LIMIT)
         REF_ERROR_EXIT("assembly failed. Try a larger min overlap length.\n");
     if (rc)
-        dev_fail("graph copy", rc);
-    if ((rc = pq_graph_copy_edges(g_ctx, all, (uint32_t) sizeof(INDEX_TYPE_T))))
         dev_fail("graph copy", rc);
     if (timing)
         fprintf(stderr, "[pq] graph copy %.3f ms\n", now_ms() - tp), tp = now_ms();
@@ -375,8 +398,6 @@ string_graph_t *construct_string_graph(index_t *ix)
     if (timing)
         fprintf(stderr, "[pq] node build %.3f ms\n", now_ms() - tp);
     free(tbase);
-    free(rec);
-    free(all);
     printf("  takes %.3lf ms\n", now_ms() - t0);   /* string_graph.c:1043 */
     printf("  average overlap length: %lf\n", g->average_overlap);
     fflush(stdout);

@@ -1518,6 +1518,88 @@ __global__ void k_zero_words(unsigned long long *p, uint32_t count)
         p[threadIdx.x] = 0ull;
 }
 
+// Node records and the length sum into the SLOT_NODES scratch: words[0] = sum of
+// the overlap lengths, words[1] = flag of a side with more than 65535 edges,
+// then one uint2 record per read.
+static int enqueue_node_records(pq_ctx_t *ctx, unsigned long long **words, uint2 **recs)
+{
+    const uint64_t n = ctx->n, E = ctx->n_edges;
+    uint8_t *buf;
+    int rc = pq_slot(ctx, SLOT_NODES, n * 8 + 16, &buf);
+    if (rc)
+        return rc;
+    *words = (unsigned long long *) buf;
+    *recs = (uint2 *) (buf + 16);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, ctx->stream, *words, 2u);
+    if (n)
+        hipLaunchKernelGGL(k_node_records, dim3(pq_blocks_gs(n, 256)), dim3(256), 0, ctx->stream,
+                           ctx->d_ovB, ctx->d_ovF, ctx->d_numB, ctx->d_numF, n, *recs,
+                           (uint32_t *) (*words + 1));
+    if (E)
+        hipLaunchKernelGGL(k_edge_len_sum, dim3(std::min<unsigned>(pq_blocks_gs(E, 256), 2048u)),
+                           dim3(256), 0, ctx->stream, ctx->d_edges, E, *words);
+    PQ_CHECK(hipGetLastError());
+    return PQ_OK;
+}
+
+extern "C" int pq_host_alloc(pq_ctx_t *ctx, uint64_t bytes, void **p)
+{
+    if (!ctx || !p)
+        return PQ_ERR_ARG;
+    *p = nullptr;
+    hipError_t e = hipHostMalloc(p, bytes ? bytes : 16, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void) hipGetLastError();
+        return ctx->fail(PQ_ERR_NOMEM, std::string("hipHostMalloc of ") + std::to_string(bytes) +
+                                           " bytes: " + hipGetErrorString(e));
+    }
+    return PQ_OK;
+}
+
+extern "C" void pq_host_free(void *p)
+{
+    if (p)
+        (void) hipHostFree(p);
+}
+
+extern "C" int pq_graph_export(pq_ctx_t *ctx, pq_node_t *nodes, void *edges, uint32_t index_bytes,
+                               uint64_t *overlap_len_sum)
+{
+    static_assert(sizeof(pq_node_t) == 8, "pq_node_t is 8 bytes");
+    if (!ctx || (edges && index_bytes != 4 && index_bytes != 8))
+        return PQ_ERR_ARG;
+    if (!ctx->searched)
+        return ctx->fail(PQ_ERR_STATE, "no search result");
+    const uint64_t n = ctx->n, E = ctx->n_edges;
+    unsigned long long *words = nullptr;
+    uint2 *recs = nullptr;
+    int rc = enqueue_node_records(ctx, &words, &recs);
+    if (rc)
+        return rc;
+    const size_t rec = index_bytes == 4 ? 8 : 16;
+    uint8_t *d = nullptr;
+    if (edges && E) {
+        if ((rc = pq_slot(ctx, SLOT_EXPORT, E * rec, &d)))
+            return rc;
+        hipLaunchKernelGGL(k_ref_edges, dim3(pq_blocks(E, 256)), dim3(256), 0, ctx->stream,
+                           ctx->d_edges, E, index_bytes, d);
+        PQ_CHECK(hipGetLastError());
+    }
+    unsigned long long w[2];
+    PQ_CHECK(hipMemcpyAsync(w, words, 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (nodes && n)
+        PQ_CHECK(hipMemcpyAsync(nodes, recs, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (d)
+        PQ_CHECK(hipMemcpyAsync(edges, d, E * rec, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    if (w[1])
+        return ctx->fail(PQ_ERR_LIMIT, "a node has more than 65535 edges on one side");
+    if (overlap_len_sum)
+        *overlap_len_sum = w[0];
+    return PQ_OK;
+}
+
 extern "C" int pq_graph_nodes(pq_ctx_t *ctx, pq_node_t *nodes, uint64_t *overlap_len_sum)
 {
     static_assert(sizeof(pq_node_t) == 8, "pq_node_t is 8 bytes");
@@ -1525,22 +1607,12 @@ extern "C" int pq_graph_nodes(pq_ctx_t *ctx, pq_node_t *nodes, uint64_t *overlap
         return PQ_ERR_ARG;
     if (!ctx->searched)
         return ctx->fail(PQ_ERR_STATE, "no search result");
-    const uint64_t n = ctx->n, E = ctx->n_edges;
-    uint8_t *buf;
-    int rc = pq_slot(ctx, SLOT_NODES, n * 8 + 16, &buf);
+    const uint64_t n = ctx->n;
+    unsigned long long *words = nullptr;
+    uint2 *recs = nullptr;
+    int rc = enqueue_node_records(ctx, &words, &recs);
     if (rc)
         return rc;
-    unsigned long long *words = (unsigned long long *) buf;   // [0] length sum, [1] over flag
-    uint2 *recs = (uint2 *) (buf + 16);
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, ctx->stream, words, 2u);
-    if (n)
-        hipLaunchKernelGGL(k_node_records, dim3(pq_blocks_gs(n, 256)), dim3(256), 0, ctx->stream,
-                           ctx->d_ovB, ctx->d_ovF, ctx->d_numB, ctx->d_numF, n, recs,
-                           (uint32_t *) (words + 1));
-    if (E)
-        hipLaunchKernelGGL(k_edge_len_sum, dim3(std::min<unsigned>(pq_blocks_gs(E, 256), 2048u)),
-                           dim3(256), 0, ctx->stream, ctx->d_edges, E, words);
-    PQ_CHECK(hipGetLastError());
     unsigned long long w[2];
     PQ_CHECK(hipMemcpyAsync(w, words, 16, hipMemcpyDeviceToHost, ctx->stream));
     if (n)

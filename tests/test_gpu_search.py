@@ -48,6 +48,21 @@ def test_graph_matches_reference_fixture(ctx, name):
     assert len_sum / len(g["dest"]) == float(G["average_overlap"])
     E = np.stack([g["dest"], g["len"], g["type"]], 1).astype(np.uint64)
     assert np.array_equal(E, G["edges"])
+    # the one-call export into page-locked memory: the same node records, and the
+    # edges in the edge_t layout of both index widths (dest, overlap_len, type)
+    nb = ctx.host_buffer(n * 8)
+    for ib, dt in ((4, np.dtype([("dest", "<u4"), ("len", "<u2"), ("type", "u1"), ("pad", "u1")])),
+                   (8, np.dtype([("dest", "<u8"), ("len", "<u2"), ("type", "u1"),
+                                 ("pad", "V5")]))):
+        eb = ctx.host_buffer(len(g["dest"]) * dt.itemsize)
+        assert ctx.graph_export(nb, eb, ib) == len_sum
+        assert np.array_equal(nb.array.view(ctx.NODE_DTYPE), rec)
+        got = eb.array.view(dt)
+        assert np.array_equal(got["dest"].astype(np.uint64), G["edges"][:, 0])
+        assert np.array_equal(got["len"].astype(np.uint64), G["edges"][:, 1])
+        assert np.array_equal(got["type"].astype(np.uint64), G["edges"][:, 2])
+        eb.free()
+    nb.free()
     ex = orc.search_explicit(G["reads"], off, meta["t"])
     _same_graph(g, ex, name)
 
