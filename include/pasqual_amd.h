@@ -114,6 +114,10 @@ uint64_t pq_num_reads(const pq_ctx_t *ctx);
 uint64_t pq_text_len(const pq_ctx_t *ctx);
 
 /* ---- overlap search (string_graph.c:162-731) ------------------------------------------- */
+/* Any number of overlap candidates per read side: sides beyond the largest
+ * extraction tier (65536 candidates) are sorted by the whole device; a graph
+ * replay that meets one runs the search again eagerly.  PQ_ERR_NOMEM if their
+ * scratch (about 90 bytes per candidate) does not fit. */
 int pq_find_overlaps(pq_ctx_t *ctx, int mode);
 /* Sizes of the last search result. */
 int pq_graph_sizes(pq_ctx_t *ctx, uint64_t *n_reads, uint64_t *n_edges);
@@ -256,10 +260,11 @@ int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape);
  * (or after chains = 0) the kernel follows the size of both indexes'
  * occurrence lines: 2 below 1 GiB, 1 below 4 GiB, 3 from there on. */
 int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains);
-/* Capacity of the search's shared edge buffer: edges of read sides with more
- * than 16 irreducible edges (the first 16 of every side have slots of their
- * own).  0 (the default): max(2^20, reads / 4).  A search that needs more
- * fails with PQ_ERR_LIMIT "edge buffer exhausted" and leaves no result. */
+/* Initial capacity of the search's shared edge buffer: edges of read sides with
+ * more than 16 irreducible edges (the first 16 of every side have slots of
+ * their own).  0 (the default): max(2^20, reads / 4).  A search that needs more
+ * counts what it needed and runs once more with the buffer at that size (kept
+ * for later searches of the same reads). */
 int pq_set_edge_capacity(pq_ctx_t *ctx, uint64_t edges);
 
 /* ---- timing ---------------------------------------------------------------------------- */

@@ -1146,6 +1146,7 @@ extern "C" int pq_upload_reads(pq_ctx_t *ctx, const uint8_t *codes, const uint64
     pq_index_release(ctx);
     pq_learned_free(ctx);
     ctx->searched = false;
+    ctx->edge_cap_auto = 0;
     ctx->n = n;
     ctx->N = off[n];
     ctx->t = min_overlap;

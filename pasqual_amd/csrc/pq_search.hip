@@ -484,6 +484,28 @@ template <int GW> __device__ __forceinline__ uint64_t group_bits(bool x)
     return (b >> shift) & ((1ull << GW) - 1ull);
 }
 
+// Strict order of the extraction: valid items by (key, rank, o, m), then
+// invalid ones, then the padding indices >= k.
+__device__ __forceinline__ bool item_less(uint32_t a, uint32_t b, uint32_t k, uint32_t kw,
+                                          const uint64_t *keys, const ItemMeta *meta)
+{
+    if (a >= k || b >= k)
+        return a < b;
+    const ItemMeta ma = meta[a], mb = meta[b];
+    if (ma.valid != mb.valid)
+        return ma.valid > mb.valid;
+    if (!ma.valid)
+        return a < b;
+    int c = key_cmp(keys + (uint64_t) a * kw, ma.klen, keys + (uint64_t) b * kw, mb.klen);
+    if (c != 0)
+        return c < 0;
+    if (ma.rank != mb.rank)
+        return ma.rank < mb.rank;
+    if (ma.o != mb.o)
+        return ma.o < mb.o;
+    return ma.m < mb.m;
+}
+
 // Edge emission of one read side evaluated by one group of GW lanes on items
 // sorted by (key, rank, o, m), the order the edges are emitted in.  The valid
 // items sort first; with lcp[i] the codes shared by sorted items i-1 and i:
@@ -495,10 +517,18 @@ template <int GW> __device__ __forceinline__ uint64_t group_bits(bool x)
 //    keys starts where lcp[i] is below either length);
 //  - the read itself is never emitted;
 // the rules of the reference's depth-first walk (string_graph.c:386-731) as
-// restated in the header.  P: a power of two >= k, the same for every group of
-// a wave (the tier's item limit when several groups share a wave), so that all
-// loops have the same trip count in every group.  perm reuses the entry prefix
-// array; emitf is indexed by sorted position.
+// restated in the header.  perm[0..P) holds the sorted item indices (P a power
+// of two >= k); emitf and lcpv are indexed by sorted position.
+template <int GW>
+__device__ void extract_emit(uint64_t r, int side, uint32_t kw, uint32_t k, uint32_t P,
+                             const uint64_t *keys, const ItemMeta *meta, uint8_t *emitf,
+                             const uint32_t *perm, uint16_t *lcpv, const ExtOut &out);
+
+// Sort and emission of one read side by one group of GW lanes: a bitonic
+// network over the item indices (P: a power of two >= k, the same for every
+// group of a wave -- the tier's item limit when several groups share a wave --
+// so that all loops have the same trip count in every group).  perm reuses the
+// entry prefix array.
 template <int GW>
 __device__ void extract_sorted(uint64_t r, int side, uint32_t kw, uint32_t k, uint32_t P,
                                const uint64_t *keys, const ItemMeta *meta, uint8_t *emitf,
@@ -508,32 +538,13 @@ __device__ void extract_sorted(uint64_t r, int side, uint32_t kw, uint32_t k, ui
     for (uint32_t i = lane; i < P; i += GW)
         perm[i] = i;
     wave_sync();
-    // strict order: valid items by (key, rank, o, m), then invalid ones, then
-    // the padding up to P
-    auto less = [&](uint32_t a, uint32_t b) -> bool {
-        if (a >= k || b >= k)
-            return a < b;
-        const ItemMeta ma = meta[a], mb = meta[b];
-        if (ma.valid != mb.valid)
-            return ma.valid > mb.valid;
-        if (!ma.valid)
-            return a < b;
-        int c = key_cmp(keys + (uint64_t) a * kw, ma.klen, keys + (uint64_t) b * kw, mb.klen);
-        if (c != 0)
-            return c < 0;
-        if (ma.rank != mb.rank)
-            return ma.rank < mb.rank;
-        if (ma.o != mb.o)
-            return ma.o < mb.o;
-        return ma.m < mb.m;
-    };
     for (uint32_t s2 = 2; s2 <= P; s2 <<= 1) {
         for (uint32_t j2 = s2 >> 1; j2 > 0; j2 >>= 1) {
             for (uint32_t p = lane; p < P / 2; p += GW) {
                 const uint32_t i = 2 * j2 * (p / j2) + (p % j2), ip = i + j2;
                 const bool up = (i & s2) == 0;
                 const uint32_t x = perm[i], y = perm[ip];
-                if (less(y, x) == up) {
+                if (item_less(y, x, k, kw, keys, meta) == up) {
                     perm[i] = y;
                     perm[ip] = x;
                 }
@@ -541,6 +552,15 @@ __device__ void extract_sorted(uint64_t r, int side, uint32_t kw, uint32_t k, ui
             wave_sync();
         }
     }
+    extract_emit<GW>(r, side, kw, k, P, keys, meta, emitf, perm, lcpv, out);
+}
+
+template <int GW>
+__device__ void extract_emit(uint64_t r, int side, uint32_t kw, uint32_t k, uint32_t P,
+                             const uint64_t *keys, const ItemMeta *meta, uint8_t *emitf,
+                             const uint32_t *perm, uint16_t *lcpv, const ExtOut &out)
+{
+    const int lane = threadIdx.x & (GW - 1);
     uint32_t nv = 0;
     for (uint32_t q0 = 0; q0 < P; q0 += GW) {
         const uint32_t q = q0 + lane;
@@ -666,6 +686,42 @@ __device__ void extract_sorted(uint64_t r, int side, uint32_t kw, uint32_t k, ui
     wave_sync();
 }
 
+// Item (type of chain ch, overlap o, read m) with its extension key: FB/BB
+// m[o..) from the forward text; FF/BF rc(m)[o..) as the complement of the
+// backward text; a read contained in the overlap is no candidate (invalid).
+template <typename R>
+__device__ __forceinline__ void item_fill(const Geom &G, const DirViewT<R> &fw,
+                                          const DirViewT<R> &bw, int ch, uint32_t rank,
+                                          uint32_t o, uint64_t m, uint32_t kw, uint64_t *kq,
+                                          ItemMeta *meta)
+{
+    const uint32_t Lm = pq_read_len(G, m);
+    ItemMeta mt;
+    mt.m = (uint32_t) m;
+    mt.o = (uint16_t) o;
+    mt.valid = o < Lm;
+    mt.klen = (uint16_t) (mt.valid ? Lm - o : 0);
+    mt.rank = (uint8_t) rank;
+    mt.type = ch == 0 ? 1 : ch == 1 ? 2 : ch == 2 ? 0 : 3;   // FB FF BF BB
+    mt.pad = 0;
+    const bool plain = (ch == 0 || ch == 3);
+    const uint64_t *tx = plain ? fw.text2 : bw.text2;
+    const uint64_t ks = pq_read_start(G, m) + o;
+    for (uint32_t w = 0; w < kw; w++) {
+        uint64_t x = 0;
+        if (32 * w < mt.klen) {
+            x = pq_window(tx, ks + 32 * w);
+            if (!plain)
+                x = ~x;
+            const uint32_t left = mt.klen - 32 * w;
+            if (left < 32)
+                x &= ~0ull << (64 - 2 * left);
+        }
+        kq[w] = x;
+    }
+    *meta = mt;
+}
+
 // Process one read with one group of GW lanes (a wave or a quarter wave; the
 // groups of a wave work on different reads).  keys/meta/emitf/pre: this group's
 // storage (LDS or global) for kmax items.  Returns false, writing nothing, if a
@@ -761,38 +817,9 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
             for (int h = 0; h < 2; h++) {
                 if (!on[h])
                     continue;
-                const uint32_t q = qq[h];
                 const int ch = firsts[h] ? chA : chB;
-                const uint64_t m = mm[h];
-                const uint32_t o = oo[h];
-                uint32_t Lm = pq_read_len(G, m);
-                ItemMeta mt;
-                mt.m = (uint32_t) m;
-                mt.o = (uint16_t) o;
-                mt.valid = o < Lm;                 // contained reads are no candidates
-                mt.klen = (uint16_t) (mt.valid ? Lm - o : 0);
-                mt.rank = firsts[h] ? 0 : 1;
-                mt.type = ch == 0 ? 1 : ch == 1 ? 2 : ch == 2 ? 0 : 3;   // FB FF BF BB
-                mt.pad = 0;
-                // key: FB/BB m[o..) from the forward text; FF/BF rc(m)[o..) as the
-                // complement of the backward text
-                bool plain = (ch == 0 || ch == 3);
-                const uint64_t *tx = plain ? fw.text2 : bw.text2;
-                uint64_t ks = pq_read_start(G, m) + o;
-                uint64_t *kq = keys + (uint64_t) q * kw;
-                for (uint32_t w = 0; w < kw; w++) {
-                    uint64_t x = 0;
-                    if (32 * w < mt.klen) {
-                        x = pq_window(tx, ks + 32 * w);
-                        if (!plain)
-                            x = ~x;
-                        uint32_t left = mt.klen - 32 * w;
-                        if (left < 32)
-                            x &= ~0ull << (64 - 2 * left);
-                    }
-                    kq[w] = x;
-                }
-                meta[q] = mt;
+                item_fill(G, fw, bw, ch, firsts[h] ? 0 : 1, oo[h], mm[h], kw,
+                          keys + (uint64_t) qq[h] * kw, meta + qq[h]);
             }
         }
         wave_sync();
@@ -864,12 +891,49 @@ __global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirViewT<R> 
     }
 }
 
+// ---- read sides beyond the global tier ------------------------------------------------
+//
+// A side with more than KMAX_GLOBAL items (a read inside a high-copy repeat) is
+// listed by the global tier with the item counts of both sides.  Eager
+// searches then sort each side of such a read with all of the GPU: the items
+// are expanded into scratch sized to the side, sorted by a bitonic network of
+// one launch per stage, and the emission rules run over the sorted order by
+// one wave (extract_emit, as every tier).  There is no limit on candidates but
+// device memory; the reference keeps any number of candidates as SA intervals
+// (string_graph.c:773-778).
+
+struct HugeRead {
+    uint32_t rl, pad;
+    uint64_t k[2];       // items of side 0 (B) and side 1 (F)
+};
+
+// Items of one side of batch read rl, summed by one group of GW lanes.
+template <int GW> __device__ uint64_t side_items(const ChainBuf &cb, uint64_t rl, int side)
+{
+    const int lane = threadIdx.x & (GW - 1);
+    const int chA = side ? 0 : 2, chB = side ? 1 : 3;
+    const uint32_t nA = cb.n[rl * 4 + chA], nB = cb.n[rl * 4 + chB];
+    uint64_t k = 0;
+    for (uint32_t e = lane; e < nA + nB; e += GW) {
+        const uint64_t at = e < nA ? (rl * 4 + chA) * cb.cap + e : (rl * 4 + chB) * cb.cap + (e - nA);
+        k += cb.lc[at].y;
+    }
+    for (int d = GW / 2; d > 0; d >>= 1)
+        k += __shfl_xor(k, d, GW);
+    return k;
+}
+
+// One wave per read with scratch in global memory (up to KMAX_GLOBAL items per
+// side).  A larger read is listed in `huge` (eager searches), or, while the
+// search is being captured as a graph (huge_is_error), raises flag 2: the host
+// then runs the search eagerly.
 template <typename R>
 __global__ void __launch_bounds__(64) k_extract_global(Geom G, DirViewT<R> fw, DirViewT<R> bw,
                                                        ChainBuf cb, uint64_t r0,
                                                        const uint32_t *list,
                                                        const uint32_t *count, uint32_t kw,
-                                                       char *gscratch, ExtOut out)
+                                                       char *gscratch, ExtOut out, HugeRead *huge,
+                                                       uint32_t *n_huge, uint32_t huge_is_error)
 {
     ExtWave w = ext_wave(gscratch + ext_wave_bytes(KMAX_GLOBAL, kw) * blockIdx.x, KMAX_GLOBAL, kw);
     const uint32_t nlist = *count;
@@ -877,9 +941,109 @@ __global__ void __launch_bounds__(64) k_extract_global(Geom G, DirViewT<R> fw, D
         uint64_t rl = list[i];
         bool ok = extract_read<64, R>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
                                       w.pre, w.lcp, KMAX_GLOBAL, out);
-        if (!ok && threadIdx.x == 0)
-            atomicOr(out.flags, 2u);
+        if (ok)
+            continue;
+        const uint64_t kB = side_items<64>(cb, rl, 0), kF = side_items<64>(cb, rl, 1);
+        if (threadIdx.x == 0) {
+            if (huge_is_error) {
+                atomicOr(out.flags, 2u);
+            } else {
+                const uint32_t at = atomicAdd(n_huge, 1u);
+                HugeRead h;
+                h.rl = (uint32_t) rl;
+                h.pad = 0;
+                h.k[0] = kB;
+                h.k[1] = kF;
+                huge[at] = h;
+            }
+        }
     }
+}
+
+// Exclusive prefix of the item counts over the entries of one side; pre[ne] =
+// the side's items.  One wave.
+__global__ void __launch_bounds__(64) k_huge_prefix(ChainBuf cb, uint64_t rl, int side, uint32_t *pre)
+{
+    const int lane = threadIdx.x;
+    const int chA = side ? 0 : 2, chB = side ? 1 : 3;
+    const uint32_t nA = cb.n[rl * 4 + chA], ne = nA + cb.n[rl * 4 + chB];
+    uint32_t carry = 0;
+    for (uint32_t e0 = 0; e0 < ne; e0 += 64) {
+        const uint32_t e = e0 + lane;
+        uint32_t v = 0;
+        if (e < ne) {
+            const uint64_t at = e < nA ? (rl * 4 + chA) * cb.cap + e : (rl * 4 + chB) * cb.cap + (e - nA);
+            v = cb.lc[at].y;
+        }
+        uint32_t inc = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t x = __shfl_up(inc, d, 64);
+            if (lane >= d)
+                inc += x;
+        }
+        if (e < ne)
+            pre[e] = carry + inc - v;
+        carry += __shfl(inc, 63, 64);
+    }
+    if (lane == 0)
+        pre[ne] = carry;
+}
+
+// Items q < k of one side with their keys; perm[q] = q for all q < P.
+template <typename R>
+__global__ void k_huge_expand(Geom G, DirViewT<R> fw, DirViewT<R> bw, ChainBuf cb, uint64_t rl,
+                              int side, const uint32_t *pre, uint32_t k, uint32_t P, uint32_t kw,
+                              uint64_t *keys, ItemMeta *meta, uint32_t *perm)
+{
+    const int chA = side ? 0 : 2, chB = side ? 1 : 3;
+    const uint32_t nA = cb.n[rl * 4 + chA], ne = nA + cb.n[rl * 4 + chB];
+    PQ_GRID_STRIDE_FOR(q, P)
+    {
+        perm[q] = (uint32_t) q;
+        if (q >= k)
+            continue;
+        uint32_t lo = 0, hi = ne;   // the entry e with pre[e] <= q < pre[e + 1]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (pre[mid] <= q)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const bool first = lo < nA;
+        const int ch = first ? chA : chB;
+        const uint64_t at = first ? (rl * 4 + chA) * cb.cap + lo : (rl * 4 + chB) * cb.cap + (lo - nA);
+        const uint2 lc = cb.lc[at];
+        const uint32_t *ri = (ch == 0 || ch == 3) ? fw.reads_index : bw.reads_index;
+        const uint64_t m = ri[lc.x + ((uint32_t) q - pre[lo])];
+        item_fill(G, fw, bw, ch, first ? 0u : 1u, (uint32_t) cb.o[at], m, kw,
+                  keys + (uint64_t) q * kw, meta + q);
+    }
+}
+
+// One stage (s2, j2) of the bitonic network over perm[0..P).
+__global__ void k_huge_sort_step(uint32_t P, uint32_t s2, uint32_t j2, uint32_t k, uint32_t kw,
+                                 const uint64_t *keys, const ItemMeta *meta, uint32_t *perm)
+{
+    PQ_GRID_STRIDE_FOR(p, P / 2)
+    {
+        const uint32_t pp = (uint32_t) p;
+        const uint32_t i = 2 * j2 * (pp / j2) + (pp % j2), ip = i + j2;
+        const bool up = (i & s2) == 0;
+        const uint32_t x = perm[i], y = perm[ip];
+        if (item_less(y, x, k, kw, keys, meta) == up) {
+            perm[i] = y;
+            perm[ip] = x;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) k_huge_emit(uint64_t r, int side, uint32_t kw, uint32_t k,
+                                                  uint32_t P, const uint64_t *keys,
+                                                  const ItemMeta *meta, uint8_t *emitf,
+                                                  const uint32_t *perm, uint16_t *lcpv, ExtOut out)
+{
+    extract_emit<64>(r, side, kw, k, P, keys, meta, emitf, perm, lcpv, out);
 }
 
 // Zero `count` words (a kernel rather than a memset node, so that the sequence
@@ -919,6 +1083,58 @@ __global__ void k_csr(const uint64_t *recB, const uint64_t *recF, const uint32_t
         dst[a + i] = sb[i];
     for (uint64_t i = 0; i < nf; i++)
         dst[a + nb + i] = sf[i];
+}
+
+// Internal results of one search attempt (positive: pq_find_overlaps retries).
+#define PQ_RETRY_EAGER 1     // a graph replay met a side beyond the global tier
+#define PQ_RETRY_EDGES 2     // the shared edge pool overflowed; ctx->edge_need = demand
+
+// The listed reads of one batch whose sides exceed the global tier: each side is
+// expanded into scratch sized to it, sorted by one launch per bitonic stage over
+// the whole GPU, and its edges emitted by one wave (eager searches only).
+template <typename R>
+static int extract_huge(pq_ctx_t *ctx, const Geom &G, const DirViewT<R> &fw, const DirViewT<R> &bw,
+                        const ChainBuf &cb, uint64_t r0, uint32_t kw, const ExtOut &eo,
+                        const HugeRead *list, uint32_t count)
+{
+    for (uint32_t h = 0; h < count; h++) {
+        for (int side = 0; side < 2; side++) {
+            const uint64_t k = list[h].k[side];
+            if (k >= (1ull << 31))
+                return ctx->fail(PQ_ERR_LIMIT, "a read side has 2^31 overlap candidates or more");
+            uint32_t P = 1;
+            while (P < k)
+                P <<= 1;
+            auto up16 = [](size_t b) { return (b + 15) & ~(size_t) 15; };
+            const size_t b_keys = up16((size_t) P * kw * 8), b_meta = up16((size_t) P * sizeof(ItemMeta));
+            const size_t b_perm = up16((size_t) P * 4), b_lcp = up16((size_t) P * 2);
+            const size_t b_emit = up16(P), b_pre = up16((size_t) (2 * cb.cap + 2) * 4);
+            char *base = nullptr;
+            int rc = pq_slot(ctx, SLOT_HUGE, b_keys + b_meta + b_perm + b_lcp + b_emit + b_pre, &base);
+            if (rc)
+                return rc;
+            uint64_t *keys = (uint64_t *) base;
+            ItemMeta *meta = (ItemMeta *) (base + b_keys);
+            uint32_t *perm = (uint32_t *) (base + b_keys + b_meta);
+            uint16_t *lcpv = (uint16_t *) (base + b_keys + b_meta + b_perm);
+            uint8_t *emitf = (uint8_t *) (base + b_keys + b_meta + b_perm + b_lcp);
+            uint32_t *pre = (uint32_t *) (base + b_keys + b_meta + b_perm + b_lcp + b_emit);
+            const uint64_t rl = list[h].rl;
+            hipLaunchKernelGGL(k_huge_prefix, dim3(1), dim3(64), 0, ctx->stream, cb, rl, side, pre);
+            hipLaunchKernelGGL(k_huge_expand<R>, dim3(pq_blocks_gs(P, 256)), dim3(256), 0,
+                               ctx->stream, G, fw, bw, cb, rl, side, pre, (uint32_t) k, P, kw,
+                               keys, meta, perm);
+            for (uint32_t s2 = 2; s2 <= P; s2 <<= 1)
+                for (uint32_t j2 = s2 >> 1; j2 > 0; j2 >>= 1)
+                    hipLaunchKernelGGL(k_huge_sort_step, dim3(pq_blocks_gs(P / 2, 256)), dim3(256),
+                                       0, ctx->stream, P, s2, j2, (uint32_t) k, kw, keys, meta,
+                                       perm);
+            hipLaunchKernelGGL(k_huge_emit, dim3(1), dim3(64), 0, ctx->stream, r0 + rl, side, kw,
+                               (uint32_t) k, P, keys, meta, emitf, perm, lcpv, eo);
+            PQ_CHECK(hipGetLastError());
+        }
+    }
+    return PQ_OK;
 }
 
 static double ev_ms(hipEvent_t a, hipEvent_t b)
@@ -1005,9 +1221,11 @@ static int run_search_graph(pq_ctx_t *ctx, int mode, uint64_t n, uint64_t batch,
     ctx->timing.find_positions = (uint64_t) hm[8] | ((uint64_t) hm[9] << 32);
     ctx->timing.find_lines = (uint64_t) hm[10] | ((uint64_t) hm[11] << 32);
     if (hm[3] & 2u)
-        return ctx->fail(PQ_ERR_LIMIT, "a read has more than 65536 overlap candidates on one side");
-    if (hm[3] & 1u)
-        return ctx->fail(PQ_ERR_LIMIT, "edge buffer exhausted");
+        return PQ_RETRY_EAGER;
+    if (hm[3] & 1u) {
+        ctx->edge_need = (uint64_t) hm[0] | ((uint64_t) hm[1] << 32);
+        return PQ_RETRY_EDGES;
+    }
     ctx->n_edges = total;
     ctx->searched = true;
     return PQ_OK;
@@ -1061,10 +1279,13 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(n, budget / per_read));
     uint64_t *rec = nullptr, *recF = nullptr, *app = nullptr, *tot = nullptr, *staged = nullptr;
     uint32_t *ovl = nullptr, *misc = nullptr;
-    const uint64_t edge_cap =
-        ctx->edge_cap_set ? ctx->edge_cap_set : std::max<uint64_t>(1 << 20, n / 4);
+    // shared edge pool: the set or default capacity, or what an overflowing
+    // search of these reads asked for
+    const uint64_t edge_cap = std::max<uint64_t>(
+        ctx->edge_cap_set ? ctx->edge_cap_set : std::max<uint64_t>(1 << 20, n / 4), ctx->edge_cap_auto);
     SeedT<R> *seeds = nullptr;
     char *gscratch = nullptr;
+    HugeRead *huge = nullptr;
     // four events per batch (seed, find, extract boundaries), read after the loop
     const uint64_t nbatch = (n + batch - 1) / batch;
     std::vector<hipEvent_t> ev(4 * nbatch);
@@ -1089,6 +1310,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         (rc = pq_slot(ctx, SLOT_MISC, 64, &misc)) ||
         (rc = pq_slot(ctx, SLOT_GSCRATCH, GLOBAL_SLOTS * ext_wave_bytes(KMAX_GLOBAL, kw),
                       &gscratch)) ||
+        (rc = pq_slot(ctx, SLOT_HUGE_LIST, batch * sizeof(HugeRead), &huge)) ||
         (mode == PQ_SEARCH_LEARNED &&
          (rc = pq_slot(ctx, SLOT_SEEDS, batch * 4 * sizeof(SeedT<R>), &seeds)))) {
         cleanup();
@@ -1189,12 +1411,32 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         }
         hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64, R>), dim3(grid2), dim3(64), lds2,
                            ctx->stream, G, fw, bw, cb, r0, nr, inC, cntC, kw, eoC);
+        // sides beyond the global tier: listed (count misc[7]) and sorted by the whole
+        // GPU right here, before the next batch reuses the candidate buffers; a
+        // graph capture cannot wait for the count, so there they raise flag 2
         hipLaunchKernelGGL(k_extract_global<R>, dim3(GLOBAL_SLOTS), dim3(64), 0, ctx->stream, G, fw,
-                           bw, cb, r0, eoC.overflow_reads, eoC.n_overflow, kw, gscratch, eo);
-        // overflow counters back to zero for the next batch (misc[2], misc[4], misc[6])
+                           bw, cb, r0, eoC.overflow_reads, eoC.n_overflow, kw, gscratch, eo, huge,
+                           misc + 7, timed ? 0u : 1u);
+        PQ_CHECK(hipGetLastError());
+        if (timed) {
+            uint32_t nh = 0;
+            PQ_CHECK(hipMemcpyAsync(&nh, misc + 7, 4, hipMemcpyDeviceToHost, ctx->stream));
+            PQ_CHECK(hipStreamSynchronize(ctx->stream));
+            if (nh) {
+                std::vector<HugeRead> hl(nh);
+                PQ_CHECK(hipMemcpyAsync(hl.data(), huge, nh * sizeof(HugeRead),
+                                        hipMemcpyDeviceToHost, ctx->stream));
+                PQ_CHECK(hipStreamSynchronize(ctx->stream));
+                if ((rc = extract_huge<R>(ctx, G, fw, bw, cb, r0, kw, eo, hl.data(), nh)))
+                    break;
+            }
+        }
+        // overflow counters back to zero for the next batch (misc[2], misc[4], misc[6],
+        // misc[7])
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eo.n_overflow, 1u);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eoB.n_overflow, 1u);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eoC.n_overflow, 1u);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc + 7, 1u);
         PQ_CHECK(hipGetLastError());
         if (timed)
             (void) hipEventRecord(e[3], ctx->stream);
@@ -1264,12 +1506,12 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
             ctx->timing.compact_ms = ev_ms(ce[0], ce[1]);
             ctx->timing.find_positions = (uint64_t) hm[8] | ((uint64_t) hm[9] << 32);
             ctx->timing.find_lines = (uint64_t) hm[10] | ((uint64_t) hm[11] << 32);
-            if (hm[3] & 2u)
-                rc = ctx->fail(PQ_ERR_LIMIT,
-                               "a read has more than 65536 overlap candidates on one side");
-            else if (hm[3] & 1u)
-                rc = ctx->fail(PQ_ERR_LIMIT, "edge buffer exhausted");
-            else {
+            if (hm[3] & 2u) {
+                rc = ctx->fail(PQ_ERR_STATE, "oversized read side left unprocessed");
+            } else if (hm[3] & 1u) {
+                ctx->edge_need = (uint64_t) hm[0] | ((uint64_t) hm[1] << 32);
+                rc = PQ_RETRY_EDGES;
+            } else {
                 ctx->n_edges = total;
                 ctx->searched = true;
             }
@@ -1294,8 +1536,27 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         return ctx->fail(PQ_ERR_ARG, "unknown search mode");
     if (mode == PQ_SEARCH_LEARNED && !ctx->learned)
         return ctx->fail(PQ_ERR_STATE, "learned index not trained");
-    return ctx->wide ? find_overlaps_t<uint64_t>(ctx, mode, as_graph)
-                     : find_overlaps_t<uint32_t>(ctx, mode, as_graph);
+    auto attempt = [&](bool graph) {
+        return ctx->wide ? find_overlaps_t<uint64_t>(ctx, mode, graph)
+                         : find_overlaps_t<uint32_t>(ctx, mode, graph);
+    };
+    // a graph replay that met a read side beyond the global tier runs again
+    // eagerly; a search whose edges overflowed the shared pool runs again with
+    // the pool at the demand it counted
+    bool graph = as_graph;
+    int rc = PQ_OK;
+    for (int tries = 0; tries < 4; tries++) {
+        rc = attempt(graph);
+        if (rc == PQ_RETRY_EAGER)
+            graph = false;
+        else if (rc == PQ_RETRY_EDGES)
+            ctx->edge_cap_auto = ctx->edge_need + ctx->edge_need / 8 + 1024;
+        else
+            break;
+    }
+    if (rc > 0)
+        rc = ctx->fail(PQ_ERR_STATE, "search did not settle its buffers");
+    return rc;
 }
 
 extern "C" int pq_graph_sizes(pq_ctx_t *ctx, uint64_t *n_reads, uint64_t *n_edges)
@@ -1428,6 +1689,7 @@ extern "C" int pq_set_edge_capacity(pq_ctx_t *ctx, uint64_t edges)
     if (!ctx)
         return PQ_ERR_ARG;
     ctx->edge_cap_set = edges;
+    ctx->edge_cap_auto = 0;
     return PQ_OK;
 }
 

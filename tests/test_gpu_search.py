@@ -133,25 +133,92 @@ def _identical_copies(copies=3):
     return np.concatenate(reads), off, 6
 
 
-def test_edge_buffer_exhaustion_is_reported(ctx):
-    # read sides with more than 16 edges share one buffer; made too small, the
-    # search reports it (eager and graph mode) and the edge copy leaves every
-    # buffer alone; at its automatic size the graph is the oracle's again
+def test_edge_buffer_exhaustion_recovers(ctx):
+    # read sides with more than 16 edges share one pool; made too small, the
+    # search counts what it needed, runs again with the pool at that size and
+    # gives the oracle's graph (eager and graph mode)
     codes, off, t = _identical_copies(40)
     ctx.upload(codes, off, t)
     ctx.build_index()
     want = orc.search_explicit(codes, off, t)
     assert max(want["num_B"].max(), want["num_F"].max()) > 16
-    ctx.set_edge_capacity(8)
     try:
         for mode in (pq.SEARCH_PLAIN, pq.SEARCH_PLAIN | pq.SEARCH_GRAPH):
-            with pytest.raises(pq.PqError, match="edge buffer exhausted"):
-                ctx.find_overlaps(mode)
+            ctx.set_edge_capacity(8)
+            ctx.find_overlaps(mode)
+            _same_graph(ctx.graph(), want, f"edge pool of 8, mode {mode:#x}")
     finally:
         ctx.set_edge_capacity(0)
     for mode in (pq.SEARCH_PLAIN, pq.SEARCH_PLAIN | pq.SEARCH_GRAPH):
         ctx.find_overlaps(mode)
-        _same_graph(ctx.graph(), want, f"edge capacity restored, mode {mode:#x}")
+        _same_graph(ctx.graph(), want, f"default edge pool, mode {mode:#x}")
+
+
+def _check_every_shape_and_kernel(ctx, want, where):
+    for shape in range(3):
+        ctx.set_tuning(shape)
+        for kernel in (1, 2, 3):
+            ctx.set_find_chains(kernel)
+            for mode in (pq.SEARCH_PLAIN, pq.SEARCH_PLAIN | pq.SEARCH_GRAPH):
+                ctx.find_overlaps(mode)
+                _same_graph(ctx.graph(), want, f"{where}: shape {shape}, kernel {kernel}, "
+                                                f"mode {mode:#x}")
+    ctx.set_tuning(1)
+    ctx.set_find_chains(0)
+
+
+def test_global_tier_more_than_512_items(ctx):
+    # 600 copies of one read overlap a second read by 8 codes: that read's B side
+    # holds 600 items, beyond the one-wave LDS tier (512), so the global-memory
+    # tier sorts it; all 600 are emitted (more than 16: the shared pool)
+    codes, off, t = _identical_copies(600)
+    ctx.upload(codes, off, t)
+    ctx.build_index()
+    want = orc.search_explicit(codes, off, t)
+    assert want["overlaps_B"].max() > 512
+    _check_every_shape_and_kernel(ctx, want, "600 items")
+
+
+def _one_oversized_side(copies_fb=300, copies_ff=100, L=200, t=21, seed=11):
+    """One read n of L codes and, for every overlap o in [t, L), copies of the read
+    that starts o codes before n's end and ends one code after it (FB candidates
+    of n), and of its reverse complement (FF candidates); the extra code is one
+    of two alternatives.  n's F side holds (L - t) * (copies_fb + copies_ff) items;
+    every other read has n as its only candidate (they all end at the same place,
+    so none overlaps another), which keeps the oracle quick."""
+    rng = np.random.default_rng(seed)
+    g = rng.integers(1, 5, L).astype(np.uint8)
+    reads = [g]
+    for o in range(t, L):
+        for c in range(copies_fb + copies_ff):
+            m = np.concatenate([g[L - o:], np.array([1 if c % 3 else 4], np.uint8)])
+            reads.append(m if c < copies_fb else (5 - m[::-1]).astype(np.uint8))
+    order = rng.permutation(len(reads) - 1) + 1
+    reads = [reads[0]] + [reads[i] for i in order]
+    off = np.zeros(len(reads) + 1, np.uint64)
+    off[1:] = np.cumsum([len(r) for r in reads])
+    return np.concatenate(reads), off, t
+
+
+def test_read_side_beyond_global_tier(ctx):
+    # more than 65536 candidate reads on one side (the global tier's limit) but
+    # a few hundred edges: the side is sorted by the whole GPU and gives the
+    # oracle's graph (graph mode meets it and runs the search eagerly)
+    codes, off, t = _one_oversized_side()
+    want = orc.search_explicit(codes, off, t)
+    assert want["overlaps_F"][0] > 65536
+    assert 16 < want["num_F"][0] < 65535
+    ctx.upload(codes, off, t)
+    ctx.build_index()
+    # the side overflows every tier whatever the first tier's shape: the search
+    # kernels eagerly, and graph mode once
+    for kernel in (1, 2, 3):
+        ctx.set_find_chains(kernel)
+        ctx.find_overlaps()
+        _same_graph(ctx.graph(), want, f"oversized side, kernel {kernel}")
+    ctx.set_find_chains(0)
+    ctx.find_overlaps(pq.SEARCH_PLAIN | pq.SEARCH_GRAPH)
+    _same_graph(ctx.graph(), want, "oversized side, graph mode")
 
 
 def test_graph_edge_cases(ctx):
