@@ -356,6 +356,30 @@ int pq_dedup_reads(pq_ctx_t *ctx, const uint8_t *codes, const uint8_t *ok, uint6
                    int read_len, uint64_t bloom_bits, int n_hashfunc, int index_bits,
                    uint8_t *keep, uint32_t *passes);
 
+/* The duplicate filter of pq_dedup_reads on records already in device memory,
+ * allocating nothing: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  d_codes (nrec * read_len codes), d_ok and
+ * d_keep (nrec bytes each) are device pointers; keep[r] as pq_dedup_reads sets
+ * it.  Its scratch is a fixed device array of PQ_NOALLOC_SCRATCH_BYTES per
+ * device, held until the process ends: the call needs
+ * 2 * nrec * n_hashfunc * 8 + bloom_bits * 4 bytes of it (each part rounded up
+ * to 256 bytes, plus 256) and fails with PQ_ERR_LIMIT beyond that.  Every call
+ * on one device shares that array: calls that overlap on one device race.  It
+ * waits on ctx's stream for each pass. */
+#define PQ_NOALLOC_SCRATCH_BYTES (32u << 20)
+int pq_dedup_reads_noalloc(pq_ctx_t *ctx, const uint8_t *d_codes, const uint8_t *d_ok,
+                           uint64_t nrec, int read_len, uint64_t bloom_bits, int n_hashfunc,
+                           int index_bits, uint8_t *d_keep, uint32_t *passes);
+
+/* Device memory of ctx's device for callers of the entry points that take
+ * device pointers (pq_dedup_reads_noalloc), and copies on ctx's stream that
+ * return when the copy is done.  No counterpart in the reference. */
+int pq_dev_alloc(pq_ctx_t *ctx, uint64_t bytes, void **p);
+void pq_dev_free(void *p);
+int pq_dev_upload(pq_ctx_t *ctx, void *dst, const void *src, uint64_t bytes);
+int pq_dev_download(pq_ctx_t *ctx, void *dst, const void *src, uint64_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

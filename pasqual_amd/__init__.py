@@ -40,7 +40,9 @@ PQ_SYMBOLS = [
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
     "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read", "pq_probe_gather", "pq_set_edge_capacity",
     "pq_get_slot_sizes", "pq_graph_export", "pq_host_alloc", "pq_host_free",
+    "pq_dedup_reads_noalloc", "pq_dev_alloc", "pq_dev_free", "pq_dev_upload", "pq_dev_download",
 ]
+NOALLOC_SCRATCH_BYTES = 32 << 20   # PQ_NOALLOC_SCRATCH_BYTES
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
                "time_importreads"]
 
@@ -175,6 +177,11 @@ def lib(width="u32"):
         "pq_graph_export": (I32, [P, P, P, U32, P]),
         "pq_host_alloc": (I32, [P, U64, C.POINTER(P)]),
         "pq_host_free": (None, [P]),
+        "pq_dedup_reads_noalloc": (I32, [P, P, P, U64, I32, U64, I32, I32, P, P]),
+        "pq_dev_alloc": (I32, [P, U64, C.POINTER(P)]),
+        "pq_dev_free": (None, [P]),
+        "pq_dev_upload": (I32, [P, P, P, U64]),
+        "pq_dev_download": (I32, [P, P, P, U64]),
         "pq_probe_read": (I32, [P, U64, P]),
         "pq_probe_gather": (I32, [P, U64, I32, P]),
     }

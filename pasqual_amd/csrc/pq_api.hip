@@ -49,6 +49,42 @@ void *pq_ctx::tmp(int slot, size_t bytes)
     return b.p;
 }
 
+extern "C" int pq_dev_alloc(pq_ctx_t *ctx, uint64_t bytes, void **p)
+{
+    if (!ctx || !p)
+        return PQ_ERR_ARG;
+    if (hipSetDevice(ctx->device) != hipSuccess)
+        return ctx->fail(PQ_ERR_HIP, "hipSetDevice");
+    return pq_alloc(ctx, p, bytes);
+}
+
+extern "C" void pq_dev_free(void *p)
+{
+    pq_free(p);
+}
+
+extern "C" int pq_dev_upload(pq_ctx_t *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src)))
+        return PQ_ERR_ARG;
+    if (bytes) {
+        PQ_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    return PQ_OK;
+}
+
+extern "C" int pq_dev_download(pq_ctx_t *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src)))
+        return PQ_ERR_ARG;
+    if (bytes) {
+        PQ_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    return PQ_OK;
+}
+
 extern "C" int pq_ctx_create(pq_ctx_t **out, int device)
 {
     if (!out)

@@ -23,6 +23,7 @@
 // (a duplicate's verdict depends on the first copy, whose own verdict rarely
 // depends on anything).
 #include <algorithm>
+#include <mutex>
 
 #include "pq_ctx.h"
 
@@ -139,6 +140,46 @@ __global__ void k_verdicts(const uint8_t *ok, const uint64_t *hf, const uint64_t
     }
 }
 
+// The passes over records already on the device: hashes, then first[] and the
+// verdicts until no verdict changes.  hf/hr: nrec * nf positions each, first:
+// bits entries, changed: one word; every one of them is filled here before it is
+// read, so the buffers may hold anything on entry.
+template <typename T>
+int dedup_passes(pq_ctx_t *ctx, const uint8_t *d_codes, const uint8_t *d_ok, uint64_t nrec,
+                 uint32_t L, uint64_t bits, uint32_t nf, uint8_t *d_keep, uint64_t *d_hf,
+                 uint64_t *d_hr, uint32_t *d_first, uint32_t *d_changed, uint32_t *passes)
+{
+    hipStream_t s = ctx->stream;
+    const unsigned B = 256, gr = pq_blocks_gs(nrec, B);
+    hipLaunchKernelGGL(k_read_hashes<T>, dim3(gr), dim3(B), 0, s, d_codes, d_ok, nrec, L,
+                       (T) bits, nf, d_hf, d_hr);
+    hipLaunchKernelGGL(k_fill_u32, dim3(pq_blocks_gs(bits, B)), dim3(B), 0, s, d_first, bits,
+                       FIRST_NONE);
+    hipLaunchKernelGGL(k_keep_init, dim3(gr), dim3(B), 0, s, d_ok, nrec, d_keep, d_changed);
+    PQ_CHECK(hipGetLastError());
+    // after pass k the first k verdicts are final: n + 1 passes always suffice
+    for (uint64_t pass = 1;; pass++) {
+        if (pass > 1)
+            hipLaunchKernelGGL(k_first_clear, dim3(gr), dim3(B), 0, s, d_ok, d_hf, nrec, nf,
+                               d_first, d_changed);
+        hipLaunchKernelGGL(k_first_mark, dim3(gr), dim3(B), 0, s, d_keep, d_hf, nrec, nf,
+                           d_first);
+        hipLaunchKernelGGL(k_verdicts, dim3(gr), dim3(B), 0, s, d_ok, d_hf, d_hr, d_first, nrec,
+                           nf, d_keep, d_changed);
+        PQ_CHECK(hipGetLastError());
+        uint32_t changed = 0;
+        PQ_CHECK(hipMemcpyAsync(&changed, d_changed, 4, hipMemcpyDeviceToHost, s));
+        PQ_CHECK(hipStreamSynchronize(s));
+        if (!changed) {
+            if (passes)
+                *passes = (uint32_t) std::min<uint64_t>(pass, 0xFFFFFFFFu);
+            return PQ_OK;
+        }
+        if (pass > nrec + 1)
+            return ctx->fail(PQ_ERR_STATE, "duplicate removal did not settle");
+    }
+}
+
 template <typename T>
 int dedup_t(pq_ctx_t *ctx, const uint8_t *codes, const uint8_t *ok, uint64_t nrec, uint32_t L,
             uint64_t bits, uint32_t nf, uint8_t *keep, uint32_t *passes)
@@ -170,34 +211,10 @@ int dedup_t(pq_ctx_t *ctx, const uint8_t *codes, const uint8_t *ok, uint64_t nre
     auto run = [&]() -> int {
         PQ_CHECK(hipMemcpyAsync(d_codes, codes, nrec * L, hipMemcpyHostToDevice, s));
         PQ_CHECK(hipMemcpyAsync(d_ok, ok, nrec, hipMemcpyHostToDevice, s));
-        const unsigned B = 256, gr = pq_blocks_gs(nrec, B);
-        hipLaunchKernelGGL(k_read_hashes<T>, dim3(gr), dim3(B), 0, s, d_codes, d_ok, nrec, L,
-                           (T) bits, nf, d_hf, d_hr);
-        hipLaunchKernelGGL(k_fill_u32, dim3(pq_blocks_gs(bits, B)), dim3(B), 0, s, d_first, bits,
-                           FIRST_NONE);
-        hipLaunchKernelGGL(k_keep_init, dim3(gr), dim3(B), 0, s, d_ok, nrec, d_keep, d_changed);
-        PQ_CHECK(hipGetLastError());
-        // after pass k the first k verdicts are final: n + 1 passes always suffice
-        for (uint64_t pass = 1;; pass++) {
-            if (pass > 1)
-                hipLaunchKernelGGL(k_first_clear, dim3(gr), dim3(B), 0, s, d_ok, d_hf, nrec, nf,
-                                   d_first, d_changed);
-            hipLaunchKernelGGL(k_first_mark, dim3(gr), dim3(B), 0, s, d_keep, d_hf, nrec, nf,
-                               d_first);
-            hipLaunchKernelGGL(k_verdicts, dim3(gr), dim3(B), 0, s, d_ok, d_hf, d_hr, d_first,
-                               nrec, nf, d_keep, d_changed);
-            PQ_CHECK(hipGetLastError());
-            uint32_t changed = 0;
-            PQ_CHECK(hipMemcpyAsync(&changed, d_changed, 4, hipMemcpyDeviceToHost, s));
-            PQ_CHECK(hipStreamSynchronize(s));
-            if (!changed) {
-                if (passes)
-                    *passes = (uint32_t) std::min<uint64_t>(pass, 0xFFFFFFFFu);
-                break;
-            }
-            if (pass > nrec + 1)
-                return ctx->fail(PQ_ERR_STATE, "duplicate removal did not settle");
-        }
+        int rc = dedup_passes<T>(ctx, d_codes, d_ok, nrec, L, bits, nf, d_keep, d_hf, d_hr,
+                                 d_first, d_changed, passes);
+        if (rc)
+            return rc;
         PQ_CHECK(hipMemcpyAsync(keep, d_keep, nrec, hipMemcpyDeviceToHost, s));
         PQ_CHECK(hipStreamSynchronize(s));
         return PQ_OK;
@@ -208,6 +225,35 @@ int dedup_t(pq_ctx_t *ctx, const uint8_t *codes, const uint8_t *ok, uint64_t nre
 }
 
 }  // namespace
+
+// Scratch of pq_dedup_reads_noalloc: one copy per device, held until the
+// process ends.  In host code the name `scratch` is a host-side stand-in, not a
+// device address: host code uses only the per-device address that
+// hipGetSymbolAddress returns (noalloc_scratch).
+static __device__ __align__(256) uint8_t scratch[PQ_NOALLOC_SCRATCH_BYTES];
+
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;
+static void *noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+// Address of this device's copy of `scratch`, looked up on the first call on
+// the device (the device current), or nullptr.
+static void *noalloc_scratch(pq_ctx_t *ctx)
+{
+    if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES)
+        return nullptr;
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[ctx->device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void) hipGetLastError();
+            return nullptr;
+        }
+        noalloc_ptr[ctx->device] = p;
+    }
+    return noalloc_ptr[ctx->device];
+}
+
 
 extern "C" int pq_dedup_reads(pq_ctx_t *ctx, const uint8_t *codes, const uint8_t *ok,
                               uint64_t nrec, int read_len, uint64_t bloom_bits, int n_hashfunc,
@@ -243,4 +289,58 @@ extern "C" int pq_dedup_reads(pq_ctx_t *ctx, const uint8_t *codes, const uint8_t
                                    (uint32_t) n_hashfunc, keep, passes)
                : dedup_t<uint64_t>(ctx, codes, ok, nrec, (uint32_t) read_len, bloom_bits,
                                    (uint32_t) n_hashfunc, keep, passes);
+}
+
+static uint64_t noalloc_bytes(uint64_t nrec, uint64_t bloom_bits, int n_hashfunc)
+{
+    // hash positions of the reads and their reverse complements, first[] per
+    // filter bit, the change flag; each part on a 256-byte boundary
+    auto up = [](uint64_t b) { return (b + 255) & ~(uint64_t) 255; };
+    return 2 * up(nrec * (uint64_t) n_hashfunc * 8) + up(bloom_bits * 4) + up(4);
+}
+
+extern "C" int pq_dedup_reads_noalloc(pq_ctx_t *ctx, const uint8_t *d_codes, const uint8_t *d_ok,
+                                      uint64_t nrec, int read_len, uint64_t bloom_bits,
+                                      int n_hashfunc, int index_bits, uint8_t *d_keep,
+                                      uint32_t *passes)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (passes)
+        *passes = 0;
+    if (read_len <= 0 || n_hashfunc < 1 || n_hashfunc > 5 ||
+        (index_bits != 32 && index_bits != 64) || (nrec && (!d_codes || !d_ok || !d_keep)))
+        return ctx->fail(PQ_ERR_ARG, "pq_dedup_reads_noalloc: bad arguments");
+    if (nrec == 0)
+        return PQ_OK;
+    if (bloom_bits == 0)
+        return ctx->fail(PQ_ERR_LIMIT, "duplicate filter of 0 bits");
+    if (index_bits == 32 && bloom_bits > 0xFFFFFFFFull)
+        return ctx->fail(PQ_ERR_LIMIT, "duplicate filter beyond the 32-bit index width");
+    const uint64_t need = noalloc_bytes(nrec, bloom_bits, n_hashfunc);
+    if (nrec >= FIRST_NONE || bloom_bits > PQ_NOALLOC_SCRATCH_BYTES ||
+        need > PQ_NOALLOC_SCRATCH_BYTES)
+        return ctx->fail(PQ_ERR_LIMIT,
+                         "pq_dedup_reads_noalloc: " + std::to_string(nrec) + " records and a filter of " +
+                             std::to_string(bloom_bits) + " bits need " + std::to_string(need) +
+                             " bytes of scratch; its fixed scratch holds " +
+                             std::to_string((uint64_t) PQ_NOALLOC_SCRATCH_BYTES));
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev != ctx->device)
+        if (hipSetDevice(ctx->device) != hipSuccess)
+            return ctx->fail(PQ_ERR_HIP, "hipSetDevice");
+    uint8_t *base = (uint8_t *) noalloc_scratch(ctx);
+    if (!base)
+        return ctx->fail(PQ_ERR_HIP, "address of the fixed dedup scratch");
+    auto up = [](uint64_t b) { return (b + 255) & ~(uint64_t) 255; };
+    const uint32_t nf = (uint32_t) n_hashfunc;
+    uint64_t *d_hf = (uint64_t *) base;
+    uint64_t *d_hr = (uint64_t *) (base + up(nrec * nf * 8));
+    uint32_t *d_first = (uint32_t *) (base + 2 * up(nrec * nf * 8));
+    uint32_t *d_changed = (uint32_t *) (base + 2 * up(nrec * nf * 8) + up(bloom_bits * 4));
+    return index_bits == 32
+               ? dedup_passes<uint32_t>(ctx, d_codes, d_ok, nrec, (uint32_t) read_len, bloom_bits,
+                                        nf, d_keep, d_hf, d_hr, d_first, d_changed, passes)
+               : dedup_passes<uint64_t>(ctx, d_codes, d_ok, nrec, (uint32_t) read_len, bloom_bits,
+                                        nf, d_keep, d_hf, d_hr, d_first, d_changed, passes);
 }
