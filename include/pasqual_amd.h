@@ -221,6 +221,10 @@ int pq_learned_stats(pq_ctx_t *ctx, pq_learned_stats_t *st);
  * b1[hidden], w2[hidden], b2.  *count in: capacity, out: number of parameters
  * (params may be NULL to query it). */
 int pq_learned_params(pq_ctx_t *ctx, int dir, float *params, uint32_t *count);
+/* The trained network of direction dir evaluated on n k-code prefix values xs
+ * (first code most significant), as the learned search evaluates it on the
+ * matrix cores (mfma = 1) or by the scalar form (mfma = 0). */
+int pq_mlp_eval(pq_ctx_t *ctx, int dir, const uint64_t *xs, uint32_t n, int mfma, float *out);
 /* Host copy of the exact part of the learned index of direction dir: the distinct
  * D-code prefixes of the suffixes with at least D codes, in suffix-array order.
  * keys: two uint64 per entry (high half first; first code in the top bits, 2 bits

@@ -941,6 +941,24 @@ double orc_mlp_loss_grad(uint32_t k, uint32_t h, const double *params, const uin
     return loss / (double) n;
 }
 
+void orc_mlp_forward(uint32_t k, uint32_t h, const double *params, const uint64_t *xs,
+                     uint64_t n, double *out)
+{
+    const double *W1 = params, *b1 = params + (size_t) h * k, *w2 = b1 + h, *b2 = w2 + h;
+    for (uint64_t s = 0; s < n; s++) {
+        double y = b2[0];
+        for (uint32_t i = 0; i < h; i++) {
+            double a = b1[i];
+            for (uint32_t j = 0; j < k; j++) {
+                unsigned c = (unsigned) (xs[s] >> (2 * (k - 1 - j))) & 3u;
+                a += W1[(size_t) i * k + j] * ((double) c * 2.0 / 3.0 - 1.0);
+            }
+            y += a > 0 ? w2[i] * a : 0.0;
+        }
+        out[s] = y;
+    }
+}
+
 /* ---- first-triplet slot sizes (sufsort.c:386-450) ------------------------------ */
 
 void orc_slot_sizes(const uint8_t *dtext, const uint64_t *off, uint64_t n, uint64_t size[5],
@@ -1126,4 +1144,110 @@ int orc_import_fasta(const char *path, int read_len, int bloom_m, int n_hashfunc
     *n = kept;
     *removed = dropped;
     return 0;
+}
+
+
+/* ---- work of the device's candidate search ----------------------------------- */
+
+/* rank_c(i) = #{j < i : bwt[j] == c} from per-64-row counts and a scan. */
+typedef struct {
+    const uint8_t *bwt;
+    uint64_t *blk;     /* 5 counts per 64-row block, before the block */
+} orc_rank_t;
+
+static uint64_t chain_rank(const orc_rank_t *R, unsigned c, uint64_t i)
+{
+    uint64_t b = i >> 6, k = R->blk[b * 5 + c];
+    for (uint64_t j = b << 6; j < i; j++)
+        k += R->bwt[j] == c;
+    return k;
+}
+
+/* The code a chain prepends at depth o: n[L-o] (FB), its complement (FF),
+ * n[o-1] (BF), its complement (BB). */
+static unsigned chain_code(const uint8_t *s, uint32_t L, int ch, uint32_t o)
+{
+    const unsigned x = ch < 2 ? s[L - o] : s[o - 1];
+    return (ch == 1 || ch == 3) ? 5u - x : x;
+}
+
+int orc_chain_work(const uint8_t *text, const uint64_t *off, uint64_t n, uint32_t seed_depth,
+                   uint64_t out[2])
+{
+    out[0] = out[1] = 0;
+    const uint64_t N = off[n];
+    uint64_t C[6], sfx[4], pfx[4];
+    orc_counts(text, off, n, C, sfx, pfx);
+    uint8_t *dt = malloc(N + 1), *bwt[2] = {malloc(N + 1), malloc(N + 1)};
+    uint64_t *sa = malloc(8 * (N + 1)), *ri = malloc(8 * (n + 1));
+    uint64_t *blk[2] = {malloc(8 * 5 * (N / 64 + 2)), malloc(8 * 5 * (N / 64 + 2))};
+    int rc = 0;
+    if (!dt || !bwt[0] || !bwt[1] || !sa || !ri || !blk[0] || !blk[1]) {
+        rc = -5;
+        goto done;
+    }
+    orc_rank_t R[2];
+    uint64_t cf[2][5];
+    for (int d = 0; d < 2; d++) {
+        orc_dir_text(text, off, n, d, dt);
+        if (orc_sa(dt, off, n, sa)) {
+            rc = -5;
+            goto done;
+        }
+        orc_bwt(dt, off, n, sa, bwt[d], ri);
+        uint64_t k[5] = {0, 0, 0, 0, 0};
+        for (uint64_t i = 0; i <= N; i++) {
+            if ((i & 63) == 0)
+                memcpy(blk[d] + (i >> 6) * 5, k, sizeof(k));
+            if (i < N)
+                k[bwt[d][i]]++;
+        }
+        R[d].bwt = bwt[d];
+        R[d].blk = blk[d];
+        /* rows of suffixes starting with code c: after all rows starting with a
+         * smaller code (C[c]) and the one-code suffixes "c" that end a read of
+         * this direction (they follow no row); in the order of their tails */
+        for (unsigned c = 1; c <= 4; c++)
+            cf[d][c] = C[c] + (d ? pfx[c - 1] : sfx[c - 1]);
+        cf[d][0] = 0;
+    }
+    for (uint64_t r = 0; r < n; r++) {
+        const uint8_t *s = text + off[r];
+        const uint32_t L = (uint32_t) (off[r + 1] - off[r]);
+        for (int ch = 0; ch < 4; ch++) {
+            const int d = (ch == 0 || ch == 3) ? 0 : 1;
+            const uint64_t minsize = (ch == 0 || ch == 2) ? 2 : 1;
+            unsigned c = chain_code(s, L, ch, 1);
+            uint64_t l = C[c], size = C[c + 1] - C[c];
+            uint32_t o = 1;
+            if (seed_depth >= 1 && seed_depth < L) {
+                for (; o < seed_depth; o++) {
+                    c = chain_code(s, L, ch, o + 1);
+                    const uint64_t a = chain_rank(&R[d], c, l), b = chain_rank(&R[d], c, l + size);
+                    l = cf[d][c] + a;
+                    size = b - a;
+                }
+            }
+            while (o < L && size >= minsize) {
+                out[0] += 2;
+                out[1] += (l >> 7) == ((l + size) >> 7) ? 1 : 2;
+                if (o + 1 >= L)
+                    break;
+                c = chain_code(s, L, ch, o + 1);
+                const uint64_t a = chain_rank(&R[d], c, l), b = chain_rank(&R[d], c, l + size);
+                l = cf[d][c] + a;
+                size = b - a;
+                o++;
+            }
+        }
+    }
+done:
+    free(dt);
+    free(bwt[0]);
+    free(bwt[1]);
+    free(sa);
+    free(ri);
+    free(blk[0]);
+    free(blk[1]);
+    return rc;
 }

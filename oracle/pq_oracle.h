@@ -105,6 +105,22 @@ int orc_search_explicit(const uint8_t *text, const uint64_t *off, uint64_t n,
 
 void orc_graph_free(orc_graph_t *g);
 
+/* Work of the device's candidate search (no counterpart in the reference, whose
+ * find_intervals counts differently: the chains of pq_search.hip's k_find
+ * kernels).  Per read of L codes four backward searches, each extending its
+ * pattern by one code per step: FB n[L-o..L) in the forward index, FF
+ * comp(n[L-o..L)) in the backward index, BF rev(n[0..o)) in the backward index,
+ * BB comp(rev(n[0..o))) in the forward index.  A chain starts at depth o = 1
+ * with the interval of its first code (or, when seed_depth is in 1..L-1, at
+ * o = seed_depth with the interval of its seed_depth-code pattern) and steps
+ * while o < L and the interval holds at least 2 rows (FB, BF) or 1 row (FF,
+ * BB), the last step at o = L - 1.  A step evaluates 2 rank positions, l and
+ * u + 1 of the interval [l, u], on 1 occurrence line of 128 rows when both
+ * fall in the same one, else on 2.  out[0] = rank positions, out[1] = lines,
+ * summed over all reads.  Returns 0, or -5 without memory. */
+int orc_chain_work(const uint8_t *text, const uint64_t *off, uint64_t n, uint32_t seed_depth,
+                   uint64_t out[2]);
+
 /* Property check of a suffix array at any size: a permutation of 0..N-1 in
  * strictly increasing (suffix, position) order.  Returns -1 when it holds,
  * else the first offending SA index (or N if not a permutation). */
@@ -120,6 +136,11 @@ int64_t orc_check_sa(const uint8_t *dtext, const uint64_t *off, uint64_t n,
  * of the loss, which this function also gives. */
 double orc_mlp_loss_grad(uint32_t k, uint32_t h, const double *params, const uint64_t *xs,
                          const double *targets, uint64_t n, double *grad);
+
+/* Outputs of the same MLP in double: out[i] = b2 + sum_j w2[j] relu(b1[j] +
+ * sum_m W1[j][m] f_m(x_i)), the function whose loss orc_mlp_loss_grad takes. */
+void orc_mlp_forward(uint32_t k, uint32_t h, const double *params, const uint64_t *xs,
+                     uint64_t n, double *out);
 
 /* First-triplet slot sizes of a direction text (initial_sort, sufsort.c:
  * 386-450): the triplet code of position g is 1 + 21 c0 + [rem >= 2] (1 + 5 c1 +

@@ -61,6 +61,8 @@ def lib():
                                           C.POINTER(C.c_uint64), C.POINTER(C.c_long)]
         _lib.orc_dedup.argtypes = [P, P, C.c_uint64, C.c_int, C.c_uint64, C.c_int, C.c_int, P]
         _lib.orc_slot_sizes.argtypes = [P, P, C.c_uint64, P, P, P]
+        _lib.orc_chain_work.argtypes = [P, P, C.c_uint64, C.c_uint32, P]
+        _lib.orc_mlp_forward.argtypes = [C.c_uint32, C.c_uint32, P, P, C.c_uint64, P]
     return _lib
 
 
@@ -239,6 +241,18 @@ def search_explicit(codes, off, t):
     return _graph_to_dict(g)
 
 
+def chain_work(codes, off, seed_depth=0):
+    """(rank positions, occurrence lines) the device's candidate-search chains
+    evaluate over all reads (orc_chain_work); seed_depth > 0: chains of reads
+    longer than it start at that depth (the learned search's seeds)."""
+    codes = np.ascontiguousarray(codes, np.uint8)
+    off = np.ascontiguousarray(off, np.uint64)
+    out = np.zeros(2, np.uint64)
+    if lib().orc_chain_work(_p(codes), _p(off), len(off) - 1, int(seed_depth), _p(out)) != 0:
+        raise RuntimeError("oracle chain work failed")
+    return int(out[0]), int(out[1])
+
+
 # ---- reference programs (oracle/_ref) ------------------------------------------------
 
 def ref_available(width="u32"):
@@ -314,6 +328,15 @@ def run_reference(fasta, L, t, workdir, width="u32", threads=None, extra=()):
         raise RuntimeError(f"reference failed: {r.stderr}")
     with open(os.path.join(workdir, "contigs.fa"), "rb") as f:
         return r.stdout, f.read()
+
+
+def mlp_forward(k, h, params, xs):
+    """Outputs of the learned index's MLP in double (orc_mlp_forward)."""
+    params = np.ascontiguousarray(params, np.float64)
+    xs = np.ascontiguousarray(xs, np.uint64)
+    out = np.zeros(max(1, xs.size), np.float64)
+    lib().orc_mlp_forward(k, h, _p(params), _p(xs), xs.size, _p(out))
+    return out[:xs.size]
 
 
 def mlp_loss_grad(k, h, params, xs, targets, want_grad=True):

@@ -40,7 +40,7 @@ PQ_SYMBOLS = [
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
     "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read", "pq_probe_gather", "pq_set_edge_capacity",
     "pq_get_slot_sizes", "pq_graph_export", "pq_host_alloc", "pq_host_free",
-    "pq_dedup_reads_noalloc", "pq_dev_alloc", "pq_dev_free", "pq_dev_upload", "pq_dev_download",
+    "pq_mlp_eval", "pq_dedup_reads_noalloc", "pq_dev_alloc", "pq_dev_free", "pq_dev_upload", "pq_dev_download",
 ]
 NOALLOC_SCRATCH_BYTES = 32 << 20   # PQ_NOALLOC_SCRATCH_BYTES
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
@@ -177,6 +177,7 @@ def lib(width="u32"):
         "pq_graph_export": (I32, [P, P, P, U32, P]),
         "pq_host_alloc": (I32, [P, U64, C.POINTER(P)]),
         "pq_host_free": (None, [P]),
+        "pq_mlp_eval": (I32, [P, I32, P, U32, I32, P]),
         "pq_dedup_reads_noalloc": (I32, [P, P, P, U64, I32, U64, I32, I32, P, P]),
         "pq_dev_alloc": (I32, [P, U64, C.POINTER(P)]),
         "pq_dev_free": (None, [P]),
@@ -507,6 +508,15 @@ class Context:
         self._check(self._L.pq_learned_table(self._h, d, keys.ctypes.data, first.ctypes.data,
                                              rows.ctypes.data, C.byref(cnt)), "learned_table")
         return keys, first, rows
+
+    def mlp_eval(self, d, xs, mfma=True):
+        """Outputs of direction d's trained network on prefix values xs (the
+        search's matrix-core form, or the scalar form)."""
+        xs = np.ascontiguousarray(xs, np.uint64)
+        out = np.zeros(max(1, xs.size), np.float32)
+        self._check(self._L.pq_mlp_eval(self._h, d, xs.ctypes.data, xs.size, int(bool(mfma)),
+                                        out.ctypes.data), "mlp_eval")
+        return out[:xs.size]
 
     def mlp_gradients(self, k, hidden, params, xs, targets):
         params = np.ascontiguousarray(params, np.float32)

@@ -1269,6 +1269,54 @@ extern "C" int pq_mlp_gradients(pq_ctx_t *ctx, uint32_t k, uint32_t hidden, cons
     return PQ_OK;
 }
 
+// The trained network of one direction on n prefix values: the search's matrix-
+// core form (mfma = 1: one wave per 64 values, inference parameters in LDS) or
+// the scalar padded form (mfma = 0).
+__global__ void __launch_bounds__(64) k_mlp_eval(const float *iparams, MlpDims d,
+                                                 const uint64_t *xs, uint32_t n, int mfma,
+                                                 float *out)
+{
+    __shared__ float sp[MLP_P_MAX];
+    for (uint32_t i = threadIdx.x; i < MLP_P_MAX; i += blockDim.x)
+        sp[i] = iparams[i];
+    __syncthreads();
+    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    const uint64_t x = i < n ? xs[i] : 0;
+    float y;
+    if (mfma)
+        y = mlp_forward_mfma(sp, d, x);          // every lane of the wave takes part
+    else
+        y = mlp_forward_padded(sp, d, x);
+    if (i < n)
+        out[i] = y;
+}
+
+extern "C" int pq_mlp_eval(pq_ctx_t *ctx, int dir, const uint64_t *xs, uint32_t n, int mfma,
+                           float *out)
+{
+    if (!ctx || dir < 0 || dir > 1 || (n && (!xs || !out)))
+        return PQ_ERR_ARG;
+    Learned *LM = ctx->learned;
+    if (!LM)
+        return ctx->fail(PQ_ERR_STATE, "learned index not trained");
+    if (n == 0)
+        return PQ_OK;
+    PQ_CHECK(hipSetDevice(ctx->device));
+    char *w = (char *) ctx->tmp(SLOT_LEARNED + 1, (size_t) n * 12 + 256);
+    if (!w)
+        return ctx->fail(PQ_ERR_NOMEM, "evaluation scratch");
+    uint64_t *dx = (uint64_t *) w;
+    float *dy = (float *) (dx + n);
+    PQ_CHECK(hipMemcpyAsync(dx, xs, (size_t) n * 8, hipMemcpyHostToDevice, ctx->stream));
+    MlpDims dims = {LM->k, LM->h};
+    hipLaunchKernelGGL(k_mlp_eval, dim3((n + 63) / 64), dim3(64), 0, ctx->stream,
+                       LM->dir[dir].iparams, dims, dx, n, mfma, dy);
+    PQ_CHECK(hipGetLastError());
+    PQ_CHECK(hipMemcpyAsync(out, dy, (size_t) n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    return PQ_OK;
+}
+
 // Parameters of the trained model of direction dir (0 fw, 1 bw), host copy.
 extern "C" int pq_learned_params(pq_ctx_t *ctx, int dir, float *params, uint32_t *count)
 {

@@ -31,6 +31,41 @@ def test_mlp_gradients_match_oracle(ctx, k, h, n, seed):
     assert abs(loss - want_loss) <= 1e-5 * max(1.0, want_loss)
 
 
+@pytest.mark.parametrize("t", [63, 17])
+def test_mlp_forward_mfma_matches_oracle(ctx, t):
+    # the network as the search evaluates it (matrix cores, 64 keys per wave) and
+    # in scalar form, against the oracle's double forward pass on the trained
+    # parameters, at prefix values of the trained range and at random ones
+    rng = np.random.default_rng(t)
+    codes, off, _ = _random_reads(rng, 20000, 4000, 100, 100, err=0.003)
+    ctx.upload(codes, off, t)
+    ctx.build_index()
+    ctx.learned_train(**SMALL)
+    st = ctx.learned_stats()
+    k = st["k"]
+    for d in (0, 1):
+        params = ctx.learned_params(d)
+        h = (params.size - 1) // (k + 2)
+        keys = ctx.learned_table(d)[0]
+        # the k-code prefixes of table keys (first code in the top bits of the
+        # high half), and random values
+        top = keys[:, 0] >> np.uint64(64 - 2 * k)
+        xs = np.concatenate([top[rng.integers(0, len(top), 3000)],
+                             rng.integers(0, 1 << (2 * k), 1000, dtype=np.uint64)])
+        want = orc.mlp_forward(k, h, params.astype(np.float64), xs)
+        # FP32 over 21 inputs and 64 hidden units: sums of O(1) terms
+        scale = 1.0 + np.abs(params).max() * (k + h)
+        for mfma in (True, False):
+            got = ctx.mlp_eval(d, xs, mfma)
+            err = np.abs(got.astype(np.float64) - want).max()
+            assert err <= 1e-6 * scale, (d, mfma, err)
+        # a planted error in the oracle's output bias shows beyond the tolerance
+        bad = params.astype(np.float64)
+        bad[-1] += 1e-3
+        assert np.abs(ctx.mlp_eval(d, xs, True) - orc.mlp_forward(k, h, bad, xs)).min() > \
+            1e-6 * scale
+
+
 def _search(ctx, codes, off, t, mode, **cfg):
     ctx.upload(codes, off, t)
     ctx.build_index()

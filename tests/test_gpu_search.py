@@ -259,3 +259,30 @@ def test_every_extraction_shape_gives_the_same_graph(ctx):
     _same_graph(ctx.graph(), want, "autotuned")
     ctx.set_tuning(1)
     ctx.set_find_chains(0)
+
+
+@pytest.mark.parametrize("lmin,lmax,t,seed", [(36, 250, 31, 41), (100, 100, 63, 42)])
+def test_search_work_counts_match_oracle(ctx, lmin, lmax, t, seed):
+    # the rank positions and occurrence lines the search kernels report (the
+    # roofline's algorithmic bytes) equal the oracle's count of the same chains,
+    # for each search kernel, from depth 1 and from the learned seeds' depth
+    rng = np.random.default_rng(seed)
+    codes, off, _ = _random_reads(rng, 6000, 2000, lmin, lmax, err=0.003)
+    ctx.upload(codes, off, t)
+    ctx.build_index()
+    want = orc.chain_work(codes, off)
+    for kernel in (1, 2, 3):
+        ctx.set_find_chains(kernel)
+        ctx.find_overlaps()
+        tm = ctx.timing()
+        assert (tm["find_positions"], tm["find_lines"]) == want, f"kernel {kernel}"
+    ctx.learned_train()
+    D = ctx.learned_stats()["depth"]
+    want_l = orc.chain_work(codes, off, D)
+    assert want_l[0] < want[0]
+    for kernel in (1, 2, 3):
+        ctx.set_find_chains(kernel)
+        ctx.find_overlaps(pq.SEARCH_LEARNED)
+        tm = ctx.timing()
+        assert (tm["find_positions"], tm["find_lines"]) == want_l, f"seeded, kernel {kernel}"
+    ctx.set_find_chains(0)

@@ -1,6 +1,7 @@
 """The oracle pinned: against the reference's own outputs (golden fixtures and,
 where the reference programs are built, fresh runs), against brute force on tiny
 inputs, and against its second, independent restatement of the search."""
+import bisect
 import os
 
 import numpy as np
@@ -247,6 +248,25 @@ def test_mlp_gradient_oracle_central_differences(k, h, seed):
         assert np.isclose(loss, np.mean((out - ys) ** 2), rtol=1e-12)
 
 
+@pytest.mark.parametrize("k,h,seed", [(21, 64, 5), (3, 4, 6)])
+def test_mlp_forward_oracle_matches_its_loss(k, h, seed):
+    # the forward oracle against the loss oracle (pinned by central differences
+    # and a closed form above): for one sample, (out - 0)^2 - (out - 1)^2 = 2 out - 1
+    rng = np.random.default_rng(seed)
+    P = h * k + 2 * h + 1
+    params = rng.uniform(-1, 1, P)
+    xs = rng.integers(0, 1 << (2 * k), 32, dtype=np.uint64)
+    out = orc.mlp_forward(k, h, params, xs)
+    for i in range(len(xs)):
+        l0 = orc.mlp_loss_grad(k, h, params, xs[i:i + 1], np.zeros(1), False)[0]
+        l1 = orc.mlp_loss_grad(k, h, params, xs[i:i + 1], np.ones(1), False)[0]
+        assert np.isclose(out[i], (l0 - l1 + 1) / 2, rtol=1e-12, atol=1e-12)
+    # the output bias shifts every output by itself
+    p2 = params.copy()
+    p2[-1] += 0.25
+    assert np.allclose(orc.mlp_forward(k, h, p2, xs) - out, 0.25)
+
+
 @pytest.mark.parametrize("width", ["u32", "u64"])
 @pytest.mark.parametrize("name", CASES)
 def test_slot_sizes_match_reference(name, width):
@@ -263,3 +283,69 @@ def test_slot_sizes_match_reference(name, width):
         size, offs, mx = orc.slot_sizes(orc.dir_text(G["reads"], off, d), off)
         assert size == want[f"size_{tag}"] and offs == want[f"offset_{tag}"], (d, size)
         assert mx == want[f"max_size_{tag}"] and sum(size) == n * L
+
+
+def _chain_work_bruteforce(codes, off, seed_depth=0):
+    # the chains' work from explicit suffix strings: the interval of a pattern is
+    # [number of suffixes sorting before it, + number having it as a prefix) in the
+    # order end of read first, codes 1<2<3<4 (the suffix-array order)
+    n = len(off) - 1
+    texts = [orc.dir_text(codes, off, d) for d in (0, 1)]
+    sufs = []
+    for d in (0, 1):
+        s = []
+        for r in range(n):
+            a, b = int(off[r]), int(off[r + 1])
+            for g in range(a, b):
+                s.append(tuple(int(x) for x in texts[d][g:b]))
+        sufs.append(sorted(s))
+
+    def interval(d, pat):
+        lo = bisect.bisect_left(sufs[d], pat)
+        k = sum(1 for x in sufs[d] if x[:len(pat)] == pat)
+        return lo, k
+
+    pos = lines = 0
+    for r in range(n):
+        s = [int(x) for x in codes[int(off[r]):int(off[r + 1])]]
+        L = len(s)
+        for ch in range(4):
+            d = 0 if ch in (0, 3) else 1
+            minsize = 2 if ch in (0, 2) else 1
+
+            def sym(o):
+                x = s[L - o] if ch < 2 else s[o - 1]
+                return 5 - x if ch in (1, 3) else x
+            o = seed_depth if 1 <= seed_depth < L else 1
+            while o < L:
+                pat = tuple(sym(j) for j in range(o, 0, -1))   # prepended codes first
+                lo, k = interval(d, pat)
+                if k < minsize:
+                    break
+                pos += 2
+                lines += 1 if lo >> 7 == (lo + k) >> 7 else 2
+                o += 1
+    return pos, lines
+
+
+@pytest.mark.parametrize("seed,varlen,depth", [(0, False, 0), (1, True, 0), (2, False, 5),
+                                               (3, True, 7), (4, False, 0)])
+def test_chain_work_matches_bruteforce(seed, varlen, depth):
+    # the oracle's count of the device chains' rank positions and occurrence
+    # lines (FM walk with per-block ranks) equals the count from explicit sorted
+    # suffixes; the repeat-rich genome makes intervals span 128-row lines
+    rng = np.random.default_rng(100 + seed)
+    codes, off, _ = _random_reads(rng, 90, 60, 10, 30 if varlen else 10, err=0.02)
+    want = _chain_work_bruteforce(codes, off, depth)
+    assert orc.chain_work(codes, off, depth) == want
+    assert want[1] > want[0] // 2          # some steps span two lines
+
+
+def test_chain_work_plant_is_caught():
+    # a read whose last code changes moves the count: the count follows the reads
+    rng = np.random.default_rng(7)
+    codes, off, _ = _random_reads(rng, 90, 60, 12, 12, err=0.0)
+    a = orc.chain_work(codes, off)
+    c2 = codes.copy()
+    c2[11] = 5 - c2[11]
+    assert orc.chain_work(c2, off) != a
