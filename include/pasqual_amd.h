@@ -172,13 +172,14 @@ void pq_host_free(void *p);
 /* ---- learned index --------------------------------------------------------------------- */
 /* Counterpart of the reference's interval shortcut (sufsort.c:179-189, used at
  * string_graph.c:203-212, 289-298): every chain of the overlap search resumes at
- * depth D = min(63, t) from intervals that a learned index finds.  Per direction
- * the index holds the distinct D-code prefixes of the suffixes (each with its
- * first suffix-array row and row count) in suffix-array order.  An FP32 MLP
- * (k-code prefix -> hidden ReLU layer -> 1), fitted on the device to the table
- * built from the freshly built suffix array, routes a key to one of `leaves`
- * piecewise-linear leaves (`segments` pieces with exact end points) that predict
- * its table entry; an exact galloping + binary search corrects the prediction.
+ * depth D = min(63, t) from the suffix-array interval of its first D codes,
+ * which a learned index finds.  An FP32 MLP (k-code prefix -> hidden ReLU layer
+ * -> 1), fitted on the device to the freshly built suffix array, routes a
+ * pattern to one of `leaves` piecewise-linear leaves (`segments` pieces with
+ * exact end points) that predict its suffix-array row (to the sampling stride);
+ * an exact correction finds the interval: galloping + binary search over the
+ * keys of every sample_stride-th row (the exact part, 16 bytes per sampled row),
+ * then binary search on the suffix array itself within the bracketing stride.
  * Results equal PQ_SEARCH_PLAIN's for any model.  The model is dropped by
  * pq_upload_reads and pq_build_index. */
 typedef struct {
@@ -188,28 +189,26 @@ typedef struct {
     uint32_t segments;       /* linear pieces per leaf (<= 1024), 256 */
     uint32_t epochs;         /* Adam passes over the sampled training set, 1 */
     uint32_t batch;          /* minibatch size (multiple of 64), 4096 */
-    uint32_t samples;        /* training entries drawn from each table (multiple of 64) */
+    uint32_t samples;        /* training samples drawn from the sampled rows (multiple of 64) */
     float lr;                /* Adam step size (decays linearly to 1%) */
     uint32_t closed_form;    /* 1: solve the output layer by least squares before Adam */
     uint64_t seed;
-    uint64_t table_chunk;    /* suffix-array rows per step of the table build (24 bytes of
-                                scratch per row), 0: the whole array in one step when its
-                                scratch and a table of one entry per row fit in 3/4 of the
-                                free device memory, else steps of 2^28 rows */
+    uint64_t sample_stride;  /* suffix-array rows per sampled key of the exact part (8) */
 } pq_learned_cfg_t;
 
 typedef struct {
-    double mean_window[2];   /* mean |exact entry - predicted entry| over the last learned
-                                search's lookups, per index (fw, bw) */
+    double mean_window[2];   /* mean |exact - predicted sampled row| over the last learned
+                                search's lookups, per index (fw, bw), in sampled rows */
     uint64_t max_window[2];  /* largest such correction window */
-    double train_loss[2];    /* mean squared error of the MLP on entry / table size */
+    double train_loss[2];    /* mean squared error of the MLP on row / rows */
     double fit_loss[2];      /* the same before Adam (after the closed-form layer) */
     double train_ms;         /* wall time of pq_learned_train (both directions) */
     uint64_t lookups[2];     /* lookups in the last learned search, per index */
-    uint64_t probes[2];      /* key reads of those lookups */
-    uint64_t table_size[2];  /* distinct D-code prefixes per index */
+    uint64_t probes[2];      /* key reads of those lookups (sampled keys and
+                                suffix-array rows) */
+    uint64_t table_size[2];  /* sampled rows per index */
     uint64_t absent;         /* chains whose depth-D pattern does not occur */
-    uint64_t fallback;       /* always 0: a seed is the searched index's own table entry */
+    uint64_t fallback;       /* always 0: a seed is the searched index's own interval */
     uint32_t depth;          /* D: depth the backward search resumes from */
     uint32_t k;              /* model prefix length used */
 } pq_learned_stats_t;
@@ -225,11 +224,12 @@ int pq_learned_params(pq_ctx_t *ctx, int dir, float *params, uint32_t *count);
  * (first code most significant), as the learned search evaluates it on the
  * matrix cores (mfma = 1) or by the scalar form (mfma = 0). */
 int pq_mlp_eval(pq_ctx_t *ctx, int dir, const uint64_t *xs, uint32_t n, int mfma, float *out);
-/* Host copy of the exact part of the learned index of direction dir: the distinct
- * D-code prefixes of the suffixes with at least D codes, in suffix-array order.
- * keys: two uint64 per entry (high half first; first code in the top bits, 2 bits
- * per code, bit 0 of the low half set); first: first SA row; rows: row count.
- * *count in: capacity, out: number of entries (outputs may be NULL to query). */
+/* Host copy of the exact part of the learned index of direction dir: the keys of
+ * every sample_stride-th suffix-array row, two uint64 per sample (high half
+ * first; the suffix's first D codes, first code in the top bits, codes past the
+ * end of its read 0, bit 0 of the low half set when it has at least D codes);
+ * first: the row; rows: the rows the sample stands for.  *count in: capacity,
+ * out: number of samples (outputs may be NULL to query). */
 int pq_learned_table(pq_ctx_t *ctx, int dir, uint64_t *keys, uint64_t *first, uint32_t *rows,
                      uint64_t *count);
 /* Gradient of (1/n) sum (MLP(x_i) - y_i)^2 over n samples, by the device training

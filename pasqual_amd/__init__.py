@@ -81,7 +81,7 @@ class LearnedCfg(C.Structure):
     _fields_ = [("k", C.c_uint32), ("hidden", C.c_uint32), ("leaves", C.c_uint32),
                 ("segments", C.c_uint32), ("epochs", C.c_uint32), ("batch", C.c_uint32),
                 ("samples", C.c_uint32), ("lr", C.c_float), ("closed_form", C.c_uint32),
-                ("seed", C.c_uint64), ("table_chunk", C.c_uint64)]
+                ("seed", C.c_uint64), ("sample_stride", C.c_uint64)]
 
 
 class LearnedStats(C.Structure):

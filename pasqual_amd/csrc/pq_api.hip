@@ -7,10 +7,23 @@
 
 void pq_index_release(pq_ctx_t *ctx);
 
+// Device memory held by the suffix-array build between builds is released when
+// an allocation outside the build would otherwise fail (see pq_build_index).
+static hipError_t malloc_or_release(pq_ctx_t *ctx, void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess && !ctx->building && ctx->build_scratch_held) {
+        (void) hipGetLastError();
+        pq_release_build_scratch(ctx);
+        e = hipMalloc(p, bytes);
+    }
+    return e;
+}
+
 int pq_alloc(pq_ctx_t *ctx, void **p, size_t bytes)
 {
     *p = nullptr;
-    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    hipError_t e = malloc_or_release(ctx, p, bytes ? bytes : 16);
     if (e != hipSuccess) {
         *p = nullptr;
         (void) hipGetLastError();
@@ -40,7 +53,7 @@ void *pq_ctx::tmp(int slot, size_t bytes)
     b.p = nullptr;
     b.bytes = 0;
     size_t want = bytes + bytes / 4 + 256;
-    if (hipMalloc(&b.p, want) != hipSuccess) {
+    if (malloc_or_release(this, &b.p, want) != hipSuccess) {
         (void) hipGetLastError();
         b.p = nullptr;
         return nullptr;
@@ -169,9 +182,11 @@ extern "C" int pq_get_layout(pq_ctx_t *ctx, uint32_t *row_bytes, uint32_t *slice
     return PQ_OK;
 }
 
-// Scratch of the suffix-array build (a third of the memory of a slice for
-// 64-bit rows): freed after a build with 64-bit rows, whose texts are large
-// enough that the learned tables and the search need that memory.
+// Scratch of the suffix-array build (about 21 bytes per suffix of a slice with
+// 64-bit rows).  It stays allocated between builds: freeing and allocating tens
+// of GiB again costs more than a second on this device (the freed memory is
+// cleared before it can be handed out again), so it is released only when an
+// allocation outside the build does not fit otherwise.
 void pq_release_build_scratch(pq_ctx_t *ctx)
 {
     const int slots[] = {SLOT_RADIX, SLOT_BIGK, SLOT_BIGV, SLOT_BK, SLOT_BK2, SLOT_V2,
@@ -184,6 +199,7 @@ void pq_release_build_scratch(pq_ctx_t *ctx)
             ctx->scratch[s] = DevBuf();
         }
     }
+    ctx->build_scratch_held = false;
 }
 
 extern "C" int pq_build_index(pq_ctx_t *ctx)
@@ -199,6 +215,12 @@ extern "C" int pq_build_index(pq_ctx_t *ctx)
     for (auto &e : ev)
         PQ_CHECK(hipEventCreate(&e));
     int rc = PQ_OK;
+    struct Building {
+        pq_ctx_t *c;
+        ~Building() { c->building = false; }
+    } guard{ctx};
+    ctx->building = true;
+    ctx->build_scratch_held = true;
     for (int d = 0; d < 2 && rc == PQ_OK; d++) {
         PQ_CHECK(hipEventRecord(ev[0], ctx->stream));
         rc = pq_build_sa_dir(ctx, d);
@@ -218,8 +240,6 @@ extern "C" int pq_build_index(pq_ctx_t *ctx)
     }
     for (auto &e : ev)
         (void) hipEventDestroy(e);
-    if (ctx->wide)
-        pq_release_build_scratch(ctx);
     return rc;
 }
 

@@ -55,6 +55,8 @@ struct pq_ctx {
     uint32_t opt_row_bytes = 0;    // 0 automatic, 4 or 8
     uint64_t opt_slice = 0;        // 0 automatic (2^31)
     uint32_t last_slices = 0;      // slices of the last suffix-array build
+    bool building = false;         // inside pq_build_index
+    bool build_scratch_held = false;   // build scratch slots still allocated
 
     DirIndex dir[2];
 

@@ -70,9 +70,8 @@ struct QStats {
 };
 
 struct DirModel {
-    u128 *key = nullptr;          // nU distinct keys
-    void *rows = nullptr;         // RowCount<R>: (first SA row, rows) per key
-    uint64_t nU = 0;
+    u128 *key = nullptr;          // keys of every stride-th SA row (a scratch slot of ctx)
+    uint64_t nU = 0;              // sampled rows
     float *params = nullptr;
     float *iparams = nullptr;     // inference form (rows padded to MLP_K_MAX)
     uint64_t *range = nullptr;    // [xmin, xmax] per leaf
@@ -81,7 +80,7 @@ struct DirModel {
 };
 
 struct Learned {
-    uint32_t k = 21, h = 64, leaves = 4096, segs = 64, depth = 21;
+    uint32_t k = 21, h = 64, leaves = 4096, segs = 64, depth = 21, stride = 8;
     DirModel dir[2];
     QStats *qstats = nullptr;     // device
     double train_ms = 0;
@@ -319,67 +318,31 @@ __device__ __forceinline__ u128 top_mask(uint32_t D)
     return ~(u128) 0 << (128 - 2 * D);
 }
 
-// K[j] = first D codes of suffix SA[r0 + j] (codes past the end of its read read
-// as 0) | (suffix has at least D codes), j < cnt.  Non-decreasing along the SA.
-template <typename R>
-__global__ void k_learned_keys(Geom G, const uint64_t *t2, const R *sa, uint64_t r0,
-                               uint64_t cnt, uint32_t D, u128 *K)
+// Key of the suffix at text position g: its first D codes (codes past the end of
+// its read read as 0) | (the suffix has at least D codes).  Non-decreasing along
+// the suffix array: a suffix shorter than D codes that is a prefix of a pattern
+// sorts before the pattern's rows, as in the suffix array.
+__device__ __forceinline__ u128 suffix_key(const Geom &G, const uint64_t *t2, uint64_t g,
+                                           uint32_t D)
 {
-    uint64_t j = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cnt)
-        return;
-    uint64_t g = sa[r0 + j];
-    uint64_t rem = pq_rem(G, g);
+    const uint64_t rem = pq_rem(G, g);
     // up to 32 codes come from one 64-bit window (two words instead of three)
     u128 w = D <= 32 ? (u128) pq_window(t2, g) << 64 : text_codes(t2, g);
     w &= top_mask(D);
     if (rem < D)
         w &= ~(~(u128) 0 >> (2 * rem));
-    K[j] = w | (u128) (rem >= D ? 1u : 0u);
+    return w | (u128) (rem >= D ? 1u : 0u);
 }
 
-// The table is built over chunks of SA rows [a, a + m): K holds the keys of
-// rows a - off .. a + m (one row before the chunk when a > 0, one after it
-// when it exists), so row a + i has key K[i + off].
-
-// First row of each run of equal full-length keys.
-__global__ void k_table_flags(const u128 *K, uint64_t a, uint64_t m, uint32_t off, uint32_t *flag)
-{
-    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m)
-        return;
-    const u128 *k = K + off + i;
-    flag[i] = ((uint32_t) k[0] & 1u) && (a + i == 0 || k[-1] != k[0]) ? 1u : 0u;
-}
-
+// The exact part of the learned index: K[j] = key of suffix-array row j * stride,
+// j < ns (every stride-th row; the suffix array is read once, the text at one
+// position per stride rows).
 template <typename R>
-__global__ void k_table_starts(const u128 *K, uint64_t a, uint64_t m, uint32_t off,
-                               const uint32_t *flag, const uint32_t *pos, uint64_t base, u128 *key,
-                               RowCount<R> *rows)
+__global__ void k_sample_keys(Geom G, const uint64_t *t2, const R *sa, uint32_t stride,
+                              uint64_t ns, uint32_t D, u128 *K)
 {
-    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m || !flag[i])
-        return;
-    const uint64_t u = base + pos[i];
-    key[u] = K[off + i];
-    rows[u] = RowCount<R>{(R) (a + i), 0u};
-}
-
-// Last row of a run: the run's entry is base + pos[i] + flag[i] - 1 (an entry
-// of an earlier chunk when the run began there).
-template <typename R>
-__global__ void k_table_counts(const u128 *K, uint64_t a, uint64_t m, uint32_t off, uint64_t N,
-                               const uint32_t *flag, const uint32_t *pos, uint64_t base,
-                               RowCount<R> *rows)
-{
-    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m)
-        return;
-    const u128 *k = K + off + i;
-    if (!((uint32_t) k[0] & 1u) || (a + i + 1 < N && k[1] == k[0]))
-        return;
-    const uint64_t u = base + pos[i] + flag[i] - 1u;
-    rows[u].y = (uint32_t) ((R) (a + i + 1) - rows[u].x);
+    PQ_GRID_STRIDE_FOR(j, ns)
+        K[j] = suffix_key(G, t2, (uint64_t) sa[j * stride], D);
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
@@ -389,8 +352,8 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
-// Training rows drawn uniformly from the table: the k-code prefix of the key and
-// entry / nU.
+// Training samples drawn uniformly from the sampled rows: the k-code prefix of
+// the key and (sampled row) / nU, the suffix-array row / N to the stride.
 __global__ void k_sample_rows(const u128 *key, uint64_t nU, uint32_t k, uint32_t n, uint64_t seed,
                               uint64_t *xs, float *targets)
 {
@@ -438,7 +401,7 @@ __device__ __forceinline__ uint64_t knot_x(uint64_t xmin, uint64_t xmax, uint32_
     return xmin + ((xmax - xmin) * s) / S;
 }
 
-// Exact end points of every piece: the first table entry at or after the knot's
+// Exact end points of every piece: the first sampled row at or after the knot's
 // k-code prefix.
 __global__ void k_leaf_knots(const u128 *key, uint64_t nU, uint32_t k, const uint64_t *range,
                              uint32_t leaves, uint32_t S, uint32_t *knots)
@@ -454,13 +417,35 @@ __global__ void k_leaf_knots(const u128 *key, uint64_t nU, uint32_t k, const uin
 // ---- query ------------------------------------------------------------------------------
 
 template <typename R> struct ModelViewT {
-    const u128 *key;
-    const RowCount<R> *rows;
-    uint64_t nU;
+    const u128 *key;              // sampled keys
+    uint64_t nU;                  // sampled rows
+    uint32_t stride;
+    uint64_t N;                   // suffix-array rows
+    const R *sa;
+    const uint64_t *t2;           // packed text of the direction
     const float *iparams;
     const uint64_t *range;
     const uint32_t *knots;
 };
+
+// First row r in [lo, hi) whose suffix key is >= q, given that the key of row
+// hi is >= q (or hi = N) and that of row lo - 1 is < q: binary search on the
+// suffix array itself (a probe reads the row and the text it points at).
+template <typename R>
+__device__ __forceinline__ uint64_t row_lower_bound(const Geom &G, const ModelViewT<R> &M,
+                                                    uint64_t lo, uint64_t hi, u128 q, uint32_t D,
+                                                    uint32_t &probes)
+{
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        probes++;
+        if (suffix_key(G, M.t2, (uint64_t) M.sa[mid], D) < q)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
 
 // Parameters read by every lane of a wave at the same addresses: the pointer is
 // made wave-uniform and moved to the constant address space (scalar loads).
@@ -683,19 +668,28 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
     if (rl < nr) {
         SeedT<R> sd = {0, SEED_NONE};
         if (act) {
-            const u128 key = pat | 1u;
-            uint64_t g1 = leaf_entry(y, Sv, leaves, S, xk);
-            uint64_t u = lower_bound_from(Sv.key, Sv.nU, key, g1, probes[sdir]);
+            // the pattern's rows: the first row whose key is >= key, up to the first
+            // row whose key is > key; the model predicts the sampled row, the
+            // samples bracket the answer to a stride of rows, the suffix array
+            // finishes it
+            const u128 key = pat | 1u, above = key + 1u;   // bit 1 of a key is never set
+            const uint64_t st = Sv.stride, ns = Sv.nU;
+            const uint64_t g1 = leaf_entry(y, Sv, leaves, S, xk);
+            const uint64_t jl = lower_bound_from(Sv.key, ns, key, g1, probes[sdir]);
             looks[sdir]++;
-            win[sdir] = u > g1 ? u - g1 : g1 - u;
-            if (u < Sv.nU && Sv.key[u] == key) {
-                const RowCount<R> iv = Sv.rows[u];
-                sd.l = iv.x;
-                sd.size = iv.y;
-            } else {
-                sd.size = 0;   // no suffix starts with the pattern: the chain is empty
-                absent = 1;
-            }
+            win[sdir] = jl > g1 ? jl - g1 : g1 - jl;
+            const uint64_t lo = row_lower_bound(G, Sv, jl > 0 ? (jl - 1) * st + 1 : 0,
+                                                jl < ns ? jl * st : Sv.N, key, D, probes[sdir]);
+            uint64_t jh = jl;
+            if (jl < ns && !(Sv.key[jl] > key))
+                jh = lower_bound_from(Sv.key, ns, above, jl, probes[sdir]);
+            const uint64_t hlo = jh > 0 ? (jh - 1) * st + 1 : 0;
+            const uint64_t hi = row_lower_bound(G, Sv, hlo > lo ? hlo : lo,
+                                                jh < ns ? jh * st : Sv.N, above, D, probes[sdir]);
+            sd.l = (R) lo;
+            // (an interval of 2^32 - 1 rows or more takes the plain start instead)
+            sd.size = hi - lo < SEED_NONE ? (uint32_t) (hi - lo) : SEED_NONE;
+            absent = hi == lo;   // no suffix starts with the pattern: the chain is empty
         }
         seeds[rl * 4 + ch] = sd;
     }
@@ -747,8 +741,6 @@ void pq_learned_free(pq_ctx_t *ctx)
         return;
     for (int d = 0; d < 2; d++) {
         DirModel &M = ctx->learned->dir[d];
-        pq_free(M.key);
-        pq_free(M.rows);
         pq_free(M.params);
         pq_free(M.iparams);
         pq_free(M.range);
@@ -773,7 +765,7 @@ extern "C" void pq_learned_default(pq_learned_cfg_t *cfg)
     cfg->lr = 1e-5f;
     cfg->closed_form = 1;
     cfg->seed = 12345;
-    cfg->table_chunk = 0;
+    cfg->sample_stride = 8;
 }
 
 static uint64_t host_mix(uint64_t z)
@@ -869,103 +861,28 @@ static int upload_inference_params(pq_ctx_t *ctx, const Learned &LM, DirModel &M
     return PQ_OK;
 }
 
-// Rows per table-build chunk when the caller leaves it to the library.  The
-// chunk's scratch (24 bytes per row: key, flag, position) is held while the
-// table itself is allocated (entry_bytes per entry, at most one entry per
-// row): all N rows in one chunk when both fit in three quarters of the free
-// device memory, else chunks of 2^28 rows (6 GiB of scratch).
-static uint64_t default_table_chunk(uint64_t N, uint64_t entry_bytes)
-{
-    const uint64_t fallback = std::min<uint64_t>(N, 1ull << 28);
-    size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess)
-        return fallback;
-    const uint64_t budget = (uint64_t) free_b / 4 * 3;
-    return 24 * N + entry_bytes * (N + 1) <= budget ? N : fallback;
-}
-
-// Table of distinct full-length D-prefixes of direction d, from its SA, in
-// chunks of SA rows (the keys take 16 bytes per row): entries counted per
-// chunk, then written at the chunk's base (one pass when one chunk holds N).
-template <typename R>
-static int build_table_t(pq_ctx_t *ctx, Learned &LM, int d, uint64_t chunk)
+// Sampled keys of direction d: one gather per stride rows into a scratch slot of
+// the context that is kept between trainings (no allocation per training once
+// it is large enough).
+template <typename R> static int build_samples_t(pq_ctx_t *ctx, Learned &LM, int d)
 {
     DirModel &M = LM.dir[d];
-    const uint64_t N = ctx->N;
-    const uint64_t CH = std::min<uint64_t>(
-        N, chunk ? chunk : default_table_chunk(N, sizeof(u128) + sizeof(RowCount<R>)));
-    const uint64_t nch = (N + CH - 1) / CH;
-    u128 *K = (u128 *) ctx->tmp(SLOT_LEARNED + 3, (CH + 2) * sizeof(u128) + 16);
-    uint32_t *flag = (uint32_t *) ctx->tmp(SLOT_LEARNED + 4, CH * 4 + 16);
-    uint32_t *pos = (uint32_t *) ctx->tmp(SLOT_LEARNED + 5, CH * 4 + 16);
-    if (!K || !flag || !pos)
-        return ctx->fail(PQ_ERR_NOMEM, "learned index table scratch");
-    Geom G = ctx->geom();
-    const R *sa = (const R *) ctx->dir[d].sa;
-    auto chunk_keys = [&](uint64_t c, uint64_t &a, uint64_t &m, uint32_t &off) -> int {
-        a = c * CH;
-        m = std::min(CH, N - a);
-        off = a > 0 ? 1u : 0u;
-        const uint64_t r0 = a - off, cnt = std::min(a + m + 1, N) - r0;
-        hipLaunchKernelGGL(k_learned_keys<R>, dim3(pq_blocks(cnt, 256)), dim3(256), 0, ctx->stream,
-                           G, ctx->dir[d].text2, sa, r0, cnt, LM.depth, K);
-        hipLaunchKernelGGL(k_table_flags, dim3(pq_blocks(m, 256)), dim3(256), 0, ctx->stream, K,
-                           a, m, off, flag);
-        PQ_CHECK(hipGetLastError());
-        return PQ_OK;
-    };
-    std::vector<uint64_t> base(nch + 1, 0);
-    int rc;
-    uint64_t a = 0, m = 0;
-    uint32_t off = 0;
-    for (uint64_t c = 0; c < nch; c++) {
-        uint32_t cnt = 0;
-        if ((rc = chunk_keys(c, a, m, off)) || (rc = pq_scan_u32(ctx, flag, pos, m, &cnt)))
-            return rc;
-        base[c + 1] = base[c] + cnt;
-    }
-    const uint64_t nU = base[nch];
-    if (nU >= 0xFFFFFFFFull)
-        return ctx->fail(PQ_ERR_LIMIT, "learned table of 2^32 entries or more");
-    M.nU = nU;
-    if ((rc = pq_alloc(ctx, (void **) &M.key, (nU + 1) * sizeof(u128))) ||
-        (rc = pq_alloc(ctx, (void **) &M.rows, (nU + 1) * sizeof(RowCount<R>))))
-        return rc;
-    RowCount<R> *rows = (RowCount<R> *) M.rows;
-    for (uint64_t c = 0; c < nch; c++) {
-        // in chunk order: a run's count is written where its last row lies, from
-        // the first row written where it began.  With one chunk its keys, flags
-        // and positions are still in place.
-        if (nch > 1) {
-            uint32_t cnt = 0;
-            if ((rc = chunk_keys(c, a, m, off)) || (rc = pq_scan_u32(ctx, flag, pos, m, &cnt)))
-                return rc;
-        }
-        hipLaunchKernelGGL(k_table_starts<R>, dim3(pq_blocks(m, 256)), dim3(256), 0, ctx->stream,
-                           K, a, m, off, flag, pos, base[c], M.key, rows);
-        hipLaunchKernelGGL(k_table_counts<R>, dim3(pq_blocks(m, 256)), dim3(256), 0, ctx->stream,
-                           K, a, m, off, N, flag, pos, base[c], rows);
-        PQ_CHECK(hipGetLastError());
-    }
+    const uint64_t N = ctx->N, ns = (N + LM.stride - 1) / LM.stride;
+    M.key = (u128 *) ctx->tmp(SLOT_LEARNED + 6 + d, (ns + 1) * sizeof(u128));
+    if (!M.key)
+        return ctx->fail(PQ_ERR_NOMEM, "learned index keys");
+    M.nU = ns;
+    if (ns)
+        hipLaunchKernelGGL(k_sample_keys<R>, dim3(pq_blocks_gs(ns, 256)), dim3(256), 0,
+                           ctx->stream, ctx->geom(), ctx->dir[d].text2, (const R *) ctx->dir[d].sa,
+                           LM.stride, ns, LM.depth, M.key);
+    PQ_CHECK(hipGetLastError());
     return PQ_OK;
 }
 
-// The table build's scratch (keys, flags, positions of one chunk: up to 24
-// bytes per suffix-array row) is freed once both tables exist.
-static void release_table_scratch(pq_ctx_t *ctx)
+static int build_samples(pq_ctx_t *ctx, Learned &LM, int d)
 {
-    (void) hipStreamSynchronize(ctx->stream);
-    for (int s = SLOT_LEARNED + 3; s <= SLOT_LEARNED + 5; s++)
-        if ((size_t) s < ctx->scratch.size() && ctx->scratch[s].p) {
-            pq_free(ctx->scratch[s].p);
-            ctx->scratch[s] = DevBuf();
-        }
-}
-
-static int build_table(pq_ctx_t *ctx, Learned &LM, int d, uint64_t chunk)
-{
-    return ctx->wide ? build_table_t<uint64_t>(ctx, LM, d, chunk)
-                     : build_table_t<uint32_t>(ctx, LM, d, chunk);
+    return ctx->wide ? build_samples_t<uint64_t>(ctx, LM, d) : build_samples_t<uint32_t>(ctx, LM, d);
 }
 
 static int train_dir(pq_ctx_t *ctx, Learned &LM, int d, const pq_learned_cfg_t &cfg)
@@ -975,7 +892,7 @@ static int train_dir(pq_ctx_t *ctx, Learned &LM, int d, const pq_learned_cfg_t &
     const uint32_t P = mlp_nparams(LM.k, LM.h);
     const uint32_t S = cfg.samples;
     const uint32_t NL = LM.leaves, SG = LM.segs;
-    int rc = build_table(ctx, LM, d, cfg.table_chunk);
+    int rc = build_samples(ctx, LM, d);
     if (rc)
         return rc;
     if ((rc = pq_alloc(ctx, (void **) &M.params, P * 4)) ||
@@ -1115,9 +1032,11 @@ static int train_dir(pq_ctx_t *ctx, Learned &LM, int d, const pq_learned_cfg_t &
     return PQ_OK;
 }
 
-template <typename R> static ModelViewT<R> model_view(const DirModel &M)
+template <typename R> static ModelViewT<R> model_view(const pq_ctx_t *ctx, const Learned &LM, int d)
 {
-    return {M.key, (const RowCount<R> *) M.rows, M.nU, M.iparams, M.range, M.knots};
+    const DirModel &M = LM.dir[d];
+    return {M.key,       M.nU,     LM.stride, ctx->N, (const R *) ctx->dir[d].sa,
+            ctx->dir[d].text2, M.iparams, M.range, M.knots};
 }
 
 extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
@@ -1133,7 +1052,8 @@ extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
     if (cfg.k == 0 || cfg.k > MLP_K_MAX || cfg.hidden == 0 || cfg.hidden > MLP_H_MAX ||
         cfg.leaves == 0 || cfg.leaves > (1u << 20) || cfg.segments == 0 ||
         cfg.segments > SEG_MAX || cfg.batch == 0 || cfg.batch % TRAIN_BLOCK ||
-        cfg.samples < cfg.batch || cfg.samples % TRAIN_BLOCK)
+        cfg.samples < cfg.batch || cfg.samples % TRAIN_BLOCK || cfg.sample_stride == 0 ||
+        cfg.sample_stride > 4096)
         return ctx->fail(PQ_ERR_ARG, "learned index configuration out of range");
     if (ctx->N == 0 || ctx->t < 2)
         return ctx->fail(PQ_ERR_STATE, "no suffixes to learn (empty text or t < 2)");
@@ -1149,6 +1069,7 @@ extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
     LM->h = cfg.hidden;
     LM->leaves = cfg.leaves;
     LM->segs = cfg.segments;
+    LM->stride = (uint32_t) cfg.sample_stride;
     ctx->learned = LM;
     ctx->learned_gen++;
     int rc = pq_alloc(ctx, (void **) &LM->qstats, sizeof(QStats));
@@ -1158,7 +1079,6 @@ extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
         rc = ctx->fail(PQ_ERR_HIP, "clearing learned statistics");
     if (rc == PQ_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
         rc = ctx->fail(PQ_ERR_HIP, "learned index training");
-    release_table_scratch(ctx);
     if (rc) {
         pq_learned_free(ctx);
         return rc;
@@ -1202,7 +1122,7 @@ template <typename R>
 static void launch_seeds(pq_ctx_t *ctx, const Learned *LM, uint64_t r0, uint64_t nr,
                          SeedT<R> *d_seeds)
 {
-    ModelViewT<R> v[2] = {model_view<R>(LM->dir[0]), model_view<R>(LM->dir[1])};
+    ModelViewT<R> v[2] = {model_view<R>(ctx, *LM, 0), model_view<R>(ctx, *LM, 1)};
     MlpDims dims = {LM->k, LM->h};
     hipLaunchKernelGGL(k_learned_seed<R>, dim3(pq_blocks(nr, 64)), dim3(256), 0, ctx->stream,
                        ctx->geom(), ctx->dir[0].text2, r0, nr, LM->depth, dims, LM->leaves,
@@ -1337,10 +1257,11 @@ extern "C" int pq_learned_params(pq_ctx_t *ctx, int dir, float *params, uint32_t
     return PQ_OK;
 }
 
-// Host copy of the table of direction dir: per entry the key as two 64-bit
-// halves (high first; first code in the top bits, bit 0 of the low half set),
-// its first SA row and its row count.  *count in: capacity (entries), out: table
-// size (any output may be NULL to query it).
+// Host copy of the sampled keys of direction dir: per sample the key as two
+// 64-bit halves (high first; first code in the top bits, bit 0 of the low half
+// set when the suffix has at least D codes), its suffix-array row (j * stride)
+// and the rows it stands for (the stride; fewer for the last).  *count in:
+// capacity, out: samples (any output may be NULL to query it).
 extern "C" int pq_learned_table(pq_ctx_t *ctx, int dir, uint64_t *keys, uint64_t *first,
                                 uint32_t *rows, uint64_t *count)
 {
@@ -1365,24 +1286,12 @@ extern "C" int pq_learned_table(pq_ctx_t *ctx, int dir, uint64_t *keys, uint64_t
             keys[2 * i + 1] = (uint64_t) k[i];
         }
     }
-    auto copy_rows = [&](auto tag) -> int {
-        using RC = decltype(tag);
-        std::vector<RC> r(M.nU);
-        PQ_CHECK(hipMemcpyAsync(r.data(), M.rows, M.nU * sizeof(RC), hipMemcpyDeviceToHost,
-                                ctx->stream));
-        PQ_CHECK(hipStreamSynchronize(ctx->stream));
-        for (uint64_t i = 0; i < M.nU; i++) {
-            if (first)
-                first[i] = r[i].x;
-            if (rows)
-                rows[i] = r[i].y;
-        }
-        return PQ_OK;
-    };
-    if (first || rows) {
-        int rc = ctx->wide ? copy_rows(RowCount<uint64_t>{}) : copy_rows(RowCount<uint32_t>{});
-        if (rc)
-            return rc;
+    for (uint64_t i = 0; i < M.nU; i++) {
+        const uint64_t r = i * LM->stride;
+        if (first)
+            first[i] = r;
+        if (rows)
+            rows[i] = (uint32_t) std::min<uint64_t>(LM->stride, ctx->N - r);
     }
     return PQ_OK;
 }

@@ -70,12 +70,12 @@ def test_layout_search_and_learned_match_oracle(wctx, row_bytes, slice_, lmin, l
     want = orc.search_explicit(codes, off, t)
     wctx.find_overlaps(pq.SEARCH_PLAIN)
     _same_graph(wctx.graph(), want, "plain")
-    # learned table built in chunks of 997 rows (chunk seams inside runs)
-    wctx.learned_train(samples=1 << 14, batch=1024, epochs=1, table_chunk=997)
+    # sampled keys every 7 rows
+    wctx.learned_train(samples=1 << 14, batch=1024, epochs=1, sample_stride=7)
     D = min(63, t)
     for d in (0, 1):
         keys, first, rows = wctx.learned_table(d)
-        wk, wf, wr = _host_table(codes, off, d, D)
+        wk, wf, wr = _host_table(codes, off, d, D, 7)
         assert [(int(a) << 64) | int(b) for a, b in keys.tolist()] == wk, d
         assert first.tolist() == wf and rows.tolist() == wr, d
     for mode in (pq.SEARCH_LEARNED, pq.SEARCH_LEARNED | pq.SEARCH_GRAPH,
