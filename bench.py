@@ -294,24 +294,30 @@ def _mean(xs):
     return sum(xs) / len(xs) if xs else 0.0
 
 
-def search_roofline(positions, lines, kernel_ms, launches, copy_gbs, read_gbs, gather, kname,
-                    n, steps):
-    """Roofline of the candidate-search kernel from its live HIP-event launch times:
-    algorithmic bytes = 64 B per distinct occurrence line a backward-search step
-    needs (one line when the rows l and u+1 share a line, else two; counted by the
-    kernel, the same count as the oracle's restatement of the chains), against
-    the HBM peak; also the 64 B per rank position form and the measured
-    ceilings."""
+VERIFY_ROW_BYTES = 8 + LINE_BYTES   # a suffix-array row and the text line left of it
+
+
+def search_roofline(positions, lines, rows, kernel_ms, launches, copy_gbs, read_gbs, gather,
+                    kname, n, steps):
+    """Roofline of the candidate-search phase (search kernel + verification) from
+    its live HIP-event times: algorithmic bytes = 64 B per distinct occurrence
+    line a backward-search step needs (one line when the rows l and u+1 share a
+    line, else two) + 72 B per row finished on the text (its suffix-array entry
+    and the 64-byte text line left of it); the kernels count both, the same
+    counts as the oracle's restatement of the chains; against the HBM peak; also
+    the 64 B per rank position form and the measured ceilings."""
     avg_launch_ms = kernel_ms / max(1, launches)
-    alg = lines / max(1, launches) * LINE_BYTES
+    alg = (lines * LINE_BYTES + rows * VERIFY_ROW_BYTES) / max(1, launches)
     pos_bytes = positions / max(1, launches) * LINE_BYTES
     achieved = alg / (avg_launch_ms / 1e3) / 1e9 if avg_launch_ms else 0.0
     best = max(gather["lane"], gather["quad"]) if gather else None
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": kname, "alg_bytes_per_launch": int(alg),
-            "alg_bytes_basis": "64 B x distinct occurrence lines per backward-search step",
+            "alg_bytes_basis": "64 B x distinct occurrence lines per backward-search step + "
+                               "72 B x rows verified on the text",
             "lines_per_read": round(lines / (n * steps), 1),
+            "verified_rows_per_read": round(rows / (n * steps), 1),
             "rank_positions_per_read": round(positions / (n * steps), 1),
             "position_bytes_per_launch": int(pos_bytes),
             "position_bytes_gbs": round(pos_bytes / (avg_launch_ms / 1e3) / 1e9, 1)
@@ -374,6 +380,7 @@ def run_workload(pq, np, config, args, cpu_baseline):
     def use(tn):
         ctx.set_tuning(tn["shape"])
         ctx.set_find_chains(tn["find_chains"])
+        ctx.set_verify(tn["verify_rows"])
 
     def export():
         nonlocal edges_buf
@@ -384,9 +391,9 @@ def run_workload(pq, np, config, args, cpu_baseline):
         return ctx.graph_export(nodes_buf, edges_buf, index_bytes)
 
     P = {"build": [], "search": [], "search_dev": [], "export": [], "kernel": [], "launches": 0,
-         "positions": 0, "lines": 0}
+         "positions": 0, "lines": 0, "rows": 0}
     Lr = {"train": [], "search": [], "search_dev": [], "export": [], "kernel": [], "seed": [],
-          "launches": 0, "positions": 0, "lines": 0, "same": True}
+          "launches": 0, "positions": 0, "lines": 0, "rows": 0, "same": True}
     digests = {}
     t_start = time.perf_counter()
     for step in range(args.steps):
@@ -408,6 +415,7 @@ def run_workload(pq, np, config, args, cpu_baseline):
         P["launches"] += tm["find_launches"]
         P["positions"] += tm["find_positions"]
         P["lines"] += tm["find_lines"]
+        P["rows"] += tm["verify_rows"]
         if step == 0:
             _, E = ctx.graph_size()
             digests["plain"] = (_digest_buf(nodes_buf.array), _digest_buf(
@@ -431,6 +439,7 @@ def run_workload(pq, np, config, args, cpu_baseline):
             Lr["launches"] += tl["find_launches"]
             Lr["positions"] += tl["find_positions"]
             Lr["lines"] += tl["find_lines"]
+            Lr["rows"] += tl["verify_rows"]
             if step == 0:
                 _, E = ctx.graph_size()
                 got = (_digest_buf(nodes_buf.array), _digest_buf(
@@ -502,7 +511,8 @@ def run_workload(pq, np, config, args, cpu_baseline):
                       "unit": "GB/s", "frac": round(build_gbs / HBM_PEAK_GBS, 4),
                       "model_bytes_per_suffix": round(b_suffix, 2), "rounds": rounds,
                       "index_width": W}
-    plain_roof = search_roofline(P["positions"], P["lines"], sum(P["kernel"]), P["launches"],
+    plain_roof = search_roofline(P["positions"], P["lines"], P["rows"], sum(P["kernel"]),
+                                 P["launches"],
                                  copy_gbs, read_gbs, gather, kname[tune["find_chains"]], n,
                                  args.steps)
     plain_roof["traffic"] = committed_traffic(config, plain_roof["kernel"])
@@ -520,13 +530,13 @@ def run_workload(pq, np, config, args, cpu_baseline):
         "edges_digest": digests["plain"][1], "nodes_digest": digests["plain"][0],
         "overlap_len_sum": digests["plain"][2],
         "extraction_shape": {k: tune[k] for k in ("group_lanes", "items")},
-        "find_kernel": tune["find_chains"],
+        "find_kernel": tune["find_chains"], "verify_rows": tune["verify_rows"],
         "plain_roofline": plain_roof, "build_roofline": build_roofline, "graph_mode": graph,
     }
     if learned_on:
         lsearch = _mean(Lr["search"])
         lookups = sum(st["lookups"])
-        lroof = search_roofline(Lr["positions"], Lr["lines"], sum(Lr["kernel"]),
+        lroof = search_roofline(Lr["positions"], Lr["lines"], Lr["rows"], sum(Lr["kernel"]),
                                 Lr["launches"], copy_gbs, read_gbs, gather,
                                 kname[tune_l["find_chains"]], n, args.steps)
         lroof["traffic"] = committed_traffic(config, lroof["kernel"] + "_seeded")
@@ -549,7 +559,8 @@ def run_workload(pq, np, config, args, cpu_baseline):
             "train_loss": [float(f"{x:.3g}") for x in st["train_loss"]],
             "edges_identical_to_plain": bool(Lr["same"]),
             "extraction_shape": {k: tune_l[k] for k in ("group_lanes", "items")},
-            "find_kernel": tune_l["find_chains"], "roofline": lroof,
+            "find_kernel": tune_l["find_chains"], "verify_rows": tune_l["verify_rows"],
+            "roofline": lroof,
         }
         res["learned_vs_plain_search"] = round(avg_search / lsearch, 3)
         # build-once / search-once pipeline: the learned search pays for its table

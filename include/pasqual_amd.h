@@ -245,7 +245,8 @@ int pq_mlp_gradients(pq_ctx_t *ctx, uint32_t k, uint32_t hidden, const float *pa
  * wider tiers); the best shape depends on the read set.  pq_autotune runs the
  * search (mode as pq_find_overlaps) once per shape on the uploaded, indexed reads,
  * keeps the fastest for later searches of this context and leaves that shape's
- * (identical) result in place.  pq_set_tuning forces a shape. */
+ * (identical) result in place; then the search kernel and the verification
+ * threshold the same way.  pq_set_tuning forces a shape. */
 typedef struct {
     uint32_t shape;          /* chosen shape index */
     uint32_t group_lanes;    /* lanes per read of that shape (8, 16, 32) */
@@ -254,6 +255,8 @@ typedef struct {
     double extract_ms[4];    /* extraction time measured per shape */
     uint32_t find_chains;    /* candidate-search kernel chosen (1, 2 or 3, below) */
     double find_ms[3];       /* candidate-search time of kernels 1, 2 and 3 */
+    uint32_t verify_rows;    /* verification threshold chosen (0, 8 or 32, pq_set_verify) */
+    double verify_ms[3];     /* candidate search + verification time with 0, 8 and 32 */
 } pq_tune_t;
 int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out);
 int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape);
@@ -264,6 +267,14 @@ int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape);
  * (or after chains = 0) the kernel follows the size of both indexes'
  * occurrence lines: 2 below 1 GiB, 1 below 4 GiB, 3 from there on. */
 int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains);
+/* A chain of the candidate search whose interval holds at most `rows` suffix-
+ * array rows (<= 32) at a depth below the minimum overlap is finished on the
+ * text: each row's read and offset come from the suffix array, and how far the
+ * chain's next codes match left of it from the packed texts, 32 codes per
+ * compare; the rows' reaches give the depths the backward search would have
+ * gone on to and the reads it would have recorded there (same result).  32 by
+ * default; 0: every chain is searched backward to its end. */
+int pq_set_verify(pq_ctx_t *ctx, uint32_t rows);
 /* Initial capacity of the search's shared edge buffer: edges of read sides with
  * more than 16 irreducible edges (the first 16 of every side have slots of
  * their own).  0 (the default): max(2^20, reads / 4).  A search that needs more
@@ -284,6 +295,7 @@ typedef struct {
     double find_kernel_ms;   /* summed duration of the candidate-search kernel */
     uint64_t find_positions; /* rank positions evaluated by it (2 per step) */
     uint64_t find_lines;     /* occurrence lines it loaded (1 or 2 per step) */
+    uint64_t verify_rows;    /* suffix-array rows finished on the text (pq_set_verify) */
 } pq_timing_t;
 int pq_get_timing(pq_ctx_t *ctx, pq_timing_t *t);
 int pq_sync(pq_ctx_t *ctx);

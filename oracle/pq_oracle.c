@@ -1172,9 +1172,9 @@ static unsigned chain_code(const uint8_t *s, uint32_t L, int ch, uint32_t o)
 }
 
 int orc_chain_work(const uint8_t *text, const uint64_t *off, uint64_t n, uint32_t seed_depth,
-                   uint64_t out[2])
+                   uint32_t t, uint32_t vmax, uint64_t out[3])
 {
-    out[0] = out[1] = 0;
+    out[0] = out[1] = out[2] = 0;
     const uint64_t N = off[n];
     uint64_t C[6], sfx[4], pfx[4];
     orc_counts(text, off, n, C, sfx, pfx);
@@ -1229,6 +1229,10 @@ int orc_chain_work(const uint8_t *text, const uint64_t *off, uint64_t n, uint32_
                 }
             }
             while (o < L && size >= minsize) {
+                if (size <= vmax && o <= t) {
+                    out[2] += size;
+                    break;
+                }
                 out[0] += 2;
                 out[1] += (l >> 7) == ((l + size) >> 7) ? 1 : 2;
                 if (o + 1 >= L)

@@ -114,12 +114,15 @@ void orc_graph_free(orc_graph_t *g);
  * with the interval of its first code (or, when seed_depth is in 1..L-1, at
  * o = seed_depth with the interval of its seed_depth-code pattern) and steps
  * while o < L and the interval holds at least 2 rows (FB, BF) or 1 row (FF,
- * BB), the last step at o = L - 1.  A step evaluates 2 rank positions, l and
- * u + 1 of the interval [l, u], on 1 occurrence line of 128 rows when both
- * fall in the same one, else on 2.  out[0] = rank positions, out[1] = lines,
- * summed over all reads.  Returns 0, or -5 without memory. */
+ * BB), the last step at o = L - 1 -- except that at a depth o <= t where the
+ * interval holds at most vmax rows (vmax > 0) it stops stepping and its rows
+ * are verified on the text instead.  A step evaluates 2 rank positions, l and
+ * u + 1 of the interval [l, u], on 1 occurrence line of 128 rows when both fall
+ * in the same one, else on 2.  out[0] = rank positions, out[1] = lines, out[2] =
+ * rows handed to verification, summed over all reads.  Returns 0, or -5 without
+ * memory. */
 int orc_chain_work(const uint8_t *text, const uint64_t *off, uint64_t n, uint32_t seed_depth,
-                   uint64_t out[2]);
+                   uint32_t t, uint32_t vmax, uint64_t out[3]);
 
 /* Property check of a suffix array at any size: a permutation of 0..N-1 in
  * strictly increasing (suffix, position) order.  Returns -1 when it holds,

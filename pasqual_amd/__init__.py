@@ -40,7 +40,7 @@ PQ_SYMBOLS = [
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
     "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read", "pq_probe_gather", "pq_set_edge_capacity",
     "pq_get_slot_sizes", "pq_graph_export", "pq_host_alloc", "pq_host_free",
-    "pq_mlp_eval", "pq_dedup_reads_noalloc", "pq_dev_alloc", "pq_dev_free", "pq_dev_upload", "pq_dev_download",
+    "pq_mlp_eval", "pq_set_verify", "pq_dedup_reads_noalloc", "pq_dev_alloc", "pq_dev_free", "pq_dev_upload", "pq_dev_download",
 ]
 NOALLOC_SCRATCH_BYTES = 32 << 20   # PQ_NOALLOC_SCRATCH_BYTES
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
@@ -96,7 +96,8 @@ class LearnedStats(C.Structure):
 class Tune(C.Structure):
     _fields_ = [("shape", C.c_uint32), ("group_lanes", C.c_uint32), ("items", C.c_uint32),
                 ("nshapes", C.c_uint32), ("extract_ms", C.c_double * 4),
-                ("find_chains", C.c_uint32), ("find_ms", C.c_double * 3)]
+                ("find_chains", C.c_uint32), ("find_ms", C.c_double * 3),
+                ("verify_rows", C.c_uint32), ("verify_ms", C.c_double * 3)]
 
 
 class Timing(C.Structure):
@@ -104,7 +105,7 @@ class Timing(C.Structure):
                 ("find_ms", C.c_double), ("extract_ms", C.c_double), ("compact_ms", C.c_double),
                 ("learned_ms", C.c_double), ("find_launches", C.c_uint64),
                 ("find_kernel_ms", C.c_double), ("find_positions", C.c_uint64),
-                ("find_lines", C.c_uint64)]
+                ("find_lines", C.c_uint64), ("verify_rows", C.c_uint64)]
 
     def as_dict(self):
         return {"upload_ms": self.upload_ms, "sa_ms": list(self.sa_ms),
@@ -112,7 +113,7 @@ class Timing(C.Structure):
                 "extract_ms": self.extract_ms, "compact_ms": self.compact_ms,
                 "learned_ms": self.learned_ms, "find_launches": self.find_launches,
                 "find_kernel_ms": self.find_kernel_ms, "find_positions": self.find_positions,
-                "find_lines": self.find_lines}
+                "find_lines": self.find_lines, "verify_rows": self.verify_rows}
 
 
 _LIBS = {}
@@ -178,6 +179,7 @@ def lib(width="u32"):
         "pq_host_alloc": (I32, [P, U64, C.POINTER(P)]),
         "pq_host_free": (None, [P]),
         "pq_mlp_eval": (I32, [P, I32, P, U32, I32, P]),
+        "pq_set_verify": (I32, [P, U32]),
         "pq_dedup_reads_noalloc": (I32, [P, P, P, U64, I32, U64, I32, I32, P, P]),
         "pq_dev_alloc": (I32, [P, U64, C.POINTER(P)]),
         "pq_dev_free": (None, [P]),
@@ -461,13 +463,19 @@ class Context:
         self._check(self._L.pq_autotune(self._h, mode, C.byref(t)), "autotune")
         return {"shape": t.shape, "group_lanes": t.group_lanes, "items": t.items,
                 "extract_ms": list(t.extract_ms)[:t.nshapes], "find_chains": t.find_chains,
-                "find_ms": list(t.find_ms)}
+                "find_ms": list(t.find_ms), "verify_rows": t.verify_rows,
+                "verify_ms": list(t.verify_ms)}
 
     def set_tuning(self, shape):
         self._check(self._L.pq_set_tuning(self._h, shape), "set_tuning")
 
     def set_find_chains(self, chains):
         self._check(self._L.pq_set_find_chains(self._h, chains), "set_find_chains")
+
+    def set_verify(self, rows):
+        """Chains of at most `rows` rows (<= 32) at depths <= t finish on the text
+        (0: backward search to the end)."""
+        self._check(self._L.pq_set_verify(self._h, int(rows)), "set_verify")
 
     def set_edge_capacity(self, edges):
         self._check(self._L.pq_set_edge_capacity(self._h, int(edges)), "set_edge_capacity")

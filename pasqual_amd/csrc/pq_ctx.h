@@ -72,6 +72,8 @@ struct pq_ctx {
     uint32_t ext_shape = 1;        // first extraction tier: index into the tuned shapes
     uint32_t find_chains = 0;      // candidate-search kernel (1, 2, 3; 0: by index size)
     uint64_t edge_cap_set = 0;     // shared edge buffer capacity, 0 automatic
+    uint32_t verify_rows = 32;     // chains of at most this many rows at depth <= t are
+                                   // verified on the text (0: backward search to the end)
     uint64_t edge_cap_auto = 0;    // capacity an overflowing search asked for (these reads)
     uint64_t edge_need = 0;        // edges the last overflowing search counted
     uint64_t learned_gen = 0;      // bumped by every learned-index training
@@ -106,6 +108,7 @@ struct pq_ctx {
         v.text2 = dir[d].text2;
         v.occ = dir[d].occ;
         v.reads_index = dir[d].reads_index;
+        v.sa = (const R *) dir[d].sa;
         for (int c = 0; c < 5; c++)
             v.cf[c] = (R) dir[d].cf[c];
         return v;
@@ -131,7 +134,7 @@ enum PqSlot {
     SLOT_TOT, SLOT_OVL, SLOT_MISC, SLOT_SEEDS, SLOT_GSCRATCH, SLOT_STAGED,
     SLOT_OVB = 34, SLOT_OVF, SLOT_NUMB, SLOT_NUMF, SLOT_EDGE_OFF, SLOT_EDGES,
     SLOT_LEARNED = 40, SLOT_BWT8 = 47, SLOT_SLICE_HIST, SLOT_SLICE_TILES, SLOT_PROBE = 50,
-    SLOT_BIGRUNS, SLOT_NODES, SLOT_MISC2, SLOT_HUGE, SLOT_HUGE_LIST,
+    SLOT_BIGRUNS, SLOT_NODES, SLOT_MISC2, SLOT_HUGE, SLOT_HUGE_LIST, SLOT_VJOB_L, SLOT_VJOB,
 };
 
 // Grow-only slot buffer of at least `bytes` bytes into *p, or PQ_ERR_NOMEM.

@@ -40,6 +40,7 @@ template <typename R> struct DirViewT {
     const uint64_t *text2;
     const OccLine *occ;
     const uint32_t *reads_index;
+    const R *sa;
     R cf[5];                 // backward-step base: C[c] + fix[c]; cf[0] = 0
 };
 
@@ -110,11 +111,20 @@ __device__ __forceinline__ unsigned pq_code(const uint64_t *t2, uint64_t g)
     return (unsigned) (t2[g >> 5] >> (62 - 2 * (g & 31))) & 3u;
 }
 
-// Read containing text position g.
+// Read containing text position g.  Fixed lengths: g / L by a double-precision
+// reciprocal (g < 2^53), corrected to the exact quotient (a 64-bit division is a
+// long instruction sequence on the device).
 __device__ __forceinline__ uint64_t pq_read_of(const Geom &G, uint64_t g)
 {
-    if (G.L)
-        return g / G.L;
+    if (G.L) {
+        uint64_t q = (uint64_t) ((double) g * (1.0 / (double) G.L));
+        const uint64_t L = G.L;
+        if (q * L > g)
+            q--;
+        else if ((q + 1) * L <= g)
+            q++;
+        return q;
+    }
     uint64_t w = g >> 6;
     uint64_t below = G.ends[w] & ((1ull << (g & 63)) - 1ull);
     return G.end_rank[w] + (uint64_t) __popcll(below);
@@ -134,7 +144,7 @@ __device__ __forceinline__ uint32_t pq_read_len(const Geom &G, uint64_t r)
 __device__ __forceinline__ uint64_t pq_rem(const Geom &G, uint64_t g)
 {
     if (G.L)
-        return G.L - (uint32_t) (g % G.L);
+        return G.L - (uint32_t) (g - pq_read_of(G, g) * G.L);
     uint64_t r = pq_read_of(G, g);
     return G.off[r + 1] - g;
 }

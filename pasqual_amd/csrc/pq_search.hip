@@ -44,11 +44,29 @@ static const uint32_t EXT_SHAPES[3][2] = {{8, 16}, {16, 32}, {32, 64}};
 #define GLOBAL_SLOTS 256
 
 struct ChainBuf {
-    uint2 *lc;               // ('$' rank in the searched index, count)
+    uint2 *lc;               // ('$' rank in the searched index, count), or (read, 1 |
+                             // ENTRY_READ) for a candidate found by verification
     uint16_t *o;             // pattern length
     uint32_t *n;             // entries per (read, chain)
     uint32_t cap;            // entries per chain
+    uint64_t *vl;            // per chain: interval start handed to verification
+    uint32_t *vjob;          // per chain: rows | depth << 16, 0 = none
+    uint32_t vmax;           // largest interval handed to verification (0: none)
 };
+
+// An entry whose count carries ENTRY_READ names its one read directly.
+#define ENTRY_READ 0x80000000u
+
+__device__ __forceinline__ uint32_t entry_count(uint2 e)
+{
+    return e.y & ~ENTRY_READ;
+}
+
+// Read i of an entry of an index with reads_index ri.
+__device__ __forceinline__ uint64_t entry_read(uint2 e, uint32_t i, const uint32_t *ri)
+{
+    return (e.y & ENTRY_READ) ? (uint64_t) e.x : (uint64_t) ri[e.x + i];
+}
 
 // Orders the LDS accesses of a wave: what one lane wrote before it is visible
 // to every lane after it.
@@ -68,6 +86,20 @@ __device__ __forceinline__ unsigned chain_sym(const uint64_t *fwt, uint64_t s, u
     if (ch == 1 || ch == 3)
         c = 3u - c;
     return c + 1u;
+}
+
+// A chain whose interval holds at most cb.vmax rows at a depth where nothing has
+// been recorded yet (o <= t) is handed to k_verify, which finishes it on the
+// text; returns true when it was (the backward search of the chain stops).
+template <typename R>
+__device__ __forceinline__ bool hand_over(const ChainBuf &cb, uint64_t chain, uint32_t o,
+                                          uint32_t t, R l, uint32_t size)
+{
+    if (size > cb.vmax || o > t)
+        return false;
+    cb.vl[chain] = (uint64_t) l;
+    cb.vjob[chain] = size | (o << 16);
+    return true;
 }
 
 // One chain per lane, four adjacent lanes per read.
@@ -106,7 +138,10 @@ __global__ void __launch_bounds__(256) k_find1(Geom G, DirViewT<R> fw, DirViewT<
     }
     uint32_t nout = 0;
     uint64_t base = (rl * 4 + (uint64_t) ch) * out.cap;
+    out.vjob[rl * 4 + ch] = 0;
     while (o < L && size >= minsize) {
+        if (hand_over(out, rl * 4 + ch, o, t, l, size))
+            break;
         // the next pattern symbol is read with the interval's lines
         const bool more = o + 1 < L;
         const unsigned c = more ? chain_sym(fw.text2, s, L, ch, o + 1) : 1u;
@@ -231,7 +266,11 @@ __global__ void __launch_bounds__(256) k_find1q(Geom G, DirViewT<R> fw, DirViewT
     }
     uint32_t nout = 0;
     const uint64_t base = (rl * 4 + (uint64_t) ch) * out.cap;
+    if (valid)
+        out.vjob[rl * 4 + ch] = 0;
     bool live = valid && o < L && size >= minsize;
+    if (live && hand_over(out, rl * 4 + ch, o, t, l, size))
+        live = false;
     const unsigned qshift = threadIdx.x & 60u;
     while ((__ballot(live) >> qshift) & 0xFull) {
         const bool more = live && o + 1 < L;
@@ -274,7 +313,7 @@ __global__ void __launch_bounds__(256) k_find1q(Geom G, DirViewT<R> fw, DirViewT
                 l = (fwsearch ? pq_pick5(fw.cf, c) : pq_pick5(bw.cf, c)) + a.c;
                 size = b.c - a.c;
                 o++;
-                live = size >= minsize;
+                live = size >= minsize && !hand_over(out, rl * 4 + ch, o, t, l, size);
             } else {
                 live = false;
             }
@@ -344,7 +383,9 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
                 size[q] = (uint32_t) (pq_pick6(C.c, c + 1) - l[q]);
             }
             nout[q] = 0;
-            live[q] = o[q] < L && size[q] >= minsize[q];
+            out.vjob[rl * 4 + ch[q]] = 0;
+            live[q] = o[q] < L && size[q] >= minsize[q] &&
+                      !hand_over(out, rl * 4 + ch[q], o[q], t, l[q], size[q]);
         }
         while (live[0] || live[1]) {
             bool more[2];
@@ -391,7 +432,8 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
                 l[q] = pq_pick5(cf, c[q]) + a.c;
                 size[q] = b.c - a.c;
                 o[q]++;
-                live[q] = o[q] < L && size[q] >= minsize[q];
+                live[q] = o[q] < L && size[q] >= minsize[q] &&
+                          !hand_over(out, rl * 4 + ch[q], o[q], t, l[q], size[q]);
             }
         }
 #pragma unroll
@@ -408,6 +450,156 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
             atomicAdd(&stats[0], (unsigned long long) (2 * s2));
             atomicAdd(&stats[1], (unsigned long long) ln);
         }
+    }
+}
+
+// ---- verification ----------------------------------------------------------------------
+//
+// A chain handed over at depth o0 with rows [l, l + size) of its searched index
+// is finished on the text instead of by further backward steps.  Row j is the
+// suffix at position g of read m, p codes into m (in the searched index's
+// text).  Extending the chain by one code moves every row one position left;
+// the row survives while the codes left of g equal the chain's next codes, so
+// its reach is o0 + e with e the codes matched, at most min(p, L - 1 - o0) (it
+// cannot pass the start of m, nor depth L - 1 of the chain).  The interval at
+// depth o holds the rows that reach o, so the backward search would have gone
+// on while at least minsize rows reached the depth (2 for FB and BF, 1 for FF and
+// BB): up to o_last = the largest (FF, BB) or second largest (FB, BF) reach.  A
+// row that reaches the start of m (e = p) is a '$' row of the interval at depth
+// o0 + p: m is a candidate there when t <= o0 + p <= o_last -- exactly the
+// entries the backward search records, written here as one read each.
+//
+// The codes left of g, read leftwards in the searched index's text, are read
+// rightwards in the other direction's text (every read reversed in place): from
+// position start(m) + len(m) - p.  The chain's next codes are the read's own
+// codes: FB n[L-o0-1], n[L-o0-2], ... = the backward text of n from start(n) + o0,
+// BF n[o0], n[o0+1], ... = the forward text from start(n) + o0, FF and BB their
+// complements.  Both are compared 32 codes at a time.
+#define VERIFY_MAX 32
+#define VERIFY_GW 8                 // lanes per chain
+#define VERIFY_ROWS (VERIFY_MAX / VERIFY_GW)
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_verify(Geom G, DirViewT<R> fw, DirViewT<R> bw,
+                                               ChainBuf cb, uint64_t r0, uint64_t nr, uint32_t t,
+                                               unsigned long long *stats)
+{
+    const uint64_t chain = ((uint64_t) blockIdx.x * blockDim.x + threadIdx.x) / VERIFY_GW;
+    const uint32_t lane = threadIdx.x & (VERIFY_GW - 1);
+    const uint64_t rl = chain >> 2;
+    const int ch = (int) (chain & 3);
+    const uint32_t job = rl < nr ? cb.vjob[chain] : 0u;
+    uint32_t verified = 0;
+    if (job) {   // uniform over the chain's lanes
+        const uint32_t size = job & 0xFFFFu, o0 = job >> 16;
+        const uint64_t r = r0 + rl, sn = pq_read_start(G, r);
+        const uint32_t Ln = pq_read_len(G, r);
+        const bool fwsearch = ch == 0 || ch == 3;
+        const R *sa = fwsearch ? fw.sa : bw.sa;
+        const uint64_t *X = fwsearch ? bw.text2 : fw.text2;
+        const uint64_t *Pt = ch < 2 ? bw.text2 : fw.text2;
+        const uint64_t flip = (ch == 1 || ch == 3) ? ~0ull : 0ull;
+        const uint32_t jmax = Ln - 1 - o0, minsize = (ch == 0 || ch == 2) ? 2u : 1u;
+        const uint64_t l = cb.vl[chain];
+        // the chain's next codes, loaded once: nw words of 32 codes (up to 8 for
+        // reads of 256 codes)
+        const uint32_t nw = (jmax + 31) / 32;
+        uint64_t pw[8];
+        {
+            const uint64_t ps = sn + o0, w0 = ps >> 5;
+            const unsigned sh = (unsigned) (ps & 31) * 2;
+            uint64_t cur = Pt[w0];
+#pragma unroll
+            for (uint32_t w = 0; w < 8; w++) {
+                const uint64_t nxt = w < nw ? Pt[w0 + w + 1] : 0ull;
+                pw[w] = ((cur << sh) | ((nxt >> 1) >> (63 - sh))) ^ flip;
+                cur = nxt;
+            }
+        }
+        uint32_t oc[VERIFY_ROWS], mm[VERIFY_ROWS];
+        bool full[VERIFY_ROWS];
+        uint32_t t1 = 0, t2 = 0;   // the two largest reaches + 1 (0: none)
+#pragma unroll
+        for (int k = 0; k < VERIFY_ROWS; k++) {
+            const uint32_t j = lane + k * VERIFY_GW;
+            full[k] = false;
+            oc[k] = mm[k] = 0;
+            if (j >= size)
+                continue;
+            const uint64_t g = (uint64_t) sa[l + j];
+            const uint64_t m = pq_read_of(G, g);
+            const uint64_t sm = pq_read_start(G, m);
+            const uint32_t p = (uint32_t) (g - sm);
+            const uint32_t J = p < jmax ? p : jmax;
+            // codes left of the row, read rightwards in the other text: consecutive
+            // words funnel-shifted, one new word per 32 codes
+            const uint64_t xs = sm + pq_read_len(G, m) - p, x0 = xs >> 5;
+            const unsigned xsh = (unsigned) (xs & 31) * 2;
+            uint64_t cur = X[x0];
+            uint32_t e = J;
+#pragma unroll
+            for (uint32_t w = 0; w < 8; w++) {
+                if (32 * w >= J)
+                    break;
+                const uint64_t nxt = X[x0 + w + 1];
+                const uint64_t x = pw[w] ^ ((cur << xsh) | ((nxt >> 1) >> (63 - xsh)));
+                if (x) {
+                    const uint32_t f = 32 * w + (uint32_t) __clzll((long long) x) / 2;
+                    e = f < J ? f : J;
+                    break;
+                }
+                cur = nxt;
+            }
+            full[k] = p <= jmax && e == p;
+            oc[k] = o0 + p;
+            mm[k] = (uint32_t) m;
+            const uint32_t v = o0 + e + 1;
+            if (v > t1) {
+                t2 = t1;
+                t1 = v;
+            } else if (v > t2) {
+                t2 = v;
+            }
+        }
+        for (int d = 1; d < VERIFY_GW; d <<= 1) {
+            const uint32_t u1 = __shfl_xor(t1, d, VERIFY_GW), u2 = __shfl_xor(t2, d, VERIFY_GW);
+            const uint32_t lo1 = t1 < u1 ? t1 : u1, hi2 = t2 > u2 ? t2 : u2;
+            t1 = t1 > u1 ? t1 : u1;
+            t2 = lo1 > hi2 ? lo1 : hi2;
+        }
+        const uint32_t top = minsize == 1 ? t1 : t2;   // >= o0 + 1: the chain was live at o0
+        const uint32_t olast = top - 1 < Ln - 1 ? top - 1 : Ln - 1;
+        uint32_t mine = 0;
+#pragma unroll
+        for (int k = 0; k < VERIFY_ROWS; k++)
+            mine += full[k] && oc[k] >= t && oc[k] <= olast ? 1u : 0u;
+        uint32_t before = mine;
+        for (int d = 1; d < VERIFY_GW; d <<= 1) {
+            const uint32_t x = __shfl_up(before, d, VERIFY_GW);
+            if (lane >= (uint32_t) d)
+                before += x;
+        }
+        const uint32_t total = __shfl(before, VERIFY_GW - 1, VERIFY_GW);
+        before -= mine;
+        const uint64_t base = chain * cb.cap;
+#pragma unroll
+        for (int k = 0; k < VERIFY_ROWS; k++) {
+            if (full[k] && oc[k] >= t && oc[k] <= olast) {
+                cb.lc[base + before] = make_uint2(mm[k], 1u | ENTRY_READ);
+                cb.o[base + before] = (uint16_t) oc[k];
+                before++;
+            }
+        }
+        if (lane == 0)
+            cb.n[chain] = total;
+        verified = lane == 0 ? size : 0u;
+    }
+    if (stats) {
+        uint64_t v = verified;
+        for (int d = 32; d > 0; d >>= 1)
+            v += __shfl_xor(v, d, 64);
+        if ((threadIdx.x & 63) == 0 && v)
+            atomicAdd(&stats[2], (unsigned long long) v);
     }
 }
 
@@ -743,7 +935,7 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
         for (uint32_t e = lane; e < cnt4[chA] + cnt4[chB]; e += GW) {
             uint64_t at = (e < cnt4[chA]) ? (rl * 4 + chA) * cb.cap + e
                                           : (rl * 4 + chB) * cb.cap + (e - cnt4[chA]);
-            k += cb.lc[at].y;
+            k += entry_count(cb.lc[at]);
         }
         for (int d = GW / 2; d > 0; d >>= 1)
             k += __shfl_xor(k, d, GW);
@@ -762,7 +954,7 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
             if (e < ne) {
                 uint64_t at = (e < nA) ? (rl * 4 + chA) * cb.cap + e
                                        : (rl * 4 + chB) * cb.cap + (e - nA);
-                v = cb.lc[at].y;
+                v = entry_count(cb.lc[at]);
             }
             uint32_t inc = v;
             for (int d = 1; d < GW; d <<= 1) {
@@ -811,7 +1003,7 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
                 // FB, BB searched the forward index; FF, BF the backward index
                 const int ch = firsts[h] ? chA : chB;
                 const uint32_t *ri = (ch == 0 || ch == 3) ? fw.reads_index : bw.reads_index;
-                mm[h] = on[h] ? (uint64_t) ri[lcs[h].x + (qq[h] - pre[ee[h]])] : 0ull;
+                mm[h] = on[h] ? entry_read(lcs[h], qq[h] - pre[ee[h]], ri) : 0ull;
             }
 #pragma unroll
             for (int h = 0; h < 2; h++) {
@@ -916,7 +1108,7 @@ template <int GW> __device__ uint64_t side_items(const ChainBuf &cb, uint64_t rl
     uint64_t k = 0;
     for (uint32_t e = lane; e < nA + nB; e += GW) {
         const uint64_t at = e < nA ? (rl * 4 + chA) * cb.cap + e : (rl * 4 + chB) * cb.cap + (e - nA);
-        k += cb.lc[at].y;
+        k += entry_count(cb.lc[at]);
     }
     for (int d = GW / 2; d > 0; d >>= 1)
         k += __shfl_xor(k, d, GW);
@@ -973,7 +1165,7 @@ __global__ void __launch_bounds__(64) k_huge_prefix(ChainBuf cb, uint64_t rl, in
         uint32_t v = 0;
         if (e < ne) {
             const uint64_t at = e < nA ? (rl * 4 + chA) * cb.cap + e : (rl * 4 + chB) * cb.cap + (e - nA);
-            v = cb.lc[at].y;
+            v = entry_count(cb.lc[at]);
         }
         uint32_t inc = v;
         for (int d = 1; d < 64; d <<= 1) {
@@ -1015,7 +1207,7 @@ __global__ void k_huge_expand(Geom G, DirViewT<R> fw, DirViewT<R> bw, ChainBuf c
         const uint64_t at = first ? (rl * 4 + chA) * cb.cap + lo : (rl * 4 + chB) * cb.cap + (lo - nA);
         const uint2 lc = cb.lc[at];
         const uint32_t *ri = (ch == 0 || ch == 3) ? fw.reads_index : bw.reads_index;
-        const uint64_t m = ri[lc.x + ((uint32_t) q - pre[lo])];
+        const uint64_t m = entry_read(lc, (uint32_t) q - pre[lo], ri);
         item_fill(G, fw, bw, ch, first ? 0u : 1u, (uint32_t) cb.o[at], m, kw,
                   keys + (uint64_t) q * kw, meta + q);
     }
@@ -1220,6 +1412,7 @@ static int run_search_graph(pq_ctx_t *ctx, int mode, uint64_t n, uint64_t batch,
     PQ_CHECK(hipStreamSynchronize(ctx->stream));
     ctx->timing.find_positions = (uint64_t) hm[8] | ((uint64_t) hm[9] << 32);
     ctx->timing.find_lines = (uint64_t) hm[10] | ((uint64_t) hm[11] << 32);
+    ctx->timing.verify_rows = (uint64_t) hm[12] | ((uint64_t) hm[13] << 32);
     if (hm[3] & 2u)
         return PQ_RETRY_EAGER;
     if (hm[3] & 1u) {
@@ -1240,7 +1433,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     ctx->timing.learned_ms = 0;
     ctx->timing.find_kernel_ms = 0;
     ctx->timing.find_launches = 0;
-    ctx->timing.find_positions = ctx->timing.find_lines = 0;
+    ctx->timing.find_positions = ctx->timing.find_lines = ctx->timing.verify_rows = 0;
     ctx->n_edges = 0;
     int rc;
     if ((rc = pq_slot(ctx, SLOT_OVB, (n + 1) * 4, &ctx->d_ovB)) ||
@@ -1265,7 +1458,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     // batch so the candidate buffers take at most a third of the free device
     // memory (at least 8 GiB): fewer, larger batches keep the GPU full to the
     // end of each launch (chains of uneven reads finish at very different times)
-    uint64_t per_read = 4ull * cb.cap * (sizeof(uint2) + sizeof(uint16_t)) + 16;
+    uint64_t per_read = 4ull * cb.cap * (sizeof(uint2) + sizeof(uint16_t)) + 16 + 4 * 12;
     uint64_t budget = 8ull << 30;
     {
         size_t free_b = 0, total_b = 0;
@@ -1298,7 +1491,12 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     cb.lc = nullptr;
     cb.o = nullptr;
     cb.n = nullptr;
-    if ((rc = pq_slot(ctx, SLOT_CAND_LC, batch * 4 * cb.cap * sizeof(uint2), &cb.lc)) ||
+    // verification takes chains of up to vmax rows; their candidates (one entry per
+    // read, at most one per row) must fit the entries of a chain
+    cb.vmax = std::min<uint32_t>(std::min<uint32_t>(ctx->verify_rows, VERIFY_MAX), cb.cap);
+    if ((rc = pq_slot(ctx, SLOT_VJOB_L, batch * 4 * 8, &cb.vl)) ||
+        (rc = pq_slot(ctx, SLOT_VJOB, batch * 4 * 4, &cb.vjob)) ||
+        (rc = pq_slot(ctx, SLOT_CAND_LC, batch * 4 * cb.cap * sizeof(uint2), &cb.lc)) ||
         (rc = pq_slot(ctx, SLOT_CAND_O, batch * 4 * cb.cap * sizeof(uint16_t), &cb.o)) ||
         (rc = pq_slot(ctx, SLOT_CAND_N, batch * 4 * sizeof(uint32_t), &cb.n)) ||
         (rc = pq_slot(ctx, SLOT_RECB, (n + 1) * 8, &rec)) ||
@@ -1384,6 +1582,9 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         else
             hipLaunchKernelGGL(k_find2<R>, dim3(pq_blocks(nr * 2, 256)), dim3(256), 0,
                                ctx->stream, G, fw, bw, r0, nr, t, Ct, seeds, kseed, cb, fstats);
+        if (cb.vmax)
+            hipLaunchKernelGGL(k_verify<R>, dim3(pq_blocks(nr * 4 * VERIFY_GW, 256)), dim3(256), 0,
+                               ctx->stream, G, fw, bw, cb, r0, nr, t, fstats);
         if (timed)
             (void) hipEventRecord(e[2], ctx->stream);
         const dim3 g1(pq_blocks(nr, groups1)), b1(64 * EXT_WAVES);
@@ -1453,7 +1654,9 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
                                         (uint64_t) sizeof(R), (uint64_t) chains,
                                         (uint64_t) (uintptr_t) G.off, (uint64_t) (uintptr_t) G.ends,
                                         (uint64_t) (uintptr_t) G.end_rank, (uint64_t) G.L,
-                                        (uint64_t) G.Lmax, (uint64_t) G.N, (uint64_t) cb.cap});
+                                        (uint64_t) G.Lmax, (uint64_t) G.N, (uint64_t) cb.cap,
+                                        (uint64_t) (uintptr_t) cb.vl, (uint64_t) (uintptr_t) cb.vjob,
+                                        (uint64_t) cb.vmax});
         cleanup();
         return rc;
     }
@@ -1506,6 +1709,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
             ctx->timing.compact_ms = ev_ms(ce[0], ce[1]);
             ctx->timing.find_positions = (uint64_t) hm[8] | ((uint64_t) hm[9] << 32);
             ctx->timing.find_lines = (uint64_t) hm[10] | ((uint64_t) hm[11] << 32);
+            ctx->timing.verify_rows = (uint64_t) hm[12] | ((uint64_t) hm[13] << 32);
             if (hm[3] & 2u) {
                 rc = ctx->fail(PQ_ERR_STATE, "oversized read side left unprocessed");
             } else if (hm[3] & 1u) {
@@ -1634,26 +1838,42 @@ extern "C" int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out)
         }
     }
     ctx->ext_shape = pick;
-    // the candidate search at one chain per lane (more waves in flight, each with
-    // one line load outstanding): better where lengths vary or the index is
-    // HBM-resident
-    ctx->find_chains = 1;
-    int rc = pq_find_overlaps(ctx, mode);
-    if (rc)
-        return rc;
-    t.find_ms[0] = ctx->timing.find_kernel_ms;
-    // one chain per lane, the lines of a read's four lanes loaded together (wins
-    // where the lines span more than the translation caches cover)
-    ctx->find_chains = 3;
-    if ((rc = pq_find_overlaps(ctx, mode)))
-        return rc;
-    t.find_ms[2] = ctx->timing.find_kernel_ms;
-    uint32_t fk = 0;
-    for (uint32_t k = 1; k < 3; k++)
-        if (t.find_ms[k] < t.find_ms[fk])
-            fk = k;
-    ctx->find_chains = fk + 1;
-    t.find_chains = ctx->find_chains;
+    // the candidate search kernel (one chain per lane: more waves in flight, each
+    // with one line load outstanding, better where lengths vary or the index is
+    // HBM-resident; two per lane; one per lane with the lines of a read's four
+    // lanes loaded together, better where the lines span more than the
+    // translation caches cover) together with the verification threshold (0: no
+    // verification, 8 or 32 rows; verification reads the suffix array, whose rows
+    // are scattered over tens of GiB at the largest sizes): every pair timed, the
+    // fastest kept (search kernel + verification time)
+    static const uint32_t VROWS[3] = {0, 8, 32};
+    double best_find = 1e300;
+    uint32_t fk = 1, vk = 2;
+    for (uint32_t k = 0; k < 3; k++)
+        t.find_ms[k] = 1e300;
+    for (uint32_t v = 0; v < 3; v++)
+        t.verify_ms[v] = 1e300;
+    for (uint32_t k = 1; k <= 3; k++) {
+        for (uint32_t v = 0; v < 3; v++) {
+            ctx->find_chains = k;
+            ctx->verify_rows = VROWS[v];
+            int rc = pq_find_overlaps(ctx, mode);
+            if (rc)
+                return rc;
+            const double ms = ctx->timing.find_ms;
+            t.find_ms[k - 1] = std::min(t.find_ms[k - 1], ms);
+            t.verify_ms[v] = std::min(t.verify_ms[v], ms);
+            if (ms < best_find) {
+                best_find = ms;
+                fk = k;
+                vk = v;
+            }
+        }
+    }
+    ctx->find_chains = fk;
+    ctx->verify_rows = VROWS[vk];
+    t.find_chains = fk;
+    t.verify_rows = VROWS[vk];
     t.shape = pick;
     t.group_lanes = EXT_SHAPES[pick][0];
     t.items = EXT_SHAPES[pick][1];
@@ -1681,6 +1901,16 @@ extern "C" int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains)
     if (chains > 3)
         return ctx->fail(PQ_ERR_ARG, "search kernel must be 0 (by index size), 1, 2 or 3");
     ctx->find_chains = chains;
+    return PQ_OK;
+}
+
+extern "C" int pq_set_verify(pq_ctx_t *ctx, uint32_t rows)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (rows > VERIFY_MAX)
+        return ctx->fail(PQ_ERR_ARG, "verification takes at most 32 rows");
+    ctx->verify_rows = rows;
     return PQ_OK;
 }
 

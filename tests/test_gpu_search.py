@@ -263,26 +263,55 @@ def test_every_extraction_shape_gives_the_same_graph(ctx):
 
 @pytest.mark.parametrize("lmin,lmax,t,seed", [(36, 250, 31, 41), (100, 100, 63, 42)])
 def test_search_work_counts_match_oracle(ctx, lmin, lmax, t, seed):
-    # the rank positions and occurrence lines the search kernels report (the
-    # roofline's algorithmic bytes) equal the oracle's count of the same chains,
-    # for each search kernel, from depth 1 and from the learned seeds' depth
+    # the rank positions, occurrence lines and verified rows the search kernels
+    # report (the roofline's algorithmic bytes) equal the oracle's count of the
+    # same chains, for each search kernel, from depth 1 and from the learned
+    # seeds' depth, with verification of narrow intervals and without
     rng = np.random.default_rng(seed)
     codes, off, _ = _random_reads(rng, 6000, 2000, lmin, lmax, err=0.003)
     ctx.upload(codes, off, t)
     ctx.build_index()
-    want = orc.chain_work(codes, off)
-    for kernel in (1, 2, 3):
-        ctx.set_find_chains(kernel)
-        ctx.find_overlaps()
-        tm = ctx.timing()
-        assert (tm["find_positions"], tm["find_lines"]) == want, f"kernel {kernel}"
     ctx.learned_train()
     D = ctx.learned_stats()["depth"]
-    want_l = orc.chain_work(codes, off, D)
-    assert want_l[0] < want[0]
-    for kernel in (1, 2, 3):
-        ctx.set_find_chains(kernel)
-        ctx.find_overlaps(pq.SEARCH_LEARNED)
-        tm = ctx.timing()
-        assert (tm["find_positions"], tm["find_lines"]) == want_l, f"seeded, kernel {kernel}"
+    for vmax in (0, 32, 5):
+        ctx.set_verify(vmax)
+        v = min(vmax, int(np.diff(off.astype(np.int64)).max()) - t)
+        want = orc.chain_work(codes, off, 0, t, v)
+        want_l = orc.chain_work(codes, off, D, t, v)
+        assert want_l[0] + want_l[2] < want[0] + want[2]
+        for kernel in (1, 2, 3):
+            ctx.set_find_chains(kernel)
+            for mode, w in ((pq.SEARCH_PLAIN, want), (pq.SEARCH_LEARNED, want_l)):
+                ctx.find_overlaps(mode)
+                tm = ctx.timing()
+                assert (tm["find_positions"], tm["find_lines"], tm["verify_rows"]) == w, \
+                    (vmax, kernel, mode)
     ctx.set_find_chains(0)
+    ctx.set_verify(32)
+
+
+@pytest.mark.parametrize("vmax", [0, 1, 7, 32])
+def test_verification_gives_the_search_graph(ctx, vmax):
+    # chains finished on the text give the backward search's candidates exactly:
+    # uneven lengths, repeats with errors, identical reads, short t
+    rng = np.random.default_rng(50 + vmax)
+    cases = [_random_reads(rng, 3000, 2500, 36, 250, err=0.004)[:2] + (31,),
+             _random_reads(rng, 800, 3000, 40, 90, err=0.01)[:2] + (21,),
+             _random_reads(rng, 20000, 4000, 60, 60, err=0.003)[:2] + (17,),
+             _identical_copies(40)]
+    ctx.set_verify(vmax)
+    try:
+        for codes, off, t in cases:
+            want = orc.search_explicit(codes, off, t)
+            ctx.upload(codes, off, t)
+            ctx.build_index()
+            for kernel in (1, 2, 3):
+                ctx.set_find_chains(kernel)
+                ctx.find_overlaps()
+                _same_graph(ctx.graph(), want, f"vmax {vmax} kernel {kernel}")
+            ctx.learned_train(samples=1 << 14, batch=1024, epochs=1)
+            ctx.find_overlaps(pq.SEARCH_LEARNED)
+            _same_graph(ctx.graph(), want, f"vmax {vmax} learned")
+    finally:
+        ctx.set_find_chains(0)
+        ctx.set_verify(32)

@@ -285,7 +285,7 @@ def test_slot_sizes_match_reference(name, width):
         assert mx == want[f"max_size_{tag}"] and sum(size) == n * L
 
 
-def _chain_work_bruteforce(codes, off, seed_depth=0):
+def _chain_work_bruteforce(codes, off, seed_depth=0, t=0, vmax=0):
     # the chains' work from explicit suffix strings: the interval of a pattern is
     # [number of suffixes sorting before it, + number having it as a prefix) in the
     # order end of read first, codes 1<2<3<4 (the suffix-array order)
@@ -305,7 +305,7 @@ def _chain_work_bruteforce(codes, off, seed_depth=0):
         k = sum(1 for x in sufs[d] if x[:len(pat)] == pat)
         return lo, k
 
-    pos = lines = 0
+    pos = lines = rows = 0
     for r in range(n):
         s = [int(x) for x in codes[int(off[r]):int(off[r + 1])]]
         L = len(s)
@@ -322,23 +322,32 @@ def _chain_work_bruteforce(codes, off, seed_depth=0):
                 lo, k = interval(d, pat)
                 if k < minsize:
                     break
+                if k <= vmax and o <= t:
+                    rows += k
+                    break
                 pos += 2
                 lines += 1 if lo >> 7 == (lo + k) >> 7 else 2
                 o += 1
-    return pos, lines
+    return pos, lines, rows
 
 
-@pytest.mark.parametrize("seed,varlen,depth", [(0, False, 0), (1, True, 0), (2, False, 5),
-                                               (3, True, 7), (4, False, 0)])
-def test_chain_work_matches_bruteforce(seed, varlen, depth):
-    # the oracle's count of the device chains' rank positions and occurrence
-    # lines (FM walk with per-block ranks) equals the count from explicit sorted
-    # suffixes; the repeat-rich genome makes intervals span 128-row lines
+@pytest.mark.parametrize("seed,varlen,depth,t,vmax", [(0, False, 0, 0, 0), (1, True, 0, 0, 0),
+                                                      (2, False, 5, 0, 0), (3, True, 7, 0, 0),
+                                                      (4, False, 0, 0, 0), (5, False, 0, 6, 4),
+                                                      (6, True, 0, 9, 8), (7, True, 5, 8, 32)])
+def test_chain_work_matches_bruteforce(seed, varlen, depth, t, vmax):
+    # the oracle's count of the device chains' rank positions, occurrence lines
+    # and verified rows (FM walk with per-block ranks) equals the count from
+    # explicit sorted suffixes; the repeat-rich genome makes intervals span
+    # 128-row lines
     rng = np.random.default_rng(100 + seed)
     codes, off, _ = _random_reads(rng, 90, 60, 10, 30 if varlen else 10, err=0.02)
-    want = _chain_work_bruteforce(codes, off, depth)
-    assert orc.chain_work(codes, off, depth) == want
-    assert want[1] > want[0] // 2          # some steps span two lines
+    want = _chain_work_bruteforce(codes, off, depth, t, vmax)
+    assert orc.chain_work(codes, off, depth, t, vmax) == want
+    if vmax:
+        assert want[2] > 0
+    else:
+        assert want[1] > want[0] // 2      # some steps span two lines
 
 
 def test_chain_work_plant_is_caught():
@@ -349,3 +358,6 @@ def test_chain_work_plant_is_caught():
     c2 = codes.copy()
     c2[11] = 5 - c2[11]
     assert orc.chain_work(c2, off) != a
+    # verification takes over from the backward steps
+    v = orc.chain_work(codes, off, 0, 11, 8)
+    assert v[2] > 0 and v[0] < a[0]
