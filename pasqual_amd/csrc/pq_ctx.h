@@ -175,8 +175,15 @@ template <typename R> struct SeedT {
     uint32_t size;
 };
 #define SEED_NONE 0xFFFFFFFFu
-// d_seeds: SeedT<uint32_t> or SeedT<uint64_t> per (read, chain) as ctx->wide says
-int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed);
+// size SEED_DONE: the seed kernel finished the chain by verification (its
+// records are written; the search kernels leave it alone)
+#define SEED_DONE 0xFFFFFFFEu
+struct ChainBuf;
+// d_seeds: SeedT<uint32_t> or SeedT<uint64_t> per (read, chain) as ctx->wide says;
+// chains whose interval holds at most cb.vmax rows are finished into cb, their
+// rows counted in stats[2]
+int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed,
+                     const ChainBuf &cb, unsigned long long *stats);
 void pq_release_build_scratch(pq_ctx_t *ctx);
 
 static inline unsigned pq_blocks(uint64_t work, unsigned per)

@@ -32,6 +32,7 @@
 // system solved on the host), then every parameter refined by minibatch Adam
 // with the forward/backward kernels (block-ordered deterministic reductions).
 #include "pq_ctx.h"
+#include "pq_chain.h"
 
 #include <algorithm>
 #include <chrono>
@@ -635,7 +636,8 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
                                                       uint64_t nr, uint32_t D, MlpDims d,
                                                       uint32_t leaves, uint32_t S,
                                                       ModelViewT<R> fw, ModelViewT<R> bw,
-                                                      SeedT<R> *seeds, QStats *qs)
+                                                      SeedT<R> *seeds, QStats *qs, ChainBuf cb,
+                                                      uint32_t t, unsigned long long *stats)
 {
     const int ch = (int) (threadIdx.x >> 6);
     const uint64_t rl = (uint64_t) blockIdx.x * 64 + (threadIdx.x & 63);
@@ -643,7 +645,7 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
     const int sdir = fwsearch ? 0 : 1;
     uint32_t probes[2] = {0, 0}, looks[2] = {0, 0};
     uint64_t win[2] = {0, 0};
-    uint32_t absent = 0;
+    uint32_t absent = 0, verified = 0;
     // both directions' inference parameters in LDS, read as wave-wide broadcasts
     __shared__ float wsh[2][MLP_P_MAX];
     for (uint32_t i = threadIdx.x; i < MLP_P_MAX; i += blockDim.x) {
@@ -687,11 +689,31 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
             const uint64_t hi = row_lower_bound(G, Sv, hlo > lo ? hlo : lo,
                                                 jh < ns ? jh * st : Sv.N, above, D, probes[sdir]);
             sd.l = (R) lo;
-            // (an interval of 2^32 - 1 rows or more takes the plain start instead)
-            sd.size = hi - lo < SEED_NONE ? (uint32_t) (hi - lo) : SEED_NONE;
+            // (an interval of 2^32 - 2 rows or more takes the plain start instead)
+            sd.size = hi - lo < SEED_DONE ? (uint32_t) (hi - lo) : SEED_NONE;
             absent = hi == lo;   // no suffix starts with the pattern: the chain is empty
+            // a narrow interval is finished right here on the text (its rows were
+            // just probed): depth D <= t, nothing recorded yet
+            const uint32_t minsize = (ch == 0 || ch == 2) ? 2u : 1u;
+            if (sd.size != SEED_NONE && sd.size >= minsize && sd.size <= cb.vmax && D <= t) {
+                const uint64_t chain = rl * 4 + ch, r = r0 + rl;
+                const uint32_t L = pq_read_len(G, r);
+                cb.n[chain] = verify_chain<R>(
+                    G, Sv.sa, fwsearch ? bw.t2 : fw.t2, ch < 2 ? bw.t2 : fw.t2,
+                    (ch == 1 || ch == 3) ? ~0ull : 0ull, pq_read_start(G, r), L, D, lo, sd.size,
+                    minsize, t, cb.lc + chain * cb.cap, cb.o + chain * cb.cap);
+                verified = sd.size;
+                sd.size = SEED_DONE;
+            }
         }
         seeds[rl * 4 + ch] = sd;
+    }
+    if (stats) {
+        uint64_t vr = verified;
+        for (int o = 32; o > 0; o >>= 1)
+            vr += __shfl_xor(vr, o, 64);
+        if ((threadIdx.x & 63) == 0 && vr)
+            atomicAdd(&stats[2], (unsigned long long) vr);
     }
     // block reduction of the statistics, one set of atomics per block
     __shared__ unsigned long long red[4][9];
@@ -1120,16 +1142,17 @@ extern "C" int pq_learned_stats(pq_ctx_t *ctx, pq_learned_stats_t *st)
 // ordered, no synchronisation).  Statistics accumulate over one search.
 template <typename R>
 static void launch_seeds(pq_ctx_t *ctx, const Learned *LM, uint64_t r0, uint64_t nr,
-                         SeedT<R> *d_seeds)
+                         SeedT<R> *d_seeds, const ChainBuf &cb, unsigned long long *stats)
 {
     ModelViewT<R> v[2] = {model_view<R>(ctx, *LM, 0), model_view<R>(ctx, *LM, 1)};
     MlpDims dims = {LM->k, LM->h};
     hipLaunchKernelGGL(k_learned_seed<R>, dim3(pq_blocks(nr, 64)), dim3(256), 0, ctx->stream,
                        ctx->geom(), ctx->dir[0].text2, r0, nr, LM->depth, dims, LM->leaves,
-                       LM->segs, v[0], v[1], d_seeds, LM->qstats);
+                       LM->segs, v[0], v[1], d_seeds, LM->qstats, cb, ctx->t, stats);
 }
 
-int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed)
+int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed,
+                     const ChainBuf &cb, unsigned long long *stats)
 {
     Learned *LM = ctx->learned;
     if (!LM)
@@ -1139,9 +1162,9 @@ int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uin
                            (unsigned long long *) LM->qstats,
                            (uint32_t) (sizeof(QStats) / sizeof(unsigned long long)));
     if (ctx->wide)
-        launch_seeds<uint64_t>(ctx, LM, r0, nr, (SeedT<uint64_t> *) d_seeds);
+        launch_seeds<uint64_t>(ctx, LM, r0, nr, (SeedT<uint64_t> *) d_seeds, cb, stats);
     else
-        launch_seeds<uint32_t>(ctx, LM, r0, nr, (SeedT<uint32_t> *) d_seeds);
+        launch_seeds<uint32_t>(ctx, LM, r0, nr, (SeedT<uint32_t> *) d_seeds, cb, stats);
     PQ_CHECK(hipGetLastError());
     *kseed = LM->depth;
     return PQ_OK;

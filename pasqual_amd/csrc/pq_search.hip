@@ -28,6 +28,7 @@
 // strings).  A group of lanes per read sorts the items with their keys in LDS
 // and reads these rules off the sorted order (extract_sorted).
 #include "pq_ctx.h"
+#include "pq_chain.h"
 
 #include <chrono>
 #include <cstring>
@@ -42,31 +43,6 @@ static const uint32_t EXT_SHAPES[3][2] = {{8, 16}, {16, 32}, {32, 64}};
 #define KMAX_LDS2 512
 #define KMAX_GLOBAL 65536
 #define GLOBAL_SLOTS 256
-
-struct ChainBuf {
-    uint2 *lc;               // ('$' rank in the searched index, count), or (read, 1 |
-                             // ENTRY_READ) for a candidate found by verification
-    uint16_t *o;             // pattern length
-    uint32_t *n;             // entries per (read, chain)
-    uint32_t cap;            // entries per chain
-    uint64_t *vl;            // per chain: interval start handed to verification
-    uint32_t *vjob;          // per chain: rows | depth << 16, 0 = none
-    uint32_t vmax;           // largest interval handed to verification (0: none)
-};
-
-// An entry whose count carries ENTRY_READ names its one read directly.
-#define ENTRY_READ 0x80000000u
-
-__device__ __forceinline__ uint32_t entry_count(uint2 e)
-{
-    return e.y & ~ENTRY_READ;
-}
-
-// Read i of an entry of an index with reads_index ri.
-__device__ __forceinline__ uint64_t entry_read(uint2 e, uint32_t i, const uint32_t *ri)
-{
-    return (e.y & ENTRY_READ) ? (uint64_t) e.x : (uint64_t) ri[e.x + i];
-}
 
 // Orders the LDS accesses of a wave: what one lane wrote before it is visible
 // to every lane after it.
@@ -125,10 +101,12 @@ __global__ void __launch_bounds__(256) k_find1(Geom G, DirViewT<R> fw, DirViewT<
     SeedT<R> sd = {0, SEED_NONE};
     if (seeds && kseed > 0 && kseed < L)
         sd = seeds[rl * 4 + ch];
+    // a chain the seed kernel finished keeps its records
+    const bool done = sd.size == SEED_DONE;
     if (sd.size != SEED_NONE) {
         o = kseed;
         l = sd.l;
-        size = sd.size;
+        size = done ? 0u : sd.size;
     } else {
         // C[0..5] = 0, 0 (A), C[2], C[3], C[4], C[5] (index.c:367-372)
         unsigned c = chain_sym(fw.text2, s, L, ch, 1);
@@ -169,7 +147,8 @@ __global__ void __launch_bounds__(256) k_find1(Geom G, DirViewT<R> fw, DirViewT<
         size = b.c - a.c;
         o++;
     }
-    out.n[rl * 4 + ch] = nout;
+    if (!done)
+        out.n[rl * 4 + ch] = nout;
     }
     if (stats) {
         uint64_t s2 = steps, ln = lines;
@@ -255,10 +234,11 @@ __global__ void __launch_bounds__(256) k_find1q(Geom G, DirViewT<R> fw, DirViewT
     SeedT<R> sd = {0, SEED_NONE};
     if (valid && seeds && kseed > 0 && kseed < L)
         sd = seeds[rl * 4 + ch];
+    const bool done = sd.size == SEED_DONE;   // finished by the seed kernel
     if (sd.size != SEED_NONE) {
         o = kseed;
         l = sd.l;
-        size = sd.size;
+        size = done ? 0u : sd.size;
     } else if (valid) {
         unsigned c = chain_sym(fw.text2, s, L, ch, 1);
         l = pq_pick6(C.c, c);
@@ -319,7 +299,7 @@ __global__ void __launch_bounds__(256) k_find1q(Geom G, DirViewT<R> fw, DirViewT
             }
         }
     }
-    if (valid)
+    if (valid && !done)
         out.n[rl * 4 + ch] = nout;
     if (stats) {
         uint64_t s2 = steps, ln = lines;
@@ -365,16 +345,17 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
         const uint32_t minsize[2] = {bwi ? 1u : 2u, bwi ? 2u : 1u};
         uint32_t o[2], size[2], nout[2];
         R l[2];
-        bool live[2];
+        bool live[2], done[2];
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             SeedT<R> sd = {0, SEED_NONE};
             if (seeds && kseed > 0 && kseed < L)
                 sd = seeds[rl * 4 + ch[q]];
+            done[q] = sd.size == SEED_DONE;   // finished by the seed kernel
             if (sd.size != SEED_NONE) {
                 o[q] = kseed;
                 l[q] = sd.l;
-                size[q] = sd.size;
+                size[q] = done[q] ? 0u : sd.size;
             } else {
                 // C[0..5] = 0, 0 (A), C[2], C[3], C[4], C[5] (index.c:367-372)
                 unsigned c = chain_sym(fw.text2, s, L, ch[q], 1);
@@ -438,7 +419,8 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
         }
 #pragma unroll
         for (int q = 0; q < 2; q++)
-            out.n[rl * 4 + ch[q]] = nout[q];
+            if (!done[q])
+                out.n[rl * 4 + ch[q]] = nout[q];
     }
     if (stats) {
         uint64_t s2 = steps, ln = lines;
@@ -454,145 +436,29 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
 }
 
 // ---- verification ----------------------------------------------------------------------
-//
-// A chain handed over at depth o0 with rows [l, l + size) of its searched index
-// is finished on the text instead of by further backward steps.  Row j is the
-// suffix at position g of read m, p codes into m (in the searched index's
-// text).  Extending the chain by one code moves every row one position left;
-// the row survives while the codes left of g equal the chain's next codes, so
-// its reach is o0 + e with e the codes matched, at most min(p, L - 1 - o0) (it
-// cannot pass the start of m, nor depth L - 1 of the chain).  The interval at
-// depth o holds the rows that reach o, so the backward search would have gone
-// on while at least minsize rows reached the depth (2 for FB and BF, 1 for FF and
-// BB): up to o_last = the largest (FF, BB) or second largest (FB, BF) reach.  A
-// row that reaches the start of m (e = p) is a '$' row of the interval at depth
-// o0 + p: m is a candidate there when t <= o0 + p <= o_last -- exactly the
-// entries the backward search records, written here as one read each.
-//
-// The codes left of g, read leftwards in the searched index's text, are read
-// rightwards in the other direction's text (every read reversed in place): from
-// position start(m) + len(m) - p.  The chain's next codes are the read's own
-// codes: FB n[L-o0-1], n[L-o0-2], ... = the backward text of n from start(n) + o0,
-// BF n[o0], n[o0+1], ... = the forward text from start(n) + o0, FF and BB their
-// complements.  Both are compared 32 codes at a time.
-#define VERIFY_MAX 32
-#define VERIFY_GW 8                 // lanes per chain
-#define VERIFY_ROWS (VERIFY_MAX / VERIFY_GW)
 
+// Chains handed over by the search kernels (cb.vjob): one lane per chain
+// finishes it on the text (verify_chain).
 template <typename R>
 __global__ void __launch_bounds__(256) k_verify(Geom G, DirViewT<R> fw, DirViewT<R> bw,
                                                ChainBuf cb, uint64_t r0, uint64_t nr, uint32_t t,
                                                unsigned long long *stats)
 {
-    const uint64_t chain = ((uint64_t) blockIdx.x * blockDim.x + threadIdx.x) / VERIFY_GW;
-    const uint32_t lane = threadIdx.x & (VERIFY_GW - 1);
+    const uint64_t chain = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t rl = chain >> 2;
     const int ch = (int) (chain & 3);
     const uint32_t job = rl < nr ? cb.vjob[chain] : 0u;
     uint32_t verified = 0;
-    if (job) {   // uniform over the chain's lanes
+    if (job) {
         const uint32_t size = job & 0xFFFFu, o0 = job >> 16;
-        const uint64_t r = r0 + rl, sn = pq_read_start(G, r);
-        const uint32_t Ln = pq_read_len(G, r);
+        const uint64_t r = r0 + rl;
         const bool fwsearch = ch == 0 || ch == 3;
-        const R *sa = fwsearch ? fw.sa : bw.sa;
-        const uint64_t *X = fwsearch ? bw.text2 : fw.text2;
-        const uint64_t *Pt = ch < 2 ? bw.text2 : fw.text2;
-        const uint64_t flip = (ch == 1 || ch == 3) ? ~0ull : 0ull;
-        const uint32_t jmax = Ln - 1 - o0, minsize = (ch == 0 || ch == 2) ? 2u : 1u;
-        const uint64_t l = cb.vl[chain];
-        // the chain's next codes, loaded once: nw words of 32 codes (up to 8 for
-        // reads of 256 codes)
-        const uint32_t nw = (jmax + 31) / 32;
-        uint64_t pw[8];
-        {
-            const uint64_t ps = sn + o0, w0 = ps >> 5;
-            const unsigned sh = (unsigned) (ps & 31) * 2;
-            uint64_t cur = Pt[w0];
-#pragma unroll
-            for (uint32_t w = 0; w < 8; w++) {
-                const uint64_t nxt = w < nw ? Pt[w0 + w + 1] : 0ull;
-                pw[w] = ((cur << sh) | ((nxt >> 1) >> (63 - sh))) ^ flip;
-                cur = nxt;
-            }
-        }
-        uint32_t oc[VERIFY_ROWS], mm[VERIFY_ROWS];
-        bool full[VERIFY_ROWS];
-        uint32_t t1 = 0, t2 = 0;   // the two largest reaches + 1 (0: none)
-#pragma unroll
-        for (int k = 0; k < VERIFY_ROWS; k++) {
-            const uint32_t j = lane + k * VERIFY_GW;
-            full[k] = false;
-            oc[k] = mm[k] = 0;
-            if (j >= size)
-                continue;
-            const uint64_t g = (uint64_t) sa[l + j];
-            const uint64_t m = pq_read_of(G, g);
-            const uint64_t sm = pq_read_start(G, m);
-            const uint32_t p = (uint32_t) (g - sm);
-            const uint32_t J = p < jmax ? p : jmax;
-            // codes left of the row, read rightwards in the other text: consecutive
-            // words funnel-shifted, one new word per 32 codes
-            const uint64_t xs = sm + pq_read_len(G, m) - p, x0 = xs >> 5;
-            const unsigned xsh = (unsigned) (xs & 31) * 2;
-            uint64_t cur = X[x0];
-            uint32_t e = J;
-#pragma unroll
-            for (uint32_t w = 0; w < 8; w++) {
-                if (32 * w >= J)
-                    break;
-                const uint64_t nxt = X[x0 + w + 1];
-                const uint64_t x = pw[w] ^ ((cur << xsh) | ((nxt >> 1) >> (63 - xsh)));
-                if (x) {
-                    const uint32_t f = 32 * w + (uint32_t) __clzll((long long) x) / 2;
-                    e = f < J ? f : J;
-                    break;
-                }
-                cur = nxt;
-            }
-            full[k] = p <= jmax && e == p;
-            oc[k] = o0 + p;
-            mm[k] = (uint32_t) m;
-            const uint32_t v = o0 + e + 1;
-            if (v > t1) {
-                t2 = t1;
-                t1 = v;
-            } else if (v > t2) {
-                t2 = v;
-            }
-        }
-        for (int d = 1; d < VERIFY_GW; d <<= 1) {
-            const uint32_t u1 = __shfl_xor(t1, d, VERIFY_GW), u2 = __shfl_xor(t2, d, VERIFY_GW);
-            const uint32_t lo1 = t1 < u1 ? t1 : u1, hi2 = t2 > u2 ? t2 : u2;
-            t1 = t1 > u1 ? t1 : u1;
-            t2 = lo1 > hi2 ? lo1 : hi2;
-        }
-        const uint32_t top = minsize == 1 ? t1 : t2;   // >= o0 + 1: the chain was live at o0
-        const uint32_t olast = top - 1 < Ln - 1 ? top - 1 : Ln - 1;
-        uint32_t mine = 0;
-#pragma unroll
-        for (int k = 0; k < VERIFY_ROWS; k++)
-            mine += full[k] && oc[k] >= t && oc[k] <= olast ? 1u : 0u;
-        uint32_t before = mine;
-        for (int d = 1; d < VERIFY_GW; d <<= 1) {
-            const uint32_t x = __shfl_up(before, d, VERIFY_GW);
-            if (lane >= (uint32_t) d)
-                before += x;
-        }
-        const uint32_t total = __shfl(before, VERIFY_GW - 1, VERIFY_GW);
-        before -= mine;
-        const uint64_t base = chain * cb.cap;
-#pragma unroll
-        for (int k = 0; k < VERIFY_ROWS; k++) {
-            if (full[k] && oc[k] >= t && oc[k] <= olast) {
-                cb.lc[base + before] = make_uint2(mm[k], 1u | ENTRY_READ);
-                cb.o[base + before] = (uint16_t) oc[k];
-                before++;
-            }
-        }
-        if (lane == 0)
-            cb.n[chain] = total;
-        verified = lane == 0 ? size : 0u;
+        cb.n[chain] = verify_chain<R>(
+            G, fwsearch ? fw.sa : bw.sa, fwsearch ? bw.text2 : fw.text2,
+            ch < 2 ? bw.text2 : fw.text2, (ch == 1 || ch == 3) ? ~0ull : 0ull,
+            pq_read_start(G, r), pq_read_len(G, r), o0, cb.vl[chain], size,
+            (ch == 0 || ch == 2) ? 2u : 1u, t, cb.lc + chain * cb.cap, cb.o + chain * cb.cap);
+        verified = size;
     }
     if (stats) {
         uint64_t v = verified;
@@ -1569,7 +1435,8 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         uint32_t kseed = 0;
         if (timed)
             (void) hipEventRecord(e[0], ctx->stream);
-        if (mode == PQ_SEARCH_LEARNED && (rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed)))
+        if (mode == PQ_SEARCH_LEARNED &&
+            (rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed, cb, fstats)))
             break;
         if (timed)
             (void) hipEventRecord(e[1], ctx->stream);
@@ -1583,7 +1450,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
             hipLaunchKernelGGL(k_find2<R>, dim3(pq_blocks(nr * 2, 256)), dim3(256), 0,
                                ctx->stream, G, fw, bw, r0, nr, t, Ct, seeds, kseed, cb, fstats);
         if (cb.vmax)
-            hipLaunchKernelGGL(k_verify<R>, dim3(pq_blocks(nr * 4 * VERIFY_GW, 256)), dim3(256), 0,
+            hipLaunchKernelGGL(k_verify<R>, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0,
                                ctx->stream, G, fw, bw, cb, r0, nr, t, fstats);
         if (timed)
             (void) hipEventRecord(e[2], ctx->stream);
