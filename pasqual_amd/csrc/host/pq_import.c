@@ -15,7 +15,7 @@
  * order-dependent filter is resolved on the device (pq_dedup_reads, exact),
  * and the kept reads are gathered by all threads again.
  */
-#include "pq_host.h"
+#include "../../../include/pasqual_amd.h"
 
 #include <omp.h>
 #include <stdio.h>

@@ -12,7 +12,7 @@
  * with probability 1/2, with i.i.d. substitutions to one of the three other
  * bases.  Output codes are 1..4 for A,C,G,T (index.c:280-321).
  */
-#include "pq_host.h"
+#include "../../../include/pasqual_amd.h"
 
 #include <math.h>
 #include <stdio.h>
