@@ -33,23 +33,40 @@ template <typename KT, typename VT> struct ArraySrc {
     }
 };
 
+// Digit counts of one tile.  The lanes of a wave holding the same digit are
+// found with 8 ballots and only the lowest of them adds their number to the
+// wave's own LDS counters (the digits of a tile repeat a lot: the low byte of a
+// bucket key is mostly its length field, and one atomic per lane on the same
+// counters serialised there).
 template <typename KT, int ROUNDS, typename Src>
 __global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(Src src, uint64_t count,
                                                              unsigned shift, uint32_t *hist,
                                                              uint32_t nblocks)
 {
     constexpr int RADIX_TILE = RADIX_THREADS * ROUNDS;
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    __shared__ uint32_t h[4][256];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < 4 * 256; i += RADIX_THREADS)
+        (&h[0][0])[i] = 0;
     __syncthreads();
-    uint64_t base = (uint64_t) blockIdx.x * RADIX_TILE;
+    const uint64_t wbase = (uint64_t) blockIdx.x * RADIX_TILE + (uint64_t) wid * (RADIX_TILE / 4);
+    const uint64_t lt = (1ull << lane) - 1ull;
     for (int r = 0; r < ROUNDS; r++) {
-        uint64_t i = base + (uint64_t) r * RADIX_THREADS + threadIdx.x;
-        if (i < count)
-            atomicAdd(&h[(src.key(i) >> shift) & 255u], 1u);
+        const uint64_t i = wbase + (uint64_t) r * 64 + lane;
+        const bool ok = i < count;
+        const unsigned d = ok ? (unsigned) (src.key(i) >> shift) & 255u : 0u;
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint64_t m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        if (ok && (peers & lt) == 0)
+            h[wid][d] += (uint32_t) __popcll(peers);
     }
     __syncthreads();
-    hist[(uint64_t) threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    const int d = threadIdx.x;
+    hist[(uint64_t) d * nblocks + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
 }
 
 template <typename KT, typename VT, int ROUNDS, typename Src>

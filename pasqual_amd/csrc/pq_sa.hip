@@ -590,7 +590,7 @@ static int find_runs(pq_ctx_t *ctx, const KT *K, uint64_t n, uint32_t *flags, ui
 // `left` codes after their common prefix (0: too long, sort by chunk levels).
 static int words_for(uint32_t left)
 {
-    return left <= 32 ? 1 : left <= 64 ? 2 : left <= 96 ? 3 : left <= 128 ? 4 : left <= 256 ? 8 : 0;
+    return left == 0 ? 1 : left <= 256 ? (int) ((left + 31) / 32) : 0;
 }
 
 static uint32_t lcap_for(int W)
@@ -641,6 +641,9 @@ static int sort_lists(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, R *sa, u
     case 2: launch_sorts<2, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
     case 3: launch_sorts<3, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
     case 4: launch_sorts<4, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
+    case 5: launch_sorts<5, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
+    case 6: launch_sorts<6, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
+    case 7: launch_sorts<7, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
     case 8: launch_sorts<8, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
     default:
         if (cnt[0] || cnt[1] || cnt[2] || cnt[3])
