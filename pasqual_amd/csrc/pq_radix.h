@@ -18,7 +18,7 @@
 // (the tile is staged in LDS).
 template <typename KT, typename VT> constexpr int radix_rounds()
 {
-    return sizeof(KT) + sizeof(VT) > 12 ? 8 : 16;
+    return sizeof(KT) + sizeof(VT) >= 12 ? 8 : 16;
 }
 
 // Pairs read from two arrays.
@@ -33,11 +33,14 @@ template <typename KT, typename VT> struct ArraySrc {
     }
 };
 
-// Digit counts of one tile.  The lanes of a wave holding the same digit are
-// found with 8 ballots and only the lowest of them adds their number to the
-// wave's own LDS counters (the digits of a tile repeat a lot: the low byte of a
-// bucket key is mostly its length field, and one atomic per lane on the same
-// counters serialised there).
+#define HIST_TILES 8          // tiles counted per block of the histogram kernel
+
+// Digit counts of HIST_TILES consecutive tiles (the scatter's tiles), each wave
+// into its own LDS counters (a quarter of the same-counter atomics per copy: the
+// digits of a tile repeat a lot, the low byte of a bucket key being mostly its
+// length field).  The counts are laid out digit-major for the scan, so a block
+// writes each digit's HIST_TILES counters as one 32-byte store instead of 256
+// scattered 4-byte stores per tile.
 template <typename KT, int ROUNDS, typename Src>
 __global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(Src src, uint64_t count,
                                                              unsigned shift, uint32_t *hist,
@@ -45,28 +48,34 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(Src src, uint64_t 
 {
     constexpr int RADIX_TILE = RADIX_THREADS * ROUNDS;
     __shared__ uint32_t h[4][256];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < 4 * 256; i += RADIX_THREADS)
-        (&h[0][0])[i] = 0;
-    __syncthreads();
-    const uint64_t wbase = (uint64_t) blockIdx.x * RADIX_TILE + (uint64_t) wid * (RADIX_TILE / 4);
-    const uint64_t lt = (1ull << lane) - 1ull;
-    for (int r = 0; r < ROUNDS; r++) {
-        const uint64_t i = wbase + (uint64_t) r * 64 + lane;
-        const bool ok = i < count;
-        const unsigned d = ok ? (unsigned) (src.key(i) >> shift) & 255u : 0u;
-        uint64_t peers = __ballot(ok);
+    const int wid = threadIdx.x >> 6, d = threadIdx.x;
+    const uint32_t t0 = blockIdx.x * HIST_TILES;
+    uint32_t cnt[HIST_TILES];
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const uint64_t m = __ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? m : ~m;
+    for (int g = 0; g < HIST_TILES; g++) {
+        for (int i = threadIdx.x; i < 4 * 256; i += RADIX_THREADS)
+            (&h[0][0])[i] = 0;
+        __syncthreads();
+        const uint64_t base = (uint64_t) (t0 + g) * RADIX_TILE;
+        for (int r = 0; r < ROUNDS; r++) {
+            const uint64_t i = base + (uint64_t) r * RADIX_THREADS + threadIdx.x;
+            if (i < count)
+                atomicAdd(&h[wid][(src.key(i) >> shift) & 255u], 1u);
         }
-        if (ok && (peers & lt) == 0)
-            h[wid][d] += (uint32_t) __popcll(peers);
+        __syncthreads();
+        cnt[g] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+        __syncthreads();
     }
-    __syncthreads();
-    const int d = threadIdx.x;
-    hist[(uint64_t) d * nblocks + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+    uint32_t *row = hist + (uint64_t) d * nblocks + t0;
+    if (t0 + HIST_TILES <= nblocks && ((nblocks | t0) & 3u) == 0) {
+        reinterpret_cast<uint4 *>(row)[0] = make_uint4(cnt[0], cnt[1], cnt[2], cnt[3]);
+        reinterpret_cast<uint4 *>(row)[1] = make_uint4(cnt[4], cnt[5], cnt[6], cnt[7]);
+    } else {
+#pragma unroll
+        for (int g = 0; g < HIST_TILES; g++)
+            if (t0 + g < nblocks)
+                row[g] = cnt[g];
+    }
 }
 
 template <typename KT, typename VT, int ROUNDS, typename Src>
@@ -179,8 +188,8 @@ static int radix_pass_src(pq_ctx_t *ctx, Src src, KT *kout, VT *vout, uint64_t c
     if (!hist)
         return ctx->fail(PQ_ERR_NOMEM, "radix histogram");
     uint32_t *offs = hist + (size_t) nb * 256;
-    hipLaunchKernelGGL((k_radix_hist<KT, ROUNDS, Src>), dim3(nb), dim3(RADIX_THREADS), 0,
-                       ctx->stream, src, count, shift, hist, nb);
+    hipLaunchKernelGGL((k_radix_hist<KT, ROUNDS, Src>), dim3((nb + HIST_TILES - 1) / HIST_TILES),
+                       dim3(RADIX_THREADS), 0, ctx->stream, src, count, shift, hist, nb);
     int rc = pq_scan_u32(ctx, hist, offs, (uint64_t) nb * 256, nullptr);
     if (rc)
         return rc;
