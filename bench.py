@@ -330,18 +330,22 @@ def search_roofline(positions, lines, rows, kernel_ms, launches, copy_gbs, read_
             "frac_of_scattered_ceiling": round(achieved / best, 4) if best else None}
 
 
-def committed_traffic(config, kname):
-    """HBM bytes per launch of kernel kname on this configuration from the committed
-    counter summary (profiles/config<N>_traffic.json: FETCH_SIZE and WRITE_SIZE
-    per dispatch, each scaled by what the counter reads for the plain copy
-    kernel), or None."""
+def committed_traffic(config, kname, field="traffic_bytes"):
+    """Bytes past the L2 per launch of kernel kname on this configuration (to HBM,
+    or served by the memory-side cache) from the committed counter summary
+    (profiles/config<N>_traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by what the counter reads for the plain copy kernel), or None.  field
+    "traffic_by_request_size": reads counted by request size instead."""
     tpath = os.path.join(ROOT, "profiles", f"config{config}_traffic.json")
     if not os.path.exists(tpath):
         return None
     with open(tpath) as f:
         kernels = json.load(f)["kernels"]
     tk = next((v for k, v in kernels.items() if k == kname or k.startswith(kname + "<")), None)
-    return tk["traffic_bytes"][0] if tk else None
+    if not tk:
+        return None
+    # the mean over the step's dispatches, as the launch time it is set against
+    return int(tk[field]) if tk.get(field) is not None else None
 
 
 def run_workload(pq, np, config, args, cpu_baseline):
@@ -390,10 +394,11 @@ def run_workload(pq, np, config, args, cpu_baseline):
             edges_buf = ctx.host_buffer(e * rec_bytes)
         return ctx.graph_export(nodes_buf, edges_buf, index_bytes)
 
-    P = {"build": [], "search": [], "search_dev": [], "export": [], "kernel": [], "launches": 0,
-         "positions": 0, "lines": 0, "rows": 0}
+    P = {"build": [], "search": [], "search_dev": [], "export": [], "kernel": [], "extract": [],
+         "launches": 0, "positions": 0, "lines": 0, "rows": 0}
     Lr = {"train": [], "search": [], "search_dev": [], "export": [], "kernel": [], "seed": [],
-          "launches": 0, "positions": 0, "lines": 0, "rows": 0, "same": True}
+          "extract": [], "launches": 0, "positions": 0, "lines": 0, "rows": 0, "seed_rows": 0,
+          "probes": 0, "same": True}
     digests = {}
     t_start = time.perf_counter()
     for step in range(args.steps):
@@ -416,6 +421,7 @@ def run_workload(pq, np, config, args, cpu_baseline):
         P["positions"] += tm["find_positions"]
         P["lines"] += tm["find_lines"]
         P["rows"] += tm["verify_rows"]
+        P["extract"].append(tm["extract_ms"])
         if step == 0:
             _, E = ctx.graph_size()
             digests["plain"] = (_digest_buf(nodes_buf.array), _digest_buf(
@@ -440,6 +446,9 @@ def run_workload(pq, np, config, args, cpu_baseline):
             Lr["positions"] += tl["find_positions"]
             Lr["lines"] += tl["find_lines"]
             Lr["rows"] += tl["verify_rows"]
+            Lr["seed_rows"] += tl["seed_rows"]
+            Lr["extract"].append(tl["extract_ms"])
+            Lr["probes"] += sum(ctx.learned_stats()["probes"])
             if step == 0:
                 _, E = ctx.graph_size()
                 got = (_digest_buf(nodes_buf.array), _digest_buf(
@@ -516,6 +525,8 @@ def run_workload(pq, np, config, args, cpu_baseline):
                                  copy_gbs, read_gbs, gather, kname[tune["find_chains"]], n,
                                  args.steps)
     plain_roof["traffic"] = committed_traffic(config, plain_roof["kernel"])
+    plain_roof["traffic_by_request_size"] = committed_traffic(config, plain_roof["kernel"],
+                                                              "traffic_by_request_size")
     res = {
         "config": config, "n": n, "N": N, "t": t, "read_len": read_len, "row_bytes": row_bytes,
         "slices": slices, "how": how, "import": imp, "steps": args.steps,
@@ -532,6 +543,8 @@ def run_workload(pq, np, config, args, cpu_baseline):
         "extraction_shape": {k: tune[k] for k in ("group_lanes", "items")},
         "find_kernel": tune["find_chains"], "verify_rows": tune["verify_rows"],
         "plain_roofline": plain_roof, "build_roofline": build_roofline, "graph_mode": graph,
+        "plain_phases_ms": {plain_roof["kernel"] + " + k_verify": round(sum(P["kernel"]) / args.steps, 3),
+                            "extraction": round(_mean(P["extract"]), 3)},
     }
     if learned_on:
         lsearch = _mean(Lr["search"])
@@ -540,6 +553,30 @@ def run_workload(pq, np, config, args, cpu_baseline):
                                 Lr["launches"], copy_gbs, read_gbs, gather,
                                 kname[tune_l["find_chains"]], n, args.steps)
         lroof["traffic"] = committed_traffic(config, lroof["kernel"] + "_seeded")
+        lroof["traffic_by_request_size"] = committed_traffic(
+            config, lroof["kernel"] + "_seeded", "traffic_by_request_size")
+        # the seed kernel: model, correction probes (sampled keys and suffix-array
+        # rows), and the rows it finishes on the text
+        seed_ms = sum(Lr["seed"])
+        seed_launches = Lr["launches"]   # one per batch, as the search kernel
+        seed_alg = (Lr["probes"] * LINE_BYTES + Lr["seed_rows"] * VERIFY_ROW_BYTES) / \
+            max(1, seed_launches)
+        seed_avg = seed_ms / max(1, seed_launches)
+        seed_gbs = seed_alg / (seed_avg / 1e3) / 1e9 if seed_avg else 0.0
+        seed_roof = {"bound": "hbm", "achieved": round(seed_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(seed_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": committed_traffic(config, "k_learned_seed"),
+                     "traffic_by_request_size": committed_traffic(config, "k_learned_seed",
+                                                                  "traffic_by_request_size"),
+                     "kernel": "k_learned_seed", "alg_bytes_per_launch": int(seed_alg),
+                     "alg_bytes_basis": "64 B per correction probe (a sampled key, or a "
+                                        "suffix-array row and its text) + 72 B per row "
+                                        "finished on the text",
+                     "probes_per_read": round(Lr["probes"] / (n * args.steps), 2),
+                     "verified_rows_per_read": round(Lr["seed_rows"] / (n * args.steps), 1),
+                     "avg_launch_ms": round(seed_avg, 3),
+                     "launches_per_search": round(seed_launches / args.steps, 2),
+                     "copy_ceiling": copy_gbs, "read_ceiling": read_gbs}
         train = _mean(Lr["train"])
         res["learned"] = {
             "reads_per_s": round(n / (lsearch / 1e3), 1), "search_ms": round(lsearch, 3),
@@ -560,7 +597,11 @@ def run_workload(pq, np, config, args, cpu_baseline):
             "edges_identical_to_plain": bool(Lr["same"]),
             "extraction_shape": {k: tune_l[k] for k in ("group_lanes", "items")},
             "find_kernel": tune_l["find_chains"], "verify_rows": tune_l["verify_rows"],
-            "roofline": lroof,
+            "extract_ms": round(_mean(Lr["extract"]), 3),
+            "phases_ms": {"k_learned_seed": round(seed_ms / args.steps, 3),
+                          lroof["kernel"] + " + k_verify": round(sum(Lr["kernel"]) / args.steps, 3),
+                          "extraction": round(_mean(Lr["extract"]), 3)},
+            "roofline": lroof, "seed_roofline": seed_roof,
         }
         res["learned_vs_plain_search"] = round(avg_search / lsearch, 3)
         # build-once / search-once pipeline: the learned search pays for its table
@@ -625,7 +666,201 @@ def summary(res):
     return out
 
 
+def _kernel_key(name):
+    """Kernel name with template arguments, without the parameter list."""
+    n = name[5:] if name.startswith("void ") else name
+    depth = 0
+    for i, ch in enumerate(n):
+        depth += ch == "<"
+        depth -= ch == ">"
+        if ch == "(" and depth == 0 and i > 0:
+            return n[:i].strip()
+    return n.strip()
+
+
+def traffic_summary(fetch_csv, write_csv, out_path, probe_mib, commands, md_path=None,
+                    title="HBM/fabric traffic per dispatch", sizes_csv=None):
+    """Per-dispatch HBM traffic of one step from two rocprofv3 --pmc runs of the
+    same bench command (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled
+    by what its counter reads for the plain copy kernel of probe_mib MiB in each
+    direction.  The step is the last index build and the searches after it; a
+    search kernel launched right after the learned seed kernel is listed as
+    <name>_seeded<...>.  With sizes_csv (a third run collecting
+    TCC_EA0_RDREQ_{32B,64B,128B}_sum) each kernel also gets its read bytes counted
+    by request size, which needs no scale: FETCH_SIZE counts a 128-byte request
+    as 64 bytes on this GPU (hence the copy kernel's scale of 2), while a kernel
+    of scattered 64-byte reads is counted exactly, so the scaled figure
+    overstates such kernels' reads up to twice."""
+    import csv
+
+    def load(path):
+        with open(path) as f:
+            rows = list(csv.DictReader(f))
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        return rows
+
+    F, W = load(fetch_csv), load(write_csv)
+    # optional third run: read requests by size, one row per (dispatch, counter)
+    Q = None
+    if sizes_csv:
+        Q = {}
+        for r in load(sizes_csv):
+            Q.setdefault(int(r["Dispatch_Id"]), {})[r["Counter_Name"]] = float(r["Counter_Value"])
+
+    def read_bytes(did):
+        q = Q.get(did, {}) if Q is not None else {}
+        return int(32 * q.get("TCC_EA0_RDREQ_32B_sum", 0) + 64 * q.get("TCC_EA0_RDREQ_64B_sum", 0)
+                   + 128 * q.get("TCC_EA0_RDREQ_128B_sum", 0))
+
+    names = [_kernel_key(r["Kernel_Name"]) for r in F]
+    if len(F) != len(W) or names != [_kernel_key(r["Kernel_Name"]) for r in W]:
+        raise SystemExit("the two counter runs dispatched different kernels")
+    probe = [i for i, n in enumerate(names) if n.startswith("k_copy_probe")]
+    if not probe:
+        raise SystemExit("no copy probe in the counter runs (run the bench with --probe-copy)")
+    pb = probe_mib << 20
+    fs = pb / (float(F[probe[0]]["Counter_Value"]) * 1024)
+    ws = pb / (float(W[probe[0]]["Counter_Value"]) * 1024)
+    # each build starts with one key histogram per direction: the step is the last build
+    kh = [i for i, n in enumerate(names) if n.startswith("k_key_hist")]
+    start = kh[-2] if len(kh) >= 2 else 0
+    kernels, prev = {}, ""
+    for i in range(start, len(F)):
+        n = names[i]
+        if n.startswith("k_copy_probe") or n.startswith("k_read_probe") or \
+                n.startswith("k_gather_probe"):
+            continue
+        key = n
+        if prev.startswith("k_learned_seed") and n.startswith("k_find"):
+            key = n.replace("<", "_seeded<", 1) if "<" in n else n + "_seeded"
+        e = kernels.setdefault(key, {"fetch_kib": [], "write_kib": [], "traffic_bytes": [],
+                                     "read_request_bytes": []})
+        fk, wk = float(F[i]["Counter_Value"]), float(W[i]["Counter_Value"])
+        e["fetch_kib"].append(round(fk, 1))
+        e["write_kib"].append(round(wk, 1))
+        e["traffic_bytes"].append(int(fk * 1024 * fs + wk * 1024 * ws))
+        if Q is not None:
+            e["read_request_bytes"].append(read_bytes(int(F[i]["Dispatch_Id"])))
+        prev = n
+    for e in kernels.values():
+        c, tb, rq = len(e["traffic_bytes"]), e["traffic_bytes"], e.pop("read_request_bytes")
+        e.update({"dispatches": c, "fetch_kib": round(sum(e["fetch_kib"]) / c, 1),
+                  "write_kib": round(sum(e["write_kib"]) / c, 1), "traffic_bytes": sum(tb) // c,
+                  "traffic_min": min(tb), "traffic_max": max(tb)})
+        if Q is not None:
+            # reads counted exactly by request size, writes by the scaled WRITE_SIZE
+            e["traffic_by_request_size"] = int(sum(rq) / c + e["write_kib"] * 1024 * ws)
+    res = {"command_fetch": commands[0], "command_write": commands[1],
+           "command_read_sizes": commands[2] if Q is not None else None,
+           "probe_bytes": pb, "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+           "dispatches": "the last index build of the run and the searches after it; "
+                         "per kernel the mean KiB and bytes per dispatch; a search kernel "
+                         "launched right after the learned seed kernel is listed as "
+                         "<name>_seeded",
+           "counts": "FETCH_SIZE and WRITE_SIZE count the bytes that leave the L2 for the "
+                     "fabric: served by HBM or by the memory-side cache",
+           "kernels": kernels}
+    with open(out_path, "w") as f:
+        json.dump(res, f, indent=0)
+        f.write("\n")
+    if md_path:
+        out = [f"# {title}", "", "Counter runs, each in its own process:", "", "```",
+               commands[0], commands[1], "```", "",
+               f"Scale: the plain copy kernel `k_copy_probe` moves {pb} bytes each way; "
+               f"FETCH_SIZE reads {float(F[probe[0]]['Counter_Value']):.0f} KiB for it (scale "
+               f"{fs:.3f}), WRITE_SIZE {float(W[probe[0]]['Counter_Value']):.0f} KiB (scale "
+               f"{ws:.3f}).  The counters count what leaves the L2 for the fabric, served "
+               "by HBM or by the memory-side cache.  Dispatches: " + res["dispatches"] + ".",
+               "", "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | scaled traffic "
+               "(GB, mean) | min | max | reads by request size + writes (GB) |",
+               "|---|---:|---:|---:|---:|---:|---:|---:|"]
+        if Q is not None:
+            out[7:7] = [commands[2]]
+        top = sorted(kernels.items(), key=lambda x: -x[1]["traffic_bytes"] * x[1]["dispatches"])
+        for k, e in top[:20]:
+            exact = e.get("traffic_by_request_size")
+            out.append(f"| `{k}` | {e['dispatches']} | {e['fetch_kib']:.0f} | "
+                       f"{e['write_kib']:.0f} | {e['traffic_bytes'] / 1e9:.2f} | "
+                       f"{e['traffic_min'] / 1e9:.2f} | {e['traffic_max'] / 1e9:.2f} | "
+                       + (f"{exact / 1e9:.2f} |" if exact is not None else "- |"))
+        with open(md_path, "w") as f:
+            f.write("\n".join(out) + "\n")
+
+
+def stats_summary(trace_csv, stats_csv, bench_json, out_md, command, title):
+    """Markdown summary of one rocprofv3 --kernel-trace --stats run of the bench: the
+    step's kernels (the last index build and the searches after it, from the trace)
+    and the whole run's top kernels (from the stats file), beside the bench line's
+    own launch times measured with HIP events in the same run."""
+    import csv
+    with open(trace_csv) as f:
+        rows = sorted(csv.DictReader(f), key=lambda r: int(r["Dispatch_Id"]))
+    names = [_kernel_key(r["Kernel_Name"]) for r in rows]
+    kh = [i for i, n in enumerate(names) if n.startswith("k_key_hist")]
+    start = kh[-2] if len(kh) >= 2 else 0
+    step = {}
+    for r, n in zip(rows[start:], names[start:]):
+        e = step.setdefault(n, [0, 0.0])
+        e[0] += 1
+        e[1] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+    with open(stats_csv) as f:
+        stats = list(csv.DictReader(f))
+    line = {}
+    if bench_json and os.path.exists(bench_json):
+        with open(bench_json) as f:
+            txt = [x for x in f.read().splitlines() if x.startswith("{")]
+        line = json.loads(txt[-1]) if txt else {}
+    out = [f"# {title}", "", "Command (run on the GPU box from the repository root):", "",
+           "```", command, "```", ""]
+    if line:
+        roof = line.get("roofline", {})
+        out += [f"Bench line of the same run: value {line.get('value')} {line.get('unit')}, "
+                f"{line.get('ms_per_step')} ms per step; headline kernel "
+                f"`{roof.get('kernel')}` averages {roof.get('avg_launch_ms')} ms per launch "
+                "(HIP events on the library's stream).", ""]
+    out += ["## The step (last index build and the searches after it)", "",
+            "| kernel | launches | total ms | average ms |", "|---|---:|---:|---:|"]
+    for n, (c, t) in sorted(step.items(), key=lambda x: -x[1][1])[:30]:
+        out.append(f"| `{n}` | {c} | {t:.2f} | {t / c:.3f} |")
+    out += ["", "## Whole run (rocprofv3 --stats)", "",
+            "| kernel | calls | total ms | average ms | % |", "|---|---:|---:|---:|---:|"]
+    for r in stats[:25]:
+        out.append(f"| `{_kernel_key(r['Name'])}` | {r['Calls']} | "
+                   f"{int(r['TotalDurationNs']) / 1e6:.2f} | {float(r['AverageNs']) / 1e6:.3f} | "
+                   f"{float(r['Percentage']):.1f} |")
+    with open(out_md, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--stats-summary":
+        sp = argparse.ArgumentParser(prog="bench.py --stats-summary")
+        sp.add_argument("trace_csv")
+        sp.add_argument("stats_csv")
+        sp.add_argument("out")
+        sp.add_argument("--bench-json", default="")
+        sp.add_argument("--command", default="")
+        sp.add_argument("--title", default="Kernel statistics")
+        a = sp.parse_args(sys.argv[2:])
+        stats_summary(a.trace_csv, a.stats_csv, a.bench_json, a.out, a.command, a.title)
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--traffic-summary":
+        tp = argparse.ArgumentParser(prog="bench.py --traffic-summary")
+        tp.add_argument("fetch_csv")
+        tp.add_argument("write_csv")
+        tp.add_argument("out")
+        tp.add_argument("--probe-mib", type=int, default=1024)
+        tp.add_argument("--command-fetch", default="")
+        tp.add_argument("--command-write", default="")
+        tp.add_argument("--md", default="")
+        tp.add_argument("--sizes-csv", default="")
+        tp.add_argument("--command-sizes", default="")
+        tp.add_argument("--title", default="HBM/fabric traffic per dispatch")
+        a = tp.parse_args(sys.argv[2:])
+        traffic_summary(a.fetch_csv, a.write_csv, a.out, a.probe_mib,
+                        (a.command_fetch, a.command_write, a.command_sizes), a.md, a.title,
+                        a.sizes_csv or None)
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -658,7 +893,14 @@ def main():
 
     headline_learned = args.config in LEARNED_HEADLINE and "learned" in res
     value = res["learned"]["reads_per_s"] if headline_learned else res["plain_search_reads_per_s"]
-    roofline = res["learned"]["roofline"] if headline_learned else res["plain_roofline"]
+    # the roofline of the headline search's dominant kernel (by time per search)
+    if headline_learned:
+        L = res["learned"]
+        ph = L["phases_ms"]
+        roofline = L["seed_roofline"] if max(ph, key=ph.get) == "k_learned_seed" \
+            else L["roofline"]
+    else:
+        roofline = res["plain_roofline"]
     line = {
         "metric": "overlap-search reads/s + SA build Gsuffix/s, learned vs plain index, 1 GPU",
         "value": round(value, 1),
@@ -694,7 +936,8 @@ def main():
         "nodes_digest": res["nodes_digest"],
         "overlap_len_sum": res["overlap_len_sum"],
         "extraction_shape": res["extraction_shape"],
-        "find_kernel": res["find_kernel"],
+        "find_kernel": res["find_kernel"], "verify_rows": res["verify_rows"],
+        "plain_phases_ms": res["plain_phases_ms"],
         "roofline": roofline,
         "plain_roofline": res["plain_roofline"],
         "build_roofline": res["build_roofline"],

@@ -295,7 +295,9 @@ typedef struct {
     double find_kernel_ms;   /* summed duration of the candidate-search kernel */
     uint64_t find_positions; /* rank positions evaluated by it (2 per step) */
     uint64_t find_lines;     /* occurrence lines it loaded (1 or 2 per step) */
-    uint64_t verify_rows;    /* suffix-array rows finished on the text (pq_set_verify) */
+    uint64_t verify_rows;    /* suffix-array rows finished on the text after backward
+                                steps (pq_set_verify) */
+    uint64_t seed_rows;      /* rows the learned seed kernel finished on the text */
 } pq_timing_t;
 int pq_get_timing(pq_ctx_t *ctx, pq_timing_t *t);
 int pq_sync(pq_ctx_t *ctx);

@@ -181,7 +181,7 @@ template <typename R> struct SeedT {
 struct ChainBuf;
 // d_seeds: SeedT<uint32_t> or SeedT<uint64_t> per (read, chain) as ctx->wide says;
 // chains whose interval holds at most cb.vmax rows are finished into cb, their
-// rows counted in stats[2]
+// rows counted in stats[3]
 int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed,
                      const ChainBuf &cb, unsigned long long *stats);
 void pq_release_build_scratch(pq_ctx_t *ctx);

@@ -713,7 +713,7 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
         for (int o = 32; o > 0; o >>= 1)
             vr += __shfl_xor(vr, o, 64);
         if ((threadIdx.x & 63) == 0 && vr)
-            atomicAdd(&stats[2], (unsigned long long) vr);
+            atomicAdd(&stats[3], (unsigned long long) vr);
     }
     // block reduction of the statistics, one set of atomics per block
     __shared__ unsigned long long red[4][9];

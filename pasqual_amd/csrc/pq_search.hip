@@ -1279,6 +1279,7 @@ static int run_search_graph(pq_ctx_t *ctx, int mode, uint64_t n, uint64_t batch,
     ctx->timing.find_positions = (uint64_t) hm[8] | ((uint64_t) hm[9] << 32);
     ctx->timing.find_lines = (uint64_t) hm[10] | ((uint64_t) hm[11] << 32);
     ctx->timing.verify_rows = (uint64_t) hm[12] | ((uint64_t) hm[13] << 32);
+    ctx->timing.seed_rows = (uint64_t) hm[14] | ((uint64_t) hm[15] << 32);
     if (hm[3] & 2u)
         return PQ_RETRY_EAGER;
     if (hm[3] & 1u) {
@@ -1300,6 +1301,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     ctx->timing.find_kernel_ms = 0;
     ctx->timing.find_launches = 0;
     ctx->timing.find_positions = ctx->timing.find_lines = ctx->timing.verify_rows = 0;
+    ctx->timing.seed_rows = 0;
     ctx->n_edges = 0;
     int rc;
     if ((rc = pq_slot(ctx, SLOT_OVB, (n + 1) * 4, &ctx->d_ovB)) ||
@@ -1577,6 +1579,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
             ctx->timing.find_positions = (uint64_t) hm[8] | ((uint64_t) hm[9] << 32);
             ctx->timing.find_lines = (uint64_t) hm[10] | ((uint64_t) hm[11] << 32);
             ctx->timing.verify_rows = (uint64_t) hm[12] | ((uint64_t) hm[13] << 32);
+    ctx->timing.seed_rows = (uint64_t) hm[14] | ((uint64_t) hm[15] << 32);
             if (hm[3] & 2u) {
                 rc = ctx->fail(PQ_ERR_STATE, "oversized read side left unprocessed");
             } else if (hm[3] & 1u) {
