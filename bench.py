@@ -586,7 +586,7 @@ def run_workload(pq, np, config, args, cpu_baseline):
             "find_kernel_ms": round(_mean(Lr["kernel"]), 3),
             "train_ms": round(train, 1),
             "train_ms_in_library": round(st["train_ms"], 1),
-            "seed_depth": st["depth"], "model_k": st["k"],
+            "seed_depth": st["depth"], "model_k": st["k"], "model_segments": st["segments"],
             "table_size": list(st["table_size"]),
             "mean_window": [round(x, 2) for x in st["mean_window"]],
             "max_window": list(st["max_window"]),

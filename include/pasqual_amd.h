@@ -179,14 +179,18 @@ void pq_host_free(void *p);
  * exact end points) that predict its suffix-array row (to the sampling stride);
  * an exact correction finds the interval: galloping + binary search over the
  * keys of every sample_stride-th row (the exact part, 16 bytes per sampled row),
- * then binary search on the suffix array itself within the bracketing stride.
+ * then the rows within the bracketing strides on the suffix array and the text:
+ * an interval that may hold at most the verification limit's rows (pq_set_verify)
+ * is scanned outward from the samples, and its chain finished on the text on
+ * the way; a wider one by binary search.
  * Results equal PQ_SEARCH_PLAIN's for any model.  The model is dropped by
  * pq_upload_reads and pq_build_index. */
 typedef struct {
     uint32_t k;              /* prefix length fed to the model (<= 21; capped at D), 21 */
     uint32_t hidden;         /* MLP hidden width (<= 64), 64 */
     uint32_t leaves;         /* piecewise-linear leaves, 4096 */
-    uint32_t segments;       /* linear pieces per leaf (<= 1024), 256 */
+    uint32_t segments;       /* linear pieces per leaf (<= 65536); 0 (the default):
+                                about 8 sampled rows per piece, 16..65536 */
     uint32_t epochs;         /* Adam passes over the sampled training set, 1 */
     uint32_t batch;          /* minibatch size (multiple of 64), 4096 */
     uint32_t samples;        /* training samples drawn from the sampled rows (multiple of 64) */
@@ -211,6 +215,7 @@ typedef struct {
     uint64_t fallback;       /* always 0: a seed is the searched index's own interval */
     uint32_t depth;          /* D: depth the backward search resumes from */
     uint32_t k;              /* model prefix length used */
+    uint32_t segments;       /* linear pieces per leaf used */
 } pq_learned_stats_t;
 
 void pq_learned_default(pq_learned_cfg_t *cfg);

@@ -1171,8 +1171,34 @@ static unsigned chain_code(const uint8_t *s, uint32_t L, int ch, uint32_t o)
     return (ch == 1 || ch == 3) ? 5u - x : x;
 }
 
+/* Rows the learned seed kernel's outward scan reads for a pattern with rows
+ * [l, l + size) of an index of N rows sampled every st-th row (ns samples),
+ * or 0 when it takes the binary searches instead (*scan = 0).  jl = the first
+ * sample at or after row l, jh = the first at or after row l + size; with a
+ * sample inside the interval the scan starts there and reads the members plus
+ * one row past each end that exists above the previous sample; without one it
+ * reads down from the sample's row to one row past the interval. */
+static uint64_t seed_scan_rows(uint64_t l, uint64_t size, uint64_t N, uint64_t st, uint32_t vmax,
+                               int *scan)
+{
+    const uint64_t ns = (N + st - 1) / st;
+    uint64_t jl = (l + st - 1) / st, jh = (l + size + st - 1) / st;
+    if (jl > ns)
+        jl = ns;
+    if (jh > ns)
+        jh = ns;
+    const int m1 = jl < ns && jl * st < l + size;
+    const uint64_t bl = jl > 0 ? (jl - 1) * st + 1 : 0, r1 = jl < ns ? jl * st : N;
+    *scan = vmax > 0 && (!m1 || (jh - jl - 1) * st + 1 <= vmax);
+    if (!*scan)
+        return 0;
+    if (m1)
+        return size + (l > bl) + (l + size < N);
+    return (r1 - l) + (l > bl);
+}
+
 int orc_chain_work(const uint8_t *text, const uint64_t *off, uint64_t n, uint32_t seed_depth,
-                   uint32_t t, uint32_t vmax, uint64_t out[3])
+                   uint32_t t, uint32_t vmax, uint32_t stride, uint64_t out[3])
 {
     out[0] = out[1] = out[2] = 0;
     const uint64_t N = off[n];
@@ -1227,6 +1253,13 @@ int orc_chain_work(const uint8_t *text, const uint64_t *off, uint64_t n, uint32_
                     l = cf[d][c] + a;
                     size = b - a;
                 }
+                int scan = 0;
+                if (stride && o <= t)
+                    out[2] += seed_scan_rows(l, size, N, stride, vmax, &scan);
+                /* a scanned interval of minsize..vmax rows is finished there */
+                if (scan && size >= minsize && size <= vmax)
+                    continue;
+                /* otherwise size < minsize or size > vmax: the walk below goes on */
             }
             while (o < L && size >= minsize) {
                 if (size <= vmax && o <= t) {

@@ -119,10 +119,16 @@ void orc_graph_free(orc_graph_t *g);
  * are verified on the text instead.  A step evaluates 2 rank positions, l and
  * u + 1 of the interval [l, u], on 1 occurrence line of 128 rows when both fall
  * in the same one, else on 2.  out[0] = rank positions, out[1] = lines, out[2] =
- * rows handed to verification, summed over all reads.  Returns 0, or -5 without
- * memory. */
+ * rows handed to verification, summed over all reads.  stride > 0 (with
+ * seed_depth): the seeded chains are found as the learned seed kernel finds
+ * them, every stride-th suffix-array row sampled, and a seeded interval that may
+ * hold at most vmax rows (at depth <= t) is scanned outward from the samples on
+ * the text instead: out[2] counts the rows the scan reads (the members and the
+ * rows just past its ends; for an interval between two samples, the rows from
+ * the upper sample down to one past the interval), and a scanned interval of
+ * minsize..vmax rows is finished there.  Returns 0, or -5 without memory. */
 int orc_chain_work(const uint8_t *text, const uint64_t *off, uint64_t n, uint32_t seed_depth,
-                   uint32_t t, uint32_t vmax, uint64_t out[3]);
+                   uint32_t t, uint32_t vmax, uint32_t stride, uint64_t out[3]);
 
 /* Property check of a suffix array at any size: a permutation of 0..N-1 in
  * strictly increasing (suffix, position) order.  Returns -1 when it holds,

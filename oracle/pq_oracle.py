@@ -61,7 +61,8 @@ def lib():
                                           C.POINTER(C.c_uint64), C.POINTER(C.c_long)]
         _lib.orc_dedup.argtypes = [P, P, C.c_uint64, C.c_int, C.c_uint64, C.c_int, C.c_int, P]
         _lib.orc_slot_sizes.argtypes = [P, P, C.c_uint64, P, P, P]
-        _lib.orc_chain_work.argtypes = [P, P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P]
+        _lib.orc_chain_work.argtypes = [P, P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_uint32, P]
         _lib.orc_mlp_forward.argtypes = [C.c_uint32, C.c_uint32, P, P, C.c_uint64, P]
     return _lib
 
@@ -241,17 +242,18 @@ def search_explicit(codes, off, t):
     return _graph_to_dict(g)
 
 
-def chain_work(codes, off, seed_depth=0, t=0, vmax=0):
+def chain_work(codes, off, seed_depth=0, t=0, vmax=0, stride=0):
     """(rank positions, occurrence lines, verified rows) of the device's
     candidate-search chains over all reads (orc_chain_work); seed_depth > 0:
     chains of reads longer than it start at that depth (the learned search's
     seeds); vmax > 0: chains of at most vmax rows at depths <= t are verified on
-    the text."""
+    the text; stride > 0: seeded intervals are scanned as the learned seed
+    kernel scans them over suffix-array rows sampled every stride rows."""
     codes = np.ascontiguousarray(codes, np.uint8)
     off = np.ascontiguousarray(off, np.uint64)
     out = np.zeros(3, np.uint64)
     if lib().orc_chain_work(_p(codes), _p(off), len(off) - 1, int(seed_depth), int(t), int(vmax),
-                            _p(out)) != 0:
+                            int(stride), _p(out)) != 0:
         raise RuntimeError("oracle chain work failed")
     return int(out[0]), int(out[1]), int(out[2])
 

@@ -90,7 +90,7 @@ class LearnedStats(C.Structure):
                 ("train_ms", C.c_double), ("lookups", C.c_uint64 * 2),
                 ("probes", C.c_uint64 * 2), ("table_size", C.c_uint64 * 2),
                 ("absent", C.c_uint64), ("fallback", C.c_uint64),
-                ("depth", C.c_uint32), ("k", C.c_uint32)]
+                ("depth", C.c_uint32), ("k", C.c_uint32), ("segments", C.c_uint32)]
 
 
 class Tune(C.Structure):
@@ -496,7 +496,8 @@ class Context:
                 "train_loss": list(st.train_loss), "fit_loss": list(st.fit_loss),
                 "train_ms": st.train_ms, "lookups": list(st.lookups), "probes": list(st.probes),
                 "table_size": list(st.table_size), "absent": st.absent,
-                "fallback": st.fallback, "depth": st.depth, "k": st.k}
+                "fallback": st.fallback, "depth": st.depth, "k": st.k,
+                "segments": st.segments}
 
     def learned_params(self, d):
         cnt = C.c_uint32(0)

@@ -32,6 +32,66 @@ __device__ __forceinline__ uint64_t entry_read(uint2 e, uint32_t i, const uint32
 
 #define VERIFY_MAX 32        // largest interval a chain hands to verification
 
+// Candidate records of one lane's chain, staged in LDS before they go out.  A
+// lane records its candidates over a long run of dependent random reads; written
+// one by one, each 8- or 2-byte store reached the L2 alone and its line was
+// evicted (by the read stream) before the next store to it, so every record cost
+// a write request of its own.  Staged, a chain's records leave in one burst at
+// its end (or STAGE_SLOTS at a time).  stg: the lane's first slot; slot s at
+// stg[s * STAGE_STRIDE] (one bank-conflict-free plane per slot across the block's
+// 256 lanes).  Records: read m (an ENTRY_READ entry) and pattern length o.
+#define STAGE_SLOTS 8
+#define STAGE_STRIDE 256
+
+struct CandStage {
+    uint64_t *stg;
+    uint2 *lc;
+    uint16_t *oo;
+    uint32_t n, flushed;     // records so far; the first `flushed` are in lc/oo
+
+    __device__ __forceinline__ CandStage(uint64_t *s, uint2 *l, uint16_t *o)
+        : stg(s), lc(l), oo(o), n(0), flushed(0) {}
+
+    __device__ __forceinline__ void add(uint32_t m, uint32_t o)
+    {
+        stg[(n - flushed) * STAGE_STRIDE] = (uint64_t) m | ((uint64_t) o << 32);
+        n++;
+        if (n - flushed == STAGE_SLOTS) {
+            for (uint32_t s = 0; s < STAGE_SLOTS; s++) {
+                const uint64_t v = stg[s * STAGE_STRIDE];
+                lc[flushed + s] = make_uint2((uint32_t) v, 1u | ENTRY_READ);
+                oo[flushed + s] = (uint16_t) (v >> 32);
+            }
+            flushed = n;
+        }
+    }
+
+    // Keeps the records with o <= olast, in order, in lc/oo; returns their count.
+    __device__ __forceinline__ uint32_t finish(uint32_t olast)
+    {
+        uint32_t k = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t m, o;
+            if (i < flushed) {
+                m = lc[i].x;
+                o = oo[i];
+            } else {
+                const uint64_t v = stg[(i - flushed) * STAGE_STRIDE];
+                m = (uint32_t) v;
+                o = (uint32_t) (v >> 32);
+            }
+            if (o <= olast) {
+                if (i >= flushed || k != i) {
+                    lc[k] = make_uint2(m, 1u | ENTRY_READ);
+                    oo[k] = (uint16_t) o;
+                }
+                k++;
+            }
+        }
+        return k;
+    }
+};
+
 // Verification of one chain (one lane).  The chain of read n (Ln codes, text
 // start sn) stands at depth o0 <= t with rows [l, l + size) of its searched
 // index, size >= minsize, nothing recorded yet.  Row j is the suffix at position
@@ -52,14 +112,15 @@ __device__ __forceinline__ uint64_t entry_read(uint2 e, uint32_t i, const uint32
 // codes: FB n[Ln-o0-1], n[Ln-o0-2], ... = the backward text from sn + o0; BF
 // n[o0], n[o0+1], ... = the forward text from sn + o0; FF and BB their
 // complements (flip = all ones).  Both are compared 32 codes at a time.
-// Candidates are written to lc/oo (at most size entries); returns their count.
-// Two rows are processed per iteration so that two chains of dependent loads
-// (suffix-array row, then the text left of it) are in flight.
+// Candidates are written to lc/oo (at most size entries, staged in stg as in
+// CandStage); returns their count.  Two rows are processed per iteration so that
+// two chains of dependent loads (suffix-array row, then the text left of it) are
+// in flight.
 template <typename R>
 __device__ uint32_t verify_chain(const Geom &G, const R *sa, const uint64_t *X, const uint64_t *Pt,
                                  uint64_t flip, uint64_t sn, uint32_t Ln, uint32_t o0, uint64_t l,
                                  uint32_t size, uint32_t minsize, uint32_t t, uint2 *lc,
-                                 uint16_t *oo)
+                                 uint16_t *oo, uint64_t *stg)
 {
     const uint32_t jmax = Ln - 1 - o0;
     const uint32_t nw = (jmax + 31) / 32;
@@ -76,7 +137,7 @@ __device__ uint32_t verify_chain(const Geom &G, const R *sa, const uint64_t *X, 
         }
     }
     uint32_t t1 = 0, t2 = 0;   // the two largest reaches + 1
-    uint32_t nfull = 0;
+    CandStage cs(stg, lc, oo);
     for (uint32_t j = 0; j < size; j += 2) {
         uint64_t g[2];
 #pragma unroll
@@ -114,23 +175,11 @@ __device__ uint32_t verify_chain(const Geom &G, const R *sa, const uint64_t *X, 
             } else if (v > t2) {
                 t2 = v;
             }
-            if (p <= jmax && e == p && o0 + p >= t) {   // tentative: o_last not known yet
-                lc[nfull] = make_uint2((uint32_t) m, 1u | ENTRY_READ);
-                oo[nfull] = (uint16_t) (o0 + p);
-                nfull++;
-            }
+            if (p <= jmax && e == p && o0 + p >= t)   // tentative: o_last not known yet
+                cs.add((uint32_t) m, o0 + p);
         }
     }
     const uint32_t top = minsize == 1 ? t1 : t2;   // >= o0 + 1: the chain was live at o0
     const uint32_t olast = top - 1 < Ln - 1 ? top - 1 : Ln - 1;
-    uint32_t k = 0;
-    for (uint32_t i = 0; i < nfull; i++) {
-        const uint16_t oc = oo[i];
-        if (oc <= olast) {
-            lc[k] = lc[i];
-            oo[k] = oc;
-            k++;
-        }
-    }
-    return k;
+    return cs.finish(olast);
 }

@@ -9,25 +9,29 @@
 // recorded from depth t on, so no step below D records anything.
 //
 // Per direction:
-//   table   = the distinct D-code prefixes of the suffixes with at least D codes,
-//             in suffix-array order: key (128 bits, first code in the top bits,
-//             bit 0 set), first SA row and number of rows.  The SA rows of a
-//             D-code pattern P are exactly the rows of its table entry.
+//   keys    = the keys of every stride-th suffix-array row (stride 8 by default):
+//             the row's first D codes (first code in the top bits of 128), bit 0
+//             set when the suffix has at least D codes.  Keys are non-decreasing
+//             along the suffix array, so they bracket any pattern's rows to a
+//             stride.
 //   model   = FP32 MLP on the k-code prefix of a key (k = 21 codes mapped to
-//             -1..1) -> 64 ReLU units -> 1 output, fitted to the entry's
-//             position / size of the table, which routes the key to one of 4096
+//             -1..1) -> 64 ReLU units -> 1 output, fitted to the sampled row's
+//             position / number of samples, which routes the key to one of 4096
 //             leaves.
 //   leaves  = each leaf is piecewise linear in the k-code prefix value x: over
 //             the x range of the training keys routed to it, S equal-width
-//             pieces whose end points are exact (x, first entry >= x) pairs.
-//   query   = predicted entry -> exact lower bound of the key in the table by
-//             galloping outward from the prediction until bracketed, then
-//             binary search; the entry gives the SA interval.  The prediction
-//             only narrows the search: the result is exact for any model,
-//             including an untrained one.
+//             pieces (by default about 8 sampled rows each) whose end points are
+//             exact (x, first sampled row >= x) pairs.
+//   query   = predicted sampled row -> exact lower bound of the pattern's key
+//             among the samples by galloping outward from the prediction, then
+//             binary search; the rows between two samples are resolved on the
+//             suffix array and the text (an outward scan that also finishes
+//             narrow chains, or binary searches for wide intervals).  The
+//             prediction only narrows the search: the result is exact for any
+//             model, including an untrained one.
 //
-// Training, all on the device with the kernels below: rows sampled from the
-// table; the output layer first solved in closed form (least squares over the
+// Training, all on the device with the kernels below: training rows drawn from
+// the sampled keys; the output layer first solved in closed form (least squares over the
 // hidden activations, Gram matrix accumulated by a device kernel, 65 x 65
 // system solved on the host), then every parameter refined by minibatch Adam
 // with the forward/backward kernels (block-ordered deterministic reductions).
@@ -44,7 +48,8 @@
 #define MLP_H_MAX 64
 #define MLP_P_MAX (MLP_H_MAX * MLP_K_MAX + 2 * MLP_H_MAX + 1)
 #define TRAIN_BLOCK 64      // samples per block of the gradient kernel (one wave)
-#define SEG_MAX 1024
+#define SEG_MAX 65536       // linear pieces per leaf
+#define SEG_AUTO_ROWS 8     // sampled rows per piece when the count is automatic
 #define REROUTE_MAX 64      // leaf corrections per query
 #define GRAM_SUB 128        // samples staged in LDS per round of the Gram kernel
 #define GRAM_CHUNK 1024     // samples per block of the Gram kernel
@@ -403,16 +408,32 @@ __device__ __forceinline__ uint64_t knot_x(uint64_t xmin, uint64_t xmax, uint32_
 }
 
 // Exact end points of every piece: the first sampled row at or after the knot's
-// k-code prefix.
-__global__ void k_leaf_knots(const u128 *key, uint64_t nU, uint32_t k, const uint64_t *range,
-                             uint32_t leaves, uint32_t S, uint32_t *knots)
+// k-code prefix.  One block per leaf: the knots of leaf j lie between the lower
+// bounds of its range's two ends, which two lanes find over all keys; every knot
+// is then searched within that bracket only.
+__global__ void __launch_bounds__(256) k_leaf_knots(const u128 *key, uint64_t nU, uint32_t k,
+                                                    const uint64_t *range, uint32_t S,
+                                                    uint32_t *knots)
 {
-    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (uint64_t) leaves * (S + 1))
-        return;
-    uint32_t j = (uint32_t) (i / (S + 1)), s = (uint32_t) (i % (S + 1));
-    uint64_t x = knot_x(range[2 * j], range[2 * j + 1], S, s);
-    knots[i] = (uint32_t) lower_bound_all(key, nU, (u128) x << (128 - 2 * k));
+    const uint32_t j = blockIdx.x;
+    const uint64_t xmin = range[2 * j], xmax = range[2 * j + 1];
+    const unsigned sh = 128 - 2 * k;
+    __shared__ uint64_t br[2];
+    if (threadIdx.x < 2)
+        br[threadIdx.x] = lower_bound_all(key, nU, (u128) (threadIdx.x ? xmax : xmin) << sh);
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s <= S; s += blockDim.x) {
+        const u128 q = (u128) knot_x(xmin, xmax, S, s) << sh;
+        uint64_t lo = br[0], hi = br[1];   // xmin <= knot <= xmax: its bound is in [lo, hi]
+        while (lo < hi) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (key[mid] < q)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        knots[(uint64_t) j * (S + 1) + s] = (uint32_t) lo;
+    }
 }
 
 // ---- query ------------------------------------------------------------------------------
@@ -628,6 +649,166 @@ __device__ __forceinline__ u128 chain_pattern(const uint64_t *fwt, uint64_t s, u
     }
 }
 
+// Interval of a pattern found by scanning the suffix array outward from the
+// sampled rows while the chain is finished on the text (one lane).  Row r's key
+// (its first D codes, as suffix_key) is read from the other direction's text X,
+// where the codes of the suffix at g = start(m) + p stand reversed left of
+// xs = start(m) + len(m) - p; the same rows' codes right of xs are the ones
+// verify_chain compares with the chain's next codes, so one window per row both
+// places the row and, for a member, gives its reach.  m1: row r1 is a member
+// (a sampled key equals the pattern's); else the rows in (bl, r1) may hold the
+// whole interval, below rows whose key is larger.  Rows below bl and row bl - 1
+// have smaller keys.  Every member is processed as in verify_chain (reaches, the
+// top two, tentative candidates: at most vmax, staged in stg); a row is visited
+// only if its neighbour towards the sampled row belongs to the interval (or lies
+// above it), so the scan reads the members and at most one row past either end:
+// rows read = size + [lo > bl] + [hi < N] with m1, (r1 - lo) + [lo > bl] without.
+// Two rows are visited per iteration (two chains of dependent loads in flight);
+// a speculative second row beyond an end changes neither the result nor the
+// count of rows read.
+struct ScanOut {
+    uint64_t lo, hi;         // the pattern's rows [lo, hi)
+    uint32_t k;              // candidates kept after the o_last rule (size <= vmax)
+    uint32_t visited;        // rows read
+};
+
+template <typename R>
+__device__ ScanOut scan_chain(const Geom &G, const R *sa, uint64_t N, const uint64_t *X,
+                              const uint64_t *Pt, uint64_t flip, uint64_t sn, uint32_t Ln,
+                              uint32_t D, u128 key, uint64_t bl, uint64_t r1, bool m1,
+                              uint32_t minsize, uint32_t t, uint32_t vmax, uint2 *lc, uint16_t *oo,
+                              uint64_t *stg)
+{
+    const uint32_t o0 = D, jmax = Ln - 1 - o0;
+    const uint32_t nw = (jmax + 31) / 32;
+    uint64_t pw[8];
+    {
+        const uint64_t ps = sn + o0, w0 = ps >> 5;
+        const unsigned sh = (unsigned) (ps & 31) * 2;
+        uint64_t cur = Pt[w0];
+#pragma unroll
+        for (uint32_t w = 0; w < 8; w++) {
+            const uint64_t nxt = w < nw ? Pt[w0 + w + 1] : 0ull;
+            pw[w] = ((cur << sh) | ((nxt >> 1) >> (63 - sh))) ^ flip;
+            cur = nxt;
+        }
+    }
+    uint32_t t1 = 0, t2 = 0, visited = 0;
+    CandStage cs(stg, lc, oo);
+    // sign of (key of the row at text position g) - key; a member's reach and
+    // tentative candidate are recorded on the way
+    auto visit = [&](uint64_t g) -> int {
+        const uint64_t m = pq_read_of(G, g);
+        const uint64_t sm = pq_read_start(G, m);
+        const uint32_t len = pq_read_len(G, m);
+        const uint32_t p = (uint32_t) (g - sm), rem = len - p;   // rem >= 1
+        const uint64_t xs = sm + rem;
+        const uint32_t c = rem < D ? rem : D;
+        const u128 rk = rev_pattern(text_codes(X, xs - c), c) | (u128) (rem >= D ? 1u : 0u);
+        if (rk != key)
+            return rk < key ? -1 : 1;
+        const uint32_t J = p < jmax ? p : jmax;
+        const uint64_t x0 = xs >> 5;
+        const unsigned xsh = (unsigned) (xs & 31) * 2;
+        uint64_t cur = X[x0];
+        uint32_t e = J;
+#pragma unroll
+        for (uint32_t w = 0; w < 8; w++) {
+            if (32 * w >= J)
+                break;
+            const uint64_t nxt = X[x0 + w + 1];
+            const uint64_t x = pw[w] ^ ((cur << xsh) | ((nxt >> 1) >> (63 - xsh)));
+            if (x) {
+                const uint32_t f = 32 * w + (uint32_t) __clzll((long long) x) / 2;
+                e = f < J ? f : J;
+                break;
+            }
+            cur = nxt;
+        }
+        const uint32_t v = o0 + e + 1;
+        if (v > t1) {
+            t2 = t1;
+            t1 = v;
+        } else if (v > t2) {
+            t2 = v;
+        }
+        if (p <= jmax && e == p && o0 + p >= t && cs.n < vmax)   // tentative
+            cs.add((uint32_t) m, o0 + p);
+        return 0;
+    };
+    // rows [lo, hi) are members so far; dn: next row down is dn - 1, up: next row up
+    bool found = m1;
+    uint64_t lo = r1, hi = r1, dn = r1, up = r1 + 1;
+    if (m1) {
+        visit((uint64_t) sa[r1]);
+        hi = r1 + 1;
+        visited = 1;
+    }
+    bool dact = dn > bl, uact = m1 && up < N;
+    while (dact || uact) {
+        // two rows: one each way, or two in the one direction still open
+        uint64_t rr[2];
+        int dir[2];   // -1 down, +1 up, 0 none
+        if (dact && uact) {
+            rr[0] = dn - 1, dir[0] = -1;
+            rr[1] = up, dir[1] = 1;
+        } else if (dact) {
+            rr[0] = dn - 1, dir[0] = -1;
+            rr[1] = dn - 2, dir[1] = dn - 1 > bl ? -1 : 0;
+        } else {
+            rr[0] = up, dir[0] = 1;
+            rr[1] = up + 1, dir[1] = up + 1 < N ? 1 : 0;
+        }
+        uint64_t g[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            g[h] = dir[h] ? (uint64_t) sa[rr[h]] : 0ull;
+        int cmp[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            cmp[h] = dir[h] ? visit(g[h]) : 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint64_t r = rr[h];
+            // rows whose outcome is used are counted (not a speculative one past an end)
+            visited += (dir[h] < 0 && dact) || (dir[h] > 0 && uact);
+            if (dir[h] < 0 && dact) {
+                if (cmp[h] < 0) {           // below the interval
+                    if (!found)
+                        lo = hi = r + 1;
+                    dact = false;
+                } else {
+                    if (cmp[h] == 0) {
+                        if (!found)
+                            hi = r + 1;
+                        found = true;
+                        lo = r;
+                    } else {                // above it: only before the first member
+                        lo = hi = r;
+                    }
+                    dn = r;
+                    dact = r > bl;
+                }
+            } else if (dir[h] > 0 && uact) {
+                if (cmp[h] != 0) {
+                    uact = false;
+                } else {
+                    hi = r + 1;
+                    up = r + 1;
+                    uact = up < N;
+                }
+            }
+        }
+    }
+    uint32_t k = 0;
+    const uint64_t size = hi - lo;
+    if (size >= minsize && size <= vmax) {
+        const uint32_t top = minsize == 1 ? t1 : t2;   // >= o0 + 1: members reach o0
+        k = cs.finish(top - 1 < Ln - 1 ? top - 1 : Ln - 1);
+    }
+    return {lo, hi, k, visited};
+}
+
 // Block of 256 threads = 64 reads; wave q handles chain q of those reads, so the
 // model of a wave is uniform.  Writes Seed{search interval} at depth D for
 // chains of reads longer than D (size 0 when the pattern does not occur).
@@ -646,8 +827,14 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
     uint32_t probes[2] = {0, 0}, looks[2] = {0, 0};
     uint64_t win[2] = {0, 0};
     uint32_t absent = 0, verified = 0;
+    uint32_t nrec = ~0u;   // records of a chain finished here
     // both directions' inference parameters in LDS, read as wave-wide broadcasts
     __shared__ float wsh[2][MLP_P_MAX];
+    // candidate staging; the block's seeds and record counts, written out in the
+    // search kernels' order (read-major: 4 chains per read) in one coalesced pass
+    __shared__ uint64_t stg[STAGE_SLOTS * STAGE_STRIDE];
+    __shared__ SeedT<R> sds[256];
+    __shared__ uint32_t nsh[256];
     for (uint32_t i = threadIdx.x; i < MLP_P_MAX; i += blockDim.x) {
         wsh[0][i] = fw.iparams[i];
         wsh[1][i] = bw.iparams[i];
@@ -671,42 +858,56 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
         SeedT<R> sd = {0, SEED_NONE};
         if (act) {
             // the pattern's rows: the first row whose key is >= key, up to the first
-            // row whose key is > key; the model predicts the sampled row, the
-            // samples bracket the answer to a stride of rows, the suffix array
-            // finishes it
+            // row whose key is > key; the model predicts the sampled row and the
+            // sampled keys bracket both ends to a stride of rows
             const u128 key = pat | 1u, above = key + 1u;   // bit 1 of a key is never set
             const uint64_t st = Sv.stride, ns = Sv.nU;
             const uint64_t g1 = leaf_entry(y, Sv, leaves, S, xk);
             const uint64_t jl = lower_bound_from(Sv.key, ns, key, g1, probes[sdir]);
             looks[sdir]++;
             win[sdir] = jl > g1 ? jl - g1 : g1 - jl;
-            const uint64_t lo = row_lower_bound(G, Sv, jl > 0 ? (jl - 1) * st + 1 : 0,
-                                                jl < ns ? jl * st : Sv.N, key, D, probes[sdir]);
-            uint64_t jh = jl;
-            if (jl < ns && !(Sv.key[jl] > key))
-                jh = lower_bound_from(Sv.key, ns, above, jl, probes[sdir]);
-            const uint64_t hlo = jh > 0 ? (jh - 1) * st + 1 : 0;
-            const uint64_t hi = row_lower_bound(G, Sv, hlo > lo ? hlo : lo,
-                                                jh < ns ? jh * st : Sv.N, above, D, probes[sdir]);
-            sd.l = (R) lo;
-            // (an interval of 2^32 - 2 rows or more takes the plain start instead)
-            sd.size = hi - lo < SEED_DONE ? (uint32_t) (hi - lo) : SEED_NONE;
-            absent = hi == lo;   // no suffix starts with the pattern: the chain is empty
-            // a narrow interval is finished right here on the text (its rows were
-            // just probed): depth D <= t, nothing recorded yet
+            // sampled rows jl..jh-1 hold the pattern (none when jh == jl)
+            const bool m1 = jl < ns && Sv.key[jl] == key;
+            const uint64_t jh = m1 ? lower_bound_from(Sv.key, ns, above, jl, probes[sdir]) : jl;
             const uint32_t minsize = (ch == 0 || ch == 2) ? 2u : 1u;
-            if (sd.size != SEED_NONE && sd.size >= minsize && sd.size <= cb.vmax && D <= t) {
-                const uint64_t chain = rl * 4 + ch, r = r0 + rl;
+            const uint64_t chain = rl * 4 + ch, r = r0 + rl;
+            uint64_t lo, hi;
+            bool done = false;
+            if (cb.vmax > 0 && D <= t && (!m1 || (jh - jl - 1) * st + 1 <= cb.vmax)) {
+                // the interval may be narrow enough to finish the chain on the text
+                // (depth D <= t, nothing recorded yet): scan it outward from the
+                // sampled rows, verifying its rows on the way
                 const uint32_t L = pq_read_len(G, r);
-                cb.n[chain] = verify_chain<R>(
-                    G, Sv.sa, fwsearch ? bw.t2 : fw.t2, ch < 2 ? bw.t2 : fw.t2,
-                    (ch == 1 || ch == 3) ? ~0ull : 0ull, pq_read_start(G, r), L, D, lo, sd.size,
-                    minsize, t, cb.lc + chain * cb.cap, cb.o + chain * cb.cap);
-                verified = sd.size;
-                sd.size = SEED_DONE;
+                const ScanOut so = scan_chain<R>(
+                    G, Sv.sa, Sv.N, fwsearch ? bw.t2 : fw.t2, ch < 2 ? bw.t2 : fw.t2,
+                    (ch == 1 || ch == 3) ? ~0ull : 0ull, pq_read_start(G, r), L, D, key,
+                    jl > 0 ? (jl - 1) * st + 1 : 0, jl < ns ? jl * st : Sv.N, m1, minsize, t,
+                    cb.vmax, cb.lc + chain * cb.cap, cb.o + chain * cb.cap, stg + threadIdx.x);
+                lo = so.lo;
+                hi = so.hi;
+                verified = so.visited;
+                done = hi - lo >= minsize && hi - lo <= cb.vmax;
+                if (done)
+                    nrec = so.k;
+            } else {
+                // a wide interval: binary search on the suffix array within the
+                // bracketing strides
+                lo = row_lower_bound(G, Sv, jl > 0 ? (jl - 1) * st + 1 : 0,
+                                     jl < ns ? jl * st : Sv.N, key, D, probes[sdir]);
+                const uint64_t hlo = jh > 0 ? (jh - 1) * st + 1 : 0;
+                hi = row_lower_bound(G, Sv, hlo > lo ? hlo : lo, jh < ns ? jh * st : Sv.N, above,
+                                     D, probes[sdir]);
             }
+            sd.l = (R) lo;
+            // finished chains keep their records; an interval of 2^32 - 2 rows or
+            // more takes the plain start instead
+            sd.size = done                 ? SEED_DONE
+                      : hi - lo < SEED_DONE ? (uint32_t) (hi - lo)
+                                            : SEED_NONE;
+            absent = hi == lo;   // no suffix starts with the pattern: the chain is empty
         }
-        seeds[rl * 4 + ch] = sd;
+        sds[(threadIdx.x & 63) * 4 + ch] = sd;
+        nsh[(threadIdx.x & 63) * 4 + ch] = nrec;
     }
     if (stats) {
         uint64_t vr = verified;
@@ -728,6 +929,14 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
         for (int q = 0; q < 9; q++)
             red[ch][q] = v[q];
     __syncthreads();
+    {
+        const uint64_t i = (uint64_t) blockIdx.x * 256 + threadIdx.x;   // chain rl * 4 + ch
+        if (i < nr * 4) {
+            seeds[i] = sds[threadIdx.x];
+            if (nsh[threadIdx.x] != ~0u)
+                cb.n[i] = nsh[threadIdx.x];
+        }
+    }
     if (threadIdx.x < 9) {
         int q = threadIdx.x;
         bool mx = (q == 6 || q == 7);
@@ -780,7 +989,7 @@ extern "C" void pq_learned_default(pq_learned_cfg_t *cfg)
     cfg->k = 21;
     cfg->hidden = 64;
     cfg->leaves = 4096;
-    cfg->segments = 256;
+    cfg->segments = 0;
     cfg->epochs = 1;
     cfg->batch = 4096;
     cfg->samples = 1u << 20;
@@ -1047,8 +1256,8 @@ static int train_dir(pq_ctx_t *ctx, Learned &LM, int d, const pq_learned_cfg_t &
     }
     PQ_CHECK(hipMemcpyAsync(M.range, range.data(), range.size() * 8, hipMemcpyHostToDevice,
                             ctx->stream));
-    hipLaunchKernelGGL(k_leaf_knots, dim3(pq_blocks((uint64_t) NL * (SG + 1), 256)), dim3(256), 0,
-                       ctx->stream, M.key, M.nU, LM.k, M.range, NL, SG, M.knots);
+    hipLaunchKernelGGL(k_leaf_knots, dim3(NL), dim3(256), 0, ctx->stream, M.key, M.nU, LM.k,
+                       M.range, SG, M.knots);
     PQ_CHECK(hipGetLastError());
     PQ_CHECK(hipStreamSynchronize(ctx->stream));
     return PQ_OK;
@@ -1072,8 +1281,8 @@ extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
     if (cfg_in)
         cfg = *cfg_in;
     if (cfg.k == 0 || cfg.k > MLP_K_MAX || cfg.hidden == 0 || cfg.hidden > MLP_H_MAX ||
-        cfg.leaves == 0 || cfg.leaves > (1u << 20) || cfg.segments == 0 ||
-        cfg.segments > SEG_MAX || cfg.batch == 0 || cfg.batch % TRAIN_BLOCK ||
+        cfg.leaves == 0 || cfg.leaves > (1u << 20) || cfg.segments > SEG_MAX || cfg.batch == 0 ||
+        cfg.batch % TRAIN_BLOCK ||
         cfg.samples < cfg.batch || cfg.samples % TRAIN_BLOCK || cfg.sample_stride == 0 ||
         cfg.sample_stride > 4096)
         return ctx->fail(PQ_ERR_ARG, "learned index configuration out of range");
@@ -1090,7 +1299,13 @@ extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
     LM->k = std::min<uint32_t>(cfg.k, LM->depth);
     LM->h = cfg.hidden;
     LM->leaves = cfg.leaves;
-    LM->segs = cfg.segments;
+    // automatic piece count: about SEG_AUTO_ROWS sampled rows per piece, so that
+    // interpolation inside a piece is off by a few sampled rows at most
+    const uint64_t ns = (ctx->N + cfg.sample_stride - 1) / cfg.sample_stride;
+    const uint64_t sauto = (ns + (uint64_t) cfg.leaves * SEG_AUTO_ROWS - 1) /
+                           ((uint64_t) cfg.leaves * SEG_AUTO_ROWS);
+    LM->segs = cfg.segments ? cfg.segments
+                            : (uint32_t) std::min<uint64_t>(SEG_MAX, std::max<uint64_t>(16, sauto));
     LM->stride = (uint32_t) cfg.sample_stride;
     ctx->learned = LM;
     ctx->learned_gen++;
@@ -1135,6 +1350,7 @@ extern "C" int pq_learned_stats(pq_ctx_t *ctx, pq_learned_stats_t *st)
     st->train_ms = LM->train_ms;
     st->depth = LM->depth;
     st->k = LM->k;
+    st->segments = LM->segs;
     return PQ_OK;
 }
 

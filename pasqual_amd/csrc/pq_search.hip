@@ -449,6 +449,7 @@ __global__ void __launch_bounds__(256) k_verify(Geom G, DirViewT<R> fw, DirViewT
     const int ch = (int) (chain & 3);
     const uint32_t job = rl < nr ? cb.vjob[chain] : 0u;
     uint32_t verified = 0;
+    __shared__ uint64_t stg[STAGE_SLOTS * STAGE_STRIDE];   // candidate staging
     if (job) {
         const uint32_t size = job & 0xFFFFu, o0 = job >> 16;
         const uint64_t r = r0 + rl;
@@ -457,7 +458,8 @@ __global__ void __launch_bounds__(256) k_verify(Geom G, DirViewT<R> fw, DirViewT
             G, fwsearch ? fw.sa : bw.sa, fwsearch ? bw.text2 : fw.text2,
             ch < 2 ? bw.text2 : fw.text2, (ch == 1 || ch == 3) ? ~0ull : 0ull,
             pq_read_start(G, r), pq_read_len(G, r), o0, cb.vl[chain], size,
-            (ch == 0 || ch == 2) ? 2u : 1u, t, cb.lc + chain * cb.cap, cb.o + chain * cb.cap);
+            (ch == 0 || ch == 2) ? 2u : 1u, t, cb.lc + chain * cb.cap, cb.o + chain * cb.cap,
+            stg + threadIdx.x);
         verified = size;
     }
     if (stats) {

@@ -266,18 +266,20 @@ def test_search_work_counts_match_oracle(ctx, lmin, lmax, t, seed):
     # the rank positions, occurrence lines and verified rows the search kernels
     # report (the roofline's algorithmic bytes) equal the oracle's count of the
     # same chains, for each search kernel, from depth 1 and from the learned
-    # seeds' depth, with verification of narrow intervals and without
+    # seeds' depth, with verification of narrow intervals and without (the
+    # learned seeds' narrow intervals: the rows their scan reads)
     rng = np.random.default_rng(seed)
     codes, off, _ = _random_reads(rng, 6000, 2000, lmin, lmax, err=0.003)
     ctx.upload(codes, off, t)
     ctx.build_index()
-    ctx.learned_train()
+    stride = 5   # every 5th suffix-array row sampled by the learned index
+    ctx.learned_train(sample_stride=stride)
     D = ctx.learned_stats()["depth"]
     for vmax in (0, 32, 5):
         ctx.set_verify(vmax)
         v = min(vmax, int(np.diff(off.astype(np.int64)).max()) - t)
         want = orc.chain_work(codes, off, 0, t, v)
-        want_l = orc.chain_work(codes, off, D, t, v)
+        want_l = orc.chain_work(codes, off, D, t, v, stride)
         assert want_l[0] + want_l[2] < want[0] + want[2]
         for kernel in (1, 2, 3):
             ctx.set_find_chains(kernel)

@@ -285,7 +285,47 @@ def test_slot_sizes_match_reference(name, width):
         assert mx == want[f"max_size_{tag}"] and sum(size) == n * L
 
 
-def _chain_work_bruteforce(codes, off, seed_depth=0, t=0, vmax=0):
+def _seed_scan_walk(sufs, pat, st, vmax):
+    # rows the learned seed kernel's scan reads, walked over the sorted suffixes
+    # themselves: samples are rows 0, st, 2 st, ...; the first sample whose
+    # D-code prefix is >= the pattern, and whether it holds the pattern; the scan
+    # goes down (and up, from a sample holding the pattern) until a row leaves
+    # the pattern, or the previous sample's row is reached
+    D, N = len(pat), len(sufs)
+    samples = list(range(0, N, st))
+    jl = next((j for j, r in enumerate(samples) if sufs[r][:D] >= pat), len(samples))
+    m1 = jl < len(samples) and sufs[samples[jl]][:D] == pat
+    if m1:
+        jh = next((j for j in range(jl, len(samples)) if sufs[samples[j]][:D] > pat), len(samples))
+        if (jh - jl - 1) * st + 1 > vmax:
+            return None
+    floor = samples[jl - 1] + 1 if jl > 0 else 0
+    top = samples[jl] if jl < len(samples) else N
+    rows = 0
+    if m1:
+        rows, r = 1, top - 1
+        while r >= floor:
+            rows += 1
+            if sufs[r][:D] != pat:
+                break
+            r -= 1
+        r = top + 1
+        while r < N:
+            rows += 1
+            if sufs[r][:D] != pat:
+                break
+            r += 1
+    else:
+        r = top - 1
+        while r >= floor:
+            rows += 1
+            if sufs[r][:D] < pat:
+                break
+            r -= 1
+    return rows
+
+
+def _chain_work_bruteforce(codes, off, seed_depth=0, t=0, vmax=0, stride=0):
     # the chains' work from explicit suffix strings: the interval of a pattern is
     # [number of suffixes sorting before it, + number having it as a prefix) in the
     # order end of read first, codes 1<2<3<4 (the suffix-array order)
@@ -316,10 +356,17 @@ def _chain_work_bruteforce(codes, off, seed_depth=0, t=0, vmax=0):
             def sym(o):
                 x = s[L - o] if ch < 2 else s[o - 1]
                 return 5 - x if ch in (1, 3) else x
-            o = seed_depth if 1 <= seed_depth < L else 1
+            seeded = 1 <= seed_depth < L
+            o = seed_depth if seeded else 1
             while o < L:
                 pat = tuple(sym(j) for j in range(o, 0, -1))   # prepended codes first
                 lo, k = interval(d, pat)
+                if seeded and o == seed_depth and stride and vmax and o <= t:
+                    scanned = _seed_scan_walk(sufs[d], pat, stride, vmax)
+                    if scanned is not None:
+                        rows += scanned
+                        if minsize <= k <= vmax:
+                            break
                 if k < minsize:
                     break
                 if k <= vmax and o <= t:
@@ -331,19 +378,21 @@ def _chain_work_bruteforce(codes, off, seed_depth=0, t=0, vmax=0):
     return pos, lines, rows
 
 
-@pytest.mark.parametrize("seed,varlen,depth,t,vmax", [(0, False, 0, 0, 0), (1, True, 0, 0, 0),
-                                                      (2, False, 5, 0, 0), (3, True, 7, 0, 0),
-                                                      (4, False, 0, 0, 0), (5, False, 0, 6, 4),
-                                                      (6, True, 0, 9, 8), (7, True, 5, 8, 32)])
-def test_chain_work_matches_bruteforce(seed, varlen, depth, t, vmax):
+@pytest.mark.parametrize("seed,varlen,depth,t,vmax,stride",
+                         [(0, False, 0, 0, 0, 0), (1, True, 0, 0, 0, 0), (2, False, 5, 0, 0, 0),
+                          (3, True, 7, 0, 0, 0), (4, False, 0, 0, 0, 0), (5, False, 0, 6, 4, 0),
+                          (6, True, 0, 9, 8, 0), (7, True, 5, 8, 32, 0), (8, True, 5, 8, 32, 8),
+                          (9, False, 4, 6, 5, 3), (10, True, 6, 9, 7, 1), (11, True, 6, 6, 2, 2)])
+def test_chain_work_matches_bruteforce(seed, varlen, depth, t, vmax, stride):
     # the oracle's count of the device chains' rank positions, occurrence lines
     # and verified rows (FM walk with per-block ranks) equals the count from
     # explicit sorted suffixes; the repeat-rich genome makes intervals span
-    # 128-row lines
+    # 128-row lines; with a sampling stride, the seeded chains' scan is walked
+    # row by row over the sorted suffixes
     rng = np.random.default_rng(100 + seed)
     codes, off, _ = _random_reads(rng, 90, 60, 10, 30 if varlen else 10, err=0.02)
-    want = _chain_work_bruteforce(codes, off, depth, t, vmax)
-    assert orc.chain_work(codes, off, depth, t, vmax) == want
+    want = _chain_work_bruteforce(codes, off, depth, t, vmax, stride)
+    assert orc.chain_work(codes, off, depth, t, vmax, stride) == want
     if vmax:
         assert want[2] > 0
     else:
