@@ -258,7 +258,8 @@ typedef struct {
     uint32_t items;          /* candidates per side held in LDS (16, 32, 64) */
     uint32_t nshapes;
     double extract_ms[4];    /* extraction time measured per shape */
-    uint32_t find_chains;    /* candidate-search kernel chosen (1, 2 or 3, below) */
+    uint32_t find_chains;    /* candidate-search kernel chosen (1, 2 or 3, below; 0 for
+                                PQ_SEARCH_LEARNED, which has one) */
     double find_ms[3];       /* candidate-search time of kernels 1, 2 and 3 */
     uint32_t verify_rows;    /* verification threshold chosen (0, 8 or 32, pq_set_verify) */
     double verify_ms[3];     /* candidate search + verification time with 0, 8 and 32 */
@@ -270,7 +271,10 @@ int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape);
  * load each other's occurrence lines together, one 64-byte access per line (3);
  * same result.  pq_autotune times all three and keeps the fastest; until then
  * (or after chains = 0) the kernel follows the size of both indexes'
- * occurrence lines: 2 below 1 GiB, 1 below 4 GiB, 3 from there on. */
+ * occurrence lines: 2 below 1 GiB, 1 below 4 GiB, 3 from there on.  The seeded
+ * search (PQ_SEARCH_LEARNED) ignores the choice: it walks the chains the seed
+ * kernel left unfinished from a work list, one chain per lane, each lane taking
+ * the next listed chain as its own ends. */
 int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains);
 /* A chain of the candidate search whose interval holds at most `rows` suffix-
  * array rows (<= 32) at a depth below the minimum overlap is finished on the

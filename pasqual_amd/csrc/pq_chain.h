@@ -14,6 +14,9 @@ struct ChainBuf {
     uint64_t *vl;            // per chain: interval start handed to verification
     uint32_t *vjob;          // per chain: rows | depth << 16, 0 = none
     uint32_t vmax;           // largest interval finished by verification (0: none)
+    uint32_t *wq;            // seeded search: the chains left to search (batch chain
+                             // indices), appended by the seed kernel; nullptr otherwise
+    uint32_t *wq_cnt;        // [0] chains listed, [1] chains taken by the search kernel
 };
 
 // An entry whose count carries ENTRY_READ names its one read directly.

@@ -930,11 +930,32 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
             red[ch][q] = v[q];
     __syncthreads();
     {
-        const uint64_t i = (uint64_t) blockIdx.x * 256 + threadIdx.x;   // chain rl * 4 + ch
+        // thread i of the block writes chain i of its 64 reads (rl * 4 + ch order);
+        // chains the search kernel must still walk go to the work list
+        const uint64_t i = (uint64_t) blockIdx.x * 256 + threadIdx.x;
+        bool listed = false;
         if (i < nr * 4) {
-            seeds[i] = sds[threadIdx.x];
+            const SeedT<R> sd = sds[threadIdx.x];
+            const uint32_t q = threadIdx.x & 3, ms = (q == 0 || q == 2) ? 2u : 1u;
+            seeds[i] = sd;
+            cb.vjob[i] = 0;
+            listed = sd.size == SEED_NONE || (sd.size != SEED_DONE && sd.size >= ms);
             if (nsh[threadIdx.x] != ~0u)
-                cb.n[i] = nsh[threadIdx.x];
+                cb.n[i] = nsh[threadIdx.x];   // finished here
+            else if (!listed)
+                cb.n[i] = 0;                  // empty at depth D: no records
+        }
+        if (cb.wq) {
+            const uint64_t bal = __ballot(listed);
+            if (bal) {
+                const int lane = threadIdx.x & 63, lead = __ffsll((long long) bal) - 1;
+                uint32_t base = 0;
+                if (lane == lead)
+                    base = atomicAdd(&cb.wq_cnt[0], (uint32_t) __popcll(bal));
+                base = __shfl(base, lead, 64);
+                if (listed)
+                    cb.wq[base + (uint32_t) __popcll(bal & ((1ull << lane) - 1))] = (uint32_t) i;
+            }
         }
     }
     if (threadIdx.x < 9) {

@@ -435,6 +435,123 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
     }
 }
 
+// The seeded search's chains, from the work list the seed kernel wrote (only the
+// chains it left unfinished: most are finished or empty at the seed depth).  One
+// chain per lane; a lane whose chain ends takes the next listed chain, so the
+// lanes of a wave stay busy until the list is drained instead of idling behind
+// the wave's longest chain.  Idle lanes take chains together, one atomic per
+// wave, once a quarter of the wave is idle or none of it is busy.  Launched as a
+// grid that fills the GPU once.
+template <typename R>
+__global__ void __launch_bounds__(256) k_find_list(Geom G, DirViewT<R> fw, DirViewT<R> bw,
+                                                   uint64_t r0, uint32_t t, CTab<R> C,
+                                                   const SeedT<R> *seeds, uint32_t kseed,
+                                                   ChainBuf out, unsigned long long *stats)
+{
+    const uint32_t total = out.wq_cnt[0];
+    const int lane = threadIdx.x & 63;
+    uint32_t steps = 0, lines = 0;
+    bool live = false, more = true;
+    uint64_t chain = 0, s = 0;
+    uint32_t L = 0, o = 0, size = 0, nout = 0, minsize = 1;
+    int ch = 0;
+    R l = 0;
+    for (;;) {
+        const bool want = !live && more;
+        const uint64_t wm = __ballot(want);
+        if (wm && (__popcll(wm) >= 16 || !__ballot(live))) {
+            const int lead = __ffsll((long long) wm) - 1;
+            uint32_t base = 0;
+            if (lane == lead)
+                base = atomicAdd(&out.wq_cnt[1], (uint32_t) __popcll(wm));
+            base = __shfl(base, lead, 64);
+            if (want) {
+                const uint32_t idx = base + (uint32_t) __popcll(wm & ((1ull << lane) - 1));
+                if (idx < total) {
+                    chain = out.wq[idx];
+                    const uint64_t r = r0 + (chain >> 2);
+                    ch = (int) (chain & 3);
+                    s = pq_read_start(G, r);
+                    L = pq_read_len(G, r);
+                    minsize = (ch == 0 || ch == 2) ? 2u : 1u;
+                    SeedT<R> sd = {0, SEED_NONE};
+                    if (kseed > 0 && kseed < L)
+                        sd = seeds[chain];
+                    if (sd.size != SEED_NONE) {
+                        o = kseed;
+                        l = sd.l;
+                        size = sd.size;
+                    } else {
+                        // C[0..5] = 0, 0 (A), C[2], C[3], C[4], C[5] (index.c:367-372)
+                        const unsigned c = chain_sym(fw.text2, s, L, ch, 1);
+                        o = 1;
+                        l = pq_pick6(C.c, c);
+                        size = (uint32_t) (pq_pick6(C.c, c + 1) - l);
+                    }
+                    nout = 0;
+                    live = o < L && size >= minsize && !hand_over(out, chain, o, t, l, size);
+                    if (!live)
+                        out.n[chain] = 0;
+                } else {
+                    more = false;
+                }
+            }
+        }
+        if (!__ballot(live)) {
+            if (!__ballot(more))
+                break;
+            continue;
+        }
+        if (live) {
+            const bool fwsearch = ch == 0 || ch == 3;
+            const bool next = o + 1 < L;
+            const unsigned c = next ? chain_sym(fw.text2, s, L, ch, o + 1) : 1u;
+            const uint64_t u1 = (uint64_t) l + size;
+            const uint64_t wl = (uint64_t) l >> 7, wu = u1 >> 7;
+            const OccLine *occ = fwsearch ? fw.occ : bw.occ;
+            OccLine la, lb;
+            pq_load_line(occ, wl, la);
+            if (wu != wl)
+                pq_load_line(occ, wu, lb);
+            else
+                lb = la;
+            const StepRanks a = pq_step_ranks(la, wl, (unsigned) (l & 127), c);
+            const StepRanks b = pq_step_ranks(lb, wu, (unsigned) (u1 & 127), c);
+            steps++;
+            lines += wl == wu ? 1u : 2u;
+            if (o >= t) {
+                const uint32_t cnt = b.d - a.d;
+                if (cnt && nout < out.cap) {
+                    out.lc[chain * out.cap + nout] = make_uint2(a.d, cnt);
+                    out.o[chain * out.cap + nout] = (uint16_t) o;
+                    nout++;
+                }
+            }
+            if (next) {
+                l = (fwsearch ? pq_pick5(fw.cf, c) : pq_pick5(bw.cf, c)) + a.c;
+                size = b.c - a.c;
+                o++;
+                live = size >= minsize && !hand_over(out, chain, o, t, l, size);
+            } else {
+                live = false;
+            }
+            if (!live)
+                out.n[chain] = nout;
+        }
+    }
+    if (stats) {
+        uint64_t s2 = steps, ln = lines;
+        for (int d = 32; d > 0; d >>= 1) {
+            s2 += __shfl_xor(s2, d, 64);
+            ln += __shfl_xor(ln, d, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&stats[0], (unsigned long long) (2 * s2));
+            atomicAdd(&stats[1], (unsigned long long) ln);
+        }
+    }
+}
+
 // ---- verification ----------------------------------------------------------------------
 
 // Chains handed over by the search kernels (cb.vjob): one lane per chain
@@ -1380,10 +1497,27 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
                       &gscratch)) ||
         (rc = pq_slot(ctx, SLOT_HUGE_LIST, batch * sizeof(HugeRead), &huge)) ||
         (mode == PQ_SEARCH_LEARNED &&
-         (rc = pq_slot(ctx, SLOT_SEEDS, batch * 4 * sizeof(SeedT<R>), &seeds)))) {
+         ((rc = pq_slot(ctx, SLOT_SEEDS, batch * 4 * sizeof(SeedT<R>), &seeds)) ||
+          (rc = pq_slot(ctx, SLOT_WORKQ, (batch * 4 + 64) * 4, &cb.wq))))) {
         cleanup();
         return rc;
     }
+    // the seeded search walks the seed kernel's work list with a grid that fills
+    // the GPU once (as many blocks per compute unit as fit at once)
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+            hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_find_list<R>, 256, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 4;
+    const uint64_t list_grid = (uint64_t) cus * per_cu;
+    if (mode == PQ_SEARCH_LEARNED)
+        cb.wq_cnt = cb.wq + batch * 4;
+    else
+        cb.wq = cb.wq_cnt = nullptr;
     ExtOut eo;
     eo.ovB = ctx->d_ovB;
     eo.ovF = ctx->d_ovF;
@@ -1439,12 +1573,20 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         uint32_t kseed = 0;
         if (timed)
             (void) hipEventRecord(e[0], ctx->stream);
-        if (mode == PQ_SEARCH_LEARNED &&
-            (rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed, cb, fstats)))
-            break;
+        if (mode == PQ_SEARCH_LEARNED) {
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, cb.wq_cnt, 2u);
+            if ((rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed, cb, fstats)))
+                break;
+        }
         if (timed)
             (void) hipEventRecord(e[1], ctx->stream);
-        if (chains == 1)
+        if (mode == PQ_SEARCH_LEARNED)
+            hipLaunchKernelGGL(k_find_list<R>,
+                               dim3((uint32_t) std::min<uint64_t>(pq_blocks(nr * 4, 256),
+                                                                  list_grid)),
+                               dim3(256), 0, ctx->stream, G, fw, bw, r0, t, Ct, seeds, kseed, cb,
+                               fstats);
+        else if (chains == 1)
             hipLaunchKernelGGL(k_find1<R>, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0,
                                ctx->stream, G, fw, bw, r0, nr, t, Ct, seeds, kseed, cb, fstats);
         else if (chains == 3)
@@ -1717,7 +1859,8 @@ extern "C" int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out)
     // translation caches cover) together with the verification threshold (0: no
     // verification, 8 or 32 rows; verification reads the suffix array, whose rows
     // are scattered over tens of GiB at the largest sizes): every pair timed, the
-    // fastest kept (search kernel + verification time)
+    // fastest kept (search kernel + verification time).  The seeded search has
+    // one search kernel (the work-list walk): only the threshold is timed.
     static const uint32_t VROWS[3] = {0, 8, 32};
     double best_find = 1e300;
     uint32_t fk = 1, vk = 2;
@@ -1725,7 +1868,8 @@ extern "C" int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out)
         t.find_ms[k] = 1e300;
     for (uint32_t v = 0; v < 3; v++)
         t.verify_ms[v] = 1e300;
-    for (uint32_t k = 1; k <= 3; k++) {
+    const uint32_t klast = mode == PQ_SEARCH_LEARNED ? 1 : 3;
+    for (uint32_t k = 1; k <= klast; k++) {
         for (uint32_t v = 0; v < 3; v++) {
             ctx->find_chains = k;
             ctx->verify_rows = VROWS[v];
@@ -1742,9 +1886,9 @@ extern "C" int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out)
             }
         }
     }
-    ctx->find_chains = fk;
+    ctx->find_chains = mode == PQ_SEARCH_LEARNED ? 0 : fk;
     ctx->verify_rows = VROWS[vk];
-    t.find_chains = fk;
+    t.find_chains = mode == PQ_SEARCH_LEARNED ? 0 : fk;
     t.verify_rows = VROWS[vk];
     t.shape = pick;
     t.group_lanes = EXT_SHAPES[pick][0];
