@@ -507,7 +507,7 @@ def run_workload(pq, np, config, args, cpu_baseline):
     lens = np.diff(off)
     read_len = int(lens[0]) if int(lens.min()) == int(lens.max()) else \
         f"{int(lens.min())}..{int(lens.max())}"
-    kname = {1: "k_find1", 2: "k_find2", 3: "k_find1q"}
+    kname = {0: "k_find_list", 1: "k_find1", 2: "k_find2", 3: "k_find1q", 4: "k_find_list"}
     avg_build = _mean(P["build"])
     avg_search = _mean(P["search"])
     gsuffix_per_s = 2 * N / (avg_build / 1e3) / 1e9

@@ -258,9 +258,9 @@ typedef struct {
     uint32_t items;          /* candidates per side held in LDS (16, 32, 64) */
     uint32_t nshapes;
     double extract_ms[4];    /* extraction time measured per shape */
-    uint32_t find_chains;    /* candidate-search kernel chosen (1, 2 or 3, below; 0 for
+    uint32_t find_chains;    /* candidate-search kernel chosen (1 to 4, below; 0 for
                                 PQ_SEARCH_LEARNED, which has one) */
-    double find_ms[3];       /* candidate-search time of kernels 1, 2 and 3 */
+    double find_ms[4];       /* candidate-search time of kernels 1 to 4 */
     uint32_t verify_rows;    /* verification threshold chosen (0, 8 or 32, pq_set_verify) */
     double verify_ms[3];     /* candidate search + verification time with 0, 8 and 32 */
 } pq_tune_t;
@@ -268,8 +268,9 @@ int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out);
 int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape);
 /* The candidate search runs the four chains of a read on four lanes (1), the
  * two chains of each index interleaved on one lane (2), or on four lanes that
- * load each other's occurrence lines together, one 64-byte access per line (3);
- * same result.  pq_autotune times all three and keeps the fastest; until then
+ * load each other's occurrence lines together, one 64-byte access per line (3),
+ * or one chain per lane with each lane taking the batch's next chain as its own
+ * ends (4); same result.  pq_autotune times all four and keeps the fastest; until then
  * (or after chains = 0) the kernel follows the size of both indexes'
  * occurrence lines: 2 below 1 GiB, 1 below 4 GiB, 3 from there on.  The seeded
  * search (PQ_SEARCH_LEARNED) ignores the choice: it walks the chains the seed
@@ -306,7 +307,11 @@ typedef struct {
     uint64_t find_lines;     /* occurrence lines it loaded (1 or 2 per step) */
     uint64_t verify_rows;    /* suffix-array rows finished on the text after backward
                                 steps (pq_set_verify) */
-    uint64_t seed_rows;      /* rows the learned seed kernel finished on the text */
+    uint64_t seed_rows;      /* rows the learned seed kernel read on the text (members
+                                of the intervals it scanned and the rows just past them) */
+    uint64_t extract_items;  /* candidate items the extraction expanded (both sides) */
+    uint64_t extract_reads[4]; /* reads finished per extraction tier: the three LDS
+                                tiers in launch order, then global scratch */
 } pq_timing_t;
 int pq_get_timing(pq_ctx_t *ctx, pq_timing_t *t);
 int pq_sync(pq_ctx_t *ctx);

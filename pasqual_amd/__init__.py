@@ -96,7 +96,7 @@ class LearnedStats(C.Structure):
 class Tune(C.Structure):
     _fields_ = [("shape", C.c_uint32), ("group_lanes", C.c_uint32), ("items", C.c_uint32),
                 ("nshapes", C.c_uint32), ("extract_ms", C.c_double * 4),
-                ("find_chains", C.c_uint32), ("find_ms", C.c_double * 3),
+                ("find_chains", C.c_uint32), ("find_ms", C.c_double * 4),
                 ("verify_rows", C.c_uint32), ("verify_ms", C.c_double * 3)]
 
 
@@ -106,7 +106,8 @@ class Timing(C.Structure):
                 ("learned_ms", C.c_double), ("find_launches", C.c_uint64),
                 ("find_kernel_ms", C.c_double), ("find_positions", C.c_uint64),
                 ("find_lines", C.c_uint64), ("verify_rows", C.c_uint64),
-                ("seed_rows", C.c_uint64)]
+                ("seed_rows", C.c_uint64), ("extract_items", C.c_uint64),
+                ("extract_reads", C.c_uint64 * 4)]
 
     def as_dict(self):
         return {"upload_ms": self.upload_ms, "sa_ms": list(self.sa_ms),
@@ -115,7 +116,8 @@ class Timing(C.Structure):
                 "learned_ms": self.learned_ms, "find_launches": self.find_launches,
                 "find_kernel_ms": self.find_kernel_ms, "find_positions": self.find_positions,
                 "find_lines": self.find_lines, "verify_rows": self.verify_rows,
-                "seed_rows": self.seed_rows}
+                "seed_rows": self.seed_rows, "extract_items": self.extract_items,
+                "extract_reads": list(self.extract_reads)}
 
 
 _LIBS = {}

@@ -135,7 +135,7 @@ enum PqSlot {
     SLOT_OVB = 34, SLOT_OVF, SLOT_NUMB, SLOT_NUMF, SLOT_EDGE_OFF, SLOT_EDGES,
     SLOT_LEARNED = 40, SLOT_BWT8 = 47, SLOT_SLICE_HIST, SLOT_SLICE_TILES, SLOT_PROBE = 50,
     SLOT_BIGRUNS, SLOT_NODES, SLOT_MISC2, SLOT_HUGE, SLOT_HUGE_LIST, SLOT_VJOB_L, SLOT_VJOB,
-    SLOT_WORKQ,
+    SLOT_WORKQ, SLOT_WSTATS,
 };
 
 // Grow-only slot buffer of at least `bytes` bytes into *p, or PQ_ERR_NOMEM.

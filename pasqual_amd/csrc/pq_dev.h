@@ -82,6 +82,50 @@ template <typename R> __device__ __forceinline__ R pq_pick5(const R (&v)[5], uns
     for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < (count);   \
          i += (uint64_t) gridDim.x * blockDim.x)
 
+// Work counters, spread over WSTAT_SLOTS copies WSTAT_STRIDE words apart (4 KiB
+// + 128 B: different lines and memory channels).  A device-scope atomic is
+// resolved on the memory side of this GPU (each XCD has its own L2), where
+// atomics to one address serialize at roughly 10 ns each: a million of them
+// from one launch to one counter cost about 10 ms.  Block b adds to copy
+// b % WSTAT_SLOTS; the copies are summed once at the end.
+#define WSTAT_SLOTS 64
+#define WSTAT_STRIDE 528
+#define WSTAT_WORDS 16           // counters per copy
+enum {
+    WS_POSITIONS = 0,            // rank positions of backward steps
+    WS_LINES,                    // occurrence lines loaded by them
+    WS_VERIFY_ROWS,              // rows finished on the text by k_verify
+    WS_SEED_ROWS,                // rows the learned seed kernel read on the text
+    WS_ITEMS,                    // candidate items expanded by the extraction
+    WS_TIER_READS,               // + tier: reads finished by extraction tier 0..3
+};
+
+__device__ __forceinline__ void wstat_add(unsigned long long *ws, int q, unsigned long long v)
+{
+    if (v)
+        atomicAdd(ws + (blockIdx.x % WSTAT_SLOTS) * WSTAT_STRIDE + q, v);
+}
+
+// Zeroes counters [0, words) of every copy (grid: WSTAT_SLOTS blocks of >= words).
+static __global__ void k_wstat_clear(unsigned long long *ws, uint32_t words)
+{
+    if (threadIdx.x < words)
+        ws[(uint64_t) blockIdx.x * WSTAT_STRIDE + threadIdx.x] = 0;
+}
+
+// out[q] = counter q summed over the copies, q < words (one block of >= words).
+static __global__ void k_wstat_sum(const unsigned long long *ws, uint32_t words,
+                                   unsigned long long *out)
+{
+    const uint32_t q = threadIdx.x;
+    if (q >= words)
+        return;
+    unsigned long long v = 0;
+    for (uint32_t b = 0; b < WSTAT_SLOTS; b++)
+        v += ws[(uint64_t) b * WSTAT_STRIDE + q];
+    out[q] = v;
+}
+
 // Read geometry: either a fixed read length or per-read offsets + end bitmap.
 struct Geom {
     uint64_t n;              // reads

@@ -74,6 +74,8 @@ static __host__ __device__ inline uint32_t mlp_nparams(uint32_t k, uint32_t h)
 struct QStats {
     unsigned long long lookups[2], probes[2], win_sum[2], win_max[2], absent;
 };
+// one copy per WSTAT_SLOTS (blocks add to copy blockIdx % WSTAT_SLOTS)
+#define QSTATS_BYTES ((size_t) WSTAT_SLOTS * WSTAT_STRIDE * 8)
 
 struct DirModel {
     u128 *key = nullptr;          // keys of every stride-th SA row (a scratch slot of ctx)
@@ -914,7 +916,7 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
         for (int o = 32; o > 0; o >>= 1)
             vr += __shfl_xor(vr, o, 64);
         if ((threadIdx.x & 63) == 0 && vr)
-            atomicAdd(&stats[3], (unsigned long long) vr);
+            wstat_add(stats, WS_SEED_ROWS, vr);
     }
     // block reduction of the statistics, one set of atomics per block
     __shared__ unsigned long long red[4][9];
@@ -946,16 +948,23 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
                 cb.n[i] = 0;                  // empty at depth D: no records
         }
         if (cb.wq) {
+            // one atomic per block: the waves' counts, their prefix, one reservation
+            __shared__ uint32_t wcnt[4], wbase;
             const uint64_t bal = __ballot(listed);
-            if (bal) {
-                const int lane = threadIdx.x & 63, lead = __ffsll((long long) bal) - 1;
-                uint32_t base = 0;
-                if (lane == lead)
-                    base = atomicAdd(&cb.wq_cnt[0], (uint32_t) __popcll(bal));
-                base = __shfl(base, lead, 64);
-                if (listed)
-                    cb.wq[base + (uint32_t) __popcll(bal & ((1ull << lane) - 1))] = (uint32_t) i;
+            const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+            if (lane == 0)
+                wcnt[wv] = (uint32_t) __popcll(bal);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const uint32_t tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+                wbase = tot ? atomicAdd(&cb.wq_cnt[0], tot) : 0u;
             }
+            __syncthreads();
+            uint32_t base = wbase;
+            for (int w = 0; w < wv; w++)
+                base += wcnt[w];
+            if (listed)
+                cb.wq[base + (uint32_t) __popcll(bal & ((1ull << lane) - 1))] = (uint32_t) i;
         }
     }
     if (threadIdx.x < 9) {
@@ -964,11 +973,13 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
         unsigned long long a = red[0][q];
         for (int w = 1; w < 4; w++)
             a = mx ? (red[w][q] > a ? red[w][q] : a) : a + red[w][q];
-        unsigned long long *dst = q < 2   ? &qs->lookups[q]
-                                  : q < 4 ? &qs->probes[q - 2]
-                                  : q < 6 ? &qs->win_sum[q - 4]
-                                  : q < 8 ? &qs->win_max[q - 6]
-                                          : &qs->absent;
+        // this block's copy of the statistics (spread as the work counters)
+        QStats *qb = (QStats *) ((unsigned long long *) qs + (blockIdx.x % WSTAT_SLOTS) * WSTAT_STRIDE);
+        unsigned long long *dst = q < 2   ? &qb->lookups[q]
+                                  : q < 4 ? &qb->probes[q - 2]
+                                  : q < 6 ? &qb->win_sum[q - 4]
+                                  : q < 8 ? &qb->win_max[q - 6]
+                                          : &qb->absent;
         if (a) {
             if (mx)
                 atomicMax(dst, a);
@@ -978,12 +989,6 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
     }
 }
 
-__global__ void k_zero_u64(unsigned long long *p, uint32_t count)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count)
-        p[i] = 0;
-}
 
 // ---- host driver -------------------------------------------------------------------------
 
@@ -1330,10 +1335,10 @@ extern "C" int pq_learned_train(pq_ctx_t *ctx, const pq_learned_cfg_t *cfg_in)
     LM->stride = (uint32_t) cfg.sample_stride;
     ctx->learned = LM;
     ctx->learned_gen++;
-    int rc = pq_alloc(ctx, (void **) &LM->qstats, sizeof(QStats));
+    int rc = pq_alloc(ctx, (void **) &LM->qstats, QSTATS_BYTES);
     for (int d = 0; d < 2 && rc == PQ_OK; d++)
         rc = train_dir(ctx, *LM, d, cfg);
-    if (rc == PQ_OK && hipMemsetAsync(LM->qstats, 0, sizeof(QStats), ctx->stream) != hipSuccess)
+    if (rc == PQ_OK && hipMemsetAsync(LM->qstats, 0, QSTATS_BYTES, ctx->stream) != hipSuccess)
         rc = ctx->fail(PQ_ERR_HIP, "clearing learned statistics");
     if (rc == PQ_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
         rc = ctx->fail(PQ_ERR_HIP, "learned index training");
@@ -1353,9 +1358,24 @@ extern "C" int pq_learned_stats(pq_ctx_t *ctx, pq_learned_stats_t *st)
     Learned *LM = ctx->learned;
     if (!LM)
         return ctx->fail(PQ_ERR_STATE, "learned index not trained");
-    QStats q;
-    PQ_CHECK(hipMemcpyAsync(&q, LM->qstats, sizeof(q), hipMemcpyDeviceToHost, ctx->stream));
+    // the copies summed (maxima for the windows)
+    std::vector<unsigned long long> all(QSTATS_BYTES / 8);
+    PQ_CHECK(hipMemcpyAsync(all.data(), LM->qstats, QSTATS_BYTES, hipMemcpyDeviceToHost,
+                            ctx->stream));
     PQ_CHECK(hipStreamSynchronize(ctx->stream));
+    QStats q;
+    memset(&q, 0, sizeof(q));
+    for (uint32_t b = 0; b < WSTAT_SLOTS; b++) {
+        QStats c;
+        memcpy(&c, all.data() + (size_t) b * WSTAT_STRIDE, sizeof(c));
+        for (int d = 0; d < 2; d++) {
+            q.lookups[d] += c.lookups[d];
+            q.probes[d] += c.probes[d];
+            q.win_sum[d] += c.win_sum[d];
+            q.win_max[d] = std::max(q.win_max[d], c.win_max[d]);
+        }
+        q.absent += c.absent;
+    }
     memset(st, 0, sizeof(*st));
     for (int d = 0; d < 2; d++) {
         st->lookups[d] = q.lookups[d];
@@ -1395,7 +1415,7 @@ int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uin
     if (!LM)
         return ctx->fail(PQ_ERR_STATE, "learned index not trained");
     if (r0 == 0)   // a kernel, not a memset, so that the search can be captured
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream,
+        hipLaunchKernelGGL(k_wstat_clear, dim3(WSTAT_SLOTS), dim3(64), 0, ctx->stream,
                            (unsigned long long *) LM->qstats,
                            (uint32_t) (sizeof(QStats) / sizeof(unsigned long long)));
     if (ctx->wide)

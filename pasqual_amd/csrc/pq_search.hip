@@ -157,8 +157,8 @@ __global__ void __launch_bounds__(256) k_find1(Geom G, DirViewT<R> fw, DirViewT<
             ln += __shfl_xor(ln, d, 64);
         }
         if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&stats[0], (unsigned long long) (2 * s2));
-            atomicAdd(&stats[1], (unsigned long long) ln);
+            wstat_add(stats, WS_POSITIONS, 2 * s2);
+            wstat_add(stats, WS_LINES, ln);
         }
     }
 }
@@ -308,8 +308,8 @@ __global__ void __launch_bounds__(256) k_find1q(Geom G, DirViewT<R> fw, DirViewT
             ln += __shfl_xor(ln, d, 64);
         }
         if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&stats[0], (unsigned long long) (2 * s2));
-            atomicAdd(&stats[1], (unsigned long long) ln);
+            wstat_add(stats, WS_POSITIONS, 2 * s2);
+            wstat_add(stats, WS_LINES, ln);
         }
     }
 }
@@ -429,8 +429,8 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
             ln += __shfl_xor(ln, d, 64);
         }
         if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&stats[0], (unsigned long long) (2 * s2));
-            atomicAdd(&stats[1], (unsigned long long) ln);
+            wstat_add(stats, WS_POSITIONS, 2 * s2);
+            wstat_add(stats, WS_LINES, ln);
         }
     }
 }
@@ -444,11 +444,13 @@ __global__ void __launch_bounds__(256) k_find2(Geom G, DirViewT<R> fw, DirViewT<
 // grid that fills the GPU once.
 template <typename R>
 __global__ void __launch_bounds__(256) k_find_list(Geom G, DirViewT<R> fw, DirViewT<R> bw,
-                                                   uint64_t r0, uint32_t t, CTab<R> C,
-                                                   const SeedT<R> *seeds, uint32_t kseed,
-                                                   ChainBuf out, unsigned long long *stats)
+                                                   uint64_t r0, uint32_t nchains, uint32_t t,
+                                                   CTab<R> C, const SeedT<R> *seeds,
+                                                   uint32_t kseed, ChainBuf out,
+                                                   unsigned long long *stats)
 {
-    const uint32_t total = out.wq_cnt[0];
+    // without a list (the plain search) every chain of the batch is listed in order
+    const uint32_t total = out.wq ? out.wq_cnt[0] : nchains;
     const int lane = threadIdx.x & 63;
     uint32_t steps = 0, lines = 0;
     bool live = false, more = true;
@@ -468,14 +470,14 @@ __global__ void __launch_bounds__(256) k_find_list(Geom G, DirViewT<R> fw, DirVi
             if (want) {
                 const uint32_t idx = base + (uint32_t) __popcll(wm & ((1ull << lane) - 1));
                 if (idx < total) {
-                    chain = out.wq[idx];
+                    chain = out.wq ? out.wq[idx] : idx;
                     const uint64_t r = r0 + (chain >> 2);
                     ch = (int) (chain & 3);
                     s = pq_read_start(G, r);
                     L = pq_read_len(G, r);
                     minsize = (ch == 0 || ch == 2) ? 2u : 1u;
                     SeedT<R> sd = {0, SEED_NONE};
-                    if (kseed > 0 && kseed < L)
+                    if (seeds && kseed > 0 && kseed < L)
                         sd = seeds[chain];
                     if (sd.size != SEED_NONE) {
                         o = kseed;
@@ -489,6 +491,8 @@ __global__ void __launch_bounds__(256) k_find_list(Geom G, DirViewT<R> fw, DirVi
                         size = (uint32_t) (pq_pick6(C.c, c + 1) - l);
                     }
                     nout = 0;
+                    if (!out.wq)
+                        out.vjob[chain] = 0;
                     live = o < L && size >= minsize && !hand_over(out, chain, o, t, l, size);
                     if (!live)
                         out.n[chain] = 0;
@@ -546,8 +550,8 @@ __global__ void __launch_bounds__(256) k_find_list(Geom G, DirViewT<R> fw, DirVi
             ln += __shfl_xor(ln, d, 64);
         }
         if (lane == 0) {
-            atomicAdd(&stats[0], (unsigned long long) (2 * s2));
-            atomicAdd(&stats[1], (unsigned long long) ln);
+            wstat_add(stats, WS_POSITIONS, 2 * s2);
+            wstat_add(stats, WS_LINES, ln);
         }
     }
 }
@@ -584,7 +588,7 @@ __global__ void __launch_bounds__(256) k_verify(Geom G, DirViewT<R> fw, DirViewT
         for (int d = 32; d > 0; d >>= 1)
             v += __shfl_xor(v, d, 64);
         if ((threadIdx.x & 63) == 0 && v)
-            atomicAdd(&stats[2], (unsigned long long) v);
+            wstat_add(stats, WS_VERIFY_ROWS, v);
     }
 }
 
@@ -647,7 +651,25 @@ struct ExtOut {
     uint32_t *overflow_reads;     // reads left for the global-memory pass
     uint32_t *n_overflow;
     uint32_t *flags;              // bit0: edge buffer full, bit1: item limit
+    unsigned long long *xstats;   // work counters (WS_ITEMS, WS_TIER_READS + tier)
+    uint32_t tier;                // 0..2: LDS tiers in launch order, 3: global scratch
 };
+
+// Items and reads the groups of a wave finished, added to the work counters.
+template <int GW>
+__device__ __forceinline__ void ext_count(const ExtOut &out, uint64_t items, uint64_t reads)
+{
+    if ((threadIdx.x & (GW - 1)) != 0)
+        items = reads = 0;
+    for (int d = 32; d > 0; d >>= 1) {
+        items += __shfl_xor(items, d, 64);
+        reads += __shfl_xor(reads, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wstat_add(out.xstats, WS_ITEMS, items);
+        wstat_add(out.xstats, WS_TIER_READS + out.tier, reads);
+    }
+}
 
 
 // Lanes of this group of GW lanes set in a wave-wide ballot, at the group's
@@ -907,10 +929,11 @@ template <int GW, typename R>
 __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirViewT<R> &bw,
                              const ChainBuf &cb, uint64_t rl, uint64_t r, uint32_t kw,
                              uint64_t *keys, ItemMeta *meta, uint8_t *emitf, uint32_t *pre,
-                             uint16_t *lcpv, uint32_t kmax, const ExtOut &out)
+                             uint16_t *lcpv, uint32_t kmax, const ExtOut &out, uint32_t &items)
 {
     const int lane = threadIdx.x & (GW - 1);
     uint32_t cnt4[4];
+    items = 0;
     for (int c = 0; c < 4; c++)
         cnt4[c] = cb.n[rl * 4 + c];
     // items per side first, so that an oversized read is left untouched
@@ -952,6 +975,7 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
             carry += __shfl(inc, GW - 1, GW);
         }
         const uint32_t k = carry;
+        items += k;
         wave_sync();
         // expand the candidate intervals into items with their extension keys:
         // two items per lane at a time, so that each lane has two independent
@@ -1056,16 +1080,33 @@ __global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirViewT<R> 
     const int grp = threadIdx.x / GW;
     const uint64_t total = list ? (uint64_t) *count : nr;
     ExtWave w = ext_wave((char *) lds + ext_wave_bytes(KMAX, kw) * grp, KMAX, kw);
+    uint64_t nitems = 0, nreads = 0;
     for (uint64_t i = (uint64_t) blockIdx.x * GROUPS + grp; i < total;
          i += (uint64_t) gridDim.x * GROUPS) {
         uint64_t rl = list ? list[i] : i;
+        uint32_t items;
         bool ok = extract_read<GW, R>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
-                                      w.pre, w.lcp, KMAX, out);
-        if (!ok && (threadIdx.x & (GW - 1)) == 0) {
-            uint32_t at = atomicAdd(out.n_overflow, 1u);
-            out.overflow_reads[at] = (uint32_t) rl;
+                                      w.pre, w.lcp, KMAX, out, items);
+        if (ok) {
+            nitems += items;
+            nreads++;
+        }
+        // reads left for the next tier, appended with one atomic per wave
+        const bool ov = !ok && (threadIdx.x & (GW - 1)) == 0;
+        const uint64_t bal = __ballot(ov);
+        if (bal) {
+            const int lane = threadIdx.x & 63, lead = __ffsll((long long) bal) - 1;
+            uint32_t base = 0;
+            if (lane == lead)
+                base = atomicAdd(out.n_overflow, (uint32_t) __popcll(bal));
+            base = __shfl(base, lead, 64);
+            if (ov)
+                out.overflow_reads[base + (uint32_t) __popcll(bal & ((1ull << lane) - 1))] =
+                    (uint32_t) rl;
         }
     }
+    if (out.xstats)
+        ext_count<GW>(out, nitems, nreads);
 }
 
 // ---- read sides beyond the global tier ------------------------------------------------
@@ -1114,12 +1155,17 @@ __global__ void __launch_bounds__(64) k_extract_global(Geom G, DirViewT<R> fw, D
 {
     ExtWave w = ext_wave(gscratch + ext_wave_bytes(KMAX_GLOBAL, kw) * blockIdx.x, KMAX_GLOBAL, kw);
     const uint32_t nlist = *count;
+    uint64_t nitems = 0, nreads = 0;
     for (uint32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
         uint64_t rl = list[i];
+        uint32_t items;
         bool ok = extract_read<64, R>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
-                                      w.pre, w.lcp, KMAX_GLOBAL, out);
-        if (ok)
+                                      w.pre, w.lcp, KMAX_GLOBAL, out, items);
+        if (ok) {
+            nitems += items;
+            nreads++;
             continue;
+        }
         const uint64_t kB = side_items<64>(cb, rl, 0), kF = side_items<64>(cb, rl, 1);
         if (threadIdx.x == 0) {
             if (huge_is_error) {
@@ -1135,6 +1181,8 @@ __global__ void __launch_bounds__(64) k_extract_global(Geom G, DirViewT<R> fw, D
             }
         }
     }
+    if (out.xstats)
+        ext_count<64>(out, nitems, nreads);
 }
 
 // Exclusive prefix of the item counts over the entries of one side; pre[ne] =
@@ -1332,6 +1380,20 @@ void pq_search_graph_free(pq_ctx_t *ctx)
     ctx->search_key.clear();
 }
 
+// The search's work counters from its misc words: WSTAT_WORDS 64-bit sums (low
+// word first) at misc[8..39], in WS_* order.
+static void search_counters(pq_ctx_t *ctx, const uint32_t *hm)
+{
+    auto u64 = [&](int q) { return (uint64_t) hm[8 + 2 * q] | ((uint64_t) hm[9 + 2 * q] << 32); };
+    ctx->timing.find_positions = u64(WS_POSITIONS);
+    ctx->timing.find_lines = u64(WS_LINES);
+    ctx->timing.verify_rows = u64(WS_VERIFY_ROWS);
+    ctx->timing.seed_rows = u64(WS_SEED_ROWS);
+    ctx->timing.extract_items = u64(WS_ITEMS);
+    for (int k = 0; k < 4; k++)
+        ctx->timing.extract_reads[k] = u64(WS_TIER_READS + k);
+}
+
 // Graph mode of pq_find_overlaps: the search sequence plus the edge CSR (into an
 // edge array sized for the worst case) as one graph; recaptured when any launch
 // argument it depends on changes.
@@ -1390,15 +1452,12 @@ static int run_search_graph(pq_ctx_t *ctx, int mode, uint64_t n, uint64_t batch,
         ctx->search_key = key;
     }
     PQ_CHECK(hipGraphLaunch(ctx->search_exec, ctx->stream));
-    uint32_t hm[16];
+    uint32_t hm[40];
     uint64_t total = 0;
-    PQ_CHECK(hipMemcpyAsync(hm, misc, 64, hipMemcpyDeviceToHost, ctx->stream));
+    PQ_CHECK(hipMemcpyAsync(hm, misc, sizeof(hm), hipMemcpyDeviceToHost, ctx->stream));
     PQ_CHECK(hipMemcpyAsync(&total, ctx->d_edge_off + n, 8, hipMemcpyDeviceToHost, ctx->stream));
     PQ_CHECK(hipStreamSynchronize(ctx->stream));
-    ctx->timing.find_positions = (uint64_t) hm[8] | ((uint64_t) hm[9] << 32);
-    ctx->timing.find_lines = (uint64_t) hm[10] | ((uint64_t) hm[11] << 32);
-    ctx->timing.verify_rows = (uint64_t) hm[12] | ((uint64_t) hm[13] << 32);
-    ctx->timing.seed_rows = (uint64_t) hm[14] | ((uint64_t) hm[15] << 32);
+    search_counters(ctx, hm);
     if (hm[3] & 2u)
         return PQ_RETRY_EAGER;
     if (hm[3] & 1u) {
@@ -1420,7 +1479,9 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     ctx->timing.find_kernel_ms = 0;
     ctx->timing.find_launches = 0;
     ctx->timing.find_positions = ctx->timing.find_lines = ctx->timing.verify_rows = 0;
-    ctx->timing.seed_rows = 0;
+    ctx->timing.seed_rows = ctx->timing.extract_items = 0;
+    for (int k = 0; k < 4; k++)
+        ctx->timing.extract_reads[k] = 0;
     ctx->n_edges = 0;
     int rc;
     if ((rc = pq_slot(ctx, SLOT_OVB, (n + 1) * 4, &ctx->d_ovB)) ||
@@ -1459,6 +1520,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(n, budget / per_read));
     uint64_t *rec = nullptr, *recF = nullptr, *app = nullptr, *tot = nullptr, *staged = nullptr;
     uint32_t *ovl = nullptr, *misc = nullptr;
+    unsigned long long *wstats = nullptr;
     // shared edge pool: the set or default capacity, or what an overflowing
     // search of these reads asked for
     const uint64_t edge_cap = std::max<uint64_t>(
@@ -1492,13 +1554,15 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         (rc = pq_slot(ctx, SLOT_STAGED, (n + 1) * 2 * EDGE_SLOT * 8, &staged)) ||
         (rc = pq_slot(ctx, SLOT_TOT, (n + 1) * 8, &tot)) ||
         (rc = pq_slot(ctx, SLOT_OVL, (3 * batch + 3) * 4, &ovl)) ||
-        (rc = pq_slot(ctx, SLOT_MISC, 64, &misc)) ||
+        (rc = pq_slot(ctx, SLOT_MISC, 160, &misc)) ||
+        (rc = pq_slot(ctx, SLOT_WSTATS, (size_t) WSTAT_SLOTS * WSTAT_STRIDE * 8, &wstats)) ||
         (rc = pq_slot(ctx, SLOT_GSCRATCH, GLOBAL_SLOTS * ext_wave_bytes(KMAX_GLOBAL, kw),
                       &gscratch)) ||
         (rc = pq_slot(ctx, SLOT_HUGE_LIST, batch * sizeof(HugeRead), &huge)) ||
+        (rc = pq_slot(ctx, SLOT_WORKQ,
+                      ((mode == PQ_SEARCH_LEARNED ? batch * 4 : 0) + 64) * 4, &cb.wq)) ||
         (mode == PQ_SEARCH_LEARNED &&
-         ((rc = pq_slot(ctx, SLOT_SEEDS, batch * 4 * sizeof(SeedT<R>), &seeds)) ||
-          (rc = pq_slot(ctx, SLOT_WORKQ, (batch * 4 + 64) * 4, &cb.wq))))) {
+         (rc = pq_slot(ctx, SLOT_SEEDS, batch * 4 * sizeof(SeedT<R>), &seeds)))) {
         cleanup();
         return rc;
     }
@@ -1514,10 +1578,14 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         per_cu <= 0)
         per_cu = 4;
     const uint64_t list_grid = (uint64_t) cus * per_cu;
-    if (mode == PQ_SEARCH_LEARNED)
+    // the work list of the seeded search (the plain search's list kernel walks every
+    // chain in order and only uses the counters)
+    if (mode == PQ_SEARCH_LEARNED) {
         cb.wq_cnt = cb.wq + batch * 4;
-    else
-        cb.wq = cb.wq_cnt = nullptr;
+    } else {
+        cb.wq_cnt = cb.wq;
+        cb.wq = nullptr;
+    }
     ExtOut eo;
     eo.ovB = ctx->d_ovB;
     eo.ovF = ctx->d_ovF;
@@ -1532,7 +1600,10 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     eo.n_overflow = misc + 2;
     eo.flags = misc + 3;
     eo.overflow_reads = ovl;
-    unsigned long long *fstats = (unsigned long long *) (misc + 8);   // misc[8..11]
+    // work counters: spread copies, summed into misc[8..39] at the end
+    unsigned long long *fstats = wstats;
+    eo.xstats = wstats;
+    eo.tier = 0;
     // the whole search as a sequence of stream operations (captured into a graph
     // in graph mode: kernels only, no memsets, no host synchronisation)
     // search kernel: the autotuner's or the caller's choice, else by the size of
@@ -1541,13 +1612,15 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     // lane up to a few GiB, quad-loaded lines beyond, where scattered loads of
     // whole lines by single lanes lose half their rate)
     const uint64_t occ_bytes = (ctx->dir[0].nlines + ctx->dir[1].nlines) * sizeof(OccLine);
-    const uint32_t chains = ctx->find_chains >= 1 && ctx->find_chains <= 3 ? ctx->find_chains
+    const uint32_t chains = ctx->find_chains >= 1 && ctx->find_chains <= 4 ? ctx->find_chains
                             : occ_bytes < (1ull << 30)                     ? 2u
                             : occ_bytes < (4ull << 30)                     ? 1u
                                                                            : 3u;
     auto enqueue = [&](bool timed) -> int {
     int rc = PQ_OK;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc, 16u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc, 40u);
+    hipLaunchKernelGGL(k_wstat_clear, dim3(WSTAT_SLOTS), dim3(64), 0, ctx->stream, wstats,
+                       (uint32_t) WSTAT_WORDS);
     const uint32_t shape = ctx->ext_shape < EXT_NSHAPES ? ctx->ext_shape : 1;
     const uint32_t gw1 = EXT_SHAPES[shape][0], kmax1 = EXT_SHAPES[shape][1];
     const uint32_t groups1 = EXT_WAVES * 64 / gw1;
@@ -1560,8 +1633,12 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     ExtOut eoB = eo, eoC = eo;
     eoB.overflow_reads = ovl + batch;
     eoB.n_overflow = misc + 4;
+    eoB.tier = 1;
     eoC.overflow_reads = ovl + 2 * batch;
     eoC.n_overflow = misc + 6;
+    eoC.tier = 2;
+    ExtOut eoG = eo;
+    eoG.tier = 3;
     const size_t ldsB = ext_wave_bytes(64, kw) * (EXT_WAVES * 64 / 32);
     const uint32_t gridB = (uint32_t) std::min<uint64_t>(pq_blocks(batch, EXT_WAVES * 2), 4096);
     // per batch: seeds, candidates, then the extraction tiers; the later tiers
@@ -1573,19 +1650,20 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         uint32_t kseed = 0;
         if (timed)
             (void) hipEventRecord(e[0], ctx->stream);
-        if (mode == PQ_SEARCH_LEARNED) {
+        const bool listed = mode == PQ_SEARCH_LEARNED || chains == 4;
+        if (listed)
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, cb.wq_cnt, 2u);
-            if ((rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed, cb, fstats)))
-                break;
-        }
+        if (mode == PQ_SEARCH_LEARNED &&
+            (rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed, cb, fstats)))
+            break;
         if (timed)
             (void) hipEventRecord(e[1], ctx->stream);
-        if (mode == PQ_SEARCH_LEARNED)
+        if (listed)
             hipLaunchKernelGGL(k_find_list<R>,
                                dim3((uint32_t) std::min<uint64_t>(pq_blocks(nr * 4, 256),
                                                                   list_grid)),
-                               dim3(256), 0, ctx->stream, G, fw, bw, r0, t, Ct, seeds, kseed, cb,
-                               fstats);
+                               dim3(256), 0, ctx->stream, G, fw, bw, r0, (uint32_t) (nr * 4), t,
+                               Ct, seeds, kseed, cb, fstats);
         else if (chains == 1)
             hipLaunchKernelGGL(k_find1<R>, dim3(pq_blocks(nr * 4, 256)), dim3(256), 0,
                                ctx->stream, G, fw, bw, r0, nr, t, Ct, seeds, kseed, cb, fstats);
@@ -1629,7 +1707,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         // GPU right here, before the next batch reuses the candidate buffers; a
         // graph capture cannot wait for the count, so there they raise flag 2
         hipLaunchKernelGGL(k_extract_global<R>, dim3(GLOBAL_SLOTS), dim3(64), 0, ctx->stream, G, fw,
-                           bw, cb, r0, eoC.overflow_reads, eoC.n_overflow, kw, gscratch, eo, huge,
+                           bw, cb, r0, eoC.overflow_reads, eoC.n_overflow, kw, gscratch, eoG, huge,
                            misc + 7, timed ? 0u : 1u);
         PQ_CHECK(hipGetLastError());
         if (timed) {
@@ -1655,6 +1733,9 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         if (timed)
             (void) hipEventRecord(e[3], ctx->stream);
     }
+    // the work counters' copies summed into misc[8..39]
+    hipLaunchKernelGGL(k_wstat_sum, dim3(1), dim3(64), 0, ctx->stream, wstats,
+                       (uint32_t) WSTAT_WORDS, (unsigned long long *) (misc + 8));
     return rc;
     };
     if (as_graph) {
@@ -1701,9 +1782,9 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
             rc = ctx->fail(PQ_ERR_HIP, "edge compaction launch");
     }
     if (rc == PQ_OK) {
-        uint32_t hm[16];
+        uint32_t hm[40];
         uint64_t total = 0;
-        if (hipMemcpyAsync(hm, misc, 64, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        if (hipMemcpyAsync(hm, misc, sizeof(hm), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
             hipMemcpyAsync(&total, ctx->d_edge_off + n, 8, hipMemcpyDeviceToHost, ctx->stream) !=
                 hipSuccess ||
             hipStreamSynchronize(ctx->stream) != hipSuccess)
@@ -1720,10 +1801,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
                 ctx->timing.extract_ms += ev_ms(e[2], e[3]);
             }
             ctx->timing.compact_ms = ev_ms(ce[0], ce[1]);
-            ctx->timing.find_positions = (uint64_t) hm[8] | ((uint64_t) hm[9] << 32);
-            ctx->timing.find_lines = (uint64_t) hm[10] | ((uint64_t) hm[11] << 32);
-            ctx->timing.verify_rows = (uint64_t) hm[12] | ((uint64_t) hm[13] << 32);
-    ctx->timing.seed_rows = (uint64_t) hm[14] | ((uint64_t) hm[15] << 32);
+            search_counters(ctx, hm);
             if (hm[3] & 2u) {
                 rc = ctx->fail(PQ_ERR_STATE, "oversized read side left unprocessed");
             } else if (hm[3] & 1u) {
@@ -1864,11 +1942,11 @@ extern "C" int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out)
     static const uint32_t VROWS[3] = {0, 8, 32};
     double best_find = 1e300;
     uint32_t fk = 1, vk = 2;
-    for (uint32_t k = 0; k < 3; k++)
+    for (uint32_t k = 0; k < 4; k++)
         t.find_ms[k] = 1e300;
     for (uint32_t v = 0; v < 3; v++)
         t.verify_ms[v] = 1e300;
-    const uint32_t klast = mode == PQ_SEARCH_LEARNED ? 1 : 3;
+    const uint32_t klast = mode == PQ_SEARCH_LEARNED ? 1 : 4;
     for (uint32_t k = 1; k <= klast; k++) {
         for (uint32_t v = 0; v < 3; v++) {
             ctx->find_chains = k;
@@ -1914,8 +1992,8 @@ extern "C" int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains)
 {
     if (!ctx)
         return PQ_ERR_ARG;
-    if (chains > 3)
-        return ctx->fail(PQ_ERR_ARG, "search kernel must be 0 (by index size), 1, 2 or 3");
+    if (chains > 4)
+        return ctx->fail(PQ_ERR_ARG, "search kernel must be 0 (by index size), 1, 2, 3 or 4");
     ctx->find_chains = chains;
     return PQ_OK;
 }

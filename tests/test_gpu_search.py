@@ -157,7 +157,7 @@ def test_edge_buffer_exhaustion_recovers(ctx):
 def _check_every_shape_and_kernel(ctx, want, where):
     for shape in range(3):
         ctx.set_tuning(shape)
-        for kernel in (1, 2, 3):
+        for kernel in (1, 2, 3, 4):
             ctx.set_find_chains(kernel)
             for mode in (pq.SEARCH_PLAIN, pq.SEARCH_PLAIN | pq.SEARCH_GRAPH):
                 ctx.find_overlaps(mode)
@@ -212,7 +212,7 @@ def test_read_side_beyond_global_tier(ctx):
     ctx.build_index()
     # the side overflows every tier whatever the first tier's shape: the search
     # kernels eagerly, and graph mode once
-    for kernel in (1, 2, 3):
+    for kernel in (1, 2, 3, 4):
         ctx.set_find_chains(kernel)
         ctx.find_overlaps()
         _same_graph(ctx.graph(), want, f"oversized side, kernel {kernel}")
@@ -249,13 +249,13 @@ def test_every_extraction_shape_gives_the_same_graph(ctx):
         ctx.set_tuning(shape)
         ctx.find_overlaps()
         _same_graph(ctx.graph(), want, f"shape {shape}")
-    for kernel in (0, 1, 2, 3):
+    for kernel in (0, 1, 2, 3, 4):
         ctx.set_find_chains(kernel)
         ctx.find_overlaps()
         _same_graph(ctx.graph(), want, f"search kernel {kernel}")
     t = ctx.autotune()
     assert 0 <= t["shape"] < 3 and len(t["extract_ms"]) == 3
-    assert t["find_chains"] in (1, 2, 3) and min(t["find_ms"]) > 0
+    assert t["find_chains"] in (1, 2, 3, 4) and min(t["find_ms"]) > 0
     _same_graph(ctx.graph(), want, "autotuned")
     ctx.set_tuning(1)
     ctx.set_find_chains(0)
@@ -281,7 +281,7 @@ def test_search_work_counts_match_oracle(ctx, lmin, lmax, t, seed):
         want = orc.chain_work(codes, off, 0, t, v)
         want_l = orc.chain_work(codes, off, D, t, v, stride)
         assert want_l[0] + want_l[2] < want[0] + want[2]
-        for kernel in (1, 2, 3):
+        for kernel in (1, 2, 3, 4):
             ctx.set_find_chains(kernel)
             for mode, w in ((pq.SEARCH_PLAIN, want), (pq.SEARCH_LEARNED, want_l)):
                 ctx.find_overlaps(mode)
@@ -308,7 +308,7 @@ def test_verification_gives_the_search_graph(ctx, vmax):
             want = orc.search_explicit(codes, off, t)
             ctx.upload(codes, off, t)
             ctx.build_index()
-            for kernel in (1, 2, 3):
+            for kernel in (1, 2, 3, 4):
                 ctx.set_find_chains(kernel)
                 ctx.find_overlaps()
                 _same_graph(ctx.graph(), want, f"vmax {vmax} kernel {kernel}")

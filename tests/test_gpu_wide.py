@@ -84,7 +84,7 @@ def test_layout_search_and_learned_match_oracle(wctx, row_bytes, slice_, lmin, l
         _same_graph(wctx.graph(), want, f"mode {mode:#x}")
     assert wctx.learned_stats()["fallback"] == 0
     # every candidate-search kernel, plain and seeded, on these rows
-    for kernel in (1, 2, 3):
+    for kernel in (1, 2, 3, 4):
         wctx.set_find_chains(kernel)
         for mode in (pq.SEARCH_PLAIN, pq.SEARCH_LEARNED):
             wctx.find_overlaps(mode)
