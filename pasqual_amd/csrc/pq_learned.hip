@@ -811,13 +811,57 @@ __device__ ScanOut scan_chain(const Geom &G, const R *sa, uint64_t N, const uint
     return {lo, hi, k, visited};
 }
 
+// The model's prediction for every chain of a batch that takes a seed: the
+// sampled row its depth-D pattern should start at (the network on the matrix
+// cores, then the leaf's piecewise-linear piece), written to pred[ch * nr + rl].
+// Kept apart from the seed kernel, whose long chain of dependent reads then runs
+// without the network's registers and LDS.  Same block shape as the seed kernel:
+// wave q handles chain q of 64 reads, so the model of a wave is uniform.
+template <typename R>
+__global__ void __launch_bounds__(256) k_learned_predict(Geom G, const uint64_t *fwt, uint64_t r0,
+                                                         uint64_t nr, uint32_t D, MlpDims d,
+                                                         uint32_t leaves, uint32_t S,
+                                                         ModelViewT<R> fw, ModelViewT<R> bw,
+                                                         uint64_t *pred)
+{
+    const int ch = (int) (threadIdx.x >> 6);
+    const uint64_t rl = (uint64_t) blockIdx.x * 64 + (threadIdx.x & 63);
+    const bool fwsearch = (ch == 0 || ch == 3);
+    // both directions' inference parameters in LDS, read as wave-wide broadcasts
+    __shared__ float wsh[2][MLP_P_MAX];
+    for (uint32_t i = threadIdx.x; i < MLP_P_MAX; i += blockDim.x) {
+        wsh[0][i] = fw.iparams[i];
+        wsh[1][i] = bw.iparams[i];
+    }
+    __syncthreads();
+    const ModelViewT<R> &Sv = fwsearch ? fw : bw;
+    bool act = false;
+    u128 pat = 0;
+    if (rl < nr) {
+        const uint64_t r = r0 + rl;
+        const uint32_t L = pq_read_len(G, r);
+        act = D < L && Sv.nU > 0;
+        if (act)
+            pat = chain_pattern(fwt, pq_read_start(G, r), L, ch, D);
+    }
+    // every lane's prefix (zero where the chain takes no seed), the network for the
+    // whole wave at once
+    const uint64_t xk = (uint64_t) (pat >> (128 - 2 * d.k));
+    const float y = mlp_forward_mfma(&wsh[fwsearch ? 0 : 1][0], d, xk);
+    if (act)
+        pred[(uint64_t) ch * nr + rl] = leaf_entry(y, Sv, leaves, S, xk);
+}
+
 // Block of 256 threads = 64 reads; wave q handles chain q of those reads, so the
 // model of a wave is uniform.  Writes Seed{search interval} at depth D for
 // chains of reads longer than D (size 0 when the pattern does not occur).
 template <typename R>
-__global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fwt, uint64_t r0,
-                                                      uint64_t nr, uint32_t D, MlpDims d,
-                                                      uint32_t leaves, uint32_t S,
+// (launch bounds: 8 waves per SIMD asked for; the compiler settles at 96 vector
+// registers, 5 waves per SIMD instead of 4, without spilling vector registers --
+// more lanes in flight for this latency-bound kernel)
+__global__ void __launch_bounds__(256, 8) k_learned_seed(Geom G, const uint64_t *fwt, uint64_t r0,
+                                                      uint64_t nr, uint32_t D,
+                                                      const uint64_t *pred,
                                                       ModelViewT<R> fw, ModelViewT<R> bw,
                                                       SeedT<R> *seeds, QStats *qs, ChainBuf cb,
                                                       uint32_t t, unsigned long long *stats)
@@ -830,32 +874,24 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
     uint64_t win[2] = {0, 0};
     uint32_t absent = 0, verified = 0;
     uint32_t nrec = ~0u;   // records of a chain finished here
-    // both directions' inference parameters in LDS, read as wave-wide broadcasts
-    __shared__ float wsh[2][MLP_P_MAX];
     // candidate staging; the block's seeds and record counts, written out in the
     // search kernels' order (read-major: 4 chains per read) in one coalesced pass
     __shared__ uint64_t stg[STAGE_SLOTS * STAGE_STRIDE];
     __shared__ SeedT<R> sds[256];
     __shared__ uint32_t nsh[256];
-    for (uint32_t i = threadIdx.x; i < MLP_P_MAX; i += blockDim.x) {
-        wsh[0][i] = fw.iparams[i];
-        wsh[1][i] = bw.iparams[i];
-    }
-    __syncthreads();
-    // every lane's key (zero where the chain takes no seed), then the network for
-    // the whole wave on the matrix cores
     const ModelViewT<R> &Sv = fwsearch ? fw : bw;
     bool act = false;
     u128 pat = 0;
+    uint64_t g1 = 0;
     if (rl < nr) {
         const uint64_t r = r0 + rl;
         const uint32_t L = pq_read_len(G, r);
         act = D < L && Sv.nU > 0;
-        if (act)
+        if (act) {
             pat = chain_pattern(fwt, pq_read_start(G, r), L, ch, D);
+            g1 = pred[(uint64_t) ch * nr + rl];   // the model's sampled row (k_learned_predict)
+        }
     }
-    const uint64_t xk = (uint64_t) (pat >> (128 - 2 * d.k));
-    const float y = mlp_forward_mfma(&wsh[sdir][0], d, xk);
     if (rl < nr) {
         SeedT<R> sd = {0, SEED_NONE};
         if (act) {
@@ -864,7 +900,6 @@ __global__ void __launch_bounds__(256) k_learned_seed(Geom G, const uint64_t *fw
             // sampled keys bracket both ends to a stride of rows
             const u128 key = pat | 1u, above = key + 1u;   // bit 1 of a key is never set
             const uint64_t st = Sv.stride, ns = Sv.nU;
-            const uint64_t g1 = leaf_entry(y, Sv, leaves, S, xk);
             const uint64_t jl = lower_bound_from(Sv.key, ns, key, g1, probes[sdir]);
             looks[sdir]++;
             win[sdir] = jl > g1 ? jl - g1 : g1 - jl;
@@ -1403,9 +1438,14 @@ static void launch_seeds(pq_ctx_t *ctx, const Learned *LM, uint64_t r0, uint64_t
 {
     ModelViewT<R> v[2] = {model_view<R>(ctx, *LM, 0), model_view<R>(ctx, *LM, 1)};
     MlpDims dims = {LM->k, LM->h};
-    hipLaunchKernelGGL(k_learned_seed<R>, dim3(pq_blocks(nr, 64)), dim3(256), 0, ctx->stream,
+    // the predictions go to the chains' verification-start words (cb.vl, 8 bytes
+    // per chain), which the search kernels only write after the seed kernel
+    hipLaunchKernelGGL(k_learned_predict<R>, dim3(pq_blocks(nr, 64)), dim3(256), 0, ctx->stream,
                        ctx->geom(), ctx->dir[0].text2, r0, nr, LM->depth, dims, LM->leaves,
-                       LM->segs, v[0], v[1], d_seeds, LM->qstats, cb, ctx->t, stats);
+                       LM->segs, v[0], v[1], cb.vl);
+    hipLaunchKernelGGL(k_learned_seed<R>, dim3(pq_blocks(nr, 64)), dim3(256), 0, ctx->stream,
+                       ctx->geom(), ctx->dir[0].text2, r0, nr, LM->depth, cb.vl, v[0], v[1],
+                       d_seeds, LM->qstats, cb, ctx->t, stats);
 }
 
 int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed,
