@@ -300,7 +300,8 @@ typedef struct {
     double find_ms;          /* candidate intervals (bidirectional search) */
     double extract_ms;       /* irreducible-edge extraction */
     double compact_ms;       /* edge CSR compaction */
-    double learned_ms;       /* learned seeding (learned mode only) */
+    double learned_ms;       /* learned seeding (learned mode only): prediction + seeds */
+    double seed_kernel_ms;   /* the seed kernel alone (k_learned_seed) */
     uint64_t find_launches;
     double find_kernel_ms;   /* summed duration of the candidate-search kernel */
     uint64_t find_positions; /* rank positions evaluated by it (2 per step) */

@@ -103,7 +103,8 @@ class Tune(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("upload_ms", C.c_double), ("sa_ms", C.c_double * 2), ("occ_ms", C.c_double * 2),
                 ("find_ms", C.c_double), ("extract_ms", C.c_double), ("compact_ms", C.c_double),
-                ("learned_ms", C.c_double), ("find_launches", C.c_uint64),
+                ("learned_ms", C.c_double), ("seed_kernel_ms", C.c_double),
+                ("find_launches", C.c_uint64),
                 ("find_kernel_ms", C.c_double), ("find_positions", C.c_uint64),
                 ("find_lines", C.c_uint64), ("verify_rows", C.c_uint64),
                 ("seed_rows", C.c_uint64), ("extract_items", C.c_uint64),
@@ -113,7 +114,8 @@ class Timing(C.Structure):
         return {"upload_ms": self.upload_ms, "sa_ms": list(self.sa_ms),
                 "occ_ms": list(self.occ_ms), "find_ms": self.find_ms,
                 "extract_ms": self.extract_ms, "compact_ms": self.compact_ms,
-                "learned_ms": self.learned_ms, "find_launches": self.find_launches,
+                "learned_ms": self.learned_ms, "seed_kernel_ms": self.seed_kernel_ms,
+                "find_launches": self.find_launches,
                 "find_kernel_ms": self.find_kernel_ms, "find_positions": self.find_positions,
                 "find_lines": self.find_lines, "verify_rows": self.verify_rows,
                 "seed_rows": self.seed_rows, "extract_items": self.extract_items,

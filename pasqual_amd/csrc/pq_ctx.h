@@ -181,10 +181,11 @@ template <typename R> struct SeedT {
 #define SEED_DONE 0xFFFFFFFEu
 struct ChainBuf;
 // d_seeds: SeedT<uint32_t> or SeedT<uint64_t> per (read, chain) as ctx->wide says;
-// chains whose interval holds at most cb.vmax rows are finished into cb, their
-// rows counted in stats[3]
+// chains whose interval holds at most cb.vmax rows are finished into cb, the rows
+// read counted as WS_SEED_ROWS; `mid` (optional) is recorded between the model's
+// prediction and the seed kernel
 int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed,
-                     const ChainBuf &cb, unsigned long long *stats);
+                     const ChainBuf &cb, unsigned long long *stats, hipEvent_t mid);
 void pq_release_build_scratch(pq_ctx_t *ctx);
 
 static inline unsigned pq_blocks(uint64_t work, unsigned per)

@@ -1434,7 +1434,8 @@ extern "C" int pq_learned_stats(pq_ctx_t *ctx, pq_learned_stats_t *st)
 // ordered, no synchronisation).  Statistics accumulate over one search.
 template <typename R>
 static void launch_seeds(pq_ctx_t *ctx, const Learned *LM, uint64_t r0, uint64_t nr,
-                         SeedT<R> *d_seeds, const ChainBuf &cb, unsigned long long *stats)
+                         SeedT<R> *d_seeds, const ChainBuf &cb, unsigned long long *stats,
+                         hipEvent_t mid)
 {
     ModelViewT<R> v[2] = {model_view<R>(ctx, *LM, 0), model_view<R>(ctx, *LM, 1)};
     MlpDims dims = {LM->k, LM->h};
@@ -1443,13 +1444,15 @@ static void launch_seeds(pq_ctx_t *ctx, const Learned *LM, uint64_t r0, uint64_t
     hipLaunchKernelGGL(k_learned_predict<R>, dim3(pq_blocks(nr, 64)), dim3(256), 0, ctx->stream,
                        ctx->geom(), ctx->dir[0].text2, r0, nr, LM->depth, dims, LM->leaves,
                        LM->segs, v[0], v[1], cb.vl);
+    if (mid)
+        (void) hipEventRecord(mid, ctx->stream);
     hipLaunchKernelGGL(k_learned_seed<R>, dim3(pq_blocks(nr, 64)), dim3(256), 0, ctx->stream,
                        ctx->geom(), ctx->dir[0].text2, r0, nr, LM->depth, cb.vl, v[0], v[1],
                        d_seeds, LM->qstats, cb, ctx->t, stats);
 }
 
 int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed,
-                     const ChainBuf &cb, unsigned long long *stats)
+                     const ChainBuf &cb, unsigned long long *stats, hipEvent_t mid)
 {
     Learned *LM = ctx->learned;
     if (!LM)
@@ -1459,9 +1462,9 @@ int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uin
                            (unsigned long long *) LM->qstats,
                            (uint32_t) (sizeof(QStats) / sizeof(unsigned long long)));
     if (ctx->wide)
-        launch_seeds<uint64_t>(ctx, LM, r0, nr, (SeedT<uint64_t> *) d_seeds, cb, stats);
+        launch_seeds<uint64_t>(ctx, LM, r0, nr, (SeedT<uint64_t> *) d_seeds, cb, stats, mid);
     else
-        launch_seeds<uint32_t>(ctx, LM, r0, nr, (SeedT<uint32_t> *) d_seeds, cb, stats);
+        launch_seeds<uint32_t>(ctx, LM, r0, nr, (SeedT<uint32_t> *) d_seeds, cb, stats, mid);
     PQ_CHECK(hipGetLastError());
     *kseed = LM->depth;
     return PQ_OK;

@@ -1475,7 +1475,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     const uint32_t t = ctx->t;
     ctx->searched = false;
     ctx->timing.find_ms = ctx->timing.extract_ms = ctx->timing.compact_ms = 0;
-    ctx->timing.learned_ms = 0;
+    ctx->timing.learned_ms = ctx->timing.seed_kernel_ms = 0;
     ctx->timing.find_kernel_ms = 0;
     ctx->timing.find_launches = 0;
     ctx->timing.find_positions = ctx->timing.find_lines = ctx->timing.verify_rows = 0;
@@ -1528,9 +1528,10 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     SeedT<R> *seeds = nullptr;
     char *gscratch = nullptr;
     HugeRead *huge = nullptr;
-    // four events per batch (seed, find, extract boundaries), read after the loop
+    // five events per batch (seed, find, extract boundaries; the seed kernel's start
+    // after the model's prediction), read after the loop
     const uint64_t nbatch = (n + batch - 1) / batch;
-    std::vector<hipEvent_t> ev(4 * nbatch);
+    std::vector<hipEvent_t> ev(5 * nbatch);
     for (auto &e : ev)
         (void) hipEventCreate(&e);
     auto cleanup = [&]() {
@@ -1646,7 +1647,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     const uint32_t grid2 = (uint32_t) std::min<uint64_t>(batch, 4096);
     for (uint64_t b = 0; b < nbatch && rc == PQ_OK; b++) {
         const uint64_t r0 = b * batch, nr = std::min(batch, n - r0);
-        hipEvent_t *e = &ev[4 * b];
+        hipEvent_t *e = &ev[5 * b];
         uint32_t kseed = 0;
         if (timed)
             (void) hipEventRecord(e[0], ctx->stream);
@@ -1654,7 +1655,8 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         if (listed)
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, cb.wq_cnt, 2u);
         if (mode == PQ_SEARCH_LEARNED &&
-            (rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed, cb, fstats)))
+            (rc = pq_learned_seeds(ctx, r0, nr, seeds, &kseed, cb, fstats,
+                                   timed ? e[4] : nullptr)))
             break;
         if (timed)
             (void) hipEventRecord(e[1], ctx->stream);
@@ -1791,9 +1793,11 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
             rc = ctx->fail(PQ_ERR_HIP, "search result read-back");
         if (rc == PQ_OK) {
             for (uint64_t b = 0; b < nbatch; b++) {
-                hipEvent_t *e = &ev[4 * b];
-                if (mode == PQ_SEARCH_LEARNED)
+                hipEvent_t *e = &ev[5 * b];
+                if (mode == PQ_SEARCH_LEARNED) {
                     ctx->timing.learned_ms += ev_ms(e[0], e[1]);
+                    ctx->timing.seed_kernel_ms += ev_ms(e[4], e[1]);
+                }
                 double fms = ev_ms(e[1], e[2]);
                 ctx->timing.find_ms += fms;
                 ctx->timing.find_kernel_ms += fms;
