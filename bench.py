@@ -330,6 +330,23 @@ def search_roofline(positions, lines, rows, kernel_ms, launches, copy_gbs, read_
             "frac_of_scattered_ceiling": round(achieved / best, 4) if best else None}
 
 
+def reference_record_match(config, edges, digest):
+    """Whether this run's plain graph (node and edge_t record digests, sum of overlap
+    lengths) equals the committed full-size record of the reference's own graph for
+    this configuration (profiles/config<N>_parity.json, written by
+    oracle/full_parity.py from the reference's construct_string_graph), or None
+    when there is no record or the digests are of another kind."""
+    path = os.path.join(ROOT, "profiles", f"config{config}_parity.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        ref = json.load(f)["reference"]
+    if not str(ref.get("edges_digest", "")).split(":")[0] == digest[1].split(":")[0]:
+        return None
+    return (ref["edges"] == edges and ref["nodes_digest"] == digest[0] and
+            ref["edges_digest"] == digest[1] and ref["overlap_len_sum"] == digest[2])
+
+
 def extract_roofline(items, edges, extract_ms, steps):
     """The extraction phase (all tiers of one search) against the HBM peak: 64 B per
     candidate item (its extension key, a scattered read of the text) + 8 B per item
@@ -557,6 +574,7 @@ def run_workload(pq, np, config, args, cpu_baseline):
         "edges": edges, "edge_record_bytes": rec_bytes,
         "edges_digest": digests["plain"][1], "nodes_digest": digests["plain"][0],
         "overlap_len_sum": digests["plain"][2],
+        "matches_reference_record": reference_record_match(config, edges, digests["plain"]),
         "extraction_shape": {k: tune[k] for k in ("group_lanes", "items")},
         "find_kernel": tune["find_chains"], "verify_rows": tune["verify_rows"],
         "plain_roofline": plain_roof, "build_roofline": build_roofline, "graph_mode": graph,
