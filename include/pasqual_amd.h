@@ -163,6 +163,26 @@ int pq_graph_nodes(pq_ctx_t *ctx, pq_node_t *nodes, uint64_t *overlap_len_sum);
  * at the host link's rate. */
 int pq_graph_export(pq_ctx_t *ctx, pq_node_t *nodes, void *edges, uint32_t index_bytes,
                     uint64_t *overlap_len_sum);
+/* Search (as pq_find_overlaps; eager modes only, PQ_SEARCH_GRAPH is refused)
+ * that exports the graph as it goes: once a batch's edges are extracted, its
+ * node records (pq_graph_nodes) and its edges in the edge_t layout
+ * (pq_graph_copy_edges) leave for nodes[n] and edges[edge_capacity] on a second
+ * stream while the next batch is searched, so that only the last batch's
+ * transfer follows the search.  Fills *n_edges and the overlap-length sum (either
+ * may be NULL).  An edges buffer too small for the graph: the node records are
+ * delivered, *n_edges says how many edges there are, the result stays on the
+ * device for pq_graph_export, and PQ_ERR_LIMIT is returned.  Both the search and
+ * the graph it hands over are the ones construct_string_graph produces
+ * (string_graph.c:734-1021); nodes or edges may be NULL.  Into memory from
+ * pq_host_alloc the copies run at the host link's rate. */
+int pq_find_overlaps_export(pq_ctx_t *ctx, int mode, pq_node_t *nodes, void *edges,
+                            uint64_t edge_capacity, uint32_t index_bytes, uint64_t *n_edges,
+                            uint64_t *overlap_len_sum);
+/* Reads per search batch (0: automatic, batches whose candidate buffers take a
+ * third of the free device memory).  Smaller batches cost launches; the result
+ * is the same for any batch size.  No counterpart in the reference, which
+ * searches read by read. */
+int pq_set_search_batch(pq_ctx_t *ctx, uint64_t reads);
 /* Page-locked host memory (hipHostMalloc) for the transfers above; released
  * by pq_host_free.  No counterpart in the reference, whose results never leave
  * host memory. */

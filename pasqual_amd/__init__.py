@@ -41,6 +41,7 @@ PQ_SYMBOLS = [
     "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read", "pq_probe_gather", "pq_set_edge_capacity",
     "pq_get_slot_sizes", "pq_graph_export", "pq_host_alloc", "pq_host_free",
     "pq_mlp_eval", "pq_set_verify", "pq_dedup_reads_noalloc", "pq_dev_alloc", "pq_dev_free", "pq_dev_upload", "pq_dev_download",
+    "pq_find_overlaps_export", "pq_set_search_batch",
 ]
 NOALLOC_SCRATCH_BYTES = 32 << 20   # PQ_NOALLOC_SCRATCH_BYTES
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
@@ -182,6 +183,8 @@ def lib(width="u32"):
         "pq_dedup_reads": (I32, [P, P, P, U64, I32, U64, I32, I32, P, P]),
         "pq_graph_nodes": (I32, [P, P, P]),
         "pq_graph_export": (I32, [P, P, P, U32, P]),
+        "pq_find_overlaps_export": (I32, [P, I32, P, P, U64, U32, P, P]),
+        "pq_set_search_batch": (I32, [P, U64]),
         "pq_host_alloc": (I32, [P, U64, C.POINTER(P)]),
         "pq_host_free": (None, [P]),
         "pq_mlp_eval": (I32, [P, I32, P, U32, I32, P]),
@@ -457,6 +460,25 @@ class Context:
                                             None if edges is None else edges.ptr, index_bytes,
                                             C.byref(total)), "graph_export")
         return int(total.value)
+
+    def find_overlaps_export(self, nodes, edges, index_bytes, mode=SEARCH_PLAIN):
+        """Search that hands over the graph as it goes (pq_find_overlaps_export):
+        node records into `nodes` (n * 8 bytes) and edge_t records into `edges`
+        while later batches are searched; either may be None.  Returns (edges,
+        overlap-length sum).  An edges buffer too small raises PqError with the
+        edge count left in graph_size()."""
+        E, total = C.c_uint64(), C.c_uint64()
+        rec = 8 if index_bytes == 4 else 16
+        cap = 0 if edges is None else edges.nbytes // rec
+        self._check(self._L.pq_find_overlaps_export(
+            self._h, mode, None if nodes is None else nodes.ptr,
+            None if edges is None else edges.ptr, cap, index_bytes, C.byref(E),
+            C.byref(total)), "find_overlaps_export")
+        return int(E.value), int(total.value)
+
+    def set_search_batch(self, reads):
+        """Reads per search batch (0: automatic)."""
+        self._check(self._L.pq_set_search_batch(self._h, int(reads)), "set_search_batch")
 
     def graph_size(self):
         """(reads, edges) of the last search."""

@@ -292,34 +292,42 @@ string_graph_t *construct_string_graph(index_t *ix)
     }
     const int timing = env_int("PQ_TIMING", 0);
     double tp = now_ms();
-    int rc = pq_find_overlaps(g_ctx, mode);
+    /* the search hands node records and edge_t records over batch by batch into
+     * page-locked staging (set up by construct_index for about four edges per
+     * read); a graph with more edges than the staging holds is exported again
+     * after the staging has grown */
+    const uint64_t n0 = pq_num_reads(g_ctx);
+    pq_node_t *rec = stage(&g_stage_nodes, &g_stage_nodes_bytes, sizeof(pq_node_t) * (n0 + 1));
+    edge_t *all = stage(&g_stage_edges, &g_stage_edges_bytes,
+                        g_stage_edges_bytes > sizeof(edge_t) ? g_stage_edges_bytes
+                                                             : sizeof(edge_t));
+    if (!rec || !all)
+        dev_fail("graph buffers", PQ_ERR_NOMEM);
+    uint64_t n = n0, E = 0, len_sum = 0;
+    int rc = pq_find_overlaps_export(g_ctx, mode, rec, all, g_stage_edges_bytes / sizeof(edge_t),
+                                     (uint32_t) sizeof(INDEX_TYPE_T), &E, &len_sum);
+    if (rc == PQ_ERR_LIMIT && E > g_stage_edges_bytes / sizeof(edge_t)) {
+        all = stage(&g_stage_edges, &g_stage_edges_bytes, sizeof(edge_t) * (E + 1));
+        if (!all)
+            dev_fail("graph buffers", PQ_ERR_NOMEM);
+        rc = pq_graph_export(g_ctx, rec, all, (uint32_t) sizeof(INDEX_TYPE_T), &len_sum);
+    }
     if (timing) {
         pq_timing_t st;
         pq_get_timing(g_ctx, &st);
         fprintf(stderr, "[pq] search %.3f ms\n", now_ms() - tp);
-        fprintf(stderr, "[pq] search find %.3f ms\n[pq] search extract %.3f ms\n"
-                "[pq] search compact %.3f ms\n", st.find_ms, st.extract_ms, st.compact_ms);
+        fprintf(stderr, "[pq] search find %.3f ms\n[pq] search extract %.3f ms\n",
+                st.find_ms, st.extract_ms);
         tp = now_ms();
     }
     if (rc == PQ_ERR_LIMIT)
         REF_ERROR_EXIT("assembly failed. Try a larger min overlap length.\n");
     if (rc)
         dev_fail("overlap search", rc);
-    uint64_t n = 0, E = 0;
     pq_graph_sizes(g_ctx, &n, &E);
-    /* node records (types, marks, counts, computed on the device) and every
-     * node's edges in CSR order, into page-locked staging */
-    pq_node_t *rec = stage(&g_stage_nodes, &g_stage_nodes_bytes, sizeof(pq_node_t) * (n + 1));
-    edge_t *all = stage(&g_stage_edges, &g_stage_edges_bytes, sizeof(edge_t) * (E + 1));
     string_graph_t *g = malloc(sizeof(string_graph_t));
-    if (!rec || !all || !g)
+    if (!g)
         dev_fail("graph buffers", PQ_ERR_NOMEM);
-    uint64_t len_sum = 0;
-    rc = pq_graph_export(g_ctx, rec, all, (uint32_t) sizeof(INDEX_TYPE_T), &len_sum);
-    if (rc == PQ_ERR_LIMIT)
-        REF_ERROR_EXIT("assembly failed. Try a larger min overlap length.\n");
-    if (rc)
-        dev_fail("graph copy", rc);
     if (timing)
         fprintf(stderr, "[pq] graph copy %.3f ms\n", now_ms() - tp), tp = now_ms();
     g->nodes = calloc(n ? n : 1, sizeof(node_t));

@@ -48,6 +48,8 @@ void *pq_ctx::tmp(int slot, size_t bytes)
         return b.p;
     if (b.p) {
         (void) hipStreamSynchronize(stream);
+        if (xstream)
+            (void) hipStreamSynchronize(xstream);
         (void) hipFree(b.p);
     }
     b.p = nullptr;
@@ -131,6 +133,10 @@ extern "C" void pq_ctx_destroy(pq_ctx_t *ctx)
     pq_index_release(ctx);
     pq_learned_free(ctx);   // search results live in scratch slots, freed below
     pq_search_graph_free(ctx);
+    if (ctx->xstream) {
+        (void) hipStreamSynchronize(ctx->xstream);
+        (void) hipStreamDestroy(ctx->xstream);
+    }
     for (DevBuf &b : ctx->scratch)
         pq_free(b.p);
     (void) hipStreamDestroy(ctx->stream);

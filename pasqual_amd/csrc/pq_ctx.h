@@ -39,6 +39,7 @@ struct Learned;                    // pq_learned.hip
 struct pq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t xstream = nullptr; // transfers to the host beside the search (created on use)
     std::string err;
 
     // reads
@@ -76,6 +77,7 @@ struct pq_ctx {
                                    // verified on the text (0: backward search to the end)
     uint64_t edge_cap_auto = 0;    // capacity an overflowing search asked for (these reads)
     uint64_t edge_need = 0;        // edges the last overflowing search counted
+    uint64_t search_batch = 0;     // reads per search batch, 0 automatic
     uint64_t learned_gen = 0;      // bumped by every learned-index training
     // search captured as a graph, valid while its key (launch arguments) holds
     hipGraph_t search_graph = nullptr;
@@ -136,6 +138,7 @@ enum PqSlot {
     SLOT_LEARNED = 40, SLOT_BWT8 = 47, SLOT_SLICE_HIST, SLOT_SLICE_TILES, SLOT_PROBE = 50,
     SLOT_BIGRUNS, SLOT_NODES, SLOT_MISC2, SLOT_HUGE, SLOT_HUGE_LIST, SLOT_VJOB_L, SLOT_VJOB,
     SLOT_WORKQ, SLOT_WSTATS,
+    SLOT_XSTAGE0 = 60, SLOT_XSTAGE1, SLOT_XTOT, SLOT_XSCAN,
 };
 
 // Grow-only slot buffer of at least `bytes` bytes into *p, or PQ_ERR_NOMEM.

@@ -1310,6 +1310,88 @@ __global__ void k_csr(const uint64_t *recB, const uint64_t *recF, const uint32_t
         dst[a + nb + i] = sf[i];
 }
 
+// Edges in the reference's edge_t layout (string_graph.h: INDEX_TYPE_T dest,
+// unsigned short overlap_len, char type, padded to the alignment of dest).
+__global__ void k_ref_edges(const uint64_t *pk, uint64_t E, uint32_t index_bytes, uint8_t *out)
+{
+    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E)
+        return;
+    const uint64_t e = pk[i];
+    const uint64_t dest = e >> 32;
+    const uint16_t len = (uint16_t) ((e >> 8) & 0xFFFF);
+    const uint8_t type = (uint8_t) (e & 0xFF);
+    if (index_bytes == 4) {
+        // 8 bytes: dest u32 | len u16 | type u8 | pad
+        uint2 v = make_uint2((uint32_t) dest, (uint32_t) len | ((uint32_t) type << 16));
+        ((uint2 *) out)[i] = v;
+    } else {
+        // 16 bytes: dest u64 | len u16 | type u8 | pad
+        uint4 v = make_uint4((uint32_t) dest, (uint32_t) (dest >> 32),
+                             (uint32_t) len | ((uint32_t) type << 16), 0u);
+        ((uint4 *) out)[i] = v;
+    }
+}
+
+// Node records, string_graph.c:842-889: SUSPECT mark from the candidate counts
+// (ratio in double, as the reference computes it), B/F types from the edge
+// counts, REMOVED for a node without edges.  Flags a side of more than 65535
+// edges (the 16-bit counts of node_t).
+__global__ void k_node_records(const int32_t *ovB, const int32_t *ovF, const uint32_t *nB,
+                               const uint32_t *nF, uint64_t n, uint2 *out, uint32_t *over)
+{
+    PQ_GRID_STRIDE_FOR(i, n)
+    {
+        const uint32_t b = nB[i], f = nF[i];
+        if (b > 65535u || f > 65535u)
+            atomicOr(over, 1u);
+        const int32_t a = ovB[i], c = ovF[i], mx = a > c ? a : c;
+        const double ratio = mx == 0 ? 0.0 : (double) abs(a - c) / (double) mx;
+        const uint32_t marked = ratio > 0.6 ? 4u : 0u;
+        uint32_t bt = 4u, ft = 4u, status = 8u;          // ENDING, ENDING, REMOVED
+        if (b || f) {
+            bt = b == 1 ? 1u : b > 1 ? 2u : 4u;
+            ft = f == 1 ? 1u : f > 1 ? 2u : 4u;
+            status = 0u;
+        }
+        out[i] = make_uint2((b & 0xFFFFu) | (f << 16), bt | (ft << 8) | (status << 16) | (marked << 24));
+    }
+}
+
+// Sum of the overlap lengths of all edges (packed dest<<32 | len<<8 | type).
+__global__ void k_edge_len_sum(const uint64_t *pk, uint64_t E, unsigned long long *sum)
+{
+    uint64_t s = 0;
+    PQ_GRID_STRIDE_FOR(i, E)
+        s += (pk[i] >> 8) & 0xFFFFu;
+    for (int o = 32; o; o >>= 1)
+        s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63u) == 0 && s)
+        atomicAdd(sum, (unsigned long long) s);
+}
+
+__global__ void k_zero_words(unsigned long long *p, uint32_t count)
+{
+    if (threadIdx.x < count)
+        p[threadIdx.x] = 0ull;
+}
+
+// Scan input of one batch's part of the CSR offsets: in[0] = the offset where the
+// batch's edges start (final once the batches before it are done), in[1 + j] =
+// edges of batch read j, in[nr + 1] = 0.  Entries 1 .. nr + 1 of its exclusive
+// scan are edge_off[r0 .. r0 + nr].
+__global__ void k_batch_totals(const uint32_t *nb, const uint32_t *nf, uint64_t nr,
+                               const uint64_t *base, uint64_t *in)
+{
+    const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0)
+        in[0] = *base;
+    else if (i <= nr)
+        in[i] = (uint64_t) nb[i - 1] + nf[i - 1];
+    else if (i == nr + 1)
+        in[i] = 0;
+}
+
 // Internal results of one search attempt (positive: pq_find_overlaps retries).
 #define PQ_RETRY_EAGER 1     // a graph replay met a side beyond the global tier
 #define PQ_RETRY_EDGES 2     // the shared edge pool overflowed; ctx->edge_need = demand
@@ -1469,7 +1551,21 @@ static int run_search_graph(pq_ctx_t *ctx, int mode, uint64_t n, uint64_t batch,
     return PQ_OK;
 }
 
-template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph)
+// The graph exported into caller memory during an eager search: once a batch's
+// edges are extracted, its part of the CSR, its node records and its edge_t
+// records are made on the device and leave for the host on the transfer stream
+// while the next batch is searched.
+struct ExportSpec {
+    pq_node_t *nodes;          // n records, or nullptr
+    uint8_t *edges;            // edge_capacity edge_t records, or nullptr
+    uint64_t edge_capacity;
+    uint32_t index_bytes;      // 4 or 8
+    uint64_t len_sum;          // out: sum of the overlap lengths
+    bool edges_short;          // out: the edges did not fit edge_capacity
+};
+
+template <typename R>
+static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *xp = nullptr)
 {
     const uint64_t n = ctx->n;
     const uint32_t t = ctx->t;
@@ -1518,6 +1614,8 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
             budget = std::max<uint64_t>(budget, ((uint64_t) free_b + held) / 3);
     }
     uint64_t batch = std::max<uint64_t>(1, std::min<uint64_t>(n, budget / per_read));
+    if (ctx->search_batch)
+        batch = std::min<uint64_t>(batch, ctx->search_batch);
     uint64_t *rec = nullptr, *recF = nullptr, *app = nullptr, *tot = nullptr, *staged = nullptr;
     uint32_t *ovl = nullptr, *misc = nullptr;
     unsigned long long *wstats = nullptr;
@@ -1605,6 +1703,56 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     unsigned long long *fstats = wstats;
     eo.xstats = wstats;
     eo.tier = 0;
+    // export during the search (eager only): the batch's scan input and output,
+    // node records and the two words of enqueue_node_records' layout, two staging
+    // buffers of edge_t records used in turn, and the events ordering the
+    // conversions on the search stream against the copies on the transfer stream
+    uint64_t *xtot = nullptr, *xscan = nullptr;
+    unsigned long long *xwords = nullptr;
+    uint2 *xrecs = nullptr;
+    uint8_t *xstage[2] = {nullptr, nullptr};
+    size_t xstage_bytes[2] = {0, 0};
+    hipEvent_t xconv[2] = {nullptr, nullptr}, xdone[2] = {nullptr, nullptr};
+    bool xdone_set[2] = {false, false};
+    uint64_t x_lo = 0;          // edges of the batches exported so far
+    bool x_stop = false;        // an error flag or a full edge buffer: no more edge copies
+    const size_t x_rec = xp && xp->index_bytes == 4 ? 8 : 16;
+    auto xcleanup = [&]() {
+        if (ctx->xstream)
+            (void) hipStreamSynchronize(ctx->xstream);
+        for (int i = 0; i < 2; i++) {
+            if (xconv[i])
+                (void) hipEventDestroy(xconv[i]);
+            if (xdone[i])
+                (void) hipEventDestroy(xdone[i]);
+        }
+    };
+    if (xp) {
+        xp->len_sum = 0;
+        xp->edges_short = false;
+        uint8_t *nb = nullptr;
+        if (!ctx->xstream &&
+            hipStreamCreateWithFlags(&ctx->xstream, hipStreamNonBlocking) != hipSuccess) {
+            ctx->xstream = nullptr;
+            cleanup();
+            return ctx->fail(PQ_ERR_HIP, "transfer stream");
+        }
+        if ((rc = pq_slot(ctx, SLOT_XTOT, (batch + 2) * 8, &xtot)) ||
+            (rc = pq_slot(ctx, SLOT_XSCAN, (batch + 2) * 8, &xscan)) ||
+            (rc = pq_slot(ctx, SLOT_NODES, n * 8 + 16, &nb))) {
+            cleanup();
+            return rc;
+        }
+        xwords = (unsigned long long *) nb;
+        xrecs = (uint2 *) (nb + 16);
+        for (int i = 0; i < 2; i++)
+            if (hipEventCreateWithFlags(&xconv[i], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&xdone[i], hipEventDisableTiming) != hipSuccess) {
+                xcleanup();
+                cleanup();
+                return ctx->fail(PQ_ERR_HIP, "transfer events");
+            }
+    }
     // the whole search as a sequence of stream operations (captured into a graph
     // in graph mode: kernels only, no memsets, no host synchronisation)
     // search kernel: the autotuner's or the caller's choice, else by the size of
@@ -1734,6 +1882,73 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
         PQ_CHECK(hipGetLastError());
         if (timed)
             (void) hipEventRecord(e[3], ctx->stream);
+        if (xp && timed) {
+            // this batch's part of the CSR (its reads' edges are final) and its node
+            // records
+            if (b == 0) {
+                hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream,
+                                   (uint32_t *) ctx->d_edge_off, 2u);
+                hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, ctx->stream, xwords, 2u);
+            }
+            hipLaunchKernelGGL(k_batch_totals, dim3(pq_blocks(nr + 2, 256)), dim3(256), 0,
+                               ctx->stream, ctx->d_numB + r0, ctx->d_numF + r0, nr,
+                               ctx->d_edge_off + r0, xtot);
+            if ((rc = pq_scan_u64(ctx, xtot, xscan, nr + 2, nullptr)))
+                break;
+            PQ_CHECK(hipMemcpyAsync(ctx->d_edge_off + r0, xscan + 1, (nr + 1) * 8,
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+            hipLaunchKernelGGL(k_csr, dim3(pq_blocks(nr, 256)), dim3(256), 0, ctx->stream,
+                               rec + r0, recF + r0, ctx->d_numB + r0, ctx->d_numF + r0,
+                               ctx->d_edge_off + r0, staged + r0 * 2 * EDGE_SLOT, app, nr,
+                               misc + 3, ctx->d_edges);
+            hipLaunchKernelGGL(k_node_records, dim3(pq_blocks_gs(nr, 256)), dim3(256), 0,
+                               ctx->stream, ctx->d_ovB + r0, ctx->d_ovF + r0, ctx->d_numB + r0,
+                               ctx->d_numF + r0, nr, xrecs + r0, (uint32_t *) (xwords + 1));
+            PQ_CHECK(hipGetLastError());
+            uint64_t hi = 0;
+            uint32_t fl = 0;
+            PQ_CHECK(hipMemcpyAsync(&hi, ctx->d_edge_off + r0 + nr, 8, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+            PQ_CHECK(hipMemcpyAsync(&fl, misc + 3, 4, hipMemcpyDeviceToHost, ctx->stream));
+            PQ_CHECK(hipStreamSynchronize(ctx->stream));
+            // an error flag leaves the counts beyond the buffers: the search is run
+            // again or fails, and nothing more is converted
+            if (fl & 3u)
+                x_stop = true;
+            if (!x_stop && xp->edges && hi > xp->edge_capacity) {
+                xp->edges_short = true;
+                x_stop = true;
+            }
+            const int sb = (int) (b & 1);
+            const bool conv = !x_stop && xp->edges && hi > x_lo;
+            if (conv) {
+                const size_t need = (hi - x_lo) * x_rec;
+                if (xstage_bytes[sb] < need) {
+                    if ((rc = pq_slot(ctx, SLOT_XSTAGE0 + sb, need, &xstage[sb])))
+                        break;
+                    xstage_bytes[sb] = need;
+                }
+                // the staging buffer's previous copy (two batches back) must be done
+                if (xdone_set[sb])
+                    PQ_CHECK(hipStreamWaitEvent(ctx->stream, xdone[sb], 0));
+                hipLaunchKernelGGL(k_ref_edges, dim3(pq_blocks(hi - x_lo, 256)), dim3(256), 0,
+                                   ctx->stream, ctx->d_edges + x_lo, hi - x_lo, xp->index_bytes,
+                                   xstage[sb]);
+                PQ_CHECK(hipGetLastError());
+            }
+            PQ_CHECK(hipEventRecord(xconv[sb], ctx->stream));
+            PQ_CHECK(hipStreamWaitEvent(ctx->xstream, xconv[sb], 0));
+            if (conv)
+                PQ_CHECK(hipMemcpyAsync(xp->edges + x_lo * x_rec, xstage[sb], (hi - x_lo) * x_rec,
+                                        hipMemcpyDeviceToHost, ctx->xstream));
+            if (xp->nodes)
+                PQ_CHECK(hipMemcpyAsync(xp->nodes + r0, xrecs + r0, nr * 8, hipMemcpyDeviceToHost,
+                                        ctx->xstream));
+            PQ_CHECK(hipEventRecord(xdone[sb], ctx->xstream));
+            xdone_set[sb] = true;
+            if (!x_stop)
+                x_lo = hi;
+        }
     }
     // the work counters' copies summed into misc[8..39]
     hipLaunchKernelGGL(k_wstat_sum, dim3(1), dim3(64), 0, ctx->stream, wstats,
@@ -1753,6 +1968,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
                                         (uint64_t) G.Lmax, (uint64_t) G.N, (uint64_t) cb.cap,
                                         (uint64_t) (uintptr_t) cb.vl, (uint64_t) (uintptr_t) cb.vjob,
                                         (uint64_t) cb.vmax});
+        xcleanup();
         cleanup();
         return rc;
     }
@@ -1762,6 +1978,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     if ((rc = pq_slot(ctx, SLOT_EDGES, ub * 8, &ctx->d_edges)) ||
         (!ctx->tmp(SLOT_SCAN, pq_scan_scratch_bytes(n + 1, 8)) &&
          (rc = ctx->fail(PQ_ERR_NOMEM, "scan scratch")))) {
+        xcleanup();
         cleanup();
         return rc;
     }
@@ -1769,13 +1986,13 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
     (void) hipEventCreate(&ce[0]);
     (void) hipEventCreate(&ce[1]);
     rc = enqueue(true);
-    if (rc == PQ_OK) {
+    if (rc == PQ_OK && !xp) {
         (void) hipEventRecord(ce[0], ctx->stream);
         hipLaunchKernelGGL(k_edge_totals, dim3(pq_blocks(n + 1, 256)), dim3(256), 0, ctx->stream,
                            ctx->d_numB, ctx->d_numF, n, tot);
         rc = pq_scan_u64(ctx, tot, ctx->d_edge_off, n + 1, nullptr);
     }
-    if (rc == PQ_OK) {
+    if (rc == PQ_OK && !xp) {
         hipLaunchKernelGGL(k_csr, dim3(pq_blocks(n, 256)), dim3(256), 0, ctx->stream, rec, recF,
                            ctx->d_numB, ctx->d_numF, ctx->d_edge_off, staged, app, n, misc + 3,
                            ctx->d_edges);
@@ -1804,7 +2021,7 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
                 ctx->timing.find_launches++;
                 ctx->timing.extract_ms += ev_ms(e[2], e[3]);
             }
-            ctx->timing.compact_ms = ev_ms(ce[0], ce[1]);
+            ctx->timing.compact_ms = xp ? 0.0 : ev_ms(ce[0], ce[1]);
             search_counters(ctx, hm);
             if (hm[3] & 2u) {
                 rc = ctx->fail(PQ_ERR_STATE, "oversized read side left unprocessed");
@@ -1817,17 +2034,37 @@ template <typename R> static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool a
             }
         }
     }
+    if (rc == PQ_OK && xp) {
+        // the length sum over all edges and the 65535-edge flag; then the last
+        // batch's copies
+        unsigned long long w[2] = {0, 0};
+        if (ctx->n_edges)
+            hipLaunchKernelGGL(k_edge_len_sum,
+                               dim3(std::min<unsigned>(pq_blocks_gs(ctx->n_edges, 256), 2048u)),
+                               dim3(256), 0, ctx->stream, ctx->d_edges, ctx->n_edges, xwords);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(w, xwords, 16, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->xstream) != hipSuccess)
+            rc = ctx->fail(PQ_ERR_HIP, "graph export");
+        else if (w[1])
+            rc = ctx->fail(PQ_ERR_LIMIT, "a node has more than 65535 edges on one side");
+        else
+            xp->len_sum = w[0];
+    }
     for (auto &e : ce)
         if (e)
             (void) hipEventDestroy(e);
+    xcleanup();
     cleanup();
     return rc;
 }
 
-extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
+// One search with its retries: a graph replay that met a read side beyond the
+// global tier runs again eagerly; a search whose edges overflowed the shared
+// pool runs again with the pool at the demand it counted.
+static int find_overlaps_retry(pq_ctx_t *ctx, int mode, ExportSpec *xp)
 {
-    if (!ctx)
-        return PQ_ERR_ARG;
     if (!ctx->dir[0].built || !ctx->dir[1].built)
         return ctx->fail(PQ_ERR_STATE, "index not built");
     const bool as_graph = (mode & PQ_SEARCH_GRAPH) != 0;
@@ -1836,13 +2073,12 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
         return ctx->fail(PQ_ERR_ARG, "unknown search mode");
     if (mode == PQ_SEARCH_LEARNED && !ctx->learned)
         return ctx->fail(PQ_ERR_STATE, "learned index not trained");
+    if (xp && as_graph)
+        return ctx->fail(PQ_ERR_ARG, "the export during a search needs an eager search");
     auto attempt = [&](bool graph) {
-        return ctx->wide ? find_overlaps_t<uint64_t>(ctx, mode, graph)
-                         : find_overlaps_t<uint32_t>(ctx, mode, graph);
+        return ctx->wide ? find_overlaps_t<uint64_t>(ctx, mode, graph, xp)
+                         : find_overlaps_t<uint32_t>(ctx, mode, graph, xp);
     };
-    // a graph replay that met a read side beyond the global tier runs again
-    // eagerly; a search whose edges overflowed the shared pool runs again with
-    // the pool at the demand it counted
     bool graph = as_graph;
     int rc = PQ_OK;
     for (int tries = 0; tries < 4; tries++) {
@@ -1857,6 +2093,49 @@ extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
     if (rc > 0)
         rc = ctx->fail(PQ_ERR_STATE, "search did not settle its buffers");
     return rc;
+}
+
+extern "C" int pq_find_overlaps(pq_ctx_t *ctx, int mode)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    return find_overlaps_retry(ctx, mode, nullptr);
+}
+
+extern "C" int pq_find_overlaps_export(pq_ctx_t *ctx, int mode, pq_node_t *nodes, void *edges,
+                                       uint64_t edge_capacity, uint32_t index_bytes,
+                                       uint64_t *n_edges, uint64_t *overlap_len_sum)
+{
+    if (!ctx || (index_bytes != 4 && index_bytes != 8))
+        return PQ_ERR_ARG;
+    ExportSpec xp;
+    xp.nodes = nodes;
+    xp.edges = (uint8_t *) edges;
+    xp.edge_capacity = edges ? edge_capacity : 0;
+    xp.index_bytes = index_bytes;
+    xp.len_sum = 0;
+    xp.edges_short = false;
+    int rc = find_overlaps_retry(ctx, mode, &xp);
+    if (rc)
+        return rc;
+    if (n_edges)
+        *n_edges = ctx->n_edges;
+    if (overlap_len_sum)
+        *overlap_len_sum = xp.len_sum;
+    if (xp.edges_short)
+        return ctx->fail(PQ_ERR_LIMIT, "edge buffer holds " + std::to_string(edge_capacity) +
+                                           " records, the graph has " +
+                                           std::to_string(ctx->n_edges));
+    return PQ_OK;
+}
+
+extern "C" int pq_set_search_batch(pq_ctx_t *ctx, uint64_t reads)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    ctx->search_batch = reads;
+    pq_search_graph_free(ctx);
+    return PQ_OK;
 }
 
 extern "C" int pq_graph_sizes(pq_ctx_t *ctx, uint64_t *n_reads, uint64_t *n_edges)
@@ -2021,29 +2300,6 @@ extern "C" int pq_set_edge_capacity(pq_ctx_t *ctx, uint64_t edges)
     return PQ_OK;
 }
 
-// Edges in the reference's edge_t layout (string_graph.h: INDEX_TYPE_T dest,
-// unsigned short overlap_len, char type, padded to the alignment of dest).
-__global__ void k_ref_edges(const uint64_t *pk, uint64_t E, uint32_t index_bytes, uint8_t *out)
-{
-    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= E)
-        return;
-    const uint64_t e = pk[i];
-    const uint64_t dest = e >> 32;
-    const uint16_t len = (uint16_t) ((e >> 8) & 0xFFFF);
-    const uint8_t type = (uint8_t) (e & 0xFF);
-    if (index_bytes == 4) {
-        // 8 bytes: dest u32 | len u16 | type u8 | pad
-        uint2 v = make_uint2((uint32_t) dest, (uint32_t) len | ((uint32_t) type << 16));
-        ((uint2 *) out)[i] = v;
-    } else {
-        // 16 bytes: dest u64 | len u16 | type u8 | pad
-        uint4 v = make_uint4((uint32_t) dest, (uint32_t) (dest >> 32),
-                             (uint32_t) len | ((uint32_t) type << 16), 0u);
-        ((uint4 *) out)[i] = v;
-    }
-}
-
 extern "C" int pq_graph_copy_edges(pq_ctx_t *ctx, void *edges, uint32_t index_bytes)
 {
     if (!ctx || !edges || (index_bytes != 4 && index_bytes != 8))
@@ -2063,49 +2319,6 @@ extern "C" int pq_graph_copy_edges(pq_ctx_t *ctx, void *edges, uint32_t index_by
     PQ_CHECK(hipMemcpyAsync(edges, d, E * rec, hipMemcpyDeviceToHost, ctx->stream));
     PQ_CHECK(hipStreamSynchronize(ctx->stream));
     return PQ_OK;
-}
-
-// Node records, string_graph.c:842-889: SUSPECT mark from the candidate counts
-// (ratio in double, as the reference computes it), B/F types from the edge
-// counts, REMOVED for a node without edges.  Flags a side of more than 65535
-// edges (the 16-bit counts of node_t).
-__global__ void k_node_records(const int32_t *ovB, const int32_t *ovF, const uint32_t *nB,
-                               const uint32_t *nF, uint64_t n, uint2 *out, uint32_t *over)
-{
-    PQ_GRID_STRIDE_FOR(i, n)
-    {
-        const uint32_t b = nB[i], f = nF[i];
-        if (b > 65535u || f > 65535u)
-            atomicOr(over, 1u);
-        const int32_t a = ovB[i], c = ovF[i], mx = a > c ? a : c;
-        const double ratio = mx == 0 ? 0.0 : (double) abs(a - c) / (double) mx;
-        const uint32_t marked = ratio > 0.6 ? 4u : 0u;
-        uint32_t bt = 4u, ft = 4u, status = 8u;          // ENDING, ENDING, REMOVED
-        if (b || f) {
-            bt = b == 1 ? 1u : b > 1 ? 2u : 4u;
-            ft = f == 1 ? 1u : f > 1 ? 2u : 4u;
-            status = 0u;
-        }
-        out[i] = make_uint2((b & 0xFFFFu) | (f << 16), bt | (ft << 8) | (status << 16) | (marked << 24));
-    }
-}
-
-// Sum of the overlap lengths of all edges (packed dest<<32 | len<<8 | type).
-__global__ void k_edge_len_sum(const uint64_t *pk, uint64_t E, unsigned long long *sum)
-{
-    uint64_t s = 0;
-    PQ_GRID_STRIDE_FOR(i, E)
-        s += (pk[i] >> 8) & 0xFFFFu;
-    for (int o = 32; o; o >>= 1)
-        s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63u) == 0 && s)
-        atomicAdd(sum, (unsigned long long) s);
-}
-
-__global__ void k_zero_words(unsigned long long *p, uint32_t count)
-{
-    if (threadIdx.x < count)
-        p[threadIdx.x] = 0ull;
 }
 
 // Node records and the length sum into the SLOT_NODES scratch: words[0] = sum of
