@@ -1652,8 +1652,8 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
         (rc = pq_slot(ctx, SLOT_APPEND, edge_cap * 8, &app)) ||
         (rc = pq_slot(ctx, SLOT_STAGED, (n + 1) * 2 * EDGE_SLOT * 8, &staged)) ||
         (rc = pq_slot(ctx, SLOT_TOT, (n + 1) * 8, &tot)) ||
-        (rc = pq_slot(ctx, SLOT_OVL, (3 * batch + 3) * 4, &ovl)) ||
-        (rc = pq_slot(ctx, SLOT_MISC, 160, &misc)) ||
+        (rc = pq_slot(ctx, SLOT_OVL, (5 * batch + 5) * 4, &ovl)) ||
+        (rc = pq_slot(ctx, SLOT_MISC, 192, &misc)) ||
         (rc = pq_slot(ctx, SLOT_WSTATS, (size_t) WSTAT_SLOTS * WSTAT_STRIDE * 8, &wstats)) ||
         (rc = pq_slot(ctx, SLOT_GSCRATCH, GLOBAL_SLOTS * ext_wave_bytes(KMAX_GLOBAL, kw),
                       &gscratch)) ||
@@ -1767,32 +1767,72 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
                                                                            : 3u;
     auto enqueue = [&](bool timed) -> int {
     int rc = PQ_OK;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc, 40u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc, 48u);
     hipLaunchKernelGGL(k_wstat_clear, dim3(WSTAT_SLOTS), dim3(64), 0, ctx->stream, wstats,
                        (uint32_t) WSTAT_WORDS);
     const uint32_t shape = ctx->ext_shape < EXT_NSHAPES ? ctx->ext_shape : 1;
     const uint32_t gw1 = EXT_SHAPES[shape][0], kmax1 = EXT_SHAPES[shape][1];
-    const uint32_t groups1 = EXT_WAVES * 64 / gw1;
-    const size_t lds1 = ext_wave_bytes(kmax1, kw) * groups1;
-    const size_t lds2 = ext_wave_bytes(KMAX_LDS2, kw);
-    // tiers after the first: (32 lanes, 64 items, 8 reads per block) unless the
-    // first tier already has that shape, then (64 lanes, 512 items, one read per
-    // block), then global scratch; each takes the previous one's overflow list
-    // and count: list i at ovl + i * batch, counts misc[2], misc[4], misc[6]
-    ExtOut eoB = eo, eoC = eo;
-    eoB.overflow_reads = ovl + batch;
-    eoB.n_overflow = misc + 4;
-    eoB.tier = 1;
-    eoC.overflow_reads = ovl + 2 * batch;
-    eoC.n_overflow = misc + 6;
-    eoC.tier = 2;
+    // extraction tiers: the first (the tuned shape) over every read of the batch,
+    // then each later tier over the previous tier's overflow list, read with its
+    // count from the device (no host round trip): (items, lanes per read, waves
+    // per block); the one-wave tiers hold a read per block, so that the LDS of a
+    // 256-item read does not keep a 512-item read's share from other waves
+    struct TierShape {
+        uint32_t kmax, gw, waves;
+    };
+    static const TierShape TIERS[5] = {
+        {16, 8, EXT_WAVES}, {32, 16, EXT_WAVES}, {64, 32, EXT_WAVES}, {256, 64, 1}, {512, 64, 1}};
+    uint32_t first_tier = 0;
+    while (first_tier < 4 && TIERS[first_tier].kmax < kmax1)
+        first_tier++;
+    const uint32_t ntiers = 5 - first_tier;
+    // tier i writes its overflow list at ovl + i * batch and its count at misc[40 + i]
+    uint32_t *const tier_cnt = misc + 40;
+    ExtOut eoT[5];
+    for (uint32_t i = 0; i < ntiers; i++) {
+        eoT[i] = eo;
+        eoT[i].overflow_reads = ovl + i * batch;
+        eoT[i].n_overflow = tier_cnt + i;
+        // work counters per tier class: the first, the multi-read ones after it,
+        // the one-wave ones
+        const uint32_t kmx = TIERS[first_tier + i].kmax;
+        eoT[i].tier = i == 0 ? 0u : kmx <= 64 ? 1u : 2u;
+    }
     ExtOut eoG = eo;
     eoG.tier = 3;
-    const size_t ldsB = ext_wave_bytes(64, kw) * (EXT_WAVES * 64 / 32);
-    const uint32_t gridB = (uint32_t) std::min<uint64_t>(pq_blocks(batch, EXT_WAVES * 2), 4096);
+    auto launch_tier = [&](uint32_t i, uint64_t r0, uint64_t nr) {
+        const TierShape ts = TIERS[first_tier + i];
+        const uint32_t groups = ts.waves * 64 / ts.gw;
+        const size_t lds = ext_wave_bytes(ts.kmax, kw) * groups;
+        const uint32_t *list = i ? eoT[i - 1].overflow_reads : nullptr;
+        const uint32_t *cnt = i ? eoT[i - 1].n_overflow : nullptr;
+        const dim3 grid(i ? (uint32_t) std::min<uint64_t>(pq_blocks(batch, groups), 4096)
+                          : pq_blocks(nr, groups));
+        const dim3 block(64 * ts.waves);
+        switch (ts.kmax) {
+        case 16:
+            hipLaunchKernelGGL((k_extract_lds<16, EXT_WAVES, 8, R>), grid, block, lds, ctx->stream,
+                               G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
+            break;
+        case 32:
+            hipLaunchKernelGGL((k_extract_lds<32, EXT_WAVES, 16, R>), grid, block, lds, ctx->stream,
+                               G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
+            break;
+        case 64:
+            hipLaunchKernelGGL((k_extract_lds<64, EXT_WAVES, 32, R>), grid, block, lds, ctx->stream,
+                               G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
+            break;
+        case 256:
+            hipLaunchKernelGGL((k_extract_lds<256, 1, 64, R>), grid, block, lds, ctx->stream, G, fw,
+                               bw, cb, r0, nr, list, cnt, kw, eoT[i]);
+            break;
+        default:
+            hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64, R>), grid, block, lds, ctx->stream,
+                               G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
+        }
+    };
     // per batch: seeds, candidates, then the extraction tiers; the later tiers
     // take their read lists and counts from the device (no host round trip)
-    const uint32_t grid2 = (uint32_t) std::min<uint64_t>(batch, 4096);
     for (uint64_t b = 0; b < nbatch && rc == PQ_OK; b++) {
         const uint64_t r0 = b * batch, nr = std::min(batch, n - r0);
         hipEvent_t *e = &ev[5 * b];
@@ -1828,36 +1868,14 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
                                ctx->stream, G, fw, bw, cb, r0, nr, t, fstats);
         if (timed)
             (void) hipEventRecord(e[2], ctx->stream);
-        const dim3 g1(pq_blocks(nr, groups1)), b1(64 * EXT_WAVES);
-        const uint32_t *nl = nullptr;
-        switch (shape) {
-        case 0:
-            hipLaunchKernelGGL((k_extract_lds<16, EXT_WAVES, 8, R>), g1, b1, lds1, ctx->stream, G, fw,
-                               bw, cb, r0, nr, nl, nl, kw, eo);
-            break;
-        case 2:
-            hipLaunchKernelGGL((k_extract_lds<64, EXT_WAVES, 32, R>), g1, b1, lds1, ctx->stream, G,
-                               fw, bw, cb, r0, nr, nl, nl, kw, eo);
-            break;
-        default:
-            hipLaunchKernelGGL((k_extract_lds<32, EXT_WAVES, 16, R>), g1, b1, lds1, ctx->stream, G,
-                               fw, bw, cb, r0, nr, nl, nl, kw, eo);
-        }
-        const uint32_t *inC = eo.overflow_reads, *cntC = eo.n_overflow;
-        if (shape < 2) {
-            hipLaunchKernelGGL((k_extract_lds<64, EXT_WAVES, 32, R>), dim3(gridB), b1, ldsB,
-                               ctx->stream, G, fw, bw, cb, r0, nr, eo.overflow_reads,
-                               eo.n_overflow, kw, eoB);
-            inC = eoB.overflow_reads;
-            cntC = eoB.n_overflow;
-        }
-        hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64, R>), dim3(grid2), dim3(64), lds2,
-                           ctx->stream, G, fw, bw, cb, r0, nr, inC, cntC, kw, eoC);
+        for (uint32_t i = 0; i < ntiers; i++)
+            launch_tier(i, r0, nr);
         // sides beyond the global tier: listed (count misc[7]) and sorted by the whole
         // GPU right here, before the next batch reuses the candidate buffers; a
         // graph capture cannot wait for the count, so there they raise flag 2
         hipLaunchKernelGGL(k_extract_global<R>, dim3(GLOBAL_SLOTS), dim3(64), 0, ctx->stream, G, fw,
-                           bw, cb, r0, eoC.overflow_reads, eoC.n_overflow, kw, gscratch, eoG, huge,
+                           bw, cb, r0, eoT[ntiers - 1].overflow_reads, eoT[ntiers - 1].n_overflow,
+                           kw, gscratch, eoG, huge,
                            misc + 7, timed ? 0u : 1u);
         PQ_CHECK(hipGetLastError());
         if (timed) {
@@ -1873,11 +1891,8 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
                     break;
             }
         }
-        // overflow counters back to zero for the next batch (misc[2], misc[4], misc[6],
-        // misc[7])
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eo.n_overflow, 1u);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eoB.n_overflow, 1u);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, eoC.n_overflow, 1u);
+        // overflow counters back to zero for the next batch (misc[40..44], misc[7])
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, tier_cnt, 5u);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc + 7, 1u);
         PQ_CHECK(hipGetLastError());
         if (timed)
