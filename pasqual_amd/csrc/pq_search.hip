@@ -1034,12 +1034,16 @@ __device__ bool extract_read(const Geom &G, const DirViewT<R> &fw, const DirView
     return true;
 }
 
-// Per-wave storage: keys (kw words per item), metadata, emit flags, and the
-// prefix of entry counts (an entry holds at least one item, so at most kmax).
-__host__ __device__ static inline size_t ext_wave_bytes(uint32_t kmax, uint32_t kw)
+// Per-group storage: keys (kw words per item), metadata, emit flags, and the
+// prefix of entry counts (an entry holds at least one item, so at most kmax);
+// with `ent` also both sides' candidate entries (kmax per side), read once from
+// the candidate buffers.
+__host__ __device__ static inline size_t ext_wave_bytes(uint32_t kmax, uint32_t kw, bool ent = false)
 {
     size_t b = (size_t) kmax * kw * 8 + (size_t) kmax * sizeof(ItemMeta) +
                (size_t) (kmax + 2) * 4 + (size_t) kmax * 2 + kmax;
+    if (ent)
+        b += (size_t) kmax * 2 * (sizeof(uint2) + sizeof(uint16_t));
     return (b + 15) & ~(size_t) 15;
 }
 
@@ -1049,17 +1053,169 @@ struct ExtWave {
     uint32_t *pre;
     uint16_t *lcp;       // one-wave tiers: codes shared with the previous sorted item
     uint8_t *emitf;
+    uint2 *elc;          // entries of side s at [s * kmax, s * kmax + kmax) (with `ent`)
+    uint16_t *eo;
 };
 
-__device__ __forceinline__ ExtWave ext_wave(char *base, uint32_t kmax, uint32_t kw)
+__device__ __forceinline__ ExtWave ext_wave(char *base, uint32_t kmax, uint32_t kw,
+                                            bool ent = false)
 {
     ExtWave w;
     w.keys = (uint64_t *) base;
-    w.meta = (ItemMeta *) (base + (size_t) kmax * kw * 8);
+    char *p = base + (size_t) kmax * kw * 8;
+    w.elc = nullptr;
+    w.eo = nullptr;
+    if (ent) {
+        w.elc = (uint2 *) p;
+        p += (size_t) kmax * 2 * sizeof(uint2);
+    }
+    w.meta = (ItemMeta *) p;
     w.pre = (uint32_t *) ((char *) w.meta + (size_t) kmax * sizeof(ItemMeta));
     w.lcp = (uint16_t *) (w.pre + kmax + 2);
-    w.emitf = (uint8_t *) (w.lcp + kmax);
+    if (ent) {
+        w.eo = w.lcp + kmax;
+        w.emitf = (uint8_t *) (w.eo + 2 * kmax);
+    } else {
+        w.emitf = (uint8_t *) (w.lcp + kmax);
+    }
     return w;
+}
+
+// Process one read with one group of GW lanes whose candidate entries are read
+// once into LDS (w.elc / w.eo): both sides' entries in one round of loads, then
+// per side only the two dependent loads of its items (reads_index, the key's
+// text).  Same results and limits as extract_read.
+template <uint32_t KMAX, int GW, typename R>
+__device__ bool extract_read_ent(const Geom &G, const DirViewT<R> &fw, const DirViewT<R> &bw,
+                                 const ChainBuf &cb, uint64_t rl, uint64_t r, uint32_t kw,
+                                 const ExtWave &w, const ExtOut &out, uint32_t &items)
+{
+    constexpr uint32_t kmax = KMAX;
+    constexpr int IPL = (int) (KMAX / GW);   // items (and entries) per lane and side
+    const int lane = threadIdx.x & (GW - 1);
+    const uint4 c4 = *(const uint4 *) (cb.n + rl * 4);
+    const uint32_t cnt4[4] = {c4.x, c4.y, c4.z, c4.w};
+    items = 0;
+    // side 0 = B (chains 2: BF, 3: BB), side 1 = F (chains 0: FB, 1: FF); an entry
+    // holds at least one item, so a side of more than kmax entries overflows
+    const uint32_t ne0 = cnt4[2] + cnt4[3], ne1 = cnt4[0] + cnt4[1];
+    if (ne0 > kmax || ne1 > kmax)
+        return false;
+    // both sides' entries in one round of independent loads, then into LDS
+    uint32_t k0 = 0, k1 = 0;
+    uint2 lcs[2 * IPL];
+    uint16_t os[2 * IPL];
+#pragma unroll
+    for (int h = 0; h < 2 * IPL; h++) {
+        const uint32_t e = (uint32_t) lane + (uint32_t) h * GW;
+        lcs[h] = make_uint2(0, 0);
+        os[h] = 0;
+        if (e < ne0 + ne1) {
+            const bool s1 = e >= ne0;
+            const uint32_t es = s1 ? e - ne0 : e;
+            const int chA = s1 ? 0 : 2, chB = s1 ? 1 : 3;
+            const uint32_t nA = cnt4[chA];
+            const uint64_t at = es < nA ? (rl * 4 + chA) * cb.cap + es
+                                        : (rl * 4 + chB) * cb.cap + (es - nA);
+            lcs[h] = cb.lc[at];
+            os[h] = cb.o[at];
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2 * IPL; h++) {
+        const uint32_t e = (uint32_t) lane + (uint32_t) h * GW;
+        if (e < ne0 + ne1) {
+            const bool s1 = e >= ne0;
+            const uint32_t at = s1 ? kmax + (e - ne0) : e;
+            w.elc[at] = lcs[h];
+            w.eo[at] = os[h];
+            if (s1)
+                k1 += entry_count(lcs[h]);
+            else
+                k0 += entry_count(lcs[h]);
+        }
+    }
+    for (int d = GW / 2; d > 0; d >>= 1) {
+        k0 += __shfl_xor(k0, d, GW);
+        k1 += __shfl_xor(k1, d, GW);
+    }
+    if (k0 > kmax || k1 > kmax)
+        return false;
+    wave_sync();
+    for (int side = 0; side < 2; side++) {
+        const int chA = side ? 0 : 2, chB = side ? 1 : 3;
+        const uint32_t nA = cnt4[chA], ne = side ? ne1 : ne0;
+        const uint2 *elc = w.elc + (side ? kmax : 0);
+        const uint16_t *eo = w.eo + (side ? kmax : 0);
+        // exclusive prefix of the counts over the side's entries
+        uint32_t carry = 0;
+        for (uint32_t e0 = 0; e0 < ne; e0 += GW) {
+            const uint32_t e = e0 + lane;
+            const uint32_t v = e < ne ? entry_count(elc[e]) : 0u;
+            uint32_t inc = v;
+            for (int d = 1; d < GW; d <<= 1) {
+                uint32_t x = __shfl_up(inc, d, GW);
+                if (lane >= d)
+                    inc += x;
+            }
+            if (e < ne)
+                w.pre[e] = carry + inc - v;
+            carry += __shfl(inc, GW - 1, GW);
+        }
+        const uint32_t k = carry;
+        items += k;
+        wave_sync();
+        auto entry_of = [&](uint32_t q) -> uint32_t {   // e with pre[e] <= q < pre[e+1]
+            uint32_t lo = 0, hi = ne;
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (w.pre[mid] <= q)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            return lo;
+        };
+        // every item of the lane at once (at most kmax / GW): reads_index, then the
+        // key's text, each one round of independent loads
+        for (uint32_t q0 = lane; q0 < k; q0 += IPL * GW) {
+            uint64_t mm[IPL];
+            uint32_t oo[IPL];
+            bool firsts[IPL];
+#pragma unroll
+            for (int h = 0; h < IPL; h++) {
+                const uint32_t q = q0 + (uint32_t) h * GW;
+                mm[h] = 0;
+                oo[h] = 0;
+                firsts[h] = true;
+                if (q < k) {
+                    const uint32_t e = entry_of(q);
+                    firsts[h] = e < nA;
+                    const uint2 lc = elc[e];
+                    oo[h] = eo[e];
+                    // FB, BB searched the forward index; FF, BF the backward index
+                    const int ch = firsts[h] ? chA : chB;
+                    const uint32_t *ri = (ch == 0 || ch == 3) ? fw.reads_index : bw.reads_index;
+                    mm[h] = entry_read(lc, q - w.pre[e], ri);
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < IPL; h++) {
+                const uint32_t q = q0 + (uint32_t) h * GW;
+                if (q < k) {
+                    const int ch = firsts[h] ? chA : chB;
+                    item_fill(G, fw, bw, ch, firsts[h] ? 0 : 1, oo[h], mm[h], kw,
+                              w.keys + (uint64_t) q * kw, w.meta + q);
+                }
+            }
+        }
+        wave_sync();
+        uint32_t P = 1;
+        while (P < kmax)
+            P <<= 1;
+        extract_sorted<GW>(r, side, kw, k, P, w.keys, w.meta, w.emitf, w.pre, w.lcp, out);
+    }
+    return true;
 }
 
 // One group of GW lanes per read, storage in LDS, WAVES waves per block.  Reads
@@ -1079,14 +1235,19 @@ __global__ void __launch_bounds__(64 * WAVES) k_extract_lds(Geom G, DirViewT<R> 
     constexpr int GROUPS = WAVES * 64 / GW;
     const int grp = threadIdx.x / GW;
     const uint64_t total = list ? (uint64_t) *count : nr;
-    ExtWave w = ext_wave((char *) lds + ext_wave_bytes(KMAX, kw) * grp, KMAX, kw);
+    constexpr bool ENT = GW < 64;
+    ExtWave w = ext_wave((char *) lds + ext_wave_bytes(KMAX, kw, ENT) * grp, KMAX, kw, ENT);
     uint64_t nitems = 0, nreads = 0;
     for (uint64_t i = (uint64_t) blockIdx.x * GROUPS + grp; i < total;
          i += (uint64_t) gridDim.x * GROUPS) {
         uint64_t rl = list ? list[i] : i;
         uint32_t items;
-        bool ok = extract_read<GW, R>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
-                                      w.pre, w.lcp, KMAX, out, items);
+        bool ok;
+        if constexpr (ENT)
+            ok = extract_read_ent<KMAX, GW, R>(G, fw, bw, cb, rl, r0 + rl, kw, w, out, items);
+        else
+            ok = extract_read<GW, R>(G, fw, bw, cb, rl, r0 + rl, kw, w.keys, w.meta, w.emitf,
+                                     w.pre, w.lcp, KMAX, out, items);
         if (ok) {
             nitems += items;
             nreads++;
@@ -1803,7 +1964,7 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
     auto launch_tier = [&](uint32_t i, uint64_t r0, uint64_t nr) {
         const TierShape ts = TIERS[first_tier + i];
         const uint32_t groups = ts.waves * 64 / ts.gw;
-        const size_t lds = ext_wave_bytes(ts.kmax, kw) * groups;
+        const size_t lds = ext_wave_bytes(ts.kmax, kw, ts.gw < 64) * groups;
         const uint32_t *list = i ? eoT[i - 1].overflow_reads : nullptr;
         const uint32_t *cnt = i ? eoT[i - 1].n_overflow : nullptr;
         const dim3 grid(i ? (uint32_t) std::min<uint64_t>(pq_blocks(batch, groups), 4096)
