@@ -251,26 +251,6 @@ __device__ __forceinline__ bool suf_less(const SufW<W, R> &a, const SufW<W, R> &
     return a.g < b.g;
 }
 
-// The value of lane l ^ J.  J = 1, 2: a DPP quad permutation inside the VALU;
-// J < 32: ds_swizzle in bit-mask mode (xor within 32 lanes, no address
-// arithmetic); 32: ds_bpermute.
-template <int J> __device__ __forceinline__ uint32_t xor_lane(uint32_t v)
-{
-    if constexpr (J == 1)
-        return (uint32_t) __builtin_amdgcn_mov_dpp((int) v, 0xB1, 0xF, 0xF, true);
-    else if constexpr (J == 2)
-        return (uint32_t) __builtin_amdgcn_mov_dpp((int) v, 0x4E, 0xF, 0xF, true);
-    else if constexpr (J < 32)
-        return (uint32_t) __builtin_amdgcn_ds_swizzle((int) v, (J << 10) | 0x1F);
-    else
-        return (uint32_t) __shfl_xor((int) v, J, 64);
-}
-
-template <int J> __device__ __forceinline__ uint64_t xor_lane64(uint64_t v)
-{
-    return ((uint64_t) xor_lane<J>((uint32_t) (v >> 32)) << 32) | xor_lane<J>((uint32_t) v);
-}
-
 template <int J, typename R> __device__ __forceinline__ R xor_lane_r(R v)
 {
     if constexpr (sizeof(R) == 8)

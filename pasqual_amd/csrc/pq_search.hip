@@ -38,7 +38,7 @@
 // large LDS share, and the rest to global-memory scratch
 #define EXT_WAVES 4
 // first-tier shapes the autotuner chooses from: lanes per read, items per side
-static const uint32_t EXT_SHAPES[3][2] = {{8, 16}, {16, 32}, {32, 64}};
+static const uint32_t EXT_SHAPES[3][2] = {{4, 16}, {8, 32}, {16, 64}};
 #define EXT_NSHAPES 3
 #define KMAX_LDS2 512
 #define KMAX_GLOBAL 65536
@@ -1218,6 +1218,437 @@ __device__ bool extract_read_ent(const Geom &G, const DirViewT<R> &fw, const Dir
     return true;
 }
 
+// ---- register tiers ---------------------------------------------------------------------
+//
+// A read side of at most G * RI items is sorted by a group of G lanes holding
+// RI items each in registers (item i at lane i / RI, slot i % RI): the composite
+// key (the key's codes left-aligned in KW words, zero past its length; then
+// klen << 49 | rank << 48 | o << 32 | m) orders items as item_less does (a key
+// that is a proper prefix of another sorts first, which the zero padding and the
+// length word give).  The bitonic network compares slots inside a lane and
+// exchanges lanes by DPP and ds_swizzle.  Items that are no candidate (contained
+// reads, empty slots) carry all-ones keys and sort last.  The emission rules of
+// extract_emit then walk the sorted items with every lane of the group holding
+// the same state (item values broadcast inside the group).  Small groups keep
+// many reads in flight per wave: the extraction waits mostly on its scattered
+// reads (reads_index, key text).
+
+#define RI 4
+
+template <int KW> struct RegItem {
+    uint64_t w[KW];
+    uint64_t tie;
+};
+
+template <int KW>
+__device__ __forceinline__ bool reg_less(const RegItem<KW> &a, const RegItem<KW> &b)
+{
+#pragma unroll
+    for (int i = 0; i < KW; i++)
+        if (a.w[i] != b.w[i])
+            return a.w[i] < b.w[i];
+    return a.tie < b.tie;
+}
+
+template <int J, int KW> __device__ __forceinline__ RegItem<KW> reg_xor(const RegItem<KW> &s)
+{
+    RegItem<KW> o;
+#pragma unroll
+    for (int i = 0; i < KW; i++)
+        o.w[i] = xor_lane64<J>(s.w[i]);
+    o.tie = xor_lane64<J>(s.tie);
+    return o;
+}
+
+// Step J of merge stage K over the G * RI items of a group.
+template <int K, int J, int KW>
+__device__ __forceinline__ void reg_step(RegItem<KW> (&it)[RI], int li)
+{
+    if constexpr (J < RI) {
+        // partners inside the lane
+#pragma unroll
+        for (int h = 0; h < RI; h++) {
+            if ((h & J) == 0) {
+                const int i = li * RI + h;
+                const bool up = (i & K) == 0;
+                if (reg_less<KW>(it[h | J], it[h]) == up) {
+                    const RegItem<KW> x = it[h];
+                    it[h] = it[h | J];
+                    it[h | J] = x;
+                }
+            }
+        }
+    } else {
+        // partner lane li ^ (J / RI), same slot
+#pragma unroll
+        for (int h = 0; h < RI; h++) {
+            const RegItem<KW> p = reg_xor<J / RI, KW>(it[h]);
+            const int i = li * RI + h;
+            const bool up = (i & K) == 0, lower = (i & J) == 0;
+            const bool swap = (lower == up) ? reg_less<KW>(p, it[h]) : reg_less<KW>(it[h], p);
+            if (swap)
+                it[h] = p;
+        }
+    }
+    if constexpr (J > 1)
+        reg_step<K, J / 2, KW>(it, li);
+}
+
+template <int N, int K, int KW>
+__device__ __forceinline__ void reg_sort(RegItem<KW> (&it)[RI], int li)
+{
+    reg_step<K, K / 2, KW>(it, li);
+    if constexpr (K < N)
+        reg_sort<N, 2 * K, KW>(it, li);
+}
+
+// The value of lane j of this lane's group of G lanes.
+template <int G, int J> __device__ __forceinline__ uint32_t group_bcast(uint32_t v)
+{
+    if constexpr (G == 4)
+        return quad_bcast<J>(v);
+    else if constexpr (G <= 32)
+        return (uint32_t) __builtin_amdgcn_ds_swizzle((int) v, (J << 5) | (0x1F & ~(G - 1)));
+    else
+        return (uint32_t) __shfl((int) v, J, 64);
+}
+
+// The value of lane li - 1 of the group (lane 0: its own).
+template <int G> __device__ __forceinline__ uint64_t group_prev64(uint64_t v)
+{
+    if constexpr (G == 4) {
+        // quad permutation (0, 0, 1, 2)
+        const uint32_t lo = (uint32_t) __builtin_amdgcn_mov_dpp((int) (uint32_t) v, 0x90, 0xF, 0xF, false);
+        const uint32_t hi = (uint32_t) __builtin_amdgcn_mov_dpp((int) (uint32_t) (v >> 32), 0x90, 0xF, 0xF, false);
+        return ((uint64_t) hi << 32) | lo;
+    } else {
+        return __shfl_up(v, 1, G);
+    }
+}
+
+// Per-group LDS of the register tiers: both sides' entries and entry prefixes.
+template <int N> struct RegEnt {
+    uint32_t pre[2][N + 1];
+    uint2 lc[2][N];
+    uint16_t o[2][N];
+};
+
+// State of the ordered pass (extract_emit's rules), the same in every lane of
+// a group.
+struct EmitState {
+    uint32_t M, prevlen, bf, emitted;
+};
+
+template <int G, int T, int KW>
+__device__ __forceinline__ void reg_emit_steps(const uint32_t (&p1)[RI], const uint32_t (&po)[RI],
+                                               uint32_t nv, uint32_t tmax, int li, EmitState &st,
+                                               uint32_t (&pos)[RI], uint32_t &keepbits)
+{
+    constexpr int N = G * RI;
+    if (T < (int) tmax) {
+        const uint32_t x1 = group_bcast<G, T / RI>(p1[T % RI]);
+        const uint32_t x2 = group_bcast<G, T / RI>(po[T % RI]);
+        if ((uint32_t) T < nv) {
+            const uint32_t Lt = x1 & 511u, At = (x1 >> 9) & 511u, Rt = (x1 >> 18) & 1u,
+                           St = (x1 >> 19) & 1u;
+            const bool first = T == 0;
+            const bool chain = !first && st.M <= At;
+            const bool dropped = chain && st.M < Lt;
+            const bool same_key = !first && At == Lt && At == st.prevlen;
+            const uint32_t ro = (Rt << 16) | x2;
+            if (!same_key)
+                st.bf = ro;
+            const bool keep = !dropped && ro == st.bf && !St;
+            st.M = chain ? (st.M < Lt ? st.M : Lt) : Lt;
+            st.prevlen = Lt;
+            if (keep) {
+                if (li == T / RI) {
+                    keepbits |= 1u << (T % RI);
+                    pos[T % RI] = st.emitted;
+                }
+                st.emitted++;
+            }
+        }
+        if constexpr (T + 1 < N)
+            reg_emit_steps<G, T + 1, KW>(p1, po, nv, tmax, li, st, pos, keepbits);
+    }
+}
+
+// One read side (side 0 = B: chains BF, BB; side 1 = F: FB, FF) by a group of G
+// lanes: ne entries with k items (both <= G * RI) staged in e.  Sorts, applies
+// the emission rules, writes the side's counts and edges (a group without
+// `write` takes part in the group-wide steps with no items).
+template <int G, int KW, typename R>
+__device__ void reg_side(const Geom &Gm, const DirViewT<R> &fw, const DirViewT<R> &bw,
+                         uint64_t r, int side, uint32_t nA, uint32_t ne, uint32_t k,
+                         const RegEnt<G * RI> &e, bool write, const ExtOut &out)
+{
+    constexpr int N = G * RI;
+    const int lane = (int) (threadIdx.x & 63), li = lane & (G - 1);
+    const int chA = side ? 0 : 2, chB = side ? 1 : 3;
+    const uint32_t *pre = e.pre[side];
+    RegItem<KW> it[RI];
+#pragma unroll
+    for (int h = 0; h < RI; h++) {
+#pragma unroll
+        for (int i = 0; i < KW; i++)
+            it[h].w[i] = ~0ull;
+        it[h].tie = ~0ull;
+    }
+    // items li * RI + h: entry (the last e with pre[e] <= q), read, key; every
+    // slot's loads independent of the others'
+    uint64_t mm[RI];
+    uint32_t oo[RI];
+    bool fst[RI];
+#pragma unroll
+    for (int h = 0; h < RI; h++) {
+        const uint32_t q = (uint32_t) (li * RI + h);
+        mm[h] = 0;
+        oo[h] = 0;
+        fst[h] = true;
+        if (q < k) {
+            uint32_t lo = 0, hi = ne;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (pre[mid] <= q)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const uint2 lc = e.lc[side][lo];
+            oo[h] = e.o[side][lo];
+            fst[h] = lo < nA;
+            const int ch = fst[h] ? chA : chB;
+            // FB, BB searched the forward index; FF, BF the backward index
+            const bool plain = (ch == 0 || ch == 3);
+            mm[h] = entry_read(lc, q - pre[lo], plain ? fw.reads_index : bw.reads_index);
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < RI; h++) {
+        const uint32_t q = (uint32_t) (li * RI + h);
+        if (q < k) {
+            const int ch = fst[h] ? chA : chB;
+            const bool plain = (ch == 0 || ch == 3);
+            const uint32_t Lm = pq_read_len(Gm, mm[h]);
+            if (oo[h] < Lm) {
+                // key m[o..) (FB, BB) or rc(m)[o..) (FF, BF: the complement of the
+                // backward text), zero past its length
+                const uint32_t klen = Lm - oo[h];
+                const uint64_t *tx = plain ? fw.text2 : bw.text2;
+                const uint64_t ks = pq_read_start(Gm, mm[h]) + oo[h];
+#pragma unroll
+                for (int i = 0; i < KW; i++) {
+                    uint64_t x = 0;
+                    if (32u * i < klen) {
+                        x = pq_window(tx, ks + 32u * i);
+                        if (!plain)
+                            x = ~x;
+                        const uint32_t left = klen - 32u * i;
+                        if (left < 32)
+                            x &= ~0ull << (64 - 2 * left);
+                    }
+                    it[h].w[i] = x;
+                }
+                it[h].tie = ((uint64_t) klen << 49) | ((uint64_t) (fst[h] ? 0u : 1u) << 48) |
+                            ((uint64_t) oo[h] << 32) | (uint64_t) (uint32_t) mm[h];
+            }
+        }
+    }
+    reg_sort<N, 2, KW>(it, li);
+    // sorted position li * RI + h: length, codes shared with the previous item
+    uint32_t p1[RI], po[RI];
+    uint32_t nvl = 0;
+#pragma unroll
+    for (int h = 0; h < RI; h++) {
+        RegItem<KW> pv;
+        if (h > 0) {
+            pv = it[h - 1];
+        } else {
+#pragma unroll
+            for (int i = 0; i < KW; i++)
+                pv.w[i] = group_prev64<G>(it[RI - 1].w[i]);
+            pv.tie = group_prev64<G>(it[RI - 1].tie);
+        }
+        const bool valid = it[h].tie != ~0ull;
+        nvl += valid ? 1u : 0u;
+        const uint32_t L = valid ? (uint32_t) (it[h].tie >> 49) : 0u;
+        const uint32_t Lp = (uint32_t) (pv.tie >> 49);
+        const uint32_t mn = L < Lp ? L : Lp;
+        uint32_t l = mn;
+#pragma unroll
+        for (int i = KW - 1; i >= 0; i--) {
+            const uint64_t x = pv.w[i] ^ it[h].w[i];
+            if (x)
+                l = 32u * i + (uint32_t) __clzll((long long) x) / 2;
+        }
+        const uint32_t A = (li == 0 && h == 0) ? 0u : (l < mn ? l : mn);
+        const uint32_t mi = (uint32_t) it[h].tie, rk = (uint32_t) (it[h].tie >> 48) & 1u;
+        const uint32_t self = valid && mi == (uint32_t) r ? 1u : 0u;
+        p1[h] = L | (A << 9) | (rk << 18) | (self << 19);
+        po[h] = (uint32_t) (it[h].tie >> 32) & 0xFFFFu;
+    }
+    // valid items of the group (they sort first) and the wave's largest count
+    uint32_t nv = nvl;
+#pragma unroll
+    for (int d = G / 2; d > 0; d >>= 1)
+        nv += __shfl_xor(nv, d, G);
+    uint32_t tmax = nv;
+#pragma unroll
+    for (int d = 32; d >= G; d >>= 1) {
+        const uint32_t x = __shfl_xor(tmax, d, 64);
+        tmax = x > tmax ? x : tmax;
+    }
+    tmax = (uint32_t) __builtin_amdgcn_readfirstlane((int) tmax);
+    EmitState st = {0, 0, 0, 0};
+    uint32_t pos[RI] = {0, 0, 0, 0}, keepbits = 0;
+    reg_emit_steps<G, 0, KW>(p1, po, nv, tmax, li, st, pos, keepbits);
+    const uint32_t em = st.emitted;
+    uint64_t base = 0;
+    uint64_t *dst = out.staged + (r * 2 + side) * EDGE_SLOT;
+    if (li == 0 && write) {
+        if (em > EDGE_SLOT) {
+            base = atomicAdd(out.edge_count, (unsigned long long) em);
+            if (base + em > out.edge_cap)
+                atomicOr(out.flags, 1u);
+        }
+        if (side) {
+            out.ovF[r] = (int32_t) nv;
+            out.numF[r] = em;
+            out.recF[r] = base;
+        } else {
+            out.ovB[r] = (int32_t) nv;
+            out.numB[r] = em;
+            out.recB[r] = base;
+        }
+    }
+    base = ((uint64_t) group_bcast<G, 0>((uint32_t) (base >> 32)) << 32) |
+           group_bcast<G, 0>((uint32_t) base);
+    uint64_t cap = EDGE_SLOT;
+    if (em > EDGE_SLOT) {
+        dst = out.edges + base;
+        cap = base < out.edge_cap ? out.edge_cap - base : 0;
+    }
+    if (write) {
+#pragma unroll
+        for (int h = 0; h < RI; h++) {
+            if ((keepbits >> h) & 1u && pos[h] < cap) {
+                const uint32_t rk = (uint32_t) (it[h].tie >> 48) & 1u;
+                // edge type: FB 1, FF 2 (side F), BF 0, BB 3 (side B)
+                const uint32_t type = side ? (rk ? 2u : 1u) : (rk ? 3u : 0u);
+                dst[pos[h]] = ((uint64_t) (uint32_t) it[h].tie << 32) |
+                              ((uint64_t) po[h] << 8) | type;
+            }
+        }
+    }
+}
+
+// One group of G lanes per read (G * RI = 16, 32 or 64 items per side), items
+// in registers, 256-thread blocks; reads [0, nr) of the batch or the *count reads
+// of `list` (grid-stride, the loop bound the wave's).  A read with a side of
+// more than G * RI items goes to out.overflow_reads for the next tier.
+template <int G, int KW, typename R>
+__global__ void __launch_bounds__(256) k_extract_reg(Geom Gm, DirViewT<R> fw, DirViewT<R> bw,
+                                                     ChainBuf cb, uint64_t r0, uint64_t nr,
+                                                     const uint32_t *list, const uint32_t *count,
+                                                     ExtOut out)
+{
+    constexpr int N = G * RI, GROUPS = 256 / G;
+    __shared__ RegEnt<N> shs[GROUPS];
+    const int lane = (int) (threadIdx.x & 63), li = lane & (G - 1);
+    const int grp = (int) threadIdx.x / G;
+    RegEnt<N> &E = shs[grp];
+    const uint64_t total = list ? (uint64_t) *count : nr;
+    uint64_t nitems = 0, nreads = 0;
+    const uint64_t wave_first =
+        (uint64_t) blockIdx.x * GROUPS + (uint64_t) (threadIdx.x / 64) * (64 / G);
+    for (uint64_t i0 = wave_first; i0 < total; i0 += (uint64_t) gridDim.x * GROUPS) {
+        const uint64_t i = i0 + (uint64_t) (lane / G);
+        const bool have = i < total;
+        const uint64_t rl = have ? (list ? list[i] : i) : 0;
+        uint4 c4 = make_uint4(0, 0, 0, 0);
+        if (have)
+            c4 = *(const uint4 *) (cb.n + rl * 4);
+        // side 0 = B (chains 2, 3), side 1 = F (chains 0, 1); an entry holds at
+        // least one item
+        const uint32_t ne[2] = {c4.z + c4.w, c4.x + c4.y};
+        const uint32_t nA[2] = {c4.z, c4.x};
+        bool ok = have && ne[0] <= (uint32_t) N && ne[1] <= (uint32_t) N;
+        // both sides' entries (lane li: entries li * RI .. li * RI + RI - 1 of each
+        // side) in one round of loads
+        uint2 lcs[2][RI];
+        uint16_t os[2][RI];
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++)
+#pragma unroll
+            for (int j = 0; j < RI; j++) {
+                const uint32_t en = (uint32_t) (li * RI + j);
+                lcs[sd][j] = make_uint2(0, 0);
+                os[sd][j] = 0;
+                if (ok && en < ne[sd]) {
+                    const int cA = sd ? 0 : 2, cB = sd ? 1 : 3;
+                    const uint64_t at = en < nA[sd] ? (rl * 4 + cA) * cb.cap + en
+                                                    : (rl * 4 + cB) * cb.cap + (en - nA[sd]);
+                    lcs[sd][j] = cb.lc[at];
+                    os[sd][j] = cb.o[at];
+                }
+            }
+        // items per side and each entry's exclusive prefix, into LDS
+        uint32_t k[2];
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++) {
+            uint32_t part = 0;
+#pragma unroll
+            for (int j = 0; j < RI; j++)
+                part += (uint32_t) (li * RI + j) < ne[sd] ? entry_count(lcs[sd][j]) : 0u;
+            uint32_t inc = part;
+#pragma unroll
+            for (int d = 1; d < G; d <<= 1) {
+                const uint32_t x = __shfl_up(inc, d, G);
+                if (li >= d)
+                    inc += x;
+            }
+            k[sd] = (uint32_t) __shfl((int) inc, G - 1, G);
+            uint32_t run = inc - part;
+#pragma unroll
+            for (int j = 0; j < RI; j++) {
+                const uint32_t en = (uint32_t) (li * RI + j);
+                if (en < ne[sd]) {
+                    E.pre[sd][en] = run;
+                    E.lc[sd][en] = lcs[sd][j];
+                    E.o[sd][en] = os[sd][j];
+                    run += entry_count(lcs[sd][j]);
+                }
+            }
+        }
+        ok = ok && k[0] <= (uint32_t) N && k[1] <= (uint32_t) N;
+        wave_sync();
+        const uint64_t r = r0 + rl;
+        reg_side<G, KW, R>(Gm, fw, bw, r, 0, nA[0], ok ? ne[0] : 0u, ok ? k[0] : 0u, E, ok, out);
+        reg_side<G, KW, R>(Gm, fw, bw, r, 1, nA[1], ok ? ne[1] : 0u, ok ? k[1] : 0u, E, ok, out);
+        wave_sync();   // the next reads overwrite the entries
+        if (ok) {
+            nitems += k[0] + k[1];
+            nreads++;
+        }
+        // reads left for the next tier, appended with one atomic per wave
+        const bool ov = have && !ok && li == 0;
+        const uint64_t bal = __ballot(ov);
+        if (bal) {
+            const int lead = __ffsll((long long) bal) - 1;
+            uint32_t b = 0;
+            if (lane == lead)
+                b = atomicAdd(out.n_overflow, (uint32_t) __popcll(bal));
+            b = __shfl(b, lead, 64);
+            if (ov)
+                out.overflow_reads[b + (uint32_t) __popcll(bal & ((1ull << lane) - 1))] =
+                    (uint32_t) rl;
+        }
+    }
+    if (out.xstats)
+        ext_count<G>(out, nitems, nreads);
+}
+
 // One group of GW lanes per read, storage in LDS, WAVES waves per block.  Reads
 // [0, nr) of the batch, or (list != nullptr) the *count reads of `list`, taken
 // grid-stride so that the grid need not match the count known only on the
@@ -1932,7 +2363,7 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
     hipLaunchKernelGGL(k_wstat_clear, dim3(WSTAT_SLOTS), dim3(64), 0, ctx->stream, wstats,
                        (uint32_t) WSTAT_WORDS);
     const uint32_t shape = ctx->ext_shape < EXT_NSHAPES ? ctx->ext_shape : 1;
-    const uint32_t gw1 = EXT_SHAPES[shape][0], kmax1 = EXT_SHAPES[shape][1];
+    const uint32_t kmax1 = EXT_SHAPES[shape][1];
     // extraction tiers: the first (the tuned shape) over every read of the batch,
     // then each later tier over the previous tier's overflow list, read with its
     // count from the device (no host round trip): (items, lanes per read, waves
@@ -1941,10 +2372,11 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
     struct TierShape {
         uint32_t kmax, gw, waves;
     };
+    // items = lanes: the register tiers (one item per lane); then LDS tiers
     static const TierShape TIERS[5] = {
-        {16, 8, EXT_WAVES}, {32, 16, EXT_WAVES}, {64, 32, EXT_WAVES}, {256, 64, 1}, {512, 64, 1}};
+        {16, 4, 4}, {32, 8, 4}, {64, 16, 4}, {256, 64, 1}, {512, 64, 1}};
     uint32_t first_tier = 0;
-    while (first_tier < 4 && TIERS[first_tier].kmax < kmax1)
+    while (first_tier < 2 && TIERS[first_tier].kmax < kmax1)
         first_tier++;
     const uint32_t ntiers = 5 - first_tier;
     // tier i writes its overflow list at ovl + i * batch and its count at misc[40 + i]
@@ -1954,43 +2386,71 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
         eoT[i] = eo;
         eoT[i].overflow_reads = ovl + i * batch;
         eoT[i].n_overflow = tier_cnt + i;
-        // work counters per tier class: the first, the multi-read ones after it,
-        // the one-wave ones
+        // work counters per tier class: the first, the register ones after it,
+        // the LDS ones
         const uint32_t kmx = TIERS[first_tier + i].kmax;
         eoT[i].tier = i == 0 ? 0u : kmx <= 64 ? 1u : 2u;
     }
     ExtOut eoG = eo;
     eoG.tier = 3;
+    // key words of the register tiers: 1, 2, 3, 4 or 8 (zero words past a key)
+    const uint32_t kwr = kw <= 4 ? kw : 8;
     auto launch_tier = [&](uint32_t i, uint64_t r0, uint64_t nr) {
         const TierShape ts = TIERS[first_tier + i];
         const uint32_t groups = ts.waves * 64 / ts.gw;
-        const size_t lds = ext_wave_bytes(ts.kmax, kw, ts.gw < 64) * groups;
         const uint32_t *list = i ? eoT[i - 1].overflow_reads : nullptr;
         const uint32_t *cnt = i ? eoT[i - 1].n_overflow : nullptr;
         const dim3 grid(i ? (uint32_t) std::min<uint64_t>(pq_blocks(batch, groups), 4096)
                           : pq_blocks(nr, groups));
         const dim3 block(64 * ts.waves);
-        switch (ts.kmax) {
-        case 16:
-            hipLaunchKernelGGL((k_extract_lds<16, EXT_WAVES, 8, R>), grid, block, lds, ctx->stream,
-                               G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
-            break;
-        case 32:
-            hipLaunchKernelGGL((k_extract_lds<32, EXT_WAVES, 16, R>), grid, block, lds, ctx->stream,
-                               G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
-            break;
-        case 64:
-            hipLaunchKernelGGL((k_extract_lds<64, EXT_WAVES, 32, R>), grid, block, lds, ctx->stream,
-                               G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
-            break;
-        case 256:
+        if (ts.kmax <= 64 && kwr <= 4) {
+#define PQ_REG_TIER(GG, KK)                                                                    \
+    hipLaunchKernelGGL((k_extract_reg<GG, KK, R>), grid, block, 0, ctx->stream, G, fw, bw, cb, r0, \
+                       nr, list, cnt, eoT[i])
+#define PQ_REG_TIERS(GG)                                                                       \
+    switch (kwr) {                                                                             \
+    case 1: PQ_REG_TIER(GG, 1); break;                                                         \
+    case 2: PQ_REG_TIER(GG, 2); break;                                                         \
+    case 3: PQ_REG_TIER(GG, 3); break;                                                         \
+    case 4: PQ_REG_TIER(GG, 4); break;                                                         \
+    default: PQ_REG_TIER(GG, 4);                                                               \
+    }
+            if (ts.kmax == 16) {
+                PQ_REG_TIERS(4)
+            } else if (ts.kmax == 32) {
+                PQ_REG_TIERS(8)
+            } else {
+                PQ_REG_TIERS(16)
+            }
+#undef PQ_REG_TIERS
+#undef PQ_REG_TIER
+            return;
+        }
+        // keys of more than 4 words: the small tiers sort in LDS
+        if (ts.kmax <= 64) {
+            const uint32_t gw = ts.kmax / 2, gl = EXT_WAVES * 64 / gw;
+            const size_t ldsl = ext_wave_bytes(ts.kmax, kw, true) * gl;
+            const dim3 gridl(i ? (uint32_t) std::min<uint64_t>(pq_blocks(batch, gl), 4096)
+                               : pq_blocks(nr, gl));
+            const dim3 blockl(64 * EXT_WAVES);
+            if (ts.kmax == 16)
+                hipLaunchKernelGGL((k_extract_lds<16, EXT_WAVES, 8, R>), gridl, blockl, ldsl,
+                                   ctx->stream, G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
+            else if (ts.kmax == 32)
+                hipLaunchKernelGGL((k_extract_lds<32, EXT_WAVES, 16, R>), gridl, blockl, ldsl,
+                                   ctx->stream, G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
+            else
+                hipLaunchKernelGGL((k_extract_lds<64, EXT_WAVES, 32, R>), gridl, blockl, ldsl,
+                                   ctx->stream, G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
+            return;
+        }
+        const size_t lds = ext_wave_bytes(ts.kmax, kw) * groups;
+        if (ts.kmax == 256)
             hipLaunchKernelGGL((k_extract_lds<256, 1, 64, R>), grid, block, lds, ctx->stream, G, fw,
                                bw, cb, r0, nr, list, cnt, kw, eoT[i]);
-            break;
-        default:
+        else
             hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64, R>), grid, block, lds, ctx->stream,
                                G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
-        }
     };
     // per batch: seeds, candidates, then the extraction tiers; the later tiers
     // take their read lists and counts from the device (no host round trip)
