@@ -1310,7 +1310,7 @@ template <int G, int J> __device__ __forceinline__ uint32_t group_bcast(uint32_t
     else if constexpr (G <= 32)
         return (uint32_t) __builtin_amdgcn_ds_swizzle((int) v, (J << 5) | (0x1F & ~(G - 1)));
     else
-        return (uint32_t) __shfl((int) v, J, 64);
+        return (uint32_t) __builtin_amdgcn_readlane((int) v, J);   // the group is the wave
 }
 
 // The value of lane li - 1 of the group (lane 0: its own).
@@ -1379,7 +1379,7 @@ __device__ __forceinline__ void reg_emit_steps(const uint32_t (&p1)[RI], const u
 // the emission rules, writes the side's counts and edges (a group without
 // `write` takes part in the group-wide steps with no items).
 template <int G, int KW, typename R>
-__device__ void reg_side(const Geom &Gm, const DirViewT<R> &fw, const DirViewT<R> &bw,
+__device__ __forceinline__ void reg_side(const Geom &Gm, const DirViewT<R> &fw, const DirViewT<R> &bw,
                          uint64_t r, int side, uint32_t nA, uint32_t ne, uint32_t k,
                          const RegEnt<G * RI> &e, bool write, const ExtOut &out)
 {
@@ -1547,8 +1547,11 @@ __device__ void reg_side(const Geom &Gm, const DirViewT<R> &fw, const DirViewT<R
 // in registers, 256-thread blocks; reads [0, nr) of the batch or the *count reads
 // of `list` (grid-stride, the loop bound the wave's).  A read with a side of
 // more than G * RI items goes to out.overflow_reads for the next tier.
+// (launch bounds: the 16-item tier, which takes most reads, asks for 4 waves per
+// SIMD -- 128 vector registers; the others would spill there)
 template <int G, int KW, typename R>
-__global__ void __launch_bounds__(256) k_extract_reg(Geom Gm, DirViewT<R> fw, DirViewT<R> bw,
+__global__ void __launch_bounds__(256, (G == 4 ? 4 : 1)) k_extract_reg(Geom Gm, DirViewT<R> fw,
+                                                                        DirViewT<R> bw,
                                                      ChainBuf cb, uint64_t r0, uint64_t nr,
                                                      const uint32_t *list, const uint32_t *count,
                                                      ExtOut out)
@@ -2374,7 +2377,7 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
     };
     // items = lanes: the register tiers (one item per lane); then LDS tiers
     static const TierShape TIERS[5] = {
-        {16, 4, 4}, {32, 8, 4}, {64, 16, 4}, {256, 64, 1}, {512, 64, 1}};
+        {16, 4, 4}, {32, 8, 4}, {64, 16, 4}, {256, 64, 4}, {512, 64, 1}};
     uint32_t first_tier = 0;
     while (first_tier < 2 && TIERS[first_tier].kmax < kmax1)
         first_tier++;
@@ -2403,7 +2406,7 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
         const dim3 grid(i ? (uint32_t) std::min<uint64_t>(pq_blocks(batch, groups), 4096)
                           : pq_blocks(nr, groups));
         const dim3 block(64 * ts.waves);
-        if (ts.kmax <= 64 && kwr <= 4) {
+        if (ts.kmax <= 256 && kwr <= 4) {
 #define PQ_REG_TIER(GG, KK)                                                                    \
     hipLaunchKernelGGL((k_extract_reg<GG, KK, R>), grid, block, 0, ctx->stream, G, fw, bw, cb, r0, \
                        nr, list, cnt, eoT[i])
@@ -2419,8 +2422,10 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
                 PQ_REG_TIERS(4)
             } else if (ts.kmax == 32) {
                 PQ_REG_TIERS(8)
-            } else {
+            } else if (ts.kmax == 64) {
                 PQ_REG_TIERS(16)
+            } else {
+                PQ_REG_TIERS(64)
             }
 #undef PQ_REG_TIERS
 #undef PQ_REG_TIER
@@ -2444,10 +2449,11 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
                                    ctx->stream, G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
             return;
         }
-        const size_t lds = ext_wave_bytes(ts.kmax, kw) * groups;
+        const size_t lds = ext_wave_bytes(ts.kmax, kw) * (ts.kmax == 256 ? 1 : groups);
         if (ts.kmax == 256)
-            hipLaunchKernelGGL((k_extract_lds<256, 1, 64, R>), grid, block, lds, ctx->stream, G, fw,
-                               bw, cb, r0, nr, list, cnt, kw, eoT[i]);
+            hipLaunchKernelGGL((k_extract_lds<256, 1, 64, R>),
+                               dim3((uint32_t) std::min<uint64_t>(batch, 4096)), dim3(64), lds,
+                               ctx->stream, G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
         else
             hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64, R>), grid, block, lds, ctx->stream,
                                G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
