@@ -2223,7 +2223,27 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
     HugeRead *huge = nullptr;
     // five events per batch (seed, find, extract boundaries; the seed kernel's start
     // after the model's prediction), read after the loop
-    const uint64_t nbatch = (n + batch - 1) / batch;
+    // batch boundaries: equal batches of at most `batch` reads; a search that
+    // exports as it goes ends with a batch of about a quarter of the others, so
+    // that little transfer is left after its last kernels
+    std::vector<uint64_t> bnd(1, 0);
+    {
+        uint64_t head = n, last = 0;
+        if (xp && n >= (1u << 20)) {
+            const uint64_t k = (4 * n + 5 * batch - 1) / (5 * batch) > 0
+                                   ? (4 * n + 5 * batch - 1) / (5 * batch)
+                                   : 1;                       // full batches, last ~ size / 4
+            const uint64_t size = (4 * n + 4 * k) / (4 * k + 1);
+            last = n - std::min<uint64_t>(n - 1, k * size);
+            head = n - last;
+        }
+        const uint64_t nh = (head + batch - 1) / batch;
+        for (uint64_t b = 1; b <= nh; b++)
+            bnd.push_back(head * b / nh);
+        if (last)
+            bnd.push_back(n);
+    }
+    const uint64_t nbatch = bnd.size() - 1;
     std::vector<hipEvent_t> ev(5 * nbatch);
     for (auto &e : ev)
         (void) hipEventCreate(&e);
@@ -2377,7 +2397,7 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
     };
     // items = lanes: the register tiers (one item per lane); then LDS tiers
     static const TierShape TIERS[5] = {
-        {16, 4, 4}, {32, 8, 4}, {64, 16, 4}, {256, 64, 4}, {512, 64, 1}};
+        {16, 4, 4}, {32, 8, 4}, {64, 16, 4}, {256, 64, 1}, {512, 64, 1}};
     uint32_t first_tier = 0;
     while (first_tier < 2 && TIERS[first_tier].kmax < kmax1)
         first_tier++;
@@ -2406,7 +2426,7 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
         const dim3 grid(i ? (uint32_t) std::min<uint64_t>(pq_blocks(batch, groups), 4096)
                           : pq_blocks(nr, groups));
         const dim3 block(64 * ts.waves);
-        if (ts.kmax <= 256 && kwr <= 4) {
+        if (ts.kmax <= 64 && kwr <= 4) {
 #define PQ_REG_TIER(GG, KK)                                                                    \
     hipLaunchKernelGGL((k_extract_reg<GG, KK, R>), grid, block, 0, ctx->stream, G, fw, bw, cb, r0, \
                        nr, list, cnt, eoT[i])
@@ -2422,10 +2442,8 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
                 PQ_REG_TIERS(4)
             } else if (ts.kmax == 32) {
                 PQ_REG_TIERS(8)
-            } else if (ts.kmax == 64) {
-                PQ_REG_TIERS(16)
             } else {
-                PQ_REG_TIERS(64)
+                PQ_REG_TIERS(16)
             }
 #undef PQ_REG_TIERS
 #undef PQ_REG_TIER
@@ -2449,11 +2467,10 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
                                    ctx->stream, G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
             return;
         }
-        const size_t lds = ext_wave_bytes(ts.kmax, kw) * (ts.kmax == 256 ? 1 : groups);
+        const size_t lds = ext_wave_bytes(ts.kmax, kw) * groups;
         if (ts.kmax == 256)
-            hipLaunchKernelGGL((k_extract_lds<256, 1, 64, R>),
-                               dim3((uint32_t) std::min<uint64_t>(batch, 4096)), dim3(64), lds,
-                               ctx->stream, G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
+            hipLaunchKernelGGL((k_extract_lds<256, 1, 64, R>), grid, block, lds, ctx->stream, G, fw,
+                               bw, cb, r0, nr, list, cnt, kw, eoT[i]);
         else
             hipLaunchKernelGGL((k_extract_lds<KMAX_LDS2, 1, 64, R>), grid, block, lds, ctx->stream,
                                G, fw, bw, cb, r0, nr, list, cnt, kw, eoT[i]);
@@ -2461,7 +2478,7 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
     // per batch: seeds, candidates, then the extraction tiers; the later tiers
     // take their read lists and counts from the device (no host round trip)
     for (uint64_t b = 0; b < nbatch && rc == PQ_OK; b++) {
-        const uint64_t r0 = b * batch, nr = std::min(batch, n - r0);
+        const uint64_t r0 = bnd[b], nr = bnd[b + 1] - bnd[b];
         hipEvent_t *e = &ev[5 * b];
         uint32_t kseed = 0;
         if (timed)
