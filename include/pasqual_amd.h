@@ -212,7 +212,7 @@ typedef struct {
     uint32_t segments;       /* linear pieces per leaf (<= 65536); 0 (the default):
                                 about 8 sampled rows per piece, 16..65536 */
     uint32_t epochs;         /* Adam passes over the sampled training set, 1 */
-    uint32_t batch;          /* minibatch size (multiple of 64), 4096 */
+    uint32_t batch;          /* minibatch size (multiple of 64), 16384 */
     uint32_t samples;        /* training samples drawn from the sampled rows (multiple of 64) */
     float lr;                /* Adam step size (decays linearly to 1%) */
     uint32_t closed_form;    /* 1: solve the output layer by least squares before Adam */

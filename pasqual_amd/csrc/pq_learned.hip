@@ -1052,7 +1052,7 @@ extern "C" void pq_learned_default(pq_learned_cfg_t *cfg)
     cfg->leaves = 4096;
     cfg->segments = 0;
     cfg->epochs = 1;
-    cfg->batch = 4096;
+    cfg->batch = 16384;
     cfg->samples = 1u << 20;
     cfg->lr = 1e-5f;
     cfg->closed_form = 1;
