@@ -266,16 +266,18 @@ int pq_mlp_gradients(pq_ctx_t *ctx, uint32_t k, uint32_t hidden, const float *pa
 
 /* ---- autotuner ------------------------------------------------------------------------- */
 /* The first extraction tier processes one read per group of lanes holding up to
- * `items` overlap candidates per read side in LDS (larger sides fall through to
- * wider tiers); the best shape depends on the read set.  pq_autotune runs the
+ * `items` overlap candidates per read side (4 per lane in registers; in LDS for
+ * extension keys over 128 codes); larger sides fall through to wider tiers; the
+ * best shape depends on the read set (default: shape 0).  pq_autotune runs the
  * search (mode as pq_find_overlaps) once per shape on the uploaded, indexed reads,
  * keeps the fastest for later searches of this context and leaves that shape's
  * (identical) result in place; then the search kernel and the verification
  * threshold the same way.  pq_set_tuning forces a shape. */
 typedef struct {
     uint32_t shape;          /* chosen shape index */
-    uint32_t group_lanes;    /* lanes per read of that shape (8, 16, 32) */
-    uint32_t items;          /* candidates per side held in LDS (16, 32, 64) */
+    uint32_t group_lanes;    /* lanes per read of that shape (4, 8, 16; twice that
+                                for keys over 128 codes) */
+    uint32_t items;          /* candidates per side the first tier holds (16, 32, 64) */
     uint32_t nshapes;
     double extract_ms[4];    /* extraction time measured per shape */
     uint32_t find_chains;    /* candidate-search kernel chosen (1 to 4, below; 0 for

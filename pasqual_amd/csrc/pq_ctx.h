@@ -70,7 +70,7 @@ struct pq_ctx {
     bool searched = false;
 
     Learned *learned = nullptr;
-    uint32_t ext_shape = 1;        // first extraction tier: index into the tuned shapes
+    uint32_t ext_shape = 0;        // first extraction tier: index into the tuned shapes
     uint32_t find_chains = 0;      // candidate-search kernel (1, 2, 3; 0: by index size)
     uint64_t edge_cap_set = 0;     // shared edge buffer capacity, 0 automatic
     uint32_t verify_rows = 32;     // chains of at most this many rows at depth <= t are

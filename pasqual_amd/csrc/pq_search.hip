@@ -2385,7 +2385,7 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, misc, 48u);
     hipLaunchKernelGGL(k_wstat_clear, dim3(WSTAT_SLOTS), dim3(64), 0, ctx->stream, wstats,
                        (uint32_t) WSTAT_WORDS);
-    const uint32_t shape = ctx->ext_shape < EXT_NSHAPES ? ctx->ext_shape : 1;
+    const uint32_t shape = ctx->ext_shape < EXT_NSHAPES ? ctx->ext_shape : 0;
     const uint32_t kmax1 = EXT_SHAPES[shape][1];
     // extraction tiers: the first (the tuned shape) over every read of the batch,
     // then each later tier over the previous tier's overflow list, read with its

@@ -119,7 +119,7 @@ def test_graph_with_many_candidates_and_errors(ctx, G, n, lmin, lmax, err, seed)
         ctx.set_tuning(shape)
         ctx.find_overlaps()
         _same_graph(ctx.graph(), ex, f"deep with errors, shape {shape}")
-    ctx.set_tuning(1)
+    ctx.set_tuning(0)
 
 
 def _identical_copies(copies=3):
@@ -163,7 +163,7 @@ def _check_every_shape_and_kernel(ctx, want, where):
                 ctx.find_overlaps(mode)
                 _same_graph(ctx.graph(), want, f"{where}: shape {shape}, kernel {kernel}, "
                                                 f"mode {mode:#x}")
-    ctx.set_tuning(1)
+    ctx.set_tuning(0)
     ctx.set_find_chains(0)
 
 
@@ -257,7 +257,7 @@ def test_every_extraction_shape_gives_the_same_graph(ctx):
     assert 0 <= t["shape"] < 3 and len(t["extract_ms"]) == 3
     assert t["find_chains"] in (1, 2, 3, 4) and min(t["find_ms"]) > 0
     _same_graph(ctx.graph(), want, "autotuned")
-    ctx.set_tuning(1)
+    ctx.set_tuning(0)
     ctx.set_find_chains(0)
 
 

@@ -53,7 +53,7 @@ def test_graph_recaptured_for_new_arguments(ctx):
             for r in range(2):
                 ctx.find_overlaps(pq.SEARCH_PLAIN | GRAPH)
                 _same_graph(ctx.graph(), want, f"n={len(off) - 1} t={t} shape {shape} rep {r}")
-    ctx.set_tuning(1)
+    ctx.set_tuning(0)
 
 
 def test_learned_graph_replays_match_eager(ctx):
