@@ -178,6 +178,13 @@ int pq_graph_export(pq_ctx_t *ctx, pq_node_t *nodes, void *edges, uint32_t index
 int pq_find_overlaps_export(pq_ctx_t *ctx, int mode, pq_node_t *nodes, void *edges,
                             uint64_t edge_capacity, uint32_t index_bytes, uint64_t *n_edges,
                             uint64_t *overlap_len_sum);
+/* Allocates the device buffers a plain pq_find_overlaps_export of the uploaded,
+ * indexed reads will use (staging for about four edges per read), so that the
+ * search itself allocates nothing unless its graph is larger; drops the last
+ * search's result.  No counterpart in the reference; for callers that time the
+ * search apart from its set-up (construct_index prepares what
+ * construct_string_graph uses). */
+int pq_reserve_search(pq_ctx_t *ctx, uint32_t index_bytes);
 /* Reads per search batch (0: automatic, batches whose candidate buffers take a
  * third of the free device memory).  Smaller batches cost launches; the result
  * is the same for any batch size.  No counterpart in the reference, which

@@ -41,7 +41,7 @@ PQ_SYMBOLS = [
     "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read", "pq_probe_gather", "pq_set_edge_capacity",
     "pq_get_slot_sizes", "pq_graph_export", "pq_host_alloc", "pq_host_free",
     "pq_mlp_eval", "pq_set_verify", "pq_dedup_reads_noalloc", "pq_dev_alloc", "pq_dev_free", "pq_dev_upload", "pq_dev_download",
-    "pq_find_overlaps_export", "pq_set_search_batch",
+    "pq_find_overlaps_export", "pq_set_search_batch", "pq_reserve_search",
 ]
 NOALLOC_SCRATCH_BYTES = 32 << 20   # PQ_NOALLOC_SCRATCH_BYTES
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
@@ -185,6 +185,7 @@ def lib(width="u32"):
         "pq_graph_export": (I32, [P, P, P, U32, P]),
         "pq_find_overlaps_export": (I32, [P, I32, P, P, U64, U32, P, P]),
         "pq_set_search_batch": (I32, [P, U64]),
+        "pq_reserve_search": (I32, [P, U32]),
         "pq_host_alloc": (I32, [P, U64, C.POINTER(P)]),
         "pq_host_free": (None, [P]),
         "pq_mlp_eval": (I32, [P, I32, P, U32, I32, P]),
@@ -475,6 +476,10 @@ class Context:
             None if edges is None else edges.ptr, cap, index_bytes, C.byref(E),
             C.byref(total)), "find_overlaps_export")
         return int(E.value), int(total.value)
+
+    def reserve_search(self, index_bytes):
+        """Device buffers of a later plain export search (pq_reserve_search)."""
+        self._check(self._L.pq_reserve_search(self._h, int(index_bytes)), "reserve_search")
 
     def set_search_batch(self, reads):
         """Reads per search batch (0: automatic)."""

@@ -244,6 +244,10 @@ index_t *construct_index(char *single_file, char *pair_file1, char *pair_file2,
      * will need for about four edges per read) */
     stage(&g_stage_nodes, &g_stage_nodes_bytes, sizeof(pq_node_t) * ((uint64_t) ix->num_reads + 1));
     stage(&g_stage_edges, &g_stage_edges_bytes, sizeof(edge_t) * (4 * (uint64_t) ix->num_reads + 1));
+    /* and the device buffers of the search */
+    int rc = pq_reserve_search(g_ctx, (uint32_t) sizeof(INDEX_TYPE_T));
+    if (rc)
+        dev_fail("search buffers", rc);
     return ix;
 }
 

@@ -2157,6 +2157,7 @@ struct ExportSpec {
     uint32_t index_bytes;      // 4 or 8
     uint64_t len_sum;          // out: sum of the overlap lengths
     bool edges_short;          // out: the edges did not fit edge_capacity
+    bool reserve_only;         // allocate what the search needs, launch nothing
 };
 
 template <typename R>
@@ -2641,6 +2642,14 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
         cleanup();
         return rc;
     }
+    if (xp && xp->reserve_only) {
+        // staging for about four edges per read of a batch (grown on demand)
+        for (int i = 0; i < 2 && rc == PQ_OK; i++)
+            rc = pq_slot(ctx, SLOT_XSTAGE0 + i, 4 * batch * x_rec, &xstage[i]);
+        xcleanup();
+        cleanup();
+        return rc;
+    }
     hipEvent_t ce[2] = {nullptr, nullptr};
     (void) hipEventCreate(&ce[0]);
     (void) hipEventCreate(&ce[1]);
@@ -2774,6 +2783,7 @@ extern "C" int pq_find_overlaps_export(pq_ctx_t *ctx, int mode, pq_node_t *nodes
     xp.index_bytes = index_bytes;
     xp.len_sum = 0;
     xp.edges_short = false;
+    xp.reserve_only = false;
     int rc = find_overlaps_retry(ctx, mode, &xp);
     if (rc)
         return rc;
@@ -2786,6 +2796,20 @@ extern "C" int pq_find_overlaps_export(pq_ctx_t *ctx, int mode, pq_node_t *nodes
                                            " records, the graph has " +
                                            std::to_string(ctx->n_edges));
     return PQ_OK;
+}
+
+extern "C" int pq_reserve_search(pq_ctx_t *ctx, uint32_t index_bytes)
+{
+    if (!ctx || (index_bytes != 4 && index_bytes != 8))
+        return PQ_ERR_ARG;
+    if (!ctx->dir[0].built || !ctx->dir[1].built)
+        return ctx->fail(PQ_ERR_STATE, "index not built");
+    ExportSpec xp = {};
+    xp.index_bytes = index_bytes;
+    xp.reserve_only = true;
+    // (the previous search's result is dropped, as by any search)
+    return ctx->wide ? find_overlaps_t<uint64_t>(ctx, PQ_SEARCH_PLAIN, false, &xp)
+                     : find_overlaps_t<uint32_t>(ctx, PQ_SEARCH_PLAIN, false, &xp);
 }
 
 extern "C" int pq_set_search_batch(pq_ctx_t *ctx, uint64_t reads)

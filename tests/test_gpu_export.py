@@ -96,6 +96,9 @@ def test_export_during_search_pool_and_uneven(ctx):
     assert np.array_equal(want[1], _oracle_edges(ex))
     assert (np.diff(np.asarray(ex["edge_off"]).astype(np.int64)) > 32).any()
     _check_batches(ctx, pq.SEARCH_PLAIN, 8, want, [0, 257, 1000], "pool")
+    # buffers reserved ahead of the search (as construct_index does): same graph
+    ctx.reserve_search(8)
+    _check_batches(ctx, pq.SEARCH_PLAIN, 8, want, [0], "reserved")
 
 
 def test_export_edges_buffer_too_small(ctx):
