@@ -394,6 +394,9 @@ def run_workload(pq, np, config, args, cpu_baseline):
     ctx = pq.Context(0)
     ctx.upload(codes, off, t)
     del codes
+    if learned_on:
+        # the builds write the learned index's sampled keys as they sort
+        ctx.set_learned_keys(int(ctx.learned_defaults()["sample_stride"]))
     for _ in range(args.warmup):
         ctx.build_index()
         ctx.find_overlaps()

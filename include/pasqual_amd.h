@@ -212,6 +212,11 @@ void pq_host_free(void *p);
  * the way; a wider one by binary search.
  * Results equal PQ_SEARCH_PLAIN's for any model.  The model is dropped by
  * pq_upload_reads and pq_build_index. */
+/* Index builds from now on also write the sampled keys of the learned index
+ * (every stride-th suffix-array row; stride a power of two, 0 = no) while they
+ * sort, for a later pq_learned_train with that sample_stride, which then skips
+ * its own pass over the suffix array and the text.  Results are identical. */
+int pq_set_learned_keys(pq_ctx_t *ctx, uint32_t stride);
 typedef struct {
     uint32_t k;              /* prefix length fed to the model (<= 21; capped at D), 21 */
     uint32_t hidden;         /* MLP hidden width (<= 64), 64 */

@@ -41,7 +41,7 @@ PQ_SYMBOLS = [
     "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read", "pq_probe_gather", "pq_set_edge_capacity",
     "pq_get_slot_sizes", "pq_graph_export", "pq_host_alloc", "pq_host_free",
     "pq_mlp_eval", "pq_set_verify", "pq_dedup_reads_noalloc", "pq_dev_alloc", "pq_dev_free", "pq_dev_upload", "pq_dev_download",
-    "pq_find_overlaps_export", "pq_set_search_batch", "pq_reserve_search",
+    "pq_find_overlaps_export", "pq_set_search_batch", "pq_reserve_search", "pq_set_learned_keys",
 ]
 NOALLOC_SCRATCH_BYTES = 32 << 20   # PQ_NOALLOC_SCRATCH_BYTES
 REF_SYMBOLS = ["construct_index", "construct_string_graph", "destroy_graph", "free_index",
@@ -186,6 +186,7 @@ def lib(width="u32"):
         "pq_find_overlaps_export": (I32, [P, I32, P, P, U64, U32, P, P]),
         "pq_set_search_batch": (I32, [P, U64]),
         "pq_reserve_search": (I32, [P, U32]),
+        "pq_set_learned_keys": (I32, [P, U32]),
         "pq_host_alloc": (I32, [P, U64, C.POINTER(P)]),
         "pq_host_free": (None, [P]),
         "pq_mlp_eval": (I32, [P, I32, P, U32, I32, P]),
@@ -512,6 +513,17 @@ class Context:
 
     def set_edge_capacity(self, edges):
         self._check(self._L.pq_set_edge_capacity(self._h, int(edges)), "set_edge_capacity")
+
+    def learned_defaults(self):
+        """The learned index's default training configuration (pq_learned_default)."""
+        cfg = LearnedCfg()
+        self._L.pq_learned_default(C.byref(cfg))
+        return {f[0]: getattr(cfg, f[0]) for f in cfg._fields_}
+
+    def set_learned_keys(self, stride):
+        """Later index builds also write the learned index's sampled keys of every
+        stride-th row (pq_set_learned_keys; 0 = no)."""
+        self._check(self._L.pq_set_learned_keys(self._h, int(stride)), "set_learned_keys")
 
     def learned_train(self, **kw):
         cfg = LearnedCfg()

@@ -32,6 +32,10 @@ struct DirIndex {
     uint64_t cf[5] = {0, 0, 0, 0, 0};
     uint8_t *bwt8 = nullptr;       // BWT symbol per row, between the SA and occ builds
     bool built = false;
+    // sampled keys of the learned index in scratch slot SLOT_SKEY0 + direction:
+    // key of every keys_stride-th row at depth keys_D (0: none), written by the
+    // last build (pq_set_learned_keys) or training
+    uint32_t keys_stride = 0, keys_D = 0;
 };
 
 struct Learned;                    // pq_learned.hip
@@ -78,6 +82,8 @@ struct pq_ctx {
     uint64_t edge_cap_auto = 0;    // capacity an overflowing search asked for (these reads)
     uint64_t edge_need = 0;        // edges the last overflowing search counted
     uint64_t search_batch = 0;     // reads per search batch, 0 automatic
+    uint32_t key_stride = 0;       // builds also write the learned index's sampled keys
+                                   // of every key_stride-th row (a power of two; 0: no)
     uint64_t learned_gen = 0;      // bumped by every learned-index training
     // search captured as a graph, valid while its key (launch arguments) holds
     hipGraph_t search_graph = nullptr;
@@ -138,7 +144,7 @@ enum PqSlot {
     SLOT_LEARNED = 40, SLOT_BWT8 = 47, SLOT_SLICE_HIST, SLOT_SLICE_TILES, SLOT_PROBE = 50,
     SLOT_BIGRUNS, SLOT_NODES, SLOT_MISC2, SLOT_HUGE, SLOT_HUGE_LIST, SLOT_VJOB_L, SLOT_VJOB,
     SLOT_WORKQ, SLOT_WSTATS,
-    SLOT_XSTAGE0 = 60, SLOT_XSTAGE1, SLOT_XTOT, SLOT_XSCAN,
+    SLOT_XSTAGE0 = 60, SLOT_XSTAGE1, SLOT_XTOT, SLOT_XSCAN, SLOT_SKEY0, SLOT_SKEY1,
 };
 
 // Grow-only slot buffer of at least `bytes` bytes into *p, or PQ_ERR_NOMEM.
@@ -190,6 +196,9 @@ struct ChainBuf;
 int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed,
                      const ChainBuf &cb, unsigned long long *stats, hipEvent_t mid);
 void pq_release_build_scratch(pq_ctx_t *ctx);
+// Sampled keys of direction d the build's sorts left unwritten (rows placed
+// without a sort or by a deeper sort level), from the suffix array and the text.
+int pq_learned_fill_keys(pq_ctx_t *ctx, int d, uint64_t ns);
 
 static inline unsigned pq_blocks(uint64_t work, unsigned per)
 {

@@ -146,6 +146,8 @@ template <int J> __device__ __forceinline__ uint64_t xor_lane64(uint64_t v)
     return ((uint64_t) xor_lane<J>((uint32_t) (v >> 32)) << 32) | xor_lane<J>((uint32_t) v);
 }
 
+typedef unsigned __int128 u128;
+
 // Read geometry: either a fixed read length or per-read offsets + end bitmap.
 struct Geom {
     uint64_t n;              // reads

@@ -55,7 +55,6 @@
 #define GRAM_CHUNK 1024     // samples per block of the Gram kernel
 #define GRAM_DIM (MLP_H_MAX + 2)   // activations, 1, target
 
-typedef unsigned __int128 u128;
 // constant address space: wave-uniform reads through it become scalar loads
 typedef __attribute__((address_space(4))) const float cfloat;
 
@@ -1043,6 +1042,14 @@ void pq_learned_free(pq_ctx_t *ctx)
     ctx->learned = nullptr;
 }
 
+extern "C" int pq_set_learned_keys(pq_ctx_t *ctx, uint32_t stride)
+{
+    if (!ctx || (stride & (stride - 1)) || stride > (1u << 16))
+        return PQ_ERR_ARG;
+    ctx->key_stride = stride;
+    return PQ_OK;
+}
+
 extern "C" void pq_learned_default(pq_learned_cfg_t *cfg)
 {
     if (!cfg)
@@ -1156,19 +1163,57 @@ static int upload_inference_params(pq_ctx_t *ctx, const Learned &LM, DirModel &M
 // Sampled keys of direction d: one gather per stride rows into a scratch slot of
 // the context that is kept between trainings (no allocation per training once
 // it is large enough).
+// Keys of rows the build's run sorts left unwritten (all ones: bit 1 of a key is
+// never set).
+template <typename R>
+__global__ void k_sample_fill(Geom G, const uint64_t *t2, const R *sa, uint32_t stride,
+                              uint64_t ns, uint32_t D, u128 *K)
+{
+    PQ_GRID_STRIDE_FOR(j, ns)
+    if ((uint64_t) K[j] == ~0ull)
+        K[j] = suffix_key(G, t2, (uint64_t) sa[j * stride], D);
+}
+
+int pq_learned_fill_keys(pq_ctx_t *ctx, int d, uint64_t ns)
+{
+    u128 *K = (u128 *) ctx->scratch[SLOT_SKEY0 + d].p;
+    const uint32_t D = std::min<uint32_t>(63, ctx->t);
+    if (ctx->wide)
+        hipLaunchKernelGGL(k_sample_fill<uint64_t>, dim3(pq_blocks_gs(ns, 256)), dim3(256), 0,
+                           ctx->stream, ctx->geom(), ctx->dir[d].text2,
+                           (const uint64_t *) ctx->dir[d].sa, ctx->key_stride, ns, D, K);
+    else
+        hipLaunchKernelGGL(k_sample_fill<uint32_t>, dim3(pq_blocks_gs(ns, 256)), dim3(256), 0,
+                           ctx->stream, ctx->geom(), ctx->dir[d].text2,
+                           (const uint32_t *) ctx->dir[d].sa, ctx->key_stride, ns, D, K);
+    PQ_CHECK(hipGetLastError());
+    return PQ_OK;
+}
+
+// The sampled keys of direction d: the ones the last build wrote when it was
+// asked for this stride (pq_set_learned_keys), else gathered here from the
+// suffix array and the text.
 template <typename R> static int build_samples_t(pq_ctx_t *ctx, Learned &LM, int d)
 {
     DirModel &M = LM.dir[d];
+    DirIndex &Dx = ctx->dir[d];
     const uint64_t N = ctx->N, ns = (N + LM.stride - 1) / LM.stride;
-    M.key = (u128 *) ctx->tmp(SLOT_LEARNED + 6 + d, (ns + 1) * sizeof(u128));
+    M.nU = ns;
+    if (Dx.keys_stride == LM.stride && Dx.keys_D == LM.depth) {
+        M.key = (u128 *) ctx->scratch[SLOT_SKEY0 + d].p;
+        return PQ_OK;
+    }
+    Dx.keys_stride = Dx.keys_D = 0;
+    M.key = (u128 *) ctx->tmp(SLOT_SKEY0 + d, (ns + 1) * sizeof(u128));
     if (!M.key)
         return ctx->fail(PQ_ERR_NOMEM, "learned index keys");
-    M.nU = ns;
     if (ns)
         hipLaunchKernelGGL(k_sample_keys<R>, dim3(pq_blocks_gs(ns, 256)), dim3(256), 0,
-                           ctx->stream, ctx->geom(), ctx->dir[d].text2, (const R *) ctx->dir[d].sa,
+                           ctx->stream, ctx->geom(), Dx.text2, (const R *) Dx.sa,
                            LM.stride, ns, LM.depth, M.key);
     PQ_CHECK(hipGetLastError());
+    Dx.keys_stride = LM.stride;
+    Dx.keys_D = LM.depth;
     return PQ_OK;
 }
 

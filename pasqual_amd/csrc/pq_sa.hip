@@ -223,6 +223,35 @@ template <int W, typename R> __device__ __forceinline__ void suf_none(SufW<W, R>
         s.w[i] = 0;
 }
 
+// Sampled keys of the learned index written by the run sorts of the first level
+// (runs that share exactly their bucket's K codes): the key of a row that is a
+// multiple of 1 << shift is its suffix's first D codes (the bucket's K codes,
+// then the run sort's cached words), zero past the end of its read, | (the
+// suffix has at least D codes) -- the key the learned index samples.
+struct KeyOut {
+    u128 *keys;              // nullptr: no keys
+    const uint32_t *BK;      // bucket keys of the slice's rows
+    uint64_t row0;           // first row of the slice
+    uint32_t shift, D, K;
+};
+
+template <int W, typename R>
+__device__ __forceinline__ void emit_key(const KeyOut &ko, uint32_t row, uint32_t bk,
+                                         const SufW<W, R> &s)
+{
+    const uint64_t gr = ko.row0 + row;
+    if (gr & ((1ull << ko.shift) - 1ull))
+        return;
+    const uint32_t rem = ko.K + s.len;   // every suffix of such a run has >= K codes
+    uint64_t w1 = 0;
+    if constexpr (W > 1)
+        w1 = s.w[1];
+    const u128 w = ((u128) s.w[0] << 64) | (u128) w1;
+    u128 key = ((u128) (bk >> 4) << (128 - 2 * ko.K)) | (w >> (2 * ko.K));
+    key &= ~(u128) 0 << (128 - 2 * ko.D);
+    ko.keys[gr >> ko.shift] = key | (u128) (rem >= ko.D ? 1u : 0u);
+}
+
 // Canonical order of two suffixes sharing their first `base` codes: string
 // order with the end of the read first, then position.
 template <int W, typename R>
@@ -359,7 +388,7 @@ __device__ __forceinline__ void push_seg(const SegLists &S, bool valid, uint32_t
 template <int G, int W, typename R>
 __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2, R *sa,
                                                      uint8_t *bwt8, uint32_t base,
-                                                     const uint2 *segs, uint32_t nseg)
+                                                     const uint2 *segs, uint32_t nseg, KeyOut ko)
 {
     uint64_t tid = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t sid = tid / G;
@@ -376,6 +405,8 @@ __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2
     if (e) {
         sa[sg.x + li] = s.g;
         bwt8[sg.x + li] = (uint8_t) s.sym;
+        if (ko.keys)
+            emit_key<W, R>(ko, sg.x + li, ko.BK[sg.x], s);
     }
 }
 
@@ -388,7 +419,8 @@ __host__ __device__ constexpr uint32_t lds_cap(int W)
 // One block per run of 65..lds_cap(W) elements: bitonic sort in LDS.
 template <int W, typename R>
 __global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, R *sa,
-                                                  uint8_t *bwt8, uint32_t base, const uint2 *segs)
+                                                  uint8_t *bwt8, uint32_t base, const uint2 *segs,
+                                                  KeyOut ko)
 {
     constexpr uint32_t CAP = lds_cap(W);
     __shared__ SufW<W, R> sh[CAP];
@@ -419,9 +451,12 @@ __global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, R
             __syncthreads();
         }
     }
+    const uint32_t bk = ko.keys ? ko.BK[sg.x] : 0u;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         sa[sg.x + i] = sh[i].g;
         bwt8[sg.x + i] = (uint8_t) sh[i].sym;
+        if (ko.keys)
+            emit_key<W, R>(ko, sg.x + i, bk, sh[i]);
     }
 }
 
@@ -584,47 +619,49 @@ static uint32_t lcap_for(int W)
 
 template <int W, typename R>
 static void launch_sorts(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, R *sa, uint8_t *bwt8,
-                         uint32_t base, uint2 *const items[CLS_BIG], const uint32_t cnt[CLS_BIG])
+                         uint32_t base, uint2 *const items[CLS_BIG], const uint32_t cnt[CLS_BIG],
+                         const KeyOut &ko)
 {
     for (uint32_t at = 0; at < cnt[0]; at += SORT_RUNS_PER_LAUNCH) {
         const uint32_t m = std::min(cnt[0] - at, SORT_RUNS_PER_LAUNCH);
         hipLaunchKernelGGL((k_sort_groups<RUN_SMALL, W, R>),
                            dim3(pq_blocks((uint64_t) m * RUN_SMALL, 256)), dim3(256), 0,
-                           ctx->stream, G, t2, sa, bwt8, base, items[0] + at, m);
+                           ctx->stream, G, t2, sa, bwt8, base, items[0] + at, m, ko);
     }
     for (uint32_t at = 0; at < cnt[1]; at += SORT_RUNS_PER_LAUNCH) {
         const uint32_t m = std::min(cnt[1] - at, SORT_RUNS_PER_LAUNCH);
         hipLaunchKernelGGL((k_sort_groups<RUN_HALF, W, R>),
                            dim3(pq_blocks((uint64_t) m * RUN_HALF, 256)), dim3(256), 0,
-                           ctx->stream, G, t2, sa, bwt8, base, items[1] + at, m);
+                           ctx->stream, G, t2, sa, bwt8, base, items[1] + at, m, ko);
     }
     for (uint32_t at = 0; at < cnt[2]; at += SORT_RUNS_PER_LAUNCH) {
         const uint32_t m = std::min(cnt[2] - at, SORT_RUNS_PER_LAUNCH);
         hipLaunchKernelGGL((k_sort_groups<RUN_WAVE, W, R>),
                            dim3(pq_blocks((uint64_t) m * RUN_WAVE, 256)), dim3(256), 0,
-                           ctx->stream, G, t2, sa, bwt8, base, items[2] + at, m);
+                           ctx->stream, G, t2, sa, bwt8, base, items[2] + at, m, ko);
     }
     for (uint32_t at = 0; at < cnt[3]; at += SORT_RUNS_PER_LAUNCH) {
         const uint32_t m = std::min(cnt[3] - at, SORT_RUNS_PER_LAUNCH);
         hipLaunchKernelGGL((k_sort_lds<W, R>), dim3(m), dim3(256), 0, ctx->stream, G, t2, sa,
-                           bwt8, base, items[3] + at);
+                           bwt8, base, items[3] + at, ko);
     }
 }
 
 template <typename R>
 static int sort_lists(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, R *sa, uint8_t *bwt8,
-                      uint32_t base, uint2 *const items[CLS_BIG], const uint32_t cnt[CLS_BIG])
+                      uint32_t base, uint2 *const items[CLS_BIG], const uint32_t cnt[CLS_BIG],
+                      const KeyOut &ko)
 {
     uint32_t left = ctx->Lmax > base ? ctx->Lmax - base : 0;
     switch (words_for(left)) {
-    case 1: launch_sorts<1, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
-    case 2: launch_sorts<2, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
-    case 3: launch_sorts<3, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
-    case 4: launch_sorts<4, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
-    case 5: launch_sorts<5, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
-    case 6: launch_sorts<6, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
-    case 7: launch_sorts<7, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
-    case 8: launch_sorts<8, R>(ctx, G, t2, sa, bwt8, base, items, cnt); break;
+    case 1: launch_sorts<1, R>(ctx, G, t2, sa, bwt8, base, items, cnt, ko); break;
+    case 2: launch_sorts<2, R>(ctx, G, t2, sa, bwt8, base, items, cnt, ko); break;
+    case 3: launch_sorts<3, R>(ctx, G, t2, sa, bwt8, base, items, cnt, ko); break;
+    case 4: launch_sorts<4, R>(ctx, G, t2, sa, bwt8, base, items, cnt, ko); break;
+    case 5: launch_sorts<5, R>(ctx, G, t2, sa, bwt8, base, items, cnt, ko); break;
+    case 6: launch_sorts<6, R>(ctx, G, t2, sa, bwt8, base, items, cnt, ko); break;
+    case 7: launch_sorts<7, R>(ctx, G, t2, sa, bwt8, base, items, cnt, ko); break;
+    case 8: launch_sorts<8, R>(ctx, G, t2, sa, bwt8, base, items, cnt, ko); break;
     default:
         if (cnt[0] || cnt[1] || cnt[2] || cnt[3])
             return ctx->fail(PQ_ERR_STATE, "run classified for cached words it cannot hold");
@@ -660,7 +697,7 @@ template <typename R> struct SliceSort {
 // then oversized runs level by level over 29-code chunks.
 template <typename R>
 static int sort_runs(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned K, uint64_t ns,
-                     const SliceSort<R> &S)
+                     const SliceSort<R> &S, const KeyOut &ko)
 {
     int rc = PQ_OK;
     // bucket runs, classified by size with scans
@@ -689,8 +726,12 @@ static int sort_runs(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned 
     }
     // sort every bucket past its first K codes
     uint2 *l3[CLS_BIG] = {lists[0], lists[1], lists[2], lists[3]};
-    if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, K, l3, cnt)))
+    if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, K, l3, cnt, ko)))
         return rc;
+    // deeper levels: their runs share more than the bucket's codes; their keys are
+    // filled in after the build
+    KeyOut ko_none = ko;
+    ko_none.keys = nullptr;
     // oversized buckets: MSD over 29-code chunks, level by level
     std::vector<uint2> big(cnt[CLS_BIG]);
     if (cnt[CLS_BIG]) {
@@ -772,7 +813,7 @@ static int sort_runs(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned 
             return ctx->fail(PQ_ERR_LIMIT, "suffix sort run list overflow");
         base = nbase;
         uint2 *sl[CLS_BIG] = {SL.items[0], SL.items[1], SL.items[2], SL.items[3]};
-        if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, base, sl, c5)))
+        if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, base, sl, c5, ko_none)))
             return rc;
         big.assign(c5[CLS_BIG], make_uint2(0, 0));
         if (c5[CLS_BIG]) {
@@ -866,6 +907,17 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
     S.flags = BK2;
     S.pos = (uint32_t *) V2;
     PQ_CHECK(hipMemsetAsync(D.bwt8, BWT_UNSET, N, ctx->stream));
+    // the learned index's sampled keys, written by the first-level run sorts; the
+    // rest (all ones until then: bit 1 of a key is never set) after the slices
+    D.keys_stride = D.keys_D = 0;
+    KeyOut ko = {nullptr, nullptr, 0, 0, std::min<uint32_t>(63, ctx->t), K};
+    const uint64_t nkeys = ctx->key_stride ? (N + ctx->key_stride - 1) / ctx->key_stride : 0;
+    if (nkeys) {
+        if ((rc = pq_slot(ctx, SLOT_SKEY0 + d, (nkeys + 1) * sizeof(u128), &ko.keys)))
+            return rc;
+        ko.shift = (uint32_t) __builtin_ctz(ctx->key_stride);
+        PQ_CHECK(hipMemsetAsync(ko.keys, 0xFF, nkeys * sizeof(u128), ctx->stream));
+    }
     uint64_t off = 0;
     for (const Slice &sl : slices) {
         const uint64_t ns = sl.count;
@@ -902,12 +954,20 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
             std::swap(vc, va);
         }
         // 2-4. runs of equal keys
-        if ((rc = sort_runs<R>(ctx, G, D.text2, K, ns, S)))
+        ko.BK = S.BK;
+        ko.row0 = off;
+        if ((rc = sort_runs<R>(ctx, G, D.text2, K, ns, S, ko)))
             return rc;
         off += ns;
     }
     if (off != N)
         return ctx->fail(PQ_ERR_STATE, "slices do not cover the text");
+    if (nkeys) {
+        if ((rc = pq_learned_fill_keys(ctx, d, nkeys)))
+            return rc;
+        D.keys_stride = ctx->key_stride;
+        D.keys_D = ko.D;
+    }
     return PQ_OK;
 }
 

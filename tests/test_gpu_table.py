@@ -68,3 +68,45 @@ def test_learned_table_matches_host(ctx, L, t, seed, varlen, stride):
         got_g = ctx.graph()
         for k in want_g:
             assert np.array_equal(got_g[k], want_g[k]), (rep, k)
+
+
+@pytest.mark.parametrize("L,t,seed,varlen,stride,slice_", [(100, 63, 12, False, 8, 0),
+                                                           (100, 31, 11, False, 8, 40000),
+                                                           (0, 31, 5, True, 1, 0),
+                                                           (0, 63, 6, True, 16, 30000)])
+def test_build_written_keys_match_host(ctx, L, t, seed, varlen, stride, slice_):
+    # keys written by the index build's run sorts (pq_set_learned_keys), the rows
+    # no first-level sort placed filled in after it; sliced sorts (row offsets per
+    # slice) and 64-bit rows included
+    rng = np.random.default_rng(seed)
+    if varlen:
+        codes, off, _ = _random_reads(rng, 6000, 600, 20, 120, err=0.003)
+    else:
+        codes, off, _ = _random_reads(rng, 20000, int(20 * 20000 / L), L, L, err=0.003)
+    D = min(63, t)
+    want = [_host_table(codes, off, d, D, stride) for d in (0, 1)]
+    try:
+        ctx.set_layout(8 if slice_ else 0, slice_)
+        ctx.upload(codes, off, t)
+        ctx.set_learned_keys(stride)
+        ctx.build_index()
+        ctx.learned_train(samples=1 << 14, batch=1024, epochs=1, sample_stride=stride)
+        for d in (0, 1):
+            keys, _, _ = ctx.learned_table(d)
+            got = [(int(a) << 64) | int(b) for a, b in keys.tolist()]
+            assert got == want[d][0], d
+        ctx.find_overlaps(pq.SEARCH_PLAIN)
+        want_g = ctx.graph()
+        ctx.find_overlaps(pq.SEARCH_LEARNED)
+        got_g = ctx.graph()
+        for k in want_g:
+            assert np.array_equal(got_g[k], want_g[k]), k
+    finally:
+        ctx.set_learned_keys(0)
+        ctx.set_layout(0, 0)
+
+
+def test_learned_keys_stride_must_be_a_power_of_two(ctx):
+    with pytest.raises(pq.PqError):
+        ctx.set_learned_keys(5)
+    ctx.set_learned_keys(0)
