@@ -36,6 +36,7 @@ struct DirIndex {
     // key of every keys_stride-th row at depth keys_D (0: none), written by the
     // last build (pq_set_learned_keys) or training
     uint32_t keys_stride = 0, keys_D = 0;
+    uint32_t keys_pending = 0;     // stride of keys the running build writes (0: none)
 };
 
 struct Learned;                    // pq_learned.hip
@@ -196,9 +197,6 @@ struct ChainBuf;
 int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed,
                      const ChainBuf &cb, unsigned long long *stats, hipEvent_t mid);
 void pq_release_build_scratch(pq_ctx_t *ctx);
-// Sampled keys of direction d the build's sorts left unwritten (rows placed
-// without a sort or by a deeper sort level), from the suffix array and the text.
-int pq_learned_fill_keys(pq_ctx_t *ctx, int d, uint64_t ns);
 
 static inline unsigned pq_blocks(uint64_t work, unsigned per)
 {

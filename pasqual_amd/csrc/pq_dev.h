@@ -319,3 +319,34 @@ __device__ __forceinline__ void pq_ranks2(const OccLine *occ, uint64_t l, uint64
         pq_load_line(occ, wu, ln);
     pq_line_ranks(ln, wu, (unsigned) (u1 & 127), ru);
 }
+
+// ---- keys of the learned index -------------------------------------------------------
+
+// 64 codes from text position g, first code in the top bits.
+__device__ __forceinline__ u128 text_codes(const uint64_t *t2, uint64_t g)
+{
+    return ((u128) pq_window(t2, g) << 64) | (u128) pq_window(t2, g + 32);
+}
+
+// Key bits kept for D codes: the top 2D bits.
+__device__ __forceinline__ u128 top_mask(uint32_t D)
+{
+    return ~(u128) 0 << (128 - 2 * D);
+}
+
+// Key of the suffix at text position g: its first D codes (codes past the end of
+// its read read as 0) | (the suffix has at least D codes).  Non-decreasing along
+// the suffix array: a suffix shorter than D codes that is a prefix of a pattern
+// sorts before the pattern's rows, as in the suffix array.
+__device__ __forceinline__ u128 suffix_key(const Geom &G, const uint64_t *t2, uint64_t g,
+                                           uint32_t D)
+{
+    const uint64_t rem = pq_rem(G, g);
+    // up to 32 codes come from one 64-bit window (two words instead of three)
+    u128 w = D <= 32 ? (u128) pq_window(t2, g) << 64 : text_codes(t2, g);
+    w &= top_mask(D);
+    if (rem < D)
+        w &= ~(~(u128) 0 >> (2 * rem));
+    return w | (u128) (rem >= D ? 1u : 0u);
+}
+

@@ -313,34 +313,6 @@ __global__ void __launch_bounds__(256) k_ls_gram(const float *params, MlpDims d,
 
 // ---- keys, table, samples, leaves ------------------------------------------------------
 
-// 64 codes from text position g, first code in the top bits.
-__device__ __forceinline__ u128 text_codes(const uint64_t *t2, uint64_t g)
-{
-    return ((u128) pq_window(t2, g) << 64) | (u128) pq_window(t2, g + 32);
-}
-
-// Key bits kept for D codes: the top 2D bits.
-__device__ __forceinline__ u128 top_mask(uint32_t D)
-{
-    return ~(u128) 0 << (128 - 2 * D);
-}
-
-// Key of the suffix at text position g: its first D codes (codes past the end of
-// its read read as 0) | (the suffix has at least D codes).  Non-decreasing along
-// the suffix array: a suffix shorter than D codes that is a prefix of a pattern
-// sorts before the pattern's rows, as in the suffix array.
-__device__ __forceinline__ u128 suffix_key(const Geom &G, const uint64_t *t2, uint64_t g,
-                                           uint32_t D)
-{
-    const uint64_t rem = pq_rem(G, g);
-    // up to 32 codes come from one 64-bit window (two words instead of three)
-    u128 w = D <= 32 ? (u128) pq_window(t2, g) << 64 : text_codes(t2, g);
-    w &= top_mask(D);
-    if (rem < D)
-        w &= ~(~(u128) 0 >> (2 * rem));
-    return w | (u128) (rem >= D ? 1u : 0u);
-}
-
 // The exact part of the learned index: K[j] = key of suffix-array row j * stride,
 // j < ns (every stride-th row; the suffix array is read once, the text at one
 // position per stride rows).
@@ -1163,33 +1135,6 @@ static int upload_inference_params(pq_ctx_t *ctx, const Learned &LM, DirModel &M
 // Sampled keys of direction d: one gather per stride rows into a scratch slot of
 // the context that is kept between trainings (no allocation per training once
 // it is large enough).
-// Keys of rows the build's run sorts left unwritten (all ones: bit 1 of a key is
-// never set).
-template <typename R>
-__global__ void k_sample_fill(Geom G, const uint64_t *t2, const R *sa, uint32_t stride,
-                              uint64_t ns, uint32_t D, u128 *K)
-{
-    PQ_GRID_STRIDE_FOR(j, ns)
-    if ((uint64_t) K[j] == ~0ull)
-        K[j] = suffix_key(G, t2, (uint64_t) sa[j * stride], D);
-}
-
-int pq_learned_fill_keys(pq_ctx_t *ctx, int d, uint64_t ns)
-{
-    u128 *K = (u128 *) ctx->scratch[SLOT_SKEY0 + d].p;
-    const uint32_t D = std::min<uint32_t>(63, ctx->t);
-    if (ctx->wide)
-        hipLaunchKernelGGL(k_sample_fill<uint64_t>, dim3(pq_blocks_gs(ns, 256)), dim3(256), 0,
-                           ctx->stream, ctx->geom(), ctx->dir[d].text2,
-                           (const uint64_t *) ctx->dir[d].sa, ctx->key_stride, ns, D, K);
-    else
-        hipLaunchKernelGGL(k_sample_fill<uint32_t>, dim3(pq_blocks_gs(ns, 256)), dim3(256), 0,
-                           ctx->stream, ctx->geom(), ctx->dir[d].text2,
-                           (const uint32_t *) ctx->dir[d].sa, ctx->key_stride, ns, D, K);
-    PQ_CHECK(hipGetLastError());
-    return PQ_OK;
-}
-
 // The sampled keys of direction d: the ones the last build wrote when it was
 // asked for this stride (pq_set_learned_keys), else gathered here from the
 // suffix array and the text.

@@ -223,32 +223,39 @@ template <int W, typename R> __device__ __forceinline__ void suf_none(SufW<W, R>
         s.w[i] = 0;
 }
 
-// Sampled keys of the learned index written by the run sorts of the first level
-// (runs that share exactly their bucket's K codes): the key of a row that is a
-// multiple of 1 << shift is its suffix's first D codes (the bucket's K codes,
-// then the run sort's cached words), zero past the end of its read, | (the
-// suffix has at least D codes) -- the key the learned index samples.
+// Sampled keys of the learned index written by the run sorts as they place their
+// rows: the key of a row that is a multiple of 1 << shift is its suffix's first D
+// codes, zero past the end of its read, | (the suffix has at least D codes) --
+// suffix_key, the key the learned index samples.  First-level runs (sharing
+// exactly their bucket's K codes) make it from the bucket's codes and the cached
+// words; deeper levels (from_text) read it from the text.  Rows no sort places
+// get theirs from k_bwt_fill.
 struct KeyOut {
     u128 *keys;              // nullptr: no keys
     const uint32_t *BK;      // bucket keys of the slice's rows
     uint64_t row0;           // first row of the slice
     uint32_t shift, D, K;
+    uint32_t from_text;
 };
 
 template <int W, typename R>
-__device__ __forceinline__ void emit_key(const KeyOut &ko, uint32_t row, uint32_t bk,
-                                         const SufW<W, R> &s)
+__device__ __forceinline__ void emit_key(const KeyOut &ko, const Geom &G, const uint64_t *t2,
+                                         uint32_t row, uint32_t bk, const SufW<W, R> &s)
 {
     const uint64_t gr = ko.row0 + row;
     if (gr & ((1ull << ko.shift) - 1ull))
         return;
+    if (ko.from_text) {
+        ko.keys[gr >> ko.shift] = suffix_key(G, t2, (uint64_t) s.g, ko.D);
+        return;
+    }
     const uint32_t rem = ko.K + s.len;   // every suffix of such a run has >= K codes
     uint64_t w1 = 0;
     if constexpr (W > 1)
         w1 = s.w[1];
     const u128 w = ((u128) s.w[0] << 64) | (u128) w1;
     u128 key = ((u128) (bk >> 4) << (128 - 2 * ko.K)) | (w >> (2 * ko.K));
-    key &= ~(u128) 0 << (128 - 2 * ko.D);
+    key &= top_mask(ko.D);
     ko.keys[gr >> ko.shift] = key | (u128) (rem >= ko.D ? 1u : 0u);
 }
 
@@ -406,7 +413,7 @@ __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2
         sa[sg.x + li] = s.g;
         bwt8[sg.x + li] = (uint8_t) s.sym;
         if (ko.keys)
-            emit_key<W, R>(ko, sg.x + li, ko.BK[sg.x], s);
+            emit_key<W, R>(ko, Gm, t2, sg.x + li, ko.BK[sg.x], s);
     }
 }
 
@@ -451,12 +458,12 @@ __global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, R
             __syncthreads();
         }
     }
-    const uint32_t bk = ko.keys ? ko.BK[sg.x] : 0u;
+    const uint32_t bk = ko.keys && !ko.from_text ? ko.BK[sg.x] : 0u;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         sa[sg.x + i] = sh[i].g;
         bwt8[sg.x + i] = (uint8_t) sh[i].sym;
         if (ko.keys)
-            emit_key<W, R>(ko, sg.x + i, bk, sh[i]);
+            emit_key<W, R>(ko, Gm, t2, sg.x + i, bk, sh[i]);
     }
 }
 
@@ -728,10 +735,10 @@ static int sort_runs(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned 
     uint2 *l3[CLS_BIG] = {lists[0], lists[1], lists[2], lists[3]};
     if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, K, l3, cnt, ko)))
         return rc;
-    // deeper levels: their runs share more than the bucket's codes; their keys are
-    // filled in after the build
-    KeyOut ko_none = ko;
-    ko_none.keys = nullptr;
+    // deeper levels: their runs share more than the bucket's codes; their keys come
+    // from the text
+    KeyOut ko_text = ko;
+    ko_text.from_text = 1;
     // oversized buckets: MSD over 29-code chunks, level by level
     std::vector<uint2> big(cnt[CLS_BIG]);
     if (cnt[CLS_BIG]) {
@@ -813,7 +820,7 @@ static int sort_runs(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned 
             return ctx->fail(PQ_ERR_LIMIT, "suffix sort run list overflow");
         base = nbase;
         uint2 *sl[CLS_BIG] = {SL.items[0], SL.items[1], SL.items[2], SL.items[3]};
-        if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, base, sl, c5, ko_none)))
+        if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, base, sl, c5, ko_text)))
             return rc;
         big.assign(c5[CLS_BIG], make_uint2(0, 0));
         if (c5[CLS_BIG]) {
@@ -907,16 +914,15 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
     S.flags = BK2;
     S.pos = (uint32_t *) V2;
     PQ_CHECK(hipMemsetAsync(D.bwt8, BWT_UNSET, N, ctx->stream));
-    // the learned index's sampled keys, written by the first-level run sorts; the
-    // rest (all ones until then: bit 1 of a key is never set) after the slices
-    D.keys_stride = D.keys_D = 0;
-    KeyOut ko = {nullptr, nullptr, 0, 0, std::min<uint32_t>(63, ctx->t), K};
+    // the learned index's sampled keys, written by the run sorts as they place their
+    // rows, and by k_bwt_fill for the rows no sort placed
+    D.keys_stride = D.keys_D = D.keys_pending = 0;
+    KeyOut ko = {nullptr, nullptr, 0, 0, std::min<uint32_t>(63, ctx->t), K, 0};
     const uint64_t nkeys = ctx->key_stride ? (N + ctx->key_stride - 1) / ctx->key_stride : 0;
     if (nkeys) {
         if ((rc = pq_slot(ctx, SLOT_SKEY0 + d, (nkeys + 1) * sizeof(u128), &ko.keys)))
             return rc;
         ko.shift = (uint32_t) __builtin_ctz(ctx->key_stride);
-        PQ_CHECK(hipMemsetAsync(ko.keys, 0xFF, nkeys * sizeof(u128), ctx->stream));
     }
     uint64_t off = 0;
     for (const Slice &sl : slices) {
@@ -962,12 +968,7 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
     }
     if (off != N)
         return ctx->fail(PQ_ERR_STATE, "slices do not cover the text");
-    if (nkeys) {
-        if ((rc = pq_learned_fill_keys(ctx, d, nkeys)))
-            return rc;
-        D.keys_stride = ctx->key_stride;
-        D.keys_D = ko.D;
-    }
+    D.keys_pending = nkeys ? ctx->key_stride : 0;
     return PQ_OK;
 }
 
@@ -981,13 +982,19 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d)
 // ---- BWT occurrence lines + reads_index ---------------------------------------------------
 
 // BWT symbols of the rows no sort wrote (one-suffix buckets, buckets of
-// identical complete suffixes, rows placed by the chunk-level radix passes).
+// identical complete suffixes, rows placed by the chunk-level radix passes), and
+// those rows' sampled keys when the build writes keys (keys != nullptr).
 template <typename R>
-__global__ void k_bwt_fill(Geom G, const uint64_t *t2, const R *sa, uint64_t N, uint8_t *bwt8)
+__global__ void k_bwt_fill(Geom G, const uint64_t *t2, const R *sa, uint64_t N, uint8_t *bwt8,
+                           u128 *keys, uint32_t shift, uint32_t D)
 {
     PQ_GRID_STRIDE_FOR(row, N)
-    if (bwt8[row] == BWT_UNSET)
-        bwt8[row] = (uint8_t) bwt_sym(G, t2, (uint64_t) sa[row]);
+    if (bwt8[row] == BWT_UNSET) {
+        const uint64_t g = (uint64_t) sa[row];
+        bwt8[row] = (uint8_t) bwt_sym(G, t2, g);
+        if (keys && !(row & ((1ull << shift) - 1ull)))
+            keys[row >> shift] = suffix_key(G, t2, g, D);
+    }
 }
 
 // 256 threads = 2 lines per tile; each wave builds one 64-row half of a line.
@@ -1070,12 +1077,22 @@ int pq_build_occ_dir(pq_ctx_t *ctx, int d)
     uint4 *excl = cnts + D.nlines;
     if (!D.bwt8)
         return ctx->fail(PQ_ERR_STATE, "BWT symbols missing: suffix array not built");
+    u128 *keys = D.keys_pending ? (u128 *) ctx->scratch[SLOT_SKEY0 + d].p : nullptr;
+    const uint32_t kshift = D.keys_pending ? (uint32_t) __builtin_ctz(D.keys_pending) : 0u;
+    const uint32_t kD = std::min<uint32_t>(63, ctx->t);
     if (ctx->wide)
         hipLaunchKernelGGL(k_bwt_fill<uint64_t>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
-                           ctx->stream, G, D.text2, (const uint64_t *) D.sa, N, D.bwt8);
+                           ctx->stream, G, D.text2, (const uint64_t *) D.sa, N, D.bwt8, keys,
+                           kshift, kD);
     else
         hipLaunchKernelGGL(k_bwt_fill<uint32_t>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
-                           ctx->stream, G, D.text2, (const uint32_t *) D.sa, N, D.bwt8);
+                           ctx->stream, G, D.text2, (const uint32_t *) D.sa, N, D.bwt8, keys,
+                           kshift, kD);
+    if (D.keys_pending) {
+        D.keys_stride = D.keys_pending;
+        D.keys_D = kD;
+        D.keys_pending = 0;
+    }
     hipLaunchKernelGGL(k_occ_planes, dim3(pq_blocks_gs((D.nlines + 1) / 2, 1)), dim3(256), 0,
                        ctx->stream, D.bwt8, N, D.nlines, D.occ, cnts);
     uint32_t tot[4];
