@@ -469,12 +469,64 @@ __global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, R
 
 // ---- runs of equal keys ---------------------------------------------------------------
 
+#define RUN_TILE 4096        // keys per block of the run-start kernels (16 x 256)
+
+// Run heads (keys that differ from the key before them) per tile of RUN_TILE keys.
 template <typename KT>
-__global__ void k_head_flags(const KT *K, uint64_t n, uint32_t *flags)
+__global__ void __launch_bounds__(256) k_run_counts(const KT *K, uint64_t n, uint32_t *tcnt)
 {
-    uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        flags[i] = (i == 0 || K[i] != K[i - 1]) ? 1u : 0u;
+    __shared__ uint32_t ws[4];
+    const uint64_t tile = (uint64_t) blockIdx.x * RUN_TILE;
+    uint32_t c = 0;
+    for (int r = 0; r < RUN_TILE / 256; r++) {
+        const uint64_t i = tile + (uint64_t) r * 256 + threadIdx.x;
+        if (i < n)
+            c += (i == 0 || K[i] != K[i - 1]) ? 1u : 0u;
+    }
+    for (int d = 32; d > 0; d >>= 1)
+        c += __shfl_xor(c, d, 64);
+    if ((threadIdx.x & 63) == 0)
+        ws[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        tcnt[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// starts[j] = index of the first key of run j, in key order (toff: exclusive
+// prefix of the tile counts); starts[nruns] = n.
+template <typename KT>
+__global__ void __launch_bounds__(256) k_run_starts(const KT *K, uint64_t n, const uint32_t *toff,
+                                                    uint32_t *starts, uint32_t nruns)
+{
+    __shared__ uint32_t ws[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t tile = (uint64_t) blockIdx.x * RUN_TILE;
+    uint32_t base = toff[blockIdx.x];
+    for (int r = 0; r < RUN_TILE / 256; r++) {
+        const uint64_t i = tile + (uint64_t) r * 256 + threadIdx.x;
+        const bool head = i < n && (i == 0 || K[i] != K[i - 1]);
+        const uint64_t m = __ballot(head);
+        if (lane == 0)
+            ws[wid] = (uint32_t) __popcll(m);
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (int w = 0; w < 4; w++) {
+            pre += w < wid ? ws[w] : 0u;
+            tot += ws[w];
+        }
+        if (head)
+            starts[base + pre + (uint32_t) __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t) i;
+        base += tot;
+        __syncthreads();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        starts[nruns] = (uint32_t) n;   // sentinel
+}
+
+__global__ void k_zero_u32_sa(uint32_t *p, uint32_t count)
+{
+    if (threadIdx.x < count)
+        p[threadIdx.x] = 0;
 }
 
 __global__ void k_compact_heads(const uint32_t *flags, const uint32_t *pos, uint64_t n,
@@ -597,13 +649,22 @@ template <typename KT>
 static int find_runs(pq_ctx_t *ctx, const KT *K, uint64_t n, uint32_t *flags, uint32_t *pos,
                      uint32_t *starts, uint32_t *nruns)
 {
-    unsigned nb = pq_blocks(n, 256);
-    hipLaunchKernelGGL(k_head_flags<KT>, dim3(nb), dim3(256), 0, ctx->stream, K, n, flags);
-    int rc = pq_scan_u32(ctx, flags, pos, n, nruns);
+    // heads counted per tile, the tile counts scanned, the heads written in order:
+    // the keys are read twice, no per-key flag or position array
+    const uint64_t ntiles = (n + RUN_TILE - 1) / RUN_TILE;
+    if (n == 0) {
+        *nruns = 0;
+        hipLaunchKernelGGL(k_zero_u32_sa, dim3(1), dim3(64), 0, ctx->stream, starts, 1u);
+        PQ_CHECK(hipGetLastError());
+        return PQ_OK;
+    }
+    hipLaunchKernelGGL(k_run_counts<KT>, dim3((unsigned) ntiles), dim3(256), 0, ctx->stream, K, n,
+                       flags);
+    int rc = pq_scan_u32(ctx, flags, pos, ntiles, nruns);
     if (rc)
         return rc;
-    hipLaunchKernelGGL(k_compact_heads, dim3(nb), dim3(256), 0, ctx->stream, flags, pos, n,
-                       starts);
+    hipLaunchKernelGGL(k_run_starts<KT>, dim3((unsigned) ntiles), dim3(256), 0, ctx->stream, K, n,
+                       pos, starts, *nruns);
     PQ_CHECK(hipGetLastError());
     return PQ_OK;
 }
