@@ -552,19 +552,74 @@ __global__ void k_bucket_class(const uint32_t *BK, const uint32_t *starts, uint3
     cls[k] = done ? CLS_DONE : (uint8_t) seg_class(size, lcap);
 }
 
-__global__ void k_class_flags(const uint8_t *cls, uint32_t nruns, uint8_t c, uint32_t *flags)
+// Runs per size class per tile of RUN_TILE runs, class-major: tcnt[c * ntiles +
+// tile] for the classes c < CLS_BIG + 1 that are sorted or split (CLS_DONE runs
+// are left out), so that one exclusive scan of the array gives every (class,
+// tile) its place in the class lists laid one after the other.
+__global__ void __launch_bounds__(256) k_class_counts(const uint8_t *cls, uint32_t nruns,
+                                                      uint32_t ntiles, uint32_t *tcnt)
 {
-    uint64_t k = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nruns)
-        flags[k] = cls[k] == c ? 1u : 0u;
+    __shared__ uint32_t ws[4][NCLS];
+    const uint64_t tile = (uint64_t) blockIdx.x * RUN_TILE;
+    uint32_t c[NCLS] = {0, 0, 0, 0, 0};
+    for (int r = 0; r < RUN_TILE / 256; r++) {
+        const uint64_t k = tile + (uint64_t) r * 256 + threadIdx.x;
+        const uint32_t x = k < nruns ? cls[k] : (uint32_t) CLS_DONE;
+#pragma unroll
+        for (int q = 0; q < NCLS; q++)
+            c[q] += x == (uint32_t) q ? 1u : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < NCLS; q++) {
+        for (int d = 32; d > 0; d >>= 1)
+            c[q] += __shfl_xor(c[q], d, 64);
+        if ((threadIdx.x & 63) == 0)
+            ws[threadIdx.x >> 6][q] = c[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < NCLS)
+        tcnt[(uint64_t) threadIdx.x * ntiles + blockIdx.x] =
+            ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
 }
 
-__global__ void k_class_compact(const uint32_t *flags, const uint32_t *pos, const uint32_t *starts,
-                                uint32_t nruns, uint2 *out)
+// Every run of a class c < NCLS as (start, size) at its place in the class lists
+// (toff: exclusive scan of k_class_counts' array), in run order within a class.
+__global__ void __launch_bounds__(256) k_class_scatter(const uint8_t *cls, const uint32_t *starts,
+                                                       uint32_t nruns, uint32_t ntiles,
+                                                       const uint32_t *toff, uint2 *out)
 {
-    uint64_t k = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nruns && flags[k])
-        out[pos[k]] = make_uint2(starts[k], starts[k + 1] - starts[k]);
+    __shared__ uint32_t ws[4][NCLS];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t tile = (uint64_t) blockIdx.x * RUN_TILE;
+    uint32_t base[NCLS];
+#pragma unroll
+    for (int q = 0; q < NCLS; q++)
+        base[q] = toff[(uint64_t) q * ntiles + blockIdx.x];
+    for (int r = 0; r < RUN_TILE / 256; r++) {
+        const uint64_t k = tile + (uint64_t) r * 256 + threadIdx.x;
+        const uint32_t x = k < nruns ? cls[k] : (uint32_t) CLS_DONE;
+        uint64_t m[NCLS];
+#pragma unroll
+        for (int q = 0; q < NCLS; q++) {
+            m[q] = __ballot(x == (uint32_t) q);
+            if (lane == 0)
+                ws[wid][q] = (uint32_t) __popcll(m[q]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NCLS; q++) {
+            uint32_t pre = 0, tot = 0;
+            for (int w = 0; w < 4; w++) {
+                pre += w < wid ? ws[w][q] : 0u;
+                tot += ws[w][q];
+            }
+            if (x == (uint32_t) q)
+                out[base[q] + pre + (uint32_t) __popcll(m[q] & ((1ull << lane) - 1ull))] =
+                    make_uint2(starts[k], starts[k + 1] - starts[k]);
+            base[q] += tot;
+        }
+        __syncthreads();
+    }
 }
 
 // ---- oversized runs: every run of one level at once -------------------------------
@@ -775,22 +830,40 @@ static int sort_runs(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned 
     uint32_t lcap0 = lcap_for(words_for(ctx->Lmax > K ? ctx->Lmax - K : 0));
     hipLaunchKernelGGL(k_bucket_class, dim3(pq_blocks(nruns, 256)), dim3(256), 0, ctx->stream,
                        S.BK, S.starts, nruns, K, lcap0, S.cls);
-    // the class lists one after the other in one buffer (together at most nruns)
+    // the class lists one after the other in one buffer (together at most nruns):
+    // runs per class and tile, one scan of those counts, one scatter
     uint32_t cnt[NCLS] = {0, 0, 0, 0, 0};
     uint2 *lists[NCLS];
     uint2 *lbuf = nullptr;
     if ((rc = pq_slot(ctx, SLOT_LISTS, ((size_t) nruns + NCLS) * sizeof(uint2), &lbuf)))
         return rc;
-    uint64_t at = 0;
-    for (int c = 0; c < NCLS; c++) {
-        lists[c] = lbuf + at;
-        hipLaunchKernelGGL(k_class_flags, dim3(pq_blocks(nruns, 256)), dim3(256), 0, ctx->stream,
-                           S.cls, nruns, (uint8_t) c, S.flags);
-        if ((rc = pq_scan_u32(ctx, S.flags, S.pos, nruns, &cnt[c])))
-            return rc;
-        hipLaunchKernelGGL(k_class_compact, dim3(pq_blocks(nruns, 256)), dim3(256), 0,
-                           ctx->stream, S.flags, S.pos, S.starts, nruns, lists[c]);
-        at += cnt[c];
+    {
+        const uint32_t ntiles = (uint32_t) (((uint64_t) nruns + RUN_TILE - 1) / RUN_TILE);
+        const uint64_t nc = (uint64_t) ntiles * NCLS;
+        uint32_t first[NCLS + 1];
+        if (ntiles) {
+            hipLaunchKernelGGL(k_class_counts, dim3(ntiles), dim3(256), 0, ctx->stream, S.cls,
+                               nruns, ntiles, S.flags);
+            uint32_t total = 0;
+            if ((rc = pq_scan_u32(ctx, S.flags, S.pos, nc, &total)))
+                return rc;
+            hipLaunchKernelGGL(k_class_scatter, dim3(ntiles), dim3(256), 0, ctx->stream, S.cls,
+                               S.starts, nruns, ntiles, S.pos, lbuf);
+            PQ_CHECK(hipGetLastError());
+            // each class's first place (the scan at its first tile) back on the host
+            for (int c = 0; c < NCLS; c++)
+                PQ_CHECK(hipMemcpyAsync(&first[c], S.pos + (uint64_t) c * ntiles, 4,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+            PQ_CHECK(hipStreamSynchronize(ctx->stream));
+            first[NCLS] = total;
+        } else {
+            for (int c = 0; c <= NCLS; c++)
+                first[c] = 0;
+        }
+        for (int c = 0; c < NCLS; c++) {
+            lists[c] = lbuf + first[c];
+            cnt[c] = first[c + 1] - first[c];
+        }
     }
     // sort every bucket past its first K codes
     uint2 *l3[CLS_BIG] = {lists[0], lists[1], lists[2], lists[3]};
