@@ -132,6 +132,38 @@ __global__ void __launch_bounds__(256) k_slice_count(Geom G, const uint64_t *t2,
         bcnt[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
+// Suffixes of every slice per tile of SLICE_TILE positions, in one pass over the
+// text: bcnt[s * ntiles + tile] for slice s = [bounds[s], bounds[s + 1]) of keys.
+#define SLICE_MAX 16
+__global__ void __launch_bounds__(256) k_slice_count_all(Geom G, const uint64_t *t2, uint64_t N,
+                                                         unsigned K, const uint64_t *bounds,
+                                                         uint32_t nsl, uint64_t ntiles,
+                                                         uint32_t *bcnt)
+{
+    __shared__ uint32_t cnt[SLICE_MAX];
+    __shared__ uint64_t bd[SLICE_MAX + 1];
+    if (threadIdx.x < SLICE_MAX)
+        cnt[threadIdx.x] = 0;
+    if (threadIdx.x <= nsl)
+        bd[threadIdx.x] = bounds[threadIdx.x];
+    __syncthreads();
+    const uint64_t tile = (uint64_t) blockIdx.x * SLICE_TILE;
+    for (int r = 0; r < SLICE_TILE / 256; r++) {
+        const uint64_t g = tile + (uint64_t) r * 256 + threadIdx.x;
+        if (g < N) {
+            const uint64_t k = bucket_key(G, t2, g, K);
+            uint32_t sl = 0;
+            while (sl + 1 < nsl && k >= bd[sl + 1])
+                sl++;
+            if (k >= bd[0] && k < bd[nsl])
+                atomicAdd(&cnt[sl], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nsl)
+        bcnt[(uint64_t) threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
 // Keys and positions of the slice's suffixes, in position order (the radix
 // passes that follow are stable, and runs of identical complete suffixes rely
 // on position order).
@@ -1043,8 +1075,25 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
         (rc = pq_slot(ctx, SLOT_CLS, smax + 1, &S.cls)) ||
         (rc = pq_slot(ctx, SLOT_COUNTS, 64, &S.counts)) ||
         (rc = pq_slot(ctx, SLOT_BWT8, N + 1, &D.bwt8)) ||
-        (slices.size() > 1 && (rc = pq_slot(ctx, SLOT_SLICE_TILES, ntiles * 8 + 8, &bcnt))))
+        (slices.size() > 1 &&
+         (rc = pq_slot(ctx, SLOT_SLICE_TILES, (slices.size() + 1) * (ntiles + 1) * 4, &bcnt))))
         return rc;
+    // every slice's suffixes per tile, in one pass (up to SLICE_MAX slices)
+    const bool count_all = slices.size() > 1 && slices.size() <= SLICE_MAX;
+    if (count_all) {
+        std::vector<uint64_t> hb(slices.size() + 1);
+        for (size_t i = 0; i < slices.size(); i++)
+            hb[i] = slices[i].lo;
+        hb[slices.size()] = slices.back().hi;
+        uint64_t *db = nullptr;
+        if ((rc = pq_slot(ctx, SLOT_SLICE_BOUNDS, hb.size() * 8, &db)))
+            return rc;
+        PQ_CHECK(hipMemcpyAsync(db, hb.data(), hb.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_slice_count_all, dim3((unsigned) ntiles), dim3(256), 0, ctx->stream,
+                           G, D.text2, N, K, db, (uint32_t) slices.size(), ntiles, bcnt);
+        PQ_CHECK(hipGetLastError());
+        PQ_CHECK(hipStreamSynchronize(ctx->stream));   // hb leaves scope
+    }
     S.flags = BK2;
     S.pos = (uint32_t *) V2;
     PQ_CHECK(hipMemsetAsync(D.bwt8, BWT_UNSET, N, ctx->stream));
@@ -1076,11 +1125,14 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
             std::swap(vc, va);
             p0 = 1;
         } else {
-            uint32_t *boff = bcnt + ntiles + 1;
-            hipLaunchKernelGGL(k_slice_count, dim3((unsigned) ntiles), dim3(256), 0, ctx->stream,
-                               G, D.text2, N, K, sl.lo, sl.hi, bcnt);
+            const size_t si = (size_t) (&sl - slices.data());
+            uint32_t *boff = bcnt + slices.size() * ntiles;
+            uint32_t *scnt = bcnt + (count_all ? si * ntiles : 0);
+            if (!count_all)
+                hipLaunchKernelGGL(k_slice_count, dim3((unsigned) ntiles), dim3(256), 0,
+                                   ctx->stream, G, D.text2, N, K, sl.lo, sl.hi, scnt);
             uint32_t got = 0;
-            if ((rc = pq_scan_u32(ctx, bcnt, boff, ntiles, &got)))
+            if ((rc = pq_scan_u32(ctx, scnt, boff, ntiles, &got)))
                 return rc;
             if (got != ns)
                 return ctx->fail(PQ_ERR_STATE, "slice count disagrees with its histogram");
