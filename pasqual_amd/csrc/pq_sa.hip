@@ -1229,20 +1229,28 @@ __global__ void k_occ_counts(OccLine *occ, const uint4 *excl, uint64_t nlines)
     }
 }
 
+// reads_index: the read of every '$' row at its '$' rank.  One lane per 64-row
+// half line: the '$' rows are the zeros of its three planes (rows past N carry
+// symbol 7), about one in 150 rows for 150-code reads.
 template <typename R>
 __global__ void k_reads_index(Geom G, const R *sa, const OccLine *occ, uint64_t N, uint32_t *ri)
 {
-    PQ_GRID_STRIDE_FOR(row, N)
+    const uint64_t nh = (N + 63) / 64;
+    PQ_GRID_STRIDE_FOR(h, nh)
     {
-        uint64_t line = row >> 7;
-        int half = (int) ((row >> 6) & 1);
-        uint64_t a = occ[line].p0[half], b = occ[line].p1[half], c = occ[line].p2[half];
-        unsigned j = (unsigned) (row & 63);
-        if ((a >> j) & 1 || (b >> j) & 1 || (c >> j) & 1)
-            continue;
-        Ranks rk;
-        pq_ranks(occ, row, rk);
-        ri[rk.r[0]] = (uint32_t) pq_read_of(G, (uint64_t) sa[row]);
+        const uint64_t line = h >> 1;
+        const int half = (int) (h & 1);
+        uint64_t dollar = ~(occ[line].p0[half] | occ[line].p1[half] | occ[line].p2[half]);
+        const uint64_t row0 = h * 64;
+        if (N - row0 < 64)
+            dollar &= (1ull << (N - row0)) - 1ull;
+        while (dollar) {
+            const uint64_t row = row0 + (uint64_t) (__ffsll((long long) dollar) - 1);
+            dollar &= dollar - 1ull;
+            Ranks rk;
+            pq_ranks(occ, row, rk);
+            ri[rk.r[0]] = (uint32_t) pq_read_of(G, (uint64_t) sa[row]);
+        }
     }
 }
 
@@ -1287,10 +1295,10 @@ int pq_build_occ_dir(pq_ctx_t *ctx, int d)
     hipLaunchKernelGGL(k_occ_counts, dim3(pq_blocks(D.nlines, 256)), dim3(256), 0, ctx->stream,
                        D.occ, excl, D.nlines);
     if (ctx->wide)
-        hipLaunchKernelGGL(k_reads_index<uint64_t>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_reads_index<uint64_t>, dim3(pq_blocks_gs((N + 63) / 64, 256)), dim3(256), 0,
                            ctx->stream, G, (const uint64_t *) D.sa, D.occ, N, D.reads_index);
     else
-        hipLaunchKernelGGL(k_reads_index<uint32_t>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_reads_index<uint32_t>, dim3(pq_blocks_gs((N + 63) / 64, 256)), dim3(256), 0,
                            ctx->stream, G, (const uint32_t *) D.sa, D.occ, N, D.reads_index);
     PQ_CHECK(hipGetLastError());
     D.bwt8 = nullptr;   // the byte array is scratch shared by both directions
