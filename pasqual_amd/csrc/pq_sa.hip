@@ -1183,37 +1183,35 @@ __global__ void k_bwt_fill(Geom G, const uint64_t *t2, const R *sa, uint64_t N, 
     }
 }
 
-// 256 threads = 2 lines per tile; each wave builds one 64-row half of a line.
+// One wave per occurrence line of 128 rows (grid-stride over lines): the three
+// bit planes of both 64-row halves by ballots, written by one lane with the
+// line's counts of A, C, G, T (no LDS, no block barrier).  Rows past N carry
+// symbol 7.
 __global__ void __launch_bounds__(256) k_occ_planes(const uint8_t *bwt8, uint64_t N,
                                                     uint64_t nlines, OccLine *occ, uint4 *cnts)
 {
-    __shared__ uint32_t part[4][4];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const uint64_t ntiles = (nlines + 1) / 2;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        uint64_t row = tile * 256 + threadIdx.x;
-        unsigned sym = row < N ? bwt8[row] : 7u;
-        uint64_t b0 = __ballot(sym & 1u), b1 = __ballot(sym & 2u), b2 = __ballot(sym & 4u);
-        uint64_t line = row >> 7;
-        int half = (int) ((row >> 6) & 1);
-        if (lane == 0 && line < nlines) {
-            occ[line].p0[half] = b0;
-            occ[line].p1[half] = b1;
-            occ[line].p2[half] = b2;
-            part[wid][0] = __popcll(b0 & ~b1 & ~b2);
-            part[wid][1] = __popcll(~b0 & b1 & ~b2);
-            part[wid][2] = __popcll(b0 & b1 & ~b2);
-            part[wid][3] = __popcll(~b0 & ~b1 & b2);
+    const int lane = threadIdx.x & 63;
+    const uint64_t nwaves = ((uint64_t) gridDim.x * blockDim.x) >> 6;
+    for (uint64_t line = ((uint64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6; line < nlines;
+         line += nwaves) {
+        const uint64_t r0 = line * 128 + (uint64_t) lane, r1 = r0 + 64;
+        const unsigned s0 = r0 < N ? bwt8[r0] : 7u, s1 = r1 < N ? bwt8[r1] : 7u;
+        const uint64_t a0 = __ballot(s0 & 1u), b0 = __ballot(s0 & 2u), c0 = __ballot(s0 & 4u);
+        const uint64_t a1 = __ballot(s1 & 1u), b1 = __ballot(s1 & 2u), c1 = __ballot(s1 & 4u);
+        if (lane == 0) {
+            OccLine &ln = occ[line];
+            ln.p0[0] = a0;
+            ln.p0[1] = a1;
+            ln.p1[0] = b0;
+            ln.p1[1] = b1;
+            ln.p2[0] = c0;
+            ln.p2[1] = c1;
+            cnts[line] = make_uint4(
+                __popcll(a0 & ~b0 & ~c0) + __popcll(a1 & ~b1 & ~c1),
+                __popcll(~a0 & b0 & ~c0) + __popcll(~a1 & b1 & ~c1),
+                __popcll(a0 & b0 & ~c0) + __popcll(a1 & b1 & ~c1),
+                __popcll(~a0 & ~b0 & c0) + __popcll(~a1 & ~b1 & c1));
         }
-        __syncthreads();
-        if (threadIdx.x < 2) {
-            uint64_t ln = tile * 2 + threadIdx.x;
-            int w0 = 2 * threadIdx.x;
-            if (ln < nlines)
-                cnts[ln] = make_uint4(part[w0][0] + part[w0 + 1][0], part[w0][1] + part[w0 + 1][1],
-                                      part[w0][2] + part[w0 + 1][2], part[w0][3] + part[w0 + 1][3]);
-        }
-        __syncthreads();
     }
 }
 
@@ -1287,7 +1285,7 @@ int pq_build_occ_dir(pq_ctx_t *ctx, int d)
         D.keys_D = kD;
         D.keys_pending = 0;
     }
-    hipLaunchKernelGGL(k_occ_planes, dim3(pq_blocks_gs((D.nlines + 1) / 2, 1)), dim3(256), 0,
+    hipLaunchKernelGGL(k_occ_planes, dim3(pq_blocks_gs(D.nlines, 4)), dim3(256), 0,
                        ctx->stream, D.bwt8, N, D.nlines, D.occ, cnts);
     uint32_t tot[4];
     if ((rc = pq_scan_u4(ctx, cnts, excl, D.nlines, tot)))
