@@ -2224,13 +2224,14 @@ static int find_overlaps_t(pq_ctx_t *ctx, int mode, bool as_graph, ExportSpec *x
     HugeRead *huge = nullptr;
     // five events per batch (seed, find, extract boundaries; the seed kernel's start
     // after the model's prediction), read after the loop
-    // batch boundaries: equal batches of at most `batch` reads; a search that
-    // exports as it goes ends with a batch of about a quarter of the others, so
-    // that little transfer is left after its last kernels
+    // batch boundaries: equal batches of at most `batch` reads, then (above a
+    // million reads) a batch of about a quarter of the others, so that a search
+    // that exports as it goes has little transfer left after its last kernels
+    // (every search is batched alike: the same launches with or without export)
     std::vector<uint64_t> bnd(1, 0);
     {
         uint64_t head = n, last = 0;
-        if (xp && n >= (1u << 20)) {
+        if (n >= (1u << 20)) {
             const uint64_t k = (4 * n + 5 * batch - 1) / (5 * batch) > 0
                                    ? (4 * n + 5 * batch - 1) / (5 * batch)
                                    : 1;                       // full batches, last ~ size / 4
