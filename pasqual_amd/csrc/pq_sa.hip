@@ -245,14 +245,16 @@ __device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, R g,
     }
 }
 
+// A padding element: all-ones words, length and position, so that it sorts after
+// every suffix.
 template <int W, typename R> __device__ __forceinline__ void suf_none(SufW<W, R> &s)
 {
     s.g = (R) ~(R) 0;
-    s.len = 0;
+    s.len = ~0u;
     s.sym = BWT_UNSET;
 #pragma unroll
     for (int i = 0; i < W; i++)
-        s.w[i] = 0;
+        s.w[i] = ~0ull;
 }
 
 // Sampled keys of the learned index written by the run sorts as they place their
@@ -292,31 +294,22 @@ __device__ __forceinline__ void emit_key(const KeyOut &ko, const Geom &G, const 
 }
 
 // Canonical order of two suffixes sharing their first `base` codes: string
-// order with the end of the read first, then position.
+// order with the end of the read first, then position.  The cached words are zero
+// past a suffix's end, so comparing the words, then the lengths, then the
+// positions gives that order (a proper prefix is smaller or equal word by word,
+// then shorter) with no branch; padding elements (suf_none) compare last.
 template <int W, typename R>
 __device__ __forceinline__ bool suf_less(const SufW<W, R> &a, const SufW<W, R> &b)
 {
-    const R none = (R) ~(R) 0;
-    if (a.g == none || b.g == none)
-        return a.g < b.g;
-    uint32_t m = a.len < b.len ? a.len : b.len;
+    bool lt = false, eq = true;
 #pragma unroll
     for (int i = 0; i < W; i++) {
-        if ((uint32_t) (32 * i) < m) {
-            uint64_t x = a.w[i], y = b.w[i];
-            uint32_t left = m - 32 * i;
-            if (left < 32) {
-                uint64_t mask = ~0ull << (64 - 2 * left);
-                x &= mask;
-                y &= mask;
-            }
-            if (x != y)
-                return x < y;
-        }
+        lt = lt | (eq & (a.w[i] < b.w[i]));
+        eq = eq & (a.w[i] == b.w[i]);
     }
-    if (a.len != b.len)
-        return a.len < b.len;
-    return a.g < b.g;
+    lt = lt | (eq & (a.len < b.len));
+    eq = eq & (a.len == b.len);
+    return lt | (eq & (a.g < b.g));
 }
 
 template <int J, typename R> __device__ __forceinline__ R xor_lane_r(R v)
