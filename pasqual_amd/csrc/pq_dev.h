@@ -218,7 +218,7 @@ __device__ __forceinline__ uint64_t pq_rem(const Geom &G, uint64_t g)
 __device__ __forceinline__ bool pq_is_start(const Geom &G, uint64_t g)
 {
     if (G.L)
-        return g % G.L == 0;
+        return pq_read_of(G, g) * G.L == g;   // no 64-bit division
     if (g == 0)
         return true;
     uint64_t p = g - 1;

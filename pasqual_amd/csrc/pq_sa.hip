@@ -1163,16 +1163,33 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d)
 // BWT symbols of the rows no sort wrote (one-suffix buckets, buckets of
 // identical complete suffixes, rows placed by the chunk-level radix passes), and
 // those rows' sampled keys when the build writes keys (keys != nullptr).
+// One lane per 8 rows: their BWT bytes are read as one word, and only the unset
+// ones (a few percent) do any work.
 template <typename R>
 __global__ void k_bwt_fill(Geom G, const uint64_t *t2, const R *sa, uint64_t N, uint8_t *bwt8,
                            u128 *keys, uint32_t shift, uint32_t D)
 {
-    PQ_GRID_STRIDE_FOR(row, N)
-    if (bwt8[row] == BWT_UNSET) {
-        const uint64_t g = (uint64_t) sa[row];
-        bwt8[row] = (uint8_t) bwt_sym(G, t2, g);
-        if (keys && !(row & ((1ull << shift) - 1ull)))
-            keys[row >> shift] = suffix_key(G, t2, g, D);
+    const uint64_t n8 = (N + 7) / 8;
+    PQ_GRID_STRIDE_FOR(q, n8)
+    {
+        const uint64_t r0 = q * 8;
+        // bwt8 holds N + 1 bytes: the last word may be partial
+        uint64_t w = 0;
+        if (r0 + 8 <= N) {
+            w = *(const uint64_t *) (bwt8 + r0);
+        } else {
+            for (uint64_t r = r0; r < N; r++)
+                w |= (uint64_t) bwt8[r] << (8 * (r - r0));
+        }
+        for (int j = 0; j < 8; j++) {
+            const uint64_t row = r0 + (uint64_t) j;
+            if (row >= N || ((w >> (8 * j)) & 0xFFu) != BWT_UNSET)
+                continue;
+            const uint64_t g = (uint64_t) sa[row];
+            bwt8[row] = (uint8_t) bwt_sym(G, t2, g);
+            if (keys && !(row & ((1ull << shift) - 1ull)))
+                keys[row >> shift] = suffix_key(G, t2, g, D);
+        }
     }
 }
 
@@ -1266,11 +1283,11 @@ int pq_build_occ_dir(pq_ctx_t *ctx, int d)
     const uint32_t kshift = D.keys_pending ? (uint32_t) __builtin_ctz(D.keys_pending) : 0u;
     const uint32_t kD = std::min<uint32_t>(63, ctx->t);
     if (ctx->wide)
-        hipLaunchKernelGGL(k_bwt_fill<uint64_t>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_bwt_fill<uint64_t>, dim3(pq_blocks_gs((N + 7) / 8, 256)), dim3(256), 0,
                            ctx->stream, G, D.text2, (const uint64_t *) D.sa, N, D.bwt8, keys,
                            kshift, kD);
     else
-        hipLaunchKernelGGL(k_bwt_fill<uint32_t>, dim3(pq_blocks_gs(N, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_bwt_fill<uint32_t>, dim3(pq_blocks_gs((N + 7) / 8, 256)), dim3(256), 0,
                            ctx->stream, G, D.text2, (const uint32_t *) D.sa, N, D.bwt8, keys,
                            kshift, kD);
     if (D.keys_pending) {
