@@ -95,7 +95,14 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(Src src, KT *ko
         (&wcnt[0][0])[i] = 0;
     __syncthreads();
 
-    const uint64_t tile = (uint64_t) blockIdx.x * RADIX_TILE;
+    // Blocks go round-robin to the 8 XCDs; the first 8 * (nblocks / 8) blocks take
+    // tiles so that each XCD has a contiguous range of them, and the rest keep
+    // their own tile.  A 64-byte line of `offs` holds one digit's offsets of 16
+    // consecutive tiles, so it is then read through one XCD's L2.
+    const uint32_t q8 = nblocks >> 3;
+    const uint32_t tb = blockIdx.x < 8 * q8 ? (blockIdx.x & 7u) * q8 + (blockIdx.x >> 3)
+                                            : blockIdx.x;
+    const uint64_t tile = (uint64_t) tb * RADIX_TILE;
     const uint64_t wbase = tile + (uint64_t) wid * (RADIX_TILE / 4);
     const uint64_t lt = (1ull << lane) - 1ull;
     KT key[RADIX_ROUNDS];
@@ -166,7 +173,7 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(Src src, KT *ko
     for (uint32_t p = threadIdx.x; p < valid; p += RADIX_THREADS) {
         KT k = sk[p];
         unsigned d = (unsigned) (k >> shift) & 255u;
-        uint64_t o = (uint64_t) offs[(uint64_t) d * nblocks + blockIdx.x] + (p - tstart[d]);
+        uint64_t o = (uint64_t) offs[(uint64_t) d * nblocks + tb] + (p - tstart[d]);
         kout[o] = k;
         vout[o] = sv[p];
     }
