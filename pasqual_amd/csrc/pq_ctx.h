@@ -146,7 +146,6 @@ enum PqSlot {
     SLOT_BIGRUNS, SLOT_NODES, SLOT_MISC2, SLOT_HUGE, SLOT_HUGE_LIST, SLOT_VJOB_L, SLOT_VJOB,
     SLOT_WORKQ, SLOT_WSTATS,
     SLOT_XSTAGE0 = 60, SLOT_XSTAGE1, SLOT_XTOT, SLOT_XSCAN, SLOT_SKEY0, SLOT_SKEY1,
-    SLOT_SLICE_BOUNDS,
 };
 
 // Grow-only slot buffer of at least `bytes` bytes into *p, or PQ_ERR_NOMEM.

@@ -90,7 +90,40 @@ template <typename R> struct BucketSrc {
 // scratch is bounded by the slice size instead of N.
 
 #define SLICE_BINS 4096      // histogram over the top 12 key bits
-#define SLICE_TILE 4096      // positions per block of the compaction (16 x 256)
+#define SLICE_TILE 8192      // positions per block of the slice kernels (32 per thread)
+
+// Bucket keys of the text positions 32 j .. 32 j + 31 below N, in order:
+// keys[i] for position 32 j + i (i < returned count).  The codes come from two
+// words loaded once and the codes left in the read are carried from position
+// to position: one read lookup per 32 positions instead of a window load and a
+// read lookup per position, which made the slice passes compute-bound.
+__device__ __forceinline__ uint32_t bucket_keys32(const Geom &G, const uint64_t *t2, uint64_t N,
+                                                  unsigned K, uint64_t j, uint32_t keys[32])
+{
+    const uint64_t g0 = j << 5;
+    if (g0 >= N)
+        return 0;
+    const uint64_t w0 = t2[j], w1 = t2[j + 1];   // slack words follow the text
+    uint64_t r = pq_read_of(G, g0);
+    uint32_t rem = (uint32_t) (pq_read_start(G, r) + pq_read_len(G, r) - g0);
+    const uint32_t n = N - g0 < 32 ? (uint32_t) (N - g0) : 32u;
+#pragma unroll
+    for (uint32_t i = 0; i < 32; i++) {
+        if (i < n) {
+            while (rem == 0)
+                rem = pq_read_len(G, ++r);
+            const unsigned sh = 2 * i;
+            const uint64_t win = (w0 << sh) | ((w1 >> 1) >> (63 - sh));
+            const uint32_t f = rem < K ? rem : K;
+            uint32_t chars = (uint32_t) (win >> (64 - 2 * K));
+            if (f < K)
+                chars &= ~((1u << (2 * (K - f))) - 1u);
+            keys[i] = (chars << 4) | f;
+            rem--;
+        }
+    }
+    return n;
+}
 
 __global__ void __launch_bounds__(256) k_key_hist(Geom G, const uint64_t *t2, uint64_t N,
                                                   unsigned K, unsigned shift,
@@ -100,12 +133,31 @@ __global__ void __launch_bounds__(256) k_key_hist(Geom G, const uint64_t *t2, ui
     for (uint32_t i = threadIdx.x; i < SLICE_BINS; i += blockDim.x)
         h[i] = 0;
     __syncthreads();
-    PQ_GRID_STRIDE_FOR(g, N)
-    atomicAdd(&h[bucket_key(G, t2, g, K) >> shift], 1u);
+    PQ_GRID_STRIDE_FOR(j, (N + 31) >> 5)
+    {
+        uint32_t keys[32];
+        const uint32_t n = bucket_keys32(G, t2, N, K, j, keys);
+#pragma unroll
+        for (uint32_t i = 0; i < 32; i++)
+            if (i < n)
+                atomicAdd(&h[keys[i] >> shift], 1u);
+    }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < SLICE_BINS; i += blockDim.x)
         if (h[i])
             atomicAdd(&hist[i], (unsigned long long) h[i]);
+}
+
+// Block sum of one value per thread (256 threads); the result in every thread.
+__device__ __forceinline__ uint32_t block_sum256(uint32_t c, uint32_t *ws)
+{
+    for (int d = 32; d > 0; d >>= 1)
+        c += __shfl_xor(c, d, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        ws[threadIdx.x >> 6] = c;
+    __syncthreads();
+    return ws[0] + ws[1] + ws[2] + ws[3];
 }
 
 // Suffixes of the slice [lo, hi) of keys per tile of SLICE_TILE positions.
@@ -114,92 +166,114 @@ __global__ void __launch_bounds__(256) k_slice_count(Geom G, const uint64_t *t2,
                                                      uint32_t *bcnt)
 {
     __shared__ uint32_t ws[4];
-    const uint64_t tile = (uint64_t) blockIdx.x * SLICE_TILE;
+    uint32_t keys[32];
+    const uint64_t j = (uint64_t) blockIdx.x * (SLICE_TILE / 32) + threadIdx.x;
+    const uint32_t n = bucket_keys32(G, t2, N, K, j, keys);
     uint32_t c = 0;
-    for (int r = 0; r < SLICE_TILE / 256; r++) {
-        uint64_t g = tile + (uint64_t) r * 256 + threadIdx.x;
-        if (g < N) {
-            uint64_t k = bucket_key(G, t2, g, K);
-            c += (k >= lo && k < hi) ? 1u : 0u;
-        }
-    }
-    for (int d = 32; d > 0; d >>= 1)
-        c += __shfl_xor(c, d, 64);
-    if ((threadIdx.x & 63) == 0)
-        ws[threadIdx.x >> 6] = c;
-    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < 32; i++)
+        c += (i < n && keys[i] >= lo && keys[i] < hi) ? 1u : 0u;
+    c = block_sum256(c, ws);
     if (threadIdx.x == 0)
-        bcnt[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+        bcnt[blockIdx.x] = c;
 }
 
-// Suffixes of every slice per tile of SLICE_TILE positions, in one pass over the
-// text: bcnt[s * ntiles + tile] for slice s = [bounds[s], bounds[s + 1]) of keys.
+// Slice bounds for k_slice_count_all, clamped to 32 bits: b[s] = first key of
+// slice s; past the last slice all ones, which no key reaches (a key's low four
+// bits, its length field, are at most 14).
 #define SLICE_MAX 16
+struct SliceBounds {
+    uint32_t b[SLICE_MAX + 1];
+};
+
+// Suffixes of every slice per tile of SLICE_TILE positions, in one pass over the
+// text: bcnt[s * ntiles + tile] for slice s = [b[s], b[s + 1]) of keys, from the
+// counts of keys at or above each bound (bound 0 is key 0).  NS >= the slice count.
+template <int NS>
 __global__ void __launch_bounds__(256) k_slice_count_all(Geom G, const uint64_t *t2, uint64_t N,
-                                                         unsigned K, const uint64_t *bounds,
-                                                         uint32_t nsl, uint64_t ntiles,
-                                                         uint32_t *bcnt)
+                                                         unsigned K, SliceBounds sb, uint32_t nsl,
+                                                         uint64_t ntiles, uint32_t *bcnt)
 {
-    __shared__ uint32_t cnt[SLICE_MAX];
-    __shared__ uint64_t bd[SLICE_MAX + 1];
-    if (threadIdx.x < SLICE_MAX)
-        cnt[threadIdx.x] = 0;
-    if (threadIdx.x <= nsl)
-        bd[threadIdx.x] = bounds[threadIdx.x];
-    __syncthreads();
-    const uint64_t tile = (uint64_t) blockIdx.x * SLICE_TILE;
-    for (int r = 0; r < SLICE_TILE / 256; r++) {
-        const uint64_t g = tile + (uint64_t) r * 256 + threadIdx.x;
-        if (g < N) {
-            const uint64_t k = bucket_key(G, t2, g, K);
-            uint32_t sl = 0;
-            while (sl + 1 < nsl && k >= bd[sl + 1])
-                sl++;
-            if (k >= bd[0] && k < bd[nsl])
-                atomicAdd(&cnt[sl], 1u);
-        }
+    __shared__ uint32_t part[4][NS];
+    uint32_t keys[32];
+    const uint64_t j = (uint64_t) blockIdx.x * (SLICE_TILE / 32) + threadIdx.x;
+    const uint32_t n = bucket_keys32(G, t2, N, K, j, keys);
+    uint32_t ge[NS];
+    ge[0] = n;
+#pragma unroll
+    for (int s = 1; s < NS; s++)
+        ge[s] = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 32; i++)
+        if (i < n)
+#pragma unroll
+            for (int s = 1; s < NS; s++)
+                ge[s] += keys[i] >= sb.b[s] ? 1u : 0u;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        uint32_t c = ge[s];
+        for (int d = 32; d > 0; d >>= 1)
+            c += __shfl_xor(c, d, 64);
+        if (lane == 0)
+            part[wid][s] = c;
     }
     __syncthreads();
-    if (threadIdx.x < nsl)
-        bcnt[(uint64_t) threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+    const uint32_t s = threadIdx.x;
+    if (s < nsl) {
+        const uint32_t a = part[0][s] + part[1][s] + part[2][s] + part[3][s];
+        const uint32_t b = s + 1 < (uint32_t) NS
+                               ? part[0][s + 1] + part[1][s + 1] + part[2][s + 1] + part[3][s + 1]
+                               : 0u;
+        bcnt[(uint64_t) s * ntiles + blockIdx.x] = a - b;
+    }
 }
 
 // Keys and positions of the slice's suffixes, in position order (the radix
 // passes that follow are stable, and runs of identical complete suffixes rely
-// on position order).
+// on position order).  A lane makes the keys of 32 consecutive positions and
+// parks them in LDS; its wave then takes its 2048 positions 64 at a time, so the
+// slice members of each round are written as one coalesced run.
 template <typename R>
 __global__ void __launch_bounds__(256) k_slice_emit(Geom G, const uint64_t *t2, uint64_t N,
                                                     unsigned K, uint64_t lo, uint64_t hi,
                                                     const uint32_t *boff, uint32_t *BK, R *V)
 {
+    __shared__ uint32_t sk[4][64 * 33];   // a lane's 32 keys 33 words apart: no bank conflicts
     __shared__ uint32_t ws[4];
-    const uint64_t tile = (uint64_t) blockIdx.x * SLICE_TILE;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t keys[32];
+    const uint64_t j = (uint64_t) blockIdx.x * (SLICE_TILE / 32) + threadIdx.x;
+    const uint32_t n = bucket_keys32(G, t2, N, K, j, keys);
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 32; i++) {
+        const bool in = i < n && keys[i] >= lo && keys[i] < hi;
+        c += in ? 1u : 0u;
+        // all ones marks a non-member: no key has 15 in its length field
+        sk[wid][33 * lane + i] = in ? keys[i] : 0xFFFFFFFFu;
+    }
+    for (int d = 32; d > 0; d >>= 1)
+        c += __shfl_xor(c, d, 64);
+    if (lane == 0)
+        ws[wid] = c;
+    __syncthreads();
     uint32_t base = boff[blockIdx.x];
-    for (int r = 0; r < SLICE_TILE / 256; r++) {
-        uint64_t g = tile + (uint64_t) r * 256 + threadIdx.x;
-        uint32_t k = 0;
-        bool in = false;
-        if (g < N) {
-            k = bucket_key(G, t2, g, K);
-            in = (uint64_t) k >= lo && (uint64_t) k < hi;
-        }
-        uint64_t m = __ballot(in);
-        if (lane == 0)
-            ws[wid] = (uint32_t) __popcll(m);
-        __syncthreads();
-        uint32_t pre = 0, tot = 0;
-        for (int w = 0; w < 4; w++) {
-            pre += w < wid ? ws[w] : 0u;
-            tot += ws[w];
-        }
+    for (int w = 0; w < wid; w++)
+        base += ws[w];
+    const uint64_t g0 = (uint64_t) blockIdx.x * SLICE_TILE + (uint64_t) wid * 2048;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t q = 0; q < 32; q++) {
+        const uint32_t p = q * 64 + lane;
+        const uint32_t k = sk[wid][33 * (p >> 5) + (p & 31)];
+        const bool in = k != 0xFFFFFFFFu;
+        const uint64_t m = __ballot(in);
         if (in) {
-            uint32_t at = base + pre + (uint32_t) __popcll(m & ((1ull << lane) - 1ull));
+            const uint32_t at = base + (uint32_t) __popcll(m & lt);
             BK[at] = k;
-            V[at] = (R) g;
+            V[at] = (R) (g0 + p);
         }
-        base += tot;
-        __syncthreads();
+        base += (uint32_t) __popcll(m);
     }
 }
 
@@ -1014,8 +1088,8 @@ static int plan_slices(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigne
     if ((rc = pq_slot(ctx, SLOT_SLICE_HIST, SLICE_BINS * 8, &hist)))
         return rc;
     PQ_CHECK(hipMemsetAsync(hist, 0, SLICE_BINS * 8, ctx->stream));
-    hipLaunchKernelGGL(k_key_hist, dim3(std::min<unsigned>(1024, pq_blocks(N, 256))), dim3(256), 0,
-                       ctx->stream, G, t2, N, K, shift, hist);
+    hipLaunchKernelGGL(k_key_hist, dim3(std::min<unsigned>(1024, pq_blocks((N + 31) / 32, 256))),
+                       dim3(256), 0, ctx->stream, G, t2, N, K, shift, hist);
     PQ_CHECK(hipGetLastError());
     std::vector<uint64_t> h(SLICE_BINS);
     PQ_CHECK(hipMemcpyAsync(h.data(), hist, SLICE_BINS * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -1074,18 +1148,22 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
     // every slice's suffixes per tile, in one pass (up to SLICE_MAX slices)
     const bool count_all = slices.size() > 1 && slices.size() <= SLICE_MAX;
     if (count_all) {
-        std::vector<uint64_t> hb(slices.size() + 1);
-        for (size_t i = 0; i < slices.size(); i++)
-            hb[i] = slices[i].lo;
-        hb[slices.size()] = slices.back().hi;
-        uint64_t *db = nullptr;
-        if ((rc = pq_slot(ctx, SLOT_SLICE_BOUNDS, hb.size() * 8, &db)))
-            return rc;
-        PQ_CHECK(hipMemcpyAsync(db, hb.data(), hb.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(k_slice_count_all, dim3((unsigned) ntiles), dim3(256), 0, ctx->stream,
-                           G, D.text2, N, K, db, (uint32_t) slices.size(), ntiles, bcnt);
+        SliceBounds sb;
+        for (size_t i = 0; i <= SLICE_MAX; i++)
+            sb.b[i] = i < slices.size() ? (uint32_t) std::min<uint64_t>(slices[i].lo, 0xFFFFFFFFull)
+                                        : 0xFFFFFFFFu;
+        const uint32_t nsl = (uint32_t) slices.size();
+        const dim3 grid((unsigned) ntiles);
+        if (nsl <= 4)
+            hipLaunchKernelGGL(k_slice_count_all<4>, grid, dim3(256), 0, ctx->stream, G, D.text2,
+                               N, K, sb, nsl, ntiles, bcnt);
+        else if (nsl <= 8)
+            hipLaunchKernelGGL(k_slice_count_all<8>, grid, dim3(256), 0, ctx->stream, G, D.text2,
+                               N, K, sb, nsl, ntiles, bcnt);
+        else
+            hipLaunchKernelGGL(k_slice_count_all<16>, grid, dim3(256), 0, ctx->stream, G,
+                               D.text2, N, K, sb, nsl, ntiles, bcnt);
         PQ_CHECK(hipGetLastError());
-        PQ_CHECK(hipStreamSynchronize(ctx->stream));   // hb leaves scope
     }
     S.flags = BK2;
     S.pos = (uint32_t *) V2;
