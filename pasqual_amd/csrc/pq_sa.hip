@@ -432,7 +432,7 @@ __device__ __forceinline__ void bitonic_stages(SufW<W, R> &s, int li)
 
 // ---- run lists -----------------------------------------------------------------------
 
-#define NCLS 5              // run size classes, see seg_class
+#define NCLS 7              // run size classes, see seg_class
 
 struct SegLists {
     uint2 *items[NCLS];
@@ -445,18 +445,26 @@ struct SegLists {
 #define RUN_SMALL 16
 #define RUN_HALF 32
 #define RUN_WAVE 64
+#define RUN_WAVE2 128       // two suffixes per lane of one wave
+#define RUN_WAVE4 256       // four per lane
 #define CLS_BIG (NCLS - 1)
 #define CLS_DONE 255
 
-// 0: <= 16 elements, 1: <= 32, 2: <= 64 (lanes of one wave), 3: <= lcap (LDS),
-// 4: larger.  lcap 0 means the remaining suffix length exceeds the cached
-// words: everything goes to the chunk-level path.
+// 0: <= 16 elements, 1: <= 32, 2: <= 64 (lanes of one wave), 3: <= 128 and
+// 4: <= 256 (registers of one wave), 5: <= lcap (LDS, lcap >= 512), 6: larger.
+// lcap 0 means the remaining suffix length exceeds the cached words: everything
+// goes to the chunk-level path.
 __device__ __forceinline__ int seg_class(uint32_t size, uint32_t lcap)
 {
     if (lcap == 0)
         return CLS_BIG;
-    return size <= RUN_SMALL ? 0 : size <= RUN_HALF ? 1 : size <= RUN_WAVE ? 2
-                                 : size <= lcap ? 3 : CLS_BIG;
+    return size <= RUN_SMALL   ? 0
+           : size <= RUN_HALF  ? 1
+           : size <= RUN_WAVE  ? 2
+           : size <= RUN_WAVE2 ? 3
+           : size <= RUN_WAVE4 ? 4
+           : size <= lcap      ? 5
+                               : CLS_BIG;
 }
 
 // Wave-aggregated append (used only for the runs split off oversized runs);
@@ -513,6 +521,87 @@ __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2
         bwt8[sg.x + li] = (uint8_t) s.sym;
         if (ko.keys)
             emit_key<W, R>(ko, Gm, t2, sg.x + li, ko.BK[sg.x], s);
+    }
+}
+
+// Bitonic network over M x 64 suffixes held M per lane by one wave: element
+// e = m * 64 + lane.  Steps J < 64 pair lanes (DPP / swizzle / permute), steps
+// J >= 64 pair items m and m ^ (J / 64) of the same lane.
+template <int M, int K, int J, int W, typename R>
+__device__ __forceinline__ void wave_steps(SufW<W, R> (&s)[M], int lane)
+{
+    if constexpr (J >= 64) {
+        constexpr int JM = J / 64;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            if ((m & JM) == 0) {
+                // the lower item keeps the smaller of the pair when the merge runs up
+                const bool up = ((m * 64) & K) == 0;
+                const bool bless = suf_less<W, R>(s[m + JM], s[m]);
+                if (up ? bless : (!bless && s[m + JM].g != s[m].g)) {
+                    const SufW<W, R> x = s[m];
+                    s[m] = s[m + JM];
+                    s[m + JM] = x;
+                }
+            }
+        }
+    } else {
+        const bool lower = (lane & J) == 0;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const bool up = ((m * 64 + lane) & K) == 0;
+            const SufW<W, R> p = suf_xor<J, W, R>(s[m]);
+            const bool pless = suf_less<W, R>(p, s[m]);
+            const bool swap = (lower == up) ? pless : (!pless && p.g != s[m].g);
+            if (swap)
+                s[m] = p;
+        }
+    }
+    if constexpr (J > 1)
+        wave_steps<M, K, J / 2, W, R>(s, lane);
+}
+
+template <int M, int K, int W, typename R>
+__device__ __forceinline__ void wave_stages(SufW<W, R> (&s)[M], int lane)
+{
+    wave_steps<M, K, K / 2, W, R>(s, lane);
+    if constexpr (K < 64 * M)
+        wave_stages<M, 2 * K, W, R>(s, lane);
+}
+
+// One wave per run of up to M x 64 suffixes (4 runs per block), sorted in
+// registers: all of a lane's suffix words are loaded up front, and no step
+// waits on LDS or a block barrier.
+template <int M, int W, typename R>
+__global__ void __launch_bounds__(256) k_sort_wave(Geom Gm, const uint64_t *t2, R *sa,
+                                                   uint8_t *bwt8, uint32_t base,
+                                                   const uint2 *segs, uint32_t nseg, KeyOut ko)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t sid = ((uint64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (sid >= nseg)
+        return;   // whole waves: sid is uniform over a wave
+    const uint2 sg = segs[sid];
+    SufW<W, R> s[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const uint32_t e = (uint32_t) (m * 64 + lane);
+        if (e < sg.y)
+            suf_load<W, R>(Gm, t2, sa[sg.x + e], base, s[m]);
+        else
+            suf_none<W, R>(s[m]);
+    }
+    wave_stages<M, 2, W, R>(s, lane);
+    const uint32_t bk = ko.keys && !ko.from_text ? ko.BK[sg.x] : 0u;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const uint32_t e = (uint32_t) (m * 64 + lane);
+        if (e < sg.y) {
+            sa[sg.x + e] = s[m].g;
+            bwt8[sg.x + e] = (uint8_t) s[m].sym;
+            if (ko.keys)
+                emit_key<W, R>(ko, Gm, t2, sg.x + e, bk, s[m]);
+        }
     }
 }
 
@@ -660,7 +749,7 @@ __global__ void __launch_bounds__(256) k_class_counts(const uint8_t *cls, uint32
 {
     __shared__ uint32_t ws[4][NCLS];
     const uint64_t tile = (uint64_t) blockIdx.x * RUN_TILE;
-    uint32_t c[NCLS] = {0, 0, 0, 0, 0};
+    uint32_t c[NCLS] = {};
     for (int r = 0; r < RUN_TILE / 256; r++) {
         const uint64_t k = tile + (uint64_t) r * 256 + threadIdx.x;
         const uint32_t x = k < nruns ? cls[k] : (uint32_t) CLS_DONE;
@@ -864,8 +953,20 @@ static void launch_sorts(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, R *sa
     }
     for (uint32_t at = 0; at < cnt[3]; at += SORT_RUNS_PER_LAUNCH) {
         const uint32_t m = std::min(cnt[3] - at, SORT_RUNS_PER_LAUNCH);
+        hipLaunchKernelGGL((k_sort_wave<2, W, R>), dim3(pq_blocks((uint64_t) m * 64, 256)),
+                           dim3(256), 0, ctx->stream, G, t2, sa, bwt8, base, items[3] + at, m,
+                           ko);
+    }
+    for (uint32_t at = 0; at < cnt[4]; at += SORT_RUNS_PER_LAUNCH) {
+        const uint32_t m = std::min(cnt[4] - at, SORT_RUNS_PER_LAUNCH);
+        hipLaunchKernelGGL((k_sort_wave<4, W, R>), dim3(pq_blocks((uint64_t) m * 64, 256)),
+                           dim3(256), 0, ctx->stream, G, t2, sa, bwt8, base, items[4] + at, m,
+                           ko);
+    }
+    for (uint32_t at = 0; at < cnt[5]; at += SORT_RUNS_PER_LAUNCH) {
+        const uint32_t m = std::min(cnt[5] - at, SORT_RUNS_PER_LAUNCH);
         hipLaunchKernelGGL((k_sort_lds<W, R>), dim3(m), dim3(256), 0, ctx->stream, G, t2, sa,
-                           bwt8, base, items[3] + at, ko);
+                           bwt8, base, items[5] + at, ko);
     }
 }
 
@@ -885,8 +986,9 @@ static int sort_lists(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, R *sa, u
     case 7: launch_sorts<7, R>(ctx, G, t2, sa, bwt8, base, items, cnt, ko); break;
     case 8: launch_sorts<8, R>(ctx, G, t2, sa, bwt8, base, items, cnt, ko); break;
     default:
-        if (cnt[0] || cnt[1] || cnt[2] || cnt[3])
-            return ctx->fail(PQ_ERR_STATE, "run classified for cached words it cannot hold");
+        for (int c = 0; c < CLS_BIG; c++)
+            if (cnt[c])
+                return ctx->fail(PQ_ERR_STATE, "run classified for cached words it cannot hold");
     }
     PQ_CHECK(hipGetLastError());
     return PQ_OK;
@@ -931,7 +1033,7 @@ static int sort_runs(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned 
                        S.BK, S.starts, nruns, K, lcap0, S.cls);
     // the class lists one after the other in one buffer (together at most nruns):
     // runs per class and tile, one scan of those counts, one scatter
-    uint32_t cnt[NCLS] = {0, 0, 0, 0, 0};
+    uint32_t cnt[NCLS] = {};
     uint2 *lists[NCLS];
     uint2 *lbuf = nullptr;
     if ((rc = pq_slot(ctx, SLOT_LISTS, ((size_t) nruns + NCLS) * sizeof(uint2), &lbuf)))
@@ -965,8 +1067,7 @@ static int sort_runs(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned 
         }
     }
     // sort every bucket past its first K codes
-    uint2 *l3[CLS_BIG] = {lists[0], lists[1], lists[2], lists[3]};
-    if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, K, l3, cnt, ko)))
+    if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, K, lists, cnt, ko)))
         return rc;
     // deeper levels: their runs share more than the bucket's codes; their keys come
     // from the text
@@ -1052,8 +1153,7 @@ static int sort_runs(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigned 
         if (c5[NCLS])
             return ctx->fail(PQ_ERR_LIMIT, "suffix sort run list overflow");
         base = nbase;
-        uint2 *sl[CLS_BIG] = {SL.items[0], SL.items[1], SL.items[2], SL.items[3]};
-        if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, base, sl, c5, ko_text)))
+        if ((rc = sort_lists<R>(ctx, G, t2, S.sa, S.bwt8, base, SL.items, c5, ko_text)))
             return rc;
         big.assign(c5[CLS_BIG], make_uint2(0, 0));
         if (c5[CLS_BIG]) {

@@ -71,11 +71,13 @@ def test_index_matches_reference_fixture(ctx, name):
 
 @pytest.mark.parametrize("kind,seed", [("fixed20", 1), ("fixed50", 2), ("fixed101", 3),
                                        ("varlen", 4), ("varlen", 5), ("ties", 6),
-                                       ("long250", 7)])
+                                       ("long250", 7), ("deep", 8)])
 def test_index_matches_oracle(ctx, kind, seed):
     rng = np.random.default_rng(seed)
     if kind == "ties":          # tiny genome, many identical suffixes
         codes, off, _ = _random_reads(rng, 70, 3000, 30, 30)
+    elif kind == "deep":        # ~130x coverage: bucket runs of about 65..512 suffixes
+        codes, off, _ = _random_reads(rng, 3000, 8000, 50, 50, err=0.002)
     elif kind == "varlen":
         codes, off, _ = _random_reads(rng, 20000, 3000, 36, 250, err=0.003)
     elif kind == "long250":
