@@ -451,7 +451,8 @@ struct SegLists {
 #define CLS_DONE 255
 
 // 0: <= 16 elements, 1: <= 32, 2: <= 64 (lanes of one wave), 3: <= 128 and
-// 4: <= 256 (registers of one wave), 5: <= lcap (LDS, lcap >= 512), 6: larger.
+// 4: <= 256 (registers of one wave), 5: <= lcap (LDS, or the registers of one
+// wave when lcap is 512), 6: larger.
 // lcap 0 means the remaining suffix length exceeds the cached words: everything
 // goes to the chunk-level path.
 __device__ __forceinline__ int seg_class(uint32_t size, uint32_t lcap)
@@ -965,8 +966,14 @@ static void launch_sorts(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, R *sa
     }
     for (uint32_t at = 0; at < cnt[5]; at += SORT_RUNS_PER_LAUNCH) {
         const uint32_t m = std::min(cnt[5] - at, SORT_RUNS_PER_LAUNCH);
-        hipLaunchKernelGGL((k_sort_lds<W, R>), dim3(m), dim3(256), 0, ctx->stream, G, t2, sa,
-                           bwt8, base, items[5] + at, ko);
+        if constexpr (lds_cap(W) <= 512)
+            // up to 512 suffixes: eight per lane of one wave
+            hipLaunchKernelGGL((k_sort_wave<8, W, R>), dim3(pq_blocks((uint64_t) m * 64, 256)),
+                               dim3(256), 0, ctx->stream, G, t2, sa, bwt8, base, items[5] + at, m,
+                               ko);
+        else
+            hipLaunchKernelGGL((k_sort_lds<W, R>), dim3(m), dim3(256), 0, ctx->stream, G, t2, sa,
+                               bwt8, base, items[5] + at, ko);
     }
 }
 
