@@ -57,11 +57,18 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(Src src, uint64_t 
             (&h[0][0])[i] = 0;
         __syncthreads();
         const uint64_t base = (uint64_t) (t0 + g) * RADIX_TILE;
+        // loads at clamped indices, unconditionally: a load under a bounds branch
+        // is waited for before the next one is issued
+        KT kv[ROUNDS];
+#pragma unroll
         for (int r = 0; r < ROUNDS; r++) {
             const uint64_t i = base + (uint64_t) r * RADIX_THREADS + threadIdx.x;
-            if (i < count)
-                atomicAdd(&h[wid][(src.key(i) >> shift) & 255u], 1u);
+            kv[r] = src.key(i < count ? i : count - 1);
         }
+#pragma unroll
+        for (int r = 0; r < ROUNDS; r++)
+            if (base + (uint64_t) r * RADIX_THREADS + threadIdx.x < count)
+                atomicAdd(&h[wid][(kv[r] >> shift) & 255u], 1u);
         __syncthreads();
         cnt[g] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
         __syncthreads();
@@ -108,14 +115,17 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(Src src, KT *ko
     KT key[RADIX_ROUNDS];
     VT val[RADIX_ROUNDS];
     uint32_t rnk[RADIX_ROUNDS];
+    // every load of the wave's quarter first, at clamped indices and without a
+    // bounds branch (a load under one is waited for before the next is issued)
+#pragma unroll
+    for (int r = 0; r < RADIX_ROUNDS; r++) {
+        const uint64_t i = wbase + (uint64_t) r * 64 + lane;
+        src.load(i < count ? i : count - 1, key[r], val[r]);
+    }
 #pragma unroll
     for (int r = 0; r < RADIX_ROUNDS; r++) {
         uint64_t i = wbase + (uint64_t) r * 64 + lane;
         bool ok = i < count;
-        key[r] = 0;
-        val[r] = 0;
-        if (ok)
-            src.load(i, key[r], val[r]);
         unsigned d = (unsigned) (key[r] >> shift) & 255u;
         uint64_t peers = __ballot(ok);
 #pragma unroll
