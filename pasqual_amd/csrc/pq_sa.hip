@@ -302,19 +302,32 @@ template <int W, typename R>
 __device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, R g, uint32_t base,
                                          SufW<W, R> &s)
 {
-    uint32_t rem = suf_rem(G, g);
+    const uint32_t rem = suf_rem(G, g);
     s.g = g;
-    s.sym = bwt_sym(G, t2, g);
     s.len = rem > base ? rem - base : 0;
+    // The W + 1 text words the windows come from and the word holding the code
+    // before g, all loaded unconditionally (word indices clamped to the packed
+    // text's slack words), so that they are in flight together: a load under a
+    // length branch is waited for before the next one is issued.
+    const uint64_t wmax = (G.N >> 5) + 3;
+    const uint64_t p0 = (uint64_t) g + base, w0 = p0 >> 5;
+    uint64_t raw[W + 1];
+#pragma unroll
+    for (int i = 0; i <= W; i++)
+        raw[i] = t2[w0 + i < wmax ? w0 + i : wmax];
+    const uint64_t gp = g ? (uint64_t) g - 1 : 0;
+    const uint64_t prev = t2[gp >> 5];
+    s.sym = pq_is_start(G, g) ? 0u : (unsigned) ((prev >> (62 - 2 * (gp & 31))) & 3u) + 1u;
+    const unsigned sh = (unsigned) (p0 & 31) * 2;
 #pragma unroll
     for (int i = 0; i < W; i++) {
-        uint64_t x = 0;
-        if ((uint32_t) (32 * i) < s.len) {
-            x = pq_window(t2, (uint64_t) g + base + 32 * i);
-            uint32_t left = s.len - 32 * i;
-            if (left < 32)
-                x &= ~0ull << (64 - 2 * left);
-        }
+        // window i: 32 codes from p0 + 32 i, zero past the suffix's end
+        uint64_t x = (raw[i] << sh) | ((raw[i + 1] >> 1) >> (63 - sh));
+        const uint32_t lo = 32u * (uint32_t) i;
+        if (lo >= s.len)
+            x = 0;
+        else if (s.len - lo < 32)
+            x &= ~0ull << (64 - 2 * (s.len - lo));
         s.w[i] = x;
     }
 }
@@ -584,14 +597,21 @@ __global__ void __launch_bounds__(256) k_sort_wave(Geom Gm, const uint64_t *t2, 
         return;   // whole waves: sid is uniform over a wave
     const uint2 sg = segs[sid];
     SufW<W, R> s[M];
+    // every item loaded (the run's last one past its end: sg.y >= 1), then the
+    // padding marked, so that no load waits behind a branch
+    R gs[M];
 #pragma unroll
     for (int m = 0; m < M; m++) {
         const uint32_t e = (uint32_t) (m * 64 + lane);
-        if (e < sg.y)
-            suf_load<W, R>(Gm, t2, sa[sg.x + e], base, s[m]);
-        else
-            suf_none<W, R>(s[m]);
+        gs[m] = sa[sg.x + (e < sg.y ? e : sg.y - 1)];
     }
+#pragma unroll
+    for (int m = 0; m < M; m++)
+        suf_load<W, R>(Gm, t2, gs[m], base, s[m]);
+#pragma unroll
+    for (int m = 0; m < M; m++)
+        if ((uint32_t) (m * 64 + lane) >= sg.y)
+            suf_none<W, R>(s[m]);
     wave_stages<M, 2, W, R>(s, lane);
     const uint32_t bk = ko.keys && !ko.from_text ? ko.BK[sg.x] : 0u;
 #pragma unroll
