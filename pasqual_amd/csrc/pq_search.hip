@@ -1395,64 +1395,81 @@ __device__ __forceinline__ void reg_side(const Geom &Gm, const DirViewT<R> &fw, 
             it[h].w[i] = ~0ull;
         it[h].tie = ~0ull;
     }
-    // items li * RI + h: entry (the last e with pre[e] <= q), read, key; every
-    // slot's loads independent of the others'
+    // items li * RI + h: entry (the last e with pre[e] <= q), read, key.  Every
+    // load is issued unconditionally at a safe address (index 0 or a clamped text
+    // word for slots with no item), so that a lane's RI chains of loads are in
+    // flight together: a load under a per-slot branch is waited for before the
+    // next slot's is issued.
     uint64_t mm[RI];
     uint32_t oo[RI];
-    bool fst[RI];
+    bool fst[RI], act[RI];
+    uint2 lch[RI];
+    uint64_t ia[RI];
 #pragma unroll
     for (int h = 0; h < RI; h++) {
         const uint32_t q = (uint32_t) (li * RI + h);
-        mm[h] = 0;
-        oo[h] = 0;
-        fst[h] = true;
-        if (q < k) {
-            uint32_t lo = 0, hi = ne;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (pre[mid] <= q)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            const uint2 lc = e.lc[side][lo];
-            oo[h] = e.o[side][lo];
-            fst[h] = lo < nA;
-            const int ch = fst[h] ? chA : chB;
-            // FB, BB searched the forward index; FF, BF the backward index
-            const bool plain = (ch == 0 || ch == 3);
-            mm[h] = entry_read(lc, q - pre[lo], plain ? fw.reads_index : bw.reads_index);
+        act[h] = q < k;
+        uint32_t lo = 0, hi = ne;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (pre[mid] <= q)
+                lo = mid;
+            else
+                hi = mid;
         }
+        lch[h] = e.lc[side][lo];
+        oo[h] = act[h] ? (uint32_t) e.o[side][lo] : 0u;
+        fst[h] = lo < nA;
+        ia[h] = act[h] && !(lch[h].y & ENTRY_READ) ? (uint64_t) lch[h].x + (q - pre[lo]) : 0ull;
+    }
+    uint32_t rv[RI];
+#pragma unroll
+    for (int h = 0; h < RI; h++) {
+        // FB, BB searched the forward index; FF, BF the backward index
+        const int ch = fst[h] ? chA : chB;
+        rv[h] = ((ch == 0 || ch == 3) ? fw.reads_index : bw.reads_index)[ia[h]];
+    }
+#pragma unroll
+    for (int h = 0; h < RI; h++)
+        mm[h] = !act[h] ? 0ull : (lch[h].y & ENTRY_READ) ? (uint64_t) lch[h].x : (uint64_t) rv[h];
+    // key m[o..) (FB, BB) or rc(m)[o..) (FF, BF: the complement of the backward
+    // text), zero past its length: KW + 1 words per slot
+    const uint64_t wmax = (Gm.N >> 5) + 3;   // last slack word of a packed text
+    uint64_t raw[RI][KW + 1];
+    uint32_t klen[RI], sh[RI];
+    bool val[RI], pl[RI];
+#pragma unroll
+    for (int h = 0; h < RI; h++) {
+        const int ch = fst[h] ? chA : chB;
+        pl[h] = (ch == 0 || ch == 3);
+        const uint32_t Lm = pq_read_len(Gm, mm[h]);
+        val[h] = act[h] && oo[h] < Lm;
+        klen[h] = val[h] ? Lm - oo[h] : 0u;
+        const uint64_t ks = val[h] ? pq_read_start(Gm, mm[h]) + oo[h] : 0ull;
+        const uint64_t *tx = pl[h] ? fw.text2 : bw.text2;
+        const uint64_t w0 = ks >> 5;
+        sh[h] = (uint32_t) (ks & 31) * 2;
+#pragma unroll
+        for (int i = 0; i <= KW; i++)
+            raw[h][i] = tx[w0 + i < wmax ? w0 + i : wmax];
     }
 #pragma unroll
     for (int h = 0; h < RI; h++) {
-        const uint32_t q = (uint32_t) (li * RI + h);
-        if (q < k) {
-            const int ch = fst[h] ? chA : chB;
-            const bool plain = (ch == 0 || ch == 3);
-            const uint32_t Lm = pq_read_len(Gm, mm[h]);
-            if (oo[h] < Lm) {
-                // key m[o..) (FB, BB) or rc(m)[o..) (FF, BF: the complement of the
-                // backward text), zero past its length
-                const uint32_t klen = Lm - oo[h];
-                const uint64_t *tx = plain ? fw.text2 : bw.text2;
-                const uint64_t ks = pq_read_start(Gm, mm[h]) + oo[h];
+        if (val[h]) {
 #pragma unroll
-                for (int i = 0; i < KW; i++) {
-                    uint64_t x = 0;
-                    if (32u * i < klen) {
-                        x = pq_window(tx, ks + 32u * i);
-                        if (!plain)
-                            x = ~x;
-                        const uint32_t left = klen - 32u * i;
-                        if (left < 32)
-                            x &= ~0ull << (64 - 2 * left);
-                    }
-                    it[h].w[i] = x;
-                }
-                it[h].tie = ((uint64_t) klen << 49) | ((uint64_t) (fst[h] ? 0u : 1u) << 48) |
-                            ((uint64_t) oo[h] << 32) | (uint64_t) (uint32_t) mm[h];
+            for (int i = 0; i < KW; i++) {
+                uint64_t x = (raw[h][i] << sh[h]) | ((raw[h][i + 1] >> 1) >> (63 - sh[h]));
+                if (!pl[h])
+                    x = ~x;
+                const uint32_t lo = 32u * (uint32_t) i;
+                if (lo >= klen[h])
+                    x = 0;
+                else if (klen[h] - lo < 32)
+                    x &= ~0ull << (64 - 2 * (klen[h] - lo));
+                it[h].w[i] = x;
             }
+            it[h].tie = ((uint64_t) klen[h] << 49) | ((uint64_t) (fst[h] ? 0u : 1u) << 48) |
+                        ((uint64_t) oo[h] << 32) | (uint64_t) (uint32_t) mm[h];
         }
     }
     reg_sort<N, 2, KW>(it, li);
@@ -1579,6 +1596,8 @@ __global__ void __launch_bounds__(256, (G == 4 ? 4 : 1)) k_extract_reg(Geom Gm, 
         bool ok = have && ne[0] <= (uint32_t) N && ne[1] <= (uint32_t) N;
         // both sides' entries (lane li: entries li * RI .. li * RI + RI - 1 of each
         // side) in one round of loads
+        // (loaded unconditionally -- a slot with no entry reads its chain's first
+        // record -- so that the loads are in flight together)
         uint2 lcs[2][RI];
         uint16_t os[2][RI];
 #pragma unroll
@@ -1586,16 +1605,21 @@ __global__ void __launch_bounds__(256, (G == 4 ? 4 : 1)) k_extract_reg(Geom Gm, 
 #pragma unroll
             for (int j = 0; j < RI; j++) {
                 const uint32_t en = (uint32_t) (li * RI + j);
-                lcs[sd][j] = make_uint2(0, 0);
-                os[sd][j] = 0;
-                if (ok && en < ne[sd]) {
-                    const int cA = sd ? 0 : 2, cB = sd ? 1 : 3;
-                    const uint64_t at = en < nA[sd] ? (rl * 4 + cA) * cb.cap + en
-                                                    : (rl * 4 + cB) * cb.cap + (en - nA[sd]);
-                    lcs[sd][j] = cb.lc[at];
-                    os[sd][j] = cb.o[at];
-                }
+                const int cA = sd ? 0 : 2, cB = sd ? 1 : 3;
+                const uint64_t at = !(ok && en < ne[sd]) ? (rl * 4 + cA) * cb.cap
+                                    : en < nA[sd]        ? (rl * 4 + cA) * cb.cap + en
+                                                  : (rl * 4 + cB) * cb.cap + (en - nA[sd]);
+                lcs[sd][j] = cb.lc[at];
+                os[sd][j] = cb.o[at];
             }
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++)
+#pragma unroll
+            for (int j = 0; j < RI; j++)
+                if (!(ok && (uint32_t) (li * RI + j) < ne[sd])) {
+                    lcs[sd][j] = make_uint2(0, 0);
+                    os[sd][j] = 0;
+                }
         // items per side and each entry's exclusive prefix, into LDS
         uint32_t k[2];
 #pragma unroll
