@@ -68,6 +68,35 @@ __device__ __forceinline__ uint32_t bucket_key(const Geom &G, const uint64_t *t2
     return (chars << 4) | f;
 }
 
+// BWT symbol of the row holding suffix g: the code before g, or '$' (0) at a
+// read start.  Carried through the sort and written next to the row it
+// finalises.
+__device__ __forceinline__ unsigned bwt_sym(const Geom &G, const uint64_t *t2, uint64_t g)
+{
+    return pq_is_start(G, g) ? 0u : pq_code(t2, g - 1) + 1u;
+}
+
+// With 64-bit rows a suffix-array value carries its row's BWT symbol in its top
+// three bits (position | symbol << 61) from the bucket pass until a sort writes
+// the row or k_bwt_fill finalises it: the run sorts take the symbol from there,
+// and the rows no sort touches get theirs without a text read.  32-bit rows have
+// no room; their symbols come from the text.
+#define SA_SYM_SHIFT 61
+template <typename R> __device__ __forceinline__ uint64_t sa_pos(R v)
+{
+    if constexpr (sizeof(R) == 8)
+        return (uint64_t) v & ((1ull << SA_SYM_SHIFT) - 1ull);
+    else
+        return (uint64_t) v;
+}
+template <typename R> __device__ __forceinline__ R sa_pack(uint64_t g, unsigned sym)
+{
+    if constexpr (sizeof(R) == 8)
+        return (R) (g | ((uint64_t) sym << SA_SYM_SHIFT));
+    else
+        return (R) g;
+}
+
 // The (bucket key, position) pairs of all N suffixes in position order, made
 // on the fly by the first radix pass of a build in one slice (no key array is
 // written and read back).
@@ -79,7 +108,7 @@ template <typename R> struct BucketSrc {
     __device__ __forceinline__ void load(uint64_t g, uint32_t &k, R &v) const
     {
         k = bucket_key(G, t2, g, K);
-        v = (R) g;
+        v = sa_pack<R>(g, sizeof(R) == 8 ? bwt_sym(G, t2, g) : 0u);
     }
 };
 
@@ -97,21 +126,33 @@ template <typename R> struct BucketSrc {
 // words loaded once and the codes left in the read are carried from position
 // to position: one read lookup per 32 positions instead of a window load and a
 // read lookup per position, which made the slice passes compute-bound.
+template <bool SYM = false>
 __device__ __forceinline__ uint32_t bucket_keys32(const Geom &G, const uint64_t *t2, uint64_t N,
-                                                  unsigned K, uint64_t j, uint32_t keys[32])
+                                                  unsigned K, uint64_t j, uint32_t keys[32],
+                                                  uint8_t *syms = nullptr)
 {
     const uint64_t g0 = j << 5;
     if (g0 >= N)
         return 0;
     const uint64_t w0 = t2[j], w1 = t2[j + 1];   // slack words follow the text
     uint64_t r = pq_read_of(G, g0);
-    uint32_t rem = (uint32_t) (pq_read_start(G, r) + pq_read_len(G, r) - g0);
+    uint32_t rlen = pq_read_len(G, r);
+    uint32_t rem = (uint32_t) (pq_read_start(G, r) + rlen - g0);
     const uint32_t n = N - g0 < 32 ? (uint32_t) (N - g0) : 32u;
+    // with SYM, syms[i] = BWT symbol of the row of position 32 j + i: '$' (0) at
+    // a read start, else the code before it + 1
+    uint32_t prev = 0;
+    if constexpr (SYM)
+        prev = j ? (uint32_t) (t2[j - 1] & 3u) : 0u;
 #pragma unroll
     for (uint32_t i = 0; i < 32; i++) {
         if (i < n) {
-            while (rem == 0)
-                rem = pq_read_len(G, ++r);
+            while (rem == 0) {
+                rlen = pq_read_len(G, ++r);
+                rem = rlen;
+            }
+            if constexpr (SYM)
+                syms[i] = (uint8_t) (rem == rlen ? 0u : prev + 1u);
             const unsigned sh = 2 * i;
             const uint64_t win = (w0 << sh) | ((w1 >> 1) >> (63 - sh));
             const uint32_t f = rem < K ? rem : K;
@@ -119,6 +160,8 @@ __device__ __forceinline__ uint32_t bucket_keys32(const Geom &G, const uint64_t 
             if (f < K)
                 chars &= ~((1u << (2 * (K - f))) - 1u);
             keys[i] = (chars << 4) | f;
+            if constexpr (SYM)
+                prev = (uint32_t) (w0 >> (62 - 2 * i)) & 3u;
             rem--;
         }
     }
@@ -240,11 +283,14 @@ __global__ void __launch_bounds__(256) k_slice_emit(Geom G, const uint64_t *t2, 
                                                     const uint32_t *boff, uint32_t *BK, R *V)
 {
     __shared__ uint32_t sk[4][64 * 33];   // a lane's 32 keys 33 words apart: no bank conflicts
+    __shared__ uint8_t ssy[4][64 * 33];   // their BWT symbols (64-bit rows carry them)
     __shared__ uint32_t ws[4];
+    constexpr bool SYM = sizeof(R) == 8;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     uint32_t keys[32];
+    uint8_t syms[32];
     const uint64_t j = (uint64_t) blockIdx.x * (SLICE_TILE / 32) + threadIdx.x;
-    const uint32_t n = bucket_keys32(G, t2, N, K, j, keys);
+    const uint32_t n = bucket_keys32<SYM>(G, t2, N, K, j, keys, syms);
     uint32_t c = 0;
 #pragma unroll
     for (uint32_t i = 0; i < 32; i++) {
@@ -252,6 +298,8 @@ __global__ void __launch_bounds__(256) k_slice_emit(Geom G, const uint64_t *t2, 
         c += in ? 1u : 0u;
         // all ones marks a non-member: no key has 15 in its length field
         sk[wid][33 * lane + i] = in ? keys[i] : 0xFFFFFFFFu;
+        if constexpr (SYM)
+            ssy[wid][33 * lane + i] = syms[i];
     }
     for (int d = 32; d > 0; d >>= 1)
         c += __shfl_xor(c, d, 64);
@@ -271,7 +319,7 @@ __global__ void __launch_bounds__(256) k_slice_emit(Geom G, const uint64_t *t2, 
         if (in) {
             const uint32_t at = base + (uint32_t) __popcll(m & lt);
             BK[at] = k;
-            V[at] = (R) (g0 + p);
+            V[at] = sa_pack<R>(g0 + p, SYM ? ssy[wid][33 * (p >> 5) + (p & 31)] : 0u);
         }
         base += (uint32_t) __popcll(m);
     }
@@ -288,22 +336,16 @@ template <int W, typename R> struct SufW {
     uint64_t w[W];
 };
 
-// BWT symbol of the row holding suffix g: the code before g, or '$' (0) at a
-// read start.  Loaded with the suffix's other words, carried through the sort
-// and written next to the row it finalises.
-__device__ __forceinline__ unsigned bwt_sym(const Geom &G, const uint64_t *t2, uint64_t g)
-{
-    return pq_is_start(G, g) ? 0u : pq_code(t2, g - 1) + 1u;
-}
 
 #define BWT_UNSET 0xFFu
 
 template <int W, typename R>
-__device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, R g, uint32_t base,
+__device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, R v, uint32_t base,
                                          SufW<W, R> &s)
 {
+    const uint64_t g = sa_pos<R>(v);   // v: a suffix-array value of the slice being sorted
     const uint32_t rem = suf_rem(G, g);
-    s.g = g;
+    s.g = (R) g;
     s.len = rem > base ? rem - base : 0;
     // The W + 1 text words the windows come from and the word holding the code
     // before g, all loaded unconditionally (word indices clamped to the packed
@@ -315,9 +357,13 @@ __device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, R g,
 #pragma unroll
     for (int i = 0; i <= W; i++)
         raw[i] = t2[w0 + i < wmax ? w0 + i : wmax];
-    const uint64_t gp = g ? (uint64_t) g - 1 : 0;
-    const uint64_t prev = t2[gp >> 5];
-    s.sym = pq_is_start(G, g) ? 0u : (unsigned) ((prev >> (62 - 2 * (gp & 31))) & 3u) + 1u;
+    if constexpr (sizeof(R) == 8) {
+        s.sym = (unsigned) ((uint64_t) v >> SA_SYM_SHIFT);
+    } else {
+        const uint64_t gp = g ? g - 1 : 0;
+        const uint64_t prev = t2[gp >> 5];
+        s.sym = pq_is_start(G, g) ? 0u : (unsigned) ((prev >> (62 - 2 * (gp & 31))) & 3u) + 1u;
+    }
     const unsigned sh = (unsigned) (p0 & 31) * 2;
 #pragma unroll
     for (int i = 0; i < W; i++) {
@@ -524,6 +570,9 @@ __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2
     bool have = sid < nseg;
     uint2 sg = have ? segs[sid] : make_uint2(0, 0);
     bool e = have && (uint32_t) li < sg.y;
+    // the run's bucket key (for the sampled keys) loaded with the suffix, not
+    // after the sort
+    const uint32_t bk = e && ko.keys && !ko.from_text ? ko.BK[sg.x] : 0u;
     SufW<W, R> s;
     if (e)
         suf_load<W, R>(Gm, t2, sa[sg.x + li], base, s);
@@ -534,7 +583,7 @@ __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2
         sa[sg.x + li] = s.g;
         bwt8[sg.x + li] = (uint8_t) s.sym;
         if (ko.keys)
-            emit_key<W, R>(ko, Gm, t2, sg.x + li, ko.BK[sg.x], s);
+            emit_key<W, R>(ko, Gm, t2, sg.x + li, bk, s);
     }
 }
 
@@ -612,8 +661,8 @@ __global__ void __launch_bounds__(256) k_sort_wave(Geom Gm, const uint64_t *t2, 
     for (int m = 0; m < M; m++)
         if ((uint32_t) (m * 64 + lane) >= sg.y)
             suf_none<W, R>(s[m]);
-    wave_stages<M, 2, W, R>(s, lane);
     const uint32_t bk = ko.keys && !ko.from_text ? ko.BK[sg.x] : 0u;
+    wave_stages<M, 2, W, R>(s, lane);
 #pragma unroll
     for (int m = 0; m < M; m++) {
         const uint32_t e = (uint32_t) (m * 64 + lane);
@@ -686,11 +735,21 @@ __global__ void __launch_bounds__(256) k_run_counts(const KT *K, uint64_t n, uin
 {
     __shared__ uint32_t ws[4];
     const uint64_t tile = (uint64_t) blockIdx.x * RUN_TILE;
-    uint32_t c = 0;
+    // keys at clamped indices, loaded without a bounds branch (loads under one are
+    // waited for in turn)
+    KT a[RUN_TILE / 256], b[RUN_TILE / 256];
+#pragma unroll
     for (int r = 0; r < RUN_TILE / 256; r++) {
         const uint64_t i = tile + (uint64_t) r * 256 + threadIdx.x;
-        if (i < n)
-            c += (i == 0 || K[i] != K[i - 1]) ? 1u : 0u;
+        const uint64_t ic = i < n ? i : n - 1;
+        a[r] = K[ic];
+        b[r] = K[ic ? ic - 1 : 0];
+    }
+    uint32_t c = 0;
+#pragma unroll
+    for (int r = 0; r < RUN_TILE / 256; r++) {
+        const uint64_t i = tile + (uint64_t) r * 256 + threadIdx.x;
+        c += (i < n && (i == 0 || a[r] != b[r])) ? 1u : 0u;
     }
     for (int d = 32; d > 0; d >>= 1)
         c += __shfl_xor(c, d, 64);
@@ -711,9 +770,18 @@ __global__ void __launch_bounds__(256) k_run_starts(const KT *K, uint64_t n, con
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint64_t tile = (uint64_t) blockIdx.x * RUN_TILE;
     uint32_t base = toff[blockIdx.x];
+    KT a[RUN_TILE / 256], b[RUN_TILE / 256];   // as in k_run_counts
+#pragma unroll
     for (int r = 0; r < RUN_TILE / 256; r++) {
         const uint64_t i = tile + (uint64_t) r * 256 + threadIdx.x;
-        const bool head = i < n && (i == 0 || K[i] != K[i - 1]);
+        const uint64_t ic = i < n ? i : n - 1;
+        a[r] = K[ic];
+        b[r] = K[ic ? ic - 1 : 0];
+    }
+#pragma unroll
+    for (int r = 0; r < RUN_TILE / 256; r++) {
+        const uint64_t i = tile + (uint64_t) r * 256 + threadIdx.x;
+        const bool head = i < n && (i == 0 || a[r] != b[r]);
         const uint64_t m = __ballot(head);
         if (lane == 0)
             ws[wid] = (uint32_t) __popcll(m);
@@ -851,10 +919,11 @@ __global__ void __launch_bounds__(256) k_big_load(Geom G, const uint64_t *t2, co
         const uint4 r = run[j];
         for (uint32_t i = threadIdx.x; i < r.y; i += blockDim.x) {
             const uint32_t e = r.z + i;
-            const uint64_t g = sa[r.x + i];
+            const R v = sa[r.x + i];   // with its BWT symbol, for the rows no sort writes
+            const uint64_t g = sa_pos<R>(v);
             const uint64_t k = chunk_key(t2, g, suf_rem(G, g), base);
             rid[e] = j;
-            pos[e] = (R) g;
+            pos[e] = v;
             korig[e] = k;
             K[e] = k;
             E[e] = e;
@@ -1371,7 +1440,7 @@ int pq_build_sa_dir(pq_ctx_t *ctx, int d)
 // One lane per 8 rows: their BWT bytes are read as one word, and only the unset
 // ones (a few percent) do any work.
 template <typename R>
-__global__ void k_bwt_fill(Geom G, const uint64_t *t2, const R *sa, uint64_t N, uint8_t *bwt8,
+__global__ void k_bwt_fill(Geom G, const uint64_t *t2, R *sa, uint64_t N, uint8_t *bwt8,
                            u128 *keys, uint32_t shift, uint32_t D)
 {
     const uint64_t n8 = (N + 7) / 8;
@@ -1386,12 +1455,22 @@ __global__ void k_bwt_fill(Geom G, const uint64_t *t2, const R *sa, uint64_t N, 
             for (uint64_t r = r0; r < N; r++)
                 w |= (uint64_t) bwt8[r] << (8 * (r - r0));
         }
+        if (r0 + 8 <= N && (w & 0x8080808080808080ull) == 0)
+            continue;   // all eight set (a symbol is at most 4, BWT_UNSET has bit 7)
         for (int j = 0; j < 8; j++) {
             const uint64_t row = r0 + (uint64_t) j;
             if (row >= N || ((w >> (8 * j)) & 0xFFu) != BWT_UNSET)
                 continue;
-            const uint64_t g = (uint64_t) sa[row];
-            bwt8[row] = (uint8_t) bwt_sym(G, t2, g);
+            const R v = sa[row];
+            const uint64_t g = sa_pos<R>(v);
+            if constexpr (sizeof(R) == 8) {
+                // the symbol the bucket pass packed into the value; the row's
+                // final value without it
+                bwt8[row] = (uint8_t) ((uint64_t) v >> SA_SYM_SHIFT);
+                sa[row] = (R) g;
+            } else {
+                bwt8[row] = (uint8_t) bwt_sym(G, t2, g);
+            }
             if (keys && !(row & ((1ull << shift) - 1ull)))
                 keys[row >> shift] = suffix_key(G, t2, g, D);
         }
@@ -1489,11 +1568,11 @@ int pq_build_occ_dir(pq_ctx_t *ctx, int d)
     const uint32_t kD = std::min<uint32_t>(63, ctx->t);
     if (ctx->wide)
         hipLaunchKernelGGL(k_bwt_fill<uint64_t>, dim3(pq_blocks_gs((N + 7) / 8, 256)), dim3(256), 0,
-                           ctx->stream, G, D.text2, (const uint64_t *) D.sa, N, D.bwt8, keys,
+                           ctx->stream, G, D.text2, (uint64_t *) D.sa, N, D.bwt8, keys,
                            kshift, kD);
     else
         hipLaunchKernelGGL(k_bwt_fill<uint32_t>, dim3(pq_blocks_gs((N + 7) / 8, 256)), dim3(256), 0,
-                           ctx->stream, G, D.text2, (const uint32_t *) D.sa, N, D.bwt8, keys,
+                           ctx->stream, G, D.text2, (uint32_t *) D.sa, N, D.bwt8, keys,
                            kshift, kD);
     if (D.keys_pending) {
         D.keys_stride = D.keys_pending;
