@@ -419,6 +419,8 @@ def run_workload(pq, np, config, args, cpu_baseline):
         ctx.set_tuning(tn["shape"])
         ctx.set_find_chains(tn["find_chains"])
         ctx.set_verify(tn["verify_rows"])
+        if tn is tune_l:
+            ctx.set_seed_verify(tn["seed_rows"])
 
     def export():
         nonlocal edges_buf
@@ -652,6 +654,7 @@ def run_workload(pq, np, config, args, cpu_baseline):
             "edges_identical_to_plain": bool(Lr["same"]),
             "extraction_shape": {k: tune_l[k] for k in ("group_lanes", "items")},
             "find_kernel": tune_l["find_chains"], "verify_rows": tune_l["verify_rows"],
+            "seed_rows": tune_l["seed_rows"],
             "extract_ms": round(_mean(Lr["extract"]), 3),
             "phases_ms": {"k_learned_predict": round((seed_ms - seed_kernel) / args.steps, 3),
                           "k_learned_seed": round(seed_kernel / args.steps, 3),

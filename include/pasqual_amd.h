@@ -207,7 +207,7 @@ void pq_host_free(void *p);
  * an exact correction finds the interval: galloping + binary search over the
  * keys of every sample_stride-th row (the exact part, 16 bytes per sampled row),
  * then the rows within the bracketing strides on the suffix array and the text:
- * an interval that may hold at most the verification limit's rows (pq_set_verify)
+ * an interval that may hold at most the seed kernel's limit of rows (pq_set_seed_verify)
  * is scanned outward from the samples, and its chain finished on the text on
  * the way; a wider one by binary search.
  * Results equal PQ_SEARCH_PLAIN's for any model.  The model is dropped by
@@ -294,9 +294,14 @@ typedef struct {
     double extract_ms[4];    /* extraction time measured per shape */
     uint32_t find_chains;    /* candidate-search kernel chosen (1 to 4, below; 0 for
                                 PQ_SEARCH_LEARNED, which has one) */
-    double find_ms[4];       /* candidate-search time of kernels 1 to 4 */
+    double find_ms[4];       /* candidate-search time of kernels 1 to 4 (with
+                                PQ_SEARCH_LEARNED: seed phase + search) */
     uint32_t verify_rows;    /* verification threshold chosen (0, 8 or 32, pq_set_verify) */
-    double verify_ms[3];     /* candidate search + verification time with 0, 8 and 32 */
+    double verify_ms[3];     /* candidate search + verification time with 0, 8 and 32
+                                (with PQ_SEARCH_LEARNED: seed phase included) */
+    uint32_t seed_rows;      /* PQ_SEARCH_LEARNED: seed kernel's threshold chosen (0, 8
+                                or 32, pq_set_seed_verify); PQ_SEARCH_PLAIN: unchanged */
+    double seed_ms[3];       /* PQ_SEARCH_LEARNED: seed phase + search with 0, 8 and 32 */
 } pq_tune_t;
 int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out);
 int pq_set_tuning(pq_ctx_t *ctx, uint32_t shape);
@@ -319,6 +324,12 @@ int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains);
  * gone on to and the reads it would have recorded there (same result).  32 by
  * default; 0: every chain is searched backward to its end. */
 int pq_set_verify(pq_ctx_t *ctx, uint32_t rows);
+/* The seeded search's own threshold (PQ_SEARCH_LEARNED): a chain whose depth-D
+ * interval may hold at most `rows` rows (<= 32) is scanned and finished on the
+ * text by the seed kernel; wider ones, and all with 0, are handed to the search
+ * kernel (which verifies by pq_set_verify's threshold).  32 by default; same
+ * result for every value.  pq_autotune times 0, 8 and 32 for the seeded search. */
+int pq_set_seed_verify(pq_ctx_t *ctx, uint32_t rows);
 /* Initial capacity of the search's shared edge buffer: edges of read sides with
  * more than 16 irreducible edges (the first 16 of every side have slots of
  * their own).  0 (the default): max(2^20, reads / 4).  A search that needs more

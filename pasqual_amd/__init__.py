@@ -40,7 +40,7 @@ PQ_SYMBOLS = [
     "pq_synth_lengths", "pq_synth_reads", "pq_synth_fasta", "pq_import_fasta",
     "pq_dedup_reads", "pq_graph_nodes", "pq_probe_read", "pq_probe_gather", "pq_set_edge_capacity",
     "pq_get_slot_sizes", "pq_graph_export", "pq_host_alloc", "pq_host_free",
-    "pq_mlp_eval", "pq_set_verify", "pq_dedup_reads_noalloc", "pq_dev_alloc", "pq_dev_free", "pq_dev_upload", "pq_dev_download",
+    "pq_mlp_eval", "pq_set_verify", "pq_set_seed_verify", "pq_dedup_reads_noalloc", "pq_dev_alloc", "pq_dev_free", "pq_dev_upload", "pq_dev_download",
     "pq_find_overlaps_export", "pq_set_search_batch", "pq_reserve_search", "pq_set_learned_keys",
 ]
 NOALLOC_SCRATCH_BYTES = 32 << 20   # PQ_NOALLOC_SCRATCH_BYTES
@@ -98,7 +98,8 @@ class Tune(C.Structure):
     _fields_ = [("shape", C.c_uint32), ("group_lanes", C.c_uint32), ("items", C.c_uint32),
                 ("nshapes", C.c_uint32), ("extract_ms", C.c_double * 4),
                 ("find_chains", C.c_uint32), ("find_ms", C.c_double * 4),
-                ("verify_rows", C.c_uint32), ("verify_ms", C.c_double * 3)]
+                ("verify_rows", C.c_uint32), ("verify_ms", C.c_double * 3),
+                ("seed_rows", C.c_uint32), ("seed_ms", C.c_double * 3)]
 
 
 class Timing(C.Structure):
@@ -191,6 +192,7 @@ def lib(width="u32"):
         "pq_host_free": (None, [P]),
         "pq_mlp_eval": (I32, [P, I32, P, U32, I32, P]),
         "pq_set_verify": (I32, [P, U32]),
+        "pq_set_seed_verify": (I32, [P, U32]),
         "pq_dedup_reads_noalloc": (I32, [P, P, P, U64, I32, U64, I32, I32, P, P]),
         "pq_dev_alloc": (I32, [P, U64, C.POINTER(P)]),
         "pq_dev_free": (None, [P]),
@@ -498,7 +500,8 @@ class Context:
         return {"shape": t.shape, "group_lanes": t.group_lanes, "items": t.items,
                 "extract_ms": list(t.extract_ms)[:t.nshapes], "find_chains": t.find_chains,
                 "find_ms": list(t.find_ms), "verify_rows": t.verify_rows,
-                "verify_ms": list(t.verify_ms)}
+                "verify_ms": list(t.verify_ms), "seed_rows": t.seed_rows,
+                "seed_ms": list(t.seed_ms)}
 
     def set_tuning(self, shape):
         self._check(self._L.pq_set_tuning(self._h, shape), "set_tuning")
@@ -510,6 +513,11 @@ class Context:
         """Chains of at most `rows` rows (<= 32) at depths <= t finish on the text
         (0: backward search to the end)."""
         self._check(self._L.pq_set_verify(self._h, int(rows)), "set_verify")
+
+    def set_seed_verify(self, rows):
+        """Seeded search: chains whose depth-D interval holds at most `rows` rows
+        (<= 32) are finished by the seed kernel (0: all go to the search kernel)."""
+        self._check(self._L.pq_set_seed_verify(self._h, int(rows)), "set_seed_verify")
 
     def set_edge_capacity(self, edges):
         self._check(self._L.pq_set_edge_capacity(self._h, int(edges)), "set_edge_capacity")

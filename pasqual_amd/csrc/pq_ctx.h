@@ -80,6 +80,8 @@ struct pq_ctx {
     uint64_t edge_cap_set = 0;     // shared edge buffer capacity, 0 automatic
     uint32_t verify_rows = 32;     // chains of at most this many rows at depth <= t are
                                    // verified on the text (0: backward search to the end)
+    uint32_t seed_rows = 32;       // seeded search: chains of at most this many rows at
+                                   // the seed depth are finished by the seed kernel
     uint64_t edge_cap_auto = 0;    // capacity an overflowing search asked for (these reads)
     uint64_t edge_need = 0;        // edges the last overflowing search counted
     uint64_t search_batch = 0;     // reads per search batch, 0 automatic

@@ -1436,9 +1436,13 @@ static void launch_seeds(pq_ctx_t *ctx, const Learned *LM, uint64_t r0, uint64_t
                        LM->segs, v[0], v[1], cb.vl);
     if (mid)
         (void) hipEventRecord(mid, ctx->stream);
+    // the seed kernel finishes chains by its own threshold (the search kernel's is
+    // cb.vmax)
+    ChainBuf cbs = cb;
+    cbs.vmax = std::min(std::min<uint32_t>(ctx->seed_rows, VERIFY_MAX), cb.cap);
     hipLaunchKernelGGL(k_learned_seed<R>, dim3(pq_blocks(nr, 64)), dim3(256), 0, ctx->stream,
                        ctx->geom(), ctx->dir[0].text2, r0, nr, LM->depth, cb.vl, v[0], v[1],
-                       d_seeds, LM->qstats, cb, ctx->t, stats);
+                       d_seeds, LM->qstats, cbs, ctx->t, stats);
 }
 
 int pq_learned_seeds(pq_ctx_t *ctx, uint64_t r0, uint64_t nr, void *d_seeds, uint32_t *kseed,

@@ -2929,36 +2929,50 @@ extern "C" int pq_autotune(pq_ctx_t *ctx, int mode, pq_tune_t *out)
     // verification, 8 or 32 rows; verification reads the suffix array, whose rows
     // are scattered over tens of GiB at the largest sizes): every pair timed, the
     // fastest kept (search kernel + verification time).  The seeded search has
-    // one search kernel (the work-list walk): only the threshold is timed.
+    // one search kernel (the work-list walk): only the threshold is timed, with
+    // the seed phase (prediction and seed kernel, which finishes the chains whose
+    // intervals are within the threshold) counted in.
+    // With the seeded search, the seed kernel's threshold is timed with it (every
+    // pair of the two).
     static const uint32_t VROWS[3] = {0, 8, 32};
     double best_find = 1e300;
-    uint32_t fk = 1, vk = 2;
+    uint32_t fk = 1, vk = 2, sk = 2;
     for (uint32_t k = 0; k < 4; k++)
         t.find_ms[k] = 1e300;
     for (uint32_t v = 0; v < 3; v++)
-        t.verify_ms[v] = 1e300;
-    const uint32_t klast = mode == PQ_SEARCH_LEARNED ? 1 : 4;
+        t.verify_ms[v] = t.seed_ms[v] = 1e300;
+    const bool seeded = mode == PQ_SEARCH_LEARNED;
+    const uint32_t klast = seeded ? 1 : 4;
+    const uint32_t seed0 = ctx->seed_rows;
     for (uint32_t k = 1; k <= klast; k++) {
         for (uint32_t v = 0; v < 3; v++) {
-            ctx->find_chains = k;
-            ctx->verify_rows = VROWS[v];
-            int rc = pq_find_overlaps(ctx, mode);
-            if (rc)
-                return rc;
-            const double ms = ctx->timing.find_ms;
-            t.find_ms[k - 1] = std::min(t.find_ms[k - 1], ms);
-            t.verify_ms[v] = std::min(t.verify_ms[v], ms);
-            if (ms < best_find) {
-                best_find = ms;
-                fk = k;
-                vk = v;
+            for (uint32_t sd = seeded ? 0 : 2; sd < 3; sd++) {
+                ctx->find_chains = k;
+                ctx->verify_rows = VROWS[v];
+                ctx->seed_rows = seeded ? VROWS[sd] : seed0;
+                int rc = pq_find_overlaps(ctx, mode);
+                if (rc)
+                    return rc;
+                const double ms = ctx->timing.find_ms + (seeded ? ctx->timing.learned_ms : 0.0);
+                t.find_ms[k - 1] = std::min(t.find_ms[k - 1], ms);
+                t.verify_ms[v] = std::min(t.verify_ms[v], ms);
+                if (seeded)
+                    t.seed_ms[sd] = std::min(t.seed_ms[sd], ms);
+                if (ms < best_find) {
+                    best_find = ms;
+                    fk = k;
+                    vk = v;
+                    sk = sd;
+                }
             }
         }
     }
-    ctx->find_chains = mode == PQ_SEARCH_LEARNED ? 0 : fk;
+    ctx->find_chains = seeded ? 0 : fk;
     ctx->verify_rows = VROWS[vk];
-    t.find_chains = mode == PQ_SEARCH_LEARNED ? 0 : fk;
+    ctx->seed_rows = seeded ? VROWS[sk] : seed0;
+    t.find_chains = seeded ? 0 : fk;
     t.verify_rows = VROWS[vk];
+    t.seed_rows = ctx->seed_rows;
     t.shape = pick;
     t.group_lanes = EXT_SHAPES[pick][0];
     t.items = EXT_SHAPES[pick][1];
@@ -2986,6 +3000,16 @@ extern "C" int pq_set_find_chains(pq_ctx_t *ctx, uint32_t chains)
     if (chains > 4)
         return ctx->fail(PQ_ERR_ARG, "search kernel must be 0 (by index size), 1, 2, 3 or 4");
     ctx->find_chains = chains;
+    return PQ_OK;
+}
+
+extern "C" int pq_set_seed_verify(pq_ctx_t *ctx, uint32_t rows)
+{
+    if (!ctx)
+        return PQ_ERR_ARG;
+    if (rows > VERIFY_MAX)
+        return ctx->fail(PQ_ERR_ARG, "the seed kernel finishes at most 32 rows");
+    ctx->seed_rows = rows;
     return PQ_OK;
 }
 

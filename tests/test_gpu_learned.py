@@ -92,6 +92,27 @@ def test_learned_search_equals_plain(ctx, L, t, seed):
     assert tl["find_positions"] < tp["find_positions"]
 
 
+def test_seed_threshold_does_not_change_the_graph(ctx):
+    # the seed kernel finishes chains of up to 0, 8 or 32 rows at the seed depth;
+    # the search kernel, verifying by its own threshold, does the rest
+    rng = np.random.default_rng(17)
+    codes, off, _ = _random_reads(rng, 20000, 4000, 100, 100, err=0.003)
+    want, _ = _search(ctx, codes, off, 31, pq.SEARCH_PLAIN)
+    ctx.learned_train(**SMALL)
+    try:
+        for seed_rows in (0, 8, 32):
+            for verify in (0, 32):
+                ctx.set_seed_verify(seed_rows)
+                ctx.set_verify(verify)
+                ctx.find_overlaps(pq.SEARCH_LEARNED)
+                _same_graph(ctx.graph(), want, f"seed {seed_rows} verify {verify}")
+        with pytest.raises(pq.PqError):
+            ctx.set_seed_verify(33)
+    finally:
+        ctx.set_seed_verify(32)
+        ctx.set_verify(32)
+
+
 @pytest.mark.parametrize("seed", [21, 22])
 def test_learned_search_uneven_lengths(ctx, seed):
     # reads of 20..250 codes: reads not longer than the seed depth keep the
