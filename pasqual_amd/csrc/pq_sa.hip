@@ -283,12 +283,12 @@ __global__ void __launch_bounds__(256) k_slice_emit(Geom G, const uint64_t *t2, 
                                                     const uint32_t *boff, uint32_t *BK, R *V)
 {
     __shared__ uint32_t sk[4][64 * 33];   // a lane's 32 keys 33 words apart: no bank conflicts
-    __shared__ uint8_t ssy[4][64 * 33];   // their BWT symbols (64-bit rows carry them)
+    __shared__ uint32_t ssy[4][64 * 5];   // their BWT symbols, 4 bits each (64-bit rows)
     __shared__ uint32_t ws[4];
     constexpr bool SYM = sizeof(R) == 8;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     uint32_t keys[32];
-    uint8_t syms[32];
+    uint8_t syms[32] = {};
     const uint64_t j = (uint64_t) blockIdx.x * (SLICE_TILE / 32) + threadIdx.x;
     const uint32_t n = bucket_keys32<SYM>(G, t2, N, K, j, keys, syms);
     uint32_t c = 0;
@@ -298,8 +298,17 @@ __global__ void __launch_bounds__(256) k_slice_emit(Geom G, const uint64_t *t2, 
         c += in ? 1u : 0u;
         // all ones marks a non-member: no key has 15 in its length field
         sk[wid][33 * lane + i] = in ? keys[i] : 0xFFFFFFFFu;
-        if constexpr (SYM)
-            ssy[wid][33 * lane + i] = syms[i];
+    }
+    if constexpr (SYM) {
+        // eight symbols per word, five words per lane (stride 5: no bank conflicts)
+#pragma unroll
+        for (uint32_t q = 0; q < 4; q++) {
+            uint32_t x = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < 8; i++)
+                x |= (uint32_t) syms[8 * q + i] << (4 * i);
+            ssy[wid][5 * lane + q] = x;
+        }
     }
     for (int d = 32; d > 0; d >>= 1)
         c += __shfl_xor(c, d, 64);
@@ -319,7 +328,10 @@ __global__ void __launch_bounds__(256) k_slice_emit(Geom G, const uint64_t *t2, 
         if (in) {
             const uint32_t at = base + (uint32_t) __popcll(m & lt);
             BK[at] = k;
-            V[at] = sa_pack<R>(g0 + p, SYM ? ssy[wid][33 * (p >> 5) + (p & 31)] : 0u);
+            unsigned sym = 0;
+            if constexpr (SYM)
+                sym = (ssy[wid][5 * (p >> 5) + ((p & 31) >> 3)] >> (4 * (p & 7))) & 15u;
+            V[at] = sa_pack<R>(g0 + p, sym);
         }
         base += (uint32_t) __popcll(m);
     }

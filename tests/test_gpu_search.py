@@ -277,6 +277,7 @@ def test_search_work_counts_match_oracle(ctx, lmin, lmax, t, seed):
     D = ctx.learned_stats()["depth"]
     for vmax in (0, 32, 5):
         ctx.set_verify(vmax)
+        ctx.set_seed_verify(vmax)   # the seed kernel's scan by the same threshold
         v = min(vmax, int(np.diff(off.astype(np.int64)).max()) - t)
         want = orc.chain_work(codes, off, 0, t, v)
         want_l = orc.chain_work(codes, off, D, t, v, stride)
@@ -291,6 +292,7 @@ def test_search_work_counts_match_oracle(ctx, lmin, lmax, t, seed):
                     (vmax, kernel, mode)
     ctx.set_find_chains(0)
     ctx.set_verify(32)
+    ctx.set_seed_verify(32)
 
 
 @pytest.mark.parametrize("vmax", [0, 1, 7, 32])
@@ -303,6 +305,7 @@ def test_verification_gives_the_search_graph(ctx, vmax):
              _random_reads(rng, 20000, 4000, 60, 60, err=0.003)[:2] + (17,),
              _identical_copies(40)]
     ctx.set_verify(vmax)
+    ctx.set_seed_verify(vmax)
     try:
         for codes, off, t in cases:
             want = orc.search_explicit(codes, off, t)
@@ -318,3 +321,4 @@ def test_verification_gives_the_search_graph(ctx, vmax):
     finally:
         ctx.set_find_chains(0)
         ctx.set_verify(32)
+        ctx.set_seed_verify(32)
