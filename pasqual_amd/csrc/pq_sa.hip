@@ -337,15 +337,21 @@ __global__ void __launch_bounds__(256) k_slice_emit(Geom G, const uint64_t *t2, 
     }
 }
 
-// One suffix of a run being sorted: position, codes left after the run's
-// common prefix of `base` codes, and those codes packed W words (32 codes
-// each, most significant first, zero past the end).  All of it is loaded up
-// front with independent loads, so comparisons inside the sorting networks
-// never wait on memory.
+// One suffix of a run being sorted: the codes left after the run's common
+// prefix of `base` codes packed in W words (32 codes each, most significant
+// first, zero past the end), then tie = len << 44 | position << 3 | sym: that
+// count of codes (below 2^20), the text position (below 2^41) and the BWT symbol
+// of the row the suffix lands in.  Words then tie order suffixes by codes, then
+// length, then position (positions are unique: the symbol never decides); a
+// padding element is all ones.  Twelve 32-bit registers for W = 5, which every
+// step of a sorting network exchanges and selects.  All of it is loaded up front
+// with independent loads, so comparisons never wait on memory.
 template <int W, typename R> struct SufW {
-    R g;                     // position; all ones for a padding element
-    uint32_t len, sym;       // sym: BWT symbol of the row the suffix lands in
     uint64_t w[W];
+    uint64_t tie;
+    __device__ __forceinline__ R g() const { return (R) ((tie >> 3) & ((1ull << 41) - 1ull)); }
+    __device__ __forceinline__ uint32_t len() const { return (uint32_t) (tie >> 44); }
+    __device__ __forceinline__ unsigned sym() const { return (unsigned) (tie & 7u); }
 };
 
 
@@ -357,8 +363,7 @@ __device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, R v,
 {
     const uint64_t g = sa_pos<R>(v);   // v: a suffix-array value of the slice being sorted
     const uint32_t rem = suf_rem(G, g);
-    s.g = (R) g;
-    s.len = rem > base ? rem - base : 0;
+    const uint32_t len = rem > base ? rem - base : 0;
     // The W + 1 text words the windows come from and the word holding the code
     // before g, all loaded unconditionally (word indices clamped to the packed
     // text's slack words), so that they are in flight together: a load under a
@@ -369,23 +374,25 @@ __device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, R v,
 #pragma unroll
     for (int i = 0; i <= W; i++)
         raw[i] = t2[w0 + i < wmax ? w0 + i : wmax];
+    unsigned sym;
     if constexpr (sizeof(R) == 8) {
-        s.sym = (unsigned) ((uint64_t) v >> SA_SYM_SHIFT);
+        sym = (unsigned) ((uint64_t) v >> SA_SYM_SHIFT);
     } else {
         const uint64_t gp = g ? g - 1 : 0;
         const uint64_t prev = t2[gp >> 5];
-        s.sym = pq_is_start(G, g) ? 0u : (unsigned) ((prev >> (62 - 2 * (gp & 31))) & 3u) + 1u;
+        sym = pq_is_start(G, g) ? 0u : (unsigned) ((prev >> (62 - 2 * (gp & 31))) & 3u) + 1u;
     }
+    s.tie = ((uint64_t) len << 44) | (g << 3) | sym;
     const unsigned sh = (unsigned) (p0 & 31) * 2;
 #pragma unroll
     for (int i = 0; i < W; i++) {
         // window i: 32 codes from p0 + 32 i, zero past the suffix's end
         uint64_t x = (raw[i] << sh) | ((raw[i + 1] >> 1) >> (63 - sh));
         const uint32_t lo = 32u * (uint32_t) i;
-        if (lo >= s.len)
+        if (lo >= len)
             x = 0;
-        else if (s.len - lo < 32)
-            x &= ~0ull << (64 - 2 * (s.len - lo));
+        else if (len - lo < 32)
+            x &= ~0ull << (64 - 2 * (len - lo));
         s.w[i] = x;
     }
 }
@@ -394,9 +401,7 @@ __device__ __forceinline__ void suf_load(const Geom &G, const uint64_t *t2, R v,
 // every suffix.
 template <int W, typename R> __device__ __forceinline__ void suf_none(SufW<W, R> &s)
 {
-    s.g = (R) ~(R) 0;
-    s.len = ~0u;
-    s.sym = BWT_UNSET;
+    s.tie = ~0ull;
 #pragma unroll
     for (int i = 0; i < W; i++)
         s.w[i] = ~0ull;
@@ -425,10 +430,10 @@ __device__ __forceinline__ void emit_key(const KeyOut &ko, const Geom &G, const 
     if (gr & ((1ull << ko.shift) - 1ull))
         return;
     if (ko.from_text) {
-        ko.keys[gr >> ko.shift] = suffix_key(G, t2, (uint64_t) s.g, ko.D);
+        ko.keys[gr >> ko.shift] = suffix_key(G, t2, (uint64_t) s.g(), ko.D);
         return;
     }
-    const uint32_t rem = ko.K + s.len;   // every suffix of such a run has >= K codes
+    const uint32_t rem = ko.K + s.len();   // every suffix of such a run has >= K codes
     uint64_t w1 = 0;
     if constexpr (W > 1)
         w1 = s.w[1];
@@ -452,26 +457,14 @@ __device__ __forceinline__ bool suf_less(const SufW<W, R> &a, const SufW<W, R> &
         lt = lt | (eq & (a.w[i] < b.w[i]));
         eq = eq & (a.w[i] == b.w[i]);
     }
-    lt = lt | (eq & (a.len < b.len));
-    eq = eq & (a.len == b.len);
-    return lt | (eq & (a.g < b.g));
-}
-
-template <int J, typename R> __device__ __forceinline__ R xor_lane_r(R v)
-{
-    if constexpr (sizeof(R) == 8)
-        return (R) xor_lane64<J>((uint64_t) v);
-    else
-        return (R) xor_lane<J>((uint32_t) v);
+    return lt | (eq & (a.tie < b.tie));
 }
 
 template <int J, int W, typename R>
 __device__ __forceinline__ SufW<W, R> suf_xor(const SufW<W, R> &s)
 {
     SufW<W, R> o;
-    o.g = xor_lane_r<J, R>(s.g);
-    o.len = xor_lane<J>(s.len);
-    o.sym = xor_lane<J>(s.sym);
+    o.tie = xor_lane64<J>(s.tie);
 #pragma unroll
     for (int i = 0; i < W; i++)
         o.w[i] = xor_lane64<J>(s.w[i]);
@@ -486,7 +479,7 @@ __device__ __forceinline__ void bitonic_steps(SufW<W, R> &s, int li)
     const bool up = (li & K) == 0;
     const bool lower = (li & J) == 0;
     const bool pless = suf_less<W, R>(p, s);
-    const bool swap = (lower == up) ? pless : (!pless && p.g != s.g);
+    const bool swap = (lower == up) ? pless : (!pless && p.tie != s.tie);
     if (swap)
         s = p;
     if constexpr (J > 1)
@@ -592,8 +585,8 @@ __global__ void __launch_bounds__(256) k_sort_groups(Geom Gm, const uint64_t *t2
         suf_none<W, R>(s);
     bitonic_stages<G, 2, W, R>(s, li);
     if (e) {
-        sa[sg.x + li] = s.g;
-        bwt8[sg.x + li] = (uint8_t) s.sym;
+        sa[sg.x + li] = s.g();
+        bwt8[sg.x + li] = (uint8_t) s.sym();
         if (ko.keys)
             emit_key<W, R>(ko, Gm, t2, sg.x + li, bk, s);
     }
@@ -613,7 +606,7 @@ __device__ __forceinline__ void wave_steps(SufW<W, R> (&s)[M], int lane)
                 // the lower item keeps the smaller of the pair when the merge runs up
                 const bool up = ((m * 64) & K) == 0;
                 const bool bless = suf_less<W, R>(s[m + JM], s[m]);
-                if (up ? bless : (!bless && s[m + JM].g != s[m].g)) {
+                if (up ? bless : (!bless && s[m + JM].tie != s[m].tie)) {
                     const SufW<W, R> x = s[m];
                     s[m] = s[m + JM];
                     s[m + JM] = x;
@@ -627,7 +620,7 @@ __device__ __forceinline__ void wave_steps(SufW<W, R> (&s)[M], int lane)
             const bool up = ((m * 64 + lane) & K) == 0;
             const SufW<W, R> p = suf_xor<J, W, R>(s[m]);
             const bool pless = suf_less<W, R>(p, s[m]);
-            const bool swap = (lower == up) ? pless : (!pless && p.g != s[m].g);
+            const bool swap = (lower == up) ? pless : (!pless && p.tie != s[m].tie);
             if (swap)
                 s[m] = p;
         }
@@ -679,8 +672,8 @@ __global__ void __launch_bounds__(256) k_sort_wave(Geom Gm, const uint64_t *t2, 
     for (int m = 0; m < M; m++) {
         const uint32_t e = (uint32_t) (m * 64 + lane);
         if (e < sg.y) {
-            sa[sg.x + e] = s[m].g;
-            bwt8[sg.x + e] = (uint8_t) s[m].sym;
+            sa[sg.x + e] = s[m].g();
+            bwt8[sg.x + e] = (uint8_t) s[m].sym();
             if (ko.keys)
                 emit_key<W, R>(ko, Gm, t2, sg.x + e, bk, s[m]);
         }
@@ -730,8 +723,8 @@ __global__ void __launch_bounds__(256) k_sort_lds(Geom Gm, const uint64_t *t2, R
     }
     const uint32_t bk = ko.keys && !ko.from_text ? ko.BK[sg.x] : 0u;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        sa[sg.x + i] = sh[i].g;
-        bwt8[sg.x + i] = (uint8_t) sh[i].sym;
+        sa[sg.x + i] = sh[i].g();
+        bwt8[sg.x + i] = (uint8_t) sh[i].sym();
         if (ko.keys)
             emit_key<W, R>(ko, Gm, t2, sg.x + i, bk, sh[i]);
     }
@@ -1324,6 +1317,9 @@ static int plan_slices(pq_ctx_t *ctx, const Geom &G, const uint64_t *t2, unsigne
 template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
 {
     const uint64_t N = ctx->N;
+    // the run sorts pack a suffix's position and code count in one word
+    if (N >= (1ull << 41) || ctx->Lmax >= (1u << 20))
+        return ctx->fail(PQ_ERR_LIMIT, "text of 2^41 codes or reads of 2^20 codes or more");
     DirIndex &D = ctx->dir[d];
     Geom G = ctx->geom();
     const unsigned K = bucket_codes(N);
