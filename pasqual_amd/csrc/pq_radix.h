@@ -2,8 +2,8 @@
 // source of the pairs (arrays, or pairs generated on the fly by the suffix-array
 // build's first pass).
 //
-// A block owns a tile of 256 x ROUNDS pairs; each of its 4 waves owns a
-// contiguous quarter and walks it in coalesced rounds of 64.  Within a round a
+// A block owns a tile of THREADS x ROUNDS pairs; each of its waves owns a
+// contiguous part and walks it in coalesced rounds of 64.  Within a round a
 // lane finds the lanes holding the same 8-bit digit with 8 ballots, and its rank
 // among them with one popcount; a per-wave LDS counter per digit carries the
 // rank across rounds, so the ranking is stable without any block-wide barrier
@@ -14,11 +14,19 @@
 
 #define RADIX_THREADS 256
 
-// Items per thread of a radix tile: 16, or 8 when key + value exceed 12 bytes
-// (the tile is staged in LDS).
+// Items per thread of a radix tile: 16, or 8 when key + value take 12 bytes or
+// more (the tile is staged in LDS).
 template <typename KT, typename VT> constexpr int radix_rounds()
 {
     return sizeof(KT) + sizeof(VT) >= 12 ? 8 : 16;
+}
+
+// Threads of a scatter block: 512 for 12-byte pairs (a tile of 4096 pairs, 48 KiB
+// of LDS: twice the pairs per digit run of a 2048-pair tile for the same work per
+// wave), else 256.
+template <typename KT, typename VT> constexpr int radix_threads()
+{
+    return sizeof(KT) + sizeof(VT) == 12 ? 512 : 256;
 }
 
 // Pairs read from two arrays.
@@ -85,20 +93,20 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_hist(Src src, uint64_t 
     }
 }
 
-template <typename KT, typename VT, int ROUNDS, typename Src>
-__global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(Src src, KT *kout, VT *vout,
-                                                                uint64_t count, unsigned shift,
-                                                                const uint32_t *offs,
-                                                                uint32_t nblocks)
+template <typename KT, typename VT, int ROUNDS, int THREADS, typename Src>
+__global__ void __launch_bounds__(THREADS) k_radix_scatter(Src src, KT *kout, VT *vout,
+                                                          uint64_t count, unsigned shift,
+                                                          const uint32_t *offs, uint32_t nblocks)
 {
-    constexpr int RADIX_TILE = RADIX_THREADS * ROUNDS;
+    constexpr int NW = THREADS / 64;   // waves of the block, each with its own counters
+    constexpr int RADIX_TILE = THREADS * ROUNDS;
     constexpr int RADIX_ROUNDS = ROUNDS;
-    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t wcnt[NW][256];
     __shared__ uint32_t tstart[256];
     __shared__ KT sk[RADIX_TILE];
     __shared__ VT sv[RADIX_TILE];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < 4 * 256; i += RADIX_THREADS)
+    for (int i = threadIdx.x; i < NW * 256; i += THREADS)
         (&wcnt[0][0])[i] = 0;
     __syncthreads();
 
@@ -110,7 +118,7 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(Src src, KT *ko
     const uint32_t tb = blockIdx.x < 8 * q8 ? (blockIdx.x & 7u) * q8 + (blockIdx.x >> 3)
                                             : blockIdx.x;
     const uint64_t tile = (uint64_t) tb * RADIX_TILE;
-    const uint64_t wbase = tile + (uint64_t) wid * (RADIX_TILE / 4);
+    const uint64_t wbase = tile + (uint64_t) wid * (RADIX_TILE / NW);
     const uint64_t lt = (1ull << lane) - 1ull;
     KT key[RADIX_ROUNDS];
     VT val[RADIX_ROUNDS];
@@ -141,31 +149,44 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(Src src, KT *ko
             wcnt[wid][d] = base + (uint32_t) __popcll(peers);
     }
     __syncthreads();
-    // digit totals of the tile and exclusive prefix over waves and digits
+    // digit totals of the tile and exclusive prefix over waves and digits (thread d
+    // of the first four waves: digit d)
     {
-        int d = threadIdx.x;
-        uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d], c3 = wcnt[3][d];
-        uint32_t tot = c0 + c1 + c2 + c3;
-        // block-wide exclusive scan over the 256 digit totals
-        uint32_t inc = tot;
-        for (int s = 1; s < 64; s <<= 1) {
-            uint32_t o = __shfl_up(inc, s, 64);
-            if (lane >= s)
-                inc += o;
-        }
+        const int d = threadIdx.x;
         __shared__ uint32_t wsum[4];
-        if (lane == 63)
-            wsum[wid] = inc;
+        uint32_t c[NW], tot = 0, inc = 0;
+        if (d < 256) {
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                c[w] = wcnt[w][d];
+                tot += c[w];
+            }
+            // block-wide exclusive scan over the 256 digit totals
+            inc = tot;
+            for (int s = 1; s < 64; s <<= 1) {
+                uint32_t o = __shfl_up(inc, s, 64);
+                if (lane >= s)
+                    inc += o;
+            }
+            if (lane == 63)
+                wsum[wid] = inc;
+        }
         __syncthreads();
-        uint32_t pre = 0;
-        for (int w = 0; w < wid; w++)
-            pre += wsum[w];
-        tstart[d] = pre + inc - tot;
+        if (d < 256) {
+            uint32_t pre = 0;
+            for (int w = 0; w < wid; w++)
+                pre += wsum[w];
+            tstart[d] = pre + inc - tot;
+        }
         __syncthreads();
-        wcnt[0][d] = 0;
-        wcnt[1][d] = c0;
-        wcnt[2][d] = c0 + c1;
-        wcnt[3][d] = c0 + c1 + c2;
+        if (d < 256) {
+            uint32_t run = 0;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                wcnt[w][d] = run;
+                run += c[w];
+            }
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -180,7 +201,7 @@ __global__ void __launch_bounds__(RADIX_THREADS) k_radix_scatter(Src src, KT *ko
     }
     __syncthreads();
     uint64_t valid = count - tile < RADIX_TILE ? count - tile : RADIX_TILE;
-    for (uint32_t p = threadIdx.x; p < valid; p += RADIX_THREADS) {
+    for (uint32_t p = threadIdx.x; p < valid; p += THREADS) {
         KT k = sk[p];
         unsigned d = (unsigned) (k >> shift) & 255u;
         uint64_t o = (uint64_t) offs[(uint64_t) d * nblocks + tb] + (p - tstart[d]);
@@ -195,7 +216,9 @@ static int radix_pass_src(pq_ctx_t *ctx, Src src, KT *kout, VT *vout, uint64_t c
                           unsigned shift)
 {
     constexpr int ROUNDS = radix_rounds<KT, VT>();
-    constexpr int RADIX_TILE = RADIX_THREADS * ROUNDS;
+    constexpr int THREADS = radix_threads<KT, VT>();
+    constexpr int RADIX_TILE = THREADS * ROUNDS;
+    constexpr int HROUNDS = RADIX_TILE / RADIX_THREADS;   // the histogram's tiles are the same
     if (count == 0)
         return PQ_OK;
     if (count > 0xFFFFFFFFull)
@@ -205,12 +228,12 @@ static int radix_pass_src(pq_ctx_t *ctx, Src src, KT *kout, VT *vout, uint64_t c
     if (!hist)
         return ctx->fail(PQ_ERR_NOMEM, "radix histogram");
     uint32_t *offs = hist + (size_t) nb * 256;
-    hipLaunchKernelGGL((k_radix_hist<KT, ROUNDS, Src>), dim3((nb + HIST_TILES - 1) / HIST_TILES),
+    hipLaunchKernelGGL((k_radix_hist<KT, HROUNDS, Src>), dim3((nb + HIST_TILES - 1) / HIST_TILES),
                        dim3(RADIX_THREADS), 0, ctx->stream, src, count, shift, hist, nb);
     int rc = pq_scan_u32(ctx, hist, offs, (uint64_t) nb * 256, nullptr);
     if (rc)
         return rc;
-    hipLaunchKernelGGL((k_radix_scatter<KT, VT, ROUNDS, Src>), dim3(nb), dim3(RADIX_THREADS), 0,
+    hipLaunchKernelGGL((k_radix_scatter<KT, VT, ROUNDS, THREADS, Src>), dim3(nb), dim3(THREADS), 0,
                        ctx->stream, src, kout, vout, count, shift, offs, nb);
     PQ_CHECK(hipGetLastError());
     return PQ_OK;
