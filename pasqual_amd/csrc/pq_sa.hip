@@ -1485,34 +1485,55 @@ __global__ void k_bwt_fill(Geom G, const uint64_t *t2, R *sa, uint64_t N, uint8_
     }
 }
 
-// One wave per occurrence line of 128 rows (grid-stride over lines): the three
-// bit planes of both 64-row halves by ballots, written by one lane with the
-// line's counts of A, C, G, T (no LDS, no block barrier).  Rows past N carry
-// symbol 7.
+// One wave per four occurrence lines of 128 rows (grid-stride): the three bit
+// planes of each 64-row half by ballots, each line written by one lane with its
+// counts of A, C, G, T (no LDS, no block barrier).  Rows past N carry symbol 7.
 __global__ void __launch_bounds__(256) k_occ_planes(const uint8_t *bwt8, uint64_t N,
                                                     uint64_t nlines, OccLine *occ, uint4 *cnts)
 {
+    // four lines per wave and iteration: the eight bytes of a lane loaded together,
+    // without a bounds branch (clamped to byte N, which exists; rows past N read
+    // as symbol 7), then lane q writes line q
+    constexpr int LPW = 4;
     const int lane = threadIdx.x & 63;
     const uint64_t nwaves = ((uint64_t) gridDim.x * blockDim.x) >> 6;
-    for (uint64_t line = ((uint64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6; line < nlines;
-         line += nwaves) {
-        const uint64_t r0 = line * 128 + (uint64_t) lane, r1 = r0 + 64;
-        const unsigned s0 = r0 < N ? bwt8[r0] : 7u, s1 = r1 < N ? bwt8[r1] : 7u;
-        const uint64_t a0 = __ballot(s0 & 1u), b0 = __ballot(s0 & 2u), c0 = __ballot(s0 & 4u);
-        const uint64_t a1 = __ballot(s1 & 1u), b1 = __ballot(s1 & 2u), c1 = __ballot(s1 & 4u);
-        if (lane == 0) {
-            OccLine &ln = occ[line];
-            ln.p0[0] = a0;
-            ln.p0[1] = a1;
-            ln.p1[0] = b0;
-            ln.p1[1] = b1;
-            ln.p2[0] = c0;
-            ln.p2[1] = c1;
-            cnts[line] = make_uint4(
-                __popcll(a0 & ~b0 & ~c0) + __popcll(a1 & ~b1 & ~c1),
-                __popcll(~a0 & b0 & ~c0) + __popcll(~a1 & b1 & ~c1),
-                __popcll(a0 & b0 & ~c0) + __popcll(a1 & b1 & ~c1),
-                __popcll(~a0 & ~b0 & c0) + __popcll(~a1 & ~b1 & c1));
+    const uint64_t ngroups = (nlines + LPW - 1) / LPW;
+    for (uint64_t grp = ((uint64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6; grp < ngroups;
+         grp += nwaves) {
+        const uint64_t row0 = grp * (LPW * 128) + (uint64_t) lane;
+        unsigned sy[2 * LPW];
+#pragma unroll
+        for (int k = 0; k < 2 * LPW; k++) {
+            const uint64_t r = row0 + 64 * (uint64_t) k;
+            sy[k] = bwt8[r < N ? r : N];
+        }
+        uint64_t pa[2 * LPW], pb[2 * LPW], pc[2 * LPW];
+#pragma unroll
+        for (int k = 0; k < 2 * LPW; k++) {
+            const unsigned v = row0 + 64 * (uint64_t) k < N ? sy[k] : 7u;
+            pa[k] = __ballot(v & 1u);
+            pb[k] = __ballot(v & 2u);
+            pc[k] = __ballot(v & 4u);
+        }
+#pragma unroll
+        for (int q = 0; q < LPW; q++) {
+            const uint64_t line = grp * LPW + (uint64_t) q;
+            if (lane == q && line < nlines) {
+                const uint64_t a0 = pa[2 * q], a1 = pa[2 * q + 1], b0 = pb[2 * q],
+                               b1 = pb[2 * q + 1], c0 = pc[2 * q], c1 = pc[2 * q + 1];
+                OccLine &ln = occ[line];
+                ln.p0[0] = a0;
+                ln.p0[1] = a1;
+                ln.p1[0] = b0;
+                ln.p1[1] = b1;
+                ln.p2[0] = c0;
+                ln.p2[1] = c1;
+                cnts[line] = make_uint4(
+                    __popcll(a0 & ~b0 & ~c0) + __popcll(a1 & ~b1 & ~c1),
+                    __popcll(~a0 & b0 & ~c0) + __popcll(~a1 & b1 & ~c1),
+                    __popcll(a0 & b0 & ~c0) + __popcll(a1 & b1 & ~c1),
+                    __popcll(~a0 & ~b0 & c0) + __popcll(~a1 & ~b1 & c1));
+            }
         }
     }
 }
@@ -1587,7 +1608,7 @@ int pq_build_occ_dir(pq_ctx_t *ctx, int d)
         D.keys_D = kD;
         D.keys_pending = 0;
     }
-    hipLaunchKernelGGL(k_occ_planes, dim3(pq_blocks_gs(D.nlines, 4)), dim3(256), 0,
+    hipLaunchKernelGGL(k_occ_planes, dim3(pq_blocks_gs((D.nlines + 3) / 4, 4)), dim3(256), 0,
                        ctx->stream, D.bwt8, N, D.nlines, D.occ, cnts);
     uint32_t tot[4];
     if ((rc = pq_scan_u4(ctx, cnts, excl, D.nlines, tot)))
