@@ -228,6 +228,10 @@ __global__ void __launch_bounds__(256) k_slice_count(Geom G, const uint64_t *t2,
 struct SliceBounds {
     uint32_t b[SLICE_MAX + 1];
 };
+// First suffix-array row of every slice.
+struct SliceStarts {
+    uint64_t start[SLICE_MAX];
+};
 
 // Suffixes of every slice per tile of SLICE_TILE positions, in one pass over the
 // text: bcnt[s * ntiles + tile] for slice s = [b[s], b[s + 1]) of keys, from the
@@ -334,6 +338,101 @@ __global__ void __launch_bounds__(256) k_slice_emit(Geom G, const uint64_t *t2, 
             V[at] = sa_pack<R>(g0 + p, sym);
         }
         base += (uint32_t) __popcll(m);
+    }
+}
+
+// Every slice's (key, position) pairs at once, each into its slice's place
+// (start[s] + the slice's exclusive prefix of tile counts soff[s * ntiles +
+// tile] + the rank within the tile), in position order within each slice: the
+// bucket keys are made once instead of once per slice.  As k_slice_emit
+// otherwise; NS >= the slice count.
+template <typename R, int NS>
+__global__ void __launch_bounds__(256)
+    k_slice_emit_all(Geom G, const uint64_t *t2, uint64_t N, unsigned K, SliceBounds sb,
+                     SliceStarts ss, uint32_t nsl, uint64_t ntiles, const uint32_t *soff,
+                     uint32_t *BK, R *V)
+{
+    __shared__ uint32_t sk[4][64 * 33];
+    __shared__ uint32_t ssy[4][64 * 5];
+    __shared__ uint32_t wsc[4][NS];
+    constexpr bool SYM = sizeof(R) == 8;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t keys[32];
+    uint8_t syms[32] = {};
+    const uint64_t j = (uint64_t) blockIdx.x * (SLICE_TILE / 32) + threadIdx.x;
+    const uint32_t n = bucket_keys32<SYM>(G, t2, N, K, j, keys, syms);
+    // the slice of a key: how many bounds after the first it reaches
+    auto slice_of = [&](uint32_t k) -> uint32_t {
+        uint32_t sl = 0;
+#pragma unroll
+        for (int t = 1; t < NS; t++)
+            sl += k >= sb.b[t] ? 1u : 0u;
+        return sl;
+    };
+    uint32_t c[NS];
+#pragma unroll
+    for (int t = 0; t < NS; t++)
+        c[t] = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 32; i++) {
+        const uint32_t sl = slice_of(keys[i]);
+#pragma unroll
+        for (int t = 0; t < NS; t++)
+            c[t] += (i < n && sl == (uint32_t) t) ? 1u : 0u;
+        sk[wid][33 * lane + i] = i < n ? keys[i] : 0xFFFFFFFFu;
+    }
+    if constexpr (SYM) {
+#pragma unroll
+        for (uint32_t q = 0; q < 4; q++) {
+            uint32_t x = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < 8; i++)
+                x |= (uint32_t) syms[8 * q + i] << (4 * i);
+            ssy[wid][5 * lane + q] = x;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NS; t++) {
+        uint32_t x = c[t];
+        for (int d = 32; d > 0; d >>= 1)
+            x += __shfl_xor(x, d, 64);
+        if (lane == 0)
+            wsc[wid][t] = x;
+    }
+    __syncthreads();
+    uint64_t base[NS];
+#pragma unroll
+    for (int t = 0; t < NS; t++) {
+        base[t] = 0;
+        if ((uint32_t) t < nsl) {
+            base[t] = ss.start[t] + soff[(uint64_t) t * ntiles + blockIdx.x];
+            for (int w = 0; w < wid; w++)
+                base[t] += wsc[w][t];
+        }
+    }
+    const uint64_t g0 = (uint64_t) blockIdx.x * SLICE_TILE + (uint64_t) wid * 2048;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t q = 0; q < 32; q++) {
+        const uint32_t p = q * 64 + lane;
+        const uint32_t k = sk[wid][33 * (p >> 5) + (p & 31)];
+        const bool valid = k != 0xFFFFFFFFu;   // no key has 15 in its length field
+        const uint32_t sl = slice_of(k);
+        unsigned sym = 0;
+        if constexpr (SYM)
+            sym = (ssy[wid][5 * (p >> 5) + ((p & 31) >> 3)] >> (4 * (p & 7))) & 15u;
+#pragma unroll
+        for (int t = 0; t < NS; t++) {
+            if ((uint32_t) t < nsl) {
+                const bool mine = valid && sl == (uint32_t) t;
+                const uint64_t m = __ballot(mine);
+                if (mine) {
+                    const uint64_t at = base[t] + (uint64_t) __popcll(m & lt);
+                    BK[at] = k;
+                    V[at] = sa_pack<R>(g0 + p, sym);
+                }
+                base[t] += (uint64_t) __popcll(m);
+            }
+        }
     }
 }
 
@@ -1339,7 +1438,13 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
     uint32_t *BK2 = nullptr, *bcnt = nullptr;
     R *V2 = nullptr;
     const size_t vbytes = (smax + 1) * std::max<size_t>(4, sizeof(R));
-    if ((rc = pq_slot(ctx, SLOT_BK, smax * 4 + 4, &S.BK)) ||
+    // several slices (at most SLICE_MAX) with an even number of radix passes: all
+    // slices' pairs are made in one pass over the text, the keys into a key array
+    // for the whole text (the positions go to the suffix array, where each slice's
+    // last radix pass leaves them anyway); else slice by slice
+    const bool emit_all = slices.size() > 1 && slices.size() <= SLICE_MAX && !(passes & 1);
+    uint32_t *BKall = nullptr;
+    if ((rc = pq_slot(ctx, SLOT_BK, (emit_all ? N : smax) * 4 + 4, &BKall)) ||
         (rc = pq_slot(ctx, SLOT_BK2, (smax + 1) * 4, &BK2)) ||
         (rc = pq_slot(ctx, SLOT_V2, vbytes, &V2)) ||
         (rc = pq_slot(ctx, SLOT_STARTS, (smax + 2) * 4, &S.starts)) ||
@@ -1347,7 +1452,8 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
         (rc = pq_slot(ctx, SLOT_COUNTS, 64, &S.counts)) ||
         (rc = pq_slot(ctx, SLOT_BWT8, N + 1, &D.bwt8)) ||
         (slices.size() > 1 &&
-         (rc = pq_slot(ctx, SLOT_SLICE_TILES, (slices.size() + 1) * (ntiles + 1) * 4, &bcnt))))
+         (rc = pq_slot(ctx, SLOT_SLICE_TILES, (2 * slices.size() + 1) * (ntiles + 1) * 4,
+                       &bcnt))))
         return rc;
     // every slice's suffixes per tile, in one pass (up to SLICE_MAX slices)
     const bool count_all = slices.size() > 1 && slices.size() <= SLICE_MAX;
@@ -1369,6 +1475,38 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
                                D.text2, N, K, sb, nsl, ntiles, bcnt);
         PQ_CHECK(hipGetLastError());
     }
+    S.BK = BKall;
+    if (emit_all) {
+        // each slice's tile offsets (a scan per slice), then the one emit pass
+        SliceBounds sb;
+        SliceStarts ss;
+        uint64_t at = 0;
+        for (size_t i = 0; i <= SLICE_MAX; i++)
+            sb.b[i] = i < slices.size() ? (uint32_t) std::min<uint64_t>(slices[i].lo, 0xFFFFFFFFull)
+                                        : 0xFFFFFFFFu;
+        uint32_t *soff = bcnt + slices.size() * ntiles;   // every slice's tile offsets
+        for (size_t i = 0; i < slices.size(); i++) {
+            ss.start[i] = at;
+            uint32_t got = 0;
+            if ((rc = pq_scan_u32(ctx, bcnt + i * ntiles, soff + i * ntiles, ntiles, &got)))
+                return rc;
+            if (got != slices[i].count)
+                return ctx->fail(PQ_ERR_STATE, "slice count disagrees with its histogram");
+            at += slices[i].count;
+        }
+        const uint32_t nsl = (uint32_t) slices.size();
+        const dim3 grid((unsigned) ntiles);
+        if (nsl <= 4)
+            hipLaunchKernelGGL((k_slice_emit_all<R, 4>), grid, dim3(256), 0, ctx->stream, G,
+                               D.text2, N, K, sb, ss, nsl, ntiles, soff, BKall, (R *) D.sa);
+        else if (nsl <= 8)
+            hipLaunchKernelGGL((k_slice_emit_all<R, 8>), grid, dim3(256), 0, ctx->stream, G,
+                               D.text2, N, K, sb, ss, nsl, ntiles, soff, BKall, (R *) D.sa);
+        else
+            hipLaunchKernelGGL((k_slice_emit_all<R, 16>), grid, dim3(256), 0, ctx->stream, G,
+                               D.text2, N, K, sb, ss, nsl, ntiles, soff, BKall, (R *) D.sa);
+        PQ_CHECK(hipGetLastError());
+    }
     S.flags = BK2;
     S.pos = (uint32_t *) V2;
     PQ_CHECK(hipMemsetAsync(D.bwt8, BWT_UNSET, N, ctx->stream));
@@ -1387,6 +1525,8 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
         const uint64_t ns = sl.count;
         S.sa = (R *) D.sa + off;
         S.bwt8 = D.bwt8 + off;
+        if (emit_all)
+            S.BK = BKall + off;   // this slice's keys, where the one emit pass put them
         // start in the pair that makes the last radix pass write (BK, slice of sa)
         uint32_t *kc = (passes & 1) ? BK2 : S.BK, *ka = (passes & 1) ? S.BK : BK2;
         R *vc = (passes & 1) ? V2 : S.sa, *va = (passes & 1) ? S.sa : V2;
@@ -1399,7 +1539,7 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
             std::swap(kc, ka);
             std::swap(vc, va);
             p0 = 1;
-        } else {
+        } else if (!emit_all) {
             const size_t si = (size_t) (&sl - slices.data());
             uint32_t *boff = bcnt + slices.size() * ntiles;
             uint32_t *scnt = bcnt + (count_all ? si * ntiles : 0);

@@ -138,3 +138,28 @@ def test_repeat_families_wide_sliced(wctx, ctx):
     for c, where in ((wctx, "64-bit rows, sliced"), (ctx, "32-bit rows")):
         c.find_overlaps(pq.SEARCH_PLAIN)
         _same_graph(c.graph(), want, f"repeat families, {where}")
+
+
+@pytest.mark.parametrize("row_bytes", [8, 4])
+def test_slices_made_in_one_pass(wctx, row_bytes):
+    # 16 M suffixes per direction: 4 radix passes (even), so every slice's pairs
+    # come from one pass over the text (the path of texts above 2^31 suffixes);
+    # the suffix arrays, BWT and reads_index against the oracle's checks
+    rng = np.random.default_rng(61 + row_bytes)
+    codes, off, _ = _random_reads(rng, 2_000_000, 160_000, 100, 100, err=0.003)
+    N = int(off[-1])
+    wctx.set_layout(row_bytes, 5_000_000)
+    wctx.upload(codes, off, 31)
+    wctx.build_index()
+    rb, slices = wctx.layout()
+    assert rb == row_bytes and slices >= (N + 4_999_999) // 5_000_000
+    n = len(off) - 1
+    for d in (0, 1):
+        dt = orc.dir_text(codes, off, d)
+        sa = wctx.sa(d)
+        assert orc.check_sa(dt, off, sa) == -1, d
+        bo, ri = orc.bwt(dt, off, sa)
+        assert np.array_equal(wctx.bwt(d), bo), d
+        assert np.array_equal(wctx.reads_index(d), ri), d
+        assert np.array_equal(np.sort(ri), np.arange(n, dtype=np.uint64)), d
+    wctx.set_layout(0, 0)
