@@ -1444,7 +1444,11 @@ template <typename R> static int build_sa_dir_t(pq_ctx_t *ctx, int d)
     // last radix pass leaves them anyway); else slice by slice
     const bool emit_all = slices.size() > 1 && slices.size() <= SLICE_MAX && !(passes & 1);
     uint32_t *BKall = nullptr;
-    if ((rc = pq_slot(ctx, SLOT_BK, (emit_all ? N : smax) * 4 + 4, &BKall)) ||
+    // (the whole text's keys live in the search's candidate slot, which the build no
+    // longer needs once it is done and the search reuses: the search sizes its
+    // batches by free memory, counting that slot as free)
+    if ((rc = emit_all ? pq_slot(ctx, SLOT_CAND_LC, N * 4 + 4, &BKall)
+                       : pq_slot(ctx, SLOT_BK, smax * 4 + 4, &BKall)) ||
         (rc = pq_slot(ctx, SLOT_BK2, (smax + 1) * 4, &BK2)) ||
         (rc = pq_slot(ctx, SLOT_V2, vbytes, &V2)) ||
         (rc = pq_slot(ctx, SLOT_STARTS, (smax + 2) * 4, &S.starts)) ||
